@@ -1,0 +1,97 @@
+"""Build the in-tree HIP library (gfx950) and the C++ host adapter.
+
+``python -m hammurabix_amd._build`` compiles
+
+* ``hammurabix_amd/libhamx.so``   -- every ``csrc/*.hip`` translation unit,
+  hipcc --offload-arch=gfx950, FP64 parity flags (no FP contraction);
+* ``hammurabix_amd/host/hamx``     -- the ``hamx <params.xml>`` driver and
+  ``hammurabix_amd/libhamx_host.so`` (the C++ Param / Pipeline / Integrator
+  mirror over the C-ABI), when the host sources exist.
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libhamx.so")
+HOST_LIB = os.path.join(PKG, "libhamx_host.so")
+HOST_EXE = os.path.join(HOST, "hamx")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",
+    # FP64 parity with the CPU restatement: evaluate expressions as written
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    out = (res.stdout or "") + (res.stderr or "")
+    # hipcc's host pass does not know the packed-fp32 target feature
+    lines = [ln for ln in out.splitlines() if "packed-fp32-ops" not in ln]
+    if lines:
+        print("\n".join(lines), flush=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}")
+
+
+def build_hip() -> str:
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    if not srcs:
+        raise RuntimeError("no HIP sources found")
+    objs = []
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s) + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", s, "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_host() -> list[str]:
+    srcs = sorted(glob.glob(os.path.join(HOST, "*.cc")))
+    if not srcs:
+        return []
+    lib_srcs = [s for s in srcs if not s.endswith("main.cc")]
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", INCLUDE, "-I", HOST]
+    out = []
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", *flags, "-shared", "-o", tmp, *lib_srcs,
+          "-L", PKG, "-lhamx", f"-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    out.append(HOST_LIB)
+    main = os.path.join(HOST, "main.cc")
+    if os.path.exists(main):
+        _run(["g++", *flags, "-o", HOST_EXE, main, "-L", PKG, "-lhamx_host", "-lhamx",
+              f"-Wl,-rpath,$ORIGIN/.."])
+        out.append(HOST_EXE)
+    return out
+
+
+def build_all() -> list[str]:
+    built = [build_hip()]
+    built += build_host()
+    return built
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print("built", p)
+    sys.exit(0)

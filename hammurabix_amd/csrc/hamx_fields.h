@@ -1,0 +1,202 @@
+// Device field models evaluated along the line of sight: the regular
+// magnetic field (WMAP-LSA, cartesian, uniform), the regular thermal electron
+// density (uniform sphere), the analytic and uniform cosmic-ray electron
+// models, and the trilinear vertex-grid gathers of the random fields.
+//
+// Each function keeps the reference's operation order so that the FP64 path
+// reproduces the CPU maps to rounding (see the file:line citations).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hamx.h"
+#include "hamx_cgs.h"
+
+namespace hamx {
+
+struct Vec3 {
+  double x, y, z;
+};
+
+// Hamvec::dotprod (hamvec.h:189-195): the sum starts from +0.
+__device__ inline double dot0(const Vec3 &a, const Vec3 &b) {
+  double t = 0.;
+  t += a.x * b.x;
+  t += a.y * b.y;
+  t += a.z * b.z;
+  return t;
+}
+
+// Hamvec::length (hamvec.h:150-156)
+__device__ inline double length0(const Vec3 &a) {
+  double t = 0.;
+  t += a.x * a.x;
+  t += a.y * a.y;
+  t += a.z * a.z;
+  return sqrt(t);
+}
+
+// Vertex-grid description used by the trilinear gathers.
+struct GridGeo {
+  double x_min, x_max, y_min, y_max, z_min, z_max;
+  double fx, fy, fz; // (n-1) as double
+  uint64_t ny, nz;
+};
+
+__host__ inline GridGeo grid_geo(const hamx_box &b) {
+  GridGeo g;
+  g.x_min = b.x_min;
+  g.x_max = b.x_max;
+  g.y_min = b.y_min;
+  g.y_max = b.y_max;
+  g.z_min = b.z_min;
+  g.z_max = b.z_max;
+  g.fx = static_cast<double>(b.nx - 1);
+  g.fy = static_cast<double>(b.ny - 1);
+  g.fz = static_cast<double>(b.nz - 1);
+  g.ny = b.ny;
+  g.nz = b.nz;
+  return g;
+}
+
+// Cell lookup of Brnd::read_grid (brnd.cc:25-49): strict interior of the
+// vertex box, bin index by floor, fractional offsets in [0,1).  Returns false
+// on and outside the box faces, where the reference returns a zero field.
+struct Cell {
+  uint64_t idx; // flat index of corner (xl, yl, zl)
+  double xd, yd, zd;
+};
+
+// The interior tests are written !(0 < tmp < n-1): the same decision as the
+// reference's (tmp <= 0 || tmp >= n-1) for every finite tmp, and a NaN
+// position also falls outside instead of reaching an index.
+__device__ inline bool grid_cell(const GridGeo &g, const Vec3 &p, Cell &c) {
+  double tmp = g.fx * (p.x - g.x_min) / (g.x_max - g.x_min);
+  if (!(tmp > 0 && tmp < g.fx))
+    return false;
+  const uint64_t xl = static_cast<uint64_t>(floor(tmp));
+  c.xd = tmp - static_cast<double>(xl);
+  tmp = g.fy * (p.y - g.y_min) / (g.y_max - g.y_min);
+  if (!(tmp > 0 && tmp < g.fy))
+    return false;
+  const uint64_t yl = static_cast<uint64_t>(floor(tmp));
+  c.yd = tmp - static_cast<double>(yl);
+  tmp = g.fz * (p.z - g.z_min) / (g.z_max - g.z_min);
+  if (!(tmp > 0 && tmp < g.fz))
+    return false;
+  const uint64_t zl = static_cast<uint64_t>(floor(tmp));
+  c.zd = tmp - static_cast<double>(zl);
+  c.idx = xl * g.ny * g.nz + yl * g.nz + zl;
+  return true;
+}
+
+// z, then y, then x linear interpolation of one scalar component
+// (brnd.cc:55-87, ternd.cc:51-73).
+__device__ inline double lerp_cell(const double *f, const Cell &c, uint64_t sy,
+                                   uint64_t sx) {
+  const uint64_t i00 = c.idx, i01 = c.idx + sy, i10 = c.idx + sx,
+                 i11 = c.idx + sx + sy;
+  const double i1 = f[i00] * (1. - c.zd) + f[i00 + 1] * c.zd;
+  const double i2 = f[i01] * (1. - c.zd) + f[i01 + 1] * c.zd;
+  const double j1 = f[i10] * (1. - c.zd) + f[i10 + 1] * c.zd;
+  const double j2 = f[i11] * (1. - c.zd) + f[i11 + 1] * c.zd;
+  const double w1 = i1 * (1. - c.yd) + i2 * c.yd;
+  const double w2 = j1 * (1. - c.yd) + j2 * c.yd;
+  return w1 * (1. - c.xd) + w2 * c.xd;
+}
+
+// Packed gather: the B components and the TE density of one vertex stored
+// together as 4 doubles (32 B); the two z-neighbours of a column are one
+// contiguous 64 B record.  Same arithmetic as the separate gathers.
+__device__ inline void lerp_cell_packed(const double4 *__restrict__ f,
+                                        const Cell &c, uint64_t sy, uint64_t sx,
+                                        Vec3 &b, double &te) {
+  const double4 a0 = f[c.idx], a1 = f[c.idx + 1];
+  const double4 b0 = f[c.idx + sy], b1 = f[c.idx + sy + 1];
+  const double4 c0 = f[c.idx + sx], c1 = f[c.idx + sx + 1];
+  const double4 d0 = f[c.idx + sx + sy], d1 = f[c.idx + sx + sy + 1];
+  const double zd = c.zd, yd = c.yd, xd = c.xd;
+#define HAMX_LERP3(C)                                                          \
+  ({                                                                           \
+    const double i1 = a0.C * (1. - zd) + a1.C * zd;                            \
+    const double i2 = b0.C * (1. - zd) + b1.C * zd;                            \
+    const double j1 = c0.C * (1. - zd) + c1.C * zd;                            \
+    const double j2 = d0.C * (1. - zd) + d1.C * zd;                            \
+    const double w1 = i1 * (1. - yd) + i2 * yd;                                \
+    const double w2 = j1 * (1. - yd) + j2 * yd;                                \
+    w1 * (1. - xd) + w2 * xd;                                                  \
+  })
+  b.x = HAMX_LERP3(x);
+  b.y = HAMX_LERP3(y);
+  b.z = HAMX_LERP3(z);
+  te = HAMX_LERP3(w);
+#undef HAMX_LERP3
+}
+
+// ---------------------------------------------------------------- models --
+
+// Regular magnetic field models (breg_lsa.cc:9-26, breg_cart.cc, breg_unif.cc).
+__device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
+  Vec3 b{0., 0., 0.};
+  if (model == HAMX_BREG_LSA) {
+    const double r = sqrt(pos.x * pos.x + pos.y * pos.y);
+    if (r > (20. * cgs::kpc) || r < (3. * cgs::kpc))
+      return b;
+    const double cos_p = pos.x / r;
+    const double sin_p = pos.y / r;
+    const double psi = p[1] + p[2] * log(r / (8. * cgs::kpc));
+    const double chi = p[3] * tanh(pos.z / (1. * cgs::kpc));
+    double sps, cps, sch, cch;
+    sincos(psi, &sps, &cps);
+    sincos(chi, &sch, &cch);
+    b.x = p[0] * (sps * cch * cos_p - cps * cch * sin_p);
+    b.y = p[0] * (sps * cch * sin_p + cps * cch * cos_p);
+    b.z = p[0] * sch;
+  } else if (model == HAMX_BREG_CART) {
+    b.x = p[0];
+    b.y = p[1];
+    b.z = p[2];
+  } else if (model == HAMX_BREG_UNIF) {
+    b.x = p[0] * cos(p[2]);
+    b.y = p[0] * sin(p[2]);
+    b.z = p[1];
+  }
+  return b;
+}
+
+// TEreg_unif::write_field (tereg_unif.cc:6-12)
+__device__ inline double tereg_unif(const double *p, const Vec3 &pos,
+                                    const Vec3 &obs) {
+  const Vec3 d{pos.x - obs.x, pos.y - obs.y, pos.z - obs.z};
+  if (length0(d) > p[1])
+    return 0.;
+  return p[0];
+}
+
+// Lanczos approximation of Gamma(x) (Lanczos 1964; g = 7, n = 9 coefficients
+// as tabulated by Godfrey), with Euler's reflection for x < 1/2.  Used only
+// when the CRE spectral index varies along the ray; constant indices use the
+// host-computed value.
+__device__ inline double gamma_lanczos(double x) {
+  const double c[9] = {0.99999999999980993,  676.5203681218851,
+                       -1259.1392167224028,  771.32342877765313,
+                       -176.61502916214059,  12.507343278686905,
+                       -0.13857109526572012, 9.9843695780195716e-6,
+                       1.5056327351493116e-7};
+  bool refl = false;
+  double xr = x;
+  if (x < 0.5) {
+    refl = true;
+    xr = 1. - x;
+  }
+  xr -= 1.;
+  double a = c[0];
+  const double t = xr + 7.5;
+  for (int i = 1; i < 9; ++i)
+    a += c[i] / (xr + i);
+  const double g = sqrt(cgs::twopi) * pow(t, xr + 0.5) * exp(-t) * a;
+  return refl ? cgs::pi / (sin(cgs::pi * x) * g) : g;
+}
+
+} // namespace hamx

@@ -1,0 +1,870 @@
+// Line-of-sight integrator for gfx950: Integrator::write_grid /
+// radial_integration (integrator.cc:23-279) as HIP kernels.
+//
+// Per shell s (nside_s, step_s samples at dist_k = d0 + (k + 1/2) delta):
+//   k_los_wave / k_los_thread : march every ray of the shell, producing the
+//       shell maps DM, FD, I, Q, U (I/Q/U temp_convert-ed, integrator.cc:99-115)
+//   k_accumulate               : add the shell maps into the output maps,
+//       directly or by HEALPix interpolation at the output pixel centres
+//       (hamdis.h:698-721)
+// FD of the inner shells reaches shell s through the accumulated FD map,
+// interpolated at the ray's direction (integrator.cc:78-86).
+//
+// Two march schedules share the per-sample arithmetic:
+//   wave   : one 64-lane wavefront per ray, lane j evaluating sample
+//            base + j; the running Faraday depth is carried across chunks
+//            and formed inside a chunk by an inclusive wave scan, so every
+//            sample's polarisation phase sees the FD of all samples up to
+//            and including itself (integrator.cc:209,221).  Consecutive
+//            lanes sit 1 step apart on the ray, so the 8-corner gathers of a
+//            wave fall in a handful of grid cells (coalesced, L2-resident).
+//   thread : one thread per ray, samples summed in the reference's order.
+// Both skip, per ray, the step ranges that provably fail the galactocentric
+// radius / height limits (a conservative analytic interval, then the exact
+// per-sample checks of integrator.cc:179-186): skipped samples contribute
+// nothing in the reference either.
+//
+// The translation unit is compiled with -ffp-contract=off and keeps the
+// reference's operation order, so the FP64 maps agree with the CPU
+// restatement to rounding.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "hamx.h"
+#include "hamx_cgs.h"
+#include "hamx_fields.h"
+#include "hamx_healpix.h"
+#include "hamx_internal.h"
+
+namespace hamx {
+
+// Constants of one write_grid pass (host-computed in the reference's
+// expression order).
+struct Phys {
+  Vec3 obs;
+  double gc_r_min, gc_r_max, gc_z_min, gc_z_max;
+  int do_dm, do_fd, do_sync;
+  int breg_model, tereg_model, cre_model;
+  double breg[8], tereg[4], cre[8];
+  double fd_fore;       // -(qe^3) / (2 pi mec2^2), integrator.cc:160-161
+  double lambda_sq;     // (c/nu)^2, integrator.cc:164-165
+  double i2bt;          // c^2 / (2 kB nu^2), integrator.cc:168-170
+  double A_num;         // 2 pi nu mec, integrator.cc:341-342
+  double three_qe;      // 3 qe
+  double qe3;           // qe^3
+  double fourpi_mec2;   // 4 pi mec2
+  double sixteenpi_mec2;// 16 pi mec2
+  double cre_nu;        // j0 * gamma0^(-idx(obs)) * unit, cre_ana.cc:221-229
+  double R0;            // observer cylindrical radius, cre_ana.cc:196-197
+  int idx_const;        // spectral index constant along rays
+  double idx, g_t1, g_t2, g_p1; // index and its Gamma factors when constant
+  double tc_em1, tc_den; // temp_convert: (e^p - 1) and p^2 e^p, integrator.cc:275-279
+};
+
+struct Grids {
+  const double4 *packed; // brnd (x,y,z) + ternd (w), one box
+  GridGeo g_packed;
+  const double *brnd_x, *brnd_y, *brnd_z;
+  GridGeo g_brnd;
+  const double *ternd;
+  GridGeo g_ternd;
+  const double *breg_x, *breg_y, *breg_z;
+  GridGeo g_breg;
+  const double *tereg;
+  GridGeo g_tereg;
+};
+
+struct ShellArgs {
+  Healpix hs;
+  uint64_t step;
+  double d_start, delta;
+  const double *fd_map; // accumulated FD of inner shells (NULL: zero cache)
+  Healpix hfd;
+  double *t_dm, *t_fd, *t_is, *t_qs, *t_us; // shell maps, nside_s
+};
+
+struct RayGeo {
+  double theta, phi;
+  Vec3 l;         // LOS versor
+  Vec3 eth, eph;  // spherical unit vectors for the IAU angle
+  double fd_inner;
+  uint64_t k_lo, k_hi;
+};
+
+// ---------------------------------------------------------------- per ray --
+
+__device__ inline void ray_interval(const Phys &ph, const ShellArgs &sa,
+                                    RayGeo &r) {
+  // Conservative d-interval satisfying |pos| <= gc_r_max and
+  // gc_z_min <= z <= gc_z_max, widened by 2 steps each side.
+  r.k_lo = 0;
+  r.k_hi = sa.step;
+  if (sa.step == 0)
+    return;
+  const double slack = 2.0 * sa.delta;
+  double lo = -1e300, hi = 1e300;
+  {
+    const double R = ph.gc_r_max * (1.0 + 1e-9) + slack;
+    const double b = r.l.x * ph.obs.x + r.l.y * ph.obs.y + r.l.z * ph.obs.z;
+    const double o2 = ph.obs.x * ph.obs.x + ph.obs.y * ph.obs.y + ph.obs.z * ph.obs.z;
+    const double disc = b * b - (o2 - R * R);
+    if (!(disc >= 0.)) {
+      r.k_hi = 0;
+      return;
+    }
+    const double sq = sqrt(disc);
+    lo = fmax(lo, -b - sq);
+    hi = fmin(hi, -b + sq);
+  }
+  {
+    const double lz = r.l.z;
+    const double zlo = ph.gc_z_min - 1e-9 * fabs(ph.gc_z_min) - slack;
+    const double zhi = ph.gc_z_max + 1e-9 * fabs(ph.gc_z_max) + slack;
+    if (fabs(lz) < 1e-12) {
+      if (ph.obs.z < zlo || ph.obs.z > zhi) {
+        r.k_hi = 0;
+        return;
+      }
+    } else {
+      double t1 = (zlo - ph.obs.z) / lz, t2 = (zhi - ph.obs.z) / lz;
+      if (t1 > t2) {
+        const double t = t1;
+        t1 = t2;
+        t2 = t;
+      }
+      lo = fmax(lo, t1 - slack);
+      hi = fmin(hi, t2 + slack);
+    }
+  }
+  if (!(hi >= lo)) {
+    r.k_hi = 0;
+    return;
+  }
+  // dist_k = d0 + (k + 1/2) delta  ->  k = (d - d0)/delta - 1/2
+  const double kf_lo = (lo - sa.d_start) / sa.delta - 0.5 - 2.0;
+  const double kf_hi = (hi - sa.d_start) / sa.delta - 0.5 + 3.0;
+  const double stepd = static_cast<double>(sa.step);
+  const double clo = fmin(fmax(floor(kf_lo), 0.), stepd);
+  const double chi = fmin(fmax(ceil(kf_hi), 0.), stepd);
+  r.k_lo = static_cast<uint64_t>(clo);
+  r.k_hi = static_cast<uint64_t>(chi);
+  if (r.k_hi < r.k_lo)
+    r.k_hi = r.k_lo;
+}
+
+__device__ inline void ray_setup(const Phys &ph, const ShellArgs &sa,
+                                 uint64_t ipix, RayGeo &r) {
+  hp_pix2ang(sa.hs, ipix, r.theta, r.phi);
+  double st, ct, sp, cp;
+  sincos(r.theta, &st, &ct);
+  sincos(r.phi, &sp, &cp);
+  // los_versor (integrator.cc:268-272)
+  r.l = Vec3{cp * st, sp * st, ct};
+  // sync_ipa unit vectors (integrator.cc:260-262)
+  r.eth = Vec3{ct * cp, ct * sp, -st};
+  r.eph = Vec3{-sp, cp, 0.};
+  r.fd_inner = 0.;
+  if (sa.fd_map)
+    r.fd_inner = hp_interpolate(sa.hfd, sa.fd_map, r.theta, r.phi);
+  ray_interval(ph, sa, r);
+}
+
+// ------------------------------------------------------------- per sample --
+
+struct SampleOut {
+  double te;   // clamped density
+  double fdc;  // te * B_par * fd_fore * delta
+  double jt;   // total emissivity * delta * i2bt
+  double jp;   // polarised emissivity * delta * i2bt
+  double ipa;  // intrinsic polarisation angle
+};
+
+__device__ inline Vec3 gather3(const double *fx, const double *fy,
+                               const double *fz, const GridGeo &g,
+                               const Vec3 &pos) {
+  Cell c;
+  if (!grid_cell(g, pos, c))
+    return Vec3{0., 0., 0.};
+  const uint64_t sy = g.nz, sx = g.ny * g.nz;
+  return Vec3{lerp_cell(fx, c, sy, sx), lerp_cell(fy, c, sy, sx),
+              lerp_cell(fz, c, sy, sx)};
+}
+
+__device__ inline double gather1(const double *f, const GridGeo &g,
+                                 const Vec3 &pos) {
+  Cell c;
+  if (!grid_cell(g, pos, c))
+    return 0.;
+  return lerp_cell(f, c, g.nz, g.ny * g.nz);
+}
+
+// CRE_ana / CRE_unif spatial profile (cre_ana.cc:11-19, cre_unif.cc)
+__device__ inline double cre_profile(const Phys &ph, const Vec3 &pos) {
+  if (ph.cre_model == HAMX_CRE_ANA) {
+    const double r = sqrt(pos.x * pos.x + pos.y * pos.y);
+    const double ch = cosh(pos.z / ph.cre[4]);
+    return exp((ph.R0 - r) / ph.cre[3]) * (1. / (ch * ch));
+  }
+  const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
+  return (length0(d) > ph.cre[1]) ? 0. : 1.;
+}
+
+// CRE_ana::flux_idx (cre_ana.cc:22-28)
+__device__ inline double cre_index(const Phys &ph, const Vec3 &pos) {
+  const double r = sqrt(pos.x * pos.x + pos.y * pos.y) / cgs::kpc;
+  const double z = fabs(pos.z) / cgs::kpc;
+  return -ph.cre[0] + ph.cre[1] * r + ph.cre[2] * z;
+}
+
+template <bool PACKED>
+__device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
+                                   const ShellArgs &sa, const RayGeo &r,
+                                   uint64_t k, SampleOut &s) {
+  const double dist = sa.d_start + (static_cast<double>(k) + 0.5) * sa.delta;
+  const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y,
+                 r.l.z * dist + ph.obs.z};
+  // limit checks (integrator.cc:179-186)
+  const double len = length0(pos);
+  if (len < ph.gc_r_min)
+    return false;
+  if (len > ph.gc_r_max)
+    return false;
+  if (pos.z < ph.gc_z_min)
+    return false;
+  if (pos.z > ph.gc_z_max)
+    return false;
+  // regular + random magnetic field (integrator.cc:188-190)
+  Vec3 B;
+  if (ph.breg_model == HAMX_BREG_GRID)
+    B = gather3(gr.breg_x, gr.breg_y, gr.breg_z, gr.g_breg, pos);
+  else
+    B = breg_eval(ph.breg_model, ph.breg, pos);
+  // regular + random thermal electrons (integrator.cc:197-201)
+  double te = 0.;
+  if (ph.tereg_model == HAMX_TEREG_UNIF)
+    te = tereg_unif(ph.tereg, pos, ph.obs);
+  else if (ph.tereg_model == HAMX_TEREG_GRID)
+    te = gather1(gr.tereg, gr.g_tereg, pos);
+  if (PACKED) {
+    Cell c;
+    Vec3 br{0., 0., 0.};
+    double tr = 0.;
+    if (grid_cell(gr.g_packed, pos, c))
+      lerp_cell_packed(gr.packed, c, gr.g_packed.nz,
+                       gr.g_packed.ny * gr.g_packed.nz, br, tr);
+    B.x += br.x;
+    B.y += br.y;
+    B.z += br.z;
+    te += tr;
+  } else {
+    if (gr.brnd_x) {
+      const Vec3 br = gather3(gr.brnd_x, gr.brnd_y, gr.brnd_z, gr.g_brnd, pos);
+      B.x += br.x;
+      B.y += br.y;
+      B.z += br.z;
+    }
+    if (gr.ternd)
+      te += gather1(gr.ternd, gr.g_ternd, pos);
+  }
+  te *= static_cast<double>(te > 0.);
+  const double b_par = dot0(B, r.l);
+  s.te = te;
+  s.fdc = 0.;
+  if (ph.do_fd || ph.do_sync)
+    s.fdc = te * b_par * ph.fd_fore * sa.delta;
+  if (ph.do_sync) {
+    // los_perproj (integrator.h:39-42)
+    const Vec3 cr{B.y * r.l.z - B.z * r.l.y, B.z * r.l.x - B.x * r.l.z,
+                  B.x * r.l.y - B.y * r.l.x};
+    const double b_per = length0(cr);
+    // analytic emissivities (integrator.cc:333-346, 402-415)
+    double index, g1, g2, g3;
+    if (ph.idx_const) {
+      index = ph.idx;
+      g1 = ph.g_t1;
+      g2 = ph.g_t2;
+      g3 = ph.g_p1;
+    } else {
+      index = cre_index(ph, pos);
+      g1 = gamma_lanczos(-0.25 * index + 19. / 12.);
+      g2 = gamma_lanczos(-0.25 * index - 1. / 12.);
+      g3 = gamma_lanczos(-0.25 * index + 7. / 12.);
+    }
+    const double flux_norm = ph.cre_nu * cre_profile(ph, pos);
+    const double abs_bper = fabs(b_per);
+    const double norm_t = flux_norm * 1.73205081 * ph.qe3 * abs_bper /
+                          (ph.fourpi_mec2 * (1. - index));
+    const double norm_p =
+        flux_norm * 1.73205081 * ph.qe3 * abs_bper / ph.sixteenpi_mec2;
+    const double A = ph.A_num / (ph.three_qe * abs_bper);
+    const double pw = pow(A, 0.5 * (index + 1));
+    const double jt = norm_t * (pw * g1 * g2);
+    const double jp = norm_p * (pw * g3 * g2);
+    s.jt = jt * sa.delta * ph.i2bt;
+    s.jp = jp * sa.delta * ph.i2bt;
+    // sync_ipa (integrator.cc:257-265)
+    s.ipa = atan2(-dot0(r.eth, B), -dot0(r.eph, B));
+  }
+  return true;
+}
+
+// Shell-map epilogue: temp_convert of I/Q/U and the stores
+// (integrator.cc:99-115).
+__device__ inline void store_ray(const Phys &ph, const ShellArgs &sa,
+                                 uint64_t ipix, double dm, double fd, double I,
+                                 double Q, double U) {
+  if (ph.do_dm)
+    sa.t_dm[ipix] = dm;
+  if (ph.do_sync) {
+    sa.t_is[ipix] = I * ph.tc_em1 * ph.tc_em1 / ph.tc_den;
+    sa.t_qs[ipix] = Q * ph.tc_em1 * ph.tc_em1 / ph.tc_den;
+    sa.t_us[ipix] = U * ph.tc_em1 * ph.tc_em1 / ph.tc_den;
+  }
+  if (ph.do_fd || ph.do_sync)
+    sa.t_fd[ipix] = fd;
+}
+
+// ------------------------------------------------------------------ waves --
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline double wave_incl_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(v, o, 64);
+    if (lane >= o)
+      v += t;
+  }
+  return v;
+}
+
+template <bool PACKED>
+__global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
+                                                  ShellArgs sa) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t ipix =
+      static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ipix >= sa.hs.npix)
+    return; // uniform per wavefront
+  RayGeo r;
+  ray_setup(ph, sa, ipix, r);
+  double dm = 0., I = 0., Q = 0., U = 0., fd_part = 0., fd_carry = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    SampleOut s;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_sample<PACKED>(ph, gr, sa, r, k, s);
+    const double fdk = valid ? s.fdc : 0.;
+    if (ph.do_sync) {
+      const double incl = wave_incl_scan(fdk, lane);
+      const double fd_here = fd_carry + incl;
+      fd_carry += __shfl(incl, 63, 64);
+      if (valid) {
+        const double qui = (r.fd_inner + fd_here) * ph.lambda_sq + s.ipa;
+        double s2, c2;
+        sincos(2. * qui, &s2, &c2);
+        I += s.jt;
+        Q += c2 * s.jp;
+        U += s2 * s.jp;
+      }
+    } else {
+      fd_part += fdk;
+    }
+    if (valid && ph.do_dm)
+      dm += s.te * sa.delta;
+  }
+  dm = wave_sum(dm);
+  double fd = ph.do_sync ? fd_carry : wave_sum(fd_part);
+  I = wave_sum(I);
+  Q = wave_sum(Q);
+  U = wave_sum(U);
+  if (lane == 0)
+    store_ray(ph, sa, ipix, dm, fd, I, Q, U);
+}
+
+template <bool PACKED>
+__global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
+                                                    ShellArgs sa) {
+  const uint64_t ipix = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ipix >= sa.hs.npix)
+    return;
+  RayGeo r;
+  ray_setup(ph, sa, ipix, r);
+  double dm = 0., fd = 0., I = 0., Q = 0., U = 0.;
+  for (uint64_t k = r.k_lo; k < r.k_hi; ++k) {
+    SampleOut s;
+    if (!eval_sample<PACKED>(ph, gr, sa, r, k, s))
+      continue;
+    if (ph.do_dm)
+      dm += s.te * sa.delta;
+    if (ph.do_fd || ph.do_sync)
+      fd += s.fdc;
+    if (ph.do_sync) {
+      I += s.jt;
+      const double qui = (r.fd_inner + fd) * ph.lambda_sq + s.ipa;
+      Q += cos(2. * qui) * s.jp;
+      U += sin(2. * qui) * s.jp;
+    }
+  }
+  store_ray(ph, sa, ipix, dm, fd, I, Q, U);
+}
+
+// ------------------------------------------------------------- accumulate --
+
+struct AccPairs {
+  double *dst[5];
+  const double *src[5];
+  int n;
+};
+
+// dst += src (same nside) or dst += interpolate(src, pixel centre of dst)
+// for up to 5 map pairs sharing the two resolutions (hamdis.h:698-721).
+__global__ __launch_bounds__(256) void k_accumulate(AccPairs p, Healpix hd,
+                                                    Healpix hsrc) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= hd.npix)
+    return;
+  if (hd.npix == hsrc.npix) {
+    for (int m = 0; m < p.n; ++m)
+      p.dst[m][i] = p.dst[m][i] + p.src[m][i];
+    return;
+  }
+  double theta, phi;
+  hp_pix2ang(hd, i, theta, phi);
+  uint64_t pix[4];
+  double wght[4];
+  hp_interpol_weights(hsrc, theta, phi, pix, wght);
+  for (int m = 0; m < p.n; ++m) {
+    double wtot = 0.0, res = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double val = p.src[m][pix[q]];
+      if (val > cgs::undef_threshold) {
+        res += val * wght[q];
+        wtot += wght[q];
+      }
+    }
+    const double v = (wtot == 0.0) ? cgs::undef : res / wtot;
+    p.dst[m][i] = p.dst[m][i] + v;
+  }
+}
+
+// Pack SoA brnd (x,y,z) and ternd into one double4 record per vertex.
+__global__ void k_pack4(double4 *__restrict__ out, const double *__restrict__ bx,
+                        const double *__restrict__ by, const double *__restrict__ bz,
+                        const double *__restrict__ te, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += stride) {
+    double4 v;
+    v.x = bx ? bx[i] : 0.;
+    v.y = by ? by[i] : 0.;
+    v.z = bz ? bz[i] : 0.;
+    v.w = te ? te[i] : 0.;
+    out[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ host --
+
+static bool same_box(const hamx_box &a, const hamx_box &b) {
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.x_min == b.x_min &&
+         a.x_max == b.x_max && a.y_min == b.y_min && a.y_max == b.y_max &&
+         a.z_min == b.z_min && a.z_max == b.z_max;
+}
+
+static int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what) {
+  if (b.nx < 2 || b.ny < 2 || b.nz < 2)
+    return set_err(ctx, HAMX_ERR_ARG, "%s grid needs at least 2 vertices per axis", what);
+  if (!(b.x_max > b.x_min) || !(b.y_max > b.y_min) || !(b.z_max > b.z_min))
+    return set_err(ctx, HAMX_ERR_ARG, "%s grid box limits are not increasing", what);
+  return HAMX_OK;
+}
+
+static uint64_t box_size(const hamx_box &b) { return b.nx * b.ny * b.nz; }
+
+// Integrator::assemble_shell_ref (integrator.cc:231-244)
+static void shell_ref(const hamx_los_desc *d, uint32_t s, uint64_t &step,
+                      double &d_start, double &delta) {
+  d_start = d->radii_shell[s];
+  const double d_stop = d->radii_shell[s + 1];
+  delta = d->oc_r_res;
+  const double stepf = std::floor(d_stop / delta - d_start / delta);
+  step = stepf > 0 ? static_cast<uint64_t>(stepf) : 0;
+  delta = (d_stop - d_start) / static_cast<double>(step);
+}
+
+int validate_los(hamx_ctx *ctx, const hamx_los_desc *d) {
+  if (d->n_shell < 1 || d->n_shell > HAMX_MAX_SHELL)
+    return set_err(ctx, HAMX_ERR_ARG, "n_shell %u outside [1, %d]", d->n_shell,
+                   HAMX_MAX_SHELL);
+  for (uint32_t s = 0; s < d->n_shell; ++s)
+    if (!is_pow2(d->nside_shell[s]))
+      return set_err(ctx, HAMX_ERR_NSIDE, "shell %u nside %u is not a power of 2",
+                     s, d->nside_shell[s]);
+  if (!(d->oc_r_res > 0.))
+    return set_err(ctx, HAMX_ERR_ARG, "oc_r_res must be positive");
+  if (d->do_dm && !is_pow2(d->nside_dm))
+    return set_err(ctx, HAMX_ERR_NSIDE, "nside_dm %u is not a power of 2", d->nside_dm);
+  if (d->do_fd && !is_pow2(d->nside_fd))
+    return set_err(ctx, HAMX_ERR_NSIDE, "nside_fd %u is not a power of 2", d->nside_fd);
+  if (d->do_sync && !is_pow2(d->nside_sync))
+    return set_err(ctx, HAMX_ERR_NSIDE, "nside_sync %u is not a power of 2",
+                   d->nside_sync);
+  if (d->do_sync && !(d->sync_freq > 0.))
+    return set_err(ctx, HAMX_ERR_ARG, "sync frequency must be positive");
+  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_GRID)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
+  if (d->tereg_model < HAMX_TEREG_NONE || d->tereg_model > HAMX_TEREG_GRID)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported tereg model %d", d->tereg_model);
+  if (d->cre_model < HAMX_CRE_NONE || d->cre_model > HAMX_CRE_UNIF)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported cre model %d", d->cre_model);
+  if (d->do_sync && d->cre_model == HAMX_CRE_NONE)
+    return set_err(ctx, HAMX_ERR_ARG, "synchrotron output requires a CRE model");
+  if (d->schedule < HAMX_SCHED_AUTO || d->schedule > HAMX_SCHED_WAVE)
+    return set_err(ctx, HAMX_ERR_ARG, "unknown schedule %d", d->schedule);
+  return HAMX_OK;
+}
+
+Phys make_phys(const hamx_los_desc *d) {
+  Phys ph;
+  std::memset(&ph, 0, sizeof(ph));
+  ph.obs = Vec3{d->observer[0], d->observer[1], d->observer[2]};
+  ph.gc_r_min = d->gc_r_min;
+  ph.gc_r_max = d->gc_r_max;
+  ph.gc_z_min = d->gc_z_min;
+  ph.gc_z_max = d->gc_z_max;
+  ph.do_dm = d->do_dm != 0;
+  ph.do_fd = d->do_fd != 0;
+  ph.do_sync = d->do_sync != 0;
+  ph.breg_model = d->breg_model;
+  ph.tereg_model = d->tereg_model;
+  ph.cre_model = d->cre_model;
+  std::memcpy(ph.breg, d->breg, sizeof(ph.breg));
+  std::memcpy(ph.tereg, d->tereg, sizeof(ph.tereg));
+  std::memcpy(ph.cre, d->cre, sizeof(ph.cre));
+  ph.fd_fore = -(cgs::qe * cgs::qe * cgs::qe) / (2. * cgs::pi * cgs::mec2 * cgs::mec2);
+  ph.qe3 = cgs::qe * cgs::qe * cgs::qe;
+  ph.three_qe = 3. * cgs::qe;
+  ph.fourpi_mec2 = 4. * cgs::pi * cgs::mec2;
+  ph.sixteenpi_mec2 = 16. * cgs::pi * cgs::mec2;
+  if (d->do_sync) {
+    const double f = d->sync_freq;
+    ph.lambda_sq = (cgs::c_light / f) * (cgs::c_light / f);
+    ph.i2bt = cgs::c_light * cgs::c_light / (2. * cgs::kB * f * f);
+    ph.A_num = 2. * cgs::pi * f * cgs::mec;
+    const double p = cgs::h_planck * f / (cgs::kB * 2.725);
+    ph.tc_em1 = std::exp(p) - 1.;
+    ph.tc_den = p * p * std::exp(p);
+  }
+  // CRE normalisation constants (cre_ana.cc:30-47, cre_unif.cc)
+  if (d->cre_model == HAMX_CRE_ANA) {
+    const double alpha = d->cre[0], beta = d->cre[1], theta = d->cre[2];
+    const double E0 = d->cre[5], j0 = d->cre[6];
+    ph.R0 = std::sqrt(d->observer[0] * d->observer[0] + d->observer[1] * d->observer[1]);
+    const double gamma0 = E0 / cgs::mec2;
+    const double beta0 = std::sqrt(1. - 1. / gamma0);
+    const double unit = 4. * cgs::pi * cgs::mec /
+                        (cgs::GeV * cgs::m * cgs::m * cgs::sec * beta0);
+    const double r_obs = ph.R0 / cgs::kpc;
+    const double z_obs = std::fabs(d->observer[2]) / cgs::kpc;
+    const double idx_obs = -alpha + beta * r_obs + theta * z_obs;
+    const double norm = j0 * std::pow(gamma0, -idx_obs);
+    ph.cre_nu = norm * unit;
+    ph.idx_const = (beta == 0. && theta == 0.) ? 1 : 0;
+    ph.idx = -alpha;
+  } else if (d->cre_model == HAMX_CRE_UNIF) {
+    const double alpha = d->cre[0], E0 = d->cre[2], j0 = d->cre[3];
+    const double gamma0 = E0 / cgs::mec2;
+    const double beta0 = std::sqrt(1. - 1. / gamma0);
+    const double unit = 4. * cgs::pi * cgs::mec /
+                        (cgs::GeV * cgs::m * cgs::m * cgs::sec * beta0);
+    const double norm = j0 * std::pow(gamma0, -(-alpha));
+    ph.cre_nu = norm * unit;
+    ph.idx_const = 1;
+    ph.idx = -alpha;
+  }
+  if (ph.idx_const) {
+    ph.g_t1 = std::tgamma(-0.25 * ph.idx + 19. / 12.);
+    ph.g_t2 = std::tgamma(-0.25 * ph.idx - 1. / 12.);
+    ph.g_p1 = std::tgamma(-0.25 * ph.idx + 7. / 12.);
+  }
+  return ph;
+}
+
+} // namespace hamx
+
+using namespace hamx;
+
+extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
+  if (!d || !samples)
+    return HAMX_ERR_ARG;
+  if (d->n_shell < 1 || d->n_shell > HAMX_MAX_SHELL || !(d->oc_r_res > 0.))
+    return HAMX_ERR_ARG;
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    uint64_t step;
+    double d0, delta;
+    shell_ref(d, s, step, d0, delta);
+    const uint64_t ns = d->nside_shell[s];
+    total += 12 * ns * ns * step;
+  }
+  *samples = total;
+  return HAMX_OK;
+}
+
+extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
+                                const hamx_fields *fields, hamx_maps *out,
+                                uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !out)
+    return set_err(ctx, HAMX_ERR_ARG, "null descriptor or output maps");
+  HAMX_TRY(validate_los(ctx, d));
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  hamx_fields none;
+  std::memset(&none, 0, sizeof(none));
+  const hamx_fields &f = fields ? *fields : none;
+
+  // ---- field grids
+  Grids gr;
+  std::memset(&gr, 0, sizeof(gr));
+  const bool has_brnd = f.brnd_x || f.brnd_y || f.brnd_z;
+  if (has_brnd && !(f.brnd_x && f.brnd_y && f.brnd_z))
+    return set_err(ctx, HAMX_ERR_ARG, "brnd grid needs all three components");
+  if (has_brnd)
+    HAMX_TRY(check_box(ctx, f.brnd_box, "brnd"));
+  if (f.ternd)
+    HAMX_TRY(check_box(ctx, f.ternd_box, "ternd"));
+  if (d->breg_model == HAMX_BREG_GRID) {
+    if (!(f.breg_x && f.breg_y && f.breg_z))
+      return set_err(ctx, HAMX_ERR_ARG, "breg grid model without breg grid arrays");
+    HAMX_TRY(check_box(ctx, f.breg_box, "breg"));
+  }
+  if (d->tereg_model == HAMX_TEREG_GRID) {
+    if (!f.tereg)
+      return set_err(ctx, HAMX_ERR_ARG, "tereg grid model without tereg grid array");
+    HAMX_TRY(check_box(ctx, f.tereg_box, "tereg"));
+  }
+  const void *p;
+  if (has_brnd) {
+    const size_t nb = box_size(f.brnd_box) * sizeof(double);
+    HAMX_TRY(stage_in(ctx, kSlotBrndX, f.brnd_x, nb, in_dev, &p));
+    gr.brnd_x = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBrndY, f.brnd_y, nb, in_dev, &p));
+    gr.brnd_y = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBrndZ, f.brnd_z, nb, in_dev, &p));
+    gr.brnd_z = static_cast<const double *>(p);
+    gr.g_brnd = grid_geo(f.brnd_box);
+  }
+  if (f.ternd) {
+    HAMX_TRY(stage_in(ctx, kSlotTernd, f.ternd, box_size(f.ternd_box) * sizeof(double),
+                      in_dev, &p));
+    gr.ternd = static_cast<const double *>(p);
+    gr.g_ternd = grid_geo(f.ternd_box);
+  }
+  if (d->breg_model == HAMX_BREG_GRID) {
+    const size_t nb = box_size(f.breg_box) * sizeof(double);
+    HAMX_TRY(stage_in(ctx, kSlotBregX, f.breg_x, nb, in_dev, &p));
+    gr.breg_x = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregY, f.breg_y, nb, in_dev, &p));
+    gr.breg_y = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregZ, f.breg_z, nb, in_dev, &p));
+    gr.breg_z = static_cast<const double *>(p);
+    gr.g_breg = grid_geo(f.breg_box);
+  }
+  if (d->tereg_model == HAMX_TEREG_GRID) {
+    HAMX_TRY(stage_in(ctx, kSlotTereg, f.tereg, box_size(f.tereg_box) * sizeof(double),
+                      in_dev, &p));
+    gr.tereg = static_cast<const double *>(p);
+    gr.g_tereg = grid_geo(f.tereg_box);
+  }
+  // Pack the random B components and TE density into one record per vertex
+  // when they share a box: 64 B contiguous per z-pair of corners.
+  bool packed = false;
+  if (has_brnd && (!f.ternd || same_box(f.brnd_box, f.ternd_box))) {
+    const uint64_t n = box_size(f.brnd_box);
+    void *pk;
+    HAMX_TRY(ws_get(ctx, kSlotPacked, n * sizeof(double4), &pk));
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 16384)
+      blocks = 16384;
+    hipLaunchKernelGGL(k_pack4, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       static_cast<double4 *>(pk), gr.brnd_x, gr.brnd_y, gr.brnd_z,
+                       gr.ternd, n);
+    HAMX_HIP(ctx, hipGetLastError());
+    gr.packed = static_cast<const double4 *>(pk);
+    gr.g_packed = gr.g_brnd;
+    packed = true;
+  }
+
+  // ---- output maps (device)
+  const Phys ph = make_phys(d);
+  const uint64_t nside_fd_map = d->do_fd ? d->nside_fd : d->nside_sync;
+  const Healpix h_dm = healpix_make(d->do_dm ? d->nside_dm : 1);
+  const Healpix h_sync = healpix_make(d->do_sync ? d->nside_sync : 1);
+  const Healpix h_fd = healpix_make((d->do_fd || d->do_sync) ? nside_fd_map : 1);
+  double *o_dm = nullptr, *o_fd = nullptr, *o_is = nullptr, *o_qs = nullptr,
+         *o_us = nullptr;
+  void *q;
+  auto out_map = [&](int slot, double *user, const Healpix &h, double **dst) -> int {
+    if (out_dev && user) {
+      *dst = user;
+    } else {
+      HAMX_TRY(ws_get(ctx, slot, h.npix * sizeof(double), &q));
+      *dst = static_cast<double *>(q);
+    }
+    return fill_f64(ctx, *dst, h.npix, 0.);
+  };
+  if (d->do_dm)
+    HAMX_TRY(out_map(kSlotOutDm, out->dm, h_dm, &o_dm));
+  if (d->do_sync) {
+    HAMX_TRY(out_map(kSlotOutIs, out->is, h_sync, &o_is));
+    HAMX_TRY(out_map(kSlotOutQs, out->qs, h_sync, &o_qs));
+    HAMX_TRY(out_map(kSlotOutUs, out->us, h_sync, &o_us));
+  }
+  if (d->do_fd || d->do_sync)
+    HAMX_TRY(out_map(kSlotOutFd, out->fd, h_fd, &o_fd));
+
+  // ---- shell maps
+  uint64_t max_npix = 0;
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    const uint64_t ns = d->nside_shell[s];
+    if (12 * ns * ns > max_npix)
+      max_npix = 12 * ns * ns;
+  }
+  double *t_dm = nullptr, *t_fd = nullptr, *t_is = nullptr, *t_qs = nullptr,
+         *t_us = nullptr;
+  const size_t tb = max_npix * sizeof(double);
+  if (d->do_dm) {
+    HAMX_TRY(ws_get(ctx, kSlotTmpDm, tb, &q));
+    t_dm = static_cast<double *>(q);
+  }
+  if (d->do_fd || d->do_sync) {
+    HAMX_TRY(ws_get(ctx, kSlotTmpFd, tb, &q));
+    t_fd = static_cast<double *>(q);
+  }
+  if (d->do_sync) {
+    HAMX_TRY(ws_get(ctx, kSlotTmpIs, tb, &q));
+    t_is = static_cast<double *>(q);
+    HAMX_TRY(ws_get(ctx, kSlotTmpQs, tb, &q));
+    t_qs = static_cast<double *>(q);
+    HAMX_TRY(ws_get(ctx, kSlotTmpUs, tb, &q));
+    t_us = static_cast<double *>(q);
+  }
+
+  const int sched = d->schedule == HAMX_SCHED_THREAD ? HAMX_SCHED_THREAD : HAMX_SCHED_WAVE;
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    ShellArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.hs = healpix_make(d->nside_shell[s]);
+    shell_ref(d, s, sa.step, sa.d_start, sa.delta);
+    // FD cache: only a carried FD map can be nonzero (Appendix of the
+    // reference pipeline: passes without do_fd read a zero map).
+    sa.fd_map = (d->do_fd && s > 0) ? o_fd : nullptr;
+    sa.hfd = h_fd;
+    sa.t_dm = t_dm;
+    sa.t_fd = t_fd;
+    sa.t_is = t_is;
+    sa.t_qs = t_qs;
+    sa.t_us = t_us;
+    const uint64_t npix = sa.hs.npix;
+    ProfMark pm = prof_begin(ctx);
+    if (sched == HAMX_SCHED_WAVE) {
+      const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
+      if (packed)
+        hipLaunchKernelGGL(k_los_wave<true>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           ph, gr, sa);
+      else
+        hipLaunchKernelGGL(k_los_wave<false>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           ph, gr, sa);
+    } else {
+      const unsigned blocks = static_cast<unsigned>((npix + 255) / 256);
+      if (packed)
+        hipLaunchKernelGGL(k_los_thread<true>, dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+      else
+        hipLaunchKernelGGL(k_los_thread<false>, dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+    }
+    HAMX_HIP(ctx, hipGetLastError());
+    prof_end(ctx, pm, sched == HAMX_SCHED_WAVE ? "los_wave" : "los_thread", s,
+             npix * sa.step);
+    // accumulate shell maps into the outputs (integrator.cc:122-132)
+    auto acc = [&](AccPairs &pairs, const Healpix &hd) -> int {
+      if (pairs.n == 0)
+        return HAMX_OK;
+      const unsigned blocks = static_cast<unsigned>((hd.npix + 255) / 256);
+      ProfMark am = prof_begin(ctx);
+      hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, ctx->stream,
+                         pairs, hd, sa.hs);
+      HAMX_HIP(ctx, hipGetLastError());
+      prof_end(ctx, am, "accumulate", s, hd.npix);
+      return HAMX_OK;
+    };
+    AccPairs pr;
+    std::memset(&pr, 0, sizeof(pr));
+    if (d->do_sync) {
+      pr.dst[0] = o_is; pr.src[0] = t_is;
+      pr.dst[1] = o_qs; pr.src[1] = t_qs;
+      pr.dst[2] = o_us; pr.src[2] = t_us;
+      pr.n = 3;
+      if (d->do_dm && d->nside_dm == d->nside_sync) {
+        pr.dst[pr.n] = o_dm; pr.src[pr.n] = t_dm; ++pr.n;
+      }
+      if (d->do_fd && d->nside_fd == d->nside_sync) {
+        pr.dst[pr.n] = o_fd; pr.src[pr.n] = t_fd; ++pr.n;
+      }
+      HAMX_TRY(acc(pr, h_sync));
+    }
+    if (d->do_dm && !(d->do_sync && d->nside_dm == d->nside_sync)) {
+      AccPairs p2;
+      std::memset(&p2, 0, sizeof(p2));
+      p2.dst[0] = o_dm; p2.src[0] = t_dm; p2.n = 1;
+      if (d->do_fd && d->nside_fd == d->nside_dm &&
+          !(d->do_sync && d->nside_fd == d->nside_sync)) {
+        p2.dst[1] = o_fd; p2.src[1] = t_fd; p2.n = 2;
+      }
+      HAMX_TRY(acc(p2, h_dm));
+    }
+    const bool fd_done = (d->do_sync && d->nside_fd == d->nside_sync) ||
+                         (d->do_dm && d->nside_fd == d->nside_dm);
+    if (d->do_fd && !fd_done) {
+      AccPairs p3;
+      std::memset(&p3, 0, sizeof(p3));
+      p3.dst[0] = o_fd; p3.src[0] = t_fd; p3.n = 1;
+      HAMX_TRY(acc(p3, h_fd));
+    }
+  }
+
+  // ---- results to the caller
+  auto copy_out = [&](double *user, const double *dev, const Healpix &h) -> int {
+    if (!user || out_dev)
+      return HAMX_OK;
+    HAMX_HIP(ctx, hipMemcpyAsync(user, dev, h.npix * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    return HAMX_OK;
+  };
+  if (d->do_dm)
+    HAMX_TRY(copy_out(out->dm, o_dm, h_dm));
+  if (d->do_sync) {
+    HAMX_TRY(copy_out(out->is, o_is, h_sync));
+    HAMX_TRY(copy_out(out->qs, o_qs, h_sync));
+    HAMX_TRY(copy_out(out->us, o_us, h_sync));
+  }
+  if (d->do_fd || d->do_sync)
+    HAMX_TRY(copy_out(out->fd, o_fd, h_fd));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}

@@ -1,0 +1,609 @@
+// Gaussian random fields on gfx950: Brnd_es::write_grid (brnd_es.cc:76-308)
+// and TErnd_dft::write_grid (ternd_dft.cc:35-128).
+//
+// Pipeline (every step a grid-stride elementwise kernel over the N vertices,
+// or the LDS Stockham FFT of hamx_fft.hip):
+//   ES : draw c0,c1 -> FFT(+) -> variance -> rescale+anisotropy -> FFT(-)
+//        -> Gram-Schmidt (b = sqrt2 * GS(Re part)) -> reassemble -> FFT(+)
+//        -> Hermitian extraction / N
+//   DFT: draw te_k -> FFT(+) -> variance -> rescale
+// The variance stays on the device (two-stage reduction), so the pipeline is
+// one stream of launches with no host round trip.
+//
+// Gaussian variates: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// keyed by the seed, counter = (pair index, stream), two 53-bit uniforms per
+// call and the Box-Muller transform.  A realisation is a function of the
+// seed alone (the reference's per-thread GSL taus streams depend on the
+// OpenMP thread count, brnd_es.cc:82-104).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "hamx.h"
+#include "hamx_cgs.h"
+#include "hamx_fields.h"
+#include "hamx_internal.h"
+
+namespace hamx {
+
+// ------------------------------------------------------------------ Philox --
+__device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c[0];
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2];
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0;
+    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+__device__ inline void gauss_pair(uint64_t seed, uint32_t stream, uint64_t m,
+                                  double &g0, double &g1) {
+  uint32_t c[4] = {static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), stream, 0u};
+  philox4x32_10(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  const uint64_t a = (static_cast<uint64_t>(c[1]) << 32) | c[0];
+  const uint64_t b = (static_cast<uint64_t>(c[3]) << 32) | c[2];
+  const double u1 = (static_cast<double>(a >> 11) + 1.0) * 0x1.0p-53;
+  const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
+  const double r = sqrt(-2.0 * log(u1));
+  double s, co;
+  sincos(cgs::twopi * u2, &s, &co);
+  g0 = r * co;
+  g1 = r * s;
+}
+
+__global__ void k_gauss_pairs(double *out, uint64_t seed, uint32_t stream,
+                              uint64_t first, uint64_t count) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t m = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       m < count; m += stride) {
+    double g0, g1;
+    gauss_pair(seed, stream, first + m, g0, g1);
+    out[2 * m] = g0;
+    out[2 * m + 1] = g1;
+  }
+}
+
+// ------------------------------------------------------------------ common --
+struct BoxArgs {
+  uint64_t nx, ny, nz, n;
+  double lx, ly, lz;
+  double x_min, y_min, z_min;
+  double dk3; // kpc^3 / (lx ly lz)
+};
+
+static BoxArgs box_args(const hamx_box &b) {
+  BoxArgs a;
+  a.nx = b.nx;
+  a.ny = b.ny;
+  a.nz = b.nz;
+  a.n = b.nx * b.ny * b.nz;
+  a.lx = b.x_max - b.x_min;
+  a.ly = b.y_max - b.y_min;
+  a.lz = b.z_max - b.z_min;
+  a.x_min = b.x_min;
+  a.y_min = b.y_min;
+  a.z_min = b.z_min;
+  a.dk3 = cgs::kpc * cgs::kpc * cgs::kpc / (a.lx * a.ly * a.lz);
+  return a;
+}
+
+// Wave number of index i on an axis of n points over length len
+// (brnd_es.cc:106-122: periodic spacing n/L).
+__device__ inline double wavenum(uint64_t i, uint64_t n, double len) {
+  double k = cgs::kpc * static_cast<double>(i) / len;
+  if (i >= (n + 1) / 2)
+    k -= cgs::kpc * static_cast<double>(n) / len;
+  return k;
+}
+
+// Vertex position (brnd_es.cc:165-173: vertex spacing L/(n-1)).
+__device__ inline Vec3 vertex_pos(const BoxArgs &a, uint64_t i, uint64_t j,
+                                  uint64_t l) {
+  return Vec3{static_cast<double>(i) * a.lx / static_cast<double>(a.nx - 1) + a.x_min,
+              static_cast<double>(j) * a.ly / static_cast<double>(a.ny - 1) + a.y_min,
+              static_cast<double>(l) * a.lz / static_cast<double>(a.nz - 1) + a.z_min};
+}
+
+// Energy profile (brnd_es.cc:53-59, ternd_dft.cc:26-33).
+__device__ inline double profile(const Vec3 &obs, double r0, double z0,
+                                 const Vec3 &pos) {
+  const double r_cyl = sqrt(pos.x * pos.x + pos.y * pos.y) - fabs(obs.x);
+  const double z = fabs(pos.z) - fabs(obs.z);
+  return exp(-r_cyl / r0) * exp(-z / z0);
+}
+
+__device__ inline uint64_t sym_index(const BoxArgs &a, uint64_t idx) {
+  const uint64_t i = idx / (a.ny * a.nz);
+  const uint64_t j = (idx / a.nz) % a.ny;
+  const uint64_t l = idx % a.nz;
+  const uint64_t is = i == 0 ? 0 : a.nx - i;
+  const uint64_t js = j == 0 ? 0 : a.ny - j;
+  const uint64_t ls = l == 0 ? 0 : a.nz - l;
+  return is * a.ny * a.nz + js * a.nz + ls;
+}
+
+// ---------------------------------------------------- variance (two stage) --
+// toolkit::variance over the first n doubles (toolkit.h:48-78): mean, then
+// the mean squared deviation.  Stage 1: per-block partial sums; stage 2: one
+// block combines the partials and writes the result to out[slot].
+constexpr int kRedBlocks = 1024;
+
+__device__ inline double block_sum(double v) {
+  __shared__ double sm[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    sm[w] = v;
+  __syncthreads();
+  double t = 0.;
+  if (threadIdx.x == 0)
+    t = sm[0] + sm[1] + sm[2] + sm[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_partial(const double *x, uint64_t n,
+                                                 const double *mean,
+                                                 double *partials) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  double acc = 0.;
+  const double m = mean ? mean[0] : 0.;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += stride) {
+    const double v = x[i];
+    acc += mean ? (v - m) * (v - m) : v;
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_final(const double *partials, int np,
+                                               uint64_t n, double *out) {
+  double acc = 0.;
+  for (int i = threadIdx.x; i < np; i += blockDim.x)
+    acc += partials[i];
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0)
+    out[0] = t / static_cast<double>(n);
+}
+
+// scratch layout: [0] mean, [1] variance, [16..16+kRedBlocks) partials
+static int variance_dev(hamx_ctx *ctx, const double *x, uint64_t n,
+                        double *scratch) {
+  double *partials = scratch + 16;
+  hipLaunchKernelGGL(k_partial, dim3(kRedBlocks), dim3(256), 0, ctx->stream, x, n,
+                     static_cast<const double *>(nullptr), partials);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, ctx->stream, partials,
+                     kRedBlocks, n, scratch);
+  hipLaunchKernelGGL(k_partial, dim3(kRedBlocks), dim3(256), 0, ctx->stream, x, n,
+                     static_cast<const double *>(scratch), partials);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, ctx->stream, partials,
+                     kRedBlocks, n, scratch + 1);
+  HAMX_HIP(ctx, hipGetLastError());
+  return HAMX_OK;
+}
+
+// ---------------------------------------------------------------- Brnd_es --
+struct EsArgs {
+  BoxArgs b;
+  Vec3 obs;
+  double rms, k0, k1, a0, a1, rho, r0, z0;
+  int breg_model;
+  double breg[8];
+  const double *bgx, *bgy, *bgz;
+  GridGeo g_breg;
+};
+
+// Brnd_es::spectrum (brnd_es.cc:34-49)
+__device__ inline double es_spectrum(const EsArgs &a, double k) {
+  const double p0 = a.rms * a.rms;
+  const double unit = 1. / (4. * cgs::pi * k * k);
+  const double band1 = static_cast<double>(k < a.k1);
+  const double band2 = static_cast<double>(k > a.k1) * static_cast<double>(k < a.k0);
+  const double band3 = static_cast<double>(k > a.k0);
+  const double P = band1 * pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.) +
+                   band2 / pow(k / a.k0, a.a1) + band3 / pow(k / a.k0, a.a0);
+  return P * p0 * unit;
+}
+
+// STEP I (brnd_es.cc:92-140)
+__global__ __launch_bounds__(256) void k_es_draw(double2 *c0, double2 *c1,
+                                                 EsArgs a, uint64_t seed) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    if (idx == 0) {
+      c0[0] = make_double2(0., 0.);
+      c1[0] = make_double2(0., 0.);
+      continue;
+    }
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const double kx = wavenum(i, a.b.nx, a.b.lx);
+    const double ky = wavenum(j, a.b.ny, a.b.ly);
+    const double kz = wavenum(l, a.b.nz, a.b.lz);
+    const double ks = sqrt(kx * kx + ky * ky + kz * kz);
+    const double sigma = sqrt(cgs::onethird * es_spectrum(a, ks) * a.b.dk3);
+    double g0, g1, g2, g3;
+    gauss_pair(seed, 0, 2 * idx, g0, g1);
+    gauss_pair(seed, 0, 2 * idx + 1, g2, g3);
+    c0[idx] = make_double2(sigma * g0, sigma * g1);
+    c1[idx] = make_double2(sigma * g2, sigma * g3);
+  }
+}
+
+// STEP II (brnd_es.cc:157-205)
+__global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
+                                                    EsArgs a, const double *var) {
+  const double b_var_invsq = 1. / sqrt(3. * var[0]);
+  const double rho = a.rho;
+  const double rho2 = rho * rho;
+  const double rhonorm = 1. / sqrt(cgs::onethird * rho2 + cgs::twothirds / rho2);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const Vec3 pos = vertex_pos(a.b, i, j, l);
+    const double ratio = sqrt(profile(a.obs, a.r0, a.z0, pos)) * a.rms * b_var_invsq;
+    const double2 v0 = c0[idx], v1 = c1[idx];
+    Vec3 b{v0.x * ratio, v0.y * ratio, v1.y * ratio};
+    // anisotropy direction: versor of the regular field (brnd_es.cc:19-22)
+    Vec3 braw;
+    if (a.breg_model == HAMX_BREG_GRID) {
+      Cell c;
+      if (grid_cell(a.g_breg, pos, c)) {
+        const uint64_t sy = a.g_breg.nz, sx = a.g_breg.ny * a.g_breg.nz;
+        braw = Vec3{lerp_cell(a.bgx, c, sy, sx), lerp_cell(a.bgy, c, sy, sx),
+                    lerp_cell(a.bgz, c, sy, sx)};
+      } else {
+        braw = Vec3{0., 0., 0.};
+      }
+    } else {
+      braw = breg_eval(a.breg_model, a.breg, pos);
+    }
+    Vec3 H{0., 0., 0.};
+    double l2 = 0.;
+    l2 += braw.x * braw.x;
+    l2 += braw.y * braw.y;
+    l2 += braw.z * braw.z;
+    if (l2 != 0) {
+      const double den = 1. / sqrt(l2);
+      H = Vec3{braw.x * den, braw.y * den, braw.z * den};
+    }
+    double h2 = 0.;
+    h2 += H.x * H.x;
+    h2 += H.y * H.y;
+    h2 += H.z * H.z;
+    if (h2 < 1e-10)
+      continue; // zero regular field: the raw FFT values stay
+    const double hb = dot0(H, b);
+    const Vec3 par{H.x * hb, H.y * hb, H.z * hb};
+    const Vec3 perp{b.x - par.x, b.y - par.y, b.z - par.z};
+    b = Vec3{(par.x * rho + perp.x / rho) * rhonorm,
+             (par.y * rho + perp.y / rho) * rhonorm,
+             (par.z * rho + perp.z / rho) * rhonorm};
+    c0[idx] = make_double2(b.x, b.y);
+    c1[idx] = make_double2(b.y, b.z);
+  }
+}
+
+// STEP III Gram-Schmidt (brnd_es.cc:211-259), results in bx, by, bz.
+__global__ __launch_bounds__(256) void k_es_gs(const double2 *c0, const double2 *c1,
+                                               double *bx, double *by, double *bz,
+                                               EsArgs a) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const uint64_t sym = sym_index(a.b, idx);
+    const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
+                 wavenum(l, a.b.nz, a.b.lz)};
+    const double2 p0 = c0[idx], q0 = c0[sym], p1 = c1[idx], q1 = c1[sym];
+    const Vec3 b{0.5 * (p0.x + q0.x), 0.5 * (p1.x + q1.x), 0.5 * (p1.y + q1.y)};
+    double k2 = 0.;
+    k2 += k.x * k.x;
+    k2 += k.y * k.y;
+    k2 += k.z * k.z;
+    double b2 = 0.;
+    b2 += b.x * b.x;
+    b2 += b.y * b.y;
+    b2 += b.z * b.z;
+    Vec3 fr = b;
+    if (!(k2 == 0 || b2 == 0)) {
+      const double inv = 1. / k2;
+      const double kb = dot0(k, b);
+      fr = Vec3{cgs::sqrthree * (b.x - k.x * kb * inv),
+                cgs::sqrthree * (b.y - k.y * kb * inv),
+                cgs::sqrthree * (b.z - k.z * kb * inv)};
+    }
+    bx[idx] = cgs::sqrtwo * fr.x;
+    by[idx] = cgs::sqrtwo * fr.y;
+    bz[idx] = cgs::sqrtwo * fr.z;
+  }
+}
+
+// reassemble c0 = bx + i by, c1 = by + i bz (brnd_es.cc:261-270)
+__global__ __launch_bounds__(256) void k_es_reassemble(double2 *c0, double2 *c1,
+                                                       const double *bx,
+                                                       const double *by,
+                                                       const double *bz, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < n; idx += stride) {
+    const double y = by[idx];
+    c0[idx] = make_double2(bx[idx], y);
+    c1[idx] = make_double2(y, bz[idx]);
+  }
+}
+
+// Hermitian extraction / N (brnd_es.cc:274-307)
+__global__ __launch_bounds__(256) void k_es_final(const double2 *c0, const double2 *c1,
+                                                  double *bx, double *by, double *bz,
+                                                  BoxArgs b) {
+  const double inv_n = 1. / static_cast<double>(b.n);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < b.n; idx += stride) {
+    const uint64_t sym = sym_index(b, idx);
+    const double2 p0 = c0[idx], q0 = c0[sym], p1 = c1[idx], q1 = c1[sym];
+    bx[idx] = 0.5 * (p0.x + q0.x) * inv_n;
+    by[idx] = 0.5 * (p1.x + q1.x) * inv_n;
+    bz[idx] = 0.5 * (p1.y + q1.y) * inv_n;
+  }
+}
+
+// -------------------------------------------------------------- TErnd_dft --
+struct DftArgs {
+  BoxArgs b;
+  Vec3 obs;
+  double rms, k0, a0, r0, z0;
+};
+
+// TErnd_dft::spectrum (ternd_dft.cc:17-24)
+__device__ inline double dft_spectrum(const DftArgs &a, double k) {
+  if (k < a.k0)
+    return 0.;
+  return a.rms * pow(k / a.k0, a.a0) / (4 * cgs::pi * k * k);
+}
+
+__global__ __launch_bounds__(256) void k_dft_draw(double2 *tk, DftArgs a,
+                                                  uint64_t seed) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    if (idx == 0) {
+      tk[0] = make_double2(0., 0.);
+      continue;
+    }
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const double kx = wavenum(i, a.b.nx, a.b.lx);
+    const double ky = wavenum(j, a.b.ny, a.b.ly);
+    const double kz = wavenum(l, a.b.nz, a.b.lz);
+    const double ks = sqrt(kx * kx + ky * ky + kz * kz);
+    const double sigma = sqrt(dft_spectrum(a, ks) * a.b.dk3);
+    double g0, g1;
+    gauss_pair(seed, 1, idx, g0, g1);
+    tk[idx] = make_double2(sigma * g0, sigma * g1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dft_rescale(const double2 *tk, double *te,
+                                                     DftArgs a, const double *var) {
+  const double te_var_invsq = 1. / sqrt(var[0]);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const Vec3 pos = vertex_pos(a.b, i, j, l);
+    const double ratio = sqrt(profile(a.obs, a.r0, a.z0, pos)) * a.rms * te_var_invsq;
+    te[idx] = tk[idx].x * ratio;
+  }
+}
+
+// ------------------------------------------------------------------- host --
+static unsigned ew_blocks(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  if (b > 65536)
+    b = 65536;
+  return static_cast<unsigned>(b ? b : 1);
+}
+
+static int check_gen_box(hamx_ctx *ctx, const hamx_box &b) {
+  if (b.nx < 2 || b.ny < 2 || b.nz < 2)
+    return set_err(ctx, HAMX_ERR_ARG, "random-field grid needs at least 2 vertices per axis");
+  if (!is_pow2(b.nx) || !is_pow2(b.ny) || !is_pow2(b.nz) || b.nx > 4096 ||
+      b.ny > 4096 || b.nz > 4096)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "random-field grid axes must be powers of two <= 4096");
+  if (!(b.x_max > b.x_min) || !(b.y_max > b.y_min) || !(b.z_max > b.z_min))
+    return set_err(ctx, HAMX_ERR_ARG, "random-field box limits are not increasing");
+  return HAMX_OK;
+}
+
+} // namespace hamx
+
+using namespace hamx;
+
+extern "C" int hamx_gauss_pairs(hamx_ctx *ctx, uint64_t seed, uint32_t stream,
+                                uint64_t first_pair, uint64_t count, double *out,
+                                uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!out && count)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_gauss_pairs: null output");
+  if (count == 0)
+    return HAMX_OK;
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  double *d = out;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  if (!out_dev) {
+    void *w;
+    HAMX_TRY(ws_get(ctx, kSlotAux, count * 2 * sizeof(double), &w));
+    d = static_cast<double *>(w);
+  }
+  hipLaunchKernelGGL(k_gauss_pairs, dim3(ew_blocks(count)), dim3(256), 0, ctx->stream,
+                     d, seed, stream, first_pair, count);
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev)
+    HAMX_HIP(ctx, hipMemcpyAsync(out, d, count * 2 * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+
+extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                            double *bx, double *by, double *bz, uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !bx || !by || !bz)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_brnd_es: null descriptor or output");
+  HAMX_TRY(check_gen_box(ctx, d->box));
+  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_GRID)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
+  if (!(d->rho > 0.))
+    return set_err(ctx, HAMX_ERR_ARG, "anisotropy ratio rho must be positive");
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  EsArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.b = box_args(d->box);
+  a.obs = Vec3{d->observer[0], d->observer[1], d->observer[2]};
+  a.rms = d->rms;
+  a.k0 = d->k0;
+  a.k1 = d->k1;
+  a.a0 = d->a0;
+  a.a1 = d->a1;
+  a.rho = d->rho;
+  a.r0 = d->r0;
+  a.z0 = d->z0;
+  a.breg_model = d->breg_model;
+  std::memcpy(a.breg, d->breg, sizeof(a.breg));
+  if (d->breg_model == HAMX_BREG_GRID) {
+    if (!(d->breg_x && d->breg_y && d->breg_z))
+      return set_err(ctx, HAMX_ERR_ARG, "breg grid model without breg grid arrays");
+    if (d->breg_box.nx < 2 || d->breg_box.ny < 2 || d->breg_box.nz < 2)
+      return set_err(ctx, HAMX_ERR_ARG, "breg grid needs at least 2 vertices per axis");
+    const size_t nb = d->breg_box.nx * d->breg_box.ny * d->breg_box.nz * sizeof(double);
+    const void *p;
+    HAMX_TRY(stage_in(ctx, kSlotBregX, d->breg_x, nb, in_dev, &p));
+    a.bgx = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregY, d->breg_y, nb, in_dev, &p));
+    a.bgy = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregZ, d->breg_z, nb, in_dev, &p));
+    a.bgz = static_cast<const double *>(p);
+    a.g_breg = grid_geo(d->breg_box);
+  }
+  const uint64_t n = a.b.n;
+  void *w;
+  HAMX_TRY(ws_get(ctx, kSlotRndC0, n * sizeof(double2), &w));
+  double2 *c0 = static_cast<double2 *>(w);
+  HAMX_TRY(ws_get(ctx, kSlotRndC1, n * sizeof(double2), &w));
+  double2 *c1 = static_cast<double2 *>(w);
+  HAMX_TRY(ws_get(ctx, kSlotReduce, (16 + kRedBlocks) * sizeof(double), &w));
+  double *red = static_cast<double *>(w);
+  double *ox = bx, *oy = by, *oz = bz;
+  if (!out_dev) {
+    HAMX_TRY(ws_get(ctx, kSlotRndOut0, n * sizeof(double), &w));
+    ox = static_cast<double *>(w);
+    HAMX_TRY(ws_get(ctx, kSlotRndOut1, n * sizeof(double), &w));
+    oy = static_cast<double *>(w);
+    HAMX_TRY(ws_get(ctx, kSlotRndOut2, n * sizeof(double), &w));
+    oz = static_cast<double *>(w);
+  }
+  const unsigned blocks = ew_blocks(n);
+  hipLaunchKernelGGL(k_es_draw, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a, seed);
+  HAMX_HIP(ctx, hipGetLastError());
+  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(variance_dev(ctx, reinterpret_cast<const double *>(c0), n, red));
+  hipLaunchKernelGGL(k_es_rescale, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a,
+                     static_cast<const double *>(red + 1));
+  HAMX_HIP(ctx, hipGetLastError());
+  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, -1));
+  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, -1));
+  hipLaunchKernelGGL(k_es_gs, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox, oy,
+                     oz, a);
+  hipLaunchKernelGGL(k_es_reassemble, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1,
+                     ox, oy, oz, n);
+  HAMX_HIP(ctx, hipGetLastError());
+  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  hipLaunchKernelGGL(k_es_final, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox,
+                     oy, oz, a.b);
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev) {
+    HAMX_HIP(ctx, hipMemcpyAsync(bx, ox, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HAMX_HIP(ctx, hipMemcpyAsync(by, oy, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HAMX_HIP(ctx, hipMemcpyAsync(bz, oz, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+
+extern "C" int hamx_ternd_dft(hamx_ctx *ctx, const hamx_dft_desc *d, uint64_t seed,
+                              double *te, uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !te)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_ternd_dft: null descriptor or output");
+  HAMX_TRY(check_gen_box(ctx, d->box));
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  DftArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.b = box_args(d->box);
+  a.obs = Vec3{d->observer[0], d->observer[1], d->observer[2]};
+  a.rms = d->rms;
+  a.k0 = d->k0;
+  a.a0 = d->a0;
+  a.r0 = d->r0;
+  a.z0 = d->z0;
+  const uint64_t n = a.b.n;
+  void *w;
+  HAMX_TRY(ws_get(ctx, kSlotRndC0, n * sizeof(double2), &w));
+  double2 *tk = static_cast<double2 *>(w);
+  HAMX_TRY(ws_get(ctx, kSlotReduce, (16 + kRedBlocks) * sizeof(double), &w));
+  double *red = static_cast<double *>(w);
+  double *ot = te;
+  if (!out_dev) {
+    HAMX_TRY(ws_get(ctx, kSlotRndOut0, n * sizeof(double), &w));
+    ot = static_cast<double *>(w);
+  }
+  const unsigned blocks = ew_blocks(n);
+  hipLaunchKernelGGL(k_dft_draw, dim3(blocks), dim3(256), 0, ctx->stream, tk, a, seed);
+  HAMX_HIP(ctx, hipGetLastError());
+  HAMX_TRY(fft3d_dev(ctx, tk, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(variance_dev(ctx, reinterpret_cast<const double *>(tk), n, red));
+  hipLaunchKernelGGL(k_dft_rescale, dim3(blocks), dim3(256), 0, ctx->stream, tk, ot, a,
+                     static_cast<const double *>(red + 1));
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev)
+    HAMX_HIP(ctx, hipMemcpyAsync(te, ot, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}

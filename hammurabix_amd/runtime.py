@@ -1,0 +1,181 @@
+"""Python handle on the C-ABI: a device context and typed calls.
+
+Arrays may be numpy arrays (host memory) or torch tensors on the GPU
+(device memory); all inputs of one call must share a residency, and so must
+all outputs.  Every call is synchronous and raises :class:`HamxError` on a
+negative return code, carrying the library's message -- the Python analogue
+of the reference's ``std::runtime_error``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _capi as capi
+
+
+class HamxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hamx error {code}: {msg}")
+        self.code = code
+
+
+def _is_torch(a: Any) -> bool:
+    return hasattr(a, "data_ptr") and hasattr(a, "is_cuda")
+
+
+def _ptr(a: Any) -> tuple[Optional[int], Optional[bool]]:
+    """(address, on_device) of an array, (None, None) for None."""
+    if a is None:
+        return None, None
+    if _is_torch(a):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        if a.dtype.itemsize != 8:
+            raise ValueError("tensor must hold float64 values")
+        return a.data_ptr(), bool(a.is_cuda)
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous or a.dtype != np.float64:
+            raise ValueError("numpy array must be C-contiguous float64")
+        return a.ctypes.data, False
+    raise TypeError(f"unsupported array type {type(a)!r}")
+
+
+def _residency(arrays: list[Any], what: str) -> bool:
+    res = {_ptr(a)[1] for a in arrays if a is not None}
+    if len(res) > 1:
+        raise ValueError(f"all {what} arrays must share one residency (host or device)")
+    return bool(res.pop()) if res else False
+
+
+def box(nx: int, ny: int, nz: int, x_min: float, x_max: float, y_min: float,
+        y_max: float, z_min: float, z_max: float) -> capi.HamxBox:
+    return capi.HamxBox(nx, ny, nz, x_min, x_max, y_min, y_max, z_min, z_max)
+
+
+class Context:
+    """One HIP stream plus the library's device workspace on `device`."""
+
+    def __init__(self, device: int = 0):
+        self.lib = capi.load()
+        h = C.c_void_p()
+        rc = self.lib.hamx_open(device, C.byref(h))
+        if rc != capi.HAMX_OK:
+            raise HamxError(rc, f"hamx_open(device={device}) failed (no usable GPU?)")
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.hamx_close(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != capi.HAMX_OK:
+            msg = self.lib.hamx_last_error(self.handle)
+            raise HamxError(rc, msg.decode() if msg else "")
+
+    def sync(self) -> None:
+        self._check(self.lib.hamx_sync(self.handle))
+
+    def profile(self, enable: bool = True) -> None:
+        """Start (clearing earlier records) or stop kernel timing."""
+        self._check(self.lib.hamx_profile(self.handle, int(enable)))
+
+    def kernel_times(self) -> list[dict]:
+        """Per-launch records [{kernel, shell, work, ms}] since profile()."""
+        n = C.c_int()
+        self._check(self.lib.hamx_profile_read(self.handle, None, 0, C.byref(n)))
+        arr = (capi.HamxKernelTime * max(n.value, 1))()
+        self._check(self.lib.hamx_profile_read(self.handle, arr, n.value, C.byref(n)))
+        return [dict(kernel=r.kernel.decode(), shell=r.shell, work=r.work, ms=r.ms)
+                for r in arr[:n.value]]
+
+    # ------------------------------------------------------------------ LOS
+    def los_forward(self, desc: capi.HamxLosDesc, maps: dict[str, Any],
+                    brnd: Optional[tuple[Any, Any, Any]] = None,
+                    brnd_box: Optional[capi.HamxBox] = None,
+                    ternd: Any = None, ternd_box: Optional[capi.HamxBox] = None,
+                    breg: Optional[tuple[Any, Any, Any]] = None,
+                    breg_box: Optional[capi.HamxBox] = None,
+                    tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None) -> None:
+        """One Integrator::write_grid pass into `maps` (keys dm, fd, is, qs, us)."""
+        f = capi.HamxFields()
+        inputs = []
+        if brnd is not None:
+            f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
+            f.brnd_box = brnd_box
+            inputs += list(brnd)
+        if ternd is not None:
+            f.ternd = _ptr(ternd)[0]
+            f.ternd_box = ternd_box
+            inputs.append(ternd)
+        if breg is not None:
+            f.breg_x, f.breg_y, f.breg_z = (_ptr(a)[0] for a in breg)
+            f.breg_box = breg_box
+            inputs += list(breg)
+        if tereg is not None:
+            f.tereg = _ptr(tereg)[0]
+            f.tereg_box = tereg_box
+            inputs.append(tereg)
+        outs = [maps.get(k) for k in ("dm", "fd", "is", "qs", "us")]
+        m = capi.HamxMaps(*[_ptr(a)[0] for a in outs])
+        flags = 0
+        if _residency(inputs, "input"):
+            flags |= capi.HAMX_IN_DEVICE
+        if _residency(outs, "output"):
+            flags |= capi.HAMX_OUT_DEVICE
+        self._check(self.lib.hamx_los_forward(self.handle, C.byref(desc), C.byref(f),
+                                              C.byref(m), flags))
+
+    # -------------------------------------------------------- random fields
+    def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
+                breg_grid: Optional[tuple[Any, Any, Any]] = None) -> None:
+        flags = capi.HAMX_OUT_DEVICE if _residency([bx, by, bz], "output") else 0
+        if breg_grid is not None:
+            desc.breg_x, desc.breg_y, desc.breg_z = (_ptr(a)[0] for a in breg_grid)
+            if _residency(list(breg_grid), "input"):
+                flags |= capi.HAMX_IN_DEVICE
+        self._check(self.lib.hamx_brnd_es(self.handle, C.byref(desc), seed,
+                                          _ptr(bx)[0], _ptr(by)[0], _ptr(bz)[0], flags))
+
+    def ternd_dft(self, desc: capi.HamxDftDesc, seed: int, te: Any) -> None:
+        flags = capi.HAMX_OUT_DEVICE if _residency([te], "output") else 0
+        self._check(self.lib.hamx_ternd_dft(self.handle, C.byref(desc), seed,
+                                            _ptr(te)[0], flags))
+
+    def fft3d(self, data: Any, nx: int, ny: int, nz: int, sign: int) -> None:
+        """In-place FFT of interleaved complex128 data stored as float64 pairs."""
+        addr, dev = _ptr(data)
+        flags = capi.HAMX_IN_DEVICE | capi.HAMX_OUT_DEVICE if dev else 0
+        self._check(self.lib.hamx_fft3d(self.handle, addr, nx, ny, nz, sign, flags))
+
+    def gauss_pairs(self, seed: int, stream: int, first: int, count: int,
+                    out: Any) -> None:
+        addr, dev = _ptr(out)
+        flags = capi.HAMX_OUT_DEVICE if dev else 0
+        self._check(self.lib.hamx_gauss_pairs(self.handle, seed, stream, first, count,
+                                              addr, flags))
+
+
+def los_samples(desc: capi.HamxLosDesc) -> int:
+    """Geometric LOS samples of one write_grid pass (host arithmetic only)."""
+    lib = capi.load()
+    n = C.c_uint64()
+    rc = lib.hamx_los_samples(C.byref(desc), C.byref(n))
+    if rc != capi.HAMX_OK:
+        raise HamxError(rc, "invalid LOS descriptor")
+    return int(n.value)

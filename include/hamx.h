@@ -1,0 +1,229 @@
+/*
+ * hamx.h -- C-ABI of the MI355X (gfx950) line-of-sight integrator and
+ * Gaussian random-field generators of hammurabiX.
+ *
+ * Plain C, POD structs, pointers and sizes only.  Every entry point returns
+ * HAMX_OK (0) or a negative HAMX_ERR_* code; hamx_last_error() gives the
+ * message.  The host adapters (C++ Integrator/Brnd_es/TErnd_dft mirrors and
+ * the Python hampyx mirror) turn a negative code into the exception the
+ * reference throws (std::runtime_error / RuntimeError).
+ *
+ * Units are the reference's CGS units after Param parsing: lengths in cm,
+ * magnetic fields in Gauss, frequencies in Hz, energies in erg, angles in rad,
+ * thermal-electron densities in cm^-3, wave numbers in 1/kpc.
+ *
+ * Grids are the reference's vertex grids: x slowest, z fastest,
+ * flat index i*ny*nz + j*nz + l (toolkit.h:29-34).
+ * Maps are HEALPix RING ordered, npix = 12*nside^2 doubles.
+ */
+#ifndef HAMX_H
+#define HAMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HAMX_ABI_VERSION 1
+#define HAMX_MAX_SHELL 16
+
+/* ---- error codes ------------------------------------------------------ */
+#define HAMX_OK 0
+#define HAMX_ERR_ARG (-1)         /* null pointer, bad size, inconsistent descriptor */
+#define HAMX_ERR_UNSUPPORTED (-2) /* model or option without a device implementation */
+#define HAMX_ERR_DEVICE (-3)      /* HIP runtime error */
+#define HAMX_ERR_NOMEM (-4)       /* device allocation failed */
+#define HAMX_ERR_NSIDE (-5)       /* HEALPix nside not a power of two */
+
+/* ---- buffer-residency flags ------------------------------------------- */
+#define HAMX_IN_DEVICE 1u  /* every input array pointer is a device pointer */
+#define HAMX_OUT_DEVICE 2u /* every output array pointer is a device pointer */
+
+/* ---- model selectors (the Param type strings) ------------------------- */
+/* regular magnetic field, Param::breg_type (param.cc:161-398) */
+#define HAMX_BREG_NONE 0 /* <regular cue="0">: Breg base class, zero field */
+#define HAMX_BREG_LSA 1  /* "lsa":  breg[0..3] = b0 [G], psi0, psi1, chi0 [rad] */
+#define HAMX_BREG_CART 2 /* "cart": breg[0..2] = bx, by, bz [G] */
+#define HAMX_BREG_UNIF 3 /* "unif": breg[0..2] = bp, bv [G], l0 [rad] */
+#define HAMX_BREG_GRID 4 /* <fieldio><breg read="1">: trilinear gather of fields.breg_* */
+/* regular thermal electrons, Param::tereg_type (param.cc:462-667) */
+#define HAMX_TEREG_NONE 0 /* zero density */
+#define HAMX_TEREG_UNIF 1 /* "unif": tereg[0..1] = n0 [cm^-3], r0 [cm] */
+#define HAMX_TEREG_GRID 2 /* <fieldio><tereg read="1">: gather of fields.tereg */
+/* cosmic-ray electrons, Param::cre_type (param.cc:731-778) */
+#define HAMX_CRE_NONE 0 /* no CRE model: sync output is rejected */
+#define HAMX_CRE_ANA 1  /* "analytic": cre[0..6] = alpha, beta, theta, r0 [cm], z0 [cm], E0 [erg], j0 */
+#define HAMX_CRE_UNIF 2 /* "unif":     cre[0..3] = alpha, r0 [cm], E0 [erg], j0 */
+
+/* ---- LOS march schedules ---------------------------------------------- */
+#define HAMX_SCHED_AUTO 0   /* wavefront per ray */
+#define HAMX_SCHED_THREAD 1 /* one thread per ray, samples summed in the reference order */
+#define HAMX_SCHED_WAVE 2   /* one wavefront per ray, lanes over consecutive steps, FD by wave scan */
+
+/* ---- descriptors ------------------------------------------------------- */
+
+/* Cartesian vertex box (Param::grid_* nx..z_max, param.h:32-34). */
+typedef struct hamx_box {
+  uint64_t nx, ny, nz;
+  double x_min, x_max, y_min, y_max, z_min, z_max; /* cm */
+} hamx_box;
+
+/* Field grids gathered by the march.  NULL arrays mean "absent".
+ * brnd_* is Grid_brnd::{bx,by,bz} (grid.h:94), ternd is Grid_ternd::te
+ * (grid.h:144), breg_* / tereg are the imported regular-field grids used by
+ * HAMX_BREG_GRID / HAMX_TEREG_GRID. */
+typedef struct hamx_fields {
+  const double *brnd_x, *brnd_y, *brnd_z;
+  hamx_box brnd_box;
+  const double *ternd;
+  hamx_box ternd_box;
+  const double *breg_x, *breg_y, *breg_z;
+  hamx_box breg_box;
+  const double *tereg;
+  hamx_box tereg_box;
+} hamx_fields;
+
+/* One Integrator::write_grid pass (integrator.cc:23-134) for one <sync>
+ * entry (the one Param::grid_obs.*.back() names). */
+typedef struct hamx_los_desc {
+  double observer[3]; /* Param::observer, cm */
+  uint32_t n_shell;   /* Param::grid_obs.total_shell */
+  uint32_t nside_shell[HAMX_MAX_SHELL];
+  double radii_shell[HAMX_MAX_SHELL + 1]; /* cm, n_shell+1 entries */
+  double oc_r_res;                        /* cm */
+  double gc_r_min, gc_r_max, gc_z_min, gc_z_max; /* cm */
+  int32_t do_dm, do_fd, do_sync;
+  uint32_t nside_dm, nside_fd, nside_sync;
+  double sync_freq; /* Hz */
+  int32_t breg_model;
+  double breg[8];
+  int32_t tereg_model;
+  double tereg[4];
+  int32_t cre_model;
+  double cre[8];
+  int32_t schedule; /* HAMX_SCHED_* */
+  int32_t reserved;
+} hamx_los_desc;
+
+/* Output maps (Grid_obs::{dm,fd,is,qs,us}_map after write_grid, grid.h:188).
+ * Sizes: dm npix(nside_dm); is/qs/us npix(nside_sync); fd npix(nside_fd) when
+ * do_fd, else npix(nside_sync).  Unused maps may be NULL.  Values are the
+ * raw write_grid results: DM in cm^-3 cm and FD in rad cm^-2 (the export
+ * rescale of grid_obs.cc:57,78 is the caller's), I/Q/U in K (CMB).
+ * Every map is overwritten. */
+typedef struct hamx_maps {
+  double *dm, *fd, *is, *qs, *us;
+} hamx_maps;
+
+/* Brnd_es generator (brnd_es.cc:76-308) on box `box`.  The anisotropy
+ * direction uses the regular field model breg_model/breg (or the grid in
+ * `breg_grid_*` for HAMX_BREG_GRID). */
+typedef struct hamx_es_desc {
+  hamx_box box;
+  double observer[3];
+  double rms;          /* Gauss */
+  double k0, k1;       /* 1/kpc */
+  double a0, a1, rho;  /* spectral indices, anisotropy ratio */
+  double r0, z0;       /* energy profile scales, cm */
+  int32_t breg_model;
+  int32_t reserved;
+  double breg[8];
+  const double *breg_x, *breg_y, *breg_z;
+  hamx_box breg_box;
+} hamx_es_desc;
+
+/* TErnd_dft generator (ternd_dft.cc:35-128) on box `box`. */
+typedef struct hamx_dft_desc {
+  hamx_box box;
+  double observer[3];
+  double rms; /* cm^-3 */
+  double k0;  /* 1/kpc */
+  double a0;
+  double r0, z0; /* cm */
+} hamx_dft_desc;
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct hamx_ctx hamx_ctx;
+
+/* Open a context on HIP device `device`: one stream of default priority and
+ * a grow-only device workspace.  One context per host thread. */
+int hamx_open(int device, hamx_ctx **out);
+void hamx_close(hamx_ctx *ctx);
+const char *hamx_last_error(const hamx_ctx *ctx);
+int hamx_abi_version(void);
+/* Synchronise the context stream. */
+int hamx_sync(hamx_ctx *ctx);
+
+/* Kernel timing on the context stream (HIP events around each launch of the
+ * march and accumulate kernels).  hamx_profile(ctx, 1) clears the record
+ * and starts recording; hamx_profile_read copies up to `max` records. */
+typedef struct hamx_kernel_time {
+  char kernel[32]; /* "los_wave", "los_thread", "accumulate", ... */
+  uint32_t shell;  /* shell index of a march / accumulate launch */
+  uint32_t reserved;
+  uint64_t work;   /* geometric samples of a march, pixels otherwise */
+  double ms;       /* launch duration */
+} hamx_kernel_time;
+int hamx_profile(hamx_ctx *ctx, int enable);
+int hamx_profile_read(hamx_ctx *ctx, hamx_kernel_time *out, int max, int *count);
+
+/* Device memory helpers for callers without their own allocator (the C++
+ * host adapter keeps grids resident across frequency passes with these). */
+int hamx_dev_alloc(hamx_ctx *ctx, size_t bytes, void **dptr);
+int hamx_dev_free(hamx_ctx *ctx, void *dptr);
+int hamx_memcpy_h2d(hamx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int hamx_memcpy_d2h(hamx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- LOS integrator ----------------------------------------------------- */
+
+/* One Integrator::write_grid pass: all shells in order, FD carried from the
+ * inner shells through fd_map (integrator.cc:78-86,130-132), shell maps
+ * temp_convert-ed (integrator.cc:102-112) and accumulated into the output
+ * maps with HEALPix interpolation across nside (hamdis.h:698-721).
+ * Replaces Integrator::write_grid (integrator.h:25-28, integrator.cc:23-134).
+ * flags: HAMX_IN_DEVICE / HAMX_OUT_DEVICE.  Synchronous. */
+int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
+                     const hamx_fields *fields, hamx_maps *out, uint32_t flags);
+
+/* Geometric LOS samples of one pass, S_geo = sum_s npix_s * step_s
+ * (integrator.cc:62,173 trip counts); host-side arithmetic only. */
+int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples);
+
+/* ---- random fields ------------------------------------------------------ */
+
+/* Brnd_es::write_grid (bfield.h:279-280, brnd_es.cc:76-308): spectrum draw,
+ * backward FFT, energy-profile rescale with LSA anisotropy, forward FFT,
+ * Gram-Schmidt divergence cleaning, backward FFT, Hermitian extraction /N.
+ * Gaussian draws come from Philox4x32-10 keyed by `seed` (a counter-based
+ * stream: one realisation per seed, independent of thread count).
+ * Outputs bx, by, bz: nx*ny*nz doubles each. */
+int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                 double *bx, double *by, double *bz, uint32_t flags);
+
+/* TErnd_dft::write_grid (tefield.h:159-160, ternd_dft.cc:35-128).
+ * Output te: nx*ny*nz doubles. */
+int hamx_ternd_dft(hamx_ctx *ctx, const hamx_dft_desc *d, uint64_t seed,
+                   double *te, uint32_t flags);
+
+/* In-place unnormalised 3D c2c FFT of interleaved complex128 data
+ * (re,im pairs, x slowest), sign = +1 (FFTW_BACKWARD) or -1 (FFTW_FORWARD),
+ * the transform of the reference's plans (grid_brnd.cc:42-54).
+ * Each of nx, ny, nz must be a power of two in [1, 4096]. */
+int hamx_fft3d(hamx_ctx *ctx, double *data, uint64_t nx, uint64_t ny,
+               uint64_t nz, int sign, uint32_t flags);
+
+/* The Gaussian draws of the spectrum step: out[2*m], out[2*m+1] are the two
+ * N(0,1) variates of draw pair m of stream `stream` (0: Brnd_es c0/c1,
+ * 1: TErnd_dft te_k) for mode index `mode` = m / pairs_per_mode.  Exposed so
+ * that a CPU restatement can be checked element-wise.  count = pairs. */
+int hamx_gauss_pairs(hamx_ctx *ctx, uint64_t seed, uint32_t stream,
+                     uint64_t first_pair, uint64_t count, double *out,
+                     uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAMX_H */

@@ -1,0 +1,708 @@
+// hamx_oracle.cc -- CPU restatement of the hammurabiX LOS integrator and
+// random-field generators (FP64, OpenMP).
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path, the CPU baseline
+// timed by bench.py, and nothing else.  The product never links it.
+//
+// Every routine follows the reference's arithmetic in the reference's
+// operation order; the citations give the file:line it restates.  Parity
+// status of each part (what pins it) is listed in oracle/README.md.
+#include "hamx_oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------- units --
+// CGS constants (hamunits.h:8-68)
+const double kPi = 3.141592653589793238462643383279502884197;
+const double kHalfPi = 1.570796326794896619231321691639751442099;
+const double kTwoPi = 6.283185307179586476925286766559005768394;
+const double kOneThird = 0.333333333333333333333333333333333333333;
+const double kTwoThirds = 0.666666666666666666666666666666666666667;
+const double kSqrt2 = 1.4142135623730950488016887242096980785697;
+const double kSqrt3 = 1.7320508075688772935274463415058723669428;
+const double kKpc = 3.0856775806e+21;
+const double kM = 1.e2;
+const double kSec = 1.;
+const double kEv = 1.60217733e-12;
+const double kGeV = 1.e9 * kEv;
+const double kC = 2.99792458e+10;
+const double kH = 6.626075540e-27;
+const double kQe = 4.80320425e-10;
+const double kMec2 = 0.51099907e-3 * kGeV;
+const double kMec = kMec2 / kC;
+const double kKB = 1.380622e-16;
+const double kUndef = -1.6375e30;
+
+// ------------------------------------------------------------- vector3 --
+struct V3 {
+  double e[3];
+  double operator[](int i) const { return e[i]; }
+  double &operator[](int i) { return e[i]; }
+};
+V3 v3(double x, double y, double z) {
+  V3 v;
+  v.e[0] = x;
+  v.e[1] = y;
+  v.e[2] = z;
+  return v;
+}
+V3 add(const V3 &a, const V3 &b) {
+  V3 t = a;
+  for (int i = 0; i < 3; ++i)
+    t[i] += b[i];
+  return t;
+}
+V3 sub(const V3 &a, const V3 &b) {
+  V3 t = a;
+  for (int i = 0; i < 3; ++i)
+    t[i] -= b[i];
+  return t;
+}
+V3 mul(const V3 &a, double s) {
+  V3 t = a;
+  for (int i = 0; i < 3; ++i)
+    t[i] *= s;
+  return t;
+}
+// hamvec.h:150-165
+double length(const V3 &a) {
+  double t = 0.;
+  for (int i = 0; i < 3; ++i)
+    t += a[i] * a[i];
+  return std::sqrt(t);
+}
+// hamvec.h:189-195
+double dot(const V3 &a, const V3 &b) {
+  double t = 0.;
+  for (int i = 0; i < 3; ++i)
+    t += a[i] * b[i];
+  return t;
+}
+// hamvec.h:198-206
+V3 cross(const V3 &a, const V3 &b) {
+  V3 t;
+  for (int i = 0; i < 3; ++i)
+    t[i] = a[(i + 1) % 3] * b[(i + 2) % 3] - a[(i + 2) % 3] * b[(i + 1) % 3];
+  return t;
+}
+
+// ------------------------------------------------------------- HEALPix --
+// Hampix RING geometry (hamdis.h:223-392)
+struct Ring {
+  uint64_t nside, npix, ncap;
+  int order;
+  double fact1, fact2;
+  explicit Ring(uint64_t n) {
+    nside = n;
+    int o = 0;
+    for (uint64_t t = n; t > 1; t >>= 1)
+      ++o;
+    order = o;
+    const uint64_t npface = n * n;
+    npix = 12 * npface;
+    ncap = (npface - n) << 1;
+    fact2 = 4. / npix;
+    fact1 = (n << 1) * fact2;
+  }
+  static uint64_t isqrt(uint64_t v) {
+    return static_cast<uint64_t>(std::floor(std::sqrt(static_cast<double>(v) + 0.5)));
+  }
+  // fillpoint (hamdis.h:302-356) + Hamp::norm (hamp.h:23-28)
+  void pix2ang(uint64_t idx, double &theta, double &phi) const {
+    double z = 0.0, sth = 0.0;
+    bool have_sth = false;
+    if (idx < ncap) {
+      const uint64_t iring = (1 + isqrt(1 + 2 * idx)) >> 1;
+      const uint64_t iphi = (idx + 1) - 2 * iring * (iring - 1);
+      const double tmp = (iring * iring) * fact2;
+      z = 1.0 - tmp;
+      if (z > 0.99) {
+        sth = std::sqrt(tmp * (2.0 - tmp));
+        have_sth = true;
+      }
+      phi = (iphi - 0.5) * kHalfPi / iring;
+    } else if (idx < (npix - ncap)) {
+      const uint64_t nl4 = 4 * nside;
+      const uint64_t ip = idx - ncap;
+      const uint64_t tmp = ip >> (order + 2);
+      const uint64_t iring = tmp + nside;
+      const uint64_t iphi = ip - nl4 * tmp + 1;
+      const double fodd = ((iring + nside) & 1) ? 1 : 0.5;
+      z = (static_cast<double>(2 * nside) - static_cast<double>(iring)) * fact1;
+      phi = (iphi - fodd) * kHalfPi * 1.5 * fact1;
+    } else {
+      const uint64_t ip = npix - idx;
+      const uint64_t iring = (1 + isqrt(2 * ip - 1)) >> 1;
+      const uint64_t iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+      const double tmp = (iring * iring) * fact2;
+      z = tmp - 1.0;
+      if (z < -0.99) {
+        sth = std::sqrt(tmp * (2.0 - tmp));
+        have_sth = true;
+      }
+      phi = (iphi - 0.5) * kHalfPi / iring;
+    }
+    theta = have_sth ? std::atan2(sth, z) : std::acos(z);
+    theta = std::fmod(theta, kPi);
+    theta += (theta < 0) * kPi;
+    phi = std::fmod(phi, kTwoPi);
+    phi += (phi < 0) * kTwoPi;
+  }
+  // ring_above (hamdis.h:358-365)
+  uint64_t rabove(double z) const {
+    const double az = std::fabs(z);
+    if (az <= kTwoThirds)
+      return static_cast<uint64_t>(nside * (2 - 1.5 * z));
+    const uint64_t iring = static_cast<uint64_t>(nside * std::sqrt(3 * (1 - az)));
+    return (z > 0) ? iring : 4 * nside - iring - 1;
+  }
+  // get_ring_info2 (hamdis.h:367-392)
+  void rinfo(uint64_t ring, uint64_t &startpix, int64_t &ringpix, double &theta,
+             bool &shifted) const {
+    const uint64_t northring = (ring > 2 * nside) ? 4 * nside - ring : ring;
+    if (northring < nside) {
+      const double tmp = northring * northring * fact2;
+      const double costheta = 1 - tmp;
+      const double sintheta = std::sqrt(tmp * (2 - tmp));
+      theta = std::atan2(sintheta, costheta);
+      ringpix = 4 * northring;
+      shifted = true;
+      startpix = 2 * northring * (northring - 1);
+    } else {
+      theta = std::acos((static_cast<double>(2 * nside) -
+                         static_cast<double>(northring)) * fact1);
+      ringpix = 4 * nside;
+      shifted = ((northring - nside) & 1) == 0;
+      startpix = ncap + (northring - nside) * ringpix;
+    }
+    if (northring != ring) {
+      theta = kPi - theta;
+      startpix = npix - startpix - ringpix;
+    }
+  }
+  // interpolate (hamdis.h:609-696)
+  double interpolate(const double *map, double theta, double phi) const {
+    uint64_t pix[4] = {0, 0, 0, 0};
+    double wght[4] = {0, 0, 0, 0};
+    const double z = std::cos(theta);
+    const uint64_t ring1 = rabove(z);
+    const uint64_t ring2 = ring1 + 1;
+    double theta1 = 0.0, theta2 = 0.0;
+    for (int r = 0; r < 2; ++r) {
+      const uint64_t ring = r == 0 ? ring1 : ring2;
+      const bool ok = r == 0 ? (ring1 > 0) : (ring2 < 4 * nside);
+      if (!ok)
+        continue;
+      uint64_t start;
+      int64_t ringpix;
+      bool shift;
+      double th;
+      rinfo(ring, start, ringpix, th, shift);
+      if (r == 0)
+        theta1 = th;
+      else
+        theta2 = th;
+      const double dphi = 6.283185307179586476925286766559005768394 / ringpix;
+      const double tmp = (phi / dphi - 0.5 * shift);
+      int64_t i1 = static_cast<int64_t>(tmp) - static_cast<int64_t>(tmp < 0);
+      const double weight = (phi - (i1 + 0.5 * shift) * dphi) / dphi;
+      int64_t i2 = i1 + 1;
+      i1 += ringpix * (i1 < 0);
+      i2 -= ringpix * (i2 >= ringpix);
+      pix[2 * r] = start + i1;
+      pix[2 * r + 1] = start + i2;
+      wght[2 * r] = 1.0 - weight;
+      wght[2 * r + 1] = weight;
+    }
+    if (ring1 == 0) {
+      const double wtheta = theta / theta2;
+      wght[2] *= wtheta;
+      wght[3] *= wtheta;
+      const double fac = (1.0 - wtheta) * 0.25;
+      wght[0] = fac;
+      wght[1] = fac;
+      wght[2] += fac;
+      wght[3] += fac;
+      pix[0] = (pix[2] + 2) & 3;
+      pix[1] = (pix[3] + 2) & 3;
+    } else if (ring2 == 4 * nside) {
+      const double wtheta = (theta - theta1) / (kPi - theta1);
+      wght[0] *= (1.0 - wtheta);
+      wght[1] *= (1.0 - wtheta);
+      const double fac = wtheta * 0.25;
+      wght[0] += fac;
+      wght[1] += fac;
+      wght[2] = fac;
+      wght[3] = fac;
+      pix[2] = ((pix[0] + 2) & 3) + npix - 4;
+      pix[3] = ((pix[1] + 2) & 3) + npix - 4;
+    } else {
+      const double wtheta = (theta - theta1) / (theta2 - theta1);
+      wght[0] *= (1.0 - wtheta);
+      wght[1] *= (1.0 - wtheta);
+      wght[2] *= wtheta;
+      wght[3] *= wtheta;
+    }
+    double wtot = 0.0, res = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      const double val = map[pix[i]];
+      if (val > -1.63749e30) {
+        res += val * wght[i];
+        wtot += wght[i];
+      }
+    }
+    return (wtot == 0.0) ? kUndef : res / wtot;
+  }
+};
+
+// accumulate (hamdis.h:698-721)
+void accumulate(double *dst, uint64_t nside_dst, const double *src,
+                uint64_t nside_src) {
+  const Ring rd(nside_dst), rs(nside_src);
+  if (rd.npix == rs.npix) {
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)rd.npix; ++i)
+      dst[i] = dst[i] + src[i];
+  } else {
+#pragma omp parallel for
+    for (int64_t i = 0; i < (int64_t)rd.npix; ++i) {
+      double th, ph;
+      rd.pix2ang(i, th, ph);
+      dst[i] = dst[i] + rs.interpolate(src, th, ph);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- grids --
+// Brnd::read_grid / TErnd::read_grid (brnd.cc:25-88, ternd.cc:21-71)
+bool cell(const hamx_box &b, const V3 &pos, uint64_t &idx, double &xd,
+          double &yd, double &zd) {
+  double tmp = (b.nx - 1) * (pos[0] - b.x_min) / (b.x_max - b.x_min);
+  if (tmp <= 0 or tmp >= b.nx - 1)
+    return false;
+  const uint64_t xl = (uint64_t)std::floor(tmp);
+  xd = tmp - xl;
+  tmp = (b.ny - 1) * (pos[1] - b.y_min) / (b.y_max - b.y_min);
+  if (tmp <= 0 or tmp >= b.ny - 1)
+    return false;
+  const uint64_t yl = (uint64_t)std::floor(tmp);
+  yd = tmp - yl;
+  tmp = (b.nz - 1) * (pos[2] - b.z_min) / (b.z_max - b.z_min);
+  if (tmp <= 0 or tmp >= b.nz - 1)
+    return false;
+  const uint64_t zl = (uint64_t)std::floor(tmp);
+  zd = tmp - zl;
+  idx = xl * b.ny * b.nz + yl * b.nz + zl;
+  return true;
+}
+
+double trilinear(const double *f, const hamx_box &b, uint64_t idx, double xd,
+                 double yd, double zd) {
+  const uint64_t sy = b.nz, sx = b.ny * b.nz;
+  const double i1 = f[idx] * (1. - zd) + f[idx + 1] * zd;
+  const double i2 = f[idx + sy] * (1. - zd) + f[idx + sy + 1] * zd;
+  const double j1 = f[idx + sx] * (1. - zd) + f[idx + sx + 1] * zd;
+  const double j2 = f[idx + sx + sy] * (1. - zd) + f[idx + sx + sy + 1] * zd;
+  const double w1 = i1 * (1. - yd) + i2 * yd;
+  const double w2 = j1 * (1. - yd) + j2 * yd;
+  return w1 * (1. - xd) + w2 * xd;
+}
+
+V3 read_grid3(const double *bx, const double *by, const double *bz,
+              const hamx_box &b, const V3 &pos) {
+  uint64_t idx;
+  double xd, yd, zd;
+  if (!cell(b, pos, idx, xd, yd, zd))
+    return v3(0., 0., 0.);
+  return v3(trilinear(bx, b, idx, xd, yd, zd), trilinear(by, b, idx, xd, yd, zd),
+            trilinear(bz, b, idx, xd, yd, zd));
+}
+
+double read_grid1(const double *f, const hamx_box &b, const V3 &pos) {
+  uint64_t idx;
+  double xd, yd, zd;
+  if (!cell(b, pos, idx, xd, yd, zd))
+    return 0.;
+  return trilinear(f, b, idx, xd, yd, zd);
+}
+
+// --------------------------------------------------------------- models --
+// Breg_lsa::write_field (breg_lsa.cc:9-26), Breg_cart, Breg_unif
+V3 breg_model(int model, const double *p, const V3 &pos) {
+  if (model == HAMX_BREG_LSA) {
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    if (r > (20. * kKpc) or r < (3. * kKpc))
+      return v3(0., 0., 0.);
+    const double cos_p = pos[0] / r;
+    const double sin_p = pos[1] / r;
+    const double psi = p[1] + p[2] * std::log(r / (8. * kKpc));
+    const double chi = p[3] * std::tanh(pos[2] / (1. * kKpc));
+    return v3(p[0] * (std::sin(psi) * std::cos(chi) * cos_p -
+                      std::cos(psi) * std::cos(chi) * sin_p),
+              p[0] * (std::sin(psi) * std::cos(chi) * sin_p +
+                      std::cos(psi) * std::cos(chi) * cos_p),
+              p[0] * std::sin(chi));
+  }
+  if (model == HAMX_BREG_CART)
+    return v3(p[0], p[1], p[2]);
+  if (model == HAMX_BREG_UNIF)
+    return v3(p[0] * std::cos(p[2]), p[0] * std::sin(p[2]), p[1]);
+  return v3(0., 0., 0.);
+}
+
+V3 breg_field(int model, const double *p, const hamx_fields *f, const V3 &pos) {
+  if (model == HAMX_BREG_GRID)
+    return read_grid3(f->breg_x, f->breg_y, f->breg_z, f->breg_box, pos);
+  return breg_model(model, p, pos);
+}
+
+// CRE_ana / CRE_unif (cre_ana.cc:11-47, cre_unif.cc)
+struct Cre {
+  int model;
+  const double *p;
+  V3 obs;
+  double flux_idx(const V3 &pos) const {
+    if (model == HAMX_CRE_UNIF)
+      return -(p[0]);
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]) / kKpc;
+    const double z = std::fabs(pos[2]) / kKpc;
+    return -p[0] + p[1] * r + p[2] * z;
+  }
+  double spatial_profile(const V3 &pos) const {
+    if (model == HAMX_CRE_UNIF)
+      return (length(sub(pos, obs)) > p[1]) ? 0 : 1;
+    const double R0 = std::sqrt(obs[0] * obs[0] + obs[1] * obs[1]);
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    return std::exp((R0 - r) / p[3]) *
+           (1. / (std::cosh(pos[2] / p[4]) * std::cosh(pos[2] / p[4])));
+  }
+  double flux_norm(const V3 &pos) const {
+    const double E0 = model == HAMX_CRE_UNIF ? p[2] : p[5];
+    const double j0 = model == HAMX_CRE_UNIF ? p[3] : p[6];
+    const double gamma0 = E0 / kMec2;
+    const double beta0 = std::sqrt(1. - 1. / gamma0);
+    const double unit = 4. * kPi * kMec / (kGeV * kM * kM * kSec * beta0);
+    const double norm = j0 * std::pow(gamma0, -flux_idx(obs));
+    return norm * unit * spatial_profile(pos);
+  }
+};
+
+// sync_emissivity_t / _p analytic branch (integrator.cc:333-346,402-415)
+double sync_t(const Cre &cre, double freq, const V3 &pos, double bper) {
+  const double index = cre.flux_idx(pos);
+  const double norm = cre.flux_norm(pos) * 1.73205081 * (kQe * kQe * kQe) *
+                      std::fabs(bper) / (4. * kPi * kMec2 * (1. - index));
+  const double A = 2. * kPi * freq * kMec / (3. * kQe * std::fabs(bper));
+  return norm * (std::pow(A, 0.5 * (index + 1)) *
+                 std::tgamma(-0.25 * index + 19. / 12.) *
+                 std::tgamma(-0.25 * index - 1. / 12.));
+}
+double sync_p(const Cre &cre, double freq, const V3 &pos, double bper) {
+  const double index = cre.flux_idx(pos);
+  const double norm = cre.flux_norm(pos) * 1.73205081 * (kQe * kQe * kQe) *
+                      std::fabs(bper) / (16. * kPi * kMec2);
+  const double A = 2. * kPi * freq * kMec / (3. * kQe * std::fabs(bper));
+  return norm * (std::pow(A, 0.5 * (index + 1)) *
+                 std::tgamma(-0.25 * index + 7. / 12.) *
+                 std::tgamma(-0.25 * index - 1. / 12.));
+}
+
+// ----------------------------------------------------------- integrator --
+V3 los_versor(double the, double phi) {
+  return v3(std::cos(phi) * std::sin(the), std::sin(phi) * std::sin(the),
+            std::cos(the));
+}
+double sync_ipa(const V3 &b, double the, double phi) {
+  const V3 et = v3(std::cos(the) * std::cos(phi), std::cos(the) * std::sin(phi),
+                   -std::sin(the));
+  const V3 ep = v3(-std::sin(phi), std::cos(phi), 0.);
+  return std::atan2(-dot(et, b), -dot(ep, b));
+}
+double temp_convert(double t, double freq) {
+  const double p = kH * freq / (kKB * 2.725);
+  return t * (std::exp(p) - 1.) * (std::exp(p) - 1.) / (p * p * std::exp(p));
+}
+
+struct Shell {
+  uint64_t step;
+  double d_start, d_stop, delta;
+  std::vector<double> dist;
+};
+// assemble_shell_ref (integrator.cc:231-244)
+Shell shell_ref(const hamx_los_desc *d, uint32_t s) {
+  Shell sh;
+  sh.d_start = d->radii_shell[s];
+  sh.d_stop = d->radii_shell[s + 1];
+  sh.delta = d->oc_r_res;
+  const double st = std::floor(sh.d_stop / sh.delta - sh.d_start / sh.delta);
+  sh.step = st > 0 ? (uint64_t)st : 0;
+  sh.delta = (sh.d_stop - sh.d_start) / sh.step;
+  sh.dist.reserve(sh.step);
+  for (uint64_t i = 0; i < sh.step; ++i)
+    sh.dist.push_back(sh.d_start + (i + 0.5) * sh.delta);
+  return sh;
+}
+
+// radial_integration (integrator.cc:136-228)
+void radial(const hamx_los_desc *d, const hamx_fields *f, const Shell &sh,
+            double the, double phi, double fd_inner, double o[5]) {
+  const V3 obs = v3(d->observer[0], d->observer[1], d->observer[2]);
+  const bool do_fd_any = d->do_fd || d->do_sync;
+  double inner = 0.;
+  if (do_fd_any)
+    inner = fd_inner;
+  double dm = 0., fd = 0., is = 0., qs = 0., us = 0.;
+  const V3 l = los_versor(the, phi);
+  double lambda_square = 0, i2bt = 0.;
+  const double fd_fore = -(kQe * kQe * kQe) / (2. * kPi * kMec2 * kMec2);
+  if (d->do_sync) {
+    lambda_square = (kC / d->sync_freq) * (kC / d->sync_freq);
+    i2bt = kC * kC / (2. * kKB * d->sync_freq * d->sync_freq);
+  }
+  Cre cre;
+  cre.model = d->cre_model;
+  cre.p = d->cre;
+  cre.obs = obs;
+  const bool has_brnd = f && f->brnd_x && f->brnd_y && f->brnd_z;
+  for (uint64_t k = 0; k < sh.step; ++k) {
+    const V3 oc = mul(l, sh.dist[k]);
+    const V3 pos = add(oc, obs);
+    if (length(pos) < d->gc_r_min)
+      continue;
+    if (length(pos) > d->gc_r_max)
+      continue;
+    if (pos[2] < d->gc_z_min)
+      continue;
+    if (pos[2] > d->gc_z_max)
+      continue;
+    V3 B = breg_field(d->breg_model, d->breg, f, pos);
+    if (has_brnd)
+      B = add(B, read_grid3(f->brnd_x, f->brnd_y, f->brnd_z, f->brnd_box, pos));
+    const double b_par = dot(B, l);
+    const double b_per = length(cross(B, l));
+    double te = 0.;
+    if (d->tereg_model == HAMX_TEREG_UNIF)
+      te = (length(sub(pos, obs)) > d->tereg[1]) ? 0. : d->tereg[0];
+    else if (d->tereg_model == HAMX_TEREG_GRID)
+      te = read_grid1(f->tereg, f->tereg_box, pos);
+    if (f && f->ternd)
+      te += read_grid1(f->ternd, f->ternd_box, pos);
+    te *= double(te > 0.);
+    if (d->do_dm)
+      dm += te * sh.delta;
+    if (do_fd_any)
+      fd += te * b_par * fd_fore * sh.delta;
+    if (d->do_sync) {
+      const double jt = sync_t(cre, d->sync_freq, pos, b_per) * sh.delta * i2bt;
+      const double jp = sync_p(cre, d->sync_freq, pos, b_per) * sh.delta * i2bt;
+      is += jt;
+      const double qui = (inner + fd) * lambda_square + sync_ipa(B, the, phi);
+      qs += std::cos(2. * qui) * jp;
+      us += std::sin(2. * qui) * jp;
+    }
+  }
+  o[0] = dm;
+  o[1] = fd;
+  o[2] = is;
+  o[3] = qs;
+  o[4] = us;
+}
+
+bool pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+} // namespace
+
+// ================================================================= C API ==
+extern "C" {
+
+void orc_pix2ang(uint64_t nside, uint64_t ipix, double *theta, double *phi) {
+  Ring(nside).pix2ang(ipix, *theta, *phi);
+}
+
+void orc_pix2ang_map(uint64_t nside, double *tp) {
+  const Ring r(nside);
+  for (uint64_t i = 0; i < r.npix; ++i)
+    r.pix2ang(i, tp[2 * i], tp[2 * i + 1]);
+}
+
+double orc_interpolate(const double *map, uint64_t nside, double theta,
+                       double phi) {
+  return Ring(nside).interpolate(map, theta, phi);
+}
+
+void orc_accumulate(double *dst, uint64_t nside_dst, const double *src,
+                    uint64_t nside_src) {
+  accumulate(dst, nside_dst, src, nside_src);
+}
+
+void orc_shell_ref(double d_start, double d_stop, double res, uint64_t *step,
+                   double *delta) {
+  hamx_los_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.radii_shell[0] = d_start;
+  d.radii_shell[1] = d_stop;
+  d.oc_r_res = res;
+  const Shell s = shell_ref(&d, 0);
+  *step = s.step;
+  *delta = s.delta;
+}
+
+void orc_los_versor(double theta, double phi, double out[3]) {
+  const V3 l = los_versor(theta, phi);
+  for (int i = 0; i < 3; ++i)
+    out[i] = l[i];
+}
+
+double orc_los_parproj(const double b[3], const double l[3]) {
+  return dot(v3(b[0], b[1], b[2]), v3(l[0], l[1], l[2]));
+}
+
+double orc_los_perproj(const double b[3], const double l[3]) {
+  return length(cross(v3(b[0], b[1], b[2]), v3(l[0], l[1], l[2])));
+}
+
+double orc_sync_ipa(const double b[3], double theta, double phi) {
+  return sync_ipa(v3(b[0], b[1], b[2]), theta, phi);
+}
+
+double orc_temp_convert(double t, double freq) { return temp_convert(t, freq); }
+
+// integrator.h:109-118
+int orc_check_upper(double value, double limit) { return value > limit; }
+int orc_check_lower(double value, double limit) { return value < limit; }
+
+void orc_breg(int model, const double *par, const double pos[3], double out[3]) {
+  const V3 b = breg_model(model, par, v3(pos[0], pos[1], pos[2]));
+  for (int i = 0; i < 3; ++i)
+    out[i] = b[i];
+}
+
+double orc_tereg_unif(const double *par, const double obs[3], const double pos[3]) {
+  const V3 d = sub(v3(pos[0], pos[1], pos[2]), v3(obs[0], obs[1], obs[2]));
+  return (length(d) > par[1]) ? 0. : par[0];
+}
+
+void orc_read_grid3(const double *bx, const double *by, const double *bz,
+                    const hamx_box *box, const double pos[3], double out[3]) {
+  const V3 b = read_grid3(bx, by, bz, *box, v3(pos[0], pos[1], pos[2]));
+  for (int i = 0; i < 3; ++i)
+    out[i] = b[i];
+}
+
+double orc_read_grid1(const double *f, const hamx_box *box, const double pos[3]) {
+  return read_grid1(f, *box, v3(pos[0], pos[1], pos[2]));
+}
+
+void orc_sync_emissivity(const hamx_los_desc *d, const double pos[3],
+                         double b_per, double *j_tot, double *j_pol) {
+  Cre cre;
+  cre.model = d->cre_model;
+  cre.p = d->cre;
+  cre.obs = v3(d->observer[0], d->observer[1], d->observer[2]);
+  const V3 p = v3(pos[0], pos[1], pos[2]);
+  *j_tot = sync_t(cre, d->sync_freq, p, b_per);
+  *j_pol = sync_p(cre, d->sync_freq, p, b_per);
+}
+
+void orc_radial_integration(const hamx_los_desc *d, const hamx_fields *f,
+                            uint32_t shell, double theta, double phi,
+                            double fd_inner, double out[5]) {
+  const Shell sh = shell_ref(d, shell);
+  radial(d, f, sh, theta, phi, fd_inner, out);
+}
+
+// Integrator::write_grid (integrator.cc:23-134) for one sync entry.
+int orc_write_grid(const hamx_los_desc *d, const hamx_fields *f, hamx_maps *out,
+                   int nthreads) {
+  if (!d || !out || d->n_shell < 1 || d->n_shell > HAMX_MAX_SHELL)
+    return HAMX_ERR_ARG;
+  for (uint32_t s = 0; s < d->n_shell; ++s)
+    if (!pow2(d->nside_shell[s]))
+      return HAMX_ERR_NSIDE;
+  if (d->do_sync && d->cre_model == HAMX_CRE_NONE)
+    return HAMX_ERR_ARG;
+#ifdef _OPENMP
+  if (nthreads > 0)
+    omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  // Grid_obs::build_grid (grid_obs.cc:21-43): maps start at zero; the FD map
+  // has nside_fd when do_fd, else nside_sync.
+  const uint64_t nside_fd = d->do_fd ? d->nside_fd : d->nside_sync;
+  std::vector<double> dm_map, is_map, qs_map, us_map, fd_map;
+  if (d->do_dm)
+    dm_map.assign(Ring(d->nside_dm).npix, 0.);
+  if (d->do_sync) {
+    const uint64_t n = Ring(d->nside_sync).npix;
+    is_map.assign(n, 0.);
+    qs_map.assign(n, 0.);
+    us_map.assign(n, 0.);
+  }
+  if (d->do_fd || d->do_sync)
+    fd_map.assign(Ring(nside_fd).npix, 0.);
+  const Ring rfd(d->do_fd || d->do_sync ? nside_fd : 1);
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    const uint64_t nside = d->nside_shell[s];
+    const Ring rs(nside);
+    const Shell sh = shell_ref(d, s);
+    std::vector<double> t_dm(d->do_dm ? rs.npix : 0), t_is, t_qs, t_us, t_fd;
+    if (d->do_sync) {
+      t_is.assign(rs.npix, 0.);
+      t_qs.assign(rs.npix, 0.);
+      t_us.assign(rs.npix, 0.);
+    }
+    if (d->do_fd || d->do_sync)
+      t_fd.assign(rs.npix, 0.);
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t ipix = 0; ipix < (int64_t)rs.npix; ++ipix) {
+      double the, phi;
+      rs.pix2ang(ipix, the, phi);
+      double fd_inner = 0.;
+      if (d->do_fd || d->do_sync)
+        fd_inner = rfd.interpolate(fd_map.data(), the, phi);
+      double o[5];
+      radial(d, f, sh, the, phi, fd_inner, o);
+      if (d->do_dm)
+        t_dm[ipix] = o[0];
+      if (d->do_sync) {
+        t_is[ipix] = temp_convert(o[2], d->sync_freq);
+        t_qs[ipix] = temp_convert(o[3], d->sync_freq);
+        t_us[ipix] = temp_convert(o[4], d->sync_freq);
+      }
+      if (d->do_fd || d->do_sync)
+        t_fd[ipix] = o[1];
+    }
+    if (d->do_dm)
+      accumulate(dm_map.data(), d->nside_dm, t_dm.data(), nside);
+    if (d->do_sync) {
+      accumulate(is_map.data(), d->nside_sync, t_is.data(), nside);
+      accumulate(qs_map.data(), d->nside_sync, t_qs.data(), nside);
+      accumulate(us_map.data(), d->nside_sync, t_us.data(), nside);
+    }
+    if (d->do_fd)
+      accumulate(fd_map.data(), nside_fd, t_fd.data(), nside);
+  }
+  auto put = [](double *dst, const std::vector<double> &v) {
+    if (dst && !v.empty())
+      std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  put(out->dm, dm_map);
+  put(out->is, is_map);
+  put(out->qs, qs_map);
+  put(out->us, us_map);
+  put(out->fd, fd_map);
+  return HAMX_OK;
+}
+
+} // extern "C"

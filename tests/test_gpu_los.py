@@ -1,0 +1,205 @@
+"""GPU parity of the LOS integrator (hamx_los_forward) against the CPU
+oracle: FP64 maps within 1e-10 relative L2 per map (the plain-path bar),
+both march schedules, grids gathered packed and separate, FD carried
+across shells, outputs accumulated across nside, and the error paths.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import HamxError, Param, _capi as capi, configs
+from hammurabix_amd.units import CGS
+from oracle_lib import fields_struct, npix, rel_l2, write_grid
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # relative L2 per map, FP64 plain path
+MAPS = ("dm", "fd", "is", "qs", "us")
+
+
+def gpu_maps(ctx, desc, **grids):
+    m = {}
+    if desc.do_dm:
+        m["dm"] = np.zeros(npix(desc.nside_dm))
+    if desc.do_sync:
+        for k in ("is", "qs", "us"):
+            m[k] = np.zeros(npix(desc.nside_sync))
+    if desc.do_fd or desc.do_sync:
+        m["fd"] = np.zeros(npix(desc.nside_fd if desc.do_fd else desc.nside_sync))
+    ctx.los_forward(desc, m, **grids)
+    return m
+
+
+def assert_parity(got, ref, tol=TOL, label=""):
+    assert set(got) == set(ref)
+    for k in ref:
+        err = rel_l2(got[k], ref[k])
+        print(f"{label} {k}: rel L2 {err:.3e}", flush=True)
+        assert np.all(np.isfinite(got[k])), k
+        assert err <= tol, f"{label} map {k}: rel L2 {err:.3e} > {tol}"
+
+
+def random_grids(shape, seed=0, te_scale=0.003):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    b = tuple(np.ascontiguousarray(rng.normal(0, 2e-6, n)) for _ in range(3))
+    te = np.ascontiguousarray(rng.normal(0.0, te_scale, n))  # negative values hit the clamp
+    box = capi.HamxBox(*shape, -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc, 20 * CGS.kpc,
+                       -4 * CGS.kpc, 4 * CGS.kpc)
+    return b, te, box
+
+
+@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE])
+def test_plain_forward_full_size(ctx, sched):
+    """LSA + uniform TE + analytic CRE, one shell at nside 32, 3000 steps."""
+    print("case: plain forward nside 32, schedule", sched, flush=True)
+    par = Param(text=configs.plain_forward())
+    d = par.los_desc(schedule=sched)
+    assert_parity(gpu_maps(ctx, d), write_grid(d), label=f"plain/{sched}")
+
+
+@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE])
+@pytest.mark.parametrize("packed", [True, False])
+def test_random_grids_multishell(ctx, sched, packed):
+    """3 shells (8/16/32), brnd + ternd grids, FD carried through the shells."""
+    print("case: multishell grids, schedule", sched, "packed", packed, flush=True)
+    par = Param(text=configs.random_forward(nside_sim=8, nside_out=32, oc_r_res=0.05,
+                                            grid=(32, 32, 16)))
+    d = par.los_desc(schedule=sched)
+    b, te, box = random_grids((32, 32, 16))
+    ternd_box = box
+    if not packed:  # a different TE box forces the separate gathers
+        ternd_box = capi.HamxBox(32, 32, 16, -21 * CGS.kpc, 19 * CGS.kpc, -20 * CGS.kpc,
+                                 20 * CGS.kpc, -4 * CGS.kpc, 4 * CGS.kpc)
+    ref = write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=ternd_box)
+    got = gpu_maps(ctx, d, brnd=b, brnd_box=box, ternd=te, ternd_box=ternd_box)
+    assert_parity(got, ref, label=f"grids/{sched}/{packed}")
+
+
+def test_output_nsides_differ_from_shells(ctx):
+    """DM at nside 16, FD at 64, sync at 8: every accumulate interpolates."""
+    par = Param(text=configs.random_forward(nside_sim=8, nside_out=32, oc_r_res=0.05,
+                                            grid=(32, 32, 16)))
+    d = par.los_desc()
+    d.nside_dm, d.nside_fd, d.nside_sync = 16, 64, 8
+    b, te, box = random_grids((32, 32, 16), seed=4)
+    ref = write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    got = gpu_maps(ctx, d, brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    assert_parity(got, ref, label="nsides")
+
+
+@pytest.mark.parametrize("flags", [(1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1)])
+def test_observable_subsets(ctx, flags):
+    """Only DM, only FD, only sync (zero FD cache), and mixtures."""
+    par = Param(text=configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1,
+                                            grid=(16, 16, 8)))
+    d = par.los_desc()
+    d.do_dm, d.do_fd, d.do_sync = flags
+    b, te, box = random_grids((16, 16, 8), seed=5)
+    ref = write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    got = gpu_maps(ctx, d, brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    assert_parity(got, ref, label=f"subset{flags}")
+
+
+def test_regular_field_grids_and_variable_cre_index(ctx):
+    """Imported breg/tereg grids (HAMX_BREG_GRID / HAMX_TEREG_GRID) and a CRE
+    spectral index varying with r and z (Gamma factors on the device)."""
+    par = Param(text=configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05))
+    d = par.los_desc()
+    b, te, box = random_grids((24, 24, 12), seed=6, te_scale=0.01)
+    d.breg_model = capi.HAMX_BREG_GRID
+    d.tereg_model = capi.HAMX_TEREG_GRID
+    d.cre[1] = 0.02  # beta
+    d.cre[2] = 0.05  # theta
+    te_pos = np.abs(te)
+    ref = write_grid(d, breg=b, breg_box=box, tereg=te_pos, tereg_box=box)
+    got = gpu_maps(ctx, d, breg=b, breg_box=box, tereg=te_pos, tereg_box=box)
+    assert_parity(got, ref, tol=1e-9, label="breg-grid")  # Lanczos Gamma ~1e-14
+
+
+@pytest.mark.parametrize("model", [capi.HAMX_BREG_CART, capi.HAMX_BREG_UNIF])
+def test_cart_unif_models_with_cre_unif(ctx, model):
+    par = Param(text=configs.plain_forward(nside_sim=4, nside_out=4, oc_r_res=0.05))
+    d = par.los_desc()
+    d.breg_model = model
+    d.breg[0:3] = [1.5e-6, -2e-6, 0.6] if model == capi.HAMX_BREG_UNIF else [1.5e-6, -2e-6, 7e-7]
+    d.cre_model = capi.HAMX_CRE_UNIF
+    d.cre[0:4] = [3.0, 4 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    assert_parity(gpu_maps(ctx, d), write_grid(d), label=f"model{model}")
+
+
+def test_empty_shell_and_no_samples(ctx):
+    """A shell thinner than one step contributes nothing; so does a sky with
+    every sample outside the galactocentric limits."""
+    par = Param(text=configs.plain_forward(nside_sim=4, nside_out=4, oc_r_res=0.05))
+    d = par.los_desc()
+    d.n_shell = 2
+    d.nside_shell[1] = 8
+    d.radii_shell[1] = 0.01 * CGS.kpc
+    d.radii_shell[2] = 30 * CGS.kpc
+    assert_parity(gpu_maps(ctx, d), write_grid(d), label="empty-shell")
+    d.gc_r_max = 1.0 * CGS.kpc  # observer sits 8.3 kpc out: no sample passes
+    got = gpu_maps(ctx, d)
+    for k in got:
+        assert np.all(got[k] == 0.0)
+
+
+def test_device_resident_buffers(ctx):
+    """Torch device tensors in and out (HAMX_IN_DEVICE | HAMX_OUT_DEVICE)."""
+    import torch
+    par = Param(text=configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1,
+                                            grid=(16, 16, 8)))
+    d = par.los_desc()
+    b, te, box = random_grids((16, 16, 8), seed=8)
+    ref = write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    db = tuple(torch.from_numpy(a).cuda() for a in b)
+    dte = torch.from_numpy(te).cuda()
+    m = {k: torch.zeros(v.shape[0], dtype=torch.float64, device="cuda") for k, v in ref.items()}
+    ctx.los_forward(d, m, brnd=db, brnd_box=box, ternd=dte, ternd_box=box)
+    assert_parity({k: v.cpu().numpy() for k, v in m.items()}, ref, label="device")
+
+
+def test_errors_are_reported(ctx):
+    par = Param(text=configs.plain_forward(nside_sim=4, nside_out=4))
+    d = par.los_desc()
+    d.nside_shell[0] = 6
+    with pytest.raises(HamxError) as e:
+        gpu_maps(ctx, d)
+    assert e.value.code == capi.HAMX_ERR_NSIDE
+    d = par.los_desc()
+    d.cre_model = capi.HAMX_CRE_NONE
+    with pytest.raises(HamxError) as e:
+        gpu_maps(ctx, d)
+    assert e.value.code == capi.HAMX_ERR_ARG
+    assert "CRE" in str(e.value)
+    # the context stays usable after an error
+    d = par.los_desc()
+    assert_parity(gpu_maps(ctx, d), write_grid(d), label="after-error")
+
+
+def test_uniform_sphere_closed_form_on_device(ctx):
+    """Same closed form as the oracle test, evaluated on the device path."""
+    d = capi.HamxLosDesc()
+    nside, r0, res, rmax, freq = 2, 2.0, 0.013, 3.0, 1.4e9
+    B = np.array([1.5e-6, -2.0e-6, 0.7e-6])
+    d.n_shell = 1
+    d.nside_shell[0] = nside
+    d.radii_shell[1] = rmax * CGS.kpc
+    d.oc_r_res = res * CGS.kpc
+    d.gc_r_max = 1e3 * CGS.kpc
+    d.gc_z_min, d.gc_z_max = -1e3 * CGS.kpc, 1e3 * CGS.kpc
+    d.do_dm = d.do_fd = d.do_sync = 1
+    d.nside_dm = d.nside_fd = d.nside_sync = nside
+    d.sync_freq = freq
+    d.breg_model = capi.HAMX_BREG_CART
+    d.breg[0:3] = B
+    d.tereg_model = capi.HAMX_TEREG_UNIF
+    d.tereg[0:2] = [0.02, r0 * CGS.kpc]
+    d.cre_model = capi.HAMX_CRE_UNIF
+    d.cre[0:4] = [3.0, r0 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    got = gpu_maps(ctx, d)
+    step = math.floor(rmax / res)
+    delta = rmax * CGS.kpc / step
+    K = int(np.sum((np.arange(step) + 0.5) * delta <= r0 * CGS.kpc))
+    np.testing.assert_allclose(got["dm"], 0.02 * K * delta, rtol=1e-12)

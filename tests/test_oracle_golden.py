@@ -1,0 +1,476 @@
+"""Pin the CPU oracle: reference fixtures, the reference's known-answer
+tests, and what the mathematics fixes (closed forms, identities).
+
+Reference tests restated here: hamdis_tests.cc:111-370 (pix2ang fixtures,
+accumulate, interpolation fixtures), integrator_tests.cc:16-209 (limits,
+shell assembly, LOS geometry), grid_tests.cc:137-302 (trilinear exactness on
+linear fields, with fixed seeds instead of std::random_device).
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import Param, _capi as capi
+from hammurabix_amd.units import CGS
+from oracle_lib import fields_struct, npix, ptr, write_grid
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    path = os.path.join(GOLDEN, name)
+    assert os.path.exists(path), f"missing fixture {name}"  # never pass silently
+    return np.fromfile(path, dtype="<f8")
+
+
+# ------------------------------------------------------------------ HEALPix
+
+@pytest.mark.parametrize("nside", [2, 4, 8, 16, 32])
+def test_pix2ang_matches_healpy_fixture(orc, nside):
+    ref = golden(f"pointing_healpix_nside{nside}.bin").reshape(-1, 2)
+    assert ref.shape[0] == npix(nside)
+    tp = np.zeros(2 * npix(nside))
+    orc.orc_pix2ang_map(nside, ptr(tp))
+    tp = tp.reshape(-1, 2)
+    np.testing.assert_allclose(tp[:, 0], ref[:, 0], atol=1e-10, rtol=0)
+    np.testing.assert_allclose(tp[:, 1], ref[:, 1], atol=1e-10, rtol=0)
+
+
+def test_accumulate_constant_maps(orc):
+    # hamdis_tests.cc:297-318
+    dst = np.zeros(npix(4))
+    same, lower, higher = (np.full(npix(4), 1.0), np.full(npix(2), 2.0),
+                           np.full(npix(8), 2.0))
+    orc.orc_accumulate(ptr(dst), 4, ptr(same), 4)
+    assert np.all(dst == 1.0)
+    orc.orc_accumulate(ptr(dst), 4, ptr(lower), 2)
+    np.testing.assert_allclose(dst, 3.0, rtol=0, atol=1e-14)
+    orc.orc_accumulate(ptr(dst), 4, ptr(higher), 8)
+    np.testing.assert_allclose(dst, 5.0, rtol=0, atol=1e-14)
+
+
+def test_interpolation_matches_fixtures(orc):
+    # hamdis_tests.cc:320-370: nside 4 map up/down-sampled
+    base = golden("random_map_nside4.bin")
+    low = np.zeros(npix(2))
+    orc.orc_accumulate(ptr(low), 2, ptr(base), 4)
+    np.testing.assert_allclose(low, golden("random_map_nside2.bin"), atol=1e-10, rtol=0)
+    high = np.zeros(npix(8))
+    orc.orc_accumulate(ptr(high), 8, ptr(base), 4)
+    np.testing.assert_allclose(high, golden("random_map_nside8.bin"), atol=1e-10, rtol=0)
+
+
+def test_interpolation_reproduces_pixel_values_and_skips_undef(orc):
+    rng = np.random.default_rng(3)
+    m = rng.normal(size=npix(8))
+    tp = np.zeros(2 * npix(8))
+    orc.orc_pix2ang_map(8, ptr(tp))
+    tp = tp.reshape(-1, 2)
+    # at a pixel centre the bilinear weights collapse onto that pixel
+    for i in range(0, npix(8), 37):
+        v = orc.orc_interpolate(ptr(m), 8, tp[i, 0], tp[i, 1])
+        assert abs(v - m[i]) < 1e-10
+    # undefined pixels are excluded and renormalised; all-undef -> undef
+    u = np.full(npix(2), -1.6375e30)
+    assert orc.orc_interpolate(ptr(u), 2, 1.0, 1.0) == -1.6375e30
+
+
+# --------------------------------------------------------------- integrator
+
+def test_limit_predicates(orc):
+    # integrator_tests.cc:16-24
+    r0 = 3.7
+    lim = (1.0 + 1.0e-5) * r0
+    assert orc.orc_check_upper(r0, lim) == 0
+    assert orc.orc_check_lower(r0, lim) == 1
+
+
+def _shell(orc, par, s):
+    step = C.c_uint64()
+    delta = C.c_double()
+    g = par.grid_obs
+    orc.orc_shell_ref(g.radii_shell[s], g.radii_shell[s + 1], g.oc_r_res,
+                      C.byref(step), C.byref(delta))
+    return step.value, delta.value
+
+
+def test_shell_assembly_auto(orc):
+    # integrator_tests.cc:28-81 with int_tests_01.xml
+    par = Param(os.path.join(GOLDEN, "int_tests_01.xml"))
+    g = par.grid_obs
+    assert g.sim_sync_freq == [23.0 * CGS.GHz, 1.4 * CGS.GHz]
+    assert g.nside_shell == [32, 64, 128]
+    step, delta = _shell(orc, par, 0)
+    assert step == 333
+    assert g.radii_shell[0] == 5 * CGS.kpc
+    assert g.radii_shell[1] == 40 * CGS.kpc / 4 + 5 * CGS.kpc
+    assert abs(delta - 10 * CGS.kpc / 333) < 1e-10 * CGS.kpc
+    step, delta = _shell(orc, par, 2)
+    assert step == 666
+    assert g.radii_shell[2] == 40 * CGS.kpc / 2 + 5 * CGS.kpc
+    assert g.radii_shell[3] == 45 * CGS.kpc
+    assert abs(delta - 20 * CGS.kpc / 666) < 1e-10 * CGS.kpc
+
+
+def test_shell_assembly_manual(orc):
+    # integrator_tests.cc:82-139 with int_tests_02.xml
+    par = Param(os.path.join(GOLDEN, "int_tests_02.xml"))
+    g = par.grid_obs
+    assert g.nside_shell == [32, 16, 8]
+    expect = [(100, 0.0, 0.1, 3), (600, 0.1, 0.7, 18), (300, 0.7, 1.0, 9)]
+    for s, (n, c0, c1, span) in enumerate(expect):
+        step, delta = _shell(orc, par, s)
+        assert step == n
+        assert g.radii_shell[s] == g.oc_r_max * c0
+        assert g.radii_shell[s + 1] == g.oc_r_max * c1
+        assert abs(delta - span * CGS.kpc / n) < 1e-10 * CGS.kpc
+
+
+def _versor(orc, th, ph):
+    out = np.zeros(3)
+    orc.orc_los_versor(th, ph, ptr(out))
+    return out
+
+
+def test_los_geometry_known_answers(orc):
+    # integrator_tests.cc:143-209
+    theta = [0.0, 90 * CGS.rad, 90 * CGS.rad]
+    phi = [0.0, 180 * CGS.rad, 270 * CGS.rad]
+    np.testing.assert_allclose(_versor(orc, theta[0], phi[0]), [0, 0, 1], atol=1e-10)
+    np.testing.assert_allclose(_versor(orc, theta[1], phi[1]), [-1, 0, 0], atol=1e-10)
+    np.testing.assert_allclose(_versor(orc, theta[2], phi[2]), [0, -1, 0], atol=1e-10)
+    A = np.array([1.0, 0.0, 0.0])
+    par = [orc.orc_los_parproj(ptr(A), ptr(_versor(orc, t, p))) for t, p in zip(theta, phi)]
+    np.testing.assert_allclose(par, [0, -1, 0], atol=1e-10)
+    per = [orc.orc_los_perproj(ptr(A), ptr(_versor(orc, t, p))) for t, p in zip(theta, phi)]
+    np.testing.assert_allclose(per, [1, 0, 1], atol=1e-10)
+    assert abs(orc.orc_sync_ipa(ptr(A), theta[0], phi[0]) + 90 * CGS.rad) < 1e-10
+    assert abs(orc.orc_sync_ipa(ptr(A), theta[2], phi[2]) - 180 * CGS.rad) < 1e-10
+
+
+def test_temp_convert_identity(orc):
+    # (e^p - 1)^2 / (p^2 e^p) = (sinh(p/2) / (p/2))^2, p = h nu / (kB 2.725 K)
+    for f in (0.408e9, 1.4e9, 23e9, 30e9, 300e9):
+        x = CGS.h_planck * f / (CGS.kB * 2.725) / 2
+        assert abs(orc.orc_temp_convert(1.0, f) - (math.sinh(x) / x) ** 2) < 1e-13 * (math.sinh(x) / x) ** 2
+    # Rayleigh-Jeans limit (the formula as written loses ~1e-16/p to
+    # cancellation in e^p - 1, so the bound is loose)
+    assert abs(orc.orc_temp_convert(2.0, 1e7) - 2.0) < 1e-8
+
+
+# ------------------------------------------------------------------- fields
+
+def _linear_grid(nx, ny, nz):
+    x = np.linspace(0.0, 1.0, nx)
+    y = np.linspace(0.0, 1.0, ny)
+    z = np.linspace(0.0, 1.0, nz)
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    return [np.ascontiguousarray(a.ravel()) for a in (X, Y, Z)]
+
+
+def test_trilinear_exact_on_linear_fields(orc):
+    # grid_tests.cc:137-302 (10 x 8 x 29 box over [0,1]^3)
+    box = capi.HamxBox(10, 8, 29, 0, 1, 0, 1, 0, 1)
+    bx, by, bz = _linear_grid(10, 8, 29)
+    te = bx + by + bz
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        p = rng.uniform(0.0, 1.0, 3)
+        out = np.zeros(3)
+        orc.orc_read_grid3(ptr(bx), ptr(by), ptr(bz), C.byref(box), ptr(p), ptr(out))
+        np.testing.assert_allclose(out, p, atol=1e-10, rtol=0)
+        v = orc.orc_read_grid1(ptr(te), C.byref(box), ptr(p))
+        assert abs(v - p.sum()) < 1e-10
+    # on and outside the faces the gather returns zero (brnd.cc:30,37,44)
+    for p in ([0.0, 0.5, 0.5], [1.0, 0.5, 0.5], [0.5, -0.1, 0.5], [0.5, 0.5, 1.2]):
+        out = np.ones(3)
+        orc.orc_read_grid3(ptr(bx), ptr(by), ptr(bz), C.byref(box), ptr(np.array(p)), ptr(out))
+        assert np.all(out == 0.0)
+
+
+def test_lsa_field_closed_form(orc):
+    # LSA: |B| = b0 inside 3 < r < 20 kpc, pitch psi(r), tilt chi(z)
+    p = np.array([6e-6, 27 * CGS.rad, 0.9 * CGS.rad, 25 * CGS.rad])
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        r = rng.uniform(3.1, 19.9) * CGS.kpc
+        ph = rng.uniform(0, 2 * math.pi)
+        z = rng.uniform(-2, 2) * CGS.kpc
+        pos = np.array([r * math.cos(ph), r * math.sin(ph), z])
+        b = np.zeros(3)
+        orc.orc_breg(capi.HAMX_BREG_LSA, ptr(p), ptr(pos), ptr(b))
+        assert abs(np.linalg.norm(b) - p[0]) < 1e-12 * p[0]
+        chi = p[3] * math.tanh(z / CGS.kpc)
+        assert abs(b[2] - p[0] * math.sin(chi)) < 1e-12 * p[0]
+        # radial component / azimuthal component = tan(psi)
+        er = pos[:2] / r
+        ephi = np.array([-er[1], er[0]])
+        psi = p[1] + p[2] * math.log(r / (8 * CGS.kpc))
+        assert abs(math.atan2(b[:2] @ er, b[:2] @ ephi) - psi) < 1e-10
+    b = np.ones(3)
+    orc.orc_breg(capi.HAMX_BREG_LSA, ptr(p), ptr(np.array([2.0 * CGS.kpc, 0, 0])), ptr(b))
+    assert np.all(b == 0)
+
+
+def _desc_uniform(freq=23e9):
+    d = capi.HamxLosDesc()
+    d.cre_model = capi.HAMX_CRE_ANA
+    d.cre[0:7] = [3.0, 0.0, 0.0, 5 * CGS.kpc, 1 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    d.sync_freq = freq
+    d.observer[:] = [-8.3 * CGS.kpc, 0.0, 0.006 * CGS.kpc]
+    return d
+
+
+def test_sync_emissivity_textbook_ratios(orc):
+    """Power-law synchrotron (Rybicki & Lightman 6.36-6.38): polarisation
+    degree (p+1)/(p+7/3) and the B^((p+1)/2), nu^(-(p-1)/2) scalings."""
+    jt, jp = C.c_double(), C.c_double()
+    pos = np.array([-8.3 * CGS.kpc, 0.0, 0.0])
+    for alpha in (2.2, 3.0, 3.6):
+        d = _desc_uniform()
+        d.cre[0] = alpha
+        orc.orc_sync_emissivity(C.byref(d), ptr(pos), 3e-6, C.byref(jt), C.byref(jp))
+        assert abs(jp.value / jt.value - (alpha + 1) / (alpha + 7 / 3)) < 1e-12
+        j1 = jt.value
+        orc.orc_sync_emissivity(C.byref(d), ptr(pos), 6e-6, C.byref(jt), C.byref(jp))
+        assert abs(jt.value / j1 - 2 ** ((alpha + 1) / 2)) < 1e-12 * 2 ** ((alpha + 1) / 2)
+        d.sync_freq = 46e9
+        orc.orc_sync_emissivity(C.byref(d), ptr(pos), 3e-6, C.byref(jt), C.byref(jp))
+        assert abs(jt.value / j1 - 2 ** (-(alpha - 1) / 2)) < 1e-12
+
+
+def test_sync_emissivity_gamma_factors_vs_scipy(orc):
+    from scipy.special import gamma
+    d = _desc_uniform()
+    pos = np.array([-8.3 * CGS.kpc, 0.0, 0.006 * CGS.kpc])
+    bper = 4e-6
+    jt, jp = C.c_double(), C.c_double()
+    orc.orc_sync_emissivity(C.byref(d), ptr(pos), bper, C.byref(jt), C.byref(jp))
+    idx = -3.0
+    gamma0 = 20.6 * CGS.GeV / CGS.mec2
+    beta0 = math.sqrt(1 - 1 / gamma0)
+    unit = 4 * math.pi * CGS.mec / (CGS.GeV * CGS.m * CGS.m * beta0)
+    # observer sits at the profile reference point: profile = 1/cosh^2(z/z0)
+    prof = math.exp(0.0) / math.cosh(0.006) ** 2
+    n0 = 0.0217 * gamma0 ** 3 * unit * prof
+    A = 2 * math.pi * 23e9 * CGS.mec / (3 * CGS.qe * bper)
+    ref_t = (n0 * 1.73205081 * CGS.qe ** 3 * bper / (4 * math.pi * CGS.mec2 * (1 - idx))
+             * A ** (0.5 * (idx + 1)) * gamma(-0.25 * idx + 19 / 12) * gamma(-0.25 * idx - 1 / 12))
+    assert abs(jt.value / ref_t - 1) < 1e-12
+
+
+# ------------------------------------------------------- LOS closed forms
+
+def _uniform_desc(nside, r0_kpc, res_kpc, freq, bvec, n0, rmax_kpc, psi_like=False):
+    d = capi.HamxLosDesc()
+    d.observer[:] = [0.0, 0.0, 0.0]
+    d.n_shell = 1
+    d.nside_shell[0] = nside
+    d.radii_shell[0] = 0.0
+    d.radii_shell[1] = rmax_kpc * CGS.kpc
+    d.oc_r_res = res_kpc * CGS.kpc
+    d.gc_r_min, d.gc_r_max = 0.0, 1e3 * CGS.kpc
+    d.gc_z_min, d.gc_z_max = -1e3 * CGS.kpc, 1e3 * CGS.kpc
+    d.do_dm = d.do_fd = d.do_sync = 1
+    d.nside_dm = d.nside_fd = d.nside_sync = nside
+    d.sync_freq = freq
+    d.breg_model = capi.HAMX_BREG_CART
+    d.breg[0:3] = bvec
+    d.tereg_model = capi.HAMX_TEREG_UNIF
+    d.tereg[0:2] = [n0, r0_kpc * CGS.kpc]
+    d.cre_model = capi.HAMX_CRE_UNIF
+    d.cre[0:4] = [3.0, r0_kpc * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    return d
+
+
+@pytest.mark.parametrize("freq", [23e9, 1.4e9, 0.408e9])
+def test_los_uniform_sphere_closed_form(orc, freq):
+    """Uniform B, uniform n_e and CRE inside r0 around the observer: DM, FD
+    and I are exact chord sums; Q + iU is a finite geometric series of the
+    Faraday-rotated phase (the sample's own FD included)."""
+    nside, r0, res, rmax = 2, 2.0, 0.013, 3.0
+    B = np.array([1.5e-6, -2.0e-6, 0.7e-6])
+    n0 = 0.02
+    d = _uniform_desc(nside, r0, res, freq, B, n0, rmax)
+    m = write_grid(d)
+    step = math.floor(rmax * CGS.kpc / d.oc_r_res - 0.0)
+    delta = rmax * CGS.kpc / step
+    dist = (np.arange(step) + 0.5) * delta
+    K = int(np.sum(dist <= r0 * CGS.kpc))
+    L = K * delta
+    fd_fore = -(CGS.qe ** 3) / (2 * math.pi * CGS.mec2 ** 2)
+    lam2 = (CGS.c_light / freq) ** 2
+    i2bt = CGS.c_light ** 2 / (2 * CGS.kB * freq ** 2)
+    x = CGS.h_planck * freq / (CGS.kB * 2.725) / 2
+    tconv = (math.sinh(x) / x) ** 2
+    tp = np.zeros(2 * npix(nside))
+    orc.orc_pix2ang_map(nside, ptr(tp))
+    tp = tp.reshape(-1, 2)
+    for ipix in range(npix(nside)):
+        th, ph = tp[ipix]
+        l = np.array([math.cos(ph) * math.sin(th), math.sin(ph) * math.sin(th), math.cos(th)])
+        bpar = B @ l
+        bper = np.linalg.norm(np.cross(B, l))
+        assert abs(m["dm"][ipix] - n0 * L) < 1e-12 * n0 * L
+        f = fd_fore * n0 * bpar * delta
+        assert abs(m["fd"][ipix] - K * f) < 1e-10 * abs(fd_fore * n0 * np.linalg.norm(B) * L)
+        jt, jp = C.c_double(), C.c_double()
+        dd = _desc_uniform(freq)
+        dd.cre_model = capi.HAMX_CRE_UNIF
+        dd.cre[0:4] = [3.0, r0 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+        dd.observer[:] = [0, 0, 0]
+        orc.orc_sync_emissivity(C.byref(dd), ptr(np.zeros(3)), bper, C.byref(jt), C.byref(jp))
+        I = jt.value * delta * i2bt * K * tconv
+        assert abs(m["is"][ipix] - I) < 1e-11 * I
+        e_t = np.array([math.cos(th) * math.cos(ph), math.cos(th) * math.sin(ph), -math.sin(th)])
+        e_p = np.array([-math.sin(ph), math.cos(ph), 0.0])
+        psi = math.atan2(-(e_t @ B), -(e_p @ B))
+        z = np.exp(2j * (psi + lam2 * f * np.arange(1, K + 1))).sum()
+        P = jp.value * delta * i2bt * tconv
+        assert abs(m["qs"][ipix] - P * z.real) < 1e-9 * P * K
+        assert abs(m["us"][ipix] - P * z.imag) < 1e-9 * P * K
+
+
+def test_fd_carry_between_shells(orc):
+    """Shell 2's polarisation phase starts from shell 1's FD (interpolated
+    at equal nside: the inner FD map value itself)."""
+    nside = 2
+    B = np.array([0.0, 0.0, 3e-6])
+    d = _uniform_desc(nside, 50.0, 0.05, 1.4e9, B, 0.03, 4.0)
+    d.n_shell = 2
+    d.nside_shell[0] = d.nside_shell[1] = nside
+    d.radii_shell[1] = 2.0 * CGS.kpc
+    d.radii_shell[2] = 4.0 * CGS.kpc
+    two = write_grid(d)
+    # restart the second shell by hand with fd_inner = FD of shell 1
+    d1 = _uniform_desc(nside, 50.0, 0.05, 1.4e9, B, 0.03, 2.0)
+    one = write_grid(d1)
+    tp = np.zeros(2 * npix(nside))
+    orc.orc_pix2ang_map(nside, ptr(tp))
+    tp = tp.reshape(-1, 2)
+    f = fields_struct()
+    for ipix in range(npix(nside)):
+        out = np.zeros(5)
+        orc.orc_radial_integration(C.byref(d), C.byref(f), 1, tp[ipix, 0], tp[ipix, 1],
+                                   one["fd"][ipix], ptr(out))
+        x = CGS.h_planck * 1.4e9 / (CGS.kB * 2.725) / 2
+        tc = (math.sinh(x) / x) ** 2
+        assert abs(two["qs"][ipix] - (one["qs"][ipix] + out[3] * tc)) < 1e-12 * (abs(one["is"][ipix]) + 1e-300) * 4
+        assert abs(two["fd"][ipix] - (one["fd"][ipix] + out[1])) < 1e-12 * abs(two["fd"][ipix])
+
+
+# ----------------------------------------------------------- random fields
+
+KAT = [  # Random123 known-answer vectors for philox4x32-10
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF),
+     (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(orc, ctr, key, expect):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    orc.orc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    assert tuple(int(v) for v in out) == expect
+
+
+def test_gauss_pairs_are_standard_normal(orc):
+    n = 200000
+    g = np.zeros(2 * n)
+    orc.orc_gauss_pairs(1, 0, 0, n, ptr(g))
+    assert abs(g.mean()) < 5 / math.sqrt(2 * n)
+    assert abs(g.var() - 1) < 5 * math.sqrt(2 / (2 * n))
+    # fourth moment of N(0,1) is 3
+    assert abs((g ** 4).mean() - 3) < 0.05
+
+
+@pytest.mark.parametrize("shape", [(8, 4, 2), (16, 32, 8), (4, 4, 64)])
+def test_fft_matches_numpy(orc, shape):
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    buf = np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1).ravel())
+    orc.orc_fft3d(ptr(buf), *shape, 1)
+    got = buf.reshape(*shape, 2)
+    ref = np.fft.ifftn(a) * a.size  # exp(+2 pi i jk/n), unnormalised
+    np.testing.assert_allclose(got[..., 0] + 1j * got[..., 1], ref, atol=1e-12 * np.abs(ref).max())
+    orc.orc_fft3d(ptr(buf), *shape, -1)
+    np.testing.assert_allclose(buf.reshape(*shape, 2)[..., 0] + 1j * buf.reshape(*shape, 2)[..., 1],
+                               a * a.size, atol=1e-11 * a.size)
+
+
+def test_variance_of_interleaved_prefix(orc):
+    rng = np.random.default_rng(9)
+    x = rng.normal(3.0, 2.0, size=4096)
+    assert abs(orc.orc_variance(ptr(x), 2048) - x[:2048].var()) < 1e-12 * x[:2048].var()
+
+
+def _es_desc(n=(32, 32, 16)):
+    d = capi.HamxEsDesc()
+    d.box = capi.HamxBox(*n, -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc, 20 * CGS.kpc,
+                         -4 * CGS.kpc, 4 * CGS.kpc)
+    d.observer[:] = [-8.3 * CGS.kpc, 0.0, 0.006 * CGS.kpc]
+    d.rms, d.k0, d.k1, d.a0, d.a1, d.rho = 2e-6, 0.1, 0.1, 1.7, 0.0, 1.0
+    d.r0, d.z0 = 8 * CGS.kpc, 1 * CGS.kpc
+    d.breg_model = capi.HAMX_BREG_LSA
+    d.breg[0:4] = [6e-6, 27 * CGS.rad, 0.9 * CGS.rad, 25 * CGS.rad]
+    return d
+
+
+def test_es_spectrum_strict_bands(orc):
+    d = _es_desc()
+    # k == k0 == k1 falls in no band (brnd_es.cc:39-43)
+    assert orc.orc_es_spectrum(C.byref(d), 0.1) == 0.0
+    k = 0.3
+    expect = (k / 0.1) ** -1.7 * (2e-6) ** 2 / (4 * math.pi * k * k)
+    assert abs(orc.orc_es_spectrum(C.byref(d), k) / expect - 1) < 1e-13
+
+
+def test_es_field_is_divergence_free(orc):
+    """After the Gram-Schmidt step and the Hermitian extraction,
+    k . B(k) = 0 for every mode (brnd_es.cc:208-307)."""
+    d = _es_desc((16, 16, 8))
+    n = 16 * 16 * 8
+    b = [np.zeros(n) for _ in range(3)]
+    assert orc.orc_brnd_es(C.byref(d), 1, *(ptr(a) for a in b)) == 0
+    shp = (16, 16, 8)
+    F = [np.fft.fftn(a.reshape(shp)) for a in b]
+    L = [40.0, 40.0, 8.0]
+    ks = np.meshgrid(*[np.fft.fftfreq(s, d=1.0 / s) / l for s, l in zip(shp, L)], indexing="ij")
+    div = sum(k * f for k, f in zip(ks, F))
+    scale = sum(np.abs(k) * np.abs(f) for k, f in zip(ks, F))
+    # On a Nyquist plane the index mirror of k is not -k, so the reference's
+    # pairing projects c(k) and c(mirror) on different wave vectors; the
+    # invariant holds on every other mode.
+    nyq = np.zeros(shp, dtype=bool)
+    nyq[shp[0] // 2, :, :] = nyq[:, shp[1] // 2, :] = nyq[:, :, shp[2] // 2] = True
+    assert np.max(np.abs(div[~nyq])) < 1e-10 * np.max(scale)
+    assert np.max(np.abs(div[nyq])) > 1e-6 * np.max(scale)
+    assert np.all(np.isfinite(b[0]))
+
+
+def test_dft_field_rms_matches_profile(orc):
+    """te / sqrt(profile) has rms equal to the requested rms (to sampling
+    error), the normalisation of ternd_dft.cc:102-124."""
+    d = capi.HamxDftDesc()
+    shp = (64, 64, 32)
+    d.box = capi.HamxBox(*shp, -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc, 20 * CGS.kpc,
+                         -4 * CGS.kpc, 4 * CGS.kpc)
+    d.observer[:] = [-8.3 * CGS.kpc, 0.0, 0.006 * CGS.kpc]
+    d.rms, d.k0, d.a0, d.r0, d.z0 = 0.001, 2.0, -1.7, 8 * CGS.kpc, 1 * CGS.kpc
+    te = np.zeros(int(np.prod(shp)))
+    assert orc.orc_ternd_dft(C.byref(d), 3, ptr(te)) == 0
+    x = np.linspace(-20, 20, shp[0])
+    y = np.linspace(-20, 20, shp[1])
+    z = np.linspace(-4, 4, shp[2])
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    prof = np.exp(-(np.sqrt(X ** 2 + Y ** 2) - 8.3) / 8) * np.exp(-(np.abs(Z) - 0.006) / 1)
+    w = te.reshape(shp) / np.sqrt(prof)
+    assert abs(np.sqrt((w ** 2).mean()) / 0.001 - 1) < 0.05
