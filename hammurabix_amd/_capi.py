@@ -128,6 +128,15 @@ def load() -> C.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP library first "
             "(python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname, libamdhip64.so.7).  Loading torch first lets the library's
+    # dependency resolve to that already-mapped runtime, so torch tensors and
+    # the library share one device context.  Without torch the system ROCm
+    # runtime is used.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in _PROTOS.items():
         fn = getattr(lib, name)

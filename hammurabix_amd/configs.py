@@ -13,7 +13,9 @@
   nside 256, sync 23 GHz.
 
 Every function takes overrides so tests can shrink a workload while keeping
-its physics.
+its physics.  A grid coarser than 1/(2 k0) kpc has no DFT mode above the TE
+cutoff k0 and the generator divides by a zero variance (as the reference's
+does), so shrunk grids also pass a smaller ``te_k0``.
 """
 from __future__ import annotations
 
@@ -108,7 +110,7 @@ _TEMPLATE = """<?xml version="1.0"?>
       <global type="dft">
         <dft>
           <rms value="0.001"/>
-          <k0 value="2.0"/>
+          <k0 value="{te_k0}"/>
           <a0 value="-1.7"/>
           <r0 value="8.0"/>
           <z0 value="1.0"/>
@@ -140,7 +142,8 @@ def _sync_lines(freqs: list[float], nside: int, out: str) -> str:
 def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.0,
         oc_r_res: float = 0.01, freqs: Optional[list[float]] = None,
         random: bool = False, grid: tuple[int, int, int] = (256, 256, 64),
-        seed: int = 1, b0: float = 6.0, psi0: float = 27.0, do_dm: bool = True,
+        seed: int = 1, b0: float = 6.0, psi0: float = 27.0, te_k0: float = 2.0,
+        do_dm: bool = True,
         do_fd: bool = True, brnd_file: str = "", ternd_file: str = "",
         out: str = "") -> str:
     freqs = [23.0] if freqs is None else freqs
@@ -151,7 +154,7 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         brnd_file=brnd_file or "brnd.bin", ternd_file=ternd_file or "ternd.bin",
         nx=grid[0], ny=grid[1], nz=grid[2], shell_num=shell_num, nside_sim=nside_sim,
         oc_r_max=oc_r_max, oc_r_res=oc_r_res, b0=b0, psi0=psi0,
-        brnd_on=int(random), ternd_on=int(random), seed=seed)
+        brnd_on=int(random), ternd_on=int(random), seed=seed, te_k0=te_k0)
 
 
 def plain_forward(**kw) -> str:

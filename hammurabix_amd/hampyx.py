@@ -1,0 +1,125 @@
+"""hampyx: Python front end of the simulator, run in-process on the GPU.
+
+Mirrors the reference's ``hampyx.Hampyx`` interface (hampyx/hampyx.py):
+
+* ``Hampyx(xml_path=None, exe_path=None)`` loads a parameter file (the first
+  ``*xml*`` file of the working directory when none is given);
+* ``mod_par`` / ``add_par`` / ``del_par`` / ``print_par`` edit the parameter
+  tree with the same key-chain arguments and ``ValueError`` messages;
+* calling the object runs the simulation and fills ``sim_map`` with the same
+  keys: ``('sync', freq, nside, 'I'|'Q'|'U')``, ``('fd', 'nan', nside,
+  'nan')``, ``('dm', 'nan', nside, 'nan')``, freq and nside being the XML
+  attribute strings.
+
+The reference launches the ``hamx`` executable as a subprocess and reads the
+maps back from temporary files; here the parameter tree is handed to the
+device pipeline in memory and the maps come back as numpy arrays.
+``exe_path`` is accepted for call compatibility and not used.
+"""
+from __future__ import annotations
+
+import os
+import xml.etree.ElementTree as et
+from typing import Optional
+
+from .pipeline import Pipeline
+from .runtime import Context
+
+
+class Hampyx:
+    def __init__(self, xml_path: Optional[str] = None, exe_path: Optional[str] = None,
+                 ctx: Optional[Context] = None):
+        self.wk_dir = os.getcwd()
+        self.exe_path = exe_path
+        if xml_path is None:
+            cands = [s for s in sorted(os.listdir(self.wk_dir)) if "xml" in s]
+            cands = [c for c in cands if os.path.isfile(os.path.join(self.wk_dir, c))]
+            if not cands:
+                raise ValueError("no XML parameter file in the working directory")
+            xml_path = os.path.join(self.wk_dir, cands[0])
+        if not os.path.isfile(xml_path):
+            raise ValueError(f"parameter file {xml_path} not found")
+        self.xml_path = os.path.abspath(xml_path)
+        self.tree = et.parse(self.xml_path)
+        self.sim_map: dict = {}
+        self.sim_map_name: dict = {}
+        self._ctx = ctx
+
+    def __call__(self, verbose: bool = False) -> None:
+        return self.run(verbose)
+
+    def run(self, verbose: bool = False) -> None:
+        """Run the pipeline on the current parameter tree."""
+        text = et.tostring(self.tree.getroot(), encoding="unicode")
+        ctx = self._ctx if self._ctx is not None else Context(0)
+        try:
+            pipe = Pipeline(text=text, ctx=ctx, export=False)
+            maps = pipe.run()
+        finally:
+            if self._ctx is None:
+                ctx.close()
+        if verbose:
+            for k, v in maps.items():
+                print(k, v.shape)
+        self.sim_map.update(maps)
+
+    # ------------------------------------------------------ XML editing
+    @staticmethod
+    def _path(keychain: list) -> str:
+        return "." + "".join("/" + k for k in keychain)
+
+    def mod_par(self, keychain=None, attrib=None) -> None:
+        if type(attrib) is not dict or type(keychain) is not list:
+            raise ValueError("wrong input %s %s" % (keychain, attrib))
+        target = self.tree.getroot().find(self._path(keychain))
+        if target is None:
+            raise ValueError("wrong path %s" % self._path(keychain))
+        for k in attrib:
+            target.set(k, attrib.get(k))
+
+    def add_par(self, keychain=None, subkey=None, attrib=None) -> None:
+        if type(keychain) is not list or type(subkey) is not str:
+            raise ValueError("wrong input %s %s %s" % (keychain, subkey, attrib))
+        target = self.tree.getroot().find(self._path(keychain))
+        if target is None:
+            raise ValueError("wrong path %s" % self._path(keychain))
+        if attrib is None:
+            et.SubElement(target, subkey)
+        elif type(attrib) is dict:
+            for existed in target.findall(subkey):
+                if existed.attrib == attrib:
+                    raise ValueError("repeatitive input %s %s %s" % (keychain, subkey, attrib))
+            et.SubElement(target, subkey, attrib)
+        else:
+            raise ValueError("wrong input %s %s %s" % (keychain, subkey, attrib))
+
+    def print_par(self, keychain=None) -> None:
+        root = self.tree.getroot()
+        if keychain is None:
+            for child in root:
+                print(child.tag, child.attrib)
+            return
+        if type(keychain) is not list:
+            raise ValueError("wrong input %s" % keychain)
+        for target in root.findall(self._path(keychain)):
+            print(target.tag, target.attrib)
+            for child in target:
+                print("|--> ", child.tag, child.attrib)
+
+    def del_par(self, keychain=None, opt=None) -> None:
+        if type(keychain) is not list or not keychain:
+            raise ValueError("wrong input %s" % keychain)
+        root = self.tree.getroot()
+        path = self._path(keychain)
+        parent_path = self._path(keychain[:-1])
+        target = root.find(path)
+        parent = root.find(parent_path)
+        if target is None or parent is None:
+            raise ValueError("wrong path %s" % path)
+        if opt is None:
+            parent.remove(target)
+        elif opt == "all":
+            for t in parent.findall(keychain[-1]):
+                parent.remove(t)
+        else:
+            raise ValueError("unsupported option at %s" % keychain)

@@ -1,0 +1,270 @@
+// Param: XML parameter set -> CGS values (param.cc:12-778 restated over
+// hamx::xml).  Unsupported type strings throw the reference's messages.
+#include <cmath>
+#include <stdexcept>
+
+#include "hamx_host.h"
+#include "hamx_xml.h"
+
+namespace hamx_host {
+
+namespace {
+
+namespace units {
+const double pi = 3.141592653589793238462643383279502884197;
+const double rad = pi / 180.;
+const double kpc = 3.0856775806e+21;
+const double eV = 1.60217733e-12;
+const double GeV = 1.e9 * eV;
+const double GHz = 1.e9;
+const double muGauss = 1.e-6;
+} // namespace units
+
+using hamx::xml::Element;
+using hamx::xml::trace;
+
+double fval(const Element *e, const char *key) { return trace(e, {key})->attr_double("value"); }
+unsigned long long uval(const Element *e, const char *key) {
+  return trace(e, {key})->attr_uint("value");
+}
+
+void read_box(const Element *root, const char *name, Param::param_grid &g) {
+  const Element *b = trace(root, {"grid", name});
+  g.nx = uval(b, "nx");
+  g.ny = uval(b, "ny");
+  g.nz = uval(b, "nz");
+  g.full_size = g.nx * g.ny * g.nz;
+  g.x_max = units::kpc * fval(b, "x_max");
+  g.x_min = units::kpc * fval(b, "x_min");
+  g.y_max = units::kpc * fval(b, "y_max");
+  g.y_min = units::kpc * fval(b, "y_min");
+  g.z_max = units::kpc * fval(b, "z_max");
+  g.z_min = units::kpc * fval(b, "z_min");
+}
+
+void read_fieldio(const Element *root, const char *name, Param::param_grid &g) {
+  const Element *fio = trace(root, {"fieldio"});
+  if (const Element *e = fio->child(name)) {
+    g.read_permission = e->attr_bool("read");
+    g.write_permission = e->attr_bool("write");
+    g.filename = e->attr_string("filename");
+  }
+}
+
+} // namespace
+
+Param::Param(const std::string &xml_file) {
+  const hamx::xml::Document doc = hamx::xml::load(xml_file);
+  const Element *root = doc.root.get();
+  if (!root || root->name != "root")
+    throw std::runtime_error("parameter file has no <root> element");
+  const Element *o = trace(root, {"grid", "observer"});
+  observer = Vec3(units::kpc * fval(o, "x"), units::kpc * fval(o, "y"),
+                  units::kpc * fval(o, "z"));
+
+  // observables and shells (param.cc:30-144)
+  const Element *ob = trace(root, {"observable"});
+  if (const Element *e = ob->child("dm")) {
+    grid_obs.write_permission = true;
+    grid_obs.do_dm = e->attr_bool("cue");
+    grid_obs.sim_dm_name = e->attr_string("filename");
+    grid_obs.nside_dm = e->attr_uint("nside");
+  }
+  if (const Element *e = ob->child("faraday")) {
+    grid_obs.write_permission = true;
+    grid_obs.do_fd = e->attr_bool("cue");
+    grid_obs.sim_fd_name = e->attr_string("filename");
+    grid_obs.nside_fd = e->attr_uint("nside");
+  }
+  const auto syncs = ob->children_named("sync");
+  if (!syncs.empty()) {
+    grid_obs.write_permission = true;
+    for (const Element *e : syncs) {
+      grid_obs.do_sync.push_back(e->attr_bool("cue"));
+      grid_obs.sim_sync_freq.push_back(e->attr_double("freq") * units::GHz);
+      grid_obs.sim_sync_name.push_back(e->attr_string("filename"));
+      grid_obs.nside_sync.push_back(e->attr_uint("nside"));
+    }
+  } else {
+    grid_obs.do_sync.push_back(false);
+  }
+  if (grid_obs.write_permission) {
+    const Element *sh = trace(root, {"grid", "shell"});
+    grid_obs.oc_r_min = fval(sh, "oc_r_min") * units::kpc;
+    grid_obs.oc_r_max = fval(sh, "oc_r_max") * units::kpc;
+    grid_obs.gc_r_min = fval(sh, "gc_r_min") * units::kpc;
+    grid_obs.gc_r_max = fval(sh, "gc_r_max") * units::kpc;
+    grid_obs.gc_z_min = fval(sh, "gc_z_min") * units::kpc;
+    grid_obs.gc_z_max = fval(sh, "gc_z_max") * units::kpc;
+    grid_obs.oc_r_res = fval(sh, "oc_r_res") * units::kpc;
+    const std::string layer = trace(sh, {"layer"})->attr_string("type");
+    if (layer == "auto") {
+      const Element *a = trace(sh, {"layer", "auto"});
+      grid_obs.total_shell = uval(a, "shell_num");
+      const unsigned long long nside_sim = uval(a, "nside_sim");
+      for (unsigned long long i = 0; i != grid_obs.total_shell; ++i)
+        grid_obs.nside_shell.push_back(
+            static_cast<unsigned long long>(std::pow(2, i) * nside_sim));
+      grid_obs.radii_shell.push_back(grid_obs.oc_r_min);
+      for (unsigned long long i = 0; i < grid_obs.total_shell; ++i)
+        grid_obs.radii_shell.push_back(
+            (grid_obs.oc_r_max - grid_obs.oc_r_min) *
+                std::pow(0.5, static_cast<double>(grid_obs.total_shell - i - 1)) +
+            grid_obs.oc_r_min);
+    } else if (layer == "manual") {
+      const Element *m = trace(sh, {"layer", "manual"});
+      for (const Element *e : m->children_named("nside_sim"))
+        grid_obs.nside_shell.push_back(e->attr_uint("value"));
+      grid_obs.total_shell = grid_obs.nside_shell.size();
+      for (const Element *e : m->children_named("cut"))
+        grid_obs.cut_shell.push_back(e->attr_double("value"));
+      grid_obs.cut_shell.push_back(1.);
+      if (grid_obs.cut_shell.size() != grid_obs.total_shell)
+        throw std::runtime_error("manual layer: cut count must be nside_sim count - 1");
+      grid_obs.radii_shell.push_back(grid_obs.oc_r_min);
+      for (double c : grid_obs.cut_shell)
+        grid_obs.radii_shell.push_back((grid_obs.oc_r_max - grid_obs.oc_r_min) * c +
+                                       grid_obs.oc_r_min);
+    } else {
+      throw std::runtime_error("unsupported layer option");
+    }
+    const Element *mk = trace(root, {"mask"});
+    if (mk->attr_bool("cue")) {
+      grid_obs.do_mask = true;
+      grid_obs.mask_name = mk->attr_string("filename");
+      grid_obs.nside_mask = mk->attr_uint("nside");
+    }
+  }
+
+  // regular magnetic field (param.cc:146-343)
+  read_fieldio(root, "breg", grid_breg);
+  const Element *mf = trace(root, {"magneticfield"});
+  grid_breg.build_permission = trace(mf, {"regular"})->attr_bool("cue");
+  if (grid_breg.build_permission && !grid_breg.read_permission) {
+    breg_type = trace(mf, {"regular"})->attr_string("type");
+    if (breg_type == "lsa") {
+      const Element *s = trace(mf, {"regular", "lsa"});
+      breg_lsa.b0 = fval(s, "b0") * units::muGauss;
+      breg_lsa.psi0 = fval(s, "psi0") * units::rad;
+      breg_lsa.psi1 = fval(s, "psi1") * units::rad;
+      breg_lsa.chi0 = fval(s, "chi0") * units::rad;
+    } else if (breg_type == "unif") {
+      const Element *s = trace(mf, {"regular", "unif"});
+      breg_unif.bp = fval(s, "bp") * units::muGauss;
+      breg_unif.bv = fval(s, "bv") * units::muGauss;
+      breg_unif.l0 = fval(s, "l0") * units::rad;
+    } else if (breg_type == "cart") {
+      const Element *s = trace(mf, {"regular", "cart"});
+      breg_cart.bx = fval(s, "bx") * units::muGauss;
+      breg_cart.by = fval(s, "by") * units::muGauss;
+      breg_cart.bz = fval(s, "bz") * units::muGauss;
+    } else if (breg_type != "jaffe" && breg_type != "helix" && breg_type != "jf12") {
+      throw std::runtime_error("unsupported breg model");
+    }
+  }
+  if (grid_breg.read_permission || grid_breg.write_permission)
+    read_box(root, "box_breg", grid_breg);
+
+  // random magnetic field (param.cc:345-460)
+  read_fieldio(root, "brnd", grid_brnd);
+  grid_brnd.build_permission = trace(mf, {"random"})->attr_bool("cue");
+  if (grid_brnd.build_permission && !grid_brnd.read_permission) {
+    brnd_seed = trace(mf, {"random"})->attr_uint("seed");
+    brnd_type = trace(mf, {"random"})->attr_string("type");
+    if (brnd_type == "global") {
+      brnd_method = trace(mf, {"random", "global"})->attr_string("type");
+      if (brnd_method == "es") {
+        const Element *s = trace(mf, {"random", "global", "es"});
+        brnd_es.rms = fval(s, "rms") * units::muGauss;
+        brnd_es.k0 = fval(s, "k0");
+        brnd_es.a0 = fval(s, "a0");
+        brnd_es.k1 = fval(s, "k1");
+        brnd_es.a1 = fval(s, "a1");
+        brnd_es.rho = fval(s, "rho");
+        brnd_es.r0 = fval(s, "r0") * units::kpc;
+        brnd_es.z0 = fval(s, "z0") * units::kpc;
+      } else if (brnd_method != "jf12" && brnd_method != "jaffe") {
+        throw std::runtime_error("unsupported brnd model");
+      }
+    } else if (brnd_type == "local") {
+      brnd_method = trace(mf, {"random", "local"})->attr_string("type");
+      if (brnd_method != "mhd")
+        throw std::runtime_error("unsupported brnd model");
+    } else {
+      throw std::runtime_error("unsupported brnd type");
+    }
+  }
+  if (grid_brnd.read_permission || grid_brnd.write_permission || grid_brnd.build_permission)
+    read_box(root, "box_brnd", grid_brnd);
+
+  // regular thermal electrons (param.cc:462-667)
+  read_fieldio(root, "tereg", grid_tereg);
+  const Element *te = trace(root, {"thermalelectron"});
+  grid_tereg.build_permission = trace(te, {"regular"})->attr_bool("cue");
+  if (grid_tereg.build_permission) {
+    tereg_type = trace(te, {"regular"})->attr_string("type");
+    if (tereg_type == "unif") {
+      const Element *s = trace(te, {"regular", "unif"});
+      tereg_unif.n0 = fval(s, "n0");
+      tereg_unif.r0 = fval(s, "r0") * units::kpc;
+    } else if (tereg_type != "ymw16") {
+      throw std::runtime_error("unsupported tereg model");
+    }
+  }
+  if (grid_tereg.read_permission || grid_tereg.write_permission)
+    read_box(root, "box_tereg", grid_tereg);
+
+  // random thermal electrons (param.cc:669-716); "dgm" names the DFT method
+  read_fieldio(root, "ternd", grid_ternd);
+  grid_ternd.build_permission = trace(te, {"random"})->attr_bool("cue");
+  if (grid_ternd.build_permission) {
+    ternd_seed = trace(te, {"random"})->attr_uint("seed");
+    ternd_type = trace(te, {"random"})->attr_string("type");
+    if (ternd_type == "global") {
+      ternd_method = trace(te, {"random", "global"})->attr_string("type");
+      if (ternd_method == "dft" || ternd_method == "dgm") {
+        ternd_method = "dft";
+        const Element *s = trace(te, {"random", "global", "dft"});
+        ternd_dft.rms = fval(s, "rms");
+        ternd_dft.k0 = fval(s, "k0");
+        ternd_dft.a0 = fval(s, "a0");
+        ternd_dft.r0 = fval(s, "r0") * units::kpc;
+        ternd_dft.z0 = fval(s, "z0") * units::kpc;
+      } else {
+        throw std::runtime_error("unsupported ternd model");
+      }
+    } else {
+      throw std::runtime_error("unsupported ternd type");
+    }
+  }
+  if (grid_ternd.read_permission || grid_ternd.write_permission || grid_ternd.build_permission)
+    read_box(root, "box_ternd", grid_ternd);
+
+  // cosmic-ray electrons (param.cc:718-778)
+  read_fieldio(root, "cre", grid_cre);
+  const Element *cr = trace(root, {"cre"});
+  grid_cre.build_permission = cr->attr_bool("cue");
+  if (grid_cre.build_permission) {
+    cre_type = cr->attr_string("type");
+    if (cre_type == "analytic") {
+      const Element *s = trace(cr, {"analytic"});
+      cre_ana.alpha = fval(s, "alpha");
+      cre_ana.beta = fval(s, "beta");
+      cre_ana.theta = fval(s, "theta");
+      cre_ana.r0 = fval(s, "r0") * units::kpc;
+      cre_ana.z0 = fval(s, "z0") * units::kpc;
+      cre_ana.E0 = fval(s, "E0") * units::GeV;
+      cre_ana.j0 = fval(s, "j0");
+    } else if (cre_type == "unif") {
+      const Element *s = trace(cr, {"unif"});
+      cre_unif.alpha = fval(s, "alpha");
+      cre_unif.r0 = fval(s, "r0") * units::kpc;
+      cre_unif.E0 = fval(s, "E0") * units::GeV;
+      cre_unif.j0 = fval(s, "j0");
+    } else {
+      throw std::runtime_error("unsupported cre model");
+    }
+  }
+}
+
+} // namespace hamx_host

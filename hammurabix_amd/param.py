@@ -133,6 +133,7 @@ class ObsParam:
     sim_dm_name: str = ""
     sim_sync_name: list = field(default_factory=list)
     sim_sync_freq: list = field(default_factory=list)
+    sim_sync_freq_str: list = field(default_factory=list)  # XML text, GHz
     do_mask: bool = False
     mask_name: str = ""
 
@@ -213,6 +214,7 @@ class Param:
             for e in syncs:
                 g.do_sync.append(e.b("cue"))
                 g.sim_sync_freq.append(e.f("freq") * CGS.GHz)
+                g.sim_sync_freq_str.append(e.el.get("freq", "0"))
                 g.sim_sync_name.append(e.s("filename"))
                 g.nside_sync.append(e.u("nside"))
         else:

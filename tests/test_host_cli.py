@@ -1,0 +1,78 @@
+"""The C++ host mirror (Param / plugins / Grid_* / Integrator / Pipeline over
+the C-ABI) driven through the reference's command line, `hamx <params.xml>`."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import Param, configs
+from hammurabix_amd.pipeline import Pipeline
+from hammurabix_amd.units import CGS
+from oracle_lib import rel_l2, write_grid
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HAMX = os.path.join(ROOT, "hammurabix_amd", "host", "hamx")
+
+
+def need_cli():
+    if not os.path.exists(HAMX):
+        pytest.skip("hamx CLI not built (python -m hammurabix_amd._build)")
+
+
+def run(*args):
+    return subprocess.run([HAMX, *args], capture_output=True, text=True, timeout=300)
+
+
+def test_cli_usage_and_bad_file():
+    need_cli()
+    r = run()
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = run("/nonexistent/params.xml")
+    assert r.returncode == 1 and "cannot open parameter file" in r.stderr
+
+
+def test_cli_rejects_unsupported_models(tmp_path):
+    need_cli()
+    x = tmp_path / "p.xml"
+    x.write_text(configs.plain_forward().replace('type="analytic"', 'type="bogus"'))
+    r = run(str(x))
+    assert r.returncode == 1 and "unsupported cre model" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_matches_python_pipeline_and_oracle(tmp_path, ctx):
+    need_cli()
+    out = str(tmp_path) + "/"
+    text = configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1, grid=(32, 32, 16),
+                                  te_k0=0.3,
+                                  freqs=[23.0, 1.4], out=out)
+    bfile = str(tmp_path / "brnd.bin")
+    text = text.replace('<brnd read="0" write="0" filename="brnd.bin"/>',
+                        f'<brnd read="0" write="1" filename="{bfile}"/>')
+    tfile = str(tmp_path / "ternd.bin")
+    text = text.replace('<ternd read="0" write="0" filename="ternd.bin"/>',
+                        f'<ternd read="0" write="1" filename="{tfile}"/>')
+    x = tmp_path / "p.xml"
+    x.write_text(text)
+    r = run(str(x))
+    assert r.returncode == 0, r.stderr
+    py = Pipeline(text=text, ctx=ctx, export=False).run()
+    files = {("dm", "nan", "16", "nan"): "dm.bin", ("fd", "nan", "16", "nan"): "fd.bin"}
+    for f in ("23.0", "1.4"):
+        for c in "IQU":
+            files[("sync", f, "16", c)] = f"sync_{f}_{c}.bin"
+    for key, name in files.items():
+        got = np.fromfile(str(tmp_path / name), dtype="<f8")
+        assert np.array_equal(got, py[key]), key
+    # the first pass against the oracle on the exported grids
+    par = Param(text=text)
+    raw = np.fromfile(bfile, dtype="<f8").reshape(-1, 3)
+    b = tuple(np.ascontiguousarray(raw[:, i]) for i in range(3))
+    te = np.fromfile(tfile, dtype="<f8")
+    ref = write_grid(par.los_desc(), brnd=b, brnd_box=par.grid_brnd.box(), ternd=te,
+                     ternd_box=par.grid_ternd.box())
+    dm = np.fromfile(str(tmp_path / "dm.bin"), dtype="<f8")
+    assert rel_l2(dm, ref["dm"] * CGS.ccm / CGS.pc) < 1e-10
+    q = np.fromfile(str(tmp_path / "sync_1.4_Q.bin"), dtype="<f8")
+    assert rel_l2(q, ref["qs"]) < 1e-10
