@@ -12,7 +12,8 @@ accumulation) with the grids already resident in HBM.  The value is LOS
 samples per second: geometric samples sum_s npix_s * step_s, times the one
 <sync> entry, per wall second.
 
-roofline: the march kernel (los_wave), algorithmic bytes = 256 B per
+roofline: the march kernel (los_fast, the specialised LSA + uniform TE +
+analytic CRE march that the default schedule selects), algorithmic bytes = 256 B per
 geometric sample (8 corners x {bx,by,bz,te} x 8 B), divided by its launch
 durations measured with HIP events on the library stream.
 cpu_baseline: the FP64 CPU oracle (OpenMP, schedule(dynamic) pixel loop) on
@@ -38,13 +39,27 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device three times (a plain copy "
+                         "kernel of known size, to calibrate FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibrate(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibrate_bytes": n * 8}), flush=True)
 
 
 def traffic_from_profiles():
     """HBM bytes per march launch from the committed rocprofv3 PMC summary
-    (profiles/los_wave_random_forward_pmc.txt), or None."""
-    path = os.path.join(ROOT, "profiles", "los_wave_random_forward_pmc.txt")
+    (profiles/los_fast_random_forward_pmc.txt), or None."""
+    path = os.path.join(ROOT, "profiles", "los_fast_random_forward_pmc.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -78,12 +93,15 @@ def main():
         print("bench.py runs on one GPU only: the LOS path has no multi-GPU "
               "implementation in this build (pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     import numpy as np
     import torch
     from hammurabix_amd import Context, Param, _capi as capi, configs, los_samples
 
     par = Param(text=configs.random_forward())
-    desc = par.los_desc(schedule=capi.HAMX_SCHED_WAVE)
+    desc = par.los_desc(schedule=capi.HAMX_SCHED_AUTO)
     S_geo = los_samples(desc)
     k_sync = max(1, sum(1 for s in par.grid_obs.do_sync if s))
     S_ref = S_geo * k_sync
@@ -113,7 +131,8 @@ def main():
         elapsed = time.perf_counter() - t0
         times = ctx.kernel_times()
         ctx.profile(False)
-        march = [t for t in times if t["kernel"] == "los_wave"]
+        march = [t for t in times if t["kernel"].startswith("los_")]
+        march_kernels = sorted({t["kernel"] for t in march})
         march_ms = sum(t["ms"] for t in march)
         march_samples = sum(t["work"] for t in march)
         acc_ms = sum(t["ms"] for t in times if t["kernel"] == "accumulate")
@@ -157,6 +176,7 @@ def main():
             "traffic": traffic,
         },
         "detail": {
+            "march_kernels": march_kernels,
             "march_ms_per_step": march_ms / args.steps,
             "accumulate_ms_per_step": acc_ms / args.steps,
             "random_field_generation_ms": gen_ms,

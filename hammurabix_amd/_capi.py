@@ -158,5 +158,11 @@ class HamxKernelTime(C.Structure):
 
 
 _PROTOS["hamx_profile"] = (C.c_int, [C.c_void_p, C.c_int])
+_PROTOS["hamx_los_adjoint"] = (C.c_int, [C.c_void_p, C.POINTER(HamxLosDesc),
+                                         C.POINTER(HamxFields), C.POINTER(HamxMaps),
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_uint32])
+HAMX_NPARAM = 6
+PARAM_NAMES = ("b0", "psi0", "psi1", "chi0", "alpha", "n0")
 _PROTOS["hamx_profile_read"] = (C.c_int, [C.c_void_p, C.POINTER(HamxKernelTime), C.c_int,
                                           C.POINTER(C.c_int)])

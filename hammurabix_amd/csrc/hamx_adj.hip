@@ -1,0 +1,544 @@
+// Reverse mode of the LOS integrator: the gradient of a scalar L of the
+// output maps with respect to the LSA field parameters, the CRE spectral
+// index, the uniform TE density and the random-field grids.  The reference
+// has no gradients; the forward being differentiated is integrator.cc:23-228.
+//
+// Forward structure (per shell s, FD carried through the accumulated map):
+//   f_k   = te_k B_par,k C delta                  (Faraday depth of sample k)
+//   FD_k  = sum_{j<=k} f_j,  qui_k = (fd_in + FD_k) lambda^2 + ipa_k
+//   Q = sum_k cos(2 qui_k) jp_k, U = sum_k sin(2 qui_k) jp_k, I = sum_k jt_k
+//   maps  = sum_s A_s(shell maps),  fd_in(s) = Interp_s(F_s),
+//   F_s   = sum_{t<s} A_t(shell FD_t)
+// Reverse:
+//   dL/dqui_k = 2 jp_k (-gQ sin 2qui_k + gU cos 2qui_k)
+//   dL/df_j   = lambda^2 (T - sum_{k<j} dL/dqui_k) + gFD,  T = sum_k dL/dqui_k
+//   dL/dfd_in = lambda^2 T
+// so each ray is marched twice: once for T, once for the per-sample
+// gradients.  Shells are processed outermost first; the FD adjoint G
+// (nside of the FD map) collects A^T of the output FD adjoint plus
+// Interp_s^T(dL/dfd_in) of every shell already processed.  The forward is
+// recomputed with the FD map checkpointed before each shell.
+//
+// Scatter: the shell-map adjoints come from A^T (transposed bilinear
+// HEALPix interpolation, FP64 atomics), the grid gradient from the
+// transposed trilinear gather (FP64 atomics into a packed {x,y,z,te} record
+// per vertex), parameter gradients from per-ray partial sums reduced by a
+// second kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "hamx_march.h"
+
+namespace hamx {
+
+// digamma(x): recurrence up to x >= 6, then the asymptotic series in 1/x^2
+// (Abramowitz & Stegun 6.3.18); reflection for x <= 0.
+static double digamma(double x) {
+  if (x <= 0.) {
+    const double pi = cgs::pi;
+    return digamma(1. - x) - pi / std::tan(pi * x);
+  }
+  double r = 0.;
+  while (x < 6.) {
+    r -= 1. / x;
+    x += 1.;
+  }
+  const double f = 1. / (x * x);
+  const double series =
+      f * (1. / 12. - f * (1. / 120. - f * (1. / 252. - f * (1. / 240. - f * (1. / 132.)))));
+  return r + std::log(x) - 0.5 / x - series;
+}
+
+struct AdjConst {
+  double tc;        // temp_convert factor of I, Q, U
+  double l2, two_l2;
+  double delta;
+  double alpha1;    // 1 - index = 1 + alpha
+  double lt, lp;    // d ln jt / d alpha, d ln jp / d alpha without the ln|B_perp| term
+  double b0;
+};
+
+struct AdjShell {
+  const double *gi, *gq, *gu, *gdm, *gfd; // shell-map adjoints (nside_s)
+  double *gfdin;                          // out: dL/d(fd_inner) per pixel
+  double *ray_par;                        // out: 8 parameter partials per ray
+  double *ggrid;                          // packed {x,y,z,te} grid gradient
+  uint64_t sy, sx;                        // corner strides of the packed grid
+};
+
+__device__ inline void grid_scatter(double *g, uint64_t idx, double w, const Vec3 &dB,
+                                    double dte) {
+  double *p = g + 4 * idx;
+  unsafeAtomicAdd(p + 0, w * dB.x);
+  unsafeAtomicAdd(p + 1, w * dB.y);
+  unsafeAtomicAdd(p + 2, w * dB.z);
+  unsafeAtomicAdd(p + 3, w * dte);
+}
+
+template <bool GRIDS, bool PW1>
+__global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst ac,
+                                                 Grids gr, ShellArgs sa, AdjShell as) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t ipix =
+      static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ipix >= sa.hs.npix)
+    return; // uniform per wavefront
+  RayGeo r;
+  ray_setup(ph, sa, ipix, r);
+  const double gq = as.gq ? as.gq[ipix] * ac.tc : 0.;
+  const double gu = as.gu ? as.gu[ipix] * ac.tc : 0.;
+  const double gi = as.gi ? as.gi[ipix] * ac.tc : 0.;
+  const double gdm = as.gdm ? as.gdm[ipix] : 0.;
+  const double gfd = as.gfd ? as.gfd[ipix] : 0.;
+
+  // sweep 1: T = sum_k dL/dqui_k
+  double carry = 0., tpart = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    FastSample o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
+    const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
+    const double fd_here = carry + incl;
+    carry += __shfl(incl, 63, 64);
+    if (valid) {
+      double sph, cph;
+      sincos(ac.two_l2 * (r.fd_inner + fd_here), &sph, &cph);
+      const double c2q = cph * o.c2i - sph * o.s2i, s2q = sph * o.c2i + cph * o.s2i;
+      tpart += 2. * o.jp * (-gq * s2q + gu * c2q);
+    }
+  }
+  const double T = wave_sum(tpart);
+
+  // sweep 2: per-sample gradients
+  double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0.;
+  double dq_carry = 0.;
+  carry = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    FastSample o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
+    const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
+    const double fd_here = carry + incl;
+    carry += __shfl(incl, 63, 64);
+    double c2q = 1., s2q = 0., dq = 0.;
+    if (valid) {
+      double sph, cph;
+      sincos(ac.two_l2 * (r.fd_inner + fd_here), &sph, &cph);
+      c2q = cph * o.c2i - sph * o.s2i;
+      s2q = sph * o.c2i + cph * o.s2i;
+      dq = 2. * o.jp * (-gq * s2q + gu * c2q);
+    }
+    const double dq_incl = wave_incl_scan(dq, lane);
+    const double excl = dq_carry + (dq_incl - dq);
+    dq_carry += __shfl(dq_incl, 63, 64);
+    if (!valid)
+      continue;
+    const double R = ac.l2 * (T - excl) + gfd; // dL/df_k
+    const double djp = gq * c2q + gu * s2q;
+    const double djt = gi;
+    const double dte = (o.te_raw > 0.) ? (R * o.b_par * fc.fd_scale + gdm * ac.delta) : 0.;
+    // dL/dB
+    const double ft = R * o.te * fc.fd_scale;
+    Vec3 dB{ft * r.l.x, ft * r.l.y, ft * r.l.z};
+    if (o.bper2 > 0.) {
+      const double coef = (djt * o.jt + djp * o.jp) * ac.alpha1 / (2. * o.bper2);
+      dB.x += coef * (o.B.x - o.b_par * r.l.x);
+      dB.y += coef * (o.B.y - o.b_par * r.l.y);
+      dB.z += coef * (o.B.z - o.b_par * r.l.z);
+      const double lnb = 0.5 * log(o.bper);
+      p4 += djt * o.jt * (ac.lt + lnb) + djp * o.jp * (ac.lp + lnb);
+    }
+    if (o.n2 > 0.) {
+      const double c = dq / o.n2;
+      dB.x += c * (o.y * r.eph.x - o.x * r.eth.x);
+      dB.y += c * (o.y * r.eph.y - o.x * r.eth.y);
+      dB.z += c * (o.y * r.eph.z - o.x * r.eth.z);
+    }
+    if (o.te_in)
+      p5 += dte;
+    if (o.lsa_in) {
+      const double u = o.sps * o.cos_p - o.cps * o.sin_p; // B_x / (b0 cos chi)
+      const double v = o.sps * o.sin_p + o.cps * o.cos_p; // B_y / (b0 cos chi)
+      p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
+      const double gpsi = ac.b0 * o.cch * (dB.x * v - dB.y * u);
+      p1 += gpsi;
+      p2 += gpsi * o.lnr;
+      const double gchi = ac.b0 * (-o.sch * (dB.x * u + dB.y * v) + o.cch * dB.z);
+      p3 += gchi * o.tnh;
+    }
+    if (GRIDS && o.cell_ok && as.ggrid) {
+      const double xd = o.cell.xd, yd = o.cell.yd, zd = o.cell.zd;
+      const uint64_t i0 = o.cell.idx;
+      const double wx[2] = {1. - xd, xd}, wy[2] = {1. - yd, yd}, wz[2] = {1. - zd, zd};
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            grid_scatter(as.ggrid, i0 + a * as.sx + b * as.sy + c, wx[a] * wy[b] * wz[c], dB,
+                         dte);
+    }
+  }
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  p2 = wave_sum(p2);
+  p3 = wave_sum(p3);
+  p4 = wave_sum(p4);
+  p5 = wave_sum(p5);
+  if (lane == 0) {
+    double *rp = as.ray_par + 8 * ipix;
+    rp[0] = p0;
+    rp[1] = p1;
+    rp[2] = p2;
+    rp[3] = p3;
+    rp[4] = p4;
+    rp[5] = p5;
+    if (as.gfdin)
+      as.gfdin[ipix] = ac.l2 * T;
+  }
+}
+
+// dst (shell maps) += A^T(src) for up to 5 map pairs (the transpose of
+// k_accumulate: same-nside add, else transposed bilinear interpolation).
+struct AccTPairs {
+  double *dst[5];
+  const double *src[5];
+  int n;
+};
+
+__global__ __launch_bounds__(256) void k_accumulate_t(AccTPairs p, Healpix hd, Healpix hs) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= hd.npix)
+    return;
+  if (hd.npix == hs.npix) {
+    for (int m = 0; m < p.n; ++m)
+      p.dst[m][i] += p.src[m][i];
+    return;
+  }
+  double theta, phi;
+  hp_pix2ang(hd, i, theta, phi);
+  uint64_t pix[4];
+  double w[4];
+  hp_interpol_weights(hs, theta, phi, pix, w);
+  const double wtot = ((w[0] + w[1]) + w[2]) + w[3];
+  for (int m = 0; m < p.n; ++m) {
+    const double g = p.src[m][i] / wtot;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      unsafeAtomicAdd(&p.dst[m][pix[q]], w[q] * g);
+  }
+}
+
+// G (FD-map adjoint) += Interp_s^T(gfdin): transposed interpolation of the
+// FD cache at the shell's pixel centres.
+__global__ __launch_bounds__(256) void k_fdcache_t(double *G, Healpix hfd,
+                                                   const double *gfdin, Healpix hs) {
+  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= hs.npix)
+    return;
+  const double g = gfdin[p];
+  if (g == 0.)
+    return;
+  double theta, phi;
+  hp_pix2ang(hs, p, theta, phi);
+  uint64_t pix[4];
+  double w[4];
+  hp_interpol_weights(hfd, theta, phi, pix, w);
+  const double wtot = ((w[0] + w[1]) + w[2]) + w[3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    unsafeAtomicAdd(&G[pix[q]], w[q] * (g / wtot));
+}
+
+// Column sums of the per-ray parameter partials into acc[0..5].
+__global__ __launch_bounds__(256) void k_reduce_params(const double *ray_par, uint64_t n,
+                                                       double *acc) {
+  __shared__ double sm[4];
+  const int j = blockIdx.y;
+  double v = 0.;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    v += ray_par[8 * i + j];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0)
+    sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    unsafeAtomicAdd(&acc[j], sm[0] + sm[1] + sm[2] + sm[3]);
+}
+
+__global__ void k_unpack4(const double *__restrict__ g, double *__restrict__ gx,
+                          double *__restrict__ gy, double *__restrict__ gz,
+                          double *__restrict__ gt, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    if (gx) gx[i] = g[4 * i + 0];
+    if (gy) gy[i] = g[4 * i + 1];
+    if (gz) gz[i] = g[4 * i + 2];
+    if (gt) gt[i] = g[4 * i + 3];
+  }
+}
+
+static unsigned blocks_for(uint64_t n, unsigned per = 256) {
+  return static_cast<unsigned>((n + per - 1) / per);
+}
+
+} // namespace hamx
+
+using namespace hamx;
+
+extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
+                                const hamx_fields *fields, const hamx_maps *adj,
+                                double *dparams, double *dbrnd_x, double *dbrnd_y,
+                                double *dbrnd_z, double *dternd, uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !adj || !dparams)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_los_adjoint: null descriptor, adjoint maps or "
+                                      "parameter gradient");
+  HAMX_TRY(validate_los(ctx, d));
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  hamx_fields none;
+  std::memset(&none, 0, sizeof(none));
+  const hamx_fields &f = fields ? *fields : none;
+  const Phys ph = make_phys(d);
+  if (!(d->breg_model == HAMX_BREG_LSA && d->tereg_model == HAMX_TEREG_UNIF &&
+        ph.idx_const && d->do_sync &&
+        (d->cre_model == HAMX_CRE_ANA || d->cre_model == HAMX_CRE_UNIF)))
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "hamx_los_adjoint supports LSA B, uniform TE and a constant-index CRE "
+                   "with synchrotron output");
+
+  // ---- forward with FD-map checkpoints
+  const uint64_t nside_fd_map = d->do_fd ? d->nside_fd : d->nside_sync;
+  const Healpix h_fd = healpix_make(nside_fd_map);
+  const Healpix h_sync = healpix_make(d->nside_sync);
+  const Healpix h_dm = healpix_make(d->do_dm ? d->nside_dm : 1);
+  void *q;
+  double *ckpt = nullptr;
+  if (d->do_fd) {
+    HAMX_TRY(ws_get(ctx, kSlotAdjCkpt, d->n_shell * h_fd.npix * sizeof(double), &q));
+    ckpt = static_cast<double *>(q);
+    hamx_maps none_maps;
+    std::memset(&none_maps, 0, sizeof(none_maps));
+    HAMX_TRY(los_forward_impl(ctx, d, fields, &none_maps, flags & HAMX_IN_DEVICE, ckpt));
+  }
+
+  // ---- grids and adjoint inputs on the device
+  Grids gr;
+  bool packed = false, grids_any = false;
+  HAMX_TRY(stage_fields(ctx, d, f, in_dev, gr, packed, grids_any));
+  if (grids_any && !packed)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "hamx_los_adjoint needs the random B and TE grids on one box");
+  const void *cp;
+  const double *a_is = nullptr, *a_qs = nullptr, *a_us = nullptr, *a_dm = nullptr,
+               *a_fd = nullptr;
+  HAMX_TRY(stage_in(ctx, kSlotAdjIs, adj->is, h_sync.npix * sizeof(double), in_dev, &cp));
+  a_is = static_cast<const double *>(cp);
+  HAMX_TRY(stage_in(ctx, kSlotAdjQs, adj->qs, h_sync.npix * sizeof(double), in_dev, &cp));
+  a_qs = static_cast<const double *>(cp);
+  HAMX_TRY(stage_in(ctx, kSlotAdjUs, adj->us, h_sync.npix * sizeof(double), in_dev, &cp));
+  a_us = static_cast<const double *>(cp);
+  if (d->do_dm) {
+    HAMX_TRY(stage_in(ctx, kSlotAdjDm, adj->dm, h_dm.npix * sizeof(double), in_dev, &cp));
+    a_dm = static_cast<const double *>(cp);
+  }
+  if (d->do_fd) {
+    HAMX_TRY(stage_in(ctx, kSlotAdjFd, adj->fd, h_fd.npix * sizeof(double), in_dev, &cp));
+    a_fd = static_cast<const double *>(cp);
+  }
+
+  // ---- work buffers
+  uint64_t max_npix = 0;
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    const uint64_t ns = d->nside_shell[s];
+    if (12 * ns * ns > max_npix)
+      max_npix = 12 * ns * ns;
+  }
+  double *sg[5];
+  for (int m = 0; m < 5; ++m) {
+    HAMX_TRY(ws_get(ctx, kSlotAdjShell0 + m, max_npix * sizeof(double), &q));
+    sg[m] = static_cast<double *>(q);
+  }
+  HAMX_TRY(ws_get(ctx, kSlotAdjG, h_fd.npix * sizeof(double), &q));
+  double *G = static_cast<double *>(q);
+  HAMX_TRY(ws_get(ctx, kSlotAdjFdin, max_npix * sizeof(double), &q));
+  double *gfdin = static_cast<double *>(q);
+  HAMX_TRY(ws_get(ctx, kSlotAdjRayPar, 8 * max_npix * sizeof(double), &q));
+  double *ray_par = static_cast<double *>(q);
+  HAMX_TRY(ws_get(ctx, kSlotAdjPar, 8 * sizeof(double), &q));
+  double *acc = static_cast<double *>(q);
+  HAMX_TRY(fill_f64(ctx, acc, 8, 0.));
+  double *ggrid = nullptr;
+  uint64_t ngrid = 0;
+  if (packed) {
+    ngrid = box_size(f.brnd_box);
+    HAMX_TRY(ws_get(ctx, kSlotAdjGrid, 4 * ngrid * sizeof(double), &q));
+    ggrid = static_cast<double *>(q);
+    HAMX_TRY(fill_f64(ctx, ggrid, 4 * ngrid, 0.));
+  }
+  if (d->do_fd) {
+    if (a_fd)
+      HAMX_HIP(ctx, hipMemcpyAsync(G, a_fd, h_fd.npix * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+    else
+      HAMX_TRY(fill_f64(ctx, G, h_fd.npix, 0.));
+  }
+
+  // ---- constants
+  AdjConst ac;
+  std::memset(&ac, 0, sizeof(ac));
+  ac.tc = ph.tc_em1 * ph.tc_em1 / ph.tc_den;
+  ac.l2 = ph.lambda_sq;
+  ac.two_l2 = 2. * ph.lambda_sq;
+  ac.alpha1 = 1. - ph.idx;
+  ac.b0 = d->breg[0];
+  {
+    const double alpha = -ph.idx;
+    const double E0 = d->cre_model == HAMX_CRE_UNIF ? d->cre[2] : d->cre[5];
+    const double ln_g0 = std::log(E0 / cgs::mec2);
+    const double half_ln_a = 0.5 * std::log(ph.A_num / ph.three_qe);
+    const double psi_t = 0.25 * digamma(alpha / 4. + 19. / 12.);
+    const double psi_p = 0.25 * digamma(alpha / 4. + 7. / 12.);
+    const double psi_c = 0.25 * digamma(alpha / 4. - 1. / 12.);
+    ac.lt = ln_g0 - half_ln_a - 1. / (1. + alpha) + psi_t + psi_c;
+    ac.lp = ln_g0 - half_ln_a + psi_p + psi_c;
+  }
+  const bool pw1 = 0.5 * (ph.idx + 1) == -1.0;
+
+  // ---- shells, outermost first
+  for (int s = static_cast<int>(d->n_shell) - 1; s >= 0; --s) {
+    ShellArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.hs = healpix_make(d->nside_shell[s]);
+    shell_ref(d, static_cast<uint32_t>(s), sa.step, sa.d_start, sa.delta);
+    const bool has_fdin = d->do_fd && s > 0;
+    sa.fd_map = has_fdin ? ckpt + static_cast<uint64_t>(s) * h_fd.npix : nullptr;
+    sa.hfd = h_fd;
+    const uint64_t npix = sa.hs.npix;
+    for (int m = 0; m < 5; ++m)
+      HAMX_TRY(fill_f64(ctx, sg[m], npix, 0.));
+    // shell-map adjoints: A_s^T of the output adjoints
+    auto acc_t = [&](AccTPairs &pairs, const Healpix &hd) -> int {
+      if (pairs.n == 0)
+        return HAMX_OK;
+      hipLaunchKernelGGL(k_accumulate_t, dim3(blocks_for(hd.npix)), dim3(256), 0,
+                         ctx->stream, pairs, hd, sa.hs);
+      HAMX_HIP(ctx, hipGetLastError());
+      return HAMX_OK;
+    };
+    AccTPairs p1;
+    std::memset(&p1, 0, sizeof(p1));
+    const double *sync_src[3] = {a_is, a_qs, a_us};
+    for (int m = 0; m < 3; ++m)
+      if (sync_src[m]) { // a NULL adjoint map is zero
+        p1.dst[p1.n] = sg[m];
+        p1.src[p1.n] = sync_src[m];
+        ++p1.n;
+      }
+    HAMX_TRY(acc_t(p1, h_sync));
+    if (d->do_dm && a_dm) {
+      AccTPairs p2;
+      std::memset(&p2, 0, sizeof(p2));
+      p2.dst[0] = sg[3]; p2.src[0] = a_dm; p2.n = 1;
+      HAMX_TRY(acc_t(p2, h_dm));
+    }
+    if (d->do_fd) {
+      AccTPairs p3;
+      std::memset(&p3, 0, sizeof(p3));
+      p3.dst[0] = sg[4]; p3.src[0] = G; p3.n = 1;
+      HAMX_TRY(acc_t(p3, h_fd));
+    }
+    AdjShell as;
+    std::memset(&as, 0, sizeof(as));
+    as.gi = sg[0];
+    as.gq = sg[1];
+    as.gu = sg[2];
+    as.gdm = d->do_dm ? sg[3] : nullptr;
+    as.gfd = d->do_fd ? sg[4] : nullptr;
+    as.gfdin = has_fdin ? gfdin : nullptr;
+    as.ray_par = ray_par;
+    as.ggrid = ggrid;
+    as.sy = gr.g_packed.nz;
+    as.sx = gr.g_packed.ny * gr.g_packed.nz;
+    const FastConst fc = make_fast(d, ph, sa.delta, pw1);
+    ac.delta = sa.delta;
+    const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
+    ProfMark pm = prof_begin(ctx);
+    if (packed && pw1)
+      hipLaunchKernelGGL((k_los_adj<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,
+                         ph, fc, ac, gr, sa, as);
+    else if (packed)
+      hipLaunchKernelGGL((k_los_adj<true, false>), dim3(blocks), dim3(256), 0, ctx->stream,
+                         ph, fc, ac, gr, sa, as);
+    else if (pw1)
+      hipLaunchKernelGGL((k_los_adj<false, true>), dim3(blocks), dim3(256), 0, ctx->stream,
+                         ph, fc, ac, gr, sa, as);
+    else
+      hipLaunchKernelGGL((k_los_adj<false, false>), dim3(blocks), dim3(256), 0, ctx->stream,
+                         ph, fc, ac, gr, sa, as);
+    HAMX_HIP(ctx, hipGetLastError());
+    prof_end(ctx, pm, "los_adjoint", static_cast<uint32_t>(s), npix * sa.step);
+    unsigned rb = blocks_for(npix);
+    if (rb > 256)
+      rb = 256;
+    hipLaunchKernelGGL(k_reduce_params, dim3(rb, 6), dim3(256), 0, ctx->stream,
+                       static_cast<const double *>(ray_par), npix, acc);
+    HAMX_HIP(ctx, hipGetLastError());
+    if (has_fdin) {
+      hipLaunchKernelGGL(k_fdcache_t, dim3(blocks_for(npix)), dim3(256), 0, ctx->stream, G,
+                         h_fd, static_cast<const double *>(gfdin), sa.hs);
+      HAMX_HIP(ctx, hipGetLastError());
+    }
+  }
+
+  // ---- outputs
+  if (packed && (dbrnd_x || dbrnd_y || dbrnd_z || dternd)) {
+    double *ox = dbrnd_x, *oy = dbrnd_y, *oz = dbrnd_z, *ot = dternd;
+    if (!out_dev) {
+      HAMX_TRY(ws_get(ctx, kSlotRndOut0, 4 * ngrid * sizeof(double), &q));
+      double *stage = static_cast<double *>(q);
+      ox = dbrnd_x ? stage : nullptr;
+      oy = dbrnd_y ? stage + ngrid : nullptr;
+      oz = dbrnd_z ? stage + 2 * ngrid : nullptr;
+      ot = dternd ? stage + 3 * ngrid : nullptr;
+    }
+    unsigned ub = blocks_for(ngrid);
+    if (ub > 16384)
+      ub = 16384;
+    hipLaunchKernelGGL(k_unpack4, dim3(ub), dim3(256), 0, ctx->stream,
+                       static_cast<const double *>(ggrid), ox, oy, oz, ot, ngrid);
+    HAMX_HIP(ctx, hipGetLastError());
+    if (!out_dev) {
+      const size_t nb = ngrid * sizeof(double);
+      if (dbrnd_x)
+        HAMX_HIP(ctx, hipMemcpyAsync(dbrnd_x, ox, nb, hipMemcpyDeviceToHost, ctx->stream));
+      if (dbrnd_y)
+        HAMX_HIP(ctx, hipMemcpyAsync(dbrnd_y, oy, nb, hipMemcpyDeviceToHost, ctx->stream));
+      if (dbrnd_z)
+        HAMX_HIP(ctx, hipMemcpyAsync(dbrnd_z, oz, nb, hipMemcpyDeviceToHost, ctx->stream));
+      if (dternd)
+        HAMX_HIP(ctx, hipMemcpyAsync(dternd, ot, nb, hipMemcpyDeviceToHost, ctx->stream));
+    }
+  }
+  if (out_dev)
+    HAMX_HIP(ctx, hipMemcpyAsync(dparams, acc, HAMX_NPARAM * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+  else
+    HAMX_HIP(ctx, hipMemcpyAsync(dparams, acc, HAMX_NPARAM * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}

@@ -109,29 +109,38 @@ __device__ inline double lerp_cell(const double *f, const Cell &c, uint64_t sy,
 // Packed gather: the B components and the TE density of one vertex stored
 // together as 4 doubles (32 B); the two z-neighbours of a column are one
 // contiguous 64 B record.  Same arithmetic as the separate gathers.
+// One x-plane of the cell: the z- then y-interpolated 4 components.
+__device__ inline double4 lerp_plane_packed(const double4 *__restrict__ f, uint64_t i0,
+                                            uint64_t sy, double zd, double yd) {
+  const double4 a0 = f[i0], a1 = f[i0 + 1];
+  const double4 b0 = f[i0 + sy], b1 = f[i0 + sy + 1];
+  double4 w;
+#define HAMX_LERP2(C)                                                          \
+  {                                                                            \
+    const double i1 = a0.C * (1. - zd) + a1.C * zd;                            \
+    const double i2 = b0.C * (1. - zd) + b1.C * zd;                            \
+    w.C = i1 * (1. - yd) + i2 * yd;                                            \
+  }
+  HAMX_LERP2(x)
+  HAMX_LERP2(y)
+  HAMX_LERP2(z)
+  HAMX_LERP2(w)
+#undef HAMX_LERP2
+  return w;
+}
+
+// The two x-planes are interpolated one after the other so that only four
+// 32-byte records are live at a time.
 __device__ inline void lerp_cell_packed(const double4 *__restrict__ f,
                                         const Cell &c, uint64_t sy, uint64_t sx,
                                         Vec3 &b, double &te) {
-  const double4 a0 = f[c.idx], a1 = f[c.idx + 1];
-  const double4 b0 = f[c.idx + sy], b1 = f[c.idx + sy + 1];
-  const double4 c0 = f[c.idx + sx], c1 = f[c.idx + sx + 1];
-  const double4 d0 = f[c.idx + sx + sy], d1 = f[c.idx + sx + sy + 1];
-  const double zd = c.zd, yd = c.yd, xd = c.xd;
-#define HAMX_LERP3(C)                                                          \
-  ({                                                                           \
-    const double i1 = a0.C * (1. - zd) + a1.C * zd;                            \
-    const double i2 = b0.C * (1. - zd) + b1.C * zd;                            \
-    const double j1 = c0.C * (1. - zd) + c1.C * zd;                            \
-    const double j2 = d0.C * (1. - zd) + d1.C * zd;                            \
-    const double w1 = i1 * (1. - yd) + i2 * yd;                                \
-    const double w2 = j1 * (1. - yd) + j2 * yd;                                \
-    w1 * (1. - xd) + w2 * xd;                                                  \
-  })
-  b.x = HAMX_LERP3(x);
-  b.y = HAMX_LERP3(y);
-  b.z = HAMX_LERP3(z);
-  te = HAMX_LERP3(w);
-#undef HAMX_LERP3
+  const double4 w1 = lerp_plane_packed(f, c.idx, sy, c.zd, c.yd);
+  const double4 w2 = lerp_plane_packed(f, c.idx + sx, sy, c.zd, c.yd);
+  const double xd = c.xd;
+  b.x = w1.x * (1. - xd) + w2.x * xd;
+  b.y = w1.y * (1. - xd) + w2.y * xd;
+  b.z = w1.z * (1. - xd) + w2.z * xd;
+  te = w1.w * (1. - xd) + w2.w * xd;
 }
 
 // ---------------------------------------------------------------- models --

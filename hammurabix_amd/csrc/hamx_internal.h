@@ -25,7 +25,7 @@ struct hamx_ctx {
   std::vector<hamx_kernel_time> times;
   // Grow-only device workspace, one buffer per slot.  Slots are owned by the
   // entry point that uses them; a call reuses the previous call's memory.
-  static constexpr int kSlots = 32;
+  static constexpr int kSlots = 48;
   void *ws[kSlots] = {};
   size_t ws_bytes[kSlots] = {};
   // Pinned host staging buffer for host-resident outputs.
@@ -64,7 +64,25 @@ enum Slot : int {
   kSlotReduce,
   kSlotFft,
   kSlotAux,
+  kSlotAdjCkpt,   // FD map before each shell
+  kSlotAdjIs,
+  kSlotAdjQs,
+  kSlotAdjUs,
+  kSlotAdjDm,
+  kSlotAdjFd,
+  kSlotAdjShell0, // five shell-map adjoints
+  kSlotAdjShell1,
+  kSlotAdjShell2,
+  kSlotAdjShell3,
+  kSlotAdjShell4,
+  kSlotAdjG,      // FD-map adjoint
+  kSlotAdjFdin,   // dL/d(fd cache) per shell pixel
+  kSlotAdjRayPar, // per-ray parameter partials
+  kSlotAdjPar,    // reduced parameter gradient
+  kSlotAdjGrid,   // packed grid gradient
+  kSlotCount
 };
+static_assert(kSlotCount <= hamx_ctx::kSlots, "workspace slots");
 
 inline int set_err(hamx_ctx *ctx, int code, const char *fmt, ...) {
   if (ctx) {

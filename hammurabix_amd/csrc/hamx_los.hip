@@ -38,139 +38,9 @@
 #include "hamx_fields.h"
 #include "hamx_healpix.h"
 #include "hamx_internal.h"
+#include "hamx_march.h"
 
 namespace hamx {
-
-// Constants of one write_grid pass (host-computed in the reference's
-// expression order).
-struct Phys {
-  Vec3 obs;
-  double gc_r_min, gc_r_max, gc_z_min, gc_z_max;
-  int do_dm, do_fd, do_sync;
-  int breg_model, tereg_model, cre_model;
-  double breg[8], tereg[4], cre[8];
-  double fd_fore;       // -(qe^3) / (2 pi mec2^2), integrator.cc:160-161
-  double lambda_sq;     // (c/nu)^2, integrator.cc:164-165
-  double i2bt;          // c^2 / (2 kB nu^2), integrator.cc:168-170
-  double A_num;         // 2 pi nu mec, integrator.cc:341-342
-  double three_qe;      // 3 qe
-  double qe3;           // qe^3
-  double fourpi_mec2;   // 4 pi mec2
-  double sixteenpi_mec2;// 16 pi mec2
-  double cre_nu;        // j0 * gamma0^(-idx(obs)) * unit, cre_ana.cc:221-229
-  double R0;            // observer cylindrical radius, cre_ana.cc:196-197
-  int idx_const;        // spectral index constant along rays
-  double idx, g_t1, g_t2, g_p1; // index and its Gamma factors when constant
-  double tc_em1, tc_den; // temp_convert: (e^p - 1) and p^2 e^p, integrator.cc:275-279
-};
-
-struct Grids {
-  const double4 *packed; // brnd (x,y,z) + ternd (w), one box
-  GridGeo g_packed;
-  const double *brnd_x, *brnd_y, *brnd_z;
-  GridGeo g_brnd;
-  const double *ternd;
-  GridGeo g_ternd;
-  const double *breg_x, *breg_y, *breg_z;
-  GridGeo g_breg;
-  const double *tereg;
-  GridGeo g_tereg;
-};
-
-struct ShellArgs {
-  Healpix hs;
-  uint64_t step;
-  double d_start, delta;
-  const double *fd_map; // accumulated FD of inner shells (NULL: zero cache)
-  Healpix hfd;
-  double *t_dm, *t_fd, *t_is, *t_qs, *t_us; // shell maps, nside_s
-};
-
-struct RayGeo {
-  double theta, phi;
-  Vec3 l;         // LOS versor
-  Vec3 eth, eph;  // spherical unit vectors for the IAU angle
-  double fd_inner;
-  uint64_t k_lo, k_hi;
-};
-
-// ---------------------------------------------------------------- per ray --
-
-__device__ inline void ray_interval(const Phys &ph, const ShellArgs &sa,
-                                    RayGeo &r) {
-  // Conservative d-interval satisfying |pos| <= gc_r_max and
-  // gc_z_min <= z <= gc_z_max, widened by 2 steps each side.
-  r.k_lo = 0;
-  r.k_hi = sa.step;
-  if (sa.step == 0)
-    return;
-  const double slack = 2.0 * sa.delta;
-  double lo = -1e300, hi = 1e300;
-  {
-    const double R = ph.gc_r_max * (1.0 + 1e-9) + slack;
-    const double b = r.l.x * ph.obs.x + r.l.y * ph.obs.y + r.l.z * ph.obs.z;
-    const double o2 = ph.obs.x * ph.obs.x + ph.obs.y * ph.obs.y + ph.obs.z * ph.obs.z;
-    const double disc = b * b - (o2 - R * R);
-    if (!(disc >= 0.)) {
-      r.k_hi = 0;
-      return;
-    }
-    const double sq = sqrt(disc);
-    lo = fmax(lo, -b - sq);
-    hi = fmin(hi, -b + sq);
-  }
-  {
-    const double lz = r.l.z;
-    const double zlo = ph.gc_z_min - 1e-9 * fabs(ph.gc_z_min) - slack;
-    const double zhi = ph.gc_z_max + 1e-9 * fabs(ph.gc_z_max) + slack;
-    if (fabs(lz) < 1e-12) {
-      if (ph.obs.z < zlo || ph.obs.z > zhi) {
-        r.k_hi = 0;
-        return;
-      }
-    } else {
-      double t1 = (zlo - ph.obs.z) / lz, t2 = (zhi - ph.obs.z) / lz;
-      if (t1 > t2) {
-        const double t = t1;
-        t1 = t2;
-        t2 = t;
-      }
-      lo = fmax(lo, t1 - slack);
-      hi = fmin(hi, t2 + slack);
-    }
-  }
-  if (!(hi >= lo)) {
-    r.k_hi = 0;
-    return;
-  }
-  // dist_k = d0 + (k + 1/2) delta  ->  k = (d - d0)/delta - 1/2
-  const double kf_lo = (lo - sa.d_start) / sa.delta - 0.5 - 2.0;
-  const double kf_hi = (hi - sa.d_start) / sa.delta - 0.5 + 3.0;
-  const double stepd = static_cast<double>(sa.step);
-  const double clo = fmin(fmax(floor(kf_lo), 0.), stepd);
-  const double chi = fmin(fmax(ceil(kf_hi), 0.), stepd);
-  r.k_lo = static_cast<uint64_t>(clo);
-  r.k_hi = static_cast<uint64_t>(chi);
-  if (r.k_hi < r.k_lo)
-    r.k_hi = r.k_lo;
-}
-
-__device__ inline void ray_setup(const Phys &ph, const ShellArgs &sa,
-                                 uint64_t ipix, RayGeo &r) {
-  hp_pix2ang(sa.hs, ipix, r.theta, r.phi);
-  double st, ct, sp, cp;
-  sincos(r.theta, &st, &ct);
-  sincos(r.phi, &sp, &cp);
-  // los_versor (integrator.cc:268-272)
-  r.l = Vec3{cp * st, sp * st, ct};
-  // sync_ipa unit vectors (integrator.cc:260-262)
-  r.eth = Vec3{ct * cp, ct * sp, -st};
-  r.eph = Vec3{-sp, cp, 0.};
-  r.fd_inner = 0.;
-  if (sa.fd_map)
-    r.fd_inner = hp_interpolate(sa.hfd, sa.fd_map, r.theta, r.phi);
-  ray_interval(ph, sa, r);
-}
 
 // ------------------------------------------------------------- per sample --
 
@@ -327,25 +197,6 @@ __device__ inline void store_ray(const Phys &ph, const ShellArgs &sa,
     sa.t_fd[ipix] = fd;
 }
 
-// ------------------------------------------------------------------ waves --
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1)
-    v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ inline double wave_incl_scan(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_up(v, o, 64);
-    if (lane >= o)
-      v += t;
-  }
-  return v;
-}
-
 template <bool PACKED>
 __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
                                                   ShellArgs sa) {
@@ -389,6 +240,63 @@ __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
   U = wave_sum(U);
   if (lane == 0)
     store_ray(ph, sa, ipix, dm, fd, I, Q, U);
+}
+
+// ---------------------------------------------------- specialised march --
+// The common model set -- WMAP-LSA regular B, uniform-sphere thermal
+// electrons, a CRE model with a constant spectral index, random grids (if
+// any) packed in one box, synchrotron on -- with the per-sample arithmetic
+// rewritten into fewer, cheaper operations that agree with the reference to
+// rounding:
+//   * the IAU angle enters only through cos/sin(2 ipa), formed from the two
+//     projections x = -e_phi.B, y = -e_theta.B as ((x^2 - y^2), 2xy)/(x^2+y^2)
+//     instead of atan2, so one sincos of the Faraday phase gives
+//     cos/sin(2 qui) by the angle-sum identity;
+//   * the emissivities fold every per-shell constant (Gamma factors, CGS
+//     products, delta, i2bt, the CRE norm) into one factor each, and when
+//     the exponent (index+1)/2 is -1 (alpha = 3) A^-1 is a product;
+//   * divisions by constants that feed no decision become products by
+//     reciprocals.  The galactocentric limits, the LSA radius window, the
+//     TE sphere test, the te > 0 clamp and the grid-cell lookup (divisions
+//     kept) decide exactly as in the reference.
+
+template <bool GRIDS, bool PW1>
+__global__ __launch_bounds__(256) void k_los_fast(Phys ph, FastConst fc, Grids gr,
+                                                  ShellArgs sa) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t ipix =
+      static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ipix >= sa.hs.npix)
+    return; // uniform per wavefront
+  RayGeo r;
+  ray_setup(ph, sa, ipix, r);
+  const double two_l2 = 2. * ph.lambda_sq;
+  double dm = 0., I = 0., Q = 0., U = 0., fd_carry = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    FastSample o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
+    const double fdk = valid ? o.fdk : 0.;
+    const double incl = wave_incl_scan(fdk, lane);
+    const double fd_here = fd_carry + incl;
+    fd_carry += __shfl(incl, 63, 64);
+    if (valid) {
+      double sph, cph;
+      sincos(two_l2 * (r.fd_inner + fd_here), &sph, &cph);
+      I += o.jt;
+      Q += (cph * o.c2i - sph * o.s2i) * o.jp;
+      U += (sph * o.c2i + cph * o.s2i) * o.jp;
+      dm += o.te;
+    }
+  }
+  dm = wave_sum(dm) * sa.delta;
+  I = wave_sum(I);
+  Q = wave_sum(Q);
+  U = wave_sum(U);
+  if (lane == 0)
+    store_ray(ph, sa, ipix, dm, fd_carry, I, Q, U);
 }
 
 template <bool PACKED>
@@ -476,13 +384,13 @@ __global__ void k_pack4(double4 *__restrict__ out, const double *__restrict__ bx
 
 // ------------------------------------------------------------------ host --
 
-static bool same_box(const hamx_box &a, const hamx_box &b) {
+bool same_box(const hamx_box &a, const hamx_box &b) {
   return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.x_min == b.x_min &&
          a.x_max == b.x_max && a.y_min == b.y_min && a.y_max == b.y_max &&
          a.z_min == b.z_min && a.z_max == b.z_max;
 }
 
-static int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what) {
+int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what) {
   if (b.nx < 2 || b.ny < 2 || b.nz < 2)
     return set_err(ctx, HAMX_ERR_ARG, "%s grid needs at least 2 vertices per axis", what);
   if (!(b.x_max > b.x_min) || !(b.y_max > b.y_min) || !(b.z_max > b.z_min))
@@ -490,11 +398,9 @@ static int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what) {
   return HAMX_OK;
 }
 
-static uint64_t box_size(const hamx_box &b) { return b.nx * b.ny * b.nz; }
-
 // Integrator::assemble_shell_ref (integrator.cc:231-244)
-static void shell_ref(const hamx_los_desc *d, uint32_t s, uint64_t &step,
-                      double &d_start, double &delta) {
+void shell_ref(const hamx_los_desc *d, uint32_t s, uint64_t &step, double &d_start,
+               double &delta) {
   d_start = d->radii_shell[s];
   const double d_stop = d->radii_shell[s + 1];
   delta = d->oc_r_res;
@@ -601,44 +507,42 @@ Phys make_phys(const hamx_los_desc *d) {
   return ph;
 }
 
-} // namespace hamx
-
-using namespace hamx;
-
-extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
-  if (!d || !samples)
-    return HAMX_ERR_ARG;
-  if (d->n_shell < 1 || d->n_shell > HAMX_MAX_SHELL || !(d->oc_r_res > 0.))
-    return HAMX_ERR_ARG;
-  uint64_t total = 0;
-  for (uint32_t s = 0; s < d->n_shell; ++s) {
-    uint64_t step;
-    double d0, delta;
-    shell_ref(d, s, step, d0, delta);
-    const uint64_t ns = d->nside_shell[s];
-    total += 12 * ns * ns * step;
+// Per-shell constants of the specialised march: the emissivity factors of
+// integrator.cc:333-346,402-415 with every sample-independent factor folded.
+FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool pw1) {
+  FastConst fc;
+  std::memset(&fc, 0, sizeof(fc));
+  fc.b0 = d->breg[0];
+  fc.psi0 = d->breg[1];
+  fc.psi1 = d->breg[2];
+  fc.chi0 = d->breg[3];
+  fc.inv_8kpc = 1.0 / (8. * cgs::kpc);
+  fc.inv_kpc = 1.0 / (1. * cgs::kpc);
+  fc.n0 = d->tereg[0];
+  fc.te_r0 = d->tereg[1];
+  fc.cre_unif = d->cre_model == HAMX_CRE_UNIF;
+  if (fc.cre_unif) {
+    fc.cre_r0 = d->cre[1];
+  } else {
+    fc.R0 = ph.R0;
+    fc.inv_cre_r0 = 1.0 / d->cre[3];
+    fc.inv_cre_z0 = 1.0 / d->cre[4];
   }
-  *samples = total;
-  return HAMX_OK;
+  fc.pw_e = 0.5 * (ph.idx + 1);
+  fc.a_coef = ph.A_num / ph.three_qe;
+  const double common = ph.cre_nu * 1.73205081 * ph.qe3 * delta * ph.i2bt;
+  fc.kt = common * ph.g_t1 * ph.g_t2 / (ph.fourpi_mec2 * (1. - ph.idx));
+  fc.kp = common * ph.g_p1 * ph.g_t2 / ph.sixteenpi_mec2;
+  if (pw1) { // A^-1 = |B_perp| / a_coef
+    fc.kt /= fc.a_coef;
+    fc.kp /= fc.a_coef;
+  }
+  fc.fd_scale = ph.fd_fore * delta;
+  return fc;
 }
 
-extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
-                                const hamx_fields *fields, hamx_maps *out,
-                                uint32_t flags) {
-  if (!ctx)
-    return HAMX_ERR_ARG;
-  if (!d || !out)
-    return set_err(ctx, HAMX_ERR_ARG, "null descriptor or output maps");
-  HAMX_TRY(validate_los(ctx, d));
-  HAMX_HIP(ctx, hipSetDevice(ctx->device));
-  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
-  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
-  hamx_fields none;
-  std::memset(&none, 0, sizeof(none));
-  const hamx_fields &f = fields ? *fields : none;
-
-  // ---- field grids
-  Grids gr;
+int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
+                 bool in_dev, Grids &gr, bool &packed, bool &grids_any) {
   std::memset(&gr, 0, sizeof(gr));
   const bool has_brnd = f.brnd_x || f.brnd_y || f.brnd_z;
   if (has_brnd && !(f.brnd_x && f.brnd_y && f.brnd_z))
@@ -692,7 +596,7 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
   }
   // Pack the random B components and TE density into one record per vertex
   // when they share a box: 64 B contiguous per z-pair of corners.
-  bool packed = false;
+  packed = false;
   if (has_brnd && (!f.ternd || same_box(f.brnd_box, f.ternd_box))) {
     const uint64_t n = box_size(f.brnd_box);
     void *pk;
@@ -708,6 +612,50 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
     gr.g_packed = gr.g_brnd;
     packed = true;
   }
+  grids_any = has_brnd || f.ternd;
+  return HAMX_OK;
+}
+
+} // namespace hamx
+
+using namespace hamx;
+
+extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
+  if (!d || !samples)
+    return HAMX_ERR_ARG;
+  if (d->n_shell < 1 || d->n_shell > HAMX_MAX_SHELL || !(d->oc_r_res > 0.))
+    return HAMX_ERR_ARG;
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    uint64_t step;
+    double d0, delta;
+    shell_ref(d, s, step, d0, delta);
+    const uint64_t ns = d->nside_shell[s];
+    total += 12 * ns * ns * step;
+  }
+  *samples = total;
+  return HAMX_OK;
+}
+
+namespace hamx {
+int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
+                     hamx_maps *out, uint32_t flags, double *ckpt) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !out)
+    return set_err(ctx, HAMX_ERR_ARG, "null descriptor or output maps");
+  HAMX_TRY(validate_los(ctx, d));
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  hamx_fields none;
+  std::memset(&none, 0, sizeof(none));
+  const hamx_fields &f = fields ? *fields : none;
+
+  // ---- field grids
+  Grids gr;
+  bool packed = false, grids_any = false;
+  HAMX_TRY(stage_fields(ctx, d, f, in_dev, gr, packed, grids_any));
 
   // ---- output maps (device)
   const Phys ph = make_phys(d);
@@ -765,6 +713,13 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
   }
 
   const int sched = d->schedule == HAMX_SCHED_THREAD ? HAMX_SCHED_THREAD : HAMX_SCHED_WAVE;
+  // the specialised kernel covers LSA + uniform TE + constant-index CRE with
+  // synchrotron on and the random grids (if any) packed in one box
+  const bool fast = d->schedule == HAMX_SCHED_AUTO && d->breg_model == HAMX_BREG_LSA &&
+                    d->tereg_model == HAMX_TEREG_UNIF && ph.idx_const && d->do_sync &&
+                    (d->cre_model == HAMX_CRE_ANA || d->cre_model == HAMX_CRE_UNIF) &&
+                    (packed || !grids_any);
+  const bool pw1 = fast && 0.5 * (ph.idx + 1) == -1.0;
   for (uint32_t s = 0; s < d->n_shell; ++s) {
     ShellArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -780,8 +735,26 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
     sa.t_qs = t_qs;
     sa.t_us = t_us;
     const uint64_t npix = sa.hs.npix;
+    if (ckpt && d->do_fd) // FD map before shell s, for the adjoint
+      HAMX_HIP(ctx, hipMemcpyAsync(ckpt + s * h_fd.npix, o_fd, h_fd.npix * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
     ProfMark pm = prof_begin(ctx);
-    if (sched == HAMX_SCHED_WAVE) {
+    if (fast) {
+      const FastConst fc = make_fast(d, ph, sa.delta, pw1);
+      const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
+      if (packed && pw1)
+        hipLaunchKernelGGL((k_los_fast<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,
+                           ph, fc, gr, sa);
+      else if (packed)
+        hipLaunchKernelGGL((k_los_fast<true, false>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, fc, gr, sa);
+      else if (pw1)
+        hipLaunchKernelGGL((k_los_fast<false, true>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, fc, gr, sa);
+      else
+        hipLaunchKernelGGL((k_los_fast<false, false>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, fc, gr, sa);
+    } else if (sched == HAMX_SCHED_WAVE) {
       const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
       if (packed)
         hipLaunchKernelGGL(k_los_wave<true>, dim3(blocks), dim3(256), 0, ctx->stream,
@@ -799,7 +772,8 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
                            ctx->stream, ph, gr, sa);
     }
     HAMX_HIP(ctx, hipGetLastError());
-    prof_end(ctx, pm, sched == HAMX_SCHED_WAVE ? "los_wave" : "los_thread", s,
+    prof_end(ctx, pm,
+             fast ? "los_fast" : (sched == HAMX_SCHED_WAVE ? "los_wave" : "los_thread"), s,
              npix * sa.step);
     // accumulate shell maps into the outputs (integrator.cc:122-132)
     auto acc = [&](AccPairs &pairs, const Healpix &hd) -> int {
@@ -867,4 +841,11 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
     HAMX_TRY(copy_out(out->fd, o_fd, h_fd));
   HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HAMX_OK;
+}
+} // namespace hamx
+
+extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
+                                const hamx_fields *fields, hamx_maps *out,
+                                uint32_t flags) {
+  return los_forward_impl(ctx, d, fields, out, flags, nullptr);
 }

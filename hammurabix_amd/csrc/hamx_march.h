@@ -1,0 +1,318 @@
+// Shared pieces of the LOS march (forward in hamx_los.hip, adjoint in
+// hamx_adj.hip): pass constants, grid and shell descriptors, per-ray set-up
+// with the conservative step interval, wavefront reductions, and the
+// host-side descriptor checks and constant folding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hamx.h"
+#include "hamx_cgs.h"
+#include "hamx_fields.h"
+#include "hamx_healpix.h"
+#include "hamx_internal.h"
+
+namespace hamx {
+
+// Constants of one write_grid pass (host-computed in the reference's
+// expression order).
+struct Phys {
+  Vec3 obs;
+  double gc_r_min, gc_r_max, gc_z_min, gc_z_max;
+  int do_dm, do_fd, do_sync;
+  int breg_model, tereg_model, cre_model;
+  double breg[8], tereg[4], cre[8];
+  double fd_fore;       // -(qe^3) / (2 pi mec2^2), integrator.cc:160-161
+  double lambda_sq;     // (c/nu)^2, integrator.cc:164-165
+  double i2bt;          // c^2 / (2 kB nu^2), integrator.cc:168-170
+  double A_num;         // 2 pi nu mec, integrator.cc:341-342
+  double three_qe;      // 3 qe
+  double qe3;           // qe^3
+  double fourpi_mec2;   // 4 pi mec2
+  double sixteenpi_mec2;// 16 pi mec2
+  double cre_nu;        // j0 * gamma0^(-idx(obs)) * unit, cre_ana.cc:221-229
+  double R0;            // observer cylindrical radius, cre_ana.cc:196-197
+  int idx_const;        // spectral index constant along rays
+  double idx, g_t1, g_t2, g_p1; // index and its Gamma factors when constant
+  double tc_em1, tc_den; // temp_convert: (e^p - 1) and p^2 e^p, integrator.cc:275-279
+};
+
+struct Grids {
+  const double4 *packed; // brnd (x,y,z) + ternd (w), one box
+  GridGeo g_packed;
+  const double *brnd_x, *brnd_y, *brnd_z;
+  GridGeo g_brnd;
+  const double *ternd;
+  GridGeo g_ternd;
+  const double *breg_x, *breg_y, *breg_z;
+  GridGeo g_breg;
+  const double *tereg;
+  GridGeo g_tereg;
+};
+
+struct ShellArgs {
+  Healpix hs;
+  uint64_t step;
+  double d_start, delta;
+  const double *fd_map; // accumulated FD of inner shells (NULL: zero cache)
+  Healpix hfd;
+  double *t_dm, *t_fd, *t_is, *t_qs, *t_us; // shell maps, nside_s
+};
+
+struct RayGeo {
+  double theta, phi;
+  Vec3 l;         // LOS versor
+  Vec3 eth, eph;  // spherical unit vectors for the IAU angle
+  double fd_inner;
+  uint64_t k_lo, k_hi;
+};
+
+// ---------------------------------------------------------------- per ray --
+
+__device__ inline void ray_interval(const Phys &ph, const ShellArgs &sa,
+                                    RayGeo &r) {
+  // Conservative d-interval satisfying |pos| <= gc_r_max and
+  // gc_z_min <= z <= gc_z_max, widened by 2 steps each side.
+  r.k_lo = 0;
+  r.k_hi = sa.step;
+  if (sa.step == 0)
+    return;
+  const double slack = 2.0 * sa.delta;
+  double lo = -1e300, hi = 1e300;
+  {
+    const double R = ph.gc_r_max * (1.0 + 1e-9) + slack;
+    const double b = r.l.x * ph.obs.x + r.l.y * ph.obs.y + r.l.z * ph.obs.z;
+    const double o2 = ph.obs.x * ph.obs.x + ph.obs.y * ph.obs.y + ph.obs.z * ph.obs.z;
+    const double disc = b * b - (o2 - R * R);
+    if (!(disc >= 0.)) {
+      r.k_hi = 0;
+      return;
+    }
+    const double sq = sqrt(disc);
+    lo = fmax(lo, -b - sq);
+    hi = fmin(hi, -b + sq);
+  }
+  {
+    const double lz = r.l.z;
+    const double zlo = ph.gc_z_min - 1e-9 * fabs(ph.gc_z_min) - slack;
+    const double zhi = ph.gc_z_max + 1e-9 * fabs(ph.gc_z_max) + slack;
+    if (fabs(lz) < 1e-12) {
+      if (ph.obs.z < zlo || ph.obs.z > zhi) {
+        r.k_hi = 0;
+        return;
+      }
+    } else {
+      double t1 = (zlo - ph.obs.z) / lz, t2 = (zhi - ph.obs.z) / lz;
+      if (t1 > t2) {
+        const double t = t1;
+        t1 = t2;
+        t2 = t;
+      }
+      lo = fmax(lo, t1 - slack);
+      hi = fmin(hi, t2 + slack);
+    }
+  }
+  if (!(hi >= lo)) {
+    r.k_hi = 0;
+    return;
+  }
+  // dist_k = d0 + (k + 1/2) delta  ->  k = (d - d0)/delta - 1/2
+  const double kf_lo = (lo - sa.d_start) / sa.delta - 0.5 - 2.0;
+  const double kf_hi = (hi - sa.d_start) / sa.delta - 0.5 + 3.0;
+  const double stepd = static_cast<double>(sa.step);
+  const double clo = fmin(fmax(floor(kf_lo), 0.), stepd);
+  const double chi = fmin(fmax(ceil(kf_hi), 0.), stepd);
+  r.k_lo = static_cast<uint64_t>(clo);
+  r.k_hi = static_cast<uint64_t>(chi);
+  if (r.k_hi < r.k_lo)
+    r.k_hi = r.k_lo;
+}
+
+__device__ inline void ray_setup(const Phys &ph, const ShellArgs &sa,
+                                 uint64_t ipix, RayGeo &r) {
+  hp_pix2ang(sa.hs, ipix, r.theta, r.phi);
+  double st, ct, sp, cp;
+  sincos(r.theta, &st, &ct);
+  sincos(r.phi, &sp, &cp);
+  // los_versor (integrator.cc:268-272)
+  r.l = Vec3{cp * st, sp * st, ct};
+  // sync_ipa unit vectors (integrator.cc:260-262)
+  r.eth = Vec3{ct * cp, ct * sp, -st};
+  r.eph = Vec3{-sp, cp, 0.};
+  r.fd_inner = 0.;
+  if (sa.fd_map)
+    r.fd_inner = hp_interpolate(sa.hfd, sa.fd_map, r.theta, r.phi);
+  ray_interval(ph, sa, r);
+}
+
+// ------------------------------------------------------------------ waves --
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+    v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline double wave_incl_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(v, o, 64);
+    if (lane >= o)
+      v += t;
+  }
+  return v;
+}
+
+// Per-shell constants of the specialised march (see make_fast).
+struct FastConst {
+  double psi0, psi1, chi0, b0;
+  double inv_8kpc, inv_kpc;
+  double n0, te_r0;
+  int cre_unif; // 0: analytic profile, 1: uniform sphere
+  double R0, inv_cre_r0, inv_cre_z0, cre_r0;
+  double kt, kp;    // folded emissivity factors (include delta and i2bt)
+  double pw_e;      // (index + 1) / 2
+  double a_coef;    // A = a_coef / |B_perp|
+  double fd_scale;  // fd_fore * delta
+};
+
+// ------------------------------------------------- specialised sample --
+// One sample of the specialised march (LSA + uniform TE + constant-index
+// CRE + packed grids), with every intermediate the adjoint needs.  The
+// forward kernel reads te, fdk, jt, jp, c2i, s2i; the rest is dead code
+// there.  Rewrites relative to the reference (each exact up to rounding):
+// cos/sin(2 ipa) from the two projections instead of atan2, folded
+// emissivity constants, A^-1 as a product when (index+1)/2 = -1, products by
+// reciprocals for divisions by constants that feed no decision.  The
+// galactocentric limits, the LSA window, the TE sphere test, the te > 0
+// clamp and the grid-cell lookup decide exactly as in the reference.
+struct FastSample {
+  Vec3 pos, B;
+  double rr;                 // cylindrical radius
+  bool lsa_in;               // 3 kpc <= rr <= 20 kpc
+  double cos_p, sin_p, sps, cps, sch, cch, lnr, tnh;
+  bool te_in;                // inside the uniform TE sphere
+  double te_raw, te;         // before / after the te > 0 clamp
+  bool cell_ok;              // inside the random-grid box
+  Cell cell;
+  double b_par, bper2, bper, prof;
+  double x, y, n2;           // IAU-angle projections
+  double fdk, jt, jp, c2i, s2i;
+};
+
+template <bool GRIDS, bool PW1>
+__device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grids &gr,
+                                 const ShellArgs &sa, const RayGeo &r, uint64_t k,
+                                 FastSample &o) {
+  const double dist = sa.d_start + (static_cast<double>(k) + 0.5) * sa.delta;
+  o.pos = Vec3{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
+  const Vec3 &pos = o.pos;
+  const double len = length0(pos);
+  if (len < ph.gc_r_min || len > ph.gc_r_max || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+    return false;
+  // WMAP-LSA (breg_lsa.cc:9-26)
+  Vec3 B{0., 0., 0.};
+  o.rr = sqrt(pos.x * pos.x + pos.y * pos.y);
+  o.lsa_in = !(o.rr > (20. * cgs::kpc) || o.rr < (3. * cgs::kpc));
+  if (o.lsa_in) {
+    const double inv_r = 1.0 / o.rr;
+    o.cos_p = pos.x * inv_r;
+    o.sin_p = pos.y * inv_r;
+    o.lnr = log(o.rr * fc.inv_8kpc);
+    o.tnh = tanh(pos.z * fc.inv_kpc);
+    const double psi = fc.psi0 + fc.psi1 * o.lnr;
+    const double chi = fc.chi0 * o.tnh;
+    sincos(psi, &o.sps, &o.cps);
+    sincos(chi, &o.sch, &o.cch);
+    const double bc = fc.b0 * o.cch;
+    B.x = bc * (o.sps * o.cos_p - o.cps * o.sin_p);
+    B.y = bc * (o.sps * o.sin_p + o.cps * o.cos_p);
+    B.z = fc.b0 * o.sch;
+  }
+  // uniform TE sphere (tereg_unif.cc:6-12)
+  double te = 0.;
+  {
+    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
+    o.te_in = !(length0(d) > fc.te_r0);
+    if (o.te_in)
+      te = fc.n0;
+  }
+  o.cell_ok = false;
+  if (GRIDS) {
+    if (grid_cell(gr.g_packed, pos, o.cell)) {
+      o.cell_ok = true;
+      Vec3 br;
+      double tr;
+      lerp_cell_packed(gr.packed, o.cell, gr.g_packed.nz, gr.g_packed.ny * gr.g_packed.nz,
+                       br, tr);
+      B.x += br.x;
+      B.y += br.y;
+      B.z += br.z;
+      te += tr;
+    }
+  }
+  o.te_raw = te;
+  te *= static_cast<double>(te > 0.);
+  o.te = te;
+  o.B = B;
+  o.b_par = dot0(B, r.l);
+  o.fdk = te * o.b_par * fc.fd_scale;
+  // perpendicular field and emissivities
+  const Vec3 cr{B.y * r.l.z - B.z * r.l.y, B.z * r.l.x - B.x * r.l.z,
+                B.x * r.l.y - B.y * r.l.x};
+  o.bper2 = cr.x * cr.x + cr.y * cr.y + cr.z * cr.z;
+  o.bper = sqrt(o.bper2);
+  if (fc.cre_unif) {
+    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
+    o.prof = (length0(d) > fc.cre_r0) ? 0. : 1.;
+  } else {
+    const double ch = cosh(pos.z * fc.inv_cre_z0);
+    o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) / (ch * ch);
+  }
+  double w;
+  if (PW1)
+    w = o.prof * o.bper2;
+  else
+    w = o.prof * o.bper * pow(fc.a_coef / o.bper, fc.pw_e);
+  o.jt = w * fc.kt;
+  o.jp = w * fc.kp;
+  // cos / sin of twice the IAU angle (integrator.cc:257-265)
+  o.x = -dot0(r.eph, B);
+  o.y = -dot0(r.eth, B);
+  o.n2 = o.x * o.x + o.y * o.y;
+  if (o.n2 > 0.) {
+    const double inv = 1.0 / o.n2;
+    o.c2i = (o.x * o.x - o.y * o.y) * inv;
+    o.s2i = 2. * o.x * o.y * inv;
+  } else {
+    o.c2i = 1.;
+    o.s2i = 0.;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------ host side --
+bool same_box(const hamx_box &a, const hamx_box &b);
+int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what);
+inline uint64_t box_size(const hamx_box &b) { return b.nx * b.ny * b.nz; }
+// Integrator::assemble_shell_ref (integrator.cc:231-244)
+void shell_ref(const hamx_los_desc *d, uint32_t s, uint64_t &step, double &d_start,
+               double &delta);
+int validate_los(hamx_ctx *ctx, const hamx_los_desc *d);
+Phys make_phys(const hamx_los_desc *d);
+FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool pw1);
+// Stage the field grids of a call on the device (uploading host inputs into
+// the workspace) and pack brnd + ternd into one double4 record per vertex
+// when they share a box.
+int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
+                 bool in_dev, Grids &gr, bool &packed, bool &grids_any);
+// One write_grid pass (hamx_los_forward); when ckpt is non-null and FD is
+// carried, the accumulated FD map before shell s is copied to
+// ckpt + s * npix(fd map) (device memory).
+int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
+                     hamx_maps *out, uint32_t flags, double *ckpt);
+
+} // namespace hamx

@@ -141,6 +141,37 @@ class Context:
         self._check(self.lib.hamx_los_forward(self.handle, C.byref(desc), C.byref(f),
                                               C.byref(m), flags))
 
+    def los_adjoint(self, desc: capi.HamxLosDesc, adj: dict[str, Any], dparams: Any,
+                    brnd: Optional[tuple[Any, Any, Any]] = None,
+                    brnd_box: Optional[capi.HamxBox] = None,
+                    ternd: Any = None, ternd_box: Optional[capi.HamxBox] = None,
+                    dbrnd: Optional[tuple[Any, Any, Any]] = None,
+                    dternd: Any = None) -> None:
+        """Reverse mode of los_forward: `adj` holds dL/d(maps) (keys dm, fd, is,
+        qs, us; missing maps are zero); writes dL/d(b0, psi0, psi1, chi0,
+        alpha, n0) into `dparams` and dL/d(grid) into `dbrnd` / `dternd`."""
+        f = capi.HamxFields()
+        inputs = [adj.get(k) for k in ("dm", "fd", "is", "qs", "us")]
+        if brnd is not None:
+            f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
+            f.brnd_box = brnd_box
+            inputs += list(brnd)
+        if ternd is not None:
+            f.ternd = _ptr(ternd)[0]
+            f.ternd_box = ternd_box
+            inputs.append(ternd)
+        m = capi.HamxMaps(*[_ptr(a)[0] for a in inputs[:5]])
+        outs = [dparams] + (list(dbrnd) if dbrnd is not None else []) + [dternd]
+        flags = 0
+        if _residency(inputs, "input"):
+            flags |= capi.HAMX_IN_DEVICE
+        if _residency(outs, "output"):
+            flags |= capi.HAMX_OUT_DEVICE
+        gx, gy, gz = (_ptr(a)[0] for a in dbrnd) if dbrnd is not None else (None,) * 3
+        self._check(self.lib.hamx_los_adjoint(self.handle, C.byref(desc), C.byref(f),
+                                              C.byref(m), _ptr(dparams)[0], gx, gy, gz,
+                                              _ptr(dternd)[0], flags))
+
     # -------------------------------------------------------- random fields
     def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
                 breg_grid: Optional[tuple[Any, Any, Any]] = None) -> None:

@@ -187,6 +187,28 @@ int hamx_memcpy_d2h(hamx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
                      const hamx_fields *fields, hamx_maps *out, uint32_t flags);
 
+/* Reverse mode of hamx_los_forward.  Given adj = dL/d(maps) for a scalar L
+ * of the pass's output maps (same sizes as the forward maps; NULL maps are
+ * zero), returns
+ *   dparams[HAMX_NPARAM]: dL/d(b0, psi0, psi1, chi0) of the LSA field,
+ *                         dL/d(alpha) of the CRE model, dL/d(n0) of the
+ *                         uniform thermal electrons (CGS units of Param);
+ *   dbrnd_x/y/z         : dL/d(random-B grid), grid layout (may be NULL);
+ *   dternd              : dL/d(random-TE grid) (may be NULL).
+ * Every output is overwritten.  The forward is recomputed with the
+ * accumulated FD map checkpointed before each shell; shells are then
+ * processed outermost first, each ray twice (total phase adjoint, then the
+ * per-sample gradients).  Supported models: LSA regular B, uniform TE,
+ * analytic or uniform CRE with a constant spectral index, synchrotron on,
+ * random grids (if any) sharing one box.  No counterpart in the reference,
+ * which has no gradients.  flags as for hamx_los_forward (inputs: fields
+ * and adj; outputs: dparams and the grid gradients). */
+#define HAMX_NPARAM 6
+int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
+                     const hamx_fields *fields, const hamx_maps *adj,
+                     double *dparams, double *dbrnd_x, double *dbrnd_y,
+                     double *dbrnd_z, double *dternd, uint32_t flags);
+
 /* Geometric LOS samples of one pass, S_geo = sum_s npix_s * step_s
  * (integrator.cc:62,173 trip counts); host-side arithmetic only. */
 int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples);

@@ -1,0 +1,215 @@
+"""GPU checks of the LOS adjoint (hamx_los_adjoint) against central finite
+differences of the GPU forward: the six model parameters, single grid
+vertices and random directions over the whole random-B and random-TE
+grids, with FD carried across shells.  L = sum over maps of w * map with
+random weights w, so dL/d(map) = w.
+
+Finite-difference error: with step h relative to the parameter, the
+truncation term is O(h^2) and the rounding term O(eps |L| / h); h = 1e-5
+keeps both below 1e-7 of the gradient on these workloads, and the check is
+rel 1e-6 (plus an absolute floor of 1e-9 of the gradient's scale for
+components that nearly vanish).
+"""
+import numpy as np
+import pytest
+
+from hammurabix_amd import HamxError, Param, _capi as capi, configs
+from hammurabix_amd.units import CGS
+from oracle_lib import npix
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+H_REL = 1e-5
+
+
+def out_maps(d):
+    m = {}
+    if d.do_dm:
+        m["dm"] = np.zeros(npix(d.nside_dm))
+    for k in ("is", "qs", "us"):
+        m[k] = np.zeros(npix(d.nside_sync))
+    if d.do_fd:
+        m["fd"] = np.zeros(npix(d.nside_fd))
+    return m
+
+
+def weights(d, seed):
+    rng = np.random.default_rng(seed)
+    w = {}
+    for k, v in out_maps(d).items():
+        w[k] = np.ascontiguousarray(rng.normal(size=v.shape))
+    return w
+
+
+def loss(ctx, d, w, grids):
+    m = out_maps(d)
+    ctx.los_forward(d, m, **grids)
+    return sum(float(np.dot(w[k], m[k])) for k in m)
+
+
+def scaled_weights(ctx, d, grids, seed):
+    """Random weights divided by each map's rms, so that every map
+    contributes to L at the same order."""
+    m = out_maps(d)
+    ctx.los_forward(d, m, **grids)
+    w = weights(d, seed)
+    for k in w:
+        s = float(np.sqrt(np.mean(m[k] ** 2)))
+        w[k] = w[k] / s if s > 0 else w[k]
+    return w
+
+
+def random_grids(shape, seed, te_scale=0.003):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    b = tuple(np.ascontiguousarray(rng.normal(0, 2e-6, n)) for _ in range(3))
+    te = np.ascontiguousarray(rng.normal(0.0, te_scale, n))
+    box = capi.HamxBox(*shape, -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc, 20 * CGS.kpc,
+                       -4 * CGS.kpc, 4 * CGS.kpc)
+    return b, te, box
+
+
+PARAMS = [("breg", 0), ("breg", 1), ("breg", 2), ("breg", 3), ("cre", 0), ("tereg", 0)]
+
+
+def get_param(d, i):
+    arr, j = PARAMS[i]
+    return getattr(d, arr)[j]
+
+
+def set_param(d, i, v):
+    arr, j = PARAMS[i]
+    getattr(d, arr)[j] = v
+
+
+def adjoint(ctx, d, w, grids, with_grid):
+    dp = np.zeros(capi.HAMX_NPARAM)
+    kw = {}
+    if with_grid:
+        n = grids["brnd"][0].shape[0]
+        kw["dbrnd"] = tuple(np.zeros(n) for _ in range(3))
+        kw["dternd"] = np.zeros(n)
+    ctx.los_adjoint(d, w, dp, **grids, **kw)
+    return dp, kw
+
+
+def check_params(ctx, d, w, grids, label):
+    dp, _ = adjoint(ctx, d, w, grids, with_grid=False)
+    scale = np.max(np.abs(dp * np.array([abs(get_param(d, i)) or 1.0 for i in range(6)])))
+    for i, name in enumerate(capi.PARAM_NAMES):
+        p0 = get_param(d, i)
+        h = H_REL * (abs(p0) if p0 != 0 else 1.0)
+        set_param(d, i, p0 + h)
+        lp = loss(ctx, d, w, grids)
+        set_param(d, i, p0 - h)
+        lm = loss(ctx, d, w, grids)
+        set_param(d, i, p0)
+        fd = (lp - lm) / (2 * h)
+        err = abs(dp[i] - fd)
+        floor = 1e-9 * scale / (abs(p0) or 1.0)
+        print(f"{label} d/d{name}: adjoint {dp[i]:.10e} fd {fd:.10e} "
+              f"rel {err / max(abs(fd), 1e-300):.2e}", flush=True)
+        assert err <= RTOL * abs(fd) + floor, f"{label} {name}: {dp[i]} vs {fd}"
+
+
+def multishell_case(seed=3):
+    par = Param(text=configs.random_forward(nside_sim=8, nside_out=32, oc_r_res=0.05,
+                                            grid=(32, 32, 16)))
+    d = par.los_desc()
+    b, te, box = random_grids((32, 32, 16), seed)
+    return d, dict(brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+
+
+def test_param_gradients_plain(ctx):
+    """One shell, analytic fields only, DM + FD + sync."""
+    print("case: adjoint params plain nside 16", flush=True)
+    d = Param(text=configs.plain_forward(nside_sim=16, nside_out=16, oc_r_res=0.02)).los_desc()
+    w = scaled_weights(ctx, d, {}, seed=1)
+    check_params(ctx, d, w, {}, "plain")
+
+
+def test_param_gradients_multishell_grids(ctx):
+    """Three shells (8/16/32), random grids, FD carried through the shells."""
+    print("case: adjoint params multishell grids", flush=True)
+    d, grids = multishell_case()
+    w = scaled_weights(ctx, d, grids, seed=2)
+    check_params(ctx, d, w, grids, "multishell")
+
+
+def test_param_gradients_mixed_nsides_cre_unif(ctx):
+    """Output maps at other nsides than the shells (transposed interpolation
+    in every accumulate) and the uniform CRE sphere; alpha != 3 takes the
+    pow() emissivity branch."""
+    print("case: adjoint params mixed nsides, CRE unif", flush=True)
+    d, grids = multishell_case(seed=4)
+    d.nside_dm, d.nside_fd, d.nside_sync = 16, 64, 8
+    d.cre_model = capi.HAMX_CRE_UNIF
+    d.cre[0:4] = [2.6, 6 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    w = scaled_weights(ctx, d, grids, seed=5)
+    check_params(ctx, d, w, grids, "mixed")
+
+
+def test_grid_gradients_directional(ctx):
+    """<dL/dgrid, v> against the central difference along a random v, for
+    the random-B grid and the random-TE grid, plus single vertices."""
+    print("case: adjoint grid directional", flush=True)
+    d, grids = multishell_case(seed=6)
+    w = scaled_weights(ctx, d, grids, seed=7)
+    _, g = adjoint(ctx, d, w, grids, with_grid=True)
+    rng = np.random.default_rng(8)
+    b = grids["brnd"]
+    te = grids["ternd"]
+    # random B direction
+    v = [rng.normal(size=a.shape) for a in b]
+    h = 1e-5 * 2e-6
+    plus = dict(grids, brnd=tuple(np.ascontiguousarray(a + h * va) for a, va in zip(b, v)))
+    minus = dict(grids, brnd=tuple(np.ascontiguousarray(a - h * va) for a, va in zip(b, v)))
+    fd = (loss(ctx, d, w, plus) - loss(ctx, d, w, minus)) / (2 * h)
+    adj = sum(float(np.dot(ga, va)) for ga, va in zip(g["dbrnd"], v))
+    print(f"brnd directional: adjoint {adj:.10e} fd {fd:.10e}", flush=True)
+    assert abs(adj - fd) <= 1e-6 * abs(fd)
+    # random TE direction (TE grid holds negative values: the te > 0 clamp)
+    vt = rng.normal(size=te.shape)
+    h = 1e-5 * 0.003
+    fd = (loss(ctx, d, w, dict(grids, ternd=np.ascontiguousarray(te + h * vt))) -
+          loss(ctx, d, w, dict(grids, ternd=np.ascontiguousarray(te - h * vt)))) / (2 * h)
+    adj = float(np.dot(g["dternd"], vt))
+    print(f"ternd directional: adjoint {adj:.10e} fd {fd:.10e}", flush=True)
+    assert abs(adj - fd) <= 1e-5 * abs(fd)
+    # the vertices with the largest B gradient, one at a time
+    gb = g["dbrnd"][0]
+    for i in np.argsort(-np.abs(gb))[:3]:
+        bp = [a.copy() for a in b]
+        bm = [a.copy() for a in b]
+        hh = 1e-4 * 2e-6
+        bp[0][i] += hh
+        bm[0][i] -= hh
+        fd = (loss(ctx, d, w, dict(grids, brnd=tuple(bp))) -
+              loss(ctx, d, w, dict(grids, brnd=tuple(bm)))) / (2 * hh)
+        print(f"vertex {i}: adjoint {gb[i]:.10e} fd {fd:.10e}", flush=True)
+        assert abs(gb[i] - fd) <= 1e-5 * abs(fd)
+
+
+def test_linearity_in_the_adjoint_maps(ctx):
+    """The gradient is linear in dL/d(maps): adj(w1 + 2 w2) = adj(w1) + 2 adj(w2)."""
+    d, grids = multishell_case(seed=9)
+    w1 = scaled_weights(ctx, d, grids, seed=10)
+    w2 = scaled_weights(ctx, d, grids, seed=11)
+    w3 = {k: np.ascontiguousarray(w1[k] + 2 * w2[k]) for k in w1}
+    r1, g1 = adjoint(ctx, d, w1, grids, True)
+    r2, g2 = adjoint(ctx, d, w2, grids, True)
+    r3, g3 = adjoint(ctx, d, w3, grids, True)
+    np.testing.assert_allclose(r3, r1 + 2 * r2, rtol=1e-9, atol=1e-12 * np.abs(r3).max())
+    for a, b_, c in zip(g1["dbrnd"], g2["dbrnd"], g3["dbrnd"]):
+        np.testing.assert_allclose(c, a + 2 * b_, rtol=1e-9, atol=1e-12 * np.abs(c).max())
+
+
+def test_adjoint_rejects_unsupported_models(ctx):
+    d = Param(text=configs.plain_forward(nside_sim=4, nside_out=4)).los_desc()
+    d.breg_model = capi.HAMX_BREG_CART
+    d.breg[0:3] = [1e-6, 0, 0]
+    w = weights(d, 0)
+    with pytest.raises(HamxError) as e:
+        ctx.los_adjoint(d, w, np.zeros(capi.HAMX_NPARAM))
+    assert e.value.code == capi.HAMX_ERR_UNSUPPORTED

@@ -50,7 +50,8 @@ def random_grids(shape, seed=0, te_scale=0.003):
     return b, te, box
 
 
-@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE])
+@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE,
+                                   capi.HAMX_SCHED_AUTO])
 def test_plain_forward_full_size(ctx, sched):
     """LSA + uniform TE + analytic CRE, one shell at nside 32, 3000 steps."""
     print("case: plain forward nside 32, schedule", sched, flush=True)
@@ -59,7 +60,8 @@ def test_plain_forward_full_size(ctx, sched):
     assert_parity(gpu_maps(ctx, d), write_grid(d), label=f"plain/{sched}")
 
 
-@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE])
+@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_THREAD, capi.HAMX_SCHED_WAVE,
+                                   capi.HAMX_SCHED_AUTO])
 @pytest.mark.parametrize("packed", [True, False])
 def test_random_grids_multishell(ctx, sched, packed):
     """3 shells (8/16/32), brnd + ternd grids, FD carried through the shells."""
