@@ -63,7 +63,7 @@ def traffic_from_profiles():
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_step:"):
+        if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 

@@ -169,14 +169,27 @@ def test_grid_gradients_directional(ctx):
     adj = sum(float(np.dot(ga, va)) for ga, va in zip(g["dbrnd"], v))
     print(f"brnd directional: adjoint {adj:.10e} fd {fd:.10e}", flush=True)
     assert abs(adj - fd) <= 1e-6 * abs(fd)
-    # random TE direction (TE grid holds negative values: the te > 0 clamp)
+    # random TE direction.  The te > 0 clamp makes L piecewise smooth in the
+    # TE grid: a central difference across a sample whose density crosses 0
+    # is off by up to half that sample's term, so the tight check runs on a
+    # positive TE grid (no sample at the clamp) and the grid with negative
+    # densities gets a loose one.
     vt = rng.normal(size=te.shape)
     h = 1e-5 * 0.003
+    te_pos = np.ascontiguousarray(np.abs(te) + 1e-4)
+    gp = dict(grids, ternd=te_pos)
+    wp = scaled_weights(ctx, d, gp, seed=7)
+    _, g_pos = adjoint(ctx, d, wp, gp, with_grid=True)
+    fd = (loss(ctx, d, wp, dict(gp, ternd=np.ascontiguousarray(te_pos + h * vt))) -
+          loss(ctx, d, wp, dict(gp, ternd=np.ascontiguousarray(te_pos - h * vt)))) / (2 * h)
+    adj = float(np.dot(g_pos["dternd"], vt))
+    print(f"ternd directional (positive grid): adjoint {adj:.10e} fd {fd:.10e}", flush=True)
+    assert abs(adj - fd) <= 1e-6 * abs(fd)
     fd = (loss(ctx, d, w, dict(grids, ternd=np.ascontiguousarray(te + h * vt))) -
           loss(ctx, d, w, dict(grids, ternd=np.ascontiguousarray(te - h * vt)))) / (2 * h)
     adj = float(np.dot(g["dternd"], vt))
-    print(f"ternd directional: adjoint {adj:.10e} fd {fd:.10e}", flush=True)
-    assert abs(adj - fd) <= 1e-5 * abs(fd)
+    print(f"ternd directional (clamped grid): adjoint {adj:.10e} fd {fd:.10e}", flush=True)
+    assert abs(adj - fd) <= 1e-3 * abs(fd)
     # the vertices with the largest B gradient, one at a time
     gb = g["dbrnd"][0]
     for i in np.argsort(-np.abs(gb))[:3]:
@@ -213,3 +226,75 @@ def test_adjoint_rejects_unsupported_models(ctx):
     with pytest.raises(HamxError) as e:
         ctx.los_adjoint(d, w, np.zeros(capi.HAMX_NPARAM))
     assert e.value.code == capi.HAMX_ERR_UNSUPPORTED
+
+
+def test_autograd_function_matches_c_abi_and_fd(ctx):
+    """torch.autograd over the adjoint: the gradient of a Gaussian
+    log-likelihood of Q/U and FD maps equals the C-ABI adjoint with
+    dL/dmap = -(map - data)/sigma^2, and a central difference in b0."""
+    import torch
+    from hammurabix_amd.autograd import desc_params, gaussian_loglike, los_maps
+    print("case: autograd likelihood", flush=True)
+    d, grids = multishell_case(seed=12)
+    dev = torch.device("cuda:0")
+    b = tuple(torch.from_numpy(a).to(dev).requires_grad_(True) for a in grids["brnd"])
+    te = torch.from_numpy(grids["ternd"]).to(dev).requires_grad_(True)
+    p = desc_params(d).requires_grad_(True)
+    kw = dict(brnd_box=grids["brnd_box"], ternd_box=grids["ternd_box"])
+    maps = los_maps(ctx, d, p, brnd=b, ternd=te, **kw)
+    rng = np.random.default_rng(13)
+    data, sigma = {}, {}
+    for k in ("qs", "us", "fd"):
+        m = maps[k].detach()
+        s = 0.1 * float(torch.sqrt(torch.mean(m * m)))
+        data[k] = m + s * torch.from_numpy(rng.normal(size=m.shape[0])).to(dev)
+        sigma[k] = s
+    ll = gaussian_loglike(maps, data, sigma)
+    ll.backward()
+    adj = {k: np.ascontiguousarray((-(maps[k].detach() - data[k]) / sigma[k] ** 2).cpu().numpy())
+           for k in data}
+    dp, g = adjoint(ctx, d, adj, grids, with_grid=True)
+    np.testing.assert_allclose(p.grad.numpy(), dp, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(b[0].grad.cpu().numpy(), g["dbrnd"][0], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(te.grad.cpu().numpy(), g["dternd"], rtol=1e-12, atol=0)
+    # central difference of the likelihood in b0
+    h = H_REL * float(p[0])
+
+    def ll_at(v):
+        q = p.detach().clone()
+        q[0] = v
+        with torch.no_grad():
+            mm = los_maps(ctx, d, q, brnd=tuple(x.detach() for x in b), ternd=te.detach(), **kw)
+            return float(gaussian_loglike(mm, data, sigma))
+    fd = (ll_at(float(p[0]) + h) - ll_at(float(p[0]) - h)) / (2 * h)
+    print(f"autograd d/db0: {float(p.grad[0]):.10e} fd {fd:.10e}", flush=True)
+    assert abs(float(p.grad[0]) - fd) <= 1e-5 * abs(fd)
+
+
+def test_param_gradients_vs_oracle_central_differences(ctx):
+    """The GPU adjoint against central differences of the FP64 CPU oracle's
+    maps (the CPU reference gradient), 3 shells with random grids."""
+    from oracle_lib import write_grid
+    print("case: adjoint vs oracle FD", flush=True)
+    par = Param(text=configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1,
+                                            grid=(16, 16, 8)))
+    d = par.los_desc()
+    b, te, box = random_grids((16, 16, 8), seed=14)
+    grids = dict(brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    w = scaled_weights(ctx, d, grids, seed=15)
+    dp, _ = adjoint(ctx, d, w, grids, with_grid=False)
+
+    def oracle_loss():
+        m = write_grid(d, **grids)
+        return sum(float(np.dot(w[k], m[k])) for k in w)
+    for i, name in enumerate(capi.PARAM_NAMES):
+        p0 = get_param(d, i)
+        h = H_REL * (abs(p0) if p0 != 0 else 1.0)
+        set_param(d, i, p0 + h)
+        lp = oracle_loss()
+        set_param(d, i, p0 - h)
+        lm = oracle_loss()
+        set_param(d, i, p0)
+        fd = (lp - lm) / (2 * h)
+        print(f"oracle d/d{name}: adjoint {dp[i]:.10e} fd {fd:.10e}", flush=True)
+        assert abs(dp[i] - fd) <= 1e-5 * abs(fd), name
