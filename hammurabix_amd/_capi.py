@@ -21,6 +21,8 @@ HAMX_ERR_NSIDE = -5
 
 HAMX_IN_DEVICE = 1
 HAMX_OUT_DEVICE = 2
+HAMX_KEEP_TAPE = 8
+HAMX_USE_TAPE = 16
 
 HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID = range(5)
 HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID = range(3)
@@ -166,3 +168,33 @@ HAMX_NPARAM = 6
 PARAM_NAMES = ("b0", "psi0", "psi1", "chi0", "alpha", "n0")
 _PROTOS["hamx_profile_read"] = (C.c_int, [C.c_void_p, C.POINTER(HamxKernelTime), C.c_int,
                                           C.POINTER(C.c_int)])
+
+# ---- batched LOS
+HAMX_MAX_FREQ = 4
+HAMX_PREC_F64 = 0
+HAMX_PREC_F32 = 1
+HAMX_BATCH_NO_TUNE = 4
+
+
+class HamxBatchInst(C.Structure):
+    _fields_ = [("los", HamxLosDesc), ("n_freq", C.c_uint32), ("reserved", C.c_uint32),
+                ("freq", C.c_double * HAMX_MAX_FREQ),
+                ("brnd_x", C.c_void_p), ("brnd_y", C.c_void_p), ("brnd_z", C.c_void_p),
+                ("ternd", C.c_void_p), ("box", HamxBox),
+                ("dm", C.c_void_p), ("fd", C.c_void_p),
+                ("is_", C.c_void_p * HAMX_MAX_FREQ), ("qs", C.c_void_p * HAMX_MAX_FREQ),
+                ("us", C.c_void_p * HAMX_MAX_FREQ)]
+
+
+class HamxBatchInfo(C.Structure):
+    _fields_ = [("samples_geo", C.c_uint64), ("samples_ref", C.c_uint64),
+                ("levels", C.c_uint32), ("variant", C.c_uint32), ("n_variants", C.c_uint32),
+                ("reserved", C.c_uint32), ("variant_ms", C.c_double * 16),
+                ("shape", C.c_char * 64)]
+
+
+_PROTOS["hamx_batch_create"] = (C.c_int, [C.c_void_p, C.POINTER(HamxBatchInst), C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)])
+_PROTOS["hamx_batch_run"] = (C.c_int, [C.c_void_p])
+_PROTOS["hamx_batch_get_info"] = (C.c_int, [C.c_void_p, C.POINTER(HamxBatchInfo)])
+_PROTOS["hamx_batch_destroy"] = (None, [C.c_void_p])

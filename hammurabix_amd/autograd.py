@@ -9,8 +9,9 @@ alpha of the CRE model, n0 of the uniform thermal electrons, in the CGS /
 radian units of the descriptor) and the random-grid tensors' ``.grad``.
 
 The forward and the adjoint run on the library's stream and are synchronous
-with respect to the caller; the autograd graph holds the inputs only (the
-adjoint recomputes the forward with FD-map checkpoints).
+with respect to the caller.  The forward keeps its tape (FD checkpoints) in
+the context and the backward uses it; if another pass has replaced the tape
+in between, the backward re-runs the forward to record it again.
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ from typing import Any, Optional
 import torch
 
 from . import _capi as capi
-from .runtime import Context
+from .runtime import Context, HamxError
 
 MAP_KEYS = ("dm", "fd", "is", "qs", "us")
 # descriptor slots of the differentiable parameters, in HAMX_NPARAM order
@@ -63,7 +64,7 @@ class _LosFunction(torch.autograd.Function):
         maps = {k: torch.empty(sizes[k], dtype=torch.float64, device=dev) for k in keys}
         brnd = (bx, by, bz) if bx is not None else None
         ctx.los_forward(d, maps, brnd=brnd, brnd_box=brnd_box, ternd=te,
-                        ternd_box=ternd_box)
+                        ternd_box=ternd_box, keep_tape=True)
         fctx.ctx, fctx.desc, fctx.keys = ctx, d, keys
         fctx.brnd_box, fctx.ternd_box = brnd_box, ternd_box
         fctx.save_for_backward(params, bx, by, bz, te)
@@ -88,8 +89,14 @@ class _LosFunction(torch.autograd.Function):
         # the TE gradient lives on the B grid's box (one packed record per vertex)
         dternd = torch.zeros(n, dtype=torch.float64, device=dev) if want_t else None
         brnd = (bx, by, bz) if bx is not None else None
-        fctx.ctx.los_adjoint(d, adj, dparams, brnd=brnd, brnd_box=fctx.brnd_box, ternd=te,
-                             ternd_box=fctx.ternd_box, dbrnd=dbrnd, dternd=dternd)
+        kw = dict(brnd=brnd, brnd_box=fctx.brnd_box, ternd=te, ternd_box=fctx.ternd_box,
+                  dbrnd=dbrnd, dternd=dternd)
+        try:  # the tape of this forward, unless another pass has replaced it
+            fctx.ctx.los_adjoint(d, adj, dparams, use_tape=True, **kw)
+        except HamxError as e:
+            if e.code != capi.HAMX_ERR_ARG:
+                raise
+            fctx.ctx.los_adjoint(d, adj, dparams, **kw)
         gp = dparams.to(params.device, params.dtype) if fctx.needs_input_grad[0] else None
         gb = dbrnd if dbrnd is not None else (None, None, None)
         return (gp, gb[0], gb[1], gb[2], dternd, None, None, None, None, None)

@@ -62,6 +62,7 @@ struct AdjConst {
 
 struct AdjShell {
   const double *gi, *gq, *gu, *gdm, *gfd; // shell-map adjoints (nside_s)
+  const double *fd_tot;                   // FD of each ray over this shell (forward tape)
   double *gfdin;                          // out: dL/d(fd_inner) per pixel
   double *ray_par;                        // out: 8 parameter partials per ray
   double *ggrid;                          // packed {x,y,z,te} grid gradient
@@ -93,39 +94,25 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
   const double gdm = as.gdm ? as.gdm[ipix] : 0.;
   const double gfd = as.gfd ? as.gfd[ipix] : 0.;
 
-  // sweep 1: T = sum_k dL/dqui_k
-  double carry = 0., tpart = 0.;
-  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
-    const uint64_t k = base + lane;
-    FastSample o;
-    bool valid = false;
-    if (k < r.k_hi)
-      valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
-    const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
-    const double fd_here = carry + incl;
-    carry += __shfl(incl, 63, 64);
-    if (valid) {
-      double sph, cph;
-      sincos(ac.two_l2 * (r.fd_inner + fd_here), &sph, &cph);
-      const double c2q = cph * o.c2i - sph * o.s2i, s2q = sph * o.c2i + cph * o.s2i;
-      tpart += 2. * o.jp * (-gq * s2q + gu * c2q);
-    }
-  }
-  const double T = wave_sum(tpart);
-
-  // sweep 2: per-sample gradients
+  // One sweep from the far end of the ray to the observer.  The running FD
+  // at sample k is the ray's total (from the forward's tape) minus the
+  // samples beyond k, and the suffix sum of dL/dqui is carried the same way,
+  // so every sample gets its full gradient in one evaluation.
+  const uint64_t n = r.k_hi - r.k_lo;
+  const double fd_total = as.fd_tot[ipix];
   double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0.;
-  double dq_carry = 0.;
-  carry = 0.;
-  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
-    const uint64_t k = base + lane;
+  double after = 0.;    // FD of the samples beyond the current chunk
+  double dq_after = 0.; // sum of dL/dqui beyond the current chunk
+  for (uint64_t c = (n + 63) / 64; c-- > 0;) {
+    const uint64_t k = r.k_lo + 64 * c + lane;
     FastSample o;
     bool valid = false;
     if (k < r.k_hi)
       valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
-    const double fd_here = carry + incl;
-    carry += __shfl(incl, 63, 64);
+    const double chunk = __shfl(incl, 63, 64);
+    const double fd_here = (fd_total - after - chunk) + incl;
+    after += chunk;
     double c2q = 1., s2q = 0., dq = 0.;
     if (valid) {
       double sph, cph;
@@ -134,12 +121,12 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
       s2q = sph * o.c2i + cph * o.s2i;
       dq = 2. * o.jp * (-gq * s2q + gu * c2q);
     }
-    const double dq_incl = wave_incl_scan(dq, lane);
-    const double excl = dq_carry + (dq_incl - dq);
-    dq_carry += __shfl(dq_incl, 63, 64);
+    const double dq_scan = wave_suffix_scan(dq, lane);
+    const double dq_suffix = dq_scan + dq_after; // sum over samples k' >= k
+    dq_after += __shfl(dq_scan, 0, 64);
     if (!valid)
       continue;
-    const double R = ac.l2 * (T - excl) + gfd; // dL/df_k
+    const double R = ac.l2 * dq_suffix + gfd; // dL/df_k
     const double djp = gq * c2q + gu * s2q;
     const double djt = gi;
     const double dte = (o.te_raw > 0.) ? (R * o.b_par * fc.fd_scale + gdm * ac.delta) : 0.;
@@ -201,7 +188,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     rp[4] = p4;
     rp[5] = p5;
     if (as.gfdin)
-      as.gfdin[ipix] = ac.l2 * T;
+      as.gfdin[ipix] = ac.l2 * dq_after;
   }
 }
 
@@ -288,6 +275,69 @@ __global__ void k_unpack4(const double *__restrict__ g, double *__restrict__ gx,
   }
 }
 
+bool same_box_bits(const hamx_box &a, const hamx_box &b) {
+  return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.x_min == b.x_min &&
+         a.x_max == b.x_max && a.y_min == b.y_min && a.y_max == b.y_max &&
+         a.z_min == b.z_min && a.z_max == b.z_max;
+}
+
+// Field-by-field equality of two pass descriptors (the march schedule aside).
+bool same_desc(const hamx_los_desc &a, const hamx_los_desc &b) {
+  bool eq = a.n_shell == b.n_shell && a.oc_r_res == b.oc_r_res && a.gc_r_min == b.gc_r_min &&
+            a.gc_r_max == b.gc_r_max && a.gc_z_min == b.gc_z_min &&
+            a.gc_z_max == b.gc_z_max && a.do_dm == b.do_dm && a.do_fd == b.do_fd &&
+            a.do_sync == b.do_sync && a.nside_dm == b.nside_dm && a.nside_fd == b.nside_fd &&
+            a.nside_sync == b.nside_sync && a.sync_freq == b.sync_freq &&
+            a.breg_model == b.breg_model && a.tereg_model == b.tereg_model &&
+            a.cre_model == b.cre_model;
+  for (int i = 0; i < 3; ++i)
+    eq = eq && a.observer[i] == b.observer[i];
+  for (int i = 0; i < HAMX_MAX_SHELL; ++i)
+    eq = eq && a.nside_shell[i] == b.nside_shell[i];
+  for (int i = 0; i <= HAMX_MAX_SHELL; ++i)
+    eq = eq && a.radii_shell[i] == b.radii_shell[i];
+  for (int i = 0; i < 8; ++i)
+    eq = eq && a.breg[i] == b.breg[i] && a.cre[i] == b.cre[i];
+  for (int i = 0; i < 4; ++i)
+    eq = eq && a.tereg[i] == b.tereg[i];
+  return eq;
+}
+
+bool same_fields(const hamx_fields &a, const hamx_fields &b) {
+  return a.brnd_x == b.brnd_x && a.brnd_y == b.brnd_y && a.brnd_z == b.brnd_z &&
+         a.ternd == b.ternd && a.breg_x == b.breg_x && a.breg_y == b.breg_y &&
+         a.breg_z == b.breg_z && a.tereg == b.tereg && same_box_bits(a.brnd_box, b.brnd_box) &&
+         same_box_bits(a.ternd_box, b.ternd_box) && same_box_bits(a.breg_box, b.breg_box) &&
+         same_box_bits(a.tereg_box, b.tereg_box);
+}
+
+int los_forward_taped(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
+                      hamx_maps *out, uint32_t flags) {
+  HAMX_TRY(validate_los(ctx, d));
+  ctx->tape_valid = false;
+  const bool fd_map = d->do_fd || d->do_sync;
+  const Healpix h_fd = healpix_make(d->do_fd ? d->nside_fd : d->nside_sync);
+  void *q;
+  double *ckpt = nullptr, *shell_fd = nullptr;
+  if (d->do_fd) {
+    HAMX_TRY(ws_get(ctx, kSlotAdjCkpt, d->n_shell * h_fd.npix * sizeof(double), &q));
+    ckpt = static_cast<double *>(q);
+  }
+  if (fd_map) {
+    HAMX_TRY(ws_get(ctx, kSlotTapeFd, shell_offset(d, d->n_shell) * sizeof(double), &q));
+    shell_fd = static_cast<double *>(q);
+  }
+  HAMX_TRY(los_forward_impl(ctx, d, fields, out, flags & ~HAMX_KEEP_TAPE, ckpt, shell_fd));
+  if (fd_map) {
+    ctx->tape_valid = true;
+    ctx->tape_desc = *d;
+    std::memset(&ctx->tape_fields, 0, sizeof(ctx->tape_fields));
+    if (fields)
+      ctx->tape_fields = *fields;
+  }
+  return HAMX_OK;
+}
+
 static unsigned blocks_for(uint64_t n, unsigned per = 256) {
   return static_cast<unsigned>((n + per - 1) / per);
 }
@@ -320,20 +370,25 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
                    "hamx_los_adjoint supports LSA B, uniform TE and a constant-index CRE "
                    "with synchrotron output");
 
-  // ---- forward with FD-map checkpoints
+  // ---- the forward's tape
   const uint64_t nside_fd_map = d->do_fd ? d->nside_fd : d->nside_sync;
   const Healpix h_fd = healpix_make(nside_fd_map);
   const Healpix h_sync = healpix_make(d->nside_sync);
   const Healpix h_dm = healpix_make(d->do_dm ? d->nside_dm : 1);
   void *q;
-  double *ckpt = nullptr;
-  if (d->do_fd) {
-    HAMX_TRY(ws_get(ctx, kSlotAdjCkpt, d->n_shell * h_fd.npix * sizeof(double), &q));
-    ckpt = static_cast<double *>(q);
+  if (flags & HAMX_USE_TAPE) {
+    if (!ctx->tape_valid || !same_desc(ctx->tape_desc, *d) ||
+        !same_fields(ctx->tape_fields, f))
+      return set_err(ctx, HAMX_ERR_ARG,
+                     "hamx_los_adjoint: no tape of a forward pass with this descriptor and "
+                     "these field arrays (run hamx_los_forward with HAMX_KEEP_TAPE)");
+  } else {
     hamx_maps none_maps;
     std::memset(&none_maps, 0, sizeof(none_maps));
-    HAMX_TRY(los_forward_impl(ctx, d, fields, &none_maps, flags & HAMX_IN_DEVICE, ckpt));
+    HAMX_TRY(los_forward_taped(ctx, d, fields, &none_maps, flags & HAMX_IN_DEVICE));
   }
+  double *ckpt = static_cast<double *>(ctx->ws[kSlotAdjCkpt]);
+  const double *tape_fd = static_cast<const double *>(ctx->ws[kSlotTapeFd]);
 
   // ---- grids and adjoint inputs on the device
   Grids gr;
@@ -469,6 +524,7 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
     as.gdm = d->do_dm ? sg[3] : nullptr;
     as.gfd = d->do_fd ? sg[4] : nullptr;
     as.gfdin = has_fdin ? gfdin : nullptr;
+    as.fd_tot = tape_fd + shell_offset(d, static_cast<uint32_t>(s));
     as.ray_par = ray_par;
     as.ggrid = ggrid;
     as.sy = gr.g_packed.nz;

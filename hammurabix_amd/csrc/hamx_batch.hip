@@ -1,0 +1,1140 @@
+// Batched LOS integration: many instances (realisations x parameter sets x
+// frequency lists) per replay of one captured HIP graph.
+//
+// Per instance the result is Pipeline::assemble_obs (pipeline.cc:165-190)
+// over its frequency list: the last-listed frequency's pass first, with DM,
+// FD and the FD cache of the inner shells (integrator.cc:75-86), then one
+// pass per remaining frequency without DM / FD and with a zero FD cache.
+// All passes of an instance share one march: position, limit tests, fields,
+// FD and the emissivity shape are evaluated once per sample; per frequency
+// only the polarisation phase (a function of the running FD) and constant
+// factors differ.
+//
+// Work is organised by shell level: level s holds shell s of every instance
+// that has one.  A level is one march launch over "items" (a few consecutive
+// rays of one instance, one 64-lane wavefront per ray, lanes = consecutive
+// samples as in k_los_fast), followed by one accumulate launch that adds the
+// shell maps of all instances into their outputs (and so into the FD cache
+// read by the next level).  The graph is: zero outputs and tickets, pack the
+// grids, then march / accumulate per level.
+//
+// Launch shapes (chosen by timing every candidate on the batch itself):
+//   one-shot    one wave per item, the hardware dispatcher balances;
+//               optionally XCD-aware (workgroups numbered so that each of the
+//               8 XCDs, which take workgroups round-robin, gets a contiguous
+//               run of items and so a compact region of sky and grid in its
+//               own L2);
+//   persistent  as many workgroups as fit on the device; each wave takes
+//               tickets from an atomic counter; optionally one queue per XCD
+//               (a contiguous, cost-balanced slice of the items) with
+//               stealing from the other queues when its own is empty.
+//
+// Precision HAMX_PREC_F32 evaluates the fields and the emissivity shape in
+// FP32 in kpc / muG units on FP32 grids, with the hardware sin/cos/exp/log;
+// positions, the galactocentric limit tests, the grid-cell lookup and every
+// in/out decision stay FP64 exactly as in the plain path, and so do the FD
+// scan, the polarisation phase (reduced to one revolution in FP64 before the
+// FP32 sine / cosine) and the cross-lane map sums.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hamx_march.h"
+
+namespace hamx {
+namespace {
+
+constexpr int kMaxF = HAMX_MAX_FREQ;
+constexpr int kQueues = 8;   // one ticket queue per XCD
+constexpr int kAccMaps = 12; // 3 Stokes maps x 4 frequencies
+
+struct BShell {
+  Healpix hs;
+  uint64_t step;
+  double d_start, delta;
+  double fdu; // FP32 mode: te * B_par[muG] -> FD contribution, fd_fore * 1e-6 * delta
+};
+
+// FP32 field constants in kpc / muG units.
+struct F32Const {
+  float psi0, psi1, chi0, b0u;
+  float n0;
+  float R0k, inv_r0k, inv_z0k;
+  float hexp; // (1 - pw_e) / 2: the emissivity shape is prof * (B_perp^2)^hexp
+};
+
+struct BInst {
+  Phys ph;                      // pass-0 physics (the last-listed frequency)
+  FastConst fc[HAMX_MAX_SHELL]; // pass-0 specialised constants per shell
+  BShell sh[HAMX_MAX_SHELL];
+  F32Const f;
+  int n_shell, n_freq, grids, pw1;
+  int do_dm, do_fd, pad0, pad1;
+  // per pass (index 0 = first pass)
+  double two_l2[kMaxF]; // FP64 phase: 2 lambda^2
+  double rev[kMaxF];    // FP32 mode phase in revolutions per unit FD: lambda^2 / pi
+  int fdin[kMaxF];      // the pass reads the FD cache
+  double st[kMaxF];     // map scale of the I sum (temp_convert included)
+  double sp[kMaxF];     // map scale of the Q / U sums
+  const void *packed;   // double4 (F64) or float4 (F32) {bx, by, bz, te} records
+  GridGeo geo;
+  Healpix hfd;
+  const double *fd_cache; // accumulated FD map of the inner shells
+  double *t_dm, *t_fd;
+  double *t_i[kMaxF], *t_q[kMaxF], *t_u[kMaxF];
+};
+
+struct LevelArgs {
+  const BInst *inst;
+  const uint32_t *ids;    // instances of this level, in item order
+  const uint64_t *prefix; // [n_ids + 1] first item of each instance
+  unsigned long long *ctr; // [kQueues] tickets, zero before the launch
+  uint64_t n_items;
+  uint64_t qbeg[kQueues], qend[kQueues];
+  uint32_t n_ids, level, rays, nq;
+  int persistent, xcd;
+};
+
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ inline ShellArgs shell_args(const BInst &in, int s) {
+  ShellArgs sa;
+  sa.hs = in.sh[s].hs;
+  sa.step = in.sh[s].step;
+  sa.d_start = in.sh[s].d_start;
+  sa.delta = in.sh[s].delta;
+  sa.fd_map = (in.do_fd && s > 0) ? in.fd_cache : nullptr;
+  sa.hfd = in.hfd;
+  sa.t_dm = sa.t_fd = sa.t_is = sa.t_qs = sa.t_us = nullptr;
+  return sa;
+}
+
+// Per-ray set-up (pointing, LOS versor, FD cache, culled step range) as an
+// out-of-line call: it runs once per ray, and keeping its FP64 sincos and
+// HEALPix integer arithmetic out of the march body leaves the body's
+// registers to the sample loop.
+__device__ __noinline__ void ray_setup_call(const Phys &ph, const ShellArgs &sa, uint64_t ipix,
+                                            RayGeo &r) {
+  ray_setup(ph, sa, ipix, r);
+}
+
+// ------------------------------------------------------------- FP64 ray --
+
+template <bool GRIDS, bool PW1>
+__device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
+  const ShellArgs sa = shell_args(in, s);
+  Grids gr;
+  gr.packed = static_cast<const double4 *>(in.packed);
+  gr.g_packed = in.geo;
+  RayGeo r;
+  ray_setup_call(in.ph, sa, ipix, r);
+  const FastConst &fc = in.fc[s];
+  const int nf = in.n_freq;
+  double dm = 0., I = 0., Q[kMaxF] = {0., 0., 0., 0.}, U[kMaxF] = {0., 0., 0., 0.};
+  double carry = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    FastSample o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_fast<GRIDS, PW1>(in.ph, fc, gr, sa, r, k, o);
+    const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
+    const double fd_here = carry + incl;
+    carry += __shfl(incl, 63, 64);
+    if (valid) {
+      dm += o.te;
+      I += o.jt;
+#pragma unroll
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+          double sph, cph;
+          sincos(in.two_l2[f] * X, &sph, &cph);
+          Q[f] += (cph * o.c2i - sph * o.s2i) * o.jp;
+          U[f] += (sph * o.c2i + cph * o.s2i) * o.jp;
+        }
+      }
+    }
+  }
+  dm = wave_sum(dm);
+  I = wave_sum(I);
+#pragma unroll
+  for (int f = 0; f < kMaxF; ++f) {
+    if (f < nf) {
+      Q[f] = wave_sum(Q[f]);
+      U[f] = wave_sum(U[f]);
+    }
+  }
+  if (lane == 0) {
+    if (in.do_dm)
+      in.t_dm[ipix] = dm * sa.delta;
+    if (in.do_fd)
+      in.t_fd[ipix] = carry;
+#pragma unroll
+    for (int f = 0; f < kMaxF; ++f) {
+      if (f < nf) {
+        in.t_i[f][ipix] = I * in.st[f];
+        in.t_q[f][ipix] = Q[f] * in.sp[f];
+        in.t_u[f][ipix] = U[f] * in.sp[f];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- FP32 ray --
+
+struct S32 {
+  float te, fdk, w, c2i, s2i;
+};
+
+__device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, uint64_t sy,
+                                     float zd, float yd, float4 &w) {
+  const float4 a0 = f[i0], a1 = f[i0 + 1];
+  const float4 b0 = f[i0 + sy], b1 = f[i0 + sy + 1];
+  const float z1 = 1.f - zd, y1 = 1.f - yd;
+  w.x = (a0.x * z1 + a1.x * zd) * y1 + (b0.x * z1 + b1.x * zd) * yd;
+  w.y = (a0.y * z1 + a1.y * zd) * y1 + (b0.y * z1 + b1.y * zd) * yd;
+  w.z = (a0.z * z1 + a1.z * zd) * y1 + (b0.z * z1 + b1.z * zd) * yd;
+  w.w = (a0.w * z1 + a1.w * zd) * y1 + (b0.w * z1 + b1.w * zd) * yd;
+}
+
+template <bool GRIDS, bool PW1>
+__device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo &r,
+                                const float3 &lf, const float3 &ethf, const float3 &ephf,
+                                uint64_t k, S32 &o) {
+  const Phys &ph = in.ph;
+  const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
+  const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
+  const double len = length0(pos);
+  if (len < ph.gc_r_min || len > ph.gc_r_max || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+    return false;
+  const double inv_kpc = 1.0 / cgs::kpc;
+  const double rr = sqrt(pos.x * pos.x + pos.y * pos.y);
+  const float rk = static_cast<float>(rr * inv_kpc);
+  const float zk = static_cast<float>(pos.z * inv_kpc);
+  const F32Const &c = in.f;
+  float bx = 0.f, by = 0.f, bz = 0.f;
+  // WMAP-LSA window decided in FP64 (breg_lsa.cc:9-26)
+  if (!(rr > (20. * cgs::kpc) || rr < (3. * cgs::kpc))) {
+    const float xk = static_cast<float>(pos.x * inv_kpc);
+    const float yk = static_cast<float>(pos.y * inv_kpc);
+    const float inv_r = 1.f / rk;
+    const float cos_p = xk * inv_r, sin_p = yk * inv_r;
+    const float lnr = __logf(rk * 0.125f);
+    const float e2 = __expf(-2.f * fabsf(zk));
+    const float tnh = copysignf((1.f - e2) / (1.f + e2), zk);
+    const float psi = c.psi0 + c.psi1 * lnr;
+    const float chi = c.chi0 * tnh;
+    const float sps = __sinf(psi), cps = __cosf(psi);
+    const float sch = __sinf(chi), cch = __cosf(chi);
+    const float bc = c.b0u * cch;
+    bx = bc * (sps * cos_p - cps * sin_p);
+    by = bc * (sps * sin_p + cps * cos_p);
+    bz = c.b0u * sch;
+  }
+  float te = 0.f;
+  {
+    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
+    if (!(length0(d) > ph.tereg[1]))
+      te = c.n0;
+  }
+  if (GRIDS) {
+    Cell cell;
+    if (grid_cell(in.geo, pos, cell)) {
+      const float4 *g = static_cast<const float4 *>(in.packed);
+      const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
+      float4 w1, w2;
+      const float zd = static_cast<float>(cell.zd), yd = static_cast<float>(cell.yd);
+      lerp_plane_f4(g, cell.idx, sy, zd, yd, w1);
+      lerp_plane_f4(g, cell.idx + sx, sy, zd, yd, w2);
+      const float xd = static_cast<float>(cell.xd), x1 = 1.f - xd;
+      bx += w1.x * x1 + w2.x * xd;
+      by += w1.y * x1 + w2.y * xd;
+      bz += w1.z * x1 + w2.z * xd;
+      te += w1.w * x1 + w2.w * xd;
+    }
+  }
+  te = te > 0.f ? te : 0.f;
+  o.te = te;
+  const float b_par = bx * lf.x + by * lf.y + bz * lf.z;
+  o.fdk = te * b_par;
+  const float cx = by * lf.z - bz * lf.y, cy = bz * lf.x - bx * lf.z,
+              cz = bx * lf.y - by * lf.x;
+  const float bper2 = cx * cx + cy * cy + cz * cz;
+  float prof;
+  if (ph.cre_model == HAMX_CRE_UNIF) {
+    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
+    prof = (length0(d) > ph.cre[1]) ? 0.f : 1.f;
+  } else {
+    const float e = __expf(zk * c.inv_z0k);
+    const float ch = 0.5f * (e + 1.f / e);
+    prof = __expf((c.R0k - rk) * c.inv_r0k) / (ch * ch);
+  }
+  if (PW1)
+    o.w = prof * bper2;
+  else
+    o.w = prof * exp2f(c.hexp * __log2f(bper2));
+  const float x = -(ephf.x * bx + ephf.y * by + ephf.z * bz);
+  const float y = -(ethf.x * bx + ethf.y * by + ethf.z * bz);
+  const float n2 = x * x + y * y;
+  if (n2 > 0.f) {
+    const float inv = 1.f / n2;
+    o.c2i = (x * x - y * y) * inv;
+    o.s2i = 2.f * x * y * inv;
+  } else {
+    o.c2i = 1.f;
+    o.s2i = 0.f;
+  }
+  return true;
+}
+
+template <bool GRIDS, bool PW1>
+__device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
+  const ShellArgs sa = shell_args(in, s);
+  const BShell &sh = in.sh[s];
+  RayGeo r;
+  ray_setup_call(in.ph, sa, ipix, r);
+  const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
+                                static_cast<float>(r.l.z));
+  const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
+                                  static_cast<float>(r.eth.z));
+  const float3 ephf = make_float3(static_cast<float>(r.eph.x), static_cast<float>(r.eph.y),
+                                  static_cast<float>(r.eph.z));
+  const int nf = in.n_freq;
+  float dm = 0.f, I = 0.f, Q[kMaxF] = {0.f, 0.f, 0.f, 0.f}, U[kMaxF] = {0.f, 0.f, 0.f, 0.f};
+  double carry = 0.;
+  for (uint64_t base = r.k_lo; base < r.k_hi; base += 64) {
+    const uint64_t k = base + lane;
+    S32 o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
+    const double fk = valid ? static_cast<double>(o.fdk) * sh.fdu : 0.;
+    const double incl = wave_incl_scan(fk, lane);
+    const double fd_here = carry + incl;
+    carry += __shfl(incl, 63, 64);
+    if (valid) {
+      dm += o.te;
+      I += o.w;
+#pragma unroll
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+          double rv = X * in.rev[f];
+          rv -= rint(rv);
+          const float a = static_cast<float>(rv);
+          const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
+          Q[f] += (cs * o.c2i - sn * o.s2i) * o.w;
+          U[f] += (sn * o.c2i + cs * o.s2i) * o.w;
+        }
+      }
+    }
+  }
+  const double dms = wave_sum(static_cast<double>(dm));
+  const double Is = wave_sum(static_cast<double>(I));
+  double Qs[kMaxF], Us[kMaxF];
+#pragma unroll
+  for (int f = 0; f < kMaxF; ++f) {
+    if (f < nf) {
+      Qs[f] = wave_sum(static_cast<double>(Q[f]));
+      Us[f] = wave_sum(static_cast<double>(U[f]));
+    }
+  }
+  if (lane == 0) {
+    if (in.do_dm)
+      in.t_dm[ipix] = dms * sh.delta;
+    if (in.do_fd)
+      in.t_fd[ipix] = carry;
+#pragma unroll
+    for (int f = 0; f < kMaxF; ++f) {
+      if (f < nf) {
+        in.t_i[f][ipix] = Is * sh.delta * in.st[f];
+        in.t_q[f][ipix] = Qs[f] * sh.delta * in.sp[f];
+        in.t_u[f][ipix] = Us[f] * sh.delta * in.sp[f];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------- work items --
+
+template <int PREC, bool GRIDS, bool PW1>
+__device__ void do_item(const LevelArgs &a, uint64_t item, int lane) {
+  uint32_t lo = 0, hi = a.n_ids;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.prefix[mid] <= item)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const uint32_t id = __builtin_amdgcn_readfirstlane(a.ids[lo]);
+  const BInst &in = a.inst[id];
+  const int s = static_cast<int>(a.level);
+  const uint64_t npix = in.sh[s].hs.npix;
+  const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
+  const uint64_t p1 = (p0 + a.rays < npix) ? p0 + a.rays : npix;
+  for (uint64_t p = p0; p < p1; ++p) {
+    if (PREC == HAMX_PREC_F32)
+      march_ray_f32<GRIDS, PW1>(in, s, p, lane);
+    else
+      march_ray_f64<GRIDS, PW1>(in, s, p, lane);
+  }
+}
+
+// One launch covers the instances of one shell level that share the
+// template case (random grids present, emissivity exponent -1).
+template <int PREC, bool GRIDS, bool PW1>
+__global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  if (!a.persistent) {
+    uint64_t b = blockIdx.x;
+    if (a.xcd) { // gridDim.x is a multiple of kQueues: b -> XCD-contiguous order
+      const uint64_t per = gridDim.x / kQueues;
+      b = (b % kQueues) * per + b / kQueues;
+    }
+    const uint64_t item = b * wpb + wave;
+    if (item < a.n_items)
+      do_item<PREC, GRIDS, PW1>(a, uniform_u64(item), lane);
+    return;
+  }
+  const uint32_t q0 = a.xcd ? (blockIdx.x % a.nq) : 0;
+  for (uint32_t qi = 0; qi < a.nq; ++qi) {
+    const uint32_t q = (q0 + qi) % a.nq;
+    for (;;) {
+      unsigned long long t = 0;
+      if (lane == 0)
+        t = atomicAdd(&a.ctr[q], 1ull);
+      t = uniform_u64(__shfl(t, 0, 64));
+      const uint64_t item = a.qbeg[q] + t;
+      if (item >= a.qend[q])
+        break; // every wave leaves once all queues are drained
+      do_item<PREC, GRIDS, PW1>(a, item, lane);
+    }
+  }
+}
+
+// ---------------------------------------------------------- accumulate --
+
+struct AccJob {
+  Healpix hd, hs;
+  int n, pad;
+  double *dst[kAccMaps];
+  const double *src[kAccMaps];
+};
+
+// dst += src (same nside) or dst += interpolate(src) at dst's pixel
+// centres (hamdis.h:698-721), as k_accumulate; one job per blockIdx.y.
+__global__ __launch_bounds__(256) void k_batch_acc(const AccJob *jobs) {
+  const AccJob &J = jobs[blockIdx.y];
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= J.hd.npix)
+    return;
+  if (J.hd.npix == J.hs.npix) {
+    for (int m = 0; m < J.n; ++m)
+      J.dst[m][i] = J.dst[m][i] + J.src[m][i];
+    return;
+  }
+  double theta, phi;
+  hp_pix2ang(J.hd, i, theta, phi);
+  uint64_t pix[4];
+  double wght[4];
+  hp_interpol_weights(J.hs, theta, phi, pix, wght);
+  for (int m = 0; m < J.n; ++m) {
+    double wtot = 0.0, res = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double val = J.src[m][pix[q]];
+      if (val > cgs::undef_threshold) {
+        res += val * wght[q];
+        wtot += wght[q];
+      }
+    }
+    const double v = (wtot == 0.0) ? cgs::undef : res / wtot;
+    J.dst[m][i] = J.dst[m][i] + v;
+  }
+}
+
+struct FillJob {
+  double *p;
+  uint64_t n;
+};
+
+__global__ __launch_bounds__(256) void k_batch_zero(const FillJob *jobs,
+                                                    unsigned long long *ctr, uint32_t nctr) {
+  const FillJob &J = jobs[blockIdx.y];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < J.n;
+       i += stride)
+    J.p[i] = 0.;
+  if (blockIdx.x == 0 && blockIdx.y == 0)
+    for (uint32_t i = threadIdx.x; i < nctr; i += blockDim.x)
+      ctr[i] = 0ull;
+}
+
+struct PackJob {
+  void *dst;
+  const double *bx, *by, *bz, *te;
+  uint64_t n;
+  double bscale; // 1 (FP64 records) or 1e6 (FP32 records in muG)
+  int f32, pad;
+};
+
+__global__ __launch_bounds__(256) void k_batch_pack(const PackJob *jobs) {
+  const PackJob &J = jobs[blockIdx.y];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < J.n;
+       i += stride) {
+    const double x = J.bx ? J.bx[i] * J.bscale : 0., y = J.by ? J.by[i] * J.bscale : 0.;
+    const double z = J.bz ? J.bz[i] * J.bscale : 0., t = J.te ? J.te[i] : 0.;
+    if (J.f32)
+      static_cast<float4 *>(J.dst)[i] =
+          make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z),
+                      static_cast<float>(t));
+    else
+      static_cast<double4 *>(J.dst)[i] = make_double4(x, y, z, t);
+  }
+}
+
+// ----------------------------------------------------------------- host --
+
+using MarchFn = void (*)(LevelArgs);
+
+// Kernel of a launch group: case c = 2 * grids + pw1.
+MarchFn march_kernel(uint32_t prec, int c) {
+  static const MarchFn f32[4] = {
+      k_batch_march<HAMX_PREC_F32, false, false>, k_batch_march<HAMX_PREC_F32, false, true>,
+      k_batch_march<HAMX_PREC_F32, true, false>, k_batch_march<HAMX_PREC_F32, true, true>};
+  static const MarchFn f64[4] = {
+      k_batch_march<HAMX_PREC_F64, false, false>, k_batch_march<HAMX_PREC_F64, false, true>,
+      k_batch_march<HAMX_PREC_F64, true, false>, k_batch_march<HAMX_PREC_F64, true, true>};
+  return prec == HAMX_PREC_F32 ? f32[c] : f64[c];
+}
+
+struct Variant {
+  int persistent, xcd;
+  uint32_t rays, waves_per_block;
+  const char *name;
+};
+
+const Variant kVariants[] = {
+    {0, 0, 1, 4, "one-shot, 1 ray/item, 4 waves/WG"},
+    {0, 1, 1, 4, "one-shot XCD-ordered, 1 ray/item, 4 waves/WG"},
+    {0, 1, 4, 4, "one-shot XCD-ordered, 4 rays/item, 4 waves/WG"},
+    {0, 1, 1, 1, "one-shot XCD-ordered, 1 ray/item, 1 wave/WG"},
+    {1, 0, 4, 4, "persistent global queue, 4 rays/item, 4 waves/WG"},
+    {1, 1, 4, 4, "persistent XCD queues, 4 rays/item, 4 waves/WG"},
+    {1, 1, 16, 4, "persistent XCD queues, 16 rays/item, 4 waves/WG"},
+    {1, 1, 1, 4, "persistent XCD queues, 1 ray/item, 4 waves/WG"},
+};
+constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+} // namespace
+} // namespace hamx
+
+using namespace hamx;
+
+struct hamx_batch {
+  hamx_ctx *ctx = nullptr;
+  uint32_t prec = 0;
+  std::vector<BInst> host_inst;
+  std::vector<void *> allocs; // every device allocation of the plan
+  BInst *d_inst = nullptr;
+  uint32_t levels = 0;
+  // per level: instance ids, item prefix (host copies for planning)
+  // per launch group g = level * 4 + case (case = 2 * grids + pw1)
+  std::vector<std::vector<uint32_t>> ids;
+  std::vector<std::vector<uint64_t>> steps; // step of each instance at the level
+  std::vector<uint32_t *> d_ids;
+  std::vector<AccJob *> d_acc;
+  std::vector<uint32_t> n_acc;
+  std::vector<uint64_t> acc_max_npix;
+  FillJob *d_fill = nullptr;
+  uint32_t n_fill = 0;
+  uint64_t fill_max = 0;
+  PackJob *d_pack = nullptr;
+  uint32_t n_pack = 0;
+  uint64_t pack_max = 0;
+  unsigned long long *d_ctr = nullptr;
+  // per variant and level: prefix arrays depend on rays per item
+  std::vector<std::vector<uint64_t *>> d_prefix; // [variant][group]
+  std::vector<std::vector<LevelArgs>> largs;     // [variant][group]
+  int n_cu = 0;
+  int per_cu[4][5] = {}; // resident march workgroups per CU, by case and waves per WG
+  uint32_t variant = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hamx_batch_info info{};
+};
+
+namespace {
+
+int dev_alloc(hamx_batch *b, size_t bytes, void **p) {
+  *p = nullptr;
+  if (bytes == 0)
+    bytes = 8;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return set_err(b->ctx, HAMX_ERR_NOMEM, "batch: device allocation of %zu bytes failed",
+                   bytes);
+  }
+  b->allocs.push_back(*p);
+  return HAMX_OK;
+}
+
+template <typename T>
+int upload(hamx_batch *b, const std::vector<T> &v, T **dptr) {
+  void *p;
+  HAMX_TRY(dev_alloc(b, v.size() * sizeof(T), &p));
+  if (!v.empty())
+    HAMX_HIP(b->ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *dptr = static_cast<T *>(p);
+  return HAMX_OK;
+}
+
+void release(hamx_batch *b) {
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (void *p : b->allocs)
+    (void)hipFree(p);
+  b->allocs.clear();
+}
+
+// Items of `level` for `rays` rays per item, split into kQueues contiguous
+// slices of about equal cost (cost of an item = rays * steps of its shell).
+void plan_level(const hamx_batch *b, uint32_t group, uint32_t rays, std::vector<uint64_t> &prefix,
+                LevelArgs &la) {
+  const uint32_t level = group / 4;
+  const auto &ids = b->ids[group];
+  prefix.assign(ids.size() + 1, 0);
+  std::vector<double> cost_prefix(ids.size() + 1, 0.);
+  for (size_t j = 0; j < ids.size(); ++j) {
+    const BInst &in = b->host_inst[ids[j]];
+    const uint64_t npix = in.sh[level].hs.npix;
+    const uint64_t items = (npix + rays - 1) / rays;
+    prefix[j + 1] = prefix[j] + items;
+    cost_prefix[j + 1] =
+        cost_prefix[j] + static_cast<double>(items) * rays * static_cast<double>(in.sh[level].step);
+  }
+  la.n_items = prefix.back();
+  la.n_ids = static_cast<uint32_t>(ids.size());
+  la.level = level;
+  la.rays = rays;
+  // cost-balanced contiguous slices
+  const double total = cost_prefix.back();
+  uint64_t prev = 0;
+  for (int q = 0; q < kQueues; ++q) {
+    uint64_t end = la.n_items;
+    if (q < kQueues - 1) {
+      const double target = total * (q + 1) / kQueues;
+      // find the item whose cumulative cost reaches target
+      size_t j = 0;
+      while (j + 1 < cost_prefix.size() && cost_prefix[j + 1] < target)
+        ++j;
+      if (j >= ids.size()) {
+        end = la.n_items;
+      } else {
+        const BInst &in = b->host_inst[ids[j]];
+        const double per_item = static_cast<double>(rays) * in.sh[level].step;
+        const double within = per_item > 0 ? (target - cost_prefix[j]) / per_item : 0.;
+        uint64_t e = prefix[j] + static_cast<uint64_t>(within);
+        if (e > prefix[j + 1])
+          e = prefix[j + 1];
+        end = e;
+      }
+    }
+    if (end < prev)
+      end = prev;
+    la.qbeg[q] = prev;
+    la.qend[q] = end;
+    prev = end;
+  }
+  la.qend[kQueues - 1] = la.n_items;
+}
+
+int launch_all(hamx_batch *b, uint32_t v) {
+  hamx_ctx *ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const Variant &V = kVariants[v];
+  // zero outputs and tickets
+  {
+    uint64_t blocks = (b->fill_max + 255) / 256;
+    blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 1024);
+    hipLaunchKernelGGL(k_batch_zero, dim3(static_cast<unsigned>(blocks), b->n_fill), dim3(256),
+                       0, st, static_cast<const FillJob *>(b->d_fill), b->d_ctr,
+                       b->levels * 4 * kQueues);
+    HAMX_HIP(ctx, hipGetLastError());
+  }
+  if (b->n_pack) {
+    uint64_t blocks = (b->pack_max + 255) / 256;
+    blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 2048);
+    hipLaunchKernelGGL(k_batch_pack, dim3(static_cast<unsigned>(blocks), b->n_pack), dim3(256),
+                       0, st, static_cast<const PackJob *>(b->d_pack));
+    HAMX_HIP(ctx, hipGetLastError());
+  }
+  const unsigned threads = 64u * V.waves_per_block;
+  for (uint32_t s = 0; s < b->levels; ++s) {
+    for (int c = 0; c < 4; ++c) {
+      const LevelArgs &la = b->largs[v][s * 4 + c];
+      if (la.n_items == 0)
+        continue;
+      unsigned blocks;
+      if (V.persistent) {
+        blocks = static_cast<unsigned>(b->n_cu * b->per_cu[c][V.waves_per_block]);
+      } else {
+        uint64_t nb = (la.n_items + V.waves_per_block - 1) / V.waves_per_block;
+        if (V.xcd)
+          nb = (nb + kQueues - 1) / kQueues * kQueues;
+        blocks = static_cast<unsigned>(nb);
+      }
+      LevelArgs arg = la;
+      void *args[] = {&arg};
+      HAMX_HIP(ctx, hipLaunchKernel(reinterpret_cast<const void *>(march_kernel(b->prec, c)),
+                                    dim3(blocks), dim3(threads), args, 0, st));
+    }
+    if (b->n_acc[s]) {
+      const unsigned ab = static_cast<unsigned>((b->acc_max_npix[s] + 255) / 256);
+      hipLaunchKernelGGL(k_batch_acc, dim3(ab, b->n_acc[s]), dim3(256), 0, st,
+                         static_cast<const AccJob *>(b->d_acc[s]));
+      HAMX_HIP(ctx, hipGetLastError());
+    }
+  }
+  return HAMX_OK;
+}
+
+int check_inst(hamx_ctx *ctx, const hamx_batch_inst &x, uint32_t i) {
+  const hamx_los_desc &d = x.los;
+  if (x.n_freq < 1 || x.n_freq > HAMX_MAX_FREQ)
+    return set_err(ctx, HAMX_ERR_ARG, "batch instance %u: n_freq %u outside [1, %d]", i,
+                   x.n_freq, HAMX_MAX_FREQ);
+  for (uint32_t f = 0; f < x.n_freq; ++f)
+    if (!(x.freq[f] > 0.))
+      return set_err(ctx, HAMX_ERR_ARG, "batch instance %u: frequency %u not positive", i, f);
+  if (d.breg_model != HAMX_BREG_LSA || d.tereg_model != HAMX_TEREG_UNIF ||
+      !(d.cre_model == HAMX_CRE_ANA || d.cre_model == HAMX_CRE_UNIF) ||
+      (d.cre_model == HAMX_CRE_ANA && (d.cre[1] != 0. || d.cre[2] != 0.)))
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "batch instance %u: the batch covers LSA B, uniform TE and a "
+                   "constant-index CRE", i);
+  const bool any_b = x.brnd_x || x.brnd_y || x.brnd_z;
+  if (any_b && !(x.brnd_x && x.brnd_y && x.brnd_z))
+    return set_err(ctx, HAMX_ERR_ARG, "batch instance %u: brnd needs all three components", i);
+  if (any_b || x.ternd)
+    HAMX_TRY(check_box(ctx, x.box, "batch grid"));
+  for (uint32_t f = 0; f < x.n_freq; ++f)
+    if (!x.is[f] || !x.qs[f] || !x.us[f])
+      return set_err(ctx, HAMX_ERR_ARG, "batch instance %u: sync maps of frequency %u missing",
+                     i, f);
+  return HAMX_OK;
+}
+
+} // namespace
+
+extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uint32_t n_inst,
+                                 uint32_t precision, uint32_t flags, hamx_batch **out) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!out || !inst || n_inst == 0)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: empty batch or null output");
+  *out = nullptr;
+  if (precision != HAMX_PREC_F64 && precision != HAMX_PREC_F32)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: unknown precision %u", precision);
+  if ((flags & (HAMX_IN_DEVICE | HAMX_OUT_DEVICE)) != (HAMX_IN_DEVICE | HAMX_OUT_DEVICE))
+    return set_err(ctx, HAMX_ERR_ARG,
+                   "hamx_batch_create: grids and maps must be device buffers "
+                   "(HAMX_IN_DEVICE | HAMX_OUT_DEVICE)");
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    HAMX_TRY(check_inst(ctx, inst[i], i));
+    hamx_los_desc d = inst[i].los;
+    d.do_sync = 1;
+    d.sync_freq = inst[i].freq[inst[i].n_freq - 1];
+    d.schedule = HAMX_SCHED_AUTO;
+    HAMX_TRY(validate_los(ctx, &d));
+  }
+  hamx_batch *b = new (std::nothrow) hamx_batch;
+  if (!b)
+    return set_err(ctx, HAMX_ERR_NOMEM, "hamx_batch_create: host allocation failed");
+  b->ctx = ctx;
+  b->prec = precision;
+  int rc = HAMX_OK;
+  auto fail = [&](int code) {
+    release(b);
+    delete b;
+    return code;
+  };
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess)
+      return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipGetDeviceProperties failed"));
+    b->n_cu = prop.multiProcessorCount;
+    for (int c = 0; c < 4; ++c)
+      for (uint32_t w : {1u, 2u, 4u}) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(march_kernel(precision, c)),
+                static_cast<int>(64 * w), 0) != hipSuccess)
+          return fail(set_err(ctx, HAMX_ERR_DEVICE, "occupancy query failed"));
+        b->per_cu[c][w] = per_cu < 1 ? 1 : per_cu;
+      }
+  }
+  b->host_inst.resize(n_inst);
+  std::vector<FillJob> fills;
+  std::vector<PackJob> packs;
+  // packed grids, shared by instances with identical grid pointers and box
+  struct GridKey {
+    const double *x, *y, *z, *t;
+    hamx_box box;
+    void *packed;
+  };
+  std::vector<GridKey> grid_keys;
+  uint64_t samples_geo = 0, samples_ref = 0;
+  for (uint32_t i = 0; i < n_inst; ++i) {
+    const hamx_batch_inst &x = inst[i];
+    BInst &in = b->host_inst[i];
+    std::memset(&in, 0, sizeof(in));
+    const uint32_t K = x.n_freq;
+    // pass f of the batch = frequency freq[K-1-f] (the reference's pop_back order)
+    hamx_los_desc d0 = x.los;
+    d0.do_sync = 1;
+    d0.sync_freq = x.freq[K - 1];
+    in.ph = make_phys(&d0);
+    const bool pw1 = 0.5 * (in.ph.idx + 1) == -1.0;
+    in.pw1 = pw1 ? 1 : 0;
+    in.n_shell = static_cast<int>(x.los.n_shell);
+    in.n_freq = static_cast<int>(K);
+    in.do_dm = x.los.do_dm ? 1 : 0;
+    in.do_fd = x.los.do_fd ? 1 : 0;
+    for (uint32_t s = 0; s < x.los.n_shell; ++s) {
+      BShell &sh = in.sh[s];
+      sh.hs = healpix_make(x.los.nside_shell[s]);
+      shell_ref(&d0, s, sh.step, sh.d_start, sh.delta);
+      sh.fdu = in.ph.fd_fore * 1e-6 * sh.delta;
+      in.fc[s] = make_fast(&d0, in.ph, sh.delta, pw1);
+      samples_geo += sh.hs.npix * sh.step;
+      samples_ref += sh.hs.npix * sh.step * K;
+    }
+    // per-pass constants
+    const FastConst raw0 = make_fast(&d0, in.ph, 1.0, false);
+    const FastConst pw0 = make_fast(&d0, in.ph, 1.0, pw1);
+    const double pw_e = raw0.pw_e;
+    for (uint32_t f = 0; f < K; ++f) {
+      hamx_los_desc df = d0;
+      df.sync_freq = x.freq[K - 1 - f];
+      const Phys pf = make_phys(&df);
+      const FastConst raw = make_fast(&df, pf, 1.0, false);
+      const FastConst pwf = make_fast(&df, pf, 1.0, pw1);
+      const double tc = pf.tc_em1 * pf.tc_em1 / pf.tc_den;
+      in.two_l2[f] = 2. * pf.lambda_sq;
+      in.rev[f] = pf.lambda_sq / cgs::pi;
+      in.fdin[f] = (f == 0 && x.los.do_fd) ? 1 : 0;
+      if (precision == HAMX_PREC_F64) {
+        // ratio of this pass's emissivity to pass 0's, as eval_fast forms it
+        double rt = pwf.kt / pw0.kt, rp = pwf.kp / pw0.kp;
+        if (!pw1) {
+          const double ra = std::pow(raw.a_coef / raw0.a_coef, pw_e);
+          rt *= ra;
+          rp *= ra;
+        }
+        in.st[f] = rt * tc;
+        in.sp[f] = rp * tc;
+      } else {
+        const double unit = std::pow(1e-6, 1. - pw_e) * std::pow(raw.a_coef, pw_e);
+        in.st[f] = raw.kt * unit * tc;
+        in.sp[f] = raw.kp * unit * tc;
+      }
+    }
+    // FP32 constants
+    in.f.psi0 = static_cast<float>(x.los.breg[1]);
+    in.f.psi1 = static_cast<float>(x.los.breg[2]);
+    in.f.chi0 = static_cast<float>(x.los.breg[3]);
+    in.f.b0u = static_cast<float>(x.los.breg[0] * 1e6);
+    in.f.n0 = static_cast<float>(x.los.tereg[0]);
+    if (x.los.cre_model == HAMX_CRE_ANA) {
+      in.f.R0k = static_cast<float>(in.ph.R0 / cgs::kpc);
+      in.f.inv_r0k = static_cast<float>(cgs::kpc / x.los.cre[3]);
+      in.f.inv_z0k = static_cast<float>(cgs::kpc / x.los.cre[4]);
+    }
+    in.f.hexp = static_cast<float>(0.5 * (1. - pw_e));
+    // grids
+    const bool any_grid = x.brnd_x || x.ternd;
+    in.grids = any_grid ? 1 : 0;
+    if (any_grid) {
+      in.geo = grid_geo(x.box);
+      void *packed = nullptr;
+      for (const GridKey &g : grid_keys)
+        if (g.x == x.brnd_x && g.y == x.brnd_y && g.z == x.brnd_z && g.t == x.ternd &&
+            std::memcmp(&g.box, &x.box, sizeof(hamx_box)) == 0)
+          packed = g.packed;
+      if (!packed) {
+        const uint64_t n = box_size(x.box);
+        const size_t rec = precision == HAMX_PREC_F32 ? sizeof(float4) : sizeof(double4);
+        if ((rc = dev_alloc(b, n * rec, &packed)) != HAMX_OK)
+          return fail(rc);
+        grid_keys.push_back(GridKey{x.brnd_x, x.brnd_y, x.brnd_z, x.ternd, x.box, packed});
+        PackJob pj;
+        std::memset(&pj, 0, sizeof(pj));
+        pj.dst = packed;
+        pj.bx = x.brnd_x;
+        pj.by = x.brnd_y;
+        pj.bz = x.brnd_z;
+        pj.te = x.ternd;
+        pj.n = n;
+        pj.f32 = precision == HAMX_PREC_F32;
+        pj.bscale = pj.f32 ? 1e6 : 1.;
+        packs.push_back(pj);
+        b->pack_max = std::max(b->pack_max, n);
+      }
+      in.packed = packed;
+    }
+    // output and shell maps
+    uint64_t max_npix = 0;
+    for (uint32_t s = 0; s < x.los.n_shell; ++s)
+      max_npix = std::max<uint64_t>(max_npix, in.sh[s].hs.npix);
+    const Healpix h_sync = healpix_make(x.los.nside_sync);
+    in.hfd = healpix_make(x.los.do_fd ? x.los.nside_fd : 1);
+    void *p;
+    if (in.do_dm) {
+      if ((rc = dev_alloc(b, max_npix * 8, &p)) != HAMX_OK)
+        return fail(rc);
+      in.t_dm = static_cast<double *>(p);
+      if (x.dm)
+        fills.push_back(FillJob{x.dm, healpix_make(x.los.nside_dm).npix});
+    }
+    if (in.do_fd) {
+      if ((rc = dev_alloc(b, max_npix * 8, &p)) != HAMX_OK)
+        return fail(rc);
+      in.t_fd = static_cast<double *>(p);
+      double *fdm = x.fd;
+      if (!fdm) { // the FD cache needs the accumulated map even when not requested
+        if ((rc = dev_alloc(b, in.hfd.npix * 8, &p)) != HAMX_OK)
+          return fail(rc);
+        fdm = static_cast<double *>(p);
+      }
+      in.fd_cache = fdm;
+      fills.push_back(FillJob{fdm, in.hfd.npix});
+    }
+    for (uint32_t f = 0; f < K; ++f) {
+      double **t[3] = {&in.t_i[f], &in.t_q[f], &in.t_u[f]};
+      for (int m = 0; m < 3; ++m) {
+        if ((rc = dev_alloc(b, max_npix * 8, &p)) != HAMX_OK)
+          return fail(rc);
+        *t[m] = static_cast<double *>(p);
+      }
+      // pass f writes the maps of frequency freq[K-1-f]
+      fills.push_back(FillJob{x.is[K - 1 - f], h_sync.npix});
+      fills.push_back(FillJob{x.qs[K - 1 - f], h_sync.npix});
+      fills.push_back(FillJob{x.us[K - 1 - f], h_sync.npix});
+    }
+    b->levels = std::max<uint32_t>(b->levels, x.los.n_shell);
+  }
+  if (fills.size() > 65535 || n_inst > 65535 / 3)
+    return fail(set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: too many instances"));
+  for (const FillJob &f : fills)
+    b->fill_max = std::max(b->fill_max, f.n);
+  b->n_fill = static_cast<uint32_t>(fills.size());
+  b->n_pack = static_cast<uint32_t>(packs.size());
+  if ((rc = upload(b, fills, &b->d_fill)) != HAMX_OK)
+    return fail(rc);
+  if (!packs.empty() && (rc = upload(b, packs, &b->d_pack)) != HAMX_OK)
+    return fail(rc);
+  {
+    void *p;
+    if ((rc = dev_alloc(b, sizeof(unsigned long long) * kQueues * 4 * b->levels, &p)) !=
+        HAMX_OK)
+      return fail(rc);
+    b->d_ctr = static_cast<unsigned long long *>(p);
+  }
+  if ((rc = upload(b, b->host_inst, &b->d_inst)) != HAMX_OK)
+    return fail(rc);
+  // levels: instance lists (instance-major item order) and accumulate jobs
+  b->ids.resize(b->levels * 4);
+  b->d_ids.resize(b->levels * 4, nullptr);
+  b->d_acc.resize(b->levels);
+  b->n_acc.assign(b->levels, 0);
+  b->acc_max_npix.assign(b->levels, 0);
+  for (uint32_t s = 0; s < b->levels; ++s) {
+    std::vector<AccJob> jobs;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+      const hamx_batch_inst &x = inst[i];
+      const BInst &in = b->host_inst[i];
+      if (s >= x.los.n_shell)
+        continue;
+      b->ids[s * 4 + 2 * in.grids + in.pw1].push_back(i);
+      const Healpix hs = in.sh[s].hs;
+      const uint32_t K = x.n_freq;
+      AccJob sj;
+      std::memset(&sj, 0, sizeof(sj));
+      sj.hd = healpix_make(x.los.nside_sync);
+      sj.hs = hs;
+      for (uint32_t f = 0; f < K; ++f) {
+        sj.dst[sj.n] = x.is[K - 1 - f];
+        sj.src[sj.n++] = in.t_i[f];
+        sj.dst[sj.n] = x.qs[K - 1 - f];
+        sj.src[sj.n++] = in.t_q[f];
+        sj.dst[sj.n] = x.us[K - 1 - f];
+        sj.src[sj.n++] = in.t_u[f];
+      }
+      jobs.push_back(sj);
+      if (in.do_dm && x.dm) {
+        AccJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.hd = healpix_make(x.los.nside_dm);
+        j.hs = hs;
+        j.n = 1;
+        j.dst[0] = x.dm;
+        j.src[0] = in.t_dm;
+        jobs.push_back(j);
+      }
+      if (in.do_fd) {
+        AccJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.hd = in.hfd;
+        j.hs = hs;
+        j.n = 1;
+        j.dst[0] = const_cast<double *>(in.fd_cache);
+        j.src[0] = in.t_fd;
+        jobs.push_back(j);
+      }
+    }
+    for (const AccJob &j : jobs)
+      b->acc_max_npix[s] = std::max(b->acc_max_npix[s], j.hd.npix);
+    b->n_acc[s] = static_cast<uint32_t>(jobs.size());
+    if ((rc = upload(b, jobs, &b->d_acc[s])) != HAMX_OK)
+      return fail(rc);
+    for (int c = 0; c < 4; ++c)
+      if ((rc = upload(b, b->ids[s * 4 + c], &b->d_ids[s * 4 + c])) != HAMX_OK)
+        return fail(rc);
+  }
+  // item plans of every variant
+  b->d_prefix.assign(kNumVariants, std::vector<uint64_t *>(b->levels * 4, nullptr));
+  b->largs.assign(kNumVariants, std::vector<LevelArgs>(b->levels * 4));
+  for (uint32_t v = 0; v < kNumVariants; ++v) {
+    for (uint32_t s = 0; s < b->levels * 4; ++s) {
+      LevelArgs &la = b->largs[v][s];
+      std::memset(&la, 0, sizeof(la));
+      std::vector<uint64_t> prefix;
+      plan_level(b, s, kVariants[v].rays, prefix, la);
+      if ((rc = upload(b, prefix, &b->d_prefix[v][s])) != HAMX_OK)
+        return fail(rc);
+      la.inst = b->d_inst;
+      la.ids = b->d_ids[s];
+      la.prefix = b->d_prefix[v][s];
+      la.ctr = b->d_ctr + static_cast<size_t>(s) * kQueues;
+      la.persistent = kVariants[v].persistent;
+      la.xcd = kVariants[v].xcd;
+      la.nq = kVariants[v].xcd ? kQueues : 1;
+      if (!la.xcd) { // one global queue
+        la.qbeg[0] = 0;
+        la.qend[0] = la.n_items;
+      }
+    }
+  }
+  // autotune: one warm-up and one timed eager run per candidate
+  b->info.samples_geo = samples_geo;
+  b->info.samples_ref = samples_ref;
+  b->info.levels = b->levels;
+  const bool tune = (flags & HAMX_BATCH_NO_TUNE) == 0;
+  uint32_t best = 0;
+  if (tune) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipEventCreate failed"));
+    double best_ms = 1e300;
+    for (uint32_t v = 0; v < kNumVariants && rc == HAMX_OK; ++v) {
+      rc = launch_all(b, v);
+      if (rc == HAMX_OK)
+        (void)hipEventRecord(e0, ctx->stream);
+      if (rc == HAMX_OK)
+        rc = launch_all(b, v);
+      if (rc == HAMX_OK) {
+        (void)hipEventRecord(e1, ctx->stream);
+        if (hipEventSynchronize(e1) != hipSuccess)
+          rc = set_err(ctx, HAMX_ERR_DEVICE, "batch autotune: %s",
+                       hipGetErrorString(hipGetLastError()));
+      }
+      if (rc == HAMX_OK) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        b->info.variant_ms[v] = ms;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = v;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != HAMX_OK)
+      return fail(rc);
+    b->info.n_variants = kNumVariants;
+  }
+  b->variant = best;
+  b->info.variant = best;
+  std::snprintf(b->info.shape, sizeof(b->info.shape), "%s", kVariants[best].name);
+  // capture the chosen variant
+  if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipStreamBeginCapture failed"));
+  rc = launch_all(b, best);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  if (rc != HAMX_OK) {
+    if (g)
+      (void)hipGraphDestroy(g);
+    return fail(rc);
+  }
+  if (ec != hipSuccess || !g)
+    return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipStreamEndCapture failed"));
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess)
+    return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipGraphInstantiate failed"));
+  *out = b;
+  return HAMX_OK;
+}
+
+extern "C" int hamx_batch_run(hamx_batch *b) {
+  if (!b || !b->exec)
+    return HAMX_ERR_ARG;
+  hamx_ctx *ctx = b->ctx;
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  HAMX_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+
+extern "C" int hamx_batch_get_info(const hamx_batch *b, hamx_batch_info *info) {
+  if (!b || !info)
+    return HAMX_ERR_ARG;
+  *info = b->info;
+  return HAMX_OK;
+}
+
+extern "C" void hamx_batch_destroy(hamx_batch *b) {
+  if (!b)
+    return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  release(b);
+  delete b;
+}

@@ -28,6 +28,11 @@ struct hamx_ctx {
   static constexpr int kSlots = 48;
   void *ws[kSlots] = {};
   size_t ws_bytes[kSlots] = {};
+  // What the tape in workspace slots kSlotAdjCkpt / kSlotTapeFd was recorded
+  // for (hamx_los_forward with HAMX_KEEP_TAPE).
+  bool tape_valid = false;
+  hamx_los_desc tape_desc{};
+  hamx_fields tape_fields{};
   // Pinned host staging buffer for host-resident outputs.
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
@@ -80,6 +85,7 @@ enum Slot : int {
   kSlotAdjRayPar, // per-ray parameter partials
   kSlotAdjPar,    // reduced parameter gradient
   kSlotAdjGrid,   // packed grid gradient
+  kSlotTapeFd,    // every ray's FD over each shell (forward tape)
   kSlotCount
 };
 static_assert(kSlotCount <= hamx_ctx::kSlots, "workspace slots");

@@ -639,7 +639,7 @@ extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
 
 namespace hamx {
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
-                     hamx_maps *out, uint32_t flags, double *ckpt) {
+                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd) {
   if (!ctx)
     return HAMX_ERR_ARG;
   if (!d || !out)
@@ -730,7 +730,8 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     sa.fd_map = (d->do_fd && s > 0) ? o_fd : nullptr;
     sa.hfd = h_fd;
     sa.t_dm = t_dm;
-    sa.t_fd = t_fd;
+    // with a tape, each shell's FD map is written into the tape itself
+    sa.t_fd = shell_fd ? shell_fd + shell_offset(d, s) : t_fd;
     sa.t_is = t_is;
     sa.t_qs = t_qs;
     sa.t_us = t_us;
@@ -798,7 +799,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
         pr.dst[pr.n] = o_dm; pr.src[pr.n] = t_dm; ++pr.n;
       }
       if (d->do_fd && d->nside_fd == d->nside_sync) {
-        pr.dst[pr.n] = o_fd; pr.src[pr.n] = t_fd; ++pr.n;
+        pr.dst[pr.n] = o_fd; pr.src[pr.n] = sa.t_fd; ++pr.n;
       }
       HAMX_TRY(acc(pr, h_sync));
     }
@@ -808,7 +809,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
       p2.dst[0] = o_dm; p2.src[0] = t_dm; p2.n = 1;
       if (d->do_fd && d->nside_fd == d->nside_dm &&
           !(d->do_sync && d->nside_fd == d->nside_sync)) {
-        p2.dst[1] = o_fd; p2.src[1] = t_fd; p2.n = 2;
+        p2.dst[1] = o_fd; p2.src[1] = sa.t_fd; p2.n = 2;
       }
       HAMX_TRY(acc(p2, h_dm));
     }
@@ -817,7 +818,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     if (d->do_fd && !fd_done) {
       AccPairs p3;
       std::memset(&p3, 0, sizeof(p3));
-      p3.dst[0] = o_fd; p3.src[0] = t_fd; p3.n = 1;
+      p3.dst[0] = o_fd; p3.src[0] = sa.t_fd; p3.n = 1;
       HAMX_TRY(acc(p3, h_fd));
     }
   }
@@ -847,5 +848,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
 extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
                                 const hamx_fields *fields, hamx_maps *out,
                                 uint32_t flags) {
-  return los_forward_impl(ctx, d, fields, out, flags, nullptr);
+  if (ctx && (flags & HAMX_KEEP_TAPE))
+    return los_forward_taped(ctx, d, fields, out, flags);
+  return los_forward_impl(ctx, d, fields, out, flags, nullptr, nullptr);
 }

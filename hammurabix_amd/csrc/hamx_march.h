@@ -166,6 +166,17 @@ __device__ inline double wave_incl_scan(double v, int lane) {
   return v;
 }
 
+// Sum over lanes >= lane (a reverse inclusive scan).
+__device__ inline double wave_suffix_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_down(v, o, 64);
+    if (lane + o < 64)
+      v += t;
+  }
+  return v;
+}
+
 // Per-shell constants of the specialised march (see make_fast).
 struct FastConst {
   double psi0, psi1, chi0, b0;
@@ -309,10 +320,22 @@ FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool p
 // when they share a box.
 int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
                  bool in_dev, Grids &gr, bool &packed, bool &grids_any);
+// First pixel of shell s in a buffer holding one value per ray of every
+// shell: sum over t < s of 12 nside_t^2.
+inline uint64_t shell_offset(const hamx_los_desc *d, uint32_t s) {
+  uint64_t off = 0;
+  for (uint32_t t = 0; t < s; ++t)
+    off += 12ull * d->nside_shell[t] * d->nside_shell[t];
+  return off;
+}
 // One write_grid pass (hamx_los_forward); when ckpt is non-null and FD is
 // carried, the accumulated FD map before shell s is copied to
-// ckpt + s * npix(fd map) (device memory).
+// ckpt + s * npix(fd map); when shell_fd is non-null each ray's FD over
+// shell s is written to shell_fd + shell_offset(d, s) (device memory).
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
-                     hamx_maps *out, uint32_t flags, double *ckpt);
+                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd);
+// hamx_los_forward recording the adjoint's tape in the context workspace.
+int los_forward_taped(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
+                      hamx_maps *out, uint32_t flags);
 
 } // namespace hamx

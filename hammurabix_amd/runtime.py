@@ -111,8 +111,10 @@ class Context:
                     ternd: Any = None, ternd_box: Optional[capi.HamxBox] = None,
                     breg: Optional[tuple[Any, Any, Any]] = None,
                     breg_box: Optional[capi.HamxBox] = None,
-                    tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None) -> None:
-        """One Integrator::write_grid pass into `maps` (keys dm, fd, is, qs, us)."""
+                    tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None,
+                    keep_tape: bool = False) -> None:
+        """One Integrator::write_grid pass into `maps` (keys dm, fd, is, qs, us).
+        keep_tape leaves the pass's tape in the context for los_adjoint."""
         f = capi.HamxFields()
         inputs = []
         if brnd is not None:
@@ -138,6 +140,8 @@ class Context:
             flags |= capi.HAMX_IN_DEVICE
         if _residency(outs, "output"):
             flags |= capi.HAMX_OUT_DEVICE
+        if keep_tape:
+            flags |= capi.HAMX_KEEP_TAPE
         self._check(self.lib.hamx_los_forward(self.handle, C.byref(desc), C.byref(f),
                                               C.byref(m), flags))
 
@@ -146,10 +150,12 @@ class Context:
                     brnd_box: Optional[capi.HamxBox] = None,
                     ternd: Any = None, ternd_box: Optional[capi.HamxBox] = None,
                     dbrnd: Optional[tuple[Any, Any, Any]] = None,
-                    dternd: Any = None) -> None:
+                    dternd: Any = None, use_tape: bool = False) -> None:
         """Reverse mode of los_forward: `adj` holds dL/d(maps) (keys dm, fd, is,
         qs, us; missing maps are zero); writes dL/d(b0, psi0, psi1, chi0,
-        alpha, n0) into `dparams` and dL/d(grid) into `dbrnd` / `dternd`."""
+        alpha, n0) into `dparams` and dL/d(grid) into `dbrnd` / `dternd`.
+        use_tape takes the tape of the last los_forward(keep_tape=True) with the
+        same descriptor and arrays instead of re-running the forward."""
         f = capi.HamxFields()
         inputs = [adj.get(k) for k in ("dm", "fd", "is", "qs", "us")]
         if brnd is not None:
@@ -167,10 +173,23 @@ class Context:
             flags |= capi.HAMX_IN_DEVICE
         if _residency(outs, "output"):
             flags |= capi.HAMX_OUT_DEVICE
+        if use_tape:
+            flags |= capi.HAMX_USE_TAPE
         gx, gy, gz = (_ptr(a)[0] for a in dbrnd) if dbrnd is not None else (None,) * 3
         self._check(self.lib.hamx_los_adjoint(self.handle, C.byref(desc), C.byref(f),
                                               C.byref(m), _ptr(dparams)[0], gx, gy, gz,
                                               _ptr(dternd)[0], flags))
+
+    # -------------------------------------------------------------- batch
+    def batch(self, instances: list[dict], precision: int = capi.HAMX_PREC_F32,
+              tune: bool = True) -> "Batch":
+        """Plan a batched LOS run (hamx_batch_create).  Each instance is a dict:
+        desc (HamxLosDesc), freqs (1-4 Hz values, <sync> list order), maps
+        (dm / fd tensors or None, is / qs / us lists of tensors, one per
+        frequency), and optionally brnd (3 tensors), ternd and box.  Every
+        array is a float64 CUDA tensor; addresses are fixed for the plan's
+        lifetime."""
+        return Batch(self, instances, precision, tune)
 
     # -------------------------------------------------------- random fields
     def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
@@ -210,3 +229,75 @@ def los_samples(desc: capi.HamxLosDesc) -> int:
     if rc != capi.HAMX_OK:
         raise HamxError(rc, "invalid LOS descriptor")
     return int(n.value)
+
+
+class Batch:
+    """A planned batch: run() replays the captured graph."""
+
+    def __init__(self, ctx: Context, instances: list[dict], precision: int, tune: bool):
+        self.ctx = ctx
+        self.keep = []
+        arr = (capi.HamxBatchInst * len(instances))()
+        for x, spec in zip(arr, instances):
+            C.memmove(C.byref(x.los), C.byref(spec["desc"]), C.sizeof(capi.HamxLosDesc))
+            freqs = list(spec["freqs"])
+            if not 1 <= len(freqs) <= capi.HAMX_MAX_FREQ:
+                raise ValueError("1 to 4 frequencies per instance")
+            x.n_freq = len(freqs)
+            for i, f in enumerate(freqs):
+                x.freq[i] = f
+            arrays = []
+            brnd = spec.get("brnd")
+            if brnd is not None:
+                x.brnd_x, x.brnd_y, x.brnd_z = (_ptr(a)[0] for a in brnd)
+                arrays += list(brnd)
+            te = spec.get("ternd")
+            if te is not None:
+                x.ternd = _ptr(te)[0]
+                arrays.append(te)
+            if spec.get("box") is not None:
+                x.box = spec["box"]
+            m = spec["maps"]
+            x.dm = _ptr(m.get("dm"))[0]
+            x.fd = _ptr(m.get("fd"))[0]
+            arrays += [m.get("dm"), m.get("fd")]
+            for key, field in (("is", x.is_), ("qs", x.qs), ("us", x.us)):
+                lst = m[key]
+                if len(lst) != len(freqs):
+                    raise ValueError(f"maps[{key!r}] needs one map per frequency")
+                for i, a in enumerate(lst):
+                    field[i] = _ptr(a)[0]
+                arrays += list(lst)
+            for a in arrays:
+                if a is not None and not _ptr(a)[1]:
+                    raise ValueError("batch arrays must be device tensors")
+            self.keep += [a for a in arrays if a is not None]
+        flags = capi.HAMX_IN_DEVICE | capi.HAMX_OUT_DEVICE
+        if not tune:
+            flags |= capi.HAMX_BATCH_NO_TUNE
+        h = C.c_void_p()
+        ctx._check(ctx.lib.hamx_batch_create(ctx.handle, arr, len(instances), precision,
+                                             flags, C.byref(h)))
+        self.handle = h
+        self._arr = arr
+
+    def run(self) -> None:
+        self.ctx._check(self.ctx.lib.hamx_batch_run(self.handle))
+
+    def info(self) -> dict:
+        i = capi.HamxBatchInfo()
+        self.ctx._check(self.ctx.lib.hamx_batch_get_info(self.handle, C.byref(i)))
+        return dict(samples_geo=i.samples_geo, samples_ref=i.samples_ref, levels=i.levels,
+                    variant=i.variant, shape=i.shape.decode(),
+                    variant_ms=list(i.variant_ms[:i.n_variants]))
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.hamx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -40,6 +40,12 @@ extern "C" {
 /* ---- buffer-residency flags ------------------------------------------- */
 #define HAMX_IN_DEVICE 1u  /* every input array pointer is a device pointer */
 #define HAMX_OUT_DEVICE 2u /* every output array pointer is a device pointer */
+#define HAMX_KEEP_TAPE 8u  /* hamx_los_forward: keep this pass's tape (the FD map
+                              before each shell and each ray's FD per shell) in the
+                              context for hamx_los_adjoint */
+#define HAMX_USE_TAPE 16u  /* hamx_los_adjoint: use the kept tape instead of
+                              re-running the forward; the descriptor and field
+                              pointers must equal the taped pass's */
 
 /* ---- model selectors (the Param type strings) ------------------------- */
 /* regular magnetic field, Param::breg_type (param.cc:161-398) */
@@ -195,10 +201,12 @@ int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
  *                         uniform thermal electrons (CGS units of Param);
  *   dbrnd_x/y/z         : dL/d(random-B grid), grid layout (may be NULL);
  *   dternd              : dL/d(random-TE grid) (may be NULL).
- * Every output is overwritten.  The forward is recomputed with the
- * accumulated FD map checkpointed before each shell; shells are then
- * processed outermost first, each ray twice (total phase adjoint, then the
- * per-sample gradients).  Supported models: LSA regular B, uniform TE,
+ * Every output is overwritten.  The adjoint needs the forward's tape (the
+ * accumulated FD map before each shell and every ray's FD over each
+ * shell): with HAMX_USE_TAPE it takes the one a hamx_los_forward call with
+ * HAMX_KEEP_TAPE left in the context, otherwise it re-runs the forward to
+ * record it.  Shells are then processed outermost first, each ray in one
+ * sweep from its far end (running FD = ray total minus the samples beyond).  Supported models: LSA regular B, uniform TE,
  * analytic or uniform CRE with a constant spectral index, synchrotron on,
  * random grids (if any) sharing one box.  No counterpart in the reference,
  * which has no gradients.  flags as for hamx_los_forward (inputs: fields
@@ -212,6 +220,65 @@ int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
 /* Geometric LOS samples of one pass, S_geo = sum_s npix_s * step_s
  * (integrator.cc:62,173 trip counts); host-side arithmetic only. */
 int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples);
+
+/* ---- batched LOS -------------------------------------------------------- */
+
+/* Many instances (random-field realisations and / or parameter sets, each
+ * with its own shells, output nsides and frequency list) integrated by one
+ * replay of a captured HIP graph.  Each instance reproduces
+ * Pipeline::assemble_obs (pipeline.cc:165-190): one write_grid pass per
+ * synchrotron frequency, the last-listed frequency first with DM and FD
+ * (FD carried through the shells when los.do_fd), the other frequencies
+ * without DM / FD and with a zero FD cache.  The passes of one instance
+ * share every per-sample field evaluation; only the polarisation phase and
+ * the emissivity constants differ per frequency.  No counterpart in the
+ * reference, which runs one pass per process call. */
+#define HAMX_MAX_FREQ 4
+#define HAMX_PREC_F64 0u /* FP64 samples: the plain path's arithmetic */
+#define HAMX_PREC_F32 1u /* FP32 field / emissivity arithmetic and FP32 grids;
+                            sample positions, the limit tests, FD accumulation,
+                            the polarisation phase and map sums in FP64 */
+
+typedef struct hamx_batch_inst {
+  hamx_los_desc los;          /* LSA B, uniform TE, constant-index CRE; sync_freq and
+                                 schedule are ignored (freq[] below) */
+  uint32_t n_freq;            /* synchrotron frequencies, 1..HAMX_MAX_FREQ */
+  uint32_t reserved;
+  double freq[HAMX_MAX_FREQ]; /* Hz, in the order of the <sync> list */
+  const double *brnd_x, *brnd_y, *brnd_z; /* device random-B grid on box, or NULL */
+  const double *ternd;                    /* device random-TE grid on box, or NULL */
+  hamx_box box;
+  double *dm, *fd;            /* device output maps (nside_dm / nside_fd), or NULL */
+  double *is[HAMX_MAX_FREQ], *qs[HAMX_MAX_FREQ], *us[HAMX_MAX_FREQ]; /* nside_sync */
+} hamx_batch_inst;
+
+typedef struct hamx_batch hamx_batch;
+
+typedef struct hamx_batch_info {
+  uint64_t samples_geo;  /* geometric samples of one pass, summed over instances */
+  uint64_t samples_ref;  /* samples_geo of each instance times its n_freq */
+  uint32_t levels;       /* shell levels (march launches per replay) */
+  uint32_t variant;      /* launch shape chosen by the autotuner */
+  uint32_t n_variants;   /* candidates timed */
+  uint32_t reserved;
+  double variant_ms[16]; /* one untimed-graph replay per candidate, ms */
+  char shape[64];        /* the chosen launch shape, readable */
+} hamx_batch_info;
+
+/* Plan a batch: validates every instance, allocates the packed grids and
+ * shell maps, times the candidate launch shapes (persistent or one-shot,
+ * rays per work item, XCD-aware placement) on the batch itself, and
+ * captures the chosen one in a HIP graph.  Input grids are re-read and
+ * output maps overwritten by every hamx_batch_run; their addresses are
+ * fixed at creation.  flags: HAMX_IN_DEVICE | HAMX_OUT_DEVICE (required),
+ * plus HAMX_BATCH_NO_TUNE to skip the timing and take the first shape. */
+#define HAMX_BATCH_NO_TUNE 4u
+int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uint32_t n_inst,
+                      uint32_t precision, uint32_t flags, hamx_batch **out);
+/* Replay the captured graph once on the context's stream.  Synchronous. */
+int hamx_batch_run(hamx_batch *b);
+int hamx_batch_get_info(const hamx_batch *b, hamx_batch_info *info);
+void hamx_batch_destroy(hamx_batch *b);
 
 /* ---- random fields ------------------------------------------------------ */
 

@@ -42,7 +42,10 @@ def test_struct_layout_matches_c(tmp_path):
         "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu\\n\","
         "sizeof(hamx_box), sizeof(hamx_fields), sizeof(hamx_los_desc),"
         "sizeof(hamx_maps), sizeof(hamx_es_desc), sizeof(hamx_dft_desc),"
-        "offsetof(hamx_los_desc, cre), offsetof(hamx_es_desc, breg_box));return 0;}\n")
+        "offsetof(hamx_los_desc, cre), offsetof(hamx_es_desc, breg_box));"
+        "printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(hamx_batch_inst), sizeof(hamx_batch_info),"
+        "offsetof(hamx_batch_inst, box), offsetof(hamx_batch_inst, us),"
+        "sizeof(hamx_kernel_time));return 0;}\n")
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -50,7 +53,10 @@ def test_struct_layout_matches_c(tmp_path):
                                               check=True).stdout.split()]
     py = [C.sizeof(capi.HamxBox), C.sizeof(capi.HamxFields), C.sizeof(capi.HamxLosDesc),
           C.sizeof(capi.HamxMaps), C.sizeof(capi.HamxEsDesc), C.sizeof(capi.HamxDftDesc),
-          capi.HamxLosDesc.cre.offset, capi.HamxEsDesc.breg_box.offset]
+          capi.HamxLosDesc.cre.offset, capi.HamxEsDesc.breg_box.offset,
+          C.sizeof(capi.HamxBatchInst), C.sizeof(capi.HamxBatchInfo),
+          capi.HamxBatchInst.box.offset, capi.HamxBatchInst.us.offset,
+          C.sizeof(capi.HamxKernelTime)]
     assert c_sizes == py
 
 

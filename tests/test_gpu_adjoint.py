@@ -298,3 +298,27 @@ def test_param_gradients_vs_oracle_central_differences(ctx):
         fd = (lp - lm) / (2 * h)
         print(f"oracle d/d{name}: adjoint {dp[i]:.10e} fd {fd:.10e}", flush=True)
         assert abs(dp[i] - fd) <= 1e-5 * abs(fd), name
+
+
+def test_tape_reuse_matches_recompute(ctx):
+    """hamx_los_adjoint on the kept tape of a forward equals the adjoint that
+    re-runs the forward; a tape of another descriptor is refused."""
+    print("case: adjoint tape reuse", flush=True)
+    d, grids = multishell_case(seed=16)
+    w = scaled_weights(ctx, d, grids, seed=17)
+    r_re, g_re = adjoint(ctx, d, w, grids, with_grid=True)
+    m = out_maps(d)
+    ctx.los_forward(d, m, keep_tape=True, **grids)
+    dp = np.zeros(capi.HAMX_NPARAM)
+    n = grids["brnd"][0].shape[0]
+    gb = tuple(np.zeros(n) for _ in range(3))
+    gt = np.zeros(n)
+    ctx.los_adjoint(d, w, dp, dbrnd=gb, dternd=gt, use_tape=True, **grids)
+    assert np.array_equal(dp, r_re)
+    assert all(np.array_equal(a, b) for a, b in zip(gb, g_re["dbrnd"]))
+    assert np.array_equal(gt, g_re["dternd"])
+    d2, _ = multishell_case(seed=16)
+    d2.breg[0] *= 1.1
+    with pytest.raises(HamxError) as e:
+        ctx.los_adjoint(d2, w, dp, use_tape=True, **grids)
+    assert e.value.code == capi.HAMX_ERR_ARG

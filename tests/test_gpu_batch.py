@@ -1,0 +1,197 @@
+"""GPU checks of the batched LOS path (hamx_batch_*) against the plain path
+run pass by pass the way Pipeline::assemble_obs runs it (pipeline.cc:165-190):
+the last-listed frequency first with DM / FD, then the others without.
+
+FP64 batches must equal the plain path to rounding (1e-12 relative L2 per
+map: per-pass emissivity factors are formed as ratios, which moves the
+last bits); FP32 batches must stay within 1e-5 relative L2 per map, the
+batched path's stated tolerance.  Every replay of the captured graph must
+reproduce the first, and a replay after the input grids change must follow
+the new grids.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import HamxError, Param, _capi as capi, configs
+from hammurabix_amd.units import CGS
+from oracle_lib import npix, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FREQS = [0.408e9, 1.4e9, 23e9, 30e9]
+
+
+def torch_dev():
+    import torch
+    return torch, torch.device("cuda:0")
+
+
+def grids(shape, seed):
+    torch, dev = torch_dev()
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    b = tuple(torch.from_numpy(rng.normal(0, 2e-6, n)).to(dev) for _ in range(3))
+    te = torch.from_numpy(rng.normal(0.0, 0.003, n)).to(dev)
+    box = capi.HamxBox(*shape, -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc, 20 * CGS.kpc,
+                       -4 * CGS.kpc, 4 * CGS.kpc)
+    return b, te, box
+
+
+def instance(nside_sim, nside_out, freqs, b0=None, psi0=None, g=None, alpha=None,
+             cre_unif=False):
+    torch, dev = torch_dev()
+    d = Param(text=configs.random_forward(nside_sim=nside_sim, nside_out=nside_out,
+                                          oc_r_res=0.05, grid=(32, 32, 16))).los_desc()
+    if b0 is not None:
+        d.breg[0] = b0
+    if psi0 is not None:
+        d.breg[1] = psi0
+    if alpha is not None:
+        d.cre[0] = alpha
+    if cre_unif:
+        d.cre_model = capi.HAMX_CRE_UNIF
+        d.cre[0:4] = [alpha or 3.0, 6 * CGS.kpc, 20.6 * CGS.GeV, 0.0217]
+    z = lambda n: torch.zeros(npix(n), dtype=torch.float64, device=dev)  # noqa: E731
+    maps = dict(dm=z(d.nside_dm), fd=z(d.nside_fd), **{k: [z(d.nside_sync) for _ in freqs]
+                                                        for k in ("is", "qs", "us")})
+    spec = dict(desc=d, freqs=list(freqs), maps=maps)
+    if g is not None:
+        spec.update(brnd=g[0], ternd=g[1], box=g[2])
+    return spec
+
+
+def plain_passes(ctx, spec):
+    """The reference pipeline's passes on the plain path -> expected maps."""
+    d0 = spec["desc"]
+    freqs = spec["freqs"]
+    exp = {"is": [None] * len(freqs), "qs": [None] * len(freqs), "us": [None] * len(freqs)}
+    kw = {}
+    if "brnd" in spec:
+        b = tuple(a.cpu().numpy() for a in spec["brnd"])
+        kw = dict(brnd=b, brnd_box=spec["box"], ternd=spec["ternd"].cpu().numpy(),
+                  ternd_box=spec["box"])
+    for p, fi in enumerate(reversed(range(len(freqs)))):
+        d = capi.HamxLosDesc()
+        C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
+        d.sync_freq = freqs[fi]
+        d.do_sync = 1
+        if p > 0:
+            d.do_dm = d.do_fd = 0
+        m = {k: np.zeros(npix(d.nside_sync)) for k in ("is", "qs", "us")}
+        m["fd"] = np.zeros(npix(d.nside_fd if d.do_fd else d.nside_sync))
+        if d.do_dm:
+            m["dm"] = np.zeros(npix(d.nside_dm))
+        ctx.los_forward(d, m, **kw)
+        for k in ("is", "qs", "us"):
+            exp[k][fi] = m[k]
+        if p == 0:
+            exp["dm"], exp["fd"] = m["dm"], m["fd"]
+    return exp
+
+
+def compare(spec, exp, tol, label):
+    worst = 0.0
+    m = spec["maps"]
+    pairs = [("dm", m["dm"], exp["dm"]), ("fd", m["fd"], exp["fd"])]
+    for k in ("is", "qs", "us"):
+        pairs += [(f"{k}[{i}]", a, e) for i, (a, e) in enumerate(zip(m[k], exp[k]))]
+    for name, got, ref in pairs:
+        g = got.cpu().numpy()
+        assert np.all(np.isfinite(g)), name
+        err = rel_l2(g, ref)
+        worst = max(worst, err)
+        assert err <= tol, f"{label} {name}: rel L2 {err:.3e} > {tol}"
+    print(f"{label}: worst rel L2 {worst:.3e}", flush=True)
+    return worst
+
+
+def mixed_batch():
+    g1 = grids((32, 32, 16), 1)
+    g2 = grids((32, 32, 16), 2)
+    return [
+        instance(8, 32, FREQS[2:3], g=g1),
+        instance(4, 16, FREQS[:2], b0=5e-6, psi0=0.45, g=g2),
+        instance(8, 32, FREQS, b0=7e-6, psi0=0.55, g=g1),
+        instance(4, 16, FREQS[1:3], g=None),               # no random grids
+        instance(4, 16, FREQS[2:], g=g2, alpha=2.6, cre_unif=True),  # pow() emissivity
+    ]
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.HAMX_PREC_F64, 1e-12), (capi.HAMX_PREC_F32, 1e-5)])
+def test_batch_matches_plain_passes(ctx, prec, tol):
+    print(f"case: batch vs plain, precision {prec}", flush=True)
+    specs = mixed_batch()
+    with ctx.batch(specs, precision=prec, tune=False) as b:
+        b.run()
+        for i, s in enumerate(specs):
+            compare(s, plain_passes(ctx, s), tol, f"prec{prec} inst{i}")
+        info = b.info()
+        assert info["levels"] == 3
+        assert info["samples_ref"] > info["samples_geo"] > 0
+
+
+def test_graph_replays_and_follows_new_grids(ctx):
+    """Three replays give identical maps; scaling the random grids in place
+    and replaying gives the plain path's maps for the new grids."""
+    print("case: batch graph replays", flush=True)
+    g = grids((32, 32, 16), 3)
+    specs = [instance(8, 32, FREQS[1:3], g=g), instance(4, 16, FREQS[2:3], g=g, b0=4e-6)]
+    with ctx.batch(specs, precision=capi.HAMX_PREC_F64, tune=False) as b:
+        b.run()
+        first = [{k: ([t.clone() for t in v] if isinstance(v, list) else v.clone())
+                  for k, v in s["maps"].items()} for s in specs]
+        for _ in range(2):
+            b.run()
+            for s, f in zip(specs, first):
+                for k, v in s["maps"].items():
+                    vs = v if isinstance(v, list) else [v]
+                    fs = f[k] if isinstance(f[k], list) else [f[k]]
+                    for a, c in zip(vs, fs):
+                        assert bool((a == c).all()), k
+        for a in g[0]:
+            a.mul_(1.7)
+        g[1].mul_(0.5)
+        b.run()
+        for i, s in enumerate(specs):
+            compare(s, plain_passes(ctx, s), 1e-12, f"regrid inst{i}")
+
+
+def test_tuned_shapes_give_identical_maps(ctx):
+    """Every launch shape computes each ray the same way: the tuned plan's
+    maps equal the untuned plan's bit for bit."""
+    print("case: batch tuned vs untuned", flush=True)
+    import torch
+    specs_a = mixed_batch()
+    with ctx.batch(specs_a, precision=capi.HAMX_PREC_F32, tune=False) as b:
+        b.run()
+    specs_b = mixed_batch()
+    with ctx.batch(specs_b, precision=capi.HAMX_PREC_F32, tune=True) as b:
+        b.run()
+        info = b.info()
+        print("tuned:", info["shape"], info["variant_ms"], flush=True)
+        assert len(info["variant_ms"]) >= 4
+    for sa, sb in zip(specs_a, specs_b):
+        for k in sa["maps"]:
+            va, vb = sa["maps"][k], sb["maps"][k]
+            for a, c in zip(va if isinstance(va, list) else [va],
+                            vb if isinstance(vb, list) else [vb]):
+                assert torch.equal(a, c), k
+
+
+def test_batch_rejects_bad_instances(ctx):
+    s = instance(4, 16, FREQS[2:3])
+    s["desc"].breg_model = capi.HAMX_BREG_CART
+    with pytest.raises(HamxError) as e:
+        ctx.batch([s], tune=False)
+    assert e.value.code == capi.HAMX_ERR_UNSUPPORTED
+    s = instance(4, 16, FREQS[2:3])
+    s["desc"].nside_shell[0] = 6
+    with pytest.raises(HamxError) as e:
+        ctx.batch([s], tune=False)
+    assert e.value.code == capi.HAMX_ERR_NSIDE
+    s = instance(4, 16, FREQS[2:3])
+    s["maps"]["is"][0] = s["maps"]["is"][0].cpu()
+    with pytest.raises(ValueError):
+        ctx.batch([s], tune=False)
