@@ -169,3 +169,43 @@ def random_forward(**kw) -> str:
     args = dict(shell_num=3, nside_sim=64, nside_out=256, random=True)
     args.update(kw)
     return xml(**args)
+
+
+# ---- the 64-instance batch (many realisations and parameter sets)
+
+BATCH_FREQS_GHZ = (0.408, 1.4, 23.0, 30.0)
+BATCH_TOP_NSIDE = (64, 128, 256)
+# (b0 [muG] ~ U[4, 8], psi0 [deg] ~ U[20, 35]) of instance i, drawn once with
+# numpy.random.default_rng(20251202) (b0 for all 64 instances, then psi0)
+# and frozen here.
+BATCH_B0_PSI0 = (
+    (6.1138, 33.4704), (4.9177, 31.7859), (6.5748, 28.1820), (7.0530, 30.1541),
+    (6.6507, 29.8894), (5.7448, 32.9300), (6.5333, 27.8276), (6.4442, 29.9588),
+    (4.1436, 27.6601), (6.8107, 31.4726), (7.7355, 34.6906), (6.9297, 24.7631),
+    (6.4880, 29.0623), (6.5368, 27.3158), (6.6774, 21.8906), (4.0631, 25.8351),
+    (7.3027, 22.9290), (7.5530, 23.1883), (6.5636, 34.6284), (5.6773, 34.5441),
+    (6.7436, 33.3368), (6.6236, 32.5872), (6.1060, 27.4111), (4.4917, 26.5854),
+    (4.0906, 30.2557), (5.0300, 33.1263), (6.4519, 20.4097), (5.3218, 29.0560),
+    (5.1073, 32.0901), (7.7029, 24.2940), (6.4919, 28.8323), (5.6668, 33.2728),
+    (6.2141, 34.4645), (7.5456, 28.6275), (6.4639, 21.5399), (6.7816, 28.3655),
+    (4.6443, 21.3638), (7.2567, 28.8985), (4.5609, 33.4839), (5.4024, 20.9197),
+    (4.3294, 32.8582), (6.9287, 27.2931), (5.4564, 27.5291), (4.8879, 26.4520),
+    (5.0011, 34.9021), (4.7695, 27.1010), (6.3286, 28.1691), (5.1859, 33.6316),
+    (5.9223, 29.4615), (6.2254, 25.5421), (5.8939, 25.6044), (7.7351, 34.5893),
+    (7.5156, 22.0591), (4.2985, 23.4984), (7.2770, 30.1869), (6.7702, 32.1974),
+    (4.1267, 26.2203), (7.1226, 27.2274), (4.4570, 26.4148), (5.5693, 30.9069),
+    (5.0068, 34.0949), (5.3785, 24.2213), (6.0916, 23.6613), (6.5287, 23.7561),
+)
+
+
+def batch_instance(i: int, **kw) -> str:
+    """Instance i (0-based) of the batch: random fields with seed i + 1, top
+    nside BATCH_TOP_NSIDE[i % 3] over 3 automatic shells (nside_sim = top/4),
+    the first 1 + i % 4 frequencies of BATCH_FREQS_GHZ and the frozen
+    (b0, psi0) draw."""
+    top = BATCH_TOP_NSIDE[i % 3]
+    b0, psi0 = BATCH_B0_PSI0[i]
+    args = dict(shell_num=3, nside_sim=top // 4, nside_out=top, random=True, seed=i + 1,
+                b0=b0, psi0=psi0, freqs=list(BATCH_FREQS_GHZ[:1 + i % 4]))
+    args.update(kw)
+    return xml(**args)
