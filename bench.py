@@ -274,6 +274,29 @@ def main():
         march_samples = sum(t["work"] for t in march)
         acc_ms = sum(t["ms"] for t in times if t["kernel"] == "accumulate")
         finite = all(bool(torch.isfinite(m).all()) for m in maps.values())
+        # map+gradient evaluations: forward keeping its tape, then the adjoint
+        # of L = sum(w * maps) into the six parameters and the full B / TE grids
+        gen = torch.Generator(device=dev).manual_seed(7)
+        adj = {k: torch.randn(v.shape[0], dtype=torch.float64, device=dev, generator=gen)
+               for k, v in maps.items()}
+        dparams = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
+        dgrid = tuple(torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3))
+        dte = torch.zeros(n, dtype=torch.float64, device=dev)
+        ctx.los_forward(desc, maps, keep_tape=True, **kw)
+        ctx.los_adjoint(desc, adj, dparams, dbrnd=dgrid, dternd=dte, use_tape=True, **kw)
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        n_mg = max(1, args.steps // 2)
+        for _ in range(n_mg):
+            ctx.los_forward(desc, maps, keep_tape=True, **kw)
+            ctx.los_adjoint(desc, adj, dparams, dbrnd=dgrid, dternd=dte, use_tape=True, **kw)
+        torch.cuda.synchronize()
+        mg_s = (time.perf_counter() - t0) / n_mg
+        adj_times = ctx.kernel_times()
+        ctx.profile(False)
+        adj_ms = sum(t["ms"] for t in adj_times if t["kernel"] == "los_adjoint") / n_mg
+        grad_finite = bool(torch.isfinite(dgrid[0]).all()) and bool(torch.isfinite(dparams).all())
         grids_host = None
         if not args.no_cpu:
             grids_host = tuple(np.ascontiguousarray(a.cpu().numpy()) for a in (bx, by, bz, te))
@@ -318,6 +341,10 @@ def main():
             "accumulate_ms_per_step": acc_ms / args.steps,
             "random_field_generation_ms": gen_ms,
             "maps_finite": finite,
+            "map_gradient_evals_per_s": 1.0 / mg_s,
+            "map_gradient_ms": mg_s * 1e3,
+            "adjoint_march_ms": adj_ms,
+            "gradients_finite": grad_finite,
         },
     }
     if not args.no_cpu:

@@ -174,6 +174,7 @@ HAMX_MAX_FREQ = 4
 HAMX_PREC_F64 = 0
 HAMX_PREC_F32 = 1
 HAMX_BATCH_NO_TUNE = 4
+HAMX_BATCH_ADJOINT = 32
 
 
 class HamxBatchInst(C.Structure):
@@ -183,7 +184,12 @@ class HamxBatchInst(C.Structure):
                 ("ternd", C.c_void_p), ("box", HamxBox),
                 ("dm", C.c_void_p), ("fd", C.c_void_p),
                 ("is_", C.c_void_p * HAMX_MAX_FREQ), ("qs", C.c_void_p * HAMX_MAX_FREQ),
-                ("us", C.c_void_p * HAMX_MAX_FREQ)]
+                ("us", C.c_void_p * HAMX_MAX_FREQ),
+                ("adj_dm", C.c_void_p), ("adj_fd", C.c_void_p),
+                ("adj_is", C.c_void_p * HAMX_MAX_FREQ), ("adj_qs", C.c_void_p * HAMX_MAX_FREQ),
+                ("adj_us", C.c_void_p * HAMX_MAX_FREQ),
+                ("dparams", C.c_void_p), ("dbrnd_x", C.c_void_p), ("dbrnd_y", C.c_void_p),
+                ("dbrnd_z", C.c_void_p), ("dternd", C.c_void_p)]
 
 
 class HamxBatchInfo(C.Structure):
@@ -196,5 +202,6 @@ class HamxBatchInfo(C.Structure):
 _PROTOS["hamx_batch_create"] = (C.c_int, [C.c_void_p, C.POINTER(HamxBatchInst), C.c_uint32,
                                           C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)])
 _PROTOS["hamx_batch_run"] = (C.c_int, [C.c_void_p])
+_PROTOS["hamx_batch_run_adjoint"] = (C.c_int, [C.c_void_p])
 _PROTOS["hamx_batch_get_info"] = (C.c_int, [C.c_void_p, C.POINTER(HamxBatchInfo)])
 _PROTOS["hamx_batch_destroy"] = (None, [C.c_void_p])

@@ -581,6 +581,8 @@ struct hamx_batch {
   uint32_t variant = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraph_t adj_graph = nullptr; // reverse mode (HAMX_BATCH_ADJOINT)
+  hipGraphExec_t adj_exec = nullptr;
   hamx_batch_info info{};
 };
 
@@ -614,8 +616,14 @@ void release(hamx_batch *b) {
     (void)hipGraphExecDestroy(b->exec);
   if (b->graph)
     (void)hipGraphDestroy(b->graph);
+  if (b->adj_exec)
+    (void)hipGraphExecDestroy(b->adj_exec);
+  if (b->adj_graph)
+    (void)hipGraphDestroy(b->adj_graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->adj_exec = nullptr;
+  b->adj_graph = nullptr;
   for (void *p : b->allocs)
     (void)hipFree(p);
   b->allocs.clear();
@@ -760,6 +768,9 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
   *out = nullptr;
   if (precision != HAMX_PREC_F64 && precision != HAMX_PREC_F32)
     return set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: unknown precision %u", precision);
+  if (flags & HAMX_BATCH_ADJOINT)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "hamx_batch_create: reverse mode is not available in this build");
   if ((flags & (HAMX_IN_DEVICE | HAMX_OUT_DEVICE)) != (HAMX_IN_DEVICE | HAMX_OUT_DEVICE))
     return set_err(ctx, HAMX_ERR_ARG,
                    "hamx_batch_create: grids and maps must be device buffers "
@@ -1119,6 +1130,19 @@ extern "C" int hamx_batch_run(hamx_batch *b) {
   hamx_ctx *ctx = b->ctx;
   HAMX_HIP(ctx, hipSetDevice(ctx->device));
   HAMX_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+
+extern "C" int hamx_batch_run_adjoint(hamx_batch *b) {
+  if (!b)
+    return HAMX_ERR_ARG;
+  if (!b->adj_exec)
+    return set_err(b->ctx, HAMX_ERR_ARG,
+                   "hamx_batch_run_adjoint: the batch was created without HAMX_BATCH_ADJOINT");
+  hamx_ctx *ctx = b->ctx;
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  HAMX_HIP(ctx, hipGraphLaunch(b->adj_exec, ctx->stream));
   HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HAMX_OK;
 }

@@ -250,6 +250,15 @@ typedef struct hamx_batch_inst {
   hamx_box box;
   double *dm, *fd;            /* device output maps (nside_dm / nside_fd), or NULL */
   double *is[HAMX_MAX_FREQ], *qs[HAMX_MAX_FREQ], *us[HAMX_MAX_FREQ]; /* nside_sync */
+  /* Reverse mode (batches created with HAMX_BATCH_ADJOINT), device arrays:
+   * inputs dL/d(maps) of the same sizes as the maps above (NULL: zero) ... */
+  const double *adj_dm, *adj_fd;
+  const double *adj_is[HAMX_MAX_FREQ], *adj_qs[HAMX_MAX_FREQ], *adj_us[HAMX_MAX_FREQ];
+  /* ... and outputs: dL/d(b0, psi0, psi1, chi0, alpha, n0) (HAMX_NPARAM values)
+   * and dL/d(random grids) on box (NULL: not wanted).  Instances naming the
+   * same grid-gradient arrays get the sum of their gradients. */
+  double *dparams;
+  double *dbrnd_x, *dbrnd_y, *dbrnd_z, *dternd;
 } hamx_batch_inst;
 
 typedef struct hamx_batch hamx_batch;
@@ -271,12 +280,20 @@ typedef struct hamx_batch_info {
  * captures the chosen one in a HIP graph.  Input grids are re-read and
  * output maps overwritten by every hamx_batch_run; their addresses are
  * fixed at creation.  flags: HAMX_IN_DEVICE | HAMX_OUT_DEVICE (required),
- * plus HAMX_BATCH_NO_TUNE to skip the timing and take the first shape. */
+ * plus HAMX_BATCH_NO_TUNE to skip the timing and take the first shape, and
+ * HAMX_BATCH_ADJOINT to record the forward's tape on every replay and
+ * capture the reverse-mode graph as well. */
 #define HAMX_BATCH_NO_TUNE 4u
+#define HAMX_BATCH_ADJOINT 32u
 int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uint32_t n_inst,
                       uint32_t precision, uint32_t flags, hamx_batch **out);
 /* Replay the captured graph once on the context's stream.  Synchronous. */
 int hamx_batch_run(hamx_batch *b);
+/* Replay the reverse-mode graph: the gradients of every instance from its
+ * adj_* maps, on the tape of the last hamx_batch_run.  Shells run outermost
+ * first, each ray in one sweep from its far end, as hamx_los_adjoint.
+ * Synchronous.  No counterpart in the reference, which has no gradients. */
+int hamx_batch_run_adjoint(hamx_batch *b);
 int hamx_batch_get_info(const hamx_batch *b, hamx_batch_info *info);
 void hamx_batch_destroy(hamx_batch *b);
 
