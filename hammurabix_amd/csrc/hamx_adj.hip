@@ -33,24 +33,6 @@
 
 namespace hamx {
 
-// digamma(x): recurrence up to x >= 6, then the asymptotic series in 1/x^2
-// (Abramowitz & Stegun 6.3.18); reflection for x <= 0.
-static double digamma(double x) {
-  if (x <= 0.) {
-    const double pi = cgs::pi;
-    return digamma(1. - x) - pi / std::tan(pi * x);
-  }
-  double r = 0.;
-  while (x < 6.) {
-    r -= 1. / x;
-    x += 1.;
-  }
-  const double f = 1. / (x * x);
-  const double series =
-      f * (1. / 12. - f * (1. / 120. - f * (1. / 252. - f * (1. / 240. - f * (1. / 132.)))));
-  return r + std::log(x) - 0.5 / x - series;
-}
-
 struct AdjConst {
   double tc;        // temp_convert factor of I, Q, U
   double l2, two_l2;

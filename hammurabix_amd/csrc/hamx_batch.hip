@@ -70,6 +70,21 @@ struct F32Const {
   float hexp; // (1 - pw_e) / 2: the emissivity shape is prof * (B_perp^2)^hexp
 };
 
+// Reverse-mode state of one instance (device pointers into the plan).
+struct BAdj {
+  double *g_i[kMaxF], *g_q[kMaxF], *g_u[kMaxF]; // shell-map adjoints of the level
+  double *g_dm, *g_fd;
+  double *G;        // FD-map adjoint (nside_fd)
+  double *gfdin;    // dL/d(FD cache) per ray of the level
+  double *ckpt;     // FD map before each level (pass 0), n_shell * npix(fd)
+  double *ray_par;  // 6 parameter partials per ray, laid out as t_fd
+  void *ggrid;      // packed grid gradient {bx, by, bz, te} (float4 / double4)
+  double lt[kMaxF]; // d ln(jt_f)/d alpha without the ln|B_perp| term
+  double lp[kMaxF]; // d ln(jp_f)/d alpha without the ln|B_perp| term
+  double l2[kMaxF]; // lambda_f^2
+  double b0;        // LSA b0 [G]
+};
+
 struct BInst {
   Phys ph;                      // pass-0 physics (the last-listed frequency)
   FastConst fc[HAMX_MAX_SHELL]; // pass-0 specialised constants per shell
@@ -87,8 +102,11 @@ struct BInst {
   GridGeo geo;
   Healpix hfd;
   const double *fd_cache; // accumulated FD map of the inner shells
-  double *t_dm, *t_fd;
+  double *t_dm;
+  double *t_fd;                    // each ray's FD per level (the adjoint's tape)
+  uint64_t fd_off[HAMX_MAX_SHELL]; // first ray of level s in t_fd / ray_par
   double *t_i[kMaxF], *t_q[kMaxF], *t_u[kMaxF];
+  BAdj adj;
 };
 
 struct LevelArgs {
@@ -179,8 +197,7 @@ __device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
   if (lane == 0) {
     if (in.do_dm)
       in.t_dm[ipix] = dm * sa.delta;
-    if (in.do_fd)
-      in.t_fd[ipix] = carry;
+    in.t_fd[in.fd_off[s] + ipix] = carry;
 #pragma unroll
     for (int f = 0; f < kMaxF; ++f) {
       if (f < nf) {
@@ -194,8 +211,16 @@ __device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
 
 // ------------------------------------------------------------- FP32 ray --
 
+// One FP32 sample with the intermediates the reverse mode needs; the
+// forward march reads te, fdk, w, c2i and s2i only.
 struct S32 {
   float te, fdk, w, c2i, s2i;
+  float bx, by, bz, b_par, bper2, x, y, n2;
+  float te_raw;
+  bool lsa_in, te_in, cell_ok;
+  float cos_p, sin_p, sps, cps, sch, cch, lnr, tnh;
+  uint64_t idx;
+  float xd, yd, zd;
 };
 
 __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, uint64_t sy,
@@ -226,52 +251,66 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   const F32Const &c = in.f;
   float bx = 0.f, by = 0.f, bz = 0.f;
   // WMAP-LSA window decided in FP64 (breg_lsa.cc:9-26)
-  if (!(rr > (20. * cgs::kpc) || rr < (3. * cgs::kpc))) {
+  o.lsa_in = !(rr > (20. * cgs::kpc) || rr < (3. * cgs::kpc));
+  if (o.lsa_in) {
     const float xk = static_cast<float>(pos.x * inv_kpc);
     const float yk = static_cast<float>(pos.y * inv_kpc);
     const float inv_r = 1.f / rk;
-    const float cos_p = xk * inv_r, sin_p = yk * inv_r;
-    const float lnr = __logf(rk * 0.125f);
+    o.cos_p = xk * inv_r;
+    o.sin_p = yk * inv_r;
+    o.lnr = __logf(rk * 0.125f);
     const float e2 = __expf(-2.f * fabsf(zk));
-    const float tnh = copysignf((1.f - e2) / (1.f + e2), zk);
-    const float psi = c.psi0 + c.psi1 * lnr;
-    const float chi = c.chi0 * tnh;
-    const float sps = __sinf(psi), cps = __cosf(psi);
-    const float sch = __sinf(chi), cch = __cosf(chi);
-    const float bc = c.b0u * cch;
-    bx = bc * (sps * cos_p - cps * sin_p);
-    by = bc * (sps * sin_p + cps * cos_p);
-    bz = c.b0u * sch;
+    o.tnh = copysignf((1.f - e2) / (1.f + e2), zk);
+    const float psi = c.psi0 + c.psi1 * o.lnr;
+    const float chi = c.chi0 * o.tnh;
+    o.sps = __sinf(psi);
+    o.cps = __cosf(psi);
+    o.sch = __sinf(chi);
+    o.cch = __cosf(chi);
+    const float bc = c.b0u * o.cch;
+    bx = bc * (o.sps * o.cos_p - o.cps * o.sin_p);
+    by = bc * (o.sps * o.sin_p + o.cps * o.cos_p);
+    bz = c.b0u * o.sch;
   }
   float te = 0.f;
   {
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
-    if (!(length0(d) > ph.tereg[1]))
+    o.te_in = !(length0(d) > ph.tereg[1]);
+    if (o.te_in)
       te = c.n0;
   }
+  o.cell_ok = false;
   if (GRIDS) {
     Cell cell;
     if (grid_cell(in.geo, pos, cell)) {
+      o.cell_ok = true;
       const float4 *g = static_cast<const float4 *>(in.packed);
       const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
       float4 w1, w2;
-      const float zd = static_cast<float>(cell.zd), yd = static_cast<float>(cell.yd);
-      lerp_plane_f4(g, cell.idx, sy, zd, yd, w1);
-      lerp_plane_f4(g, cell.idx + sx, sy, zd, yd, w2);
-      const float xd = static_cast<float>(cell.xd), x1 = 1.f - xd;
-      bx += w1.x * x1 + w2.x * xd;
-      by += w1.y * x1 + w2.y * xd;
-      bz += w1.z * x1 + w2.z * xd;
-      te += w1.w * x1 + w2.w * xd;
+      o.idx = cell.idx;
+      o.zd = static_cast<float>(cell.zd);
+      o.yd = static_cast<float>(cell.yd);
+      o.xd = static_cast<float>(cell.xd);
+      lerp_plane_f4(g, cell.idx, sy, o.zd, o.yd, w1);
+      lerp_plane_f4(g, cell.idx + sx, sy, o.zd, o.yd, w2);
+      const float x1 = 1.f - o.xd;
+      bx += w1.x * x1 + w2.x * o.xd;
+      by += w1.y * x1 + w2.y * o.xd;
+      bz += w1.z * x1 + w2.z * o.xd;
+      te += w1.w * x1 + w2.w * o.xd;
     }
   }
+  o.te_raw = te;
   te = te > 0.f ? te : 0.f;
   o.te = te;
-  const float b_par = bx * lf.x + by * lf.y + bz * lf.z;
-  o.fdk = te * b_par;
+  o.bx = bx;
+  o.by = by;
+  o.bz = bz;
+  o.b_par = bx * lf.x + by * lf.y + bz * lf.z;
+  o.fdk = te * o.b_par;
   const float cx = by * lf.z - bz * lf.y, cy = bz * lf.x - bx * lf.z,
               cz = bx * lf.y - by * lf.x;
-  const float bper2 = cx * cx + cy * cy + cz * cz;
+  o.bper2 = cx * cx + cy * cy + cz * cz;
   float prof;
   if (ph.cre_model == HAMX_CRE_UNIF) {
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
@@ -282,16 +321,16 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     prof = __expf((c.R0k - rk) * c.inv_r0k) / (ch * ch);
   }
   if (PW1)
-    o.w = prof * bper2;
+    o.w = prof * o.bper2;
   else
-    o.w = prof * exp2f(c.hexp * __log2f(bper2));
-  const float x = -(ephf.x * bx + ephf.y * by + ephf.z * bz);
-  const float y = -(ethf.x * bx + ethf.y * by + ethf.z * bz);
-  const float n2 = x * x + y * y;
-  if (n2 > 0.f) {
-    const float inv = 1.f / n2;
-    o.c2i = (x * x - y * y) * inv;
-    o.s2i = 2.f * x * y * inv;
+    o.w = prof * exp2f(c.hexp * __log2f(o.bper2));
+  o.x = -(ephf.x * bx + ephf.y * by + ephf.z * bz);
+  o.y = -(ethf.x * bx + ethf.y * by + ethf.z * bz);
+  o.n2 = o.x * o.x + o.y * o.y;
+  if (o.n2 > 0.f) {
+    const float inv = 1.f / o.n2;
+    o.c2i = (o.x * o.x - o.y * o.y) * inv;
+    o.s2i = 2.f * o.x * o.y * inv;
   } else {
     o.c2i = 1.f;
     o.s2i = 0.f;
@@ -354,8 +393,7 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
   if (lane == 0) {
     if (in.do_dm)
       in.t_dm[ipix] = dms * sh.delta;
-    if (in.do_fd)
-      in.t_fd[ipix] = carry;
+    in.t_fd[in.fd_off[s] + ipix] = carry;
 #pragma unroll
     for (int f = 0; f < kMaxF; ++f) {
       if (f < nf) {
@@ -933,10 +971,17 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
       if (x.dm)
         fills.push_back(FillJob{x.dm, healpix_make(x.los.nside_dm).npix});
     }
-    if (in.do_fd) {
-      if ((rc = dev_alloc(b, max_npix * 8, &p)) != HAMX_OK)
+    {
+      uint64_t rays = 0; // every ray's FD of every level (the reverse mode's tape)
+      for (uint32_t s = 0; s < x.los.n_shell; ++s) {
+        in.fd_off[s] = rays;
+        rays += in.sh[s].hs.npix;
+      }
+      if ((rc = dev_alloc(b, rays * 8, &p)) != HAMX_OK)
         return fail(rc);
       in.t_fd = static_cast<double *>(p);
+    }
+    if (in.do_fd) {
       double *fdm = x.fd;
       if (!fdm) { // the FD cache needs the accumulated map even when not requested
         if ((rc = dev_alloc(b, in.hfd.npix * 8, &p)) != HAMX_OK)
@@ -1025,7 +1070,7 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
         j.hs = hs;
         j.n = 1;
         j.dst[0] = const_cast<double *>(in.fd_cache);
-        j.src[0] = in.t_fd;
+        j.src[0] = in.t_fd + in.fd_off[s];
         jobs.push_back(j);
       }
     }

@@ -10,6 +10,8 @@
 
 #include "hamx.h"
 #include "hamx_cgs.h"
+#include <cmath>
+
 #include "hamx_fields.h"
 #include "hamx_healpix.h"
 #include "hamx_internal.h"
@@ -306,6 +308,23 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
 }
 
 // ------------------------------------------------------------ host side --
+
+// digamma(x): recurrence up to x >= 6, then the asymptotic series in 1/x^2
+// (Abramowitz & Stegun 6.3.18); reflection for x <= 0.  d/dx ln Gamma(x),
+// for the spectral-index derivative of the emissivity Gamma factors.
+inline double digamma(double x) {
+  if (x <= 0.)
+    return digamma(1. - x) - cgs::pi / std::tan(cgs::pi * x);
+  double r = 0.;
+  while (x < 6.) {
+    r -= 1. / x;
+    x += 1.;
+  }
+  const double f = 1. / (x * x);
+  const double series =
+      f * (1. / 12. - f * (1. / 120. - f * (1. / 252. - f * (1. / 240. - f * (1. / 132.)))));
+  return r + std::log(x) - 0.5 / x - series;
+}
 bool same_box(const hamx_box &a, const hamx_box &b);
 int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what);
 inline uint64_t box_size(const hamx_box &b) { return b.nx * b.ny * b.nz; }
