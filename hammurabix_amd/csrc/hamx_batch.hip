@@ -405,9 +405,294 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
   }
 }
 
+// -------------------------------------------------------- reverse mode --
+//
+// One sweep per ray from its far end (as k_los_adj): the running FD is the
+// ray's total from the tape minus the samples beyond, and
+//   R_k = sum over passes f and samples j >= k of lambda_f^2 dL/dqui_{f,j}
+//         + dL/d(shell FD)
+// is carried the same way.  Each pass f adds its own phase term; the
+// emissivity shape w is shared, so dL/dw sums the passes' I and Q/U terms.
+
+__device__ inline ShellArgs shell_args_rev(const BInst &in, int s) {
+  ShellArgs sa = shell_args(in, s);
+  sa.fd_map = (in.do_fd && s > 0) ? in.adj.ckpt + static_cast<uint64_t>(s) * in.hfd.npix
+                                  : nullptr;
+  return sa;
+}
+
+template <typename T>
+__device__ inline void scatter8(T *g, uint64_t idx, uint64_t sx, uint64_t sy, T xd, T yd,
+                                T zd, T gx, T gy, T gz, T gt) {
+  const T wx[2] = {T(1) - xd, xd}, wy[2] = {T(1) - yd, yd}, wz[2] = {T(1) - zd, zd};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const T w = wx[a] * wy[b] * wz[c];
+        T *p = g + 4 * (idx + a * sx + b * sy + c);
+        unsafeAtomicAdd(p + 0, w * gx);
+        unsafeAtomicAdd(p + 1, w * gy);
+        unsafeAtomicAdd(p + 2, w * gz);
+        unsafeAtomicAdd(p + 3, w * gt);
+      }
+}
+
+template <bool GRIDS, bool PW1>
+__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
+  const ShellArgs sa = shell_args_rev(in, s);
+  const BShell &sh = in.sh[s];
+  const BAdj &A = in.adj;
+  RayGeo r;
+  ray_setup_call(in.ph, sa, ipix, r);
+  const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
+                                static_cast<float>(r.l.z));
+  const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
+                                  static_cast<float>(r.eth.z));
+  const float3 ephf = make_float3(static_cast<float>(r.eph.x), static_cast<float>(r.eph.y),
+                                  static_cast<float>(r.eph.z));
+  const int nf = in.n_freq;
+  const double delta = sh.delta;
+  // per-ray adjoint constants: dL/dw from I, and the Q / U weights of each pass
+  double aI = 0., AIL = 0.;
+  float aQ[kMaxF] = {0.f, 0.f, 0.f, 0.f}, aU[kMaxF] = {0.f, 0.f, 0.f, 0.f};
+  float lpf[kMaxF] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int f = 0; f < kMaxF; ++f) {
+    if (f < nf) {
+      const double gi = A.g_i[f][ipix] * in.st[f] * delta;
+      aI += gi;
+      AIL += gi * A.lt[f];
+      aQ[f] = static_cast<float>(A.g_q[f][ipix] * in.sp[f] * delta);
+      aU[f] = static_cast<float>(A.g_u[f][ipix] * in.sp[f] * delta);
+      lpf[f] = static_cast<float>(A.lp[f]);
+    }
+  }
+  const double gdm = in.do_dm ? A.g_dm[ipix] : 0.;
+  const double gfd = in.do_fd ? A.g_fd[ipix] : 0.;
+  const double fd_total = in.t_fd[in.fd_off[s] + ipix];
+  const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
+  double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0., q0 = 0.;
+  double after = 0., d_after = 0.;
+  const uint64_t n = r.k_hi - r.k_lo;
+  for (uint64_t c = (n + 63) / 64; c-- > 0;) {
+    const uint64_t k = r.k_lo + 64 * c + lane;
+    S32 o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
+    const double incl = wave_incl_scan(valid ? static_cast<double>(o.fdk) * sh.fdu : 0., lane);
+    const double chunk = __shfl(incl, 63, 64);
+    const double fd_here = (fd_total - after - chunk) + incl;
+    after += chunk;
+    double D = 0.;
+    float dLdw = static_cast<float>(aI), alpha_p = 0.f, dipa = 0.f;
+    if (valid) {
+#pragma unroll
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+          double rv = X * in.rev[f];
+          rv -= rint(rv);
+          const float a = static_cast<float>(rv);
+          const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
+          const float c2 = cs * o.c2i - sn * o.s2i, s2 = sn * o.c2i + cs * o.s2i;
+          const float P = aQ[f] * c2 + aU[f] * s2;
+          dLdw += P;
+          alpha_p += P * lpf[f];
+          const float dq = 2.f * o.w * (aU[f] * c2 - aQ[f] * s2);
+          dipa += dq;
+          D += A.l2[f] * static_cast<double>(dq);
+          if (f == 0)
+            q0 += dq;
+        }
+      }
+    }
+    const double dscan = wave_suffix_scan(D, lane);
+    const double R = dscan + d_after + gfd; // dL/d(FD contribution of this sample)
+    d_after += __shfl(dscan, 0, 64);
+    if (!valid)
+      continue;
+    const double dte = o.te_raw > 0.f ? R * static_cast<double>(o.b_par) * sh.fdu + gdm * delta
+                                      : 0.;
+    const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
+    float dbx = ft * lf.x, dby = ft * lf.y, dbz = ft * lf.z;
+    if (o.bper2 > 0.f) {
+      const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
+      dbx += coef * (o.bx - o.b_par * lf.x);
+      dby += coef * (o.by - o.b_par * lf.y);
+      dbz += coef * (o.bz - o.b_par * lf.z);
+      // ln |B_perp| in Gauss: the B grid and b0 are in muG here
+      const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
+      p4 += static_cast<double>(o.w) *
+            (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
+    }
+    if (o.n2 > 0.f) {
+      const float cc = dipa / o.n2;
+      dbx += cc * (o.y * ephf.x - o.x * ethf.x);
+      dby += cc * (o.y * ephf.y - o.x * ethf.y);
+      dbz += cc * (o.y * ephf.z - o.x * ethf.z);
+    }
+    if (o.te_in)
+      p5 += dte;
+    if (o.lsa_in) {
+      const float u = o.sps * o.cos_p - o.cps * o.sin_p;
+      const float v = o.sps * o.sin_p + o.cps * o.cos_p;
+      p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
+      const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
+      p1 += gpsi;
+      p2 += gpsi * o.lnr;
+      const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
+      p3 += gchi * o.tnh;
+    }
+    if (GRIDS && o.cell_ok && A.ggrid)
+      scatter8<float>(static_cast<float *>(A.ggrid), o.idx, sx, sy, o.xd, o.yd, o.zd, dbx, dby,
+                      dbz, static_cast<float>(dte));
+  }
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  p2 = wave_sum(p2);
+  p3 = wave_sum(p3);
+  p4 = wave_sum(p4);
+  p5 = wave_sum(p5);
+  q0 = wave_sum(q0);
+  if (lane == 0) {
+    double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
+    rp[0] = p0;
+    rp[1] = p1;
+    rp[2] = p2;
+    rp[3] = p3;
+    rp[4] = p4;
+    rp[5] = p5;
+    if (sa.fd_map && in.fdin[0])
+      A.gfdin[ipix] = A.l2[0] * q0;
+  }
+}
+
+template <bool GRIDS, bool PW1>
+__device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
+  const ShellArgs sa = shell_args_rev(in, s);
+  const BAdj &A = in.adj;
+  Grids gr;
+  gr.packed = static_cast<const double4 *>(in.packed);
+  gr.g_packed = in.geo;
+  RayGeo r;
+  ray_setup_call(in.ph, sa, ipix, r);
+  const FastConst &fc = in.fc[s];
+  const int nf = in.n_freq;
+  // per-ray constants: dL/d(pass-0 jt) and the Q / U weights of each pass
+  double aJt = 0., AIL = 0., aQ[kMaxF] = {0., 0., 0., 0.}, aU[kMaxF] = {0., 0., 0., 0.};
+#pragma unroll
+  for (int f = 0; f < kMaxF; ++f) {
+    if (f < nf) {
+      const double gi = A.g_i[f][ipix] * in.st[f];
+      aJt += gi;
+      AIL += gi * A.lt[f];
+      aQ[f] = A.g_q[f][ipix] * in.sp[f];
+      aU[f] = A.g_u[f][ipix] * in.sp[f];
+    }
+  }
+  const double gdm = in.do_dm ? A.g_dm[ipix] : 0.;
+  const double gfd = in.do_fd ? A.g_fd[ipix] : 0.;
+  const double fd_total = in.t_fd[in.fd_off[s] + ipix];
+  const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
+  const double alpha1 = 1. - in.ph.idx;
+  double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0., q0 = 0.;
+  double after = 0., d_after = 0.;
+  const uint64_t n = r.k_hi - r.k_lo;
+  for (uint64_t c = (n + 63) / 64; c-- > 0;) {
+    const uint64_t k = r.k_lo + 64 * c + lane;
+    FastSample o;
+    bool valid = false;
+    if (k < r.k_hi)
+      valid = eval_fast<GRIDS, PW1>(in.ph, fc, gr, sa, r, k, o);
+    const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
+    const double chunk = __shfl(incl, 63, 64);
+    const double fd_here = (fd_total - after - chunk) + incl;
+    after += chunk;
+    double D = 0., dJp = 0., alpha_p = 0., dipa = 0.;
+    if (valid) {
+#pragma unroll
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+          double sph, cph;
+          sincos(in.two_l2[f] * X, &sph, &cph);
+          const double c2 = cph * o.c2i - sph * o.s2i, s2 = sph * o.c2i + cph * o.s2i;
+          const double P = aQ[f] * c2 + aU[f] * s2;
+          dJp += P;
+          alpha_p += P * A.lp[f];
+          const double dq = 2. * o.jp * (aU[f] * c2 - aQ[f] * s2);
+          dipa += dq;
+          D += A.l2[f] * dq;
+          if (f == 0)
+            q0 += dq;
+        }
+      }
+    }
+    const double dscan = wave_suffix_scan(D, lane);
+    const double R = dscan + d_after + gfd;
+    d_after += __shfl(dscan, 0, 64);
+    if (!valid)
+      continue;
+    const double dte = o.te_raw > 0. ? R * o.b_par * fc.fd_scale + gdm * sa.delta : 0.;
+    const double ft = R * o.te * fc.fd_scale;
+    Vec3 dB{ft * r.l.x, ft * r.l.y, ft * r.l.z};
+    if (o.bper2 > 0.) {
+      const double coef = (aJt * o.jt + dJp * o.jp) * alpha1 / (2. * o.bper2);
+      dB.x += coef * (o.B.x - o.b_par * r.l.x);
+      dB.y += coef * (o.B.y - o.b_par * r.l.y);
+      dB.z += coef * (o.B.z - o.b_par * r.l.z);
+      const double lnb = 0.5 * log(o.bper2);
+      p4 += o.jt * (AIL + aJt * lnb) + o.jp * (alpha_p + dJp * lnb);
+    }
+    if (o.n2 > 0.) {
+      const double cc = dipa / o.n2;
+      dB.x += cc * (o.y * r.eph.x - o.x * r.eth.x);
+      dB.y += cc * (o.y * r.eph.y - o.x * r.eth.y);
+      dB.z += cc * (o.y * r.eph.z - o.x * r.eth.z);
+    }
+    if (o.te_in)
+      p5 += dte;
+    if (o.lsa_in) {
+      const double u = o.sps * o.cos_p - o.cps * o.sin_p;
+      const double v = o.sps * o.sin_p + o.cps * o.cos_p;
+      p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
+      const double gpsi = A.b0 * o.cch * (dB.x * v - dB.y * u);
+      p1 += gpsi;
+      p2 += gpsi * o.lnr;
+      const double gchi = A.b0 * (o.cch * dB.z - o.sch * (dB.x * u + dB.y * v));
+      p3 += gchi * o.tnh;
+    }
+    if (GRIDS && o.cell_ok && A.ggrid)
+      scatter8<double>(static_cast<double *>(A.ggrid), o.cell.idx, sx, sy, o.cell.xd,
+                       o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte);
+  }
+  p0 = wave_sum(p0);
+  p1 = wave_sum(p1);
+  p2 = wave_sum(p2);
+  p3 = wave_sum(p3);
+  p4 = wave_sum(p4);
+  p5 = wave_sum(p5);
+  q0 = wave_sum(q0);
+  if (lane == 0) {
+    double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
+    rp[0] = p0;
+    rp[1] = p1;
+    rp[2] = p2;
+    rp[3] = p3;
+    rp[4] = p4;
+    rp[5] = p5;
+    if (sa.fd_map && in.fdin[0])
+      A.gfdin[ipix] = A.l2[0] * q0;
+  }
+}
+
 // ---------------------------------------------------------- work items --
 
-template <int PREC, bool GRIDS, bool PW1>
+template <int PREC, bool GRIDS, bool PW1, bool REV>
 __device__ void do_item(const LevelArgs &a, uint64_t item, int lane) {
   uint32_t lo = 0, hi = a.n_ids;
   while (hi - lo > 1) {
@@ -424,16 +709,24 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane) {
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
   const uint64_t p1 = (p0 + a.rays < npix) ? p0 + a.rays : npix;
   for (uint64_t p = p0; p < p1; ++p) {
-    if (PREC == HAMX_PREC_F32)
-      march_ray_f32<GRIDS, PW1>(in, s, p, lane);
-    else
-      march_ray_f64<GRIDS, PW1>(in, s, p, lane);
+    if (REV) {
+      if (PREC == HAMX_PREC_F32)
+        adj_ray_f32<GRIDS, PW1>(in, s, p, lane);
+      else
+        adj_ray_f64<GRIDS, PW1>(in, s, p, lane);
+    } else {
+      if (PREC == HAMX_PREC_F32)
+        march_ray_f32<GRIDS, PW1>(in, s, p, lane);
+      else
+        march_ray_f64<GRIDS, PW1>(in, s, p, lane);
+    }
   }
 }
 
 // One launch covers the instances of one shell level that share the
-// template case (random grids present, emissivity exponent -1).
-template <int PREC, bool GRIDS, bool PW1>
+// template case (random grids present, emissivity exponent -1); REV
+// selects the reverse-mode sweep.
+template <int PREC, bool GRIDS, bool PW1, bool REV>
 __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
@@ -445,7 +738,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
     }
     const uint64_t item = b * wpb + wave;
     if (item < a.n_items)
-      do_item<PREC, GRIDS, PW1>(a, uniform_u64(item), lane);
+      do_item<PREC, GRIDS, PW1, REV>(a, uniform_u64(item), lane);
     return;
   }
   const uint32_t q0 = a.xcd ? (blockIdx.x % a.nq) : 0;
@@ -459,7 +752,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
       const uint64_t item = a.qbeg[q] + t;
       if (item >= a.qend[q])
         break; // every wave leaves once all queues are drained
-      do_item<PREC, GRIDS, PW1>(a, item, lane);
+      do_item<PREC, GRIDS, PW1, REV>(a, item, lane);
     }
   }
 }
@@ -503,6 +796,142 @@ __global__ __launch_bounds__(256) void k_batch_acc(const AccJob *jobs) {
     const double v = (wtot == 0.0) ? cgs::undef : res / wtot;
     J.dst[m][i] = J.dst[m][i] + v;
   }
+}
+
+// dst (shell adjoint, nside_s) += A^T(src) (output adjoint): the transpose
+// of k_batch_acc, with atomic scatter for the interpolating case.
+__global__ __launch_bounds__(256) void k_batch_acc_t(const AccJob *jobs) {
+  const AccJob &J = jobs[blockIdx.y];
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= J.hd.npix)
+    return;
+  if (J.hd.npix == J.hs.npix) {
+    for (int m = 0; m < J.n; ++m)
+      J.dst[m][i] += J.src[m][i];
+    return;
+  }
+  double theta, phi;
+  hp_pix2ang(J.hd, i, theta, phi);
+  uint64_t pix[4];
+  double w[4];
+  hp_interpol_weights(J.hs, theta, phi, pix, w);
+  const double wtot = ((w[0] + w[1]) + w[2]) + w[3];
+  for (int m = 0; m < J.n; ++m) {
+    const double g = J.src[m][i] / wtot;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      unsafeAtomicAdd(&J.dst[m][pix[q]], w[q] * g);
+  }
+}
+
+// G (FD-map adjoint) += Interp_s^T(gfdin): the transpose of the FD-cache
+// read of level s.
+struct FdtJob {
+  double *G;
+  const double *gfdin;
+  Healpix hfd, hs;
+};
+
+__global__ __launch_bounds__(256) void k_batch_fdcache_t(const FdtJob *jobs) {
+  const FdtJob &J = jobs[blockIdx.y];
+  const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= J.hs.npix)
+    return;
+  const double g = J.gfdin[p];
+  if (g == 0.)
+    return;
+  double theta, phi;
+  hp_pix2ang(J.hs, p, theta, phi);
+  uint64_t pix[4];
+  double w[4];
+  hp_interpol_weights(J.hfd, theta, phi, pix, w);
+  const double wtot = ((w[0] + w[1]) + w[2]) + w[3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    unsafeAtomicAdd(&J.G[pix[q]], w[q] * (g / wtot));
+}
+
+// Per-instance sums of the 6 per-ray parameter partials into out[0..5].
+struct RedJob {
+  const double *ray_par;
+  uint64_t n; // rays
+  double *out;
+};
+
+__global__ __launch_bounds__(256) void k_batch_reduce(const RedJob *jobs) {
+  __shared__ double sm[6][4];
+  const RedJob &J = jobs[blockIdx.y];
+  double v[6] = {0., 0., 0., 0., 0., 0.};
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < J.n;
+       i += stride)
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      v[j] += J.ray_par[6 * i + j];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    v[j] = wave_sum(v[j]);
+    if ((threadIdx.x & 63) == 0)
+      sm[j][threadIdx.x >> 6] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6)
+    unsafeAtomicAdd(&J.out[threadIdx.x], ((sm[threadIdx.x][0] + sm[threadIdx.x][1]) +
+                                          sm[threadIdx.x][2]) + sm[threadIdx.x][3]);
+}
+
+// Packed {bx, by, bz, te} gradient -> the caller's SoA arrays (written, not
+// added); bscale converts a muG-unit FP32 gradient to per-Gauss.
+struct UnpackJob {
+  const void *g;
+  double *x, *y, *z, *t;
+  uint64_t n;
+  double bscale;
+  int f32, pad;
+};
+
+__global__ __launch_bounds__(256) void k_batch_unpack(const UnpackJob *jobs) {
+  const UnpackJob &J = jobs[blockIdx.y];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < J.n;
+       i += stride) {
+    double gx, gy, gz, gt;
+    if (J.f32) {
+      const float4 v = static_cast<const float4 *>(J.g)[i];
+      gx = v.x;
+      gy = v.y;
+      gz = v.z;
+      gt = v.w;
+    } else {
+      const double4 v = static_cast<const double4 *>(J.g)[i];
+      gx = v.x;
+      gy = v.y;
+      gz = v.z;
+      gt = v.w;
+    }
+    if (J.x)
+      J.x[i] = gx * J.bscale;
+    if (J.y)
+      J.y[i] = gy * J.bscale;
+    if (J.z)
+      J.z[i] = gz * J.bscale;
+    if (J.t)
+      J.t[i] = gt;
+  }
+}
+
+struct CopyJob {
+  double *dst;
+  const double *src;
+  uint64_t n;
+};
+
+__global__ __launch_bounds__(256) void k_batch_copy(const CopyJob *jobs) {
+  const CopyJob &J = jobs[blockIdx.y];
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < J.n;
+       i += stride)
+    J.dst[i] = J.src[i];
 }
 
 struct FillJob {
@@ -550,15 +979,27 @@ __global__ __launch_bounds__(256) void k_batch_pack(const PackJob *jobs) {
 
 using MarchFn = void (*)(LevelArgs);
 
-// Kernel of a launch group: case c = 2 * grids + pw1.
-MarchFn march_kernel(uint32_t prec, int c) {
-  static const MarchFn f32[4] = {
-      k_batch_march<HAMX_PREC_F32, false, false>, k_batch_march<HAMX_PREC_F32, false, true>,
-      k_batch_march<HAMX_PREC_F32, true, false>, k_batch_march<HAMX_PREC_F32, true, true>};
-  static const MarchFn f64[4] = {
-      k_batch_march<HAMX_PREC_F64, false, false>, k_batch_march<HAMX_PREC_F64, false, true>,
-      k_batch_march<HAMX_PREC_F64, true, false>, k_batch_march<HAMX_PREC_F64, true, true>};
-  return prec == HAMX_PREC_F32 ? f32[c] : f64[c];
+// Kernel of a launch group: case c = 2 * grids + pw1, forward or reverse.
+MarchFn march_kernel(uint32_t prec, int c, bool rev = false) {
+  static const MarchFn f32[2][4] = {
+      {k_batch_march<HAMX_PREC_F32, false, false, false>,
+       k_batch_march<HAMX_PREC_F32, false, true, false>,
+       k_batch_march<HAMX_PREC_F32, true, false, false>,
+       k_batch_march<HAMX_PREC_F32, true, true, false>},
+      {k_batch_march<HAMX_PREC_F32, false, false, true>,
+       k_batch_march<HAMX_PREC_F32, false, true, true>,
+       k_batch_march<HAMX_PREC_F32, true, false, true>,
+       k_batch_march<HAMX_PREC_F32, true, true, true>}};
+  static const MarchFn f64[2][4] = {
+      {k_batch_march<HAMX_PREC_F64, false, false, false>,
+       k_batch_march<HAMX_PREC_F64, false, true, false>,
+       k_batch_march<HAMX_PREC_F64, true, false, false>,
+       k_batch_march<HAMX_PREC_F64, true, true, false>},
+      {k_batch_march<HAMX_PREC_F64, false, false, true>,
+       k_batch_march<HAMX_PREC_F64, false, true, true>,
+       k_batch_march<HAMX_PREC_F64, true, false, true>,
+       k_batch_march<HAMX_PREC_F64, true, true, true>}};
+  return prec == HAMX_PREC_F32 ? f32[rev][c] : f64[rev][c];
 }
 
 struct Variant {
@@ -621,6 +1062,17 @@ struct hamx_batch {
   hipGraphExec_t exec = nullptr;
   hipGraph_t adj_graph = nullptr; // reverse mode (HAMX_BATCH_ADJOINT)
   hipGraphExec_t adj_exec = nullptr;
+  // reverse-mode job lists
+  bool adjoint = false;
+  int per_cu_rev[4][5] = {};
+  unsigned long long *d_actr = nullptr; // tickets of the reverse launches
+  struct Jobs {
+    void *d = nullptr;
+    uint32_t n = 0;
+    uint64_t max = 0; // largest job, elements
+  };
+  Jobs a_init_fill, a_init_copy, a_red, a_unpack;
+  std::vector<Jobs> a_zero, a_acct, a_fdt, f_ckpt; // per level
   hamx_batch_info info{};
 };
 
@@ -719,6 +1171,84 @@ void plan_level(const hamx_batch *b, uint32_t group, uint32_t rays, std::vector<
   la.qend[kQueues - 1] = la.n_items;
 }
 
+template <typename K, typename J>
+int launch_jobs(hamx_ctx *ctx, K kernel, const hamx_batch::Jobs &jobs, uint64_t cap) {
+  if (jobs.n == 0)
+    return HAMX_OK;
+  uint64_t blocks = (jobs.max + 255) / 256;
+  blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), cap);
+  hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(blocks), jobs.n), dim3(256), 0,
+                     ctx->stream, static_cast<const J *>(jobs.d));
+  HAMX_HIP(ctx, hipGetLastError());
+  return HAMX_OK;
+}
+
+unsigned march_blocks(const hamx_batch *b, const Variant &V, const LevelArgs &la, int c,
+                      bool rev) {
+  if (V.persistent)
+    return static_cast<unsigned>(b->n_cu * (rev ? b->per_cu_rev : b->per_cu)[c][V.waves_per_block]);
+  uint64_t nb = (la.n_items + V.waves_per_block - 1) / V.waves_per_block;
+  if (V.xcd)
+    nb = (nb + kQueues - 1) / kQueues * kQueues;
+  return static_cast<unsigned>(nb);
+}
+
+// The reverse-mode sequence: zero the gradients, then per level from the
+// outermost: shell-map adjoints (A^T), the reverse march, and the FD-cache
+// transpose into the FD-map adjoint; finally the parameter sums and the
+// grid-gradient unpack.
+int launch_adjoint(hamx_batch *b, uint32_t v) {
+  hamx_ctx *ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const Variant &V = kVariants[v];
+  {
+    uint64_t blocks = (std::max<uint64_t>(b->a_init_fill.max, 1) + 255) / 256;
+    blocks = std::min<uint64_t>(blocks, 1024);
+    hipLaunchKernelGGL(k_batch_zero, dim3(static_cast<unsigned>(blocks), b->a_init_fill.n),
+                       dim3(256), 0, st, static_cast<const FillJob *>(b->a_init_fill.d),
+                       b->d_actr, b->levels * 4 * kQueues);
+    HAMX_HIP(ctx, hipGetLastError());
+  }
+  HAMX_TRY((launch_jobs<decltype(&k_batch_copy), CopyJob>(ctx, k_batch_copy, b->a_init_copy,
+                                                           1024)));
+  const unsigned threads = 64u * V.waves_per_block;
+  for (uint32_t s = b->levels; s-- > 0;) {
+    if (b->a_zero[s].n) {
+      uint64_t blocks = std::min<uint64_t>((b->a_zero[s].max + 255) / 256, 1024);
+      hipLaunchKernelGGL(k_batch_zero, dim3(static_cast<unsigned>(blocks), b->a_zero[s].n),
+                         dim3(256), 0, st, static_cast<const FillJob *>(b->a_zero[s].d),
+                         static_cast<unsigned long long *>(nullptr), 0u);
+      HAMX_HIP(ctx, hipGetLastError());
+    }
+    if (b->a_acct[s].n) {
+      const unsigned ab = static_cast<unsigned>((b->a_acct[s].max + 255) / 256);
+      hipLaunchKernelGGL(k_batch_acc_t, dim3(ab, b->a_acct[s].n), dim3(256), 0, st,
+                         static_cast<const AccJob *>(b->a_acct[s].d));
+      HAMX_HIP(ctx, hipGetLastError());
+    }
+    for (int c = 0; c < 4; ++c) {
+      LevelArgs arg = b->largs[v][s * 4 + c];
+      if (arg.n_items == 0)
+        continue;
+      arg.ctr = b->d_actr + static_cast<size_t>(s * 4 + c) * kQueues;
+      void *args[] = {&arg};
+      HAMX_HIP(ctx, hipLaunchKernel(reinterpret_cast<const void *>(march_kernel(b->prec, c, true)),
+                                    dim3(march_blocks(b, V, arg, c, true)), dim3(threads), args,
+                                    0, st));
+    }
+    if (b->a_fdt[s].n) {
+      const unsigned fb = static_cast<unsigned>((b->a_fdt[s].max + 255) / 256);
+      hipLaunchKernelGGL(k_batch_fdcache_t, dim3(fb, b->a_fdt[s].n), dim3(256), 0, st,
+                         static_cast<const FdtJob *>(b->a_fdt[s].d));
+      HAMX_HIP(ctx, hipGetLastError());
+    }
+  }
+  HAMX_TRY((launch_jobs<decltype(&k_batch_reduce), RedJob>(ctx, k_batch_reduce, b->a_red, 64)));
+  HAMX_TRY((launch_jobs<decltype(&k_batch_unpack), UnpackJob>(ctx, k_batch_unpack, b->a_unpack,
+                                                               2048)));
+  return HAMX_OK;
+}
+
 int launch_all(hamx_batch *b, uint32_t v) {
   hamx_ctx *ctx = b->ctx;
   hipStream_t st = ctx->stream;
@@ -741,19 +1271,14 @@ int launch_all(hamx_batch *b, uint32_t v) {
   }
   const unsigned threads = 64u * V.waves_per_block;
   for (uint32_t s = 0; s < b->levels; ++s) {
+    if (b->adjoint) // the FD map before this level, for the reverse mode
+      HAMX_TRY((launch_jobs<decltype(&k_batch_copy), CopyJob>(ctx, k_batch_copy, b->f_ckpt[s],
+                                                               1024)));
     for (int c = 0; c < 4; ++c) {
       const LevelArgs &la = b->largs[v][s * 4 + c];
       if (la.n_items == 0)
         continue;
-      unsigned blocks;
-      if (V.persistent) {
-        blocks = static_cast<unsigned>(b->n_cu * b->per_cu[c][V.waves_per_block]);
-      } else {
-        uint64_t nb = (la.n_items + V.waves_per_block - 1) / V.waves_per_block;
-        if (V.xcd)
-          nb = (nb + kQueues - 1) / kQueues * kQueues;
-        blocks = static_cast<unsigned>(nb);
-      }
+      const unsigned blocks = march_blocks(b, V, la, c, false);
       LevelArgs arg = la;
       void *args[] = {&arg};
       HAMX_HIP(ctx, hipLaunchKernel(reinterpret_cast<const void *>(march_kernel(b->prec, c)),
@@ -806,9 +1331,6 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
   *out = nullptr;
   if (precision != HAMX_PREC_F64 && precision != HAMX_PREC_F32)
     return set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: unknown precision %u", precision);
-  if (flags & HAMX_BATCH_ADJOINT)
-    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
-                   "hamx_batch_create: reverse mode is not available in this build");
   if ((flags & (HAMX_IN_DEVICE | HAMX_OUT_DEVICE)) != (HAMX_IN_DEVICE | HAMX_OUT_DEVICE))
     return set_err(ctx, HAMX_ERR_ARG,
                    "hamx_batch_create: grids and maps must be device buffers "
@@ -827,6 +1349,7 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     return set_err(ctx, HAMX_ERR_NOMEM, "hamx_batch_create: host allocation failed");
   b->ctx = ctx;
   b->prec = precision;
+  b->adjoint = (flags & HAMX_BATCH_ADJOINT) != 0;
   int rc = HAMX_OK;
   auto fail = [&](int code) {
     release(b);
@@ -846,6 +1369,11 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
                 static_cast<int>(64 * w), 0) != hipSuccess)
           return fail(set_err(ctx, HAMX_ERR_DEVICE, "occupancy query failed"));
         b->per_cu[c][w] = per_cu < 1 ? 1 : per_cu;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void *>(march_kernel(precision, c, true)),
+                static_cast<int>(64 * w), 0) != hipSuccess)
+          return fail(set_err(ctx, HAMX_ERR_DEVICE, "occupancy query failed"));
+        b->per_cu_rev[c][w] = per_cu < 1 ? 1 : per_cu;
       }
   }
   b->host_inst.resize(n_inst);
@@ -858,6 +1386,18 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     void *packed;
   };
   std::vector<GridKey> grid_keys;
+  // reverse mode: grid-gradient buffers shared by instances naming the same
+  // output arrays, and the one-off jobs of every replay
+  struct GradKey {
+    double *x, *y, *z, *t;
+    hamx_box box;
+    void *ggrid;
+  };
+  std::vector<GradKey> grad_keys;
+  std::vector<FillJob> a_fills;
+  std::vector<CopyJob> a_copies;
+  std::vector<RedJob> a_reds;
+  std::vector<UnpackJob> a_unpacks;
   uint64_t samples_geo = 0, samples_ref = 0;
   for (uint32_t i = 0; i < n_inst; ++i) {
     const hamx_batch_inst &x = inst[i];
@@ -888,6 +1428,7 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     const FastConst raw0 = make_fast(&d0, in.ph, 1.0, false);
     const FastConst pw0 = make_fast(&d0, in.ph, 1.0, pw1);
     const double pw_e = raw0.pw_e;
+    double a_f[kMaxF] = {0., 0., 0., 0.}, l2_f[kMaxF] = {0., 0., 0., 0.};
     for (uint32_t f = 0; f < K; ++f) {
       hamx_los_desc df = d0;
       df.sync_freq = x.freq[K - 1 - f];
@@ -895,6 +1436,8 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
       const FastConst raw = make_fast(&df, pf, 1.0, false);
       const FastConst pwf = make_fast(&df, pf, 1.0, pw1);
       const double tc = pf.tc_em1 * pf.tc_em1 / pf.tc_den;
+      a_f[f] = raw.a_coef;
+      l2_f[f] = pf.lambda_sq;
       in.two_l2[f] = 2. * pf.lambda_sq;
       in.rev[f] = pf.lambda_sq / cgs::pi;
       in.fdin[f] = (f == 0 && x.los.do_fd) ? 1 : 0;
@@ -1003,6 +1546,87 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
       fills.push_back(FillJob{x.qs[K - 1 - f], h_sync.npix});
       fills.push_back(FillJob{x.us[K - 1 - f], h_sync.npix});
     }
+    if (b->adjoint) {
+      BAdj &A = in.adj;
+      auto alloc_d = [&](uint64_t n, double **dst) -> int {
+        void *q;
+        const int r2 = dev_alloc(b, n * 8, &q);
+        *dst = static_cast<double *>(q);
+        return r2;
+      };
+      for (uint32_t f = 0; f < K; ++f)
+        if ((rc = alloc_d(max_npix, &A.g_i[f])) != HAMX_OK ||
+            (rc = alloc_d(max_npix, &A.g_q[f])) != HAMX_OK ||
+            (rc = alloc_d(max_npix, &A.g_u[f])) != HAMX_OK)
+          return fail(rc);
+      if (in.do_dm && (rc = alloc_d(max_npix, &A.g_dm)) != HAMX_OK)
+        return fail(rc);
+      if (in.do_fd) {
+        if ((rc = alloc_d(max_npix, &A.g_fd)) != HAMX_OK ||
+            (rc = alloc_d(in.hfd.npix, &A.G)) != HAMX_OK ||
+            (rc = alloc_d(max_npix, &A.gfdin)) != HAMX_OK ||
+            (rc = alloc_d(x.los.n_shell * in.hfd.npix, &A.ckpt)) != HAMX_OK)
+          return fail(rc);
+        if (x.adj_fd)
+          a_copies.push_back(CopyJob{A.G, x.adj_fd, in.hfd.npix});
+        else
+          a_fills.push_back(FillJob{A.G, in.hfd.npix});
+      }
+      const uint32_t last = x.los.n_shell - 1;
+      const uint64_t rays = in.fd_off[last] + in.sh[last].hs.npix;
+      if ((rc = alloc_d(6 * rays, &A.ray_par)) != HAMX_OK)
+        return fail(rc);
+      double *dpar = x.dparams;
+      if (!dpar && (rc = alloc_d(HAMX_NPARAM, &dpar)) != HAMX_OK)
+        return fail(rc);
+      a_fills.push_back(FillJob{dpar, HAMX_NPARAM});
+      a_reds.push_back(RedJob{A.ray_par, rays, dpar});
+      // d ln(j_f)/d alpha: ln gamma0 - ln(a_f)/2 plus the Gamma factors' digammas
+      const double alpha = -in.ph.idx;
+      const double E0 = x.los.cre_model == HAMX_CRE_UNIF ? x.los.cre[2] : x.los.cre[5];
+      const double ln_g0 = std::log(E0 / cgs::mec2);
+      const double psi_t = 0.25 * digamma(alpha / 4. + 19. / 12.);
+      const double psi_p = 0.25 * digamma(alpha / 4. + 7. / 12.);
+      const double psi_c = 0.25 * digamma(alpha / 4. - 1. / 12.);
+      for (uint32_t f = 0; f < K; ++f) {
+        const double base = ln_g0 - 0.5 * std::log(a_f[f]);
+        A.lt[f] = base - 1. / (1. + alpha) + psi_t + psi_c;
+        A.lp[f] = base + psi_p + psi_c;
+        A.l2[f] = l2_f[f];
+      }
+      A.b0 = x.los.breg[0];
+      if (in.grids && (x.dbrnd_x || x.dbrnd_y || x.dbrnd_z || x.dternd)) {
+        void *gg = nullptr;
+        for (const GradKey &g : grad_keys)
+          if (g.x == x.dbrnd_x && g.y == x.dbrnd_y && g.z == x.dbrnd_z && g.t == x.dternd) {
+            if (std::memcmp(&g.box, &x.box, sizeof(hamx_box)) != 0)
+              return fail(set_err(ctx, HAMX_ERR_ARG,
+                                  "batch instance %u: shared grid-gradient arrays on "
+                                  "different boxes", i));
+            gg = g.ggrid;
+          }
+        if (!gg) {
+          const uint64_t n = box_size(x.box);
+          const bool f32 = precision == HAMX_PREC_F32;
+          if ((rc = dev_alloc(b, n * (f32 ? sizeof(float4) : sizeof(double4)), &gg)) != HAMX_OK)
+            return fail(rc);
+          grad_keys.push_back(GradKey{x.dbrnd_x, x.dbrnd_y, x.dbrnd_z, x.dternd, x.box, gg});
+          a_fills.push_back(FillJob{static_cast<double *>(gg), n * (f32 ? 2 : 4)});
+          UnpackJob u;
+          std::memset(&u, 0, sizeof(u));
+          u.g = gg;
+          u.x = x.dbrnd_x;
+          u.y = x.dbrnd_y;
+          u.z = x.dbrnd_z;
+          u.t = x.dternd;
+          u.n = n;
+          u.f32 = f32 ? 1 : 0;
+          u.bscale = f32 ? 1e6 : 1.;
+          a_unpacks.push_back(u);
+        }
+        A.ggrid = gg;
+      }
+    }
     b->levels = std::max<uint32_t>(b->levels, x.los.n_shell);
   }
   if (fills.size() > 65535 || n_inst > 65535 / 3)
@@ -1030,8 +1654,28 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
   b->d_acc.resize(b->levels);
   b->n_acc.assign(b->levels, 0);
   b->acc_max_npix.assign(b->levels, 0);
+  b->a_zero.assign(b->levels, hamx_batch::Jobs{});
+  b->a_acct.assign(b->levels, hamx_batch::Jobs{});
+  b->a_fdt.assign(b->levels, hamx_batch::Jobs{});
+  b->f_ckpt.assign(b->levels, hamx_batch::Jobs{});
+  auto put = [&](auto &vec, hamx_batch::Jobs &jobs, uint64_t mx) -> int {
+    jobs.n = static_cast<uint32_t>(vec.size());
+    jobs.max = mx;
+    if (vec.empty())
+      return HAMX_OK;
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    T *dptr;
+    HAMX_TRY(upload(b, vec, &dptr));
+    jobs.d = dptr;
+    return HAMX_OK;
+  };
   for (uint32_t s = 0; s < b->levels; ++s) {
     std::vector<AccJob> jobs;
+    std::vector<FillJob> zs;
+    std::vector<AccJob> ats;
+    std::vector<FdtJob> fts;
+    std::vector<CopyJob> cks;
+    uint64_t z_max = 0, at_max = 0, ft_max = 0, ck_max = 0;
     for (uint32_t i = 0; i < n_inst; ++i) {
       const hamx_batch_inst &x = inst[i];
       const BInst &in = b->host_inst[i];
@@ -1073,7 +1717,68 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
         j.src[0] = in.t_fd + in.fd_off[s];
         jobs.push_back(j);
       }
+      if (b->adjoint) {
+        const BAdj &A = in.adj;
+        AccJob t;
+        std::memset(&t, 0, sizeof(t));
+        t.hd = healpix_make(x.los.nside_sync);
+        t.hs = hs;
+        for (uint32_t f = 0; f < K; ++f) {
+          const double *src[3] = {x.adj_is[K - 1 - f], x.adj_qs[K - 1 - f], x.adj_us[K - 1 - f]};
+          double *dst[3] = {A.g_i[f], A.g_q[f], A.g_u[f]};
+          for (int m = 0; m < 3; ++m) {
+            zs.push_back(FillJob{dst[m], hs.npix});
+            if (src[m]) { // a NULL adjoint map is zero
+              t.dst[t.n] = dst[m];
+              t.src[t.n++] = src[m];
+            }
+          }
+        }
+        if (t.n) {
+          ats.push_back(t);
+          at_max = std::max(at_max, t.hd.npix);
+        }
+        if (in.do_dm) {
+          zs.push_back(FillJob{A.g_dm, hs.npix});
+          if (x.adj_dm) {
+            AccJob j;
+            std::memset(&j, 0, sizeof(j));
+            j.hd = healpix_make(x.los.nside_dm);
+            j.hs = hs;
+            j.n = 1;
+            j.dst[0] = A.g_dm;
+            j.src[0] = x.adj_dm;
+            ats.push_back(j);
+            at_max = std::max(at_max, j.hd.npix);
+          }
+        }
+        if (in.do_fd) {
+          zs.push_back(FillJob{A.g_fd, hs.npix});
+          AccJob j;
+          std::memset(&j, 0, sizeof(j));
+          j.hd = in.hfd;
+          j.hs = hs;
+          j.n = 1;
+          j.dst[0] = A.g_fd;
+          j.src[0] = A.G;
+          ats.push_back(j);
+          at_max = std::max(at_max, j.hd.npix);
+          if (s > 0) {
+            fts.push_back(FdtJob{A.G, A.gfdin, in.hfd, hs});
+            ft_max = std::max(ft_max, hs.npix);
+            cks.push_back(CopyJob{A.ckpt + static_cast<uint64_t>(s) * in.hfd.npix,
+                                  in.fd_cache, in.hfd.npix});
+            ck_max = std::max(ck_max, in.hfd.npix);
+          }
+        }
+        z_max = std::max(z_max, hs.npix);
+      }
     }
+    if ((rc = put(zs, b->a_zero[s], z_max)) != HAMX_OK ||
+        (rc = put(ats, b->a_acct[s], at_max)) != HAMX_OK ||
+        (rc = put(fts, b->a_fdt[s], ft_max)) != HAMX_OK ||
+        (rc = put(cks, b->f_ckpt[s], ck_max)) != HAMX_OK)
+      return fail(rc);
     for (const AccJob &j : jobs)
       b->acc_max_npix[s] = std::max(b->acc_max_npix[s], j.hd.npix);
     b->n_acc[s] = static_cast<uint32_t>(jobs.size());
@@ -1082,6 +1787,35 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     for (int c = 0; c < 4; ++c)
       if ((rc = upload(b, b->ids[s * 4 + c], &b->d_ids[s * 4 + c])) != HAMX_OK)
         return fail(rc);
+  }
+  if (b->adjoint) {
+    uint64_t mx = 0;
+    for (const FillJob &f : a_fills)
+      mx = std::max(mx, f.n);
+    if ((rc = put(a_fills, b->a_init_fill, mx)) != HAMX_OK)
+      return fail(rc);
+    mx = 0;
+    for (const CopyJob &c : a_copies)
+      mx = std::max(mx, c.n);
+    if ((rc = put(a_copies, b->a_init_copy, mx)) != HAMX_OK)
+      return fail(rc);
+    mx = 0;
+    for (const RedJob &r2 : a_reds)
+      mx = std::max(mx, r2.n);
+    if ((rc = put(a_reds, b->a_red, mx)) != HAMX_OK)
+      return fail(rc);
+    mx = 0;
+    for (const UnpackJob &u : a_unpacks)
+      mx = std::max(mx, u.n);
+    if ((rc = put(a_unpacks, b->a_unpack, mx)) != HAMX_OK)
+      return fail(rc);
+    if (b->a_init_fill.n > 65535 || b->a_red.n > 65535)
+      return fail(set_err(ctx, HAMX_ERR_ARG, "hamx_batch_create: too many instances"));
+    void *p;
+    if ((rc = dev_alloc(b, sizeof(unsigned long long) * kQueues * 4 * b->levels, &p)) !=
+        HAMX_OK)
+      return fail(rc);
+    b->d_actr = static_cast<unsigned long long *>(p);
   }
   // item plans of every variant
   b->d_prefix.assign(kNumVariants, std::vector<uint64_t *>(b->levels * 4, nullptr));
@@ -1165,6 +1899,23 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
   b->graph = g;
   if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess)
     return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipGraphInstantiate failed"));
+  if (b->adjoint) {
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipStreamBeginCapture failed"));
+    rc = launch_adjoint(b, best);
+    hipGraph_t ga = nullptr;
+    const hipError_t ea = hipStreamEndCapture(ctx->stream, &ga);
+    if (rc != HAMX_OK) {
+      if (ga)
+        (void)hipGraphDestroy(ga);
+      return fail(rc);
+    }
+    if (ea != hipSuccess || !ga)
+      return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipStreamEndCapture failed"));
+    b->adj_graph = ga;
+    if (hipGraphInstantiate(&b->adj_exec, ga, nullptr, nullptr, 0) != hipSuccess)
+      return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipGraphInstantiate failed"));
+  }
   *out = b;
   return HAMX_OK;
 }

@@ -6,11 +6,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "hamx.h"
 #include "hamx_cgs.h"
-#include <cmath>
 
 #include "hamx_fields.h"
 #include "hamx_healpix.h"

@@ -182,14 +182,16 @@ class Context:
 
     # -------------------------------------------------------------- batch
     def batch(self, instances: list[dict], precision: int = capi.HAMX_PREC_F32,
-              tune: bool = True) -> "Batch":
+              tune: bool = True, adjoint: bool = False) -> "Batch":
         """Plan a batched LOS run (hamx_batch_create).  Each instance is a dict:
         desc (HamxLosDesc), freqs (1-4 Hz values, <sync> list order), maps
         (dm / fd tensors or None, is / qs / us lists of tensors, one per
-        frequency), and optionally brnd (3 tensors), ternd and box.  Every
-        array is a float64 CUDA tensor; addresses are fixed for the plan's
-        lifetime."""
-        return Batch(self, instances, precision, tune)
+        frequency), and optionally brnd (3 tensors), ternd and box.  With
+        adjoint=True an instance may also hold adj (dL/d(maps) in the layout
+        of maps) and grads (dparams: 6 values; dbrnd: 3 grid tensors;
+        dternd), filled by Batch.run_adjoint().  Every array is a float64
+        CUDA tensor; addresses are fixed for the plan's lifetime."""
+        return Batch(self, instances, precision, tune, adjoint)
 
     # -------------------------------------------------------- random fields
     def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
@@ -234,7 +236,8 @@ def los_samples(desc: capi.HamxLosDesc) -> int:
 class Batch:
     """A planned batch: run() replays the captured graph."""
 
-    def __init__(self, ctx: Context, instances: list[dict], precision: int, tune: bool):
+    def __init__(self, ctx: Context, instances: list[dict], precision: int, tune: bool,
+                 adjoint: bool = False):
         self.ctx = ctx
         self.keep = []
         arr = (capi.HamxBatchInst * len(instances))()
@@ -268,6 +271,25 @@ class Batch:
                 for i, a in enumerate(lst):
                     field[i] = _ptr(a)[0]
                 arrays += list(lst)
+            adj = spec.get("adj") or {}
+            x.adj_dm = _ptr(adj.get("dm"))[0]
+            x.adj_fd = _ptr(adj.get("fd"))[0]
+            arrays += [adj.get("dm"), adj.get("fd")]
+            for key, field in (("is", x.adj_is), ("qs", x.adj_qs), ("us", x.adj_us)):
+                lst = adj.get(key) or [None] * len(freqs)
+                if len(lst) != len(freqs):
+                    raise ValueError(f"adj[{key!r}] needs one map per frequency")
+                for i, a in enumerate(lst):
+                    field[i] = _ptr(a)[0]
+                arrays += list(lst)
+            gr = spec.get("grads") or {}
+            x.dparams = _ptr(gr.get("dparams"))[0]
+            db = gr.get("dbrnd")
+            if db is not None:
+                x.dbrnd_x, x.dbrnd_y, x.dbrnd_z = (_ptr(a)[0] for a in db)
+                arrays += list(db)
+            x.dternd = _ptr(gr.get("dternd"))[0]
+            arrays += [gr.get("dparams"), gr.get("dternd")]
             for a in arrays:
                 if a is not None and not _ptr(a)[1]:
                     raise ValueError("batch arrays must be device tensors")
@@ -275,6 +297,8 @@ class Batch:
         flags = capi.HAMX_IN_DEVICE | capi.HAMX_OUT_DEVICE
         if not tune:
             flags |= capi.HAMX_BATCH_NO_TUNE
+        if adjoint:
+            flags |= capi.HAMX_BATCH_ADJOINT
         h = C.c_void_p()
         ctx._check(ctx.lib.hamx_batch_create(ctx.handle, arr, len(instances), precision,
                                              flags, C.byref(h)))
@@ -283,6 +307,9 @@ class Batch:
 
     def run(self) -> None:
         self.ctx._check(self.ctx.lib.hamx_batch_run(self.handle))
+
+    def run_adjoint(self) -> None:
+        self.ctx._check(self.ctx.lib.hamx_batch_run_adjoint(self.handle))
 
     def info(self) -> dict:
         i = capi.HamxBatchInfo()

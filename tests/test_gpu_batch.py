@@ -195,3 +195,114 @@ def test_batch_rejects_bad_instances(ctx):
     s["maps"]["is"][0] = s["maps"]["is"][0].cpu()
     with pytest.raises(ValueError):
         ctx.batch([s], tune=False)
+
+
+def with_adjoint(specs, seed):
+    """Random dL/d(maps) (scaled to each map's size) and gradient outputs."""
+    torch, dev = torch_dev()
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    for sp in specs:
+        m = sp["maps"]
+        rnd = lambda t: torch.randn(t.shape[0], dtype=torch.float64, device=dev,  # noqa: E731
+                                    generator=gen)
+        sp["adj"] = {"dm": rnd(m["dm"]), "fd": rnd(m["fd"]),
+                     **{k: [rnd(t) for t in m[k]] for k in ("is", "qs", "us")}}
+        gr = {"dparams": torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)}
+        if "brnd" in sp:
+            n = sp["brnd"][0].shape[0]
+            gr["dbrnd"] = tuple(torch.zeros(n, dtype=torch.float64, device=dev)
+                                for _ in range(3))
+            gr["dternd"] = torch.zeros(n, dtype=torch.float64, device=dev)
+        sp["grads"] = gr
+    return specs
+
+
+def scale_adjoint(specs):
+    """Divide each adjoint map by its map's rms, so every map weighs alike."""
+    import torch
+    for sp in specs:
+        for k, v in sp["maps"].items():
+            vs = v if isinstance(v, list) else [v]
+            adj = sp["adj"][k] if isinstance(v, list) else [sp["adj"][k]]
+            for a, mm in zip(adj, vs):
+                r = float(torch.sqrt(torch.mean(mm * mm)))
+                if r > 0:
+                    a.div_(r)
+
+
+def plain_adjoint_passes(ctx, spec):
+    """Sum over the reference pipeline's passes of the plain-path adjoint."""
+    d0, freqs, adj = spec["desc"], spec["freqs"], spec["adj"]
+    kw = {}
+    n = 0
+    if "brnd" in spec:
+        kw = dict(brnd=tuple(a.cpu().numpy() for a in spec["brnd"]), brnd_box=spec["box"],
+                  ternd=spec["ternd"].cpu().numpy(), ternd_box=spec["box"])
+        n = spec["brnd"][0].shape[0]
+    dp_tot = np.zeros(capi.HAMX_NPARAM)
+    gb_tot = [np.zeros(n) for _ in range(3)]
+    gt_tot = np.zeros(n)
+    for p, fi in enumerate(reversed(range(len(freqs)))):
+        d = capi.HamxLosDesc()
+        C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
+        d.sync_freq, d.do_sync = freqs[fi], 1
+        if p > 0:
+            d.do_dm = d.do_fd = 0
+        a = {k: np.ascontiguousarray(adj[k][fi].cpu().numpy()) for k in ("is", "qs", "us")}
+        if p == 0:
+            a["dm"] = np.ascontiguousarray(adj["dm"].cpu().numpy())
+            a["fd"] = np.ascontiguousarray(adj["fd"].cpu().numpy())
+        dp = np.zeros(capi.HAMX_NPARAM)
+        gkw = {}
+        if n:
+            gkw = dict(dbrnd=tuple(np.zeros(n) for _ in range(3)), dternd=np.zeros(n))
+        ctx.los_adjoint(d, a, dp, **kw, **gkw)
+        dp_tot += dp
+        if n:
+            for t, g in zip(gb_tot, gkw["dbrnd"]):
+                t += g
+            gt_tot += gkw["dternd"]
+    return dp_tot, gb_tot, gt_tot
+
+
+@pytest.mark.parametrize("prec,tol", [(capi.HAMX_PREC_F64, 1e-9), (capi.HAMX_PREC_F32, 1e-4)])
+def test_batch_adjoint_matches_plain_adjoint_passes(ctx, prec, tol):
+    """Batched reverse mode against the plain hamx_los_adjoint summed over
+    the pipeline's passes: parameters per component and the B / TE grid
+    gradients in relative L2."""
+    print(f"case: batch adjoint vs plain, precision {prec}", flush=True)
+    specs2 = with_adjoint(mixed_batch(), seed=21)
+    with ctx.batch(specs2, precision=prec, tune=False, adjoint=True) as b:
+        b.run()
+        scale_adjoint(specs2)  # in place: the plan's addresses stay valid
+        b.run_adjoint()
+        for i, sp in enumerate(specs2):
+            dp, gb, gt = plain_adjoint_passes(ctx, sp)
+            got = sp["grads"]["dparams"].cpu().numpy()
+            rel = np.abs(got - dp) / np.maximum(np.abs(dp), 1e-300)
+            print(f"prec{prec} inst{i} dparams rel {np.array2string(rel, precision=2)}",
+                  flush=True)
+            assert np.all(rel <= tol), (i, got, dp)
+            if "dbrnd" in sp["grads"]:
+                for name, g, r in (("bx", sp["grads"]["dbrnd"][0], gb[0]),
+                                   ("by", sp["grads"]["dbrnd"][1], gb[1]),
+                                   ("bz", sp["grads"]["dbrnd"][2], gb[2]),
+                                   ("te", sp["grads"]["dternd"], gt)):
+                    err = rel_l2(g.cpu().numpy(), r)
+                    print(f"prec{prec} inst{i} d{name} rel L2 {err:.2e}", flush=True)
+                    assert err <= tol, (i, name, err)
+        # a second replay of both graphs gives the same gradients
+        first = [sp["grads"]["dparams"].clone() for sp in specs2]
+        b.run()
+        b.run_adjoint()
+        for sp, f in zip(specs2, first):
+            np.testing.assert_allclose(sp["grads"]["dparams"].cpu().numpy(), f.cpu().numpy(),
+                                       rtol=1e-12 if prec == capi.HAMX_PREC_F64 else 1e-5)
+
+
+def test_batch_adjoint_needs_the_flag(ctx):
+    s = instance(4, 16, FREQS[2:3])
+    with ctx.batch([s], tune=False) as b:
+        with pytest.raises(HamxError) as e:
+            b.run_adjoint()
+        assert e.value.code == capi.HAMX_ERR_ARG
