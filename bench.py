@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--instances", type=int, default=64, help="batch size (batch workload)")
     ap.add_argument("--precision", choices=("f32", "f64"), default="f32",
                     help="batch sample arithmetic (batch workload)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="batch workload: forward only (no map+gradient evaluations)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy "
                          "kernel of known size, to calibrate FETCH_SIZE / WRITE_SIZE)")
@@ -108,6 +110,7 @@ def run_batch(args):
     dev = torch.device("cuda:0")
     prec = capi.HAMX_PREC_F32 if args.precision == "f32" else capi.HAMX_PREC_F64
     n_inst = args.instances
+    with_adj = not args.no_adjoint
     specs, pars = [], []
     with Context(0) as ctx:
         t0 = time.perf_counter()
@@ -124,12 +127,24 @@ def run_batch(args):
             freqs = list(g.sim_sync_freq)
             maps = dict(dm=z(d.nside_dm), fd=z(d.nside_fd),
                         **{k: [z(d.nside_sync) for _ in freqs] for k in ("is", "qs", "us")})
-            specs.append(dict(desc=d, freqs=freqs, maps=maps, brnd=(bx, by, bz), ternd=te,
-                              box=par.grid_brnd.box()))
+            spec = dict(desc=d, freqs=freqs, maps=maps, brnd=(bx, by, bz), ternd=te,
+                        box=par.grid_brnd.box())
+            if with_adj:
+                gen = torch.Generator(device=dev).manual_seed(1000 + i)
+                rnd = lambda t: torch.randn(t.shape[0], dtype=torch.float64,  # noqa: E731
+                                            device=dev, generator=gen)
+                spec["adj"] = {"dm": rnd(maps["dm"]), "fd": rnd(maps["fd"]),
+                               **{k: [rnd(t) for t in maps[k]] for k in ("is", "qs", "us")}}
+                spec["grads"] = {
+                    "dparams": torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev),
+                    "dbrnd": tuple(torch.zeros(n, dtype=torch.float64, device=dev)
+                                   for _ in range(3)),
+                    "dternd": torch.zeros(n, dtype=torch.float64, device=dev)}
+            specs.append(spec)
         torch.cuda.synchronize()
         gen_s = time.perf_counter() - t0
         t0 = time.perf_counter()
-        b = ctx.batch(specs, precision=prec, tune=True)
+        b = ctx.batch(specs, precision=prec, tune=True, adjoint=with_adj)
         plan_s = time.perf_counter() - t0
         info = b.info()
         for _ in range(args.warmup):
@@ -140,12 +155,36 @@ def run_batch(args):
             b.run()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        mg = {}
+        if with_adj:
+            # map+gradient evaluations: forward replay + reverse replay
+            b.run()
+            b.run_adjoint()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                b.run()
+                b.run_adjoint()
+            torch.cuda.synchronize()
+            mg_s = (time.perf_counter() - t0) / args.steps
+            mg = dict(batch_map_gradient_ms=mg_s * 1e3,
+                      batch_map_gradient_evals_per_s=n_inst / mg_s)
         # the plain path, pass by pass, and the maps it gives
         worst, plain_s, plain_samples = 0.0, 0.0, 0
+        plain_mg_s, worst_dp, worst_grid = 0.0, 0.0, 0.0
+        if with_adj:
+            n = specs[0]["brnd"][0].shape[0]
+            pdp = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
+            pgb = tuple(torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3))
+            pgt = torch.zeros(n, dtype=torch.float64, device=dev)
         for sp in specs:
             d0, freqs = sp["desc"], sp["freqs"]
             kw = dict(brnd=sp["brnd"], brnd_box=sp["box"], ternd=sp["ternd"],
                       ternd_box=sp["box"])
+            if with_adj:
+                acc_dp = torch.zeros_like(pdp)
+                acc_gb = [torch.zeros_like(pgt) for _ in range(3)]
+                acc_gt = torch.zeros_like(pgt)
             for p, fi in enumerate(reversed(range(len(freqs)))):
                 d = capi.HamxLosDesc()
                 C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
@@ -161,9 +200,21 @@ def run_batch(args):
                                           device=dev)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
-                ctx.los_forward(d, m, **kw)
+                ctx.los_forward(d, m, keep_tape=with_adj, **kw)
                 plain_s += time.perf_counter() - t1
                 plain_samples += los_samples(d)
+                if with_adj:
+                    a = {k: sp["adj"][k][fi] for k in ("is", "qs", "us")}
+                    if p == 0:
+                        a["dm"], a["fd"] = sp["adj"]["dm"], sp["adj"]["fd"]
+                    t1 = time.perf_counter()
+                    ctx.los_adjoint(d, a, pdp, dbrnd=pgb, dternd=pgt, use_tape=True, **kw)
+                    torch.cuda.synchronize()
+                    plain_mg_s += time.perf_counter() - t1
+                    acc_dp += pdp
+                    for t, g in zip(acc_gb, pgb):
+                        t += g
+                    acc_gt += pgt
                 pairs = [(sp["maps"][k][fi], m[k]) for k in ("is", "qs", "us")]
                 if p == 0:
                     pairs += [(sp["maps"]["dm"], m["dm"]), (sp["maps"]["fd"], m["fd"])]
@@ -171,6 +222,14 @@ def run_batch(args):
                     nr = float(torch.linalg.vector_norm(ref))
                     err = float(torch.linalg.vector_norm(got - ref)) / nr if nr else 0.0
                     worst = max(worst, err)
+            if with_adj:
+                g = sp["grads"]
+                rel = (g["dparams"] - acc_dp).abs() / acc_dp.abs().clamp_min(1e-300)
+                worst_dp = max(worst_dp, float(rel.max()))
+                for got, ref in zip(list(g["dbrnd"]) + [g["dternd"]], acc_gb + [acc_gt]):
+                    nr = float(torch.linalg.vector_norm(ref))
+                    err = float(torch.linalg.vector_norm(got - ref)) / nr if nr else 0.0
+                    worst_grid = max(worst_grid, err)
         b.close()
     ms = elapsed * 1e3 / args.steps
     value = info["samples_ref"] * args.steps / elapsed
@@ -214,6 +273,12 @@ def run_batch(args):
             "speedup_vs_plain_pass_by_pass": value / plain_rate,
             "max_rel_l2_vs_plain_fp64": worst,
             "autotune_ms": info["variant_ms"],
+            **mg,
+            **({"plain_map_gradient_evals_per_s": n_inst / (plain_s + plain_mg_s),
+                "map_gradient_speedup_vs_plain":
+                    (mg["batch_map_gradient_evals_per_s"] * (plain_s + plain_mg_s) / n_inst),
+                "max_rel_err_dparams_vs_plain_fp64": worst_dp,
+                "max_rel_l2_grid_gradient_vs_plain_fp64": worst_grid} if with_adj else {}),
             "plan_s": plan_s,
             "realisation_generation_s": gen_s,
         },
