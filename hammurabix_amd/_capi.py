@@ -45,7 +45,8 @@ class HamxFields(C.Structure):
                 ("ternd", C.c_void_p), ("ternd_box", HamxBox),
                 ("breg_x", C.c_void_p), ("breg_y", C.c_void_p), ("breg_z", C.c_void_p),
                 ("breg_box", HamxBox),
-                ("tereg", C.c_void_p), ("tereg_box", HamxBox)]
+                ("tereg", C.c_void_p), ("tereg_box", HamxBox),
+                ("mask", C.c_void_p), ("nside_mask", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class HamxLosDesc(C.Structure):

@@ -51,19 +51,12 @@ struct AdjShell {
   uint64_t sy, sx;                        // corner strides of the packed grid
 };
 
-__device__ inline void grid_scatter(double *g, uint64_t idx, double w, const Vec3 &dB,
-                                    double dte) {
-  double *p = g + 4 * idx;
-  unsafeAtomicAdd(p + 0, w * dB.x);
-  unsafeAtomicAdd(p + 1, w * dB.y);
-  unsafeAtomicAdd(p + 2, w * dB.z);
-  unsafeAtomicAdd(p + 3, w * dte);
-}
-
 template <bool GRIDS, bool PW1>
 __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst ac,
                                                  Grids gr, ShellArgs sa, AdjShell as) {
+  __shared__ double slabs[4 * kSlabValues]; // one per wavefront of the 256-thread block
   const int lane = threadIdx.x & 63;
+  double *slab = slabs + (threadIdx.x >> 6) * kSlabValues;
   const uint64_t ipix =
       static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (ipix >= sa.hs.npix)
@@ -106,53 +99,47 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     const double dq_scan = wave_suffix_scan(dq, lane);
     const double dq_suffix = dq_scan + dq_after; // sum over samples k' >= k
     dq_after += __shfl(dq_scan, 0, 64);
-    if (!valid)
-      continue;
-    const double R = ac.l2 * dq_suffix + gfd; // dL/df_k
-    const double djp = gq * c2q + gu * s2q;
-    const double djt = gi;
-    const double dte = (o.te_raw > 0.) ? (R * o.b_par * fc.fd_scale + gdm * ac.delta) : 0.;
-    // dL/dB
-    const double ft = R * o.te * fc.fd_scale;
-    Vec3 dB{ft * r.l.x, ft * r.l.y, ft * r.l.z};
-    if (o.bper2 > 0.) {
-      const double coef = (djt * o.jt + djp * o.jp) * ac.alpha1 / (2. * o.bper2);
-      dB.x += coef * (o.B.x - o.b_par * r.l.x);
-      dB.y += coef * (o.B.y - o.b_par * r.l.y);
-      dB.z += coef * (o.B.z - o.b_par * r.l.z);
-      const double lnb = 0.5 * log(o.bper);
-      p4 += djt * o.jt * (ac.lt + lnb) + djp * o.jp * (ac.lp + lnb);
+    double dte = 0.;
+    Vec3 dB{0., 0., 0.};
+    if (valid) {
+      const double R = ac.l2 * dq_suffix + gfd; // dL/df_k
+      const double djp = gq * c2q + gu * s2q;
+      const double djt = gi;
+      dte = (o.te_raw > 0.) ? (R * o.b_par * fc.fd_scale + gdm * ac.delta) : 0.;
+      // dL/dB
+      const double ft = R * o.te * fc.fd_scale;
+      dB = Vec3{ft * r.l.x, ft * r.l.y, ft * r.l.z};
+      if (o.bper2 > 0.) {
+        const double coef = (djt * o.jt + djp * o.jp) * ac.alpha1 / (2. * o.bper2);
+        dB.x += coef * (o.B.x - o.b_par * r.l.x);
+        dB.y += coef * (o.B.y - o.b_par * r.l.y);
+        dB.z += coef * (o.B.z - o.b_par * r.l.z);
+        const double lnb = 0.5 * log(o.bper);
+        p4 += djt * o.jt * (ac.lt + lnb) + djp * o.jp * (ac.lp + lnb);
+      }
+      if (o.n2 > 0.) {
+        const double c = dq / o.n2;
+        dB.x += c * (o.y * r.eph.x - o.x * r.eth.x);
+        dB.y += c * (o.y * r.eph.y - o.x * r.eth.y);
+        dB.z += c * (o.y * r.eph.z - o.x * r.eth.z);
+      }
+      if (o.te_in)
+        p5 += dte;
+      if (o.lsa_in) {
+        const double u = o.sps * o.cos_p - o.cps * o.sin_p; // B_x / (b0 cos chi)
+        const double v = o.sps * o.sin_p + o.cps * o.cos_p; // B_y / (b0 cos chi)
+        p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
+        const double gpsi = ac.b0 * o.cch * (dB.x * v - dB.y * u);
+        p1 += gpsi;
+        p2 += gpsi * o.lnr;
+        const double gchi = ac.b0 * (-o.sch * (dB.x * u + dB.y * v) + o.cch * dB.z);
+        p3 += gchi * o.tnh;
+      }
     }
-    if (o.n2 > 0.) {
-      const double c = dq / o.n2;
-      dB.x += c * (o.y * r.eph.x - o.x * r.eth.x);
-      dB.y += c * (o.y * r.eph.y - o.x * r.eth.y);
-      dB.z += c * (o.y * r.eph.z - o.x * r.eth.z);
-    }
-    if (o.te_in)
-      p5 += dte;
-    if (o.lsa_in) {
-      const double u = o.sps * o.cos_p - o.cps * o.sin_p; // B_x / (b0 cos chi)
-      const double v = o.sps * o.sin_p + o.cps * o.cos_p; // B_y / (b0 cos chi)
-      p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
-      const double gpsi = ac.b0 * o.cch * (dB.x * v - dB.y * u);
-      p1 += gpsi;
-      p2 += gpsi * o.lnr;
-      const double gchi = ac.b0 * (-o.sch * (dB.x * u + dB.y * v) + o.cch * dB.z);
-      p3 += gchi * o.tnh;
-    }
-    if (GRIDS && o.cell_ok && as.ggrid) {
-      const double xd = o.cell.xd, yd = o.cell.yd, zd = o.cell.zd;
-      const uint64_t i0 = o.cell.idx;
-      const double wx[2] = {1. - xd, xd}, wy[2] = {1. - yd, yd}, wz[2] = {1. - zd, zd};
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-          for (int c = 0; c < 2; ++c)
-            grid_scatter(as.ggrid, i0 + a * as.sx + b * as.sy + c, wx[a] * wy[b] * wz[c], dB,
-                         dte);
+    if (GRIDS && as.ggrid) { // wave-cooperative: every lane calls it
+      const bool has = valid && o.cell_ok;
+      scatter_runs<double>(as.ggrid, slab, has, has ? o.cell.idx : 0, o.cell.xd, o.cell.yd,
+                           o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane);
     }
   }
   p0 = wave_sum(p0);
@@ -290,7 +277,8 @@ bool same_fields(const hamx_fields &a, const hamx_fields &b) {
          a.ternd == b.ternd && a.breg_x == b.breg_x && a.breg_y == b.breg_y &&
          a.breg_z == b.breg_z && a.tereg == b.tereg && same_box_bits(a.brnd_box, b.brnd_box) &&
          same_box_bits(a.ternd_box, b.ternd_box) && same_box_bits(a.breg_box, b.breg_box) &&
-         same_box_bits(a.tereg_box, b.tereg_box);
+         same_box_bits(a.tereg_box, b.tereg_box) && a.mask == b.mask &&
+         a.nside_mask == b.nside_mask;
 }
 
 int los_forward_taped(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
@@ -345,6 +333,8 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
   std::memset(&none, 0, sizeof(none));
   const hamx_fields &f = fields ? *fields : none;
   const Phys ph = make_phys(d);
+  if (f.mask)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "hamx_los_adjoint: masked passes are not supported");
   if (!(d->breg_model == HAMX_BREG_LSA && d->tereg_model == HAMX_TEREG_UNIF &&
         ph.idx_const && d->do_sync &&
         (d->cre_model == HAMX_CRE_ANA || d->cre_model == HAMX_CRE_UNIF)))

@@ -44,6 +44,7 @@
 #include <new>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hamx_march.h"
@@ -195,8 +196,8 @@ __device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
     }
   }
   if (lane == 0) {
-    if (in.do_dm)
-      in.t_dm[ipix] = dm * sa.delta;
+    if (in.do_dm) // a shell thinner than one step has no samples and no finite delta
+      in.t_dm[ipix] = sa.step ? dm * sa.delta : 0.;
     in.t_fd[in.fd_off[s] + ipix] = carry;
 #pragma unroll
     for (int f = 0; f < kMaxF; ++f) {
@@ -391,15 +392,16 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
     }
   }
   if (lane == 0) {
+    const double dl = sh.step ? sh.delta : 0.; // no samples: no finite delta
     if (in.do_dm)
-      in.t_dm[ipix] = dms * sh.delta;
+      in.t_dm[ipix] = dms * dl;
     in.t_fd[in.fd_off[s] + ipix] = carry;
 #pragma unroll
     for (int f = 0; f < kMaxF; ++f) {
       if (f < nf) {
-        in.t_i[f][ipix] = Is * sh.delta * in.st[f];
-        in.t_q[f][ipix] = Qs[f] * sh.delta * in.sp[f];
-        in.t_u[f][ipix] = Us[f] * sh.delta * in.sp[f];
+        in.t_i[f][ipix] = Is * dl * in.st[f];
+        in.t_q[f][ipix] = Qs[f] * dl * in.sp[f];
+        in.t_u[f][ipix] = Us[f] * dl * in.sp[f];
       }
     }
   }
@@ -421,27 +423,8 @@ __device__ inline ShellArgs shell_args_rev(const BInst &in, int s) {
   return sa;
 }
 
-template <typename T>
-__device__ inline void scatter8(T *g, uint64_t idx, uint64_t sx, uint64_t sy, T xd, T yd,
-                                T zd, T gx, T gy, T gz, T gt) {
-  const T wx[2] = {T(1) - xd, xd}, wy[2] = {T(1) - yd, yd}, wz[2] = {T(1) - zd, zd};
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const T w = wx[a] * wy[b] * wz[c];
-        T *p = g + 4 * (idx + a * sx + b * sy + c);
-        unsafeAtomicAdd(p + 0, w * gx);
-        unsafeAtomicAdd(p + 1, w * gy);
-        unsafeAtomicAdd(p + 2, w * gz);
-        unsafeAtomicAdd(p + 3, w * gt);
-      }
-}
-
 template <bool GRIDS, bool PW1>
-__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
+__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, float *slab) {
   const ShellArgs sa = shell_args_rev(in, s);
   const BShell &sh = in.sh[s];
   const BAdj &A = in.adj;
@@ -513,43 +496,48 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
     const double dscan = wave_suffix_scan(D, lane);
     const double R = dscan + d_after + gfd; // dL/d(FD contribution of this sample)
     d_after += __shfl(dscan, 0, 64);
-    if (!valid)
-      continue;
-    const double dte = o.te_raw > 0.f ? R * static_cast<double>(o.b_par) * sh.fdu + gdm * delta
-                                      : 0.;
-    const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
-    float dbx = ft * lf.x, dby = ft * lf.y, dbz = ft * lf.z;
-    if (o.bper2 > 0.f) {
-      const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
-      dbx += coef * (o.bx - o.b_par * lf.x);
-      dby += coef * (o.by - o.b_par * lf.y);
-      dbz += coef * (o.bz - o.b_par * lf.z);
-      // ln |B_perp| in Gauss: the B grid and b0 are in muG here
-      const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
-      p4 += static_cast<double>(o.w) *
-            (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
+    double dte = 0.;
+    float dbx = 0.f, dby = 0.f, dbz = 0.f;
+    if (valid) {
+      dte = o.te_raw > 0.f ? R * static_cast<double>(o.b_par) * sh.fdu + gdm * delta : 0.;
+      const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
+      dbx = ft * lf.x;
+      dby = ft * lf.y;
+      dbz = ft * lf.z;
+      if (o.bper2 > 0.f) {
+        const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
+        dbx += coef * (o.bx - o.b_par * lf.x);
+        dby += coef * (o.by - o.b_par * lf.y);
+        dbz += coef * (o.bz - o.b_par * lf.z);
+        // ln |B_perp| in Gauss: the B grid and b0 are in muG here
+        const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
+        p4 += static_cast<double>(o.w) *
+              (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
+      }
+      if (o.n2 > 0.f) {
+        const float cc = dipa / o.n2;
+        dbx += cc * (o.y * ephf.x - o.x * ethf.x);
+        dby += cc * (o.y * ephf.y - o.x * ethf.y);
+        dbz += cc * (o.y * ephf.z - o.x * ethf.z);
+      }
+      if (o.te_in)
+        p5 += dte;
+      if (o.lsa_in) {
+        const float u = o.sps * o.cos_p - o.cps * o.sin_p;
+        const float v = o.sps * o.sin_p + o.cps * o.cos_p;
+        p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
+        const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
+        p1 += gpsi;
+        p2 += gpsi * o.lnr;
+        const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
+        p3 += gchi * o.tnh;
+      }
     }
-    if (o.n2 > 0.f) {
-      const float cc = dipa / o.n2;
-      dbx += cc * (o.y * ephf.x - o.x * ethf.x);
-      dby += cc * (o.y * ephf.y - o.x * ethf.y);
-      dbz += cc * (o.y * ephf.z - o.x * ethf.z);
+    if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
+      const bool has = valid && o.cell_ok;
+      scatter_runs<float>(static_cast<float *>(A.ggrid), slab, has, has ? o.idx : 0, o.xd,
+                          o.yd, o.zd, dbx, dby, dbz, static_cast<float>(dte), sx, sy, lane);
     }
-    if (o.te_in)
-      p5 += dte;
-    if (o.lsa_in) {
-      const float u = o.sps * o.cos_p - o.cps * o.sin_p;
-      const float v = o.sps * o.sin_p + o.cps * o.cos_p;
-      p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
-      const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
-      p1 += gpsi;
-      p2 += gpsi * o.lnr;
-      const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
-      p3 += gchi * o.tnh;
-    }
-    if (GRIDS && o.cell_ok && A.ggrid)
-      scatter8<float>(static_cast<float *>(A.ggrid), o.idx, sx, sy, o.xd, o.yd, o.zd, dbx, dby,
-                      dbz, static_cast<float>(dte));
   }
   p0 = wave_sum(p0);
   p1 = wave_sum(p1);
@@ -572,7 +560,7 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
 }
 
 template <bool GRIDS, bool PW1>
-__device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
+__device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, double *slab) {
   const ShellArgs sa = shell_args_rev(in, s);
   const BAdj &A = in.adj;
   Grids gr;
@@ -635,40 +623,44 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
     const double dscan = wave_suffix_scan(D, lane);
     const double R = dscan + d_after + gfd;
     d_after += __shfl(dscan, 0, 64);
-    if (!valid)
-      continue;
-    const double dte = o.te_raw > 0. ? R * o.b_par * fc.fd_scale + gdm * sa.delta : 0.;
-    const double ft = R * o.te * fc.fd_scale;
-    Vec3 dB{ft * r.l.x, ft * r.l.y, ft * r.l.z};
-    if (o.bper2 > 0.) {
-      const double coef = (aJt * o.jt + dJp * o.jp) * alpha1 / (2. * o.bper2);
-      dB.x += coef * (o.B.x - o.b_par * r.l.x);
-      dB.y += coef * (o.B.y - o.b_par * r.l.y);
-      dB.z += coef * (o.B.z - o.b_par * r.l.z);
-      const double lnb = 0.5 * log(o.bper2);
-      p4 += o.jt * (AIL + aJt * lnb) + o.jp * (alpha_p + dJp * lnb);
+    double dte = 0.;
+    Vec3 dB{0., 0., 0.};
+    if (valid) {
+      dte = o.te_raw > 0. ? R * o.b_par * fc.fd_scale + gdm * sa.delta : 0.;
+      const double ft = R * o.te * fc.fd_scale;
+      dB = Vec3{ft * r.l.x, ft * r.l.y, ft * r.l.z};
+      if (o.bper2 > 0.) {
+        const double coef = (aJt * o.jt + dJp * o.jp) * alpha1 / (2. * o.bper2);
+        dB.x += coef * (o.B.x - o.b_par * r.l.x);
+        dB.y += coef * (o.B.y - o.b_par * r.l.y);
+        dB.z += coef * (o.B.z - o.b_par * r.l.z);
+        const double lnb = 0.25 * log(o.bper2); // (1/2) ln |B_perp|
+        p4 += o.jt * (AIL + aJt * lnb) + o.jp * (alpha_p + dJp * lnb);
+      }
+      if (o.n2 > 0.) {
+        const double cc = dipa / o.n2;
+        dB.x += cc * (o.y * r.eph.x - o.x * r.eth.x);
+        dB.y += cc * (o.y * r.eph.y - o.x * r.eth.y);
+        dB.z += cc * (o.y * r.eph.z - o.x * r.eth.z);
+      }
+      if (o.te_in)
+        p5 += dte;
+      if (o.lsa_in) {
+        const double u = o.sps * o.cos_p - o.cps * o.sin_p;
+        const double v = o.sps * o.sin_p + o.cps * o.cos_p;
+        p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
+        const double gpsi = A.b0 * o.cch * (dB.x * v - dB.y * u);
+        p1 += gpsi;
+        p2 += gpsi * o.lnr;
+        const double gchi = A.b0 * (o.cch * dB.z - o.sch * (dB.x * u + dB.y * v));
+        p3 += gchi * o.tnh;
+      }
     }
-    if (o.n2 > 0.) {
-      const double cc = dipa / o.n2;
-      dB.x += cc * (o.y * r.eph.x - o.x * r.eth.x);
-      dB.y += cc * (o.y * r.eph.y - o.x * r.eth.y);
-      dB.z += cc * (o.y * r.eph.z - o.x * r.eth.z);
+    if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
+      const bool has = valid && o.cell_ok;
+      scatter_runs<double>(static_cast<double *>(A.ggrid), slab, has, has ? o.cell.idx : 0,
+                           o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte, sx, sy, lane);
     }
-    if (o.te_in)
-      p5 += dte;
-    if (o.lsa_in) {
-      const double u = o.sps * o.cos_p - o.cps * o.sin_p;
-      const double v = o.sps * o.sin_p + o.cps * o.cos_p;
-      p0 += dB.x * (o.cch * u) + dB.y * (o.cch * v) + dB.z * o.sch;
-      const double gpsi = A.b0 * o.cch * (dB.x * v - dB.y * u);
-      p1 += gpsi;
-      p2 += gpsi * o.lnr;
-      const double gchi = A.b0 * (o.cch * dB.z - o.sch * (dB.x * u + dB.y * v));
-      p3 += gchi * o.tnh;
-    }
-    if (GRIDS && o.cell_ok && A.ggrid)
-      scatter8<double>(static_cast<double *>(A.ggrid), o.cell.idx, sx, sy, o.cell.xd,
-                       o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte);
   }
   p0 = wave_sum(p0);
   p1 = wave_sum(p1);
@@ -693,7 +685,7 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
 // ---------------------------------------------------------- work items --
 
 template <int PREC, bool GRIDS, bool PW1, bool REV>
-__device__ void do_item(const LevelArgs &a, uint64_t item, int lane) {
+__device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab) {
   uint32_t lo = 0, hi = a.n_ids;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -711,9 +703,9 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane) {
   for (uint64_t p = p0; p < p1; ++p) {
     if (REV) {
       if (PREC == HAMX_PREC_F32)
-        adj_ray_f32<GRIDS, PW1>(in, s, p, lane);
+        adj_ray_f32<GRIDS, PW1>(in, s, p, lane, static_cast<float *>(slab));
       else
-        adj_ray_f64<GRIDS, PW1>(in, s, p, lane);
+        adj_ray_f64<GRIDS, PW1>(in, s, p, lane, static_cast<double *>(slab));
     } else {
       if (PREC == HAMX_PREC_F32)
         march_ray_f32<GRIDS, PW1>(in, s, p, lane);
@@ -730,6 +722,12 @@ template <int PREC, bool GRIDS, bool PW1, bool REV>
 __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+  void *slab = nullptr; // reverse mode: one LDS slab per wavefront for scatter_runs
+  if constexpr (REV) {
+    using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
+    __shared__ T slabs[4 * kSlabValues];
+    slab = slabs + wave * kSlabValues;
+  }
   if (!a.persistent) {
     uint64_t b = blockIdx.x;
     if (a.xcd) { // gridDim.x is a multiple of kQueues: b -> XCD-contiguous order
@@ -738,7 +736,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
     }
     const uint64_t item = b * wpb + wave;
     if (item < a.n_items)
-      do_item<PREC, GRIDS, PW1, REV>(a, uniform_u64(item), lane);
+      do_item<PREC, GRIDS, PW1, REV>(a, uniform_u64(item), lane, slab);
     return;
   }
   const uint32_t q0 = a.xcd ? (blockIdx.x % a.nq) : 0;
@@ -752,7 +750,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
       const uint64_t item = a.qbeg[q] + t;
       if (item >= a.qend[q])
         break; // every wave leaves once all queues are drained
-      do_item<PREC, GRIDS, PW1, REV>(a, item, lane);
+      do_item<PREC, GRIDS, PW1, REV>(a, item, lane, slab);
     }
   }
 }

@@ -220,4 +220,110 @@ __device__ inline double hp_interpolate(const Healpix &h, const double *map,
   return (wtot == 0.0) ? cgs::undef : res / wtot;
 }
 
+// ---- (x, y, face) <-> RING index (Hampix::rpixxyf / xyfrpix,
+// hamdis.h:515-605; the HEALPix ring2xyf / xyf2ring of Gorski et al. 2005)
+
+// face layout: ring number of each base pixel's south corner / nside and
+// its longitude index
+__host__ __device__ inline int hp_jrll(int f) { return 2 + (f >> 2); }
+__host__ __device__ inline int hp_jpll(int f) {
+  return (f < 4) ? 2 * f + 1 : (f < 8 ? 2 * (f - 4) : 2 * (f - 8) + 1);
+}
+
+__host__ __device__ inline uint64_t hp_isqrt_hd(uint64_t v) {
+  uint64_t r = static_cast<uint64_t>(sqrt(static_cast<double>(v) + 0.5));
+  while (r * r > v)
+    --r;
+  while ((r + 1) * (r + 1) <= v)
+    ++r;
+  return r;
+}
+
+__host__ __device__ inline void hp_ring2xyf(const Healpix &h, uint64_t pix, int64_t &ix,
+                                           int64_t &iy, int &face) {
+  const int64_t ns = static_cast<int64_t>(h.nside), nl2 = 2 * ns;
+  int64_t iring, iphi, kshift, nr;
+  if (pix < h.ncap) { // north polar cap
+    iring = (1 + static_cast<int64_t>(hp_isqrt_hd(1 + 2 * pix))) >> 1;
+    iphi = static_cast<int64_t>(pix) + 1 - 2 * iring * (iring - 1);
+    kshift = 0;
+    nr = iring;
+    face = static_cast<int>((iphi - 1) / nr);
+  } else if (pix < h.npix - h.ncap) { // equatorial belt
+    const int64_t ip = static_cast<int64_t>(pix - h.ncap);
+    const int64_t tmp = ip >> (h.order + 2);
+    iring = tmp + ns;
+    iphi = ip - tmp * 4 * ns + 1;
+    kshift = (iring + ns) & 1;
+    nr = ns;
+    const int64_t ire = tmp + 1, irm = nl2 + 1 - tmp;
+    const int64_t ifm = (iphi - (ire >> 1) + ns - 1) >> h.order;
+    const int64_t ifp = (iphi - (irm >> 1) + ns - 1) >> h.order;
+    face = static_cast<int>((ifp == ifm) ? (ifp | 4) : ((ifp < ifm) ? ifp : (ifm + 8)));
+  } else { // south polar cap
+    const int64_t ip = static_cast<int64_t>(h.npix - pix);
+    iring = (1 + static_cast<int64_t>(hp_isqrt_hd(2 * ip - 1))) >> 1;
+    iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+    kshift = 0;
+    nr = iring;
+    iring = 2 * nl2 - iring;
+    face = 8 + static_cast<int>((iphi - 1) / nr);
+  }
+  const int64_t irt = iring - hp_jrll(face) * ns + 1;
+  int64_t ipt = 2 * iphi - hp_jpll(face) * nr - kshift - 1;
+  if (ipt >= nl2)
+    ipt -= 8 * ns;
+  ix = (ipt - irt) >> 1;
+  iy = (-ipt - irt) >> 1;
+}
+
+__host__ __device__ inline uint64_t hp_xyf2ring(const Healpix &h, int64_t ix, int64_t iy,
+                                               int face) {
+  const int64_t ns = static_cast<int64_t>(h.nside), nl4 = 4 * ns;
+  const int64_t jr = hp_jrll(face) * ns - ix - iy - 1;
+  int64_t nr, kshift, n_before;
+  if (jr < ns) {
+    nr = jr;
+    n_before = 2 * nr * (nr - 1);
+    kshift = 0;
+  } else if (jr > 3 * ns) {
+    nr = nl4 - jr;
+    n_before = static_cast<int64_t>(h.npix) - 2 * (nr + 1) * nr;
+    kshift = 0;
+  } else {
+    nr = ns;
+    n_before = static_cast<int64_t>(h.ncap) + (jr - ns) * nl4;
+    kshift = (jr - ns) & 1;
+  }
+  int64_t jp = (hp_jpll(face) * nr + ix - iy + 1 + kshift) / 2;
+  if (jp > nl4)
+    jp -= nl4;
+  else if (jp < 1)
+    jp += nl4;
+  return static_cast<uint64_t>(n_before + jp - 1);
+}
+
+// Value of the mask at nside ht.nside, pixel m, from the pivot mask at
+// hp.nside (Hampisk::duplicate, hamsk.h:55-103): a copy at the same nside;
+// 1 if any of the pixel's sub-pixels is > 0 when downgrading (else 0); the
+// parent pixel's value when upgrading.
+__host__ __device__ inline double hp_mask_value(const Healpix &ht, uint64_t m,
+                                                const double *piv, const Healpix &hp) {
+  if (hp.nside == ht.nside)
+    return piv[m];
+  int64_t x, y;
+  int f;
+  hp_ring2xyf(ht, m, x, y, f);
+  if (hp.nside > ht.nside) {
+    const int64_t fact = static_cast<int64_t>(hp.nside / ht.nside);
+    for (int64_t j = fact * y; j < fact * (y + 1); ++j)
+      for (int64_t i = fact * x; i < fact * (x + 1); ++i)
+        if (piv[hp_xyf2ring(hp, i, j, f)] > 0)
+          return 1.0;
+    return 0.0;
+  }
+  const int64_t fact = static_cast<int64_t>(ht.nside / hp.nside);
+  return piv[hp_xyf2ring(hp, x / fact, y / fact, f)];
+}
+
 } // namespace hamx

@@ -86,6 +86,8 @@ enum Slot : int {
   kSlotAdjPar,    // reduced parameter gradient
   kSlotAdjGrid,   // packed grid gradient
   kSlotTapeFd,    // every ray's FD over each shell (forward tape)
+  kSlotMaskPivot, // the caller's mask map (host input staged)
+  kSlotMaskShell, // the mask duplicated to the current shell's nside
   kSlotCount
 };
 static_assert(kSlotCount <= hamx_ctx::kSlots, "workspace slots");

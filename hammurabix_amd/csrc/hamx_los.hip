@@ -197,6 +197,17 @@ __device__ inline void store_ray(const Phys &ph, const ShellArgs &sa,
     sa.t_fd[ipix] = fd;
 }
 
+// Masked-out pixel (integrator.cc:63-70): every observable of the shell is
+// zero.  True when the caller should return.
+__device__ inline bool masked_out(const Phys &ph, const ShellArgs &sa, uint64_t ipix,
+                                  bool writer) {
+  if (!sa.mask || sa.mask[ipix] == 1.0)
+    return false;
+  if (writer)
+    store_ray(ph, sa, ipix, 0., 0., 0., 0., 0.);
+  return true;
+}
+
 template <bool PACKED>
 __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
                                                   ShellArgs sa) {
@@ -204,6 +215,8 @@ __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
   const uint64_t ipix =
       static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (ipix >= sa.hs.npix)
+    return; // uniform per wavefront
+  if (masked_out(ph, sa, ipix, lane == 0))
     return; // uniform per wavefront
   RayGeo r;
   ray_setup(ph, sa, ipix, r);
@@ -268,6 +281,8 @@ __global__ __launch_bounds__(256) void k_los_fast(Phys ph, FastConst fc, Grids g
       static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (ipix >= sa.hs.npix)
     return; // uniform per wavefront
+  if (masked_out(ph, sa, ipix, lane == 0))
+    return;
   RayGeo r;
   ray_setup(ph, sa, ipix, r);
   const double two_l2 = 2. * ph.lambda_sq;
@@ -291,7 +306,8 @@ __global__ __launch_bounds__(256) void k_los_fast(Phys ph, FastConst fc, Grids g
       dm += o.te;
     }
   }
-  dm = wave_sum(dm) * sa.delta;
+  dm = wave_sum(dm);
+  dm = sa.step ? dm * sa.delta : 0.; // a shell thinner than one step: no finite delta
   I = wave_sum(I);
   Q = wave_sum(Q);
   U = wave_sum(U);
@@ -304,6 +320,8 @@ __global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
                                                     ShellArgs sa) {
   const uint64_t ipix = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (ipix >= sa.hs.npix)
+    return;
+  if (masked_out(ph, sa, ipix, true))
     return;
   RayGeo r;
   ray_setup(ph, sa, ipix, r);
@@ -364,6 +382,14 @@ __global__ __launch_bounds__(256) void k_accumulate(AccPairs p, Healpix hd,
     const double v = (wtot == 0.0) ? cgs::undef : res / wtot;
     p.dst[m][i] = p.dst[m][i] + v;
   }
+}
+
+// The mask at a shell's nside from the caller's map (Hampisk::duplicate).
+__global__ __launch_bounds__(256) void k_mask_dup(double *out, Healpix ht, const double *piv,
+                                                  Healpix hp) {
+  const uint64_t m = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m < ht.npix)
+    out[m] = hp_mask_value(ht, m, piv, hp);
 }
 
 // Pack SoA brnd (x,y,z) and ternd into one double4 record per vertex.
@@ -657,6 +683,19 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
   bool packed = false, grids_any = false;
   HAMX_TRY(stage_fields(ctx, d, f, in_dev, gr, packed, grids_any));
 
+  // ---- mask, duplicated to every shell's nside (integrator.cc:36-39)
+  const double *mask_piv = nullptr;
+  Healpix h_mask;
+  std::memset(&h_mask, 0, sizeof(h_mask));
+  if (f.mask) {
+    if (!is_pow2(f.nside_mask))
+      return set_err(ctx, HAMX_ERR_NSIDE, "nside_mask %u is not a power of 2", f.nside_mask);
+    h_mask = healpix_make(f.nside_mask);
+    const void *mp;
+    HAMX_TRY(stage_in(ctx, kSlotMaskPivot, f.mask, h_mask.npix * sizeof(double), in_dev, &mp));
+    mask_piv = static_cast<const double *>(mp);
+  }
+
   // ---- output maps (device)
   const Phys ph = make_phys(d);
   const uint64_t nside_fd_map = d->do_fd ? d->nside_fd : d->nside_sync;
@@ -736,6 +775,14 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     sa.t_qs = t_qs;
     sa.t_us = t_us;
     const uint64_t npix = sa.hs.npix;
+    if (mask_piv) {
+      void *mq;
+      HAMX_TRY(ws_get(ctx, kSlotMaskShell, npix * sizeof(double), &mq));
+      hipLaunchKernelGGL(k_mask_dup, dim3(static_cast<unsigned>((npix + 255) / 256)), dim3(256),
+                         0, ctx->stream, static_cast<double *>(mq), sa.hs, mask_piv, h_mask);
+      HAMX_HIP(ctx, hipGetLastError());
+      sa.mask = static_cast<const double *>(mq);
+    }
     if (ckpt && d->do_fd) // FD map before shell s, for the adjoint
       HAMX_HIP(ctx, hipMemcpyAsync(ckpt + s * h_fd.npix, o_fd, h_fd.npix * sizeof(double),
                                    hipMemcpyDeviceToDevice, ctx->stream));

@@ -58,6 +58,7 @@ struct ShellArgs {
   Healpix hs;
   uint64_t step;
   double d_start, delta;
+  const double *mask;   // shell-nside mask: integrate where == 1.0 (NULL: all)
   const double *fd_map; // accumulated FD of inner shells (NULL: zero cache)
   Healpix hfd;
   double *t_dm, *t_fd, *t_is, *t_qs, *t_us; // shell maps, nside_s
@@ -177,6 +178,78 @@ __device__ inline double wave_suffix_scan(double v, int lane) {
       v += t;
   }
   return v;
+}
+
+// Orders the LDS accesses of one wavefront around this point (the lanes of
+// a wavefront exchange data through its own LDS slab; waves of a workgroup
+// never share a slab, so no workgroup barrier is involved).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS slab of one wavefront for scatter_runs: one slot of 32 values (8
+// corners x {x, y, z, te}) per possible run.
+constexpr int kSlabValues = 64 * 32;
+
+// Trilinear scatter-add of (g0, g1, g2, g3) of every lane with `has` into
+// the packed {x, y, z, te} grid g at the 8 corners of cell idx (the
+// transpose of lerp_cell_packed).  All 64 lanes must call it.  The lanes of
+// a wavefront hold consecutive samples of one ray, which fall into few
+// cells, and a straight ray meets each cell in one contiguous run of
+// samples: the lanes of a run add their 32 corner values into one slot of
+// the wavefront's LDS slab, then the run's first lane adds the slot to the
+// grid, so a run costs 32 global atomics instead of 32 per sample.
+template <typename T>
+__device__ inline void scatter_runs(T *__restrict__ g, T *slab, bool has, uint64_t idx, T xd,
+                                    T yd, T zd, T g0, T g1, T g2, T g3, uint64_t sx,
+                                    uint64_t sy, int lane) {
+  const uint64_t key = has ? idx : ~0ull;
+  const uint64_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(head);
+  const int run = __popcll(heads & ((2ull << lane) - 1ull)) - 1; // runs starting at <= lane
+  T *slot = slab + 32 * run;
+  if (head && has) {
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+      slot[i] = T(0);
+  }
+  wave_lds_sync();
+  if (has) {
+    const T wx[2] = {T(1) - xd, xd}, wy[2] = {T(1) - yd, yd}, wz[2] = {T(1) - zd, zd};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const T w = wx[a] * wy[b] * wz[c];
+          T *q = slot + 4 * (4 * a + 2 * b + c);
+          atomicAdd(q + 0, w * g0);
+          atomicAdd(q + 1, w * g1);
+          atomicAdd(q + 2, w * g2);
+          atomicAdd(q + 3, w * g3);
+        }
+  }
+  wave_lds_sync();
+  if (head && has) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const T *q = slot + 4 * (4 * a + 2 * b + c);
+          T *p = g + 4 * (idx + a * sx + b * sy + c);
+          unsafeAtomicAdd(p + 0, q[0]);
+          unsafeAtomicAdd(p + 1, q[1]);
+          unsafeAtomicAdd(p + 2, q[2]);
+          unsafeAtomicAdd(p + 3, q[3]);
+        }
+  }
+  wave_lds_sync();
 }
 
 // Per-shell constants of the specialised march (see make_fast).

@@ -239,6 +239,10 @@ void Grid_obs::build_grid(const Param *par) {
   }
   if (g.do_fd)
     fd_map = std::make_unique<Hampix>(g.nside_fd);
+  if (g.do_mask) {
+    mask_map = std::make_unique<Hampix>(g.nside_mask);
+    read_doubles(g.mask_name, {mask_map->data.data()}, mask_map->npix());
+  }
 }
 
 // grid_obs.cc:53-83: DM in pc cm^-3, FD in rad m^-2, _I/_Q/_U file names
@@ -468,8 +472,6 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
                             const Grid_ternd *gternd, const Grid_cre *, Grid_obs *gobs,
                             const Param *par) const {
   const auto &g = par->grid_obs;
-  if (g.do_mask)
-    throw std::runtime_error("hamx: masked LOS integration has no device path yet");
   if (g.total_shell < 1 || g.total_shell > HAMX_MAX_SHELL)
     throw std::runtime_error("hamx: unsupported shell count");
   hamx_los_desc d;
@@ -497,6 +499,12 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
   }
   hamx_fields f;
   std::memset(&f, 0, sizeof(f));
+  if (g.do_mask) { // Hampisk duplicated per shell on the device
+    if (!gobs->mask_map)
+      throw std::runtime_error("hamx: mask requested but Grid_obs holds no mask map");
+    f.mask = gobs->mask_map->data.data();
+    f.nside_mask = static_cast<uint32_t>(gobs->mask_map->nside);
+  }
   // regular fields: device model, imported grid, or a user model baked
   std::unique_ptr<Grid_breg> baked_b;
   std::unique_ptr<Grid_tereg> baked_t;
@@ -605,6 +613,22 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
       f.ternd = gternd->te.get();
   } else if (f.brnd_x || f.ternd) {
     flags |= HAMX_IN_DEVICE;
+  }
+  // the mask follows the residency of the other inputs
+  struct DevMask {
+    hamx_ctx *ctx = nullptr;
+    void *p = nullptr;
+    ~DevMask() {
+      if (p)
+        hamx_dev_free(ctx, p);
+    }
+  } dev_mask;
+  if (f.mask && (flags & HAMX_IN_DEVICE)) {
+    const size_t bytes = gobs->mask_map->npix() * sizeof(double);
+    dev_mask.ctx = ctx;
+    check(hamx_dev_alloc(ctx, bytes, &dev_mask.p), ctx);
+    check(hamx_memcpy_h2d(ctx, dev_mask.p, f.mask, bytes), ctx);
+    f.mask = static_cast<const double *>(dev_mask.p);
   }
   hamx_maps m;
   std::memset(&m, 0, sizeof(m));

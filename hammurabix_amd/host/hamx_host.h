@@ -160,6 +160,7 @@ public:
   void build_grid(const Param *) override;
   void export_grid(const Param *) override;
   std::unique_ptr<Hampix> dm_map, is_map, qs_map, us_map, fd_map;
+  std::unique_ptr<Hampix> mask_map; // <mask> map at nside_mask (grid_obs.cc:44-50)
 };
 
 // --------------------------------------------------------- field plugins --

@@ -423,8 +423,6 @@ class Param:
         if d.do_sync:
             d.nside_sync = g.nside_sync[-1]
             d.sync_freq = g.sim_sync_freq[-1]
-        if g.do_mask:
-            raise RuntimeError("masked LOS integration has no device path yet")
         # regular B
         if not self.grid_breg.build_permission:
             d.breg_model = capi.HAMX_BREG_NONE

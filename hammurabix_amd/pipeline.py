@@ -43,7 +43,8 @@ def sync_names(filename: str) -> tuple[str, str, str]:
 
 
 def import_grid(path: str, n: int, components: int) -> np.ndarray:
-    """Grid_brnd / Grid_ternd::import_grid: exactly n*components doubles."""
+    """Grid_brnd / Grid_ternd::import_grid (and the mask map): exactly
+    n*components doubles."""
     size = os.path.getsize(path)
     if size != n * components * 8:
         raise RuntimeError(f"grid file {path} has {size} bytes, expected "
@@ -72,6 +73,15 @@ class Pipeline:
         import torch  # device memory for resident grids
         self._torch = torch
         self._dev = torch.device("cuda", 0)
+
+    def load_mask(self) -> None:
+        """Grid_obs::build_grid (grid_obs.cc:44-50): the mask map, raw FP64 at
+        nside_mask, when <mask> is given."""
+        g = self.par.grid_obs
+        self.mask = None
+        if g.do_mask:
+            n = 12 * g.nside_mask * g.nside_mask
+            self.mask = self._torch.from_numpy(import_grid(g.mask_name, n, 1)).to(self._dev)
 
     def _zeros(self, n: int):
         return self._torch.zeros(n, dtype=self._torch.float64, device=self._dev)
@@ -134,6 +144,8 @@ class Pipeline:
             kw.update(breg=self.breg_grid, breg_box=self.par.grid_breg.box())
         if getattr(self, "tereg_grid", None) is not None:
             kw.update(tereg=self.tereg_grid, tereg_box=self.par.grid_tereg.box())
+        if getattr(self, "mask", None) is not None:
+            kw.update(mask=self.mask, nside_mask=self.par.grid_obs.nside_mask)
         return kw
 
     def assemble_obs(self) -> None:
@@ -194,5 +206,6 @@ class Pipeline:
         self.assemble_ternd()
         self.assemble_brnd()
         self.assemble_cre()
+        self.load_mask()
         self.assemble_obs()
         return self.maps

@@ -112,11 +112,17 @@ class Context:
                     breg: Optional[tuple[Any, Any, Any]] = None,
                     breg_box: Optional[capi.HamxBox] = None,
                     tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None,
-                    keep_tape: bool = False) -> None:
+                    keep_tape: bool = False, mask: Any = None, nside_mask: int = 0) -> None:
         """One Integrator::write_grid pass into `maps` (keys dm, fd, is, qs, us).
-        keep_tape leaves the pass's tape in the context for los_adjoint."""
+        keep_tape leaves the pass's tape in the context for los_adjoint; mask
+        (a map at nside_mask) restricts the integration to pixels where its
+        duplicate at each shell's nside is 1.0."""
         f = capi.HamxFields()
         inputs = []
+        if mask is not None:
+            f.mask = _ptr(mask)[0]
+            f.nside_mask = nside_mask
+            inputs.append(mask)
         if brnd is not None:
             f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
             f.brnd_box = brnd_box

@@ -89,6 +89,13 @@ typedef struct hamx_fields {
   hamx_box breg_box;
   const double *tereg;
   hamx_box tereg_box;
+  /* Grid_obs mask map (grid_obs.cc:44-50) at nside_mask, or NULL.  Each
+   * shell uses it duplicated to its own nside (Hampisk::duplicate,
+   * hamsk.h:55-103), and a pixel is integrated only where that map equals
+   * 1.0; the others yield zero (integrator.cc:63-70). */
+  const double *mask;
+  uint32_t nside_mask;
+  uint32_t reserved;
 } hamx_fields;
 
 /* One Integrator::write_grid pass (integrator.cc:23-134) for one <sync>

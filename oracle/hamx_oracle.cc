@@ -117,6 +117,69 @@ struct Ring {
   static uint64_t isqrt(uint64_t v) {
     return static_cast<uint64_t>(std::floor(std::sqrt(static_cast<double>(v) + 0.5)));
   }
+  // base-pixel layout of the HEALPix faces
+  static int64_t jrll(int f) { static const int t[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4}; return t[f]; }
+  static int64_t jpll(int f) { static const int t[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7}; return t[f]; }
+  // Hampix::rpixxyf (hamdis.h:515-570)
+  void ring2xyf(uint64_t pix, int64_t &ix, int64_t &iy, int &face) const {
+    const int64_t ns = (int64_t)nside, nl2 = 2 * ns;
+    int64_t iring, iphi, kshift, nr;
+    if (pix < ncap) {
+      iring = (1 + (int64_t)isqrt(1 + 2 * pix)) >> 1;
+      iphi = (int64_t)pix + 1 - 2 * iring * (iring - 1);
+      kshift = 0;
+      nr = iring;
+      face = (int)((iphi - 1) / nr);
+    } else if (pix < npix - ncap) {
+      const int64_t ip = (int64_t)(pix - ncap);
+      const int64_t tmp = ip / (4 * ns);
+      iring = tmp + ns;
+      iphi = ip - tmp * 4 * ns + 1;
+      kshift = (iring + ns) & 1;
+      nr = ns;
+      const int64_t ire = tmp + 1, irm = nl2 + 1 - tmp;
+      const int64_t ifm = (iphi - (ire >> 1) + ns - 1) / ns;
+      const int64_t ifp = (iphi - (irm >> 1) + ns - 1) / ns;
+      face = (int)((ifp == ifm) ? (ifp | 4) : ((ifp < ifm) ? ifp : (ifm + 8)));
+    } else {
+      const int64_t ip = (int64_t)(npix - pix);
+      iring = (1 + (int64_t)isqrt(2 * ip - 1)) >> 1;
+      iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+      kshift = 0;
+      nr = iring;
+      iring = 2 * nl2 - iring;
+      face = 8 + (int)((iphi - 1) / nr);
+    }
+    const int64_t irt = iring - jrll(face) * ns + 1;
+    int64_t ipt = 2 * iphi - jpll(face) * nr - kshift - 1;
+    if (ipt >= nl2)
+      ipt -= 8 * ns;
+    ix = (ipt - irt) >> 1;
+    iy = (-ipt - irt) >> 1;
+  }
+  // Hampix::xyfrpix (hamdis.h:572-605)
+  uint64_t xyf2ring(int64_t ix, int64_t iy, int face) const {
+    const int64_t ns = (int64_t)nside, nl4 = 4 * ns;
+    const int64_t jr = jrll(face) * ns - ix - iy - 1;
+    int64_t nr, n_before, kshift;
+    if (jr < ns) {
+      nr = jr;
+      n_before = 2 * nr * (nr - 1);
+      kshift = 0;
+    } else if (jr < 3 * ns) {
+      nr = ns;
+      n_before = (int64_t)ncap + (jr - ns) * nl4;
+      kshift = (jr - ns) & 1;
+    } else {
+      nr = nl4 - jr;
+      n_before = (int64_t)npix - 2 * nr * (nr + 1);
+      kshift = 0;
+    }
+    int64_t jp = (jpll(face) * nr + ix - iy + 1 + kshift) / 2;
+    if (jp < 1)
+      jp += nl4;
+    return (uint64_t)(n_before + jp - 1);
+  }
   // fillpoint (hamdis.h:302-356) + Hamp::norm (hamp.h:23-28)
   void pix2ang(uint64_t idx, double &theta, double &phi) const {
     double z = 0.0, sth = 0.0;
@@ -621,6 +684,40 @@ void orc_radial_integration(const hamx_los_desc *d, const hamx_fields *f,
   radial(d, f, sh, theta, phi, fd_inner, out);
 }
 
+// Hampisk::duplicate (hamsk.h:55-103): the pivot mask at nside `piv_nside`
+// brought to nside `nside`.
+static std::vector<double> mask_duplicate(const double *piv, uint64_t piv_nside,
+                                          uint64_t nside) {
+  const Ring rt(nside), rp(piv_nside);
+  std::vector<double> out(rt.npix, 0.);
+  if (piv_nside == nside) {
+    out.assign(piv, piv + rt.npix);
+  } else if (piv_nside > nside) {
+    const int64_t fact = (int64_t)(piv_nside / nside);
+    for (uint64_t m = 0; m < rt.npix; ++m) {
+      int64_t x, y;
+      int f;
+      rt.ring2xyf(m, x, y, f);
+      bool hit = false;
+      for (int64_t j = fact * y; j < fact * (y + 1) && !hit; ++j)
+        for (int64_t i = fact * x; i < fact * (x + 1) && !hit; ++i)
+          hit = piv[rp.xyf2ring(i, j, f)] > 0;
+      out[m] = hit ? 1. : 0.;
+    }
+  } else { // upgrade: every child pixel takes its parent's value
+    const int64_t fact = (int64_t)(nside / piv_nside);
+    for (uint64_t m = 0; m < rp.npix; ++m) {
+      int64_t x, y;
+      int f;
+      rp.ring2xyf(m, x, y, f);
+      for (int64_t j = fact * y; j < fact * (y + 1); ++j)
+        for (int64_t i = fact * x; i < fact * (x + 1); ++i)
+          out[rt.xyf2ring(i, j, f)] = piv[m];
+    }
+  }
+  return out;
+}
+
 // Integrator::write_grid (integrator.cc:23-134) for one sync entry.
 int orc_write_grid(const hamx_los_desc *d, const hamx_fields *f, hamx_maps *out,
                    int nthreads) {
@@ -664,8 +761,13 @@ int orc_write_grid(const hamx_los_desc *d, const hamx_fields *f, hamx_maps *out,
     }
     if (d->do_fd || d->do_sync)
       t_fd.assign(rs.npix, 0.);
+    std::vector<double> msk; // integrator.cc:36-39
+    if (f && f->mask)
+      msk = mask_duplicate(f->mask, f->nside_mask, nside);
 #pragma omp parallel for schedule(dynamic)
     for (int64_t ipix = 0; ipix < (int64_t)rs.npix; ++ipix) {
+      if (!msk.empty() && msk[ipix] != 1.0)
+        continue; // masked: every observable of the pixel stays 0 (integrator.cc:63-70)
       double the, phi;
       rs.pix2ang(ipix, the, phi);
       double fd_inner = 0.;
@@ -703,6 +805,19 @@ int orc_write_grid(const hamx_los_desc *d, const hamx_fields *f, hamx_maps *out,
   put(out->us, us_map);
   put(out->fd, fd_map);
   return HAMX_OK;
+}
+
+void orc_ring2xyf(uint64_t nside, uint64_t pix, int64_t *ix, int64_t *iy, int *face) {
+  Ring(nside).ring2xyf(pix, *ix, *iy, *face);
+}
+
+uint64_t orc_xyf2ring(uint64_t nside, int64_t ix, int64_t iy, int face) {
+  return Ring(nside).xyf2ring(ix, iy, face);
+}
+
+void orc_mask_duplicate(const double *piv, uint64_t piv_nside, uint64_t nside, double *out) {
+  const std::vector<double> v = mask_duplicate(piv, piv_nside, nside);
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
 }
 
 } // extern "C"

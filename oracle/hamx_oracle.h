@@ -24,6 +24,11 @@ extern "C" {
 /* ---- HEALPix (hamdis.h) ---- */
 void orc_pix2ang(uint64_t nside, uint64_t ipix, double *theta, double *phi);
 void orc_pix2ang_map(uint64_t nside, double *theta_phi /* 2*npix */);
+/* Hampix::rpixxyf / xyfrpix (hamdis.h:515-605) */
+void orc_ring2xyf(uint64_t nside, uint64_t pix, int64_t *ix, int64_t *iy, int *face);
+uint64_t orc_xyf2ring(uint64_t nside, int64_t ix, int64_t iy, int face);
+/* Hampisk::duplicate (hamsk.h:55-103) */
+void orc_mask_duplicate(const double *piv, uint64_t piv_nside, uint64_t nside, double *out);
 double orc_interpolate(const double *map, uint64_t nside, double theta,
                        double phi);
 void orc_accumulate(double *dst, uint64_t nside_dst, const double *src,

@@ -46,6 +46,10 @@ PROTOS = {
     "orc_ternd_dft": (C.c_int, [C.POINTER(capi.HamxDftDesc), C.c_uint64, _vp]),
     "orc_es_spectrum": (C.c_double, [C.POINTER(capi.HamxEsDesc), C.c_double]),
     "orc_dft_spectrum": (C.c_double, [C.POINTER(capi.HamxDftDesc), C.c_double]),
+    "orc_ring2xyf": (None, [C.c_uint64, C.c_uint64, C.POINTER(C.c_int64),
+                            C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "orc_xyf2ring": (C.c_uint64, [C.c_uint64, C.c_int64, C.c_int64, C.c_int]),
+    "orc_mask_duplicate": (None, [_vp, C.c_uint64, C.c_uint64, _vp]),
 }
 
 _lib = None
@@ -73,8 +77,12 @@ def ptr(a):
 
 
 def fields_struct(brnd=None, brnd_box=None, ternd=None, ternd_box=None,
-                  breg=None, breg_box=None, tereg=None, tereg_box=None):
+                  breg=None, breg_box=None, tereg=None, tereg_box=None,
+                  mask=None, nside_mask=0):
     f = capi.HamxFields()
+    if mask is not None:
+        f.mask = ptr(mask)
+        f.nside_mask = nside_mask
     if brnd is not None:
         f.brnd_x, f.brnd_y, f.brnd_z = (ptr(a) for a in brnd)
         f.brnd_box = brnd_box

@@ -15,7 +15,7 @@ import pytest
 
 from hammurabix_amd import HamxError, Param, _capi as capi, configs
 from hammurabix_amd.units import CGS
-from oracle_lib import npix
+from oracle_lib import npix, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -254,9 +254,11 @@ def test_autograd_function_matches_c_abi_and_fd(ctx):
     adj = {k: np.ascontiguousarray((-(maps[k].detach() - data[k]) / sigma[k] ** 2).cpu().numpy())
            for k in data}
     dp, g = adjoint(ctx, d, adj, grids, with_grid=True)
-    np.testing.assert_allclose(p.grad.numpy(), dp, rtol=1e-12, atol=0)
-    np.testing.assert_allclose(b[0].grad.cpu().numpy(), g["dbrnd"][0], rtol=1e-12, atol=0)
-    np.testing.assert_allclose(te.grad.cpu().numpy(), g["dternd"], rtol=1e-12, atol=0)
+    # FP64 atomics sum in a run-dependent order: equal to ~1e-15 of each sum
+    np.testing.assert_allclose(p.grad.numpy(), dp, rtol=1e-10, atol=0)
+    for got, ref in ((b[0].grad.cpu().numpy(), g["dbrnd"][0]),
+                     (te.grad.cpu().numpy(), g["dternd"])):
+        assert rel_l2(got, ref) <= 1e-10
     # central difference of the likelihood in b0
     h = H_REL * float(p[0])
 
@@ -314,9 +316,10 @@ def test_tape_reuse_matches_recompute(ctx):
     gb = tuple(np.zeros(n) for _ in range(3))
     gt = np.zeros(n)
     ctx.los_adjoint(d, w, dp, dbrnd=gb, dternd=gt, use_tape=True, **grids)
-    assert np.array_equal(dp, r_re)
-    assert all(np.array_equal(a, b) for a, b in zip(gb, g_re["dbrnd"]))
-    assert np.array_equal(gt, g_re["dternd"])
+    # same tape values; only the order of the FP64 atomic sums differs
+    np.testing.assert_allclose(dp, r_re, rtol=1e-10, atol=0)
+    assert all(rel_l2(a, b) <= 1e-10 for a, b in zip(gb, g_re["dbrnd"]))
+    assert rel_l2(gt, g_re["dternd"]) <= 1e-10
     d2, _ = multishell_case(seed=16)
     d2.breg[0] *= 1.1
     with pytest.raises(HamxError) as e:

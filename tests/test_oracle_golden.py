@@ -13,7 +13,7 @@ import os
 import numpy as np
 import pytest
 
-from hammurabix_amd import Param, _capi as capi
+from hammurabix_amd import Param, _capi as capi, configs
 from hammurabix_amd.units import CGS
 from oracle_lib import fields_struct, npix, ptr, write_grid
 
@@ -474,3 +474,52 @@ def test_dft_field_rms_matches_profile(orc):
     prof = np.exp(-(np.sqrt(X ** 2 + Y ** 2) - 8.3) / 8) * np.exp(-(np.abs(Z) - 0.006) / 1)
     w = te.reshape(shp) / np.sqrt(prof)
     assert abs(np.sqrt((w ** 2).mean()) / 0.001 - 1) < 0.05
+
+
+# ---- masks: Hampisk::duplicate and the HEALPix (x, y, face) indexing
+
+def golden_map(name):
+    return np.fromfile(os.path.join(GOLDEN, name), dtype="<f8")
+
+
+def test_xyf_ring_round_trip(orc):
+    """ring2xyf / xyf2ring are inverse bijections at every pixel, and the
+    (x, y) coordinates stay inside the face."""
+    for nside in (1, 2, 4, 8, 16):
+        seen = set()
+        for pix in range(12 * nside * nside):
+            ix, iy, f = C.c_int64(), C.c_int64(), C.c_int()
+            orc.orc_ring2xyf(nside, pix, C.byref(ix), C.byref(iy), C.byref(f))
+            assert 0 <= ix.value < nside and 0 <= iy.value < nside and 0 <= f.value < 12
+            seen.add((ix.value, iy.value, f.value))
+            assert orc.orc_xyf2ring(nside, ix.value, iy.value, f.value) == pix
+        assert len(seen) == 12 * nside * nside
+
+
+def test_mask_duplicate_matches_reference_fixtures(orc):
+    """hamsk_tests.cc:33-56: the nside-8 mask downgraded to 2 and upgraded
+    to 64 equals the reference's fixtures element for element."""
+    piv = np.ascontiguousarray(golden_map("mask_map_nside8.bin"))
+    assert piv.size == 12 * 64
+    for nside, name in ((2, "mask_map_nside2.bin"), (64, "mask_map_nside64.bin")):
+        out = np.zeros(12 * nside * nside)
+        orc.orc_mask_duplicate(ptr(piv), 8, nside, ptr(out))
+        np.testing.assert_array_equal(out, golden_map(name))
+    same = np.zeros_like(piv)
+    orc.orc_mask_duplicate(ptr(piv), 8, 8, ptr(same))
+    np.testing.assert_array_equal(same, piv)
+
+
+def test_masked_write_grid_zeroes_masked_pixels_only(orc):
+    """One shell at the mask's nside: masked pixels give 0 in every map, the
+    others equal the unmasked pass (integrator.cc:63-70)."""
+    par = Param(text=configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.1))
+    d = par.los_desc()
+    piv = np.ascontiguousarray(golden_map("mask_map_nside8.bin"))
+    ref = write_grid(d)
+    got = write_grid(d, mask=piv, nside_mask=8)
+    keep = piv == 1.0
+    assert 0 < keep.sum() < keep.size
+    for k in ref:
+        np.testing.assert_array_equal(got[k][~keep], 0.0)
+        np.testing.assert_array_equal(got[k][keep], ref[k][keep])
