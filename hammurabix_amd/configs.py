@@ -28,7 +28,7 @@ _TEMPLATE = """<?xml version="1.0"?>
     <faraday cue="{do_fd}" filename="{out}fd.bin" nside="{nside_out}"/>
 {sync}
   </observable>
-  <mask cue="0" filename="mask.bin" nside="32"/>
+  <mask cue="{mask_on}" filename="{mask_file}" nside="{nside_mask}"/>
   <fieldio>
     <breg read="0" write="0" filename="breg.bin"/>
     <brnd read="{brnd_read}" write="0" filename="{brnd_file}"/>
@@ -145,7 +145,7 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         seed: int = 1, b0: float = 6.0, psi0: float = 27.0, te_k0: float = 2.0,
         do_dm: bool = True,
         do_fd: bool = True, brnd_file: str = "", ternd_file: str = "",
-        out: str = "") -> str:
+        out: str = "", mask_file: str = "", nside_mask: int = 32) -> str:
     freqs = [23.0] if freqs is None else freqs
     return _TEMPLATE.format(
         do_dm=int(do_dm), do_fd=int(do_fd), nside_out=nside_out,
@@ -154,7 +154,8 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         brnd_file=brnd_file or "brnd.bin", ternd_file=ternd_file or "ternd.bin",
         nx=grid[0], ny=grid[1], nz=grid[2], shell_num=shell_num, nside_sim=nside_sim,
         oc_r_max=oc_r_max, oc_r_res=oc_r_res, b0=b0, psi0=psi0,
-        brnd_on=int(random), ternd_on=int(random), seed=seed, te_k0=te_k0)
+        brnd_on=int(random), ternd_on=int(random), seed=seed, te_k0=te_k0,
+        mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask)
 
 
 def plain_forward(**kw) -> str:

@@ -76,3 +76,26 @@ def test_cli_matches_python_pipeline_and_oracle(tmp_path, ctx):
     assert rel_l2(dm, ref["dm"] * CGS.ccm / CGS.pc) < 1e-10
     q = np.fromfile(str(tmp_path / "sync_1.4_Q.bin"), dtype="<f8")
     assert rel_l2(q, ref["qs"]) < 1e-10
+
+
+@pytest.mark.gpu
+def test_cli_and_pipeline_with_mask(tmp_path, ctx):
+    """<mask cue="1">: the CLI and the Python pipeline read the mask file and
+    give the oracle's masked maps; masked pixels are zero."""
+    need_cli()
+    import os
+    out = str(tmp_path) + "/"
+    mfile = os.path.join(os.path.dirname(__file__), "golden", "mask_map_nside8.bin")
+    text = configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.1, out=out,
+                                 mask_file=mfile, nside_mask=8)
+    x = tmp_path / "m.xml"
+    x.write_text(text)
+    r = run(str(x))
+    assert r.returncode == 0, r.stderr
+    py = Pipeline(text=text, ctx=ctx, export=False).run()
+    piv = np.ascontiguousarray(np.fromfile(mfile, dtype="<f8"))
+    ref = write_grid(Param(text=text).los_desc(), mask=piv, nside_mask=8)
+    q_cli = np.fromfile(str(tmp_path / "sync_23.0_Q.bin"), dtype="<f8")
+    assert np.array_equal(q_cli, py[("sync", "23.0", "8", "Q")])
+    assert rel_l2(q_cli, ref["qs"]) < 1e-10
+    assert np.all(q_cli[piv != 1.0] == 0.0)
