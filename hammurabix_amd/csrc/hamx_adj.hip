@@ -54,21 +54,30 @@ struct AdjShell {
 template <bool GRIDS, bool PW1>
 __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst ac,
                                                  Grids gr, ShellArgs sa, AdjShell as) {
-  __shared__ double slabs[4 * kSlabValues]; // one per wavefront of the 256-thread block
-  const int lane = threadIdx.x & 63;
-  double *slab = slabs + (threadIdx.x >> 6) * kSlabValues;
-  const uint64_t ipix =
-      static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (ipix >= sa.hs.npix)
+  // kRaysPerWave consecutive rays per wavefront (set up one per lane), and a
+  // per-wavefront cell cache that merges their grid-gradient contributions
+  __shared__ RayGeo slots[4][kRaysPerWave];
+  __shared__ unsigned char cache_mem[4][cell_cache_bytes<double>()];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kRaysPerWave;
+  if (p0 >= sa.hs.npix)
     return; // uniform per wavefront
-  RayGeo r;
-  ray_setup(ph, sa, ipix, r);
+  const uint64_t nr = (sa.hs.npix - p0 < static_cast<uint64_t>(kRaysPerWave))
+                          ? sa.hs.npix - p0
+                          : static_cast<uint64_t>(kRaysPerWave);
+  if (static_cast<uint64_t>(lane) < nr)
+    ray_setup(ph, sa, p0 + lane, slots[wave][lane]);
+  const CellCache<double> cache = cell_cache_at<double>(cache_mem[wave]);
+  cell_cache_init(cache, lane);
+  uint32_t chunk_id = 0;
+  for (uint64_t jr = 0; jr < nr; ++jr) {
+  const uint64_t ipix = p0 + jr;
+  const RayGeo r = slots[wave][jr];
   const double gq = as.gq ? as.gq[ipix] * ac.tc : 0.;
   const double gu = as.gu ? as.gu[ipix] * ac.tc : 0.;
   const double gi = as.gi ? as.gi[ipix] * ac.tc : 0.;
   const double gdm = as.gdm ? as.gdm[ipix] : 0.;
   const double gfd = as.gfd ? as.gfd[ipix] : 0.;
-
   // One sweep from the far end of the ray to the observer.  The running FD
   // at sample k is the ray's total (from the forward's tape) minus the
   // samples beyond k, and the suffix sum of dL/dqui is carried the same way,
@@ -138,8 +147,8 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     }
     if (GRIDS && as.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      scatter_runs<double>(as.ggrid, slab, has, has ? o.cell.idx : 0, o.cell.xd, o.cell.yd,
-                           o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane);
+      scatter_cached<double>(cache, as.ggrid, has, has ? o.cell.idx : 0, o.cell.xd, o.cell.yd,
+                             o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane, ++chunk_id);
     }
   }
   p0 = wave_sum(p0);
@@ -159,6 +168,9 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     if (as.gfdin)
       as.gfdin[ipix] = ac.l2 * dq_after;
   }
+  } // rays of the wavefront
+  if (GRIDS && as.ggrid)
+    cell_cache_flush_all(cache, as.ggrid, as.sx, as.sy, lane);
 }
 
 // dst (shell maps) += A^T(src) for up to 5 map pairs (the transpose of
@@ -503,7 +515,8 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
     as.sx = gr.g_packed.ny * gr.g_packed.nz;
     const FastConst fc = make_fast(d, ph, sa.delta, pw1);
     ac.delta = sa.delta;
-    const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
+    const unsigned blocks =
+        static_cast<unsigned>((npix + 4 * kRaysPerWave - 1) / (4 * kRaysPerWave));
     ProfMark pm = prof_begin(ctx);
     if (packed && pw1)
       hipLaunchKernelGGL((k_los_adj<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,

@@ -55,6 +55,7 @@ namespace {
 constexpr int kMaxF = HAMX_MAX_FREQ;
 constexpr int kQueues = 8;   // one ticket queue per XCD
 constexpr int kAccMaps = 12; // 3 Stokes maps x 4 frequencies
+constexpr int kItemRaysMax = 16; // rays per work item (set up one per lane)
 
 struct BShell {
   Healpix hs;
@@ -91,6 +92,12 @@ struct BInst {
   FastConst fc[HAMX_MAX_SHELL]; // pass-0 specialised constants per shell
   BShell sh[HAMX_MAX_SHELL];
   F32Const f;
+  // FP32 mode: squared limits and grid scales (the decisions are made on
+  // FP64 positions; comparing squares instead of square roots, and scaling
+  // by a reciprocal, move a decision only for a sample within rounding of a
+  // boundary)
+  double r_min2, r_max2, te_r02, cre_r02;
+  double gx0, gy0, gz0, gsx, gsy, gsz; // grid origin and (n - 1) / extent
   int n_shell, n_freq, grids, pw1;
   int do_dm, do_fd, pad0, pad1;
   // per pass (index 0 = first pass)
@@ -151,13 +158,12 @@ __device__ __noinline__ void ray_setup_call(const Phys &ph, const ShellArgs &sa,
 // ------------------------------------------------------------- FP64 ray --
 
 template <bool GRIDS, bool PW1>
-__device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane) {
+__device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane,
+                              const RayGeo &r) {
   const ShellArgs sa = shell_args(in, s);
   Grids gr;
   gr.packed = static_cast<const double4 *>(in.packed);
   gr.g_packed = in.geo;
-  RayGeo r;
-  ray_setup_call(in.ph, sa, ipix, r);
   const FastConst &fc = in.fc[s];
   const int nf = in.n_freq;
   double dm = 0., I = 0., Q[kMaxF] = {0., 0., 0., 0.}, U[kMaxF] = {0., 0., 0., 0.};
@@ -235,6 +241,29 @@ __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, 
   w.w = (a0.w * z1 + a1.w * zd) * y1 + (b0.w * z1 + b1.w * zd) * yd;
 }
 
+// grid_cell with the division by the extent replaced by a product with
+// (n - 1) / extent (FP32 mode).
+__device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c) {
+  const GridGeo &g = in.geo;
+  double t = (p.x - in.gx0) * in.gsx;
+  if (!(t > 0 && t < g.fx))
+    return false;
+  const uint64_t xl = static_cast<uint64_t>(floor(t));
+  c.xd = t - static_cast<double>(xl);
+  t = (p.y - in.gy0) * in.gsy;
+  if (!(t > 0 && t < g.fy))
+    return false;
+  const uint64_t yl = static_cast<uint64_t>(floor(t));
+  c.yd = t - static_cast<double>(yl);
+  t = (p.z - in.gz0) * in.gsz;
+  if (!(t > 0 && t < g.fz))
+    return false;
+  const uint64_t zl = static_cast<uint64_t>(floor(t));
+  c.zd = t - static_cast<double>(zl);
+  c.idx = (xl * g.ny + yl) * g.nz + zl;
+  return true;
+}
+
 template <bool GRIDS, bool PW1>
 __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo &r,
                                 const float3 &lf, const float3 &ethf, const float3 &ephf,
@@ -242,17 +271,17 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   const Phys &ph = in.ph;
   const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
   const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
-  const double len = length0(pos);
-  if (len < ph.gc_r_min || len > ph.gc_r_max || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+  const double len2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
+  if (len2 < in.r_min2 || len2 > in.r_max2 || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
     return false;
   const double inv_kpc = 1.0 / cgs::kpc;
-  const double rr = sqrt(pos.x * pos.x + pos.y * pos.y);
-  const float rk = static_cast<float>(rr * inv_kpc);
+  const double rr2 = pos.x * pos.x + pos.y * pos.y;
+  const float rk = sqrtf(static_cast<float>(rr2 * (inv_kpc * inv_kpc)));
   const float zk = static_cast<float>(pos.z * inv_kpc);
   const F32Const &c = in.f;
   float bx = 0.f, by = 0.f, bz = 0.f;
-  // WMAP-LSA window decided in FP64 (breg_lsa.cc:9-26)
-  o.lsa_in = !(rr > (20. * cgs::kpc) || rr < (3. * cgs::kpc));
+  // WMAP-LSA window (breg_lsa.cc:9-26)
+  o.lsa_in = !(rr2 > (400. * cgs::kpc * cgs::kpc) || rr2 < (9. * cgs::kpc * cgs::kpc));
   if (o.lsa_in) {
     const float xk = static_cast<float>(pos.x * inv_kpc);
     const float yk = static_cast<float>(pos.y * inv_kpc);
@@ -275,15 +304,15 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   }
   float te = 0.f;
   {
-    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
-    o.te_in = !(length0(d) > ph.tereg[1]);
+    const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
+    o.te_in = !(dx * dx + dy * dy + dz * dz > in.te_r02);
     if (o.te_in)
       te = c.n0;
   }
   o.cell_ok = false;
   if (GRIDS) {
     Cell cell;
-    if (grid_cell(in.geo, pos, cell)) {
+    if (grid_cell_scaled(in, pos, cell)) {
       o.cell_ok = true;
       const float4 *g = static_cast<const float4 *>(in.packed);
       const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
@@ -314,8 +343,8 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   o.bper2 = cx * cx + cy * cy + cz * cz;
   float prof;
   if (ph.cre_model == HAMX_CRE_UNIF) {
-    const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
-    prof = (length0(d) > ph.cre[1]) ? 0.f : 1.f;
+    const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
+    prof = (dx * dx + dy * dy + dz * dz > in.cre_r02) ? 0.f : 1.f;
   } else {
     const float e = __expf(zk * c.inv_z0k);
     const float ch = 0.5f * (e + 1.f / e);
@@ -340,11 +369,9 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
 }
 
 template <bool GRIDS, bool PW1>
-__device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane) {
-  const ShellArgs sa = shell_args(in, s);
+__device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane,
+                              const RayGeo &r) {
   const BShell &sh = in.sh[s];
-  RayGeo r;
-  ray_setup_call(in.ph, sa, ipix, r);
   const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
                                 static_cast<float>(r.l.z));
   const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
@@ -424,12 +451,11 @@ __device__ inline ShellArgs shell_args_rev(const BInst &in, int s) {
 }
 
 template <bool GRIDS, bool PW1>
-__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, float *slab) {
-  const ShellArgs sa = shell_args_rev(in, s);
+__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, void *slab,
+                            uint32_t *chunk_id, const RayGeo &r) {
+  const bool has_cache = in.do_fd && s > 0;
   const BShell &sh = in.sh[s];
   const BAdj &A = in.adj;
-  RayGeo r;
-  ray_setup_call(in.ph, sa, ipix, r);
   const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
                                 static_cast<float>(r.l.z));
   const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
@@ -535,8 +561,9 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, flo
     }
     if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      scatter_runs<float>(static_cast<float *>(A.ggrid), slab, has, has ? o.idx : 0, o.xd,
-                          o.yd, o.zd, dbx, dby, dbz, static_cast<float>(dte), sx, sy, lane);
+      scatter_cached<float>(cell_cache_at<float>(slab), static_cast<float *>(A.ggrid), has,
+                            has ? o.idx : 0, o.xd, o.yd, o.zd, dbx, dby, dbz,
+                            static_cast<float>(dte), sx, sy, lane, ++*chunk_id);
     }
   }
   p0 = wave_sum(p0);
@@ -554,20 +581,19 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, flo
     rp[3] = p3;
     rp[4] = p4;
     rp[5] = p5;
-    if (sa.fd_map && in.fdin[0])
+    if (has_cache && in.fdin[0])
       A.gfdin[ipix] = A.l2[0] * q0;
   }
 }
 
 template <bool GRIDS, bool PW1>
-__device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, double *slab) {
+__device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, void *slab,
+                            uint32_t *chunk_id, const RayGeo &r) {
   const ShellArgs sa = shell_args_rev(in, s);
   const BAdj &A = in.adj;
   Grids gr;
   gr.packed = static_cast<const double4 *>(in.packed);
   gr.g_packed = in.geo;
-  RayGeo r;
-  ray_setup_call(in.ph, sa, ipix, r);
   const FastConst &fc = in.fc[s];
   const int nf = in.n_freq;
   // per-ray constants: dL/d(pass-0 jt) and the Q / U weights of each pass
@@ -658,8 +684,9 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, dou
     }
     if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      scatter_runs<double>(static_cast<double *>(A.ggrid), slab, has, has ? o.cell.idx : 0,
-                           o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte, sx, sy, lane);
+      scatter_cached<double>(cell_cache_at<double>(slab), static_cast<double *>(A.ggrid), has,
+                             has ? o.cell.idx : 0, o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y,
+                             dB.z, dte, sx, sy, lane, ++*chunk_id);
     }
   }
   p0 = wave_sum(p0);
@@ -685,7 +712,7 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, dou
 // ---------------------------------------------------------- work items --
 
 template <int PREC, bool GRIDS, bool PW1, bool REV>
-__device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab) {
+__device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab, RayGeo *rays) {
   uint32_t lo = 0, hi = a.n_ids;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
@@ -700,19 +727,39 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab)
   const uint64_t npix = in.sh[s].hs.npix;
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
   const uint64_t p1 = (p0 + a.rays < npix) ? p0 + a.rays : npix;
-  for (uint64_t p = p0; p < p1; ++p) {
+  // the item's rays are set up one per lane (pointing, versors, FD cache,
+  // culled step range: scalar FP64 work that every lane would otherwise
+  // repeat) into the wavefront's LDS slots, then marched one after another
+  const uint64_t nr = p1 - p0; // <= kItemRaysMax (plan)
+  if (static_cast<uint64_t>(lane) < nr) {
+    const ShellArgs sa = REV ? shell_args_rev(in, s) : shell_args(in, s);
+    ray_setup_call(in.ph, sa, p0 + lane, rays[lane]);
+  }
+  using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
+  const bool cached = REV && GRIDS && in.adj.ggrid;
+  if (cached) // the item's rays share one cell cache, flushed at the end
+    cell_cache_init(cell_cache_at<T>(slab), lane);
+  wave_lds_sync();
+  uint32_t chunk = 0;
+  for (uint64_t j = 0; j < nr; ++j) {
+    const RayGeo r = rays[j];
+    const uint64_t p = p0 + j;
     if (REV) {
       if (PREC == HAMX_PREC_F32)
-        adj_ray_f32<GRIDS, PW1>(in, s, p, lane, static_cast<float *>(slab));
+        adj_ray_f32<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
       else
-        adj_ray_f64<GRIDS, PW1>(in, s, p, lane, static_cast<double *>(slab));
+        adj_ray_f64<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
     } else {
       if (PREC == HAMX_PREC_F32)
-        march_ray_f32<GRIDS, PW1>(in, s, p, lane);
+        march_ray_f32<GRIDS, PW1>(in, s, p, lane, r);
       else
-        march_ray_f64<GRIDS, PW1>(in, s, p, lane);
+        march_ray_f64<GRIDS, PW1>(in, s, p, lane, r);
     }
   }
+  if (cached)
+    cell_cache_flush_all(cell_cache_at<T>(slab), static_cast<T *>(in.adj.ggrid),
+                         in.geo.ny * in.geo.nz, in.geo.nz, lane);
+  wave_lds_sync(); // the slots are rewritten by the next item
 }
 
 // One launch covers the instances of one shell level that share the
@@ -722,11 +769,13 @@ template <int PREC, bool GRIDS, bool PW1, bool REV>
 __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
-  void *slab = nullptr; // reverse mode: one LDS slab per wavefront for scatter_runs
-  if constexpr (REV) {
+  __shared__ RayGeo ray_slots[4][kItemRaysMax];
+  RayGeo *rays = ray_slots[wave];
+  void *slab = nullptr; // reverse mode: one LDS cell cache per wavefront
+  if constexpr (REV && GRIDS) {
     using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
-    __shared__ T slabs[4 * kSlabValues];
-    slab = slabs + wave * kSlabValues;
+    __shared__ unsigned char caches[4][cell_cache_bytes<T>()];
+    slab = caches[wave];
   }
   if (!a.persistent) {
     uint64_t b = blockIdx.x;
@@ -736,7 +785,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
     }
     const uint64_t item = b * wpb + wave;
     if (item < a.n_items)
-      do_item<PREC, GRIDS, PW1, REV>(a, uniform_u64(item), lane, slab);
+      do_item<PREC, GRIDS, PW1, REV>(a, uniform_u64(item), lane, slab, rays);
     return;
   }
   const uint32_t q0 = a.xcd ? (blockIdx.x % a.nq) : 0;
@@ -750,7 +799,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
       const uint64_t item = a.qbeg[q] + t;
       if (item >= a.qend[q])
         break; // every wave leaves once all queues are drained
-      do_item<PREC, GRIDS, PW1, REV>(a, item, lane, slab);
+      do_item<PREC, GRIDS, PW1, REV>(a, item, lane, slab, rays);
     }
   }
 }
@@ -1007,14 +1056,14 @@ struct Variant {
 };
 
 const Variant kVariants[] = {
-    {0, 0, 1, 4, "one-shot, 1 ray/item, 4 waves/WG"},
-    {0, 1, 1, 4, "one-shot XCD-ordered, 1 ray/item, 4 waves/WG"},
+    {0, 0, 4, 4, "one-shot, 4 rays/item, 4 waves/WG"},
     {0, 1, 4, 4, "one-shot XCD-ordered, 4 rays/item, 4 waves/WG"},
-    {0, 1, 1, 1, "one-shot XCD-ordered, 1 ray/item, 1 wave/WG"},
-    {1, 0, 4, 4, "persistent global queue, 4 rays/item, 4 waves/WG"},
+    {0, 0, 16, 4, "one-shot, 16 rays/item, 4 waves/WG"},
+    {0, 1, 16, 4, "one-shot XCD-ordered, 16 rays/item, 4 waves/WG"},
+    {0, 1, 16, 1, "one-shot XCD-ordered, 16 rays/item, 1 wave/WG"},
+    {1, 0, 16, 4, "persistent global queue, 16 rays/item, 4 waves/WG"},
     {1, 1, 4, 4, "persistent XCD queues, 4 rays/item, 4 waves/WG"},
     {1, 1, 16, 4, "persistent XCD queues, 16 rays/item, 4 waves/WG"},
-    {1, 1, 1, 4, "persistent XCD queues, 1 ray/item, 4 waves/WG"},
 };
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -1467,11 +1516,21 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
       in.f.inv_z0k = static_cast<float>(cgs::kpc / x.los.cre[4]);
     }
     in.f.hexp = static_cast<float>(0.5 * (1. - pw_e));
+    in.r_min2 = x.los.gc_r_min * x.los.gc_r_min;
+    in.r_max2 = x.los.gc_r_max * x.los.gc_r_max;
+    in.te_r02 = x.los.tereg[1] * x.los.tereg[1];
+    in.cre_r02 = x.los.cre[1] * x.los.cre[1];
     // grids
     const bool any_grid = x.brnd_x || x.ternd;
     in.grids = any_grid ? 1 : 0;
     if (any_grid) {
       in.geo = grid_geo(x.box);
+      in.gx0 = x.box.x_min;
+      in.gy0 = x.box.y_min;
+      in.gz0 = x.box.z_min;
+      in.gsx = in.geo.fx / (x.box.x_max - x.box.x_min);
+      in.gsy = in.geo.fy / (x.box.y_max - x.box.y_min);
+      in.gsz = in.geo.fz / (x.box.z_max - x.box.z_min);
       void *packed = nullptr;
       for (const GridKey &g : grid_keys)
         if (g.x == x.brnd_x && g.y == x.brnd_y && g.z == x.brnd_z && g.t == x.ternd &&
