@@ -96,6 +96,62 @@ def cpu_baseline(par_text_fn, grids_host, threads):
                        f"{dt:.2f} s")
 
 
+def plain_passes(d0, freqs):
+    """The plain path's passes of one instance, in the reference pipeline's
+    order: the last-listed frequency first with DM and FD, then the others
+    sync only.  Yields (pass, frequency index, descriptor)."""
+    import ctypes as C
+    from hammurabix_amd import _capi as capi
+    for p, fi in enumerate(reversed(range(len(freqs)))):
+        d = capi.HamxLosDesc()
+        C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
+        d.sync_freq, d.do_sync = freqs[fi], 1
+        if p > 0:
+            d.do_dm = d.do_fd = 0
+        yield p, fi, d
+
+
+def pass_maps(d, dev):
+    import torch
+    z = lambda ns: torch.empty(12 * ns * ns, dtype=torch.float64, device=dev)  # noqa: E731
+    m = {k: z(d.nside_sync) for k in ("is", "qs", "us")}
+    m["fd"] = z(d.nside_fd if d.do_fd else d.nside_sync)
+    if d.do_dm:
+        m["dm"] = z(d.nside_dm)
+    return m
+
+
+def likelihood_seeds(ctx, par, sp, i, dev):
+    """Fill sp["adj"] with dL/dmap of L = -1/2 sum (m - obs)^2 / sigma^2 over
+    the Q/U maps of every frequency and the FD map, m the instance's FP64
+    plain maps, obs the maps of a held-out realisation (seed i + 10001) plus
+    Gaussian noise of sigma = 0.1 x rms(obs).  The I and DM seeds are zero."""
+    import torch
+    n = sp["brnd"][0].shape[0]
+    held = tuple(torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
+    ctx.brnd_es(par.es_desc(), 10001 + i, *held[:3])
+    ctx.ternd_dft(par.dft_desc(), 10001 + i, held[3])
+    gen = torch.Generator(device=dev).manual_seed(20001 + i)
+    box = sp["box"]
+    for p, fi, d in plain_passes(sp["desc"], sp["freqs"]):
+        model, obs = pass_maps(d, dev), pass_maps(d, dev)
+        ctx.los_forward(d, model, brnd=sp["brnd"], brnd_box=box, ternd=sp["ternd"],
+                        ternd_box=box)
+        ctx.los_forward(d, obs, brnd=held[:3], brnd_box=box, ternd=held[3], ternd_box=box)
+        kinds = [("qs", sp["adj"]["qs"][fi]), ("us", sp["adj"]["us"][fi])]
+        sp["adj"]["is"][fi].zero_()
+        if p == 0:
+            kinds.append(("fd", sp["adj"]["fd"]))
+            sp["adj"]["dm"].zero_()
+        for k, seed in kinds:
+            o = obs[k]
+            sigma = 0.1 * float(o.square().mean().sqrt())
+            o = o + sigma * torch.randn(o.shape[0], dtype=torch.float64, device=dev,
+                                        generator=gen)
+            seed.copy_((o - model[k]) / (sigma * sigma))
+    del held
+
+
 def run_batch(args):
     """The batch workload: instance i has random fields of seed i + 1 on the
     256x256x64 box, top nside (64, 128, 256)[i % 3] over 3 shells, the first
@@ -103,10 +159,16 @@ def run_batch(args):
     step is one replay of the captured batch graph; the plain path is every
     instance's passes run one by one through hamx_los_forward (FP64), the way
     the reference pipeline runs them, and also gives the maps the batch is
-    checked against."""
+    checked against.
+
+    Map+gradient evaluations differentiate a Gaussian log-likelihood of the
+    sync Q/U maps of every frequency plus the FD map against 'observed' maps:
+    the same instance on a held-out realisation (seed i + 10001) with 10%
+    Gaussian noise (sigma = 0.1 x the observed map's rms)."""
     import ctypes as C
     import torch
     from hammurabix_amd import Context, Param, _capi as capi, configs, los_samples
+    from hammurabix_amd.autograd import desc_params
     dev = torch.device("cuda:0")
     prec = capi.HAMX_PREC_F32 if args.precision == "f32" else capi.HAMX_PREC_F64
     n_inst = args.instances
@@ -143,6 +205,9 @@ def run_batch(args):
             specs.append(spec)
         torch.cuda.synchronize()
         gen_s = time.perf_counter() - t0
+        if with_adj:
+            for i, sp in enumerate(specs):
+                likelihood_seeds(ctx, pars[i], sp, i, dev)
         t0 = time.perf_counter()
         b = ctx.batch(specs, precision=prec, tune=True, adjoint=with_adj)
         plan_s = time.perf_counter() - t0
@@ -177,27 +242,19 @@ def run_batch(args):
             pdp = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
             pgb = tuple(torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3))
             pgt = torch.zeros(n, dtype=torch.float64, device=dev)
+        map_err = {}  # worst rel. L2 per map kind and frequency
+        worst_dln = 0.0
+        dp_err = dict.fromkeys(capi.PARAM_NAMES, 0.0)
         for sp in specs:
-            d0, freqs = sp["desc"], sp["freqs"]
+            freqs = sp["freqs"]
             kw = dict(brnd=sp["brnd"], brnd_box=sp["box"], ternd=sp["ternd"],
                       ternd_box=sp["box"])
             if with_adj:
                 acc_dp = torch.zeros_like(pdp)
                 acc_gb = [torch.zeros_like(pgt) for _ in range(3)]
                 acc_gt = torch.zeros_like(pgt)
-            for p, fi in enumerate(reversed(range(len(freqs)))):
-                d = capi.HamxLosDesc()
-                C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
-                d.sync_freq, d.do_sync = freqs[fi], 1
-                if p > 0:
-                    d.do_dm = d.do_fd = 0
-                m = {k: torch.empty(12 * d.nside_sync ** 2, dtype=torch.float64, device=dev)
-                     for k in ("is", "qs", "us")}
-                m["fd"] = torch.empty(12 * (d.nside_fd if d.do_fd else d.nside_sync) ** 2,
-                                      dtype=torch.float64, device=dev)
-                if d.do_dm:
-                    m["dm"] = torch.empty(12 * d.nside_dm ** 2, dtype=torch.float64,
-                                          device=dev)
+            for p, fi, d in plain_passes(sp["desc"], freqs):
+                m = pass_maps(d, dev)
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
                 ctx.los_forward(d, m, keep_tape=with_adj, **kw)
@@ -215,17 +272,27 @@ def run_batch(args):
                     for t, g in zip(acc_gb, pgb):
                         t += g
                     acc_gt += pgt
-                pairs = [(sp["maps"][k][fi], m[k]) for k in ("is", "qs", "us")]
+                pairs = [(f"{k}@{freqs[fi]:g}GHz", sp["maps"][k][fi], m[k])
+                         for k in ("is", "qs", "us")]
                 if p == 0:
-                    pairs += [(sp["maps"]["dm"], m["dm"]), (sp["maps"]["fd"], m["fd"])]
-                for got, ref in pairs:
+                    pairs += [("dm", sp["maps"]["dm"], m["dm"]),
+                              ("fd", sp["maps"]["fd"], m["fd"])]
+                for name, got, ref in pairs:
                     nr = float(torch.linalg.vector_norm(ref))
                     err = float(torch.linalg.vector_norm(got - ref)) / nr if nr else 0.0
                     worst = max(worst, err)
+                    map_err[name] = max(map_err.get(name, 0.0), err)
             if with_adj:
                 g = sp["grads"]
                 rel = (g["dparams"] - acc_dp).abs() / acc_dp.abs().clamp_min(1e-300)
                 worst_dp = max(worst_dp, float(rel.max()))
+                for name, e in zip(capi.PARAM_NAMES, rel.tolist()):
+                    dp_err[name] = max(dp_err[name], e)
+                # the whole parameter gradient in log-parameter units (p dL/dp)
+                pv = desc_params(sp["desc"]).to(dev)
+                ln_err = float(torch.linalg.vector_norm(pv * (g["dparams"] - acc_dp)) /
+                               torch.linalg.vector_norm(pv * acc_dp).clamp_min(1e-300))
+                worst_dln = max(worst_dln, ln_err)
                 for got, ref in zip(list(g["dbrnd"]) + [g["dternd"]], acc_gb + [acc_gt]):
                     nr = float(torch.linalg.vector_norm(ref))
                     err = float(torch.linalg.vector_norm(got - ref)) / nr if nr else 0.0
@@ -272,12 +339,15 @@ def run_batch(args):
             "plain_path_samples_per_s": plain_rate,
             "speedup_vs_plain_pass_by_pass": value / plain_rate,
             "max_rel_l2_vs_plain_fp64": worst,
+            "max_rel_l2_by_map": map_err,
             "autotune_ms": info["variant_ms"],
             **mg,
             **({"plain_map_gradient_evals_per_s": n_inst / (plain_s + plain_mg_s),
                 "map_gradient_speedup_vs_plain":
                     (mg["batch_map_gradient_evals_per_s"] * (plain_s + plain_mg_s) / n_inst),
                 "max_rel_err_dparams_vs_plain_fp64": worst_dp,
+                "max_rel_err_by_param": dp_err,
+                "max_rel_l2_dlnparams_vs_plain_fp64": worst_dln,
                 "max_rel_l2_grid_gradient_vs_plain_fp64": worst_grid} if with_adj else {}),
             "plan_s": plan_s,
             "realisation_generation_s": gen_s,

@@ -54,21 +54,25 @@ struct AdjShell {
 template <bool GRIDS, bool PW1>
 __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst ac,
                                                  Grids gr, ShellArgs sa, AdjShell as) {
-  // kRaysPerWave consecutive rays per wavefront (set up one per lane), and a
+  // kAdjRays consecutive rays per wavefront (set up one per lane), and a
   // per-wavefront cell cache that merges their grid-gradient contributions
-  __shared__ RayGeo slots[4][kRaysPerWave];
-  __shared__ unsigned char cache_mem[4][cell_cache_bytes<double>()];
+  __shared__ RayGeo slots[4][kAdjRays];
+  static_assert(kSlabValues * sizeof(double) <= cell_cache_bytes<double>(), "slab fits");
+  __shared__ __align__(16) unsigned char cache_mem[4][cell_cache_bytes<double>()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kRaysPerWave;
+  const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kAdjRays;
   if (p0 >= sa.hs.npix)
     return; // uniform per wavefront
-  const uint64_t nr = (sa.hs.npix - p0 < static_cast<uint64_t>(kRaysPerWave))
+  const uint64_t nr = (sa.hs.npix - p0 < static_cast<uint64_t>(kAdjRays))
                           ? sa.hs.npix - p0
-                          : static_cast<uint64_t>(kRaysPerWave);
+                          : static_cast<uint64_t>(kAdjRays);
   if (static_cast<uint64_t>(lane) < nr)
     ray_setup(ph, sa, p0 + lane, slots[wave][lane]);
   const CellCache<double> cache = cell_cache_at<double>(cache_mem[wave]);
-  cell_cache_init(cache, lane);
+  if (kAdjCellCache)
+    cell_cache_init(cache, lane);
+  else
+    wave_lds_sync();
   uint32_t chunk_id = 0;
   for (uint64_t jr = 0; jr < nr; ++jr) {
   const uint64_t ipix = p0 + jr;
@@ -147,8 +151,14 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     }
     if (GRIDS && as.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      scatter_cached<double>(cache, as.ggrid, has, has ? o.cell.idx : 0, o.cell.xd, o.cell.yd,
-                             o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane, ++chunk_id);
+      if (kAdjCellCache)
+        scatter_cached<double>(cache, as.ggrid, has, has ? o.cell.idx : 0, o.cell.xd,
+                               o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane,
+                               ++chunk_id);
+      else
+        scatter_runs<double>(as.ggrid, reinterpret_cast<double *>(cache_mem[wave]), has,
+                             has ? o.cell.idx : 0, o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y,
+                             dB.z, dte, as.sx, as.sy, lane);
     }
   }
   p0 = wave_sum(p0);
@@ -169,7 +179,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
       as.gfdin[ipix] = ac.l2 * dq_after;
   }
   } // rays of the wavefront
-  if (GRIDS && as.ggrid)
+  if (kAdjCellCache && GRIDS && as.ggrid)
     cell_cache_flush_all(cache, as.ggrid, as.sx, as.sy, lane);
 }
 
@@ -516,7 +526,7 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
     const FastConst fc = make_fast(d, ph, sa.delta, pw1);
     ac.delta = sa.delta;
     const unsigned blocks =
-        static_cast<unsigned>((npix + 4 * kRaysPerWave - 1) / (4 * kRaysPerWave));
+        static_cast<unsigned>((npix + 4 * kAdjRays - 1) / (4 * kAdjRays));
     ProfMark pm = prof_begin(ctx);
     if (packed && pw1)
       hipLaunchKernelGGL((k_los_adj<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,

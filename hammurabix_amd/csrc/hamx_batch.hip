@@ -221,7 +221,8 @@ __device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane,
 // One FP32 sample with the intermediates the reverse mode needs; the
 // forward march reads te, fdk, w, c2i and s2i only.
 struct S32 {
-  float te, fdk, w, c2i, s2i;
+  double fdk, b_pard; // FD contribution and B.l: FP64 products of the FP32 field
+  float te, w, c2i, s2i;
   float bx, by, bz, b_par, bper2, x, y, n2;
   float te_raw;
   bool lsa_in, te_in, cell_ok;
@@ -239,6 +240,25 @@ __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, 
   w.y = (a0.y * z1 + a1.y * zd) * y1 + (b0.y * z1 + b1.y * zd) * yd;
   w.z = (a0.z * z1 + a1.z * zd) * y1 + (b0.z * z1 + b1.z * zd) * yd;
   w.w = (a0.w * z1 + a1.w * zd) * y1 + (b0.w * z1 + b1.w * zd) * yd;
+}
+
+// sin and cos of an FP32 angle: quadrant reduction k = rint(2x/pi) with
+// the remainder x - k pi/2 taken in FP64 (exact for any angle a field model
+// produces), then the Taylor series of sin and cos on [-pi/4, pi/4] to the
+// terms r^9 and r^10 (truncation below 2e-9 there).
+__device__ inline void sincos_f32(float x, float &sn, float &cs) {
+  const double xd = static_cast<double>(x);
+  const double k = rint(xd * 0.63661977236758134308); // 2/pi
+  const float r = static_cast<float>(xd - k * 1.57079632679489661923);
+  const float r2 = r * r;
+  const float s = r * (1.f + r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f +
+                                                                     r2 * (1.f / 362880.f)))));
+  const float c =
+      1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f +
+                                                                   r2 * (-1.f / 3628800.f)))));
+  const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
+  sn = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
+  cs = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
 }
 
 // grid_cell with the division by the extent replaced by a product with
@@ -288,15 +308,15 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     const float inv_r = 1.f / rk;
     o.cos_p = xk * inv_r;
     o.sin_p = yk * inv_r;
-    o.lnr = __logf(rk * 0.125f);
-    const float e2 = __expf(-2.f * fabsf(zk));
+    // accurate (not the hardware-approximate) functions: the field's
+    // systematic error feeds the FD, and the Q/U phase at low frequency
+    o.lnr = logf(rk * 0.125f);
+    const float e2 = expf(-2.f * fabsf(zk));
     o.tnh = copysignf((1.f - e2) / (1.f + e2), zk);
     const float psi = c.psi0 + c.psi1 * o.lnr;
     const float chi = c.chi0 * o.tnh;
-    o.sps = __sinf(psi);
-    o.cps = __cosf(psi);
-    o.sch = __sinf(chi);
-    o.cch = __cosf(chi);
+    sincos_f32(psi, o.sps, o.cps);
+    sincos_f32(chi, o.sch, o.cch);
     const float bc = c.b0u * o.cch;
     bx = bc * (o.sps * o.cos_p - o.cps * o.sin_p);
     by = bc * (o.sps * o.sin_p + o.cps * o.cos_p);
@@ -336,8 +356,12 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   o.bx = bx;
   o.by = by;
   o.bz = bz;
-  o.b_par = bx * lf.x + by * lf.y + bz * lf.z;
-  o.fdk = te * o.b_par;
+  // B.l with the FP64 direction: an FP32 direction is a systematic error
+  // along the whole ray
+  o.b_pard = static_cast<double>(bx) * r.l.x + static_cast<double>(by) * r.l.y +
+             static_cast<double>(bz) * r.l.z;
+  o.b_par = static_cast<float>(o.b_pard);
+  o.fdk = static_cast<double>(te) * o.b_pard;
   const float cx = by * lf.z - bz * lf.y, cy = bz * lf.x - bx * lf.z,
               cz = bx * lf.y - by * lf.x;
   o.bper2 = cx * cx + cy * cy + cz * cz;
@@ -387,7 +411,7 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane,
     bool valid = false;
     if (k < r.k_hi)
       valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
-    const double fk = valid ? static_cast<double>(o.fdk) * sh.fdu : 0.;
+    const double fk = valid ? o.fdk * sh.fdu : 0.;
     const double incl = wave_incl_scan(fk, lane);
     const double fd_here = carry + incl;
     carry += __shfl(incl, 63, 64);
@@ -492,7 +516,7 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, voi
     bool valid = false;
     if (k < r.k_hi)
       valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
-    const double incl = wave_incl_scan(valid ? static_cast<double>(o.fdk) * sh.fdu : 0., lane);
+    const double incl = wave_incl_scan(valid ? o.fdk * sh.fdu : 0., lane);
     const double chunk = __shfl(incl, 63, 64);
     const double fd_here = (fd_total - after - chunk) + incl;
     after += chunk;
@@ -525,7 +549,7 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, voi
     double dte = 0.;
     float dbx = 0.f, dby = 0.f, dbz = 0.f;
     if (valid) {
-      dte = o.te_raw > 0.f ? R * static_cast<double>(o.b_par) * sh.fdu + gdm * delta : 0.;
+      dte = o.te_raw > 0.f ? R * o.b_pard * sh.fdu + gdm * delta : 0.;
       const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
       dbx = ft * lf.x;
       dby = ft * lf.y;
@@ -766,7 +790,9 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
 // template case (random grids present, emissivity exponent -1); REV
 // selects the reverse-mode sweep.
 template <int PREC, bool GRIDS, bool PW1, bool REV>
-__global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(PREC == HAMX_PREC_F32 && !REV ? 3 : 1))) void k_batch_march(
+    LevelArgs a) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   __shared__ RayGeo ray_slots[4][kItemRaysMax];
@@ -774,7 +800,7 @@ __global__ __launch_bounds__(256) void k_batch_march(LevelArgs a) {
   void *slab = nullptr; // reverse mode: one LDS cell cache per wavefront
   if constexpr (REV && GRIDS) {
     using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
-    __shared__ unsigned char caches[4][cell_cache_bytes<T>()];
+    __shared__ __align__(16) unsigned char caches[4][cell_cache_bytes<T>()];
     slab = caches[wave];
   }
   if (!a.persistent) {

@@ -279,17 +279,17 @@ template <bool GRIDS, bool PW1>
 // workload; four (128 VGPRs) spills and is slower.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_los_fast(
     Phys ph, FastConst fc, Grids gr, ShellArgs sa) {
-  // kRaysPerWave consecutive rays per wavefront: their set-up (pointing,
+  // kFastRays consecutive rays per wavefront: their set-up (pointing,
   // versors, FD cache, culled step range) runs one ray per lane into LDS,
   // then the rays are marched one after another, 64 samples per step
-  __shared__ RayGeo slots[4][kRaysPerWave];
+  __shared__ RayGeo slots[4][kFastRays];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kRaysPerWave;
+  const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kFastRays;
   if (p0 >= sa.hs.npix)
     return; // uniform per wavefront
-  const uint64_t nr = (sa.hs.npix - p0 < static_cast<uint64_t>(kRaysPerWave))
+  const uint64_t nr = (sa.hs.npix - p0 < static_cast<uint64_t>(kFastRays))
                           ? sa.hs.npix - p0
-                          : static_cast<uint64_t>(kRaysPerWave);
+                          : static_cast<uint64_t>(kFastRays);
   if (static_cast<uint64_t>(lane) < nr)
     ray_setup(ph, sa, p0 + lane, slots[wave][lane]);
   wave_lds_sync();
@@ -804,7 +804,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     if (fast) {
       const FastConst fc = make_fast(d, ph, sa.delta, pw1);
       const unsigned blocks =
-          static_cast<unsigned>((npix + 4 * kRaysPerWave - 1) / (4 * kRaysPerWave));
+          static_cast<unsigned>((npix + 4 * kFastRays - 1) / (4 * kFastRays));
       if (packed && pw1)
         hipLaunchKernelGGL((k_los_fast<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,
                            ph, fc, gr, sa);

@@ -189,10 +189,18 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Rays per wavefront in the specialised march and its adjoint: their
-// set-up runs one ray per lane (8 keeps every shell of nside >= 64 at
-// several wavefronts per SIMD).
-constexpr int kRaysPerWave = 8;
+// Rays per wavefront in the specialised march (kFastRays) and its adjoint
+// (kAdjRays): their set-up runs one ray per lane into LDS, then they are
+// marched one after another.  kAdjCellCache picks the adjoint's grid
+// scatter: the per-wavefront cell cache (merges neighbouring rays) or
+// scatter_runs (merges the runs of one 64-sample chunk).  Measured on the
+// 256x256x64 three-shell workload (march / adjoint ms): 1 ray 14.0 / 44.8
+// (runs), 2 rays 13.4 / 42.1 (cache), 4 rays 14.5, 8 rays 21.8 / 66.1: more
+// rays per wavefront leave too few wavefronts to balance the uneven ray
+// lengths (rays leaving the z range early) across the SIMDs.
+constexpr int kFastRays = 2;
+constexpr int kAdjRays = 2;
+constexpr bool kAdjCellCache = true;
 
 // LDS slab of one wavefront for scatter_runs: one slot of 32 values (8
 // corners x {x, y, z, te}) per possible run.
