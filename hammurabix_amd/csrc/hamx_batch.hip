@@ -733,6 +733,190 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, voi
   }
 }
 
+// ------------------------------------------- reverse mode, lane per ray --
+//
+// The FP32 reverse sweep with one ray per lane: the running FD (total from
+// the tape minus the samples beyond) and R_k are lane-private sums, and the
+// lane keeps the 32 trilinear corner sums of the grid cell its ray is in,
+// adding them to the gradient grid when the ray leaves the cell (a straight
+// ray meets each cell in one run of samples).  No cross-lane reduction,
+// LDS atomic or wavefront barrier is left in the sample loop.
+
+__device__ inline uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+// acc (8 corners x {x, y, z, te}) -> the packed gradient grid at cell
+// `cell`, then acc = 0.
+__device__ inline void flush_cell_f32(float (&acc)[32], uint64_t cell, float *__restrict__ g,
+                                      uint64_t sx, uint64_t sy) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float *p = g + 4 * (cell + (c >> 2) * sx + ((c >> 1) & 1) * sy + (c & 1));
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      unsafeAtomicAdd(p + m, acc[4 * c + m]);
+      acc[4 * c + m] = 0.f;
+    }
+  }
+}
+
+template <bool GRIDS, bool PW1>
+__device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
+                              const RayGeo &r) {
+  const bool has_cache = in.do_fd && s > 0;
+  const BShell &sh = in.sh[s];
+  const BAdj &A = in.adj;
+  const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
+                                static_cast<float>(r.l.z));
+  const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
+                                  static_cast<float>(r.eth.z));
+  const float3 ephf = make_float3(static_cast<float>(r.eph.x), static_cast<float>(r.eph.y),
+                                  static_cast<float>(r.eph.z));
+  const int nf = in.n_freq;
+  const double delta = sh.delta;
+  double aI = 0., AIL = 0.;
+  float aQ[kMaxF] = {0.f, 0.f, 0.f, 0.f}, aU[kMaxF] = {0.f, 0.f, 0.f, 0.f};
+  float lpf[kMaxF] = {0.f, 0.f, 0.f, 0.f};
+  double gdm = 0., gfd = 0., fd_total = 0.;
+  if (live) {
+#pragma unroll
+    for (int f = 0; f < kMaxF; ++f) {
+      if (f < nf) {
+        const double gi = A.g_i[f][ipix] * in.st[f] * delta;
+        aI += gi;
+        AIL += gi * A.lt[f];
+        aQ[f] = static_cast<float>(A.g_q[f][ipix] * in.sp[f] * delta);
+        aU[f] = static_cast<float>(A.g_u[f][ipix] * in.sp[f] * delta);
+        lpf[f] = static_cast<float>(A.lp[f]);
+      }
+    }
+    gdm = in.do_dm ? A.g_dm[ipix] : 0.;
+    gfd = in.do_fd ? A.g_fd[ipix] : 0.;
+    fd_total = in.t_fd[in.fd_off[s] + ipix];
+  }
+  const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
+  float *ggrid = static_cast<float *>(A.ggrid);
+  double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0., q0 = 0.;
+  double after = 0., dacc = 0.; // FD and R contributions of the samples beyond
+  float acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    acc[i] = 0.f;
+  uint64_t cur = ~0ull; // cell whose corner sums acc holds
+  // the union of the lanes' step ranges, far end first
+  const uint64_t k_end = wave_max_u64(live ? r.k_hi : 0ull);
+  const uint64_t k_beg = wave_min_u64(live ? r.k_lo : ~0ull);
+  for (uint64_t k = k_end; k-- > k_beg;) {
+    if (!live || k < r.k_lo || k >= r.k_hi)
+      continue;
+    S32 o;
+    if (!eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o))
+      continue;
+    const double fd_here = fd_total - after; // FD through this sample
+    after += o.fdk * sh.fdu;
+    double D = 0.;
+    float dLdw = static_cast<float>(aI), alpha_p = 0.f, dipa = 0.f;
+#pragma unroll
+    for (int f = 0; f < kMaxF; ++f) {
+      if (f < nf) {
+        const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+        double rv = X * in.rev[f];
+        rv -= rint(rv);
+        const float a = static_cast<float>(rv);
+        const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
+        const float c2 = cs * o.c2i - sn * o.s2i, s2 = sn * o.c2i + cs * o.s2i;
+        const float P = aQ[f] * c2 + aU[f] * s2;
+        dLdw += P;
+        alpha_p += P * lpf[f];
+        const float dq = 2.f * o.w * (aU[f] * c2 - aQ[f] * s2);
+        dipa += dq;
+        D += A.l2[f] * static_cast<double>(dq);
+        if (f == 0)
+          q0 += dq;
+      }
+    }
+    dacc += D;
+    const double R = dacc + gfd; // dL/d(FD contribution of this sample)
+    const double dte = o.te_raw > 0.f ? R * o.b_pard * sh.fdu + gdm * delta : 0.;
+    const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
+    float dbx = ft * lf.x, dby = ft * lf.y, dbz = ft * lf.z;
+    if (o.bper2 > 0.f) {
+      const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
+      dbx += coef * (o.bx - o.b_par * lf.x);
+      dby += coef * (o.by - o.b_par * lf.y);
+      dbz += coef * (o.bz - o.b_par * lf.z);
+      // ln |B_perp| in Gauss: the B grid and b0 are in muG here
+      const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
+      p4 += static_cast<double>(o.w) *
+            (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
+    }
+    if (o.n2 > 0.f) {
+      const float cc = dipa / o.n2;
+      dbx += cc * (o.y * ephf.x - o.x * ethf.x);
+      dby += cc * (o.y * ephf.y - o.x * ethf.y);
+      dbz += cc * (o.y * ephf.z - o.x * ethf.z);
+    }
+    if (o.te_in)
+      p5 += dte;
+    if (o.lsa_in) {
+      const float u = o.sps * o.cos_p - o.cps * o.sin_p;
+      const float v = o.sps * o.sin_p + o.cps * o.cos_p;
+      p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
+      const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
+      p1 += gpsi;
+      p2 += gpsi * o.lnr;
+      const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
+      p3 += gchi * o.tnh;
+    }
+    if (GRIDS && ggrid && o.cell_ok) {
+      if (o.idx != cur) {
+        if (cur != ~0ull)
+          flush_cell_f32(acc, cur, ggrid, sx, sy);
+        cur = o.idx;
+      }
+      const float gte = static_cast<float>(dte);
+      const float wx[2] = {1.f - o.xd, o.xd}, wy[2] = {1.f - o.yd, o.yd},
+                  wz[2] = {1.f - o.zd, o.zd};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float w = wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
+        acc[4 * c + 0] += w * dbx;
+        acc[4 * c + 1] += w * dby;
+        acc[4 * c + 2] += w * dbz;
+        acc[4 * c + 3] += w * gte;
+      }
+    }
+  }
+  if (GRIDS && ggrid && cur != ~0ull)
+    flush_cell_f32(acc, cur, ggrid, sx, sy);
+  if (live) {
+    double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
+    rp[0] = p0;
+    rp[1] = p1;
+    rp[2] = p2;
+    rp[3] = p3;
+    rp[4] = p4;
+    rp[5] = p5;
+    if (has_cache && in.fdin[0])
+      A.gfdin[ipix] = A.l2[0] * q0;
+  }
+}
+
 // ---------------------------------------------------------- work items --
 
 template <int PREC, bool GRIDS, bool PW1, bool REV>
@@ -750,6 +934,21 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
   const int s = static_cast<int>(a.level);
   const uint64_t npix = in.sh[s].hs.npix;
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
+  if constexpr (REV && PREC == HAMX_PREC_F32) {
+    // one ray per lane (a.rays == 64): a NESTED 8x8 patch of the shell when
+    // its nside is a power of two (the lanes' samples at one step then lie
+    // within a few grid cells), else 64 consecutive RING pixels
+    const Healpix &hs = in.sh[s].hs;
+    const uint64_t pl = p0 + static_cast<uint64_t>(lane);
+    const bool live = pl < npix;
+    const bool nested = (hs.nside & (hs.nside - 1)) == 0;
+    const uint64_t ipix = live ? (nested ? hp_nest2ring(hs, pl) : pl) : 0;
+    RayGeo r{};
+    if (live)
+      ray_setup_call(in.ph, shell_args_rev(in, s), ipix, r);
+    adj_lanes_f32<GRIDS, PW1>(in, s, ipix, live, r);
+    return;
+  }
   const uint64_t p1 = (p0 + a.rays < npix) ? p0 + a.rays : npix;
   // the item's rays are set up one per lane (pointing, versors, FD cache,
   // culled step range: scalar FP64 work that every lane would otherwise
@@ -798,9 +997,8 @@ __attribute__((amdgpu_waves_per_eu(PREC == HAMX_PREC_F32 && !REV ? 3 : 1))) void
   __shared__ RayGeo ray_slots[4][kItemRaysMax];
   RayGeo *rays = ray_slots[wave];
   void *slab = nullptr; // reverse mode: one LDS cell cache per wavefront
-  if constexpr (REV && GRIDS) {
-    using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
-    __shared__ __align__(16) unsigned char caches[4][cell_cache_bytes<T>()];
+  if constexpr (REV && GRIDS && PREC == HAMX_PREC_F64) {
+    __shared__ __align__(16) unsigned char caches[4][cell_cache_bytes<double>()];
     slab = caches[wave];
   }
   if (!a.persistent) {
@@ -1093,6 +1291,10 @@ const Variant kVariants[] = {
 };
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
+// The FP32 reverse sweep runs one ray per lane: 64-ray items (NESTED 8x8
+// patches), persistent XCD queues.
+const Variant kLaneRev = {1, 1, 64, 4, "persistent XCD queues, 64 rays/item (lane per ray)"};
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -1127,6 +1329,7 @@ struct hamx_batch {
   unsigned long long *d_ctr = nullptr;
   // per variant and level: prefix arrays depend on rays per item
   std::vector<std::vector<uint64_t *>> d_prefix; // [variant][group]
+  std::vector<LevelArgs> rev_largs;               // [group]: FP32 reverse plan (kLaneRev)
   std::vector<std::vector<LevelArgs>> largs;     // [variant][group]
   int n_cu = 0;
   int per_cu[4][5] = {}; // resident march workgroups per CU, by case and waves per WG
@@ -1284,7 +1487,9 @@ int launch_adjoint(hamx_batch *b, uint32_t v) {
   }
   HAMX_TRY((launch_jobs<decltype(&k_batch_copy), CopyJob>(ctx, k_batch_copy, b->a_init_copy,
                                                            1024)));
-  const unsigned threads = 64u * V.waves_per_block;
+  const bool lanes = b->prec == HAMX_PREC_F32;
+  const Variant &VR = lanes ? kLaneRev : V;
+  const unsigned threads = 64u * VR.waves_per_block;
   for (uint32_t s = b->levels; s-- > 0;) {
     if (b->a_zero[s].n) {
       uint64_t blocks = std::min<uint64_t>((b->a_zero[s].max + 255) / 256, 1024);
@@ -1300,13 +1505,13 @@ int launch_adjoint(hamx_batch *b, uint32_t v) {
       HAMX_HIP(ctx, hipGetLastError());
     }
     for (int c = 0; c < 4; ++c) {
-      LevelArgs arg = b->largs[v][s * 4 + c];
+      LevelArgs arg = lanes ? b->rev_largs[s * 4 + c] : b->largs[v][s * 4 + c];
       if (arg.n_items == 0)
         continue;
       arg.ctr = b->d_actr + static_cast<size_t>(s * 4 + c) * kQueues;
       void *args[] = {&arg};
       HAMX_HIP(ctx, hipLaunchKernel(reinterpret_cast<const void *>(march_kernel(b->prec, c, true)),
-                                    dim3(march_blocks(b, V, arg, c, true)), dim3(threads), args,
+                                    dim3(march_blocks(b, VR, arg, c, true)), dim3(threads), args,
                                     0, st));
     }
     if (b->a_fdt[s].n) {
@@ -1922,6 +2127,25 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
         la.qbeg[0] = 0;
         la.qend[0] = la.n_items;
       }
+    }
+  }
+  // the FP32 reverse plan
+  if (b->adjoint && b->prec == HAMX_PREC_F32) {
+    b->rev_largs.assign(b->levels * 4, LevelArgs{});
+    for (uint32_t s = 0; s < b->levels * 4; ++s) {
+      LevelArgs &la = b->rev_largs[s];
+      std::memset(&la, 0, sizeof(la));
+      std::vector<uint64_t> prefix;
+      plan_level(b, s, kLaneRev.rays, prefix, la);
+      uint64_t *d = nullptr;
+      if ((rc = upload(b, prefix, &d)) != HAMX_OK)
+        return fail(rc);
+      la.inst = b->d_inst;
+      la.ids = b->d_ids[s];
+      la.prefix = d;
+      la.persistent = kLaneRev.persistent;
+      la.xcd = kLaneRev.xcd;
+      la.nq = kQueues;
     }
   }
   // autotune: one warm-up and one timed eager run per candidate

@@ -303,6 +303,26 @@ __host__ __device__ inline uint64_t hp_xyf2ring(const Healpix &h, int64_t ix, in
   return static_cast<uint64_t>(n_before + jp - 1);
 }
 
+// Even bits of v packed into the low half (the inverse of bit interleaving).
+__host__ __device__ inline uint64_t hp_compress_bits(uint64_t v) {
+  v &= 0x5555555555555555ull;
+  v = (v | (v >> 1)) & 0x3333333333333333ull;
+  v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
+  v = (v | (v >> 16)) & 0x00000000ffffffffull;
+  return v;
+}
+
+// RING index of NESTED pixel p (nside a power of two): p = face nside^2 +
+// the interleaved bits of (ix, iy) on the face, ix on the even bits.
+__host__ __device__ inline uint64_t hp_nest2ring(const Healpix &h, uint64_t p) {
+  const uint64_t npface = h.nside * h.nside;
+  const uint64_t face = p / npface, q = p - face * npface;
+  return hp_xyf2ring(h, static_cast<int64_t>(hp_compress_bits(q)),
+                     static_cast<int64_t>(hp_compress_bits(q >> 1)), static_cast<int>(face));
+}
+
 // Value of the mask at nside ht.nside, pixel m, from the pivot mask at
 // hp.nside (Hampisk::duplicate, hamsk.h:55-103): a copy at the same nside;
 // 1 if any of the pixel's sub-pixels is > 0 when downgrading (else 0); the
