@@ -229,6 +229,7 @@ struct S32 {
   float cos_p, sin_p, sps, cps, sch, cch, lnr, tnh;
   uint64_t idx;
   float xd, yd, zd;
+  int cx, cy, cz; // the cell's lower corner
 };
 
 __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, uint64_t sy,
@@ -263,7 +264,8 @@ __device__ inline void sincos_f32(float x, float &sn, float &cs) {
 
 // grid_cell with the division by the extent replaced by a product with
 // (n - 1) / extent (FP32 mode).
-__device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c) {
+__device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c, int &cx, int &cy,
+                                        int &cz) {
   const GridGeo &g = in.geo;
   double t = (p.x - in.gx0) * in.gsx;
   if (!(t > 0 && t < g.fx))
@@ -281,6 +283,9 @@ __device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c)
   const uint64_t zl = static_cast<uint64_t>(floor(t));
   c.zd = t - static_cast<double>(zl);
   c.idx = (xl * g.ny + yl) * g.nz + zl;
+  cx = static_cast<int>(xl);
+  cy = static_cast<int>(yl);
+  cz = static_cast<int>(zl);
   return true;
 }
 
@@ -332,7 +337,7 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   o.cell_ok = false;
   if (GRIDS) {
     Cell cell;
-    if (grid_cell_scaled(in, pos, cell)) {
+    if (grid_cell_scaled(in, pos, cell, o.cx, o.cy, o.cz)) {
       o.cell_ok = true;
       const float4 *g = static_cast<const float4 *>(in.packed);
       const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
@@ -775,6 +780,32 @@ __device__ inline void flush_cell_f32(float (&acc)[32], uint64_t cell, float *__
   }
 }
 
+// The lane's ray moves from `cell` to its neighbour along one axis (BIT: 4
+// x, 2 y, 1 z in the corner numbering 4a + 2b + c), upwards when `up`: the
+// 4 corners it leaves behind go to the grid, the 4 shared with the new cell
+// move to their place in it, and the new cell's far corners start at 0.
+template <int BIT>
+__device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
+                                      float *__restrict__ g, uint64_t sx, uint64_t sy) {
+#pragma unroll
+  for (int lo = 0; lo < 8; ++lo) {
+    if (lo & BIT)
+      continue;
+    const int hi = lo | BIT;
+    const uint64_t vlo = cell + (lo >> 2) * sx + ((lo >> 1) & 1) * sy + (lo & 1);
+    const uint64_t vhi = cell + (hi >> 2) * sx + ((hi >> 1) & 1) * sy + (hi & 1);
+    float *p = g + 4 * (up ? vlo : vhi);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float left = up ? acc[4 * lo + m] : acc[4 * hi + m];
+      const float kept = up ? acc[4 * hi + m] : acc[4 * lo + m];
+      unsafeAtomicAdd(p + m, left);
+      acc[4 * lo + m] = up ? kept : 0.f;
+      acc[4 * hi + m] = up ? 0.f : kept;
+    }
+  }
+}
+
 template <bool GRIDS, bool PW1>
 __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
                               const RayGeo &r) {
@@ -818,6 +849,7 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
   for (int i = 0; i < 32; ++i)
     acc[i] = 0.f;
   uint64_t cur = ~0ull; // cell whose corner sums acc holds
+  int ccx = 0, ccy = 0, ccz = 0;
   // the union of the lanes' step ranges, far end first
   const uint64_t k_end = wave_max_u64(live ? r.k_hi : 0ull);
   const uint64_t k_beg = wave_min_u64(live ? r.k_lo : ~0ull);
@@ -885,9 +917,30 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
     }
     if (GRIDS && ggrid && o.cell_ok) {
       if (o.idx != cur) {
-        if (cur != ~0ull)
-          flush_cell_f32(acc, cur, ggrid, sx, sy);
+        if (cur != ~0ull) {
+          const int dx = o.cx - ccx, dy = o.cy - ccy, dz = o.cz - ccz;
+          if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+            // face steps, one axis at a time (every intermediate cell lies
+            // between two valid cells, so is valid)
+            uint64_t c = cur;
+            if (dx) {
+              shift_cell_f32<4>(acc, dx > 0, c, ggrid, sx, sy);
+              c = dx > 0 ? c + sx : c - sx;
+            }
+            if (dy) {
+              shift_cell_f32<2>(acc, dy > 0, c, ggrid, sx, sy);
+              c = dy > 0 ? c + sy : c - sy;
+            }
+            if (dz)
+              shift_cell_f32<1>(acc, dz > 0, c, ggrid, sx, sy);
+          } else {
+            flush_cell_f32(acc, cur, ggrid, sx, sy);
+          }
+        }
         cur = o.idx;
+        ccx = o.cx;
+        ccy = o.cy;
+        ccz = o.cz;
       }
       const float gte = static_cast<float>(dte);
       const float wx[2] = {1.f - o.xd, o.xd}, wy[2] = {1.f - o.yd, o.yd},
