@@ -66,6 +66,7 @@ struct BShell {
 
 // FP32 field constants in kpc / muG units.
 struct F32Const {
+  double b0d, n0d; // b0 [muG] and n0 for the FD product, kept in FP64
   float psi0, psi1, chi0, b0u;
   float n0;
   float R0k, inv_r0k, inv_z0k;
@@ -305,6 +306,11 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   const float zk = static_cast<float>(pos.z * inv_kpc);
   const F32Const &c = in.f;
   float bx = 0.f, by = 0.f, bz = 0.f;
+  // FD = te B.l accumulates along the ray, so a systematic FP32 rounding of
+  // b0, n0 or the direction would be a systematic FD error: B.l is summed
+  // in FP64 with the FP64 direction, the LSA part as b0 (FP64) times its
+  // unit vector, and te as n0 (FP64) plus the FP32 grid value
+  double bpar = 0., te_d = 0.;
   // WMAP-LSA window (breg_lsa.cc:9-26)
   o.lsa_in = !(rr2 > (400. * cgs::kpc * cgs::kpc) || rr2 < (9. * cgs::kpc * cgs::kpc));
   if (o.lsa_in) {
@@ -322,17 +328,19 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     const float chi = c.chi0 * o.tnh;
     sincos_f32(psi, o.sps, o.cps);
     sincos_f32(chi, o.sch, o.cch);
-    const float bc = c.b0u * o.cch;
-    bx = bc * (o.sps * o.cos_p - o.cps * o.sin_p);
-    by = bc * (o.sps * o.sin_p + o.cps * o.cos_p);
+    const float ux = o.cch * (o.sps * o.cos_p - o.cps * o.sin_p);
+    const float uy = o.cch * (o.sps * o.sin_p + o.cps * o.cos_p);
+    bx = c.b0u * ux;
+    by = c.b0u * uy;
     bz = c.b0u * o.sch;
+    bpar = c.b0d * (static_cast<double>(ux) * r.l.x + static_cast<double>(uy) * r.l.y +
+                    static_cast<double>(o.sch) * r.l.z);
   }
-  float te = 0.f;
   {
     const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
     o.te_in = !(dx * dx + dy * dy + dz * dz > in.te_r02);
     if (o.te_in)
-      te = c.n0;
+      te_d = c.n0d;
   }
   o.cell_ok = false;
   if (GRIDS) {
@@ -349,24 +357,27 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
       lerp_plane_f4(g, cell.idx, sy, o.zd, o.yd, w1);
       lerp_plane_f4(g, cell.idx + sx, sy, o.zd, o.yd, w2);
       const float x1 = 1.f - o.xd;
-      bx += w1.x * x1 + w2.x * o.xd;
-      by += w1.y * x1 + w2.y * o.xd;
-      bz += w1.z * x1 + w2.z * o.xd;
-      te += w1.w * x1 + w2.w * o.xd;
+      const float gx = w1.x * x1 + w2.x * o.xd;
+      const float gy = w1.y * x1 + w2.y * o.xd;
+      const float gz = w1.z * x1 + w2.z * o.xd;
+      bx += gx;
+      by += gy;
+      bz += gz;
+      bpar += static_cast<double>(gx) * r.l.x + static_cast<double>(gy) * r.l.y +
+              static_cast<double>(gz) * r.l.z;
+      te_d += static_cast<double>(w1.w * x1 + w2.w * o.xd);
     }
   }
-  o.te_raw = te;
-  te = te > 0.f ? te : 0.f;
+  o.te_raw = static_cast<float>(te_d); // the clamp decision keeps te_d's sign
+  te_d = te_d > 0. ? te_d : 0.;
+  const float te = static_cast<float>(te_d);
   o.te = te;
   o.bx = bx;
   o.by = by;
   o.bz = bz;
-  // B.l with the FP64 direction: an FP32 direction is a systematic error
-  // along the whole ray
-  o.b_pard = static_cast<double>(bx) * r.l.x + static_cast<double>(by) * r.l.y +
-             static_cast<double>(bz) * r.l.z;
-  o.b_par = static_cast<float>(o.b_pard);
-  o.fdk = static_cast<double>(te) * o.b_pard;
+  o.b_pard = bpar;
+  o.b_par = static_cast<float>(bpar);
+  o.fdk = te_d * bpar;
   const float cx = by * lf.z - bz * lf.y, cy = bz * lf.x - bx * lf.z,
               cz = bx * lf.y - by * lf.x;
   o.bper2 = cx * cx + cy * cy + cz * cz;
@@ -1794,6 +1805,8 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     in.f.chi0 = static_cast<float>(x.los.breg[3]);
     in.f.b0u = static_cast<float>(x.los.breg[0] * 1e6);
     in.f.n0 = static_cast<float>(x.los.tereg[0]);
+    in.f.n0d = x.los.tereg[0];
+    in.f.b0d = x.los.breg[0] * 1e6;
     if (x.los.cre_model == HAMX_CRE_ANA) {
       in.f.R0k = static_cast<float>(in.ph.R0 / cgs::kpc);
       in.f.inv_r0k = static_cast<float>(cgs::kpc / x.los.cre[3]);
