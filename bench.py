@@ -39,11 +39,17 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", choices=("random_forward", "batch"), default="random_forward",
+    ap.add_argument("--workload", choices=("random_forward", "batch", "large"),
+                    default="random_forward",
                     help="random_forward: the plain FP64 path on one realisation (default); "
                          "batch: 64 realisations x parameter sets x 1-4 frequencies in one "
-                         "graph replay, against the plain path run pass by pass")
-    ap.add_argument("--instances", type=int, default=64, help="batch size (batch workload)")
+                         "graph replay, against the plain path run pass by pass; large: 16 "
+                         "realisations on 512x512x128 with 4 shells up to nside 1024 and 4 "
+                         "frequencies, forward + adjoint in batched graphs")
+    ap.add_argument("--instances", type=int, default=0,
+                    help="batch size (batch: 64, large: 16 by default)")
+    ap.add_argument("--check", type=int, default=0,
+                    help="instances checked against the plain path (batch: all, large: 1)")
     ap.add_argument("--precision", choices=("f32", "f64"), default="f32",
                     help="batch sample arithmetic (batch workload)")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -121,63 +127,79 @@ def pass_maps(d, dev):
     return m
 
 
-def likelihood_seeds(ctx, par, sp, i, dev):
-    """Fill sp["adj"] with dL/dmap of L = -1/2 sum (m - obs)^2 / sigma^2 over
-    the Q/U maps of every frequency and the FD map, m the instance's FP64
-    plain maps, obs the maps of a held-out realisation (seed i + 10001) plus
-    Gaussian noise of sigma = 0.1 x rms(obs).  The I and DM seeds are zero."""
+def regenerate(ctx, par, sp, seed):
+    """Draw the ES random B and DFT random TE of seed `seed` into the
+    instance's grid tensors (the batch graph re-packs them on every replay)."""
+    ctx.brnd_es(par.es_desc(), seed, *sp["brnd"])
+    ctx.ternd_dft(par.dft_desc(), seed, sp["ternd"])
+
+
+def likelihood_seeds(ctx, b, pars, specs, dev):
+    """Fill every instance's sp["adj"] with dL/dmap of
+    L = -1/2 sum (m - obs)^2 / sigma^2 over the Q/U maps of every frequency
+    and the FD map: obs is the batch's forward on a held-out realisation
+    (seed + 10000) plus Gaussian noise of sigma = 0.1 x rms(obs), m the
+    batch's forward on the instance itself.  The I and DM seeds are zero.
+    Both the batch and the plain adjoints are then given these same seeds."""
     import torch
-    n = sp["brnd"][0].shape[0]
-    held = tuple(torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
-    ctx.brnd_es(par.es_desc(), 10001 + i, *held[:3])
-    ctx.ternd_dft(par.dft_desc(), 10001 + i, held[3])
-    gen = torch.Generator(device=dev).manual_seed(20001 + i)
-    box = sp["box"]
-    for p, fi, d in plain_passes(sp["desc"], sp["freqs"]):
-        model, obs = pass_maps(d, dev), pass_maps(d, dev)
-        ctx.los_forward(d, model, brnd=sp["brnd"], brnd_box=box, ternd=sp["ternd"],
-                        ternd_box=box)
-        ctx.los_forward(d, obs, brnd=held[:3], brnd_box=box, ternd=held[3], ternd_box=box)
-        kinds = [("qs", sp["adj"]["qs"][fi]), ("us", sp["adj"]["us"][fi])]
-        sp["adj"]["is"][fi].zero_()
-        if p == 0:
-            kinds.append(("fd", sp["adj"]["fd"]))
-            sp["adj"]["dm"].zero_()
-        for k, seed in kinds:
-            o = obs[k]
+    keys = lambda sp: [(k, f) for k in ("qs", "us") for f in range(len(sp["freqs"]))] + [("fd", None)]  # noqa: E731
+    pick = lambda m, k, f: m[k][f] if f is not None else m[k]  # noqa: E731
+    for par, sp in zip(pars, specs):
+        regenerate(ctx, par, sp, par.brnd_seed + 10000)
+    b.run()
+    torch.cuda.synchronize()
+    obs = [{(k, f): pick(sp["maps"], k, f).clone() for k, f in keys(sp)} for sp in specs]
+    for par, sp in zip(pars, specs):
+        regenerate(ctx, par, sp, par.brnd_seed)
+    b.run()
+    torch.cuda.synchronize()
+    gen = torch.Generator(device=dev).manual_seed(20001)
+    for sp, ob in zip(specs, obs):
+        for t in sp["adj"]["is"]:
+            t.zero_()
+        sp["adj"]["dm"].zero_()
+        for (k, f), o in ob.items():
             sigma = 0.1 * float(o.square().mean().sqrt())
-            o = o + sigma * torch.randn(o.shape[0], dtype=torch.float64, device=dev,
-                                        generator=gen)
-            seed.copy_((o - model[k]) / (sigma * sigma))
-    del held
+            o += sigma * torch.randn(o.shape[0], dtype=torch.float64, device=dev, generator=gen)
+            pick(sp["adj"], k, f).copy_((o - pick(sp["maps"], k, f)) / (sigma * sigma))
 
 
 def run_batch(args):
-    """The batch workload: instance i has random fields of seed i + 1 on the
-    256x256x64 box, top nside (64, 128, 256)[i % 3] over 3 shells, the first
-    1 + i % 4 of (0.408, 1.4, 23, 30) GHz and a frozen (b0, psi0) draw.  One
-    step is one replay of the captured batch graph; the plain path is every
+    """The batch workloads.
+
+    batch: instance i has random fields of seed i + 1 on the 256x256x64 box,
+    top nside (64, 128, 256)[i % 3] over 3 shells, the first 1 + i % 4 of
+    (0.408, 1.4, 23, 30) GHz and a frozen (b0, psi0) draw; 64 instances.
+    large: 16 realisations (seeds 1-16) on 512x512x128 grids, 4 shells at
+    nside 128/256/512/1024 with 0.005 kpc steps, DM, FD and sync at all four
+    frequencies on nside 1024.
+
+    One step is one replay of the captured batch graph.  The plain path is an
     instance's passes run one by one through hamx_los_forward (FP64), the way
-    the reference pipeline runs them, and also gives the maps the batch is
-    checked against.
+    the reference pipeline runs them; it gives the maps (and, with the
+    adjoint, the gradients) the batch is checked against, for every instance
+    (batch) or the first --check instances (large).
 
     Map+gradient evaluations differentiate a Gaussian log-likelihood of the
     sync Q/U maps of every frequency plus the FD map against 'observed' maps:
-    the same instance on a held-out realisation (seed i + 10001) with 10%
-    Gaussian noise (sigma = 0.1 x the observed map's rms)."""
+    the same instance on a held-out realisation with 10% Gaussian noise (see
+    likelihood_seeds)."""
     import ctypes as C
     import torch
     from hammurabix_amd import Context, Param, _capi as capi, configs, los_samples
     from hammurabix_amd.autograd import desc_params
     dev = torch.device("cuda:0")
     prec = capi.HAMX_PREC_F32 if args.precision == "f32" else capi.HAMX_PREC_F64
-    n_inst = args.instances
+    large = args.workload == "large"
+    n_inst = args.instances if args.instances else (16 if large else 64)
+    n_check = min(n_inst, args.check if args.check else (1 if large else n_inst))
+    make = configs.large_instance if large else configs.batch_instance
     with_adj = not args.no_adjoint
     specs, pars = [], []
     with Context(0) as ctx:
         t0 = time.perf_counter()
         for i in range(n_inst):
-            par = Param(text=configs.batch_instance(i))
+            par = Param(text=make(i))
             pars.append(par)
             n = par.grid_brnd.full_size
             bx, by, bz, te = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
@@ -192,11 +214,9 @@ def run_batch(args):
             spec = dict(desc=d, freqs=freqs, maps=maps, brnd=(bx, by, bz), ternd=te,
                         box=par.grid_brnd.box())
             if with_adj:
-                gen = torch.Generator(device=dev).manual_seed(1000 + i)
-                rnd = lambda t: torch.randn(t.shape[0], dtype=torch.float64,  # noqa: E731
-                                            device=dev, generator=gen)
-                spec["adj"] = {"dm": rnd(maps["dm"]), "fd": rnd(maps["fd"]),
-                               **{k: [rnd(t) for t in maps[k]] for k in ("is", "qs", "us")}}
+                spec["adj"] = {"dm": z(d.nside_dm), "fd": z(d.nside_fd),
+                               **{k: [z(d.nside_sync) for _ in freqs]
+                                  for k in ("is", "qs", "us")}}
                 spec["grads"] = {
                     "dparams": torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev),
                     "dbrnd": tuple(torch.zeros(n, dtype=torch.float64, device=dev)
@@ -205,12 +225,11 @@ def run_batch(args):
             specs.append(spec)
         torch.cuda.synchronize()
         gen_s = time.perf_counter() - t0
-        if with_adj:
-            for i, sp in enumerate(specs):
-                likelihood_seeds(ctx, pars[i], sp, i, dev)
         t0 = time.perf_counter()
         b = ctx.batch(specs, precision=prec, tune=True, adjoint=with_adj)
         plan_s = time.perf_counter() - t0
+        if with_adj:
+            likelihood_seeds(ctx, b, pars, specs, dev)
         info = b.info()
         for _ in range(args.warmup):
             b.run()
@@ -245,7 +264,7 @@ def run_batch(args):
         map_err = {}  # worst rel. L2 per map kind and frequency
         worst_dln = 0.0
         dp_err = dict.fromkeys(capi.PARAM_NAMES, 0.0)
-        for sp in specs:
+        for sp in specs[:n_check]:
             freqs = sp["freqs"]
             kw = dict(brnd=sp["brnd"], brnd_box=sp["box"], ternd=sp["ternd"],
                       ternd_box=sp["box"])
@@ -318,10 +337,15 @@ def run_batch(args):
         "dtype": "f32" if prec == capi.HAMX_PREC_F32 else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"batch: {n_inst} ES/DFT realisations (seeds 1-{n_inst}) on "
-                        "256x256x64, top nside (64,128,256)[i%3] over 3 shells, first "
-                        "1+i%4 of (0.408,1.4,23,30) GHz, frozen b0/psi0 draws; one step = "
-                        "one batch graph replay",
+            "workload": (f"large: {n_inst} ES/DFT realisations (seeds 1-{n_inst}) on "
+                         "512x512x128, 4 shells at nside 128/256/512/1024 with 0.005 kpc steps, "
+                         "DM, FD and sync at 0.408/1.4/23/30 GHz on nside 1024; one step = one "
+                         f"batch graph replay; {n_check} instance(s) checked against the plain path"
+                         if large else
+                         f"batch: {n_inst} ES/DFT realisations (seeds 1-{n_inst}) on "
+                         "256x256x64, top nside (64,128,256)[i%3] over 3 shells, first "
+                         "1+i%4 of (0.408,1.4,23,30) GHz, frozen b0/psi0 draws; one step = "
+                         "one batch graph replay"),
             "instances": n_inst,
             "samples_ref_per_step": info["samples_ref"],
             "samples_geo_per_step": info["samples_geo"],
@@ -342,9 +366,9 @@ def run_batch(args):
             "max_rel_l2_by_map": map_err,
             "autotune_ms": info["variant_ms"],
             **mg,
-            **({"plain_map_gradient_evals_per_s": n_inst / (plain_s + plain_mg_s),
+            **({"plain_map_gradient_evals_per_s": n_check / (plain_s + plain_mg_s),
                 "map_gradient_speedup_vs_plain":
-                    (mg["batch_map_gradient_evals_per_s"] * (plain_s + plain_mg_s) / n_inst),
+                    (mg["batch_map_gradient_evals_per_s"] * (plain_s + plain_mg_s) / n_check),
                 "max_rel_err_dparams_vs_plain_fp64": worst_dp,
                 "max_rel_err_by_param": dp_err,
                 "max_rel_l2_dlnparams_vs_plain_fp64": worst_dln,
@@ -365,7 +389,7 @@ def main():
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
         return
-    if args.workload == "batch":
+    if args.workload in ("batch", "large"):
         run_batch(args)
         return
     import numpy as np

@@ -210,3 +210,20 @@ def batch_instance(i: int, **kw) -> str:
                 b0=b0, psi0=psi0, freqs=list(BATCH_FREQS_GHZ[:1 + i % 4]))
     args.update(kw)
     return xml(**args)
+
+
+# ---- the large batched adjoint (512x512x128 grids, 4 shells up to nside 1024)
+
+LARGE_GRID = (512, 512, 128)
+
+
+def large_instance(i: int, **kw) -> str:
+    """Instance i (0-based) of the large batch: ES random B and DFT random TE
+    with seed i + 1 on a 512x512x128 box over the same extent, 4 automatic
+    shells at nside 128/256/512/1024 (750/750/1500/3000 steps of 0.005 kpc,
+    0-30 kpc), DM, FD and sync at 0.408, 1.4, 23 and 30 GHz on nside 1024,
+    the default LSA / TE / CRE parameters."""
+    args = dict(shell_num=4, nside_sim=128, nside_out=1024, oc_r_res=0.005, random=True,
+                grid=LARGE_GRID, seed=i + 1, freqs=list(BATCH_FREQS_GHZ))
+    args.update(kw)
+    return xml(**args)
