@@ -999,14 +999,13 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
   const uint64_t npix = in.sh[s].hs.npix;
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
   if constexpr (REV && PREC == HAMX_PREC_F32) {
-    // one ray per lane (a.rays == 64): a NESTED 8x8 patch of the shell when
-    // its nside is a power of two (the lanes' samples at one step then lie
-    // within a few grid cells), else 64 consecutive RING pixels
+    // one ray per lane (a.rays == 64): a NESTED 8x8 patch of the shell (the
+    // lanes' samples at one step then lie within a few grid cells; shell
+    // nsides are powers of two, validate_los)
     const Healpix &hs = in.sh[s].hs;
     const uint64_t pl = p0 + static_cast<uint64_t>(lane);
     const bool live = pl < npix;
-    const bool nested = (hs.nside & (hs.nside - 1)) == 0;
-    const uint64_t ipix = live ? (nested ? hp_nest2ring(hs, pl) : pl) : 0;
+    const uint64_t ipix = live ? hp_nest2ring(hs, pl) : 0;
     RayGeo r{};
     if (live)
       ray_setup_call(in.ph, shell_args_rev(in, s), ipix, r);

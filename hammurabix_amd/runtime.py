@@ -43,6 +43,19 @@ def _ptr(a: Any) -> tuple[Optional[int], Optional[bool]]:
     raise TypeError(f"unsupported array type {type(a)!r}")
 
 
+def _order_after_torch(arrays: list[Any]) -> None:
+    """The library runs on its own HIP stream.  Before it reads or writes
+    torch CUDA tensors, torch's pending work on the device (zero-fills,
+    copies, elementwise ops on those tensors) must be complete, or the two
+    streams race; the library synchronises its own stream before every call
+    returns, so torch work issued after a call sees its results."""
+    for a in arrays:
+        if a is not None and _is_torch(a) and a.is_cuda:
+            import torch
+            torch.cuda.synchronize(a.device)
+            return
+
+
 def _residency(arrays: list[Any], what: str) -> bool:
     res = {_ptr(a)[1] for a in arrays if a is not None}
     if len(res) > 1:
@@ -148,6 +161,7 @@ class Context:
             flags |= capi.HAMX_OUT_DEVICE
         if keep_tape:
             flags |= capi.HAMX_KEEP_TAPE
+        _order_after_torch(inputs + outs)
         self._check(self.lib.hamx_los_forward(self.handle, C.byref(desc), C.byref(f),
                                               C.byref(m), flags))
 
@@ -182,6 +196,7 @@ class Context:
         if use_tape:
             flags |= capi.HAMX_USE_TAPE
         gx, gy, gz = (_ptr(a)[0] for a in dbrnd) if dbrnd is not None else (None,) * 3
+        _order_after_torch(inputs + outs)
         self._check(self.lib.hamx_los_adjoint(self.handle, C.byref(desc), C.byref(f),
                                               C.byref(m), _ptr(dparams)[0], gx, gy, gz,
                                               _ptr(dternd)[0], flags))
@@ -207,11 +222,13 @@ class Context:
             desc.breg_x, desc.breg_y, desc.breg_z = (_ptr(a)[0] for a in breg_grid)
             if _residency(list(breg_grid), "input"):
                 flags |= capi.HAMX_IN_DEVICE
+        _order_after_torch([bx, by, bz] + (list(breg_grid) if breg_grid is not None else []))
         self._check(self.lib.hamx_brnd_es(self.handle, C.byref(desc), seed,
                                           _ptr(bx)[0], _ptr(by)[0], _ptr(bz)[0], flags))
 
     def ternd_dft(self, desc: capi.HamxDftDesc, seed: int, te: Any) -> None:
         flags = capi.HAMX_OUT_DEVICE if _residency([te], "output") else 0
+        _order_after_torch([te])
         self._check(self.lib.hamx_ternd_dft(self.handle, C.byref(desc), seed,
                                             _ptr(te)[0], flags))
 
@@ -219,12 +236,14 @@ class Context:
         """In-place FFT of interleaved complex128 data stored as float64 pairs."""
         addr, dev = _ptr(data)
         flags = capi.HAMX_IN_DEVICE | capi.HAMX_OUT_DEVICE if dev else 0
+        _order_after_torch([data])
         self._check(self.lib.hamx_fft3d(self.handle, addr, nx, ny, nz, sign, flags))
 
     def gauss_pairs(self, seed: int, stream: int, first: int, count: int,
                     out: Any) -> None:
         addr, dev = _ptr(out)
         flags = capi.HAMX_OUT_DEVICE if dev else 0
+        _order_after_torch([out])
         self._check(self.lib.hamx_gauss_pairs(self.handle, seed, stream, first, count,
                                               addr, flags))
 
@@ -306,15 +325,18 @@ class Batch:
         if adjoint:
             flags |= capi.HAMX_BATCH_ADJOINT
         h = C.c_void_p()
+        _order_after_torch(self.keep)
         ctx._check(ctx.lib.hamx_batch_create(ctx.handle, arr, len(instances), precision,
                                              flags, C.byref(h)))
         self.handle = h
         self._arr = arr
 
     def run(self) -> None:
+        _order_after_torch(self.keep)
         self.ctx._check(self.ctx.lib.hamx_batch_run(self.handle))
 
     def run_adjoint(self) -> None:
+        _order_after_torch(self.keep)
         self.ctx._check(self.ctx.lib.hamx_batch_run_adjoint(self.handle))
 
     def info(self) -> dict:

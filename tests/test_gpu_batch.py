@@ -306,3 +306,95 @@ def test_batch_adjoint_needs_the_flag(ctx):
         with pytest.raises(HamxError) as e:
             b.run_adjoint()
         assert e.value.code == capi.HAMX_ERR_ARG
+
+
+def large_specs():
+    """Two instances in the shape of the large batched adjoint (512x512x128
+    grids, 4 shells, 4 frequencies, DM / FD / sync), at a small sky
+    resolution: shells at nside 4..32 and 8..64 (3 and 12 NESTED patches of
+    64 rays in the innermost shell of the reverse march)."""
+    torch, dev = torch_dev()
+    specs = []
+    for i, ns in enumerate((4, 8)):
+        par = Param(text=configs.large_instance(i, nside_sim=ns, nside_out=8 * ns,
+                                                oc_r_res=0.02))
+        d = par.los_desc()
+        n = par.grid_brnd.full_size
+        gen = torch.Generator(device=dev).manual_seed(100 + i)
+        b = tuple(torch.randn(n, dtype=torch.float64, device=dev, generator=gen) * 2e-6
+                  for _ in range(3))
+        te = torch.randn(n, dtype=torch.float64, device=dev, generator=gen) * 0.003
+        z = lambda k: torch.zeros(npix(k), dtype=torch.float64, device=dev)  # noqa: E731
+        freqs = list(par.grid_obs.sim_sync_freq)
+        maps = dict(dm=z(d.nside_dm), fd=z(d.nside_fd),
+                    **{k: [z(d.nside_sync) for _ in freqs] for k in ("is", "qs", "us")})
+        specs.append(dict(desc=d, freqs=freqs, maps=maps, brnd=b, ternd=te,
+                          box=par.grid_brnd.box()))
+    return specs
+
+
+def plain_dev(ctx, spec):
+    """Plain forward and adjoint passes with device-resident grids and maps:
+    (expected maps, dparams, (dbx, dby, dbz), dte) summed over the passes."""
+    torch, dev = torch_dev()
+    d0, freqs, adj = spec["desc"], spec["freqs"], spec["adj"]
+    kw = dict(brnd=spec["brnd"], brnd_box=spec["box"], ternd=spec["ternd"],
+              ternd_box=spec["box"])
+    n = spec["brnd"][0].shape[0]
+    exp = {"is": [None] * len(freqs), "qs": [None] * len(freqs), "us": [None] * len(freqs)}
+    dp_tot = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
+    gb_tot = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+    gt_tot = torch.zeros(n, dtype=torch.float64, device=dev)
+    for p, fi in enumerate(reversed(range(len(freqs)))):
+        d = capi.HamxLosDesc()
+        C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
+        d.sync_freq, d.do_sync = freqs[fi], 1
+        if p > 0:
+            d.do_dm = d.do_fd = 0
+        z = lambda k: torch.zeros(npix(k), dtype=torch.float64, device=dev)  # noqa: E731
+        m = {k: z(d.nside_sync) for k in ("is", "qs", "us")}
+        m["fd"] = z(d.nside_fd if d.do_fd else d.nside_sync)
+        if d.do_dm:
+            m["dm"] = z(d.nside_dm)
+        ctx.los_forward(d, m, keep_tape=True, **kw)
+        for k in ("is", "qs", "us"):
+            exp[k][fi] = m[k].cpu().numpy()
+        if p == 0:
+            exp["dm"], exp["fd"] = m["dm"].cpu().numpy(), m["fd"].cpu().numpy()
+        a = {k: adj[k][fi] for k in ("is", "qs", "us")}
+        if p == 0:
+            a["dm"], a["fd"] = adj["dm"], adj["fd"]
+        dp = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
+        gb = tuple(torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3))
+        gt = torch.zeros(n, dtype=torch.float64, device=dev)
+        ctx.los_adjoint(d, a, dp, dbrnd=gb, dternd=gt, use_tape=True, **kw)
+        dp_tot += dp
+        for t, g in zip(gb_tot, gb):
+            t += g
+        gt_tot += gt
+    return exp, dp_tot.cpu().numpy(), [g.cpu().numpy() for g in gb_tot], gt_tot.cpu().numpy()
+
+
+def test_large_shape_batch_forward_and_adjoint(ctx):
+    """FP32 batch on 512x512x128 grids with 4 shells and 4 frequencies: maps
+    within 1e-5 and gradients within 1e-4 of the FP64 plain path."""
+    print("case: large-shape FP32 batch, forward and adjoint", flush=True)
+    specs = with_adjoint(large_specs(), seed=31)
+    with ctx.batch(specs, precision=capi.HAMX_PREC_F32, tune=False, adjoint=True) as b:
+        b.run()
+        scale_adjoint(specs)
+        b.run_adjoint()
+        for i, sp in enumerate(specs):
+            exp, dp, gb, gt = plain_dev(ctx, sp)
+            compare(sp, exp, 1e-5, f"large inst{i}")
+            got = sp["grads"]["dparams"].cpu().numpy()
+            rel = np.abs(got - dp) / np.maximum(np.abs(dp), 1e-300)
+            print(f"large inst{i} dparams rel {np.array2string(rel, precision=2)}", flush=True)
+            assert np.all(rel <= 1e-4), (i, got, dp)
+            for name, g, r in (("bx", sp["grads"]["dbrnd"][0], gb[0]),
+                               ("by", sp["grads"]["dbrnd"][1], gb[1]),
+                               ("bz", sp["grads"]["dbrnd"][2], gb[2]),
+                               ("te", sp["grads"]["dternd"], gt)):
+                err = rel_l2(g.cpu().numpy(), r)
+                print(f"large inst{i} d{name} rel L2 {err:.2e}", flush=True)
+                assert err <= 1e-4, (i, name, err)
