@@ -70,7 +70,9 @@ def build_host() -> list[str]:
     if not srcs:
         return []
     lib_srcs = [s for s in srcs if not s.endswith("main.cc")]
-    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", INCLUDE, "-I", HOST]
+    # -I CSRC: the field models shared with the device code (hamx_models.h)
+    flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", INCLUDE, "-I", HOST,
+             "-I", CSRC]
     out = []
     tmp = HOST_LIB + ".tmp"
     _run(["g++", *flags, "-shared", "-o", tmp, *lib_srcs,

@@ -24,7 +24,9 @@ HAMX_OUT_DEVICE = 2
 HAMX_KEEP_TAPE = 8
 HAMX_USE_TAPE = 16
 
-HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID = range(5)
+(HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID,
+ HAMX_BREG_HELIX, HAMX_BREG_JF12) = range(7)
+HAMX_BREG_NPARAM = 24
 HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID = range(3)
 HAMX_CRE_NONE, HAMX_CRE_ANA, HAMX_CRE_UNIF = range(3)
 HAMX_SCHED_AUTO, HAMX_SCHED_THREAD, HAMX_SCHED_WAVE = range(3)
@@ -60,7 +62,7 @@ class HamxLosDesc(C.Structure):
                 ("do_dm", C.c_int32), ("do_fd", C.c_int32), ("do_sync", C.c_int32),
                 ("nside_dm", C.c_uint32), ("nside_fd", C.c_uint32), ("nside_sync", C.c_uint32),
                 ("sync_freq", C.c_double),
-                ("breg_model", C.c_int32), ("breg", C.c_double * 8),
+                ("breg_model", C.c_int32), ("breg", C.c_double * HAMX_BREG_NPARAM),
                 ("tereg_model", C.c_int32), ("tereg", C.c_double * 4),
                 ("cre_model", C.c_int32), ("cre", C.c_double * 8),
                 ("schedule", C.c_int32), ("reserved", C.c_int32)]
@@ -77,7 +79,7 @@ class HamxEsDesc(C.Structure):
                 ("a0", C.c_double), ("a1", C.c_double), ("rho", C.c_double),
                 ("r0", C.c_double), ("z0", C.c_double),
                 ("breg_model", C.c_int32), ("reserved", C.c_int32),
-                ("breg", C.c_double * 8),
+                ("breg", C.c_double * HAMX_BREG_NPARAM),
                 ("breg_x", C.c_void_p), ("breg_y", C.c_void_p), ("breg_z", C.c_void_p),
                 ("breg_box", HamxBox)]
 

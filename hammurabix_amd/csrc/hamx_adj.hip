@@ -287,8 +287,10 @@ bool same_desc(const hamx_los_desc &a, const hamx_los_desc &b) {
     eq = eq && a.nside_shell[i] == b.nside_shell[i];
   for (int i = 0; i <= HAMX_MAX_SHELL; ++i)
     eq = eq && a.radii_shell[i] == b.radii_shell[i];
+  for (int i = 0; i < HAMX_BREG_NPARAM; ++i)
+    eq = eq && a.breg[i] == b.breg[i];
   for (int i = 0; i < 8; ++i)
-    eq = eq && a.breg[i] == b.breg[i] && a.cre[i] == b.cre[i];
+    eq = eq && a.cre[i] == b.cre[i];
   for (int i = 0; i < 4; ++i)
     eq = eq && a.tereg[i] == b.tereg[i];
   return eq;

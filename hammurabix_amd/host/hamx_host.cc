@@ -1,5 +1,6 @@
 // Field plugins, grids, integrator and pipeline over the C-ABI.
 #include "hamx_host.h"
+#include "hamx_models.h"
 
 #include <chrono>
 #include <cstring>
@@ -342,6 +343,33 @@ int Breg_unif::device_model(const Param *par, double *p) const {
   p[1] = par->breg_unif.bv;
   p[2] = par->breg_unif.l0;
   return HAMX_BREG_UNIF;
+}
+// breg_helix.cc:13-30 and breg_jf12.cc:8-189, through the evaluators the
+// device march uses (hamx_models.h)
+int Breg_helix::device_model(const Param *par, double *p) const {
+  p[0] = par->breg_helix.bx;
+  p[1] = par->breg_helix.by;
+  p[2] = par->breg_helix.bz;
+  p[3] = par->breg_helix.r_min;
+  p[4] = par->breg_helix.r_max;
+  return HAMX_BREG_HELIX;
+}
+Vec3 Breg_helix::write_field(const Vec3 &pos, const Param *par) const {
+  double p[HAMX_BREG_NPARAM] = {}, b[3];
+  device_model(par, p);
+  hamx::breg_helix_eval(p, pos[0], pos[1], pos[2], b);
+  return Vec3(b[0], b[1], b[2]);
+}
+int Breg_jf12::device_model(const Param *par, double *p) const {
+  for (int i = 0; i < 20; ++i)
+    p[i] = par->breg_jf12[i];
+  return HAMX_BREG_JF12;
+}
+Vec3 Breg_jf12::write_field(const Vec3 &pos, const Param *par) const {
+  double p[HAMX_BREG_NPARAM] = {}, b[3];
+  device_model(par, p);
+  hamx::breg_jf12_eval(p, pos[0], pos[1], pos[2], b);
+  return Vec3(b[0], b[1], b[2]);
 }
 
 double TEreg::write_field(const Vec3 &, const Param *) const { return 0.; }
@@ -690,6 +718,10 @@ void Pipeline::assemble_breg() {
     breg = std::make_unique<Breg_cart>();
   } else if (par->breg_type == "unif") {
     breg = std::make_unique<Breg_unif>();
+  } else if (par->breg_type == "helix") {
+    breg = std::make_unique<Breg_helix>();
+  } else if (par->breg_type == "jf12") {
+    breg = std::make_unique<Breg_jf12>();
   } else {
     throw std::runtime_error("unsupported breg model");
   }

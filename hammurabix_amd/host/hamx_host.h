@@ -68,6 +68,10 @@ public:
   struct { double b0 = 0, psi0 = 0, psi1 = 0, chi0 = 0; } breg_lsa;
   struct { double bp = 0, bv = 0, l0 = 0; } breg_unif;
   struct { double bx = 0, by = 0, bz = 0; } breg_cart;
+  struct { double bx = 0, by = 0, bz = 0, r_min = 0, r_max = 0; } breg_helix;
+  // in the HAMX_BREG_JF12 parameter order (b_arm1..7, b_ring, h_disk, w_disk,
+  // Bn, Bs, rn, rs, wh, z0, B0_X, Xtheta, rpc_X, r0_X)
+  double breg_jf12[20] = {};
   unsigned long long brnd_seed = 0;
   struct { double rms = 0, k0 = 0, k1 = 0, a0 = 0, a1 = 0, rho = 0, r0 = 0, z0 = 0; } brnd_es;
   std::string tereg_type, ternd_type, ternd_method;
@@ -175,22 +179,32 @@ public:
   // sample write_field on the box_breg vertices (breg.cc:99-121)
   void write_grid(const Param *par, Grid_breg *grid) const;
   // device evaluator: HAMX_BREG_* and its parameters, or -1 (none)
-  virtual int device_model(const Param *par, double p[8]) const;
+  virtual int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const;
 };
 class Breg_lsa : public Breg {
 public:
   Vec3 write_field(const Vec3 &pos, const Param *par) const override;
-  int device_model(const Param *par, double p[8]) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
 };
 class Breg_cart : public Breg {
 public:
   Vec3 write_field(const Vec3 &pos, const Param *par) const override;
-  int device_model(const Param *par, double p[8]) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
 };
 class Breg_unif : public Breg {
 public:
   Vec3 write_field(const Vec3 &pos, const Param *par) const override;
-  int device_model(const Param *par, double p[8]) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
+};
+class Breg_helix : public Breg {
+public:
+  Vec3 write_field(const Vec3 &pos, const Param *par) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
+};
+class Breg_jf12 : public Breg {
+public:
+  Vec3 write_field(const Vec3 &pos, const Param *par) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
 };
 
 class Brnd {

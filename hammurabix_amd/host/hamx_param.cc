@@ -158,7 +158,32 @@ Param::Param(const std::string &xml_file) {
       breg_cart.bx = fval(s, "bx") * units::muGauss;
       breg_cart.by = fval(s, "by") * units::muGauss;
       breg_cart.bz = fval(s, "bz") * units::muGauss;
-    } else if (breg_type != "jaffe" && breg_type != "helix" && breg_type != "jf12") {
+    } else if (breg_type == "helix") { // param.cc:258-270
+      const Element *s = trace(mf, {"regular", "helix"});
+      breg_helix.bx = fval(s, "bx") * units::muGauss;
+      breg_helix.by = fval(s, "by") * units::muGauss;
+      breg_helix.bz = fval(s, "bz") * units::muGauss;
+      breg_helix.r_min = fval(s, "r_min") * units::kpc;
+      breg_helix.r_max = fval(s, "r_max") * units::kpc;
+    } else if (breg_type == "jf12") { // param.cc:271-322
+      const Element *s = trace(mf, {"regular", "jf12"});
+      const char *mu[] = {"b_arm1", "b_arm2", "b_arm3", "b_arm4", "b_arm5", "b_arm6",
+                          "b_arm7", "b_ring"};
+      for (int i = 0; i < 8; ++i)
+        breg_jf12[i] = fval(s, mu[i]) * units::muGauss;
+      breg_jf12[8] = fval(s, "h_disk") * units::kpc;
+      breg_jf12[9] = fval(s, "w_disk") * units::kpc;
+      breg_jf12[10] = fval(s, "Bn") * units::muGauss;
+      breg_jf12[11] = fval(s, "Bs") * units::muGauss;
+      breg_jf12[12] = fval(s, "rn") * units::kpc;
+      breg_jf12[13] = fval(s, "rs") * units::kpc;
+      breg_jf12[14] = fval(s, "wh") * units::kpc;
+      breg_jf12[15] = fval(s, "z0") * units::kpc;
+      breg_jf12[16] = fval(s, "B0_X") * units::muGauss;
+      breg_jf12[17] = fval(s, "Xtheta") * units::rad;
+      breg_jf12[18] = fval(s, "rpc_X") * units::kpc;
+      breg_jf12[19] = fval(s, "r0_X") * units::kpc;
+    } else if (breg_type != "jaffe") {
       throw std::runtime_error("unsupported breg model");
     }
   }

@@ -151,6 +151,12 @@ def _box(root: ET.Element, name: str, g: GridParam) -> None:
     g.z_min = CGS.kpc * b.f("value", "z_min")
 
 
+# Parameter order of HAMX_BREG_JF12 (include/hamx.h)
+JF12_PARAMS = ("b_arm1", "b_arm2", "b_arm3", "b_arm4", "b_arm5", "b_arm6", "b_arm7", "b_ring",
+               "h_disk", "w_disk", "Bn", "Bs", "rn", "rs", "wh", "z0", "B0_X", "Xtheta",
+               "rpc_X", "r0_X")
+
+
 class Param:
     """Parsed parameter set (param.cc:12-28)."""
 
@@ -288,7 +294,23 @@ class Param:
                 self.breg_unif = dict(bp=s.f("value", "bp") * CGS.muGauss,
                                       bv=s.f("value", "bv") * CGS.muGauss,
                                       l0=s.f("value", "l0") * CGS.rad)
-            elif self.breg_type in ("jaffe", "helix", "jf12"):
+            elif self.breg_type == "helix":  # param.cc:258-270
+                s = _trace(root, reg + ["helix"])
+                self.breg_helix = dict(bx=s.f("value", "bx") * CGS.muGauss,
+                                       by=s.f("value", "by") * CGS.muGauss,
+                                       bz=s.f("value", "bz") * CGS.muGauss,
+                                       r_min=s.f("value", "r_min") * CGS.kpc,
+                                       r_max=s.f("value", "r_max") * CGS.kpc)
+            elif self.breg_type == "jf12":  # param.cc:271-322
+                s = _trace(root, reg + ["jf12"])
+                mu = [f"b_arm{i}" for i in range(1, 8)] + ["b_ring"]
+                self.breg_jf12 = {k: s.f("value", k) * CGS.muGauss for k in mu}
+                for k in ("h_disk", "w_disk", "rn", "rs", "wh", "z0", "rpc_X", "r0_X"):
+                    self.breg_jf12[k] = s.f("value", k) * CGS.kpc
+                for k in ("Bn", "Bs", "B0_X"):
+                    self.breg_jf12[k] = s.f("value", k) * CGS.muGauss
+                self.breg_jf12["Xtheta"] = s.f("value", "Xtheta") * CGS.rad
+            elif self.breg_type == "jaffe":
                 pass  # parsed by the reference; no device evaluator yet
             else:
                 raise RuntimeError("unsupported breg model")
@@ -440,6 +462,13 @@ class Param:
             d.breg_model = capi.HAMX_BREG_UNIF
             p = self.breg_unif
             d.breg[0:3] = [p["bp"], p["bv"], p["l0"]]
+        elif self.breg_type == "helix":
+            d.breg_model = capi.HAMX_BREG_HELIX
+            p = self.breg_helix
+            d.breg[0:5] = [p["bx"], p["by"], p["bz"], p["r_min"], p["r_max"]]
+        elif self.breg_type == "jf12":
+            d.breg_model = capi.HAMX_BREG_JF12
+            d.breg[0:20] = [self.breg_jf12[k] for k in JF12_PARAMS]
         else:
             raise RuntimeError(f"breg model '{self.breg_type}' has no device evaluator")
         # regular TE

@@ -54,6 +54,13 @@ extern "C" {
 #define HAMX_BREG_CART 2 /* "cart": breg[0..2] = bx, by, bz [G] */
 #define HAMX_BREG_UNIF 3 /* "unif": breg[0..2] = bp, bv [G], l0 [rad] */
 #define HAMX_BREG_GRID 4 /* <fieldio><breg read="1">: trilinear gather of fields.breg_* */
+#define HAMX_BREG_HELIX 5 /* "helix": breg[0..4] = bx, by, bz [G], r_min, r_max [cm]
+                             (breg_helix.cc:13-30) */
+#define HAMX_BREG_JF12 6  /* "jf12": breg[0..6] = b_arm1..7, [7] b_ring [G], [8] h_disk,
+                             [9] w_disk [cm], [10] Bn, [11] Bs [G], [12] rn, [13] rs,
+                             [14] wh, [15] z0 [cm], [16] B0_X [G], [17] Xtheta [rad],
+                             [18] rpc_X, [19] r0_X [cm] (breg_jf12.cc:8-189) */
+#define HAMX_BREG_NPARAM 24 /* parameters per regular-field model */
 /* regular thermal electrons, Param::tereg_type (param.cc:462-667) */
 #define HAMX_TEREG_NONE 0 /* zero density */
 #define HAMX_TEREG_UNIF 1 /* "unif": tereg[0..1] = n0 [cm^-3], r0 [cm] */
@@ -111,7 +118,7 @@ typedef struct hamx_los_desc {
   uint32_t nside_dm, nside_fd, nside_sync;
   double sync_freq; /* Hz */
   int32_t breg_model;
-  double breg[8];
+  double breg[HAMX_BREG_NPARAM];
   int32_t tereg_model;
   double tereg[4];
   int32_t cre_model;
@@ -142,7 +149,7 @@ typedef struct hamx_es_desc {
   double r0, z0;       /* energy profile scales, cm */
   int32_t breg_model;
   int32_t reserved;
-  double breg[8];
+  double breg[HAMX_BREG_NPARAM];
   const double *breg_x, *breg_y, *breg_z;
   hamx_box breg_box;
 } hamx_es_desc;

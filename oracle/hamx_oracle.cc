@@ -400,7 +400,84 @@ double read_grid1(const double *f, const hamx_box &b, const V3 &pos) {
 }
 
 // --------------------------------------------------------------- models --
-// Breg_lsa::write_field (breg_lsa.cc:9-26), Breg_cart, Breg_unif
+// Breg_jf12::write_field (breg_jf12.cc:8-189), restated step by step: disk
+// (ring below 5 kpc, 8 spiral regions above, the 8th strength closing the
+// flux), toroidal halo, X-field; cylindrical sum, then Cartesian.
+V3 breg_jf12(const double *p, const V3 &pos) {
+  const double b_arm[7] = {p[0], p[1], p[2], p[3], p[4], p[5], p[6]};
+  const double b_ring = p[7], h_disk = p[8], w_disk = p[9];
+  const double Bn = p[10], Bs = p[11], rn = p[12], rs = p[13], wh = p[14], z0 = p[15];
+  const double B0_X = p[16], Xtheta_const = p[17], rpc_X = p[18], r0_X = p[19];
+  const double Rmax = 20 * kKpc, rho_GC = 1. * kKpc;
+  const double inc = 11.5, rmin = 5. * kKpc, rcent = 3. * kKpc;
+  const double f[8] = {0.130, 0.165, 0.094, 0.122, 0.13, 0.118, 0.084, 0.156};
+  const double rc_B[8] = {5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5};
+  const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+  const double rho = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+  const double phi = std::atan2(pos[1], pos[0]);
+  const double z = pos[2];
+  if (r > Rmax || rho < rho_GC)
+    return v3(1.e-12, 1.e-12, 1.e-12);
+  const double B0 = rmin / r;
+  const double zprofile = 1. / (1 + std::exp(-2. / w_disk * (std::fabs(z) - h_disk)));
+  double disk[3] = {0., 0., 0.};
+  if (r > rcent) {
+    if (r < rmin) {
+      disk[1] = B0 * b_ring * (1 - zprofile);
+    } else {
+      double bv[8] = {b_arm[0], b_arm[1], b_arm[2], b_arm[3], b_arm[4], b_arm[5], b_arm[6], 0.};
+      double b8 = 0.;
+      for (int i = 0; i < 7; i++)
+        b8 -= f[i] * bv[i] / f[7];
+      bv[7] = b8;
+      double b_disk = 0.;
+      double r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi - M_PI));
+      if (r_negx > rc_B[7] * kKpc)
+        r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi + M_PI));
+      if (r_negx > rc_B[7] * kKpc)
+        r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi + 3 * M_PI));
+      for (int i = 7; i >= 0; i--)
+        if (r_negx < rc_B[i] * kKpc)
+          b_disk = bv[i];
+      disk[0] = b_disk * B0 * std::sin(M_PI / 180. * inc) * (1 - zprofile);
+      disk[1] = b_disk * B0 * std::cos(M_PI / 180. * inc) * (1 - zprofile);
+    }
+  }
+  double b1, rh;
+  if (z >= 0) {
+    b1 = Bn;
+    rh = rn;
+  } else {
+    b1 = Bs;
+    rh = rs;
+  }
+  const double B_h =
+      b1 * (1. - 1. / (1. + std::exp(-2. / wh * (r - rh)))) * std::exp(-(std::fabs(z)) / (z0));
+  const double halo[3] = {0., B_h * zprofile, 0.};
+  double Xtheta, rp_X, B_X;
+  const double r_sign = z < 0 ? -1. : 1.;
+  const double rc_X = rpc_X + std::fabs(z) / std::tan(Xtheta_const);
+  if (r < rc_X) {
+    rp_X = r * rpc_X / rc_X;
+    B_X = B0_X * std::pow(rpc_X / rc_X, 2.) * std::exp(-rp_X / r0_X);
+    Xtheta = std::atan(std::fabs(z) / (r - rp_X));
+    if (z == 0.)
+      Xtheta = M_PI / 2.;
+  } else {
+    Xtheta = Xtheta_const;
+    rp_X = r - std::fabs(z) / std::tan(Xtheta);
+    B_X = B0_X * rp_X / r * std::exp(-rp_X / r0_X);
+  }
+  const double xf[3] = {B_X * std::cos(Xtheta) * r_sign, 0., B_X * std::sin(Xtheta)};
+  double cyl[3];
+  for (int i = 0; i < 3; ++i)
+    cyl[i] = disk[i] + halo[i] + xf[i];
+  return v3(cyl[0] * std::cos(phi) - cyl[1] * std::sin(phi),
+            cyl[0] * std::sin(phi) + cyl[1] * std::cos(phi), cyl[2]);
+}
+
+// Breg_lsa::write_field (breg_lsa.cc:9-26), Breg_cart, Breg_unif,
+// Breg_helix (breg_helix.cc:13-30), Breg_jf12
 V3 breg_model(int model, const double *p, const V3 &pos) {
   if (model == HAMX_BREG_LSA) {
     const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
@@ -420,6 +497,15 @@ V3 breg_model(int model, const double *p, const V3 &pos) {
     return v3(p[0], p[1], p[2]);
   if (model == HAMX_BREG_UNIF)
     return v3(p[0] * std::cos(p[2]), p[0] * std::sin(p[2]), p[1]);
+  if (model == HAMX_BREG_HELIX) {
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    const double phi = std::atan2(pos[1], pos[0]) + M_PI / 2.0;
+    if ((r > p[3]) && (r < p[4]))
+      return v3(p[0] * std::cos(phi), p[1] * std::sin(phi), p[2]);
+    return v3(0., 0., 0.);
+  }
+  if (model == HAMX_BREG_JF12)
+    return breg_jf12(p, pos);
   return v3(0., 0., 0.);
 }
 

@@ -99,3 +99,36 @@ def test_cli_and_pipeline_with_mask(tmp_path, ctx):
     assert np.array_equal(q_cli, py[("sync", "23.0", "8", "Q")])
     assert rel_l2(q_cli, ref["qs"]) < 1e-10
     assert np.all(q_cli[piv != 1.0] == 0.0)
+
+
+@pytest.mark.gpu
+def test_cli_jf12_and_helix_match_python_pipeline_and_oracle(tmp_path, ctx):
+    """The C++ host reads the reference's <jf12> / <helix> blocks and runs them
+    through the device evaluators: its maps equal the Python pipeline's and
+    the oracle's (no random fields)."""
+    need_cli()
+    jf12 = ('<jf12><b_arm1 value="0.1"/><b_arm2 value="3.0"/><b_arm3 value="-0.9"/>'
+            '<b_arm4 value="-0.8"/><b_arm5 value="-2.0"/><b_arm6 value="-4.2"/>'
+            '<b_arm7 value="0.0"/><b_ring value="0.1"/><h_disk value="0.40"/>'
+            '<w_disk value="0.27"/><Bn value="1.4"/><Bs value="-1.1"/><rn value="9.22"/>'
+            '<rs value="16.7"/><wh value="0.20"/><z0 value="5.3"/><B0_X value="4.6"/>'
+            '<Xtheta value="49"/><rpc_X value="4.8"/><r0_X value="2.9"/></jf12>')
+    helix = ('<helix><bx value="1.0"/><by value="2.0"/><bz value="0.5"/>'
+             '<r_min value="3.0"/><r_max value="12.0"/></helix>')
+    for name, block in (("jf12", jf12), ("helix", helix)):
+        out = str(tmp_path / name) + "_"
+        text = configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05, out=out)
+        text = text.replace('type="lsa"', f'type="{name}"').replace('<lsa>', block + '<lsa>')
+        x = tmp_path / f"{name}.xml"
+        x.write_text(text)
+        r = run(str(x))
+        assert r.returncode == 0, r.stderr
+        py = Pipeline(text=text, ctx=ctx, export=False).run()
+        ref = write_grid(Param(text=text).los_desc())
+        fd = np.fromfile(out + "fd.bin", dtype="<f8")
+        assert np.array_equal(fd, py[("fd", "nan", "8", "nan")]), name
+        assert rel_l2(fd, ref["fd"] * CGS.m * CGS.m) < 1e-10, name
+        for c, k in zip("IQU", ("is", "qs", "us")):
+            got = np.fromfile(f"{out}sync_23.0_{c}.bin", dtype="<f8")
+            assert np.array_equal(got, py[("sync", "23.0", "8", c)]), (name, c)
+            assert rel_l2(got, ref[k]) < 1e-10, (name, c)

@@ -1,0 +1,182 @@
+"""The oracle's helix and JF12 regular fields (breg_helix.cc:13-30,
+breg_jf12.cc:8-189) pinned by what the geometry and the published model fix,
+without the reference build: known answers of the helix, zero divergence of
+JF12 inside its spiral regions, zero net radial flux of its disk, and closed
+forms of the disk's vertical profile, the halo's transition radius and the
+X-field's elevation angles."""
+import math
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import Param, _capi as capi
+from hammurabix_amd.param import JF12_PARAMS
+from hammurabix_amd.units import CGS
+from oracle_lib import ptr
+
+KPC = CGS.kpc
+MU = 1e-6
+
+
+def field(orc, model, p, pos):
+    b = np.zeros(3)
+    par = np.zeros(capi.HAMX_BREG_NPARAM)
+    par[:len(p)] = p
+    orc.orc_breg(model, ptr(par), ptr(np.asarray(pos, dtype=np.float64)), ptr(b))
+    return b
+
+
+def jf12_params(**kw):
+    """The template's JF12 values (templates/params_template.xml jf12 block)
+    in HAMX_BREG_JF12 order and CGS units; keyword overrides in the same
+    units."""
+    v = dict(b_arm1=0.1 * MU, b_arm2=3.0 * MU, b_arm3=-0.9 * MU, b_arm4=-0.8 * MU,
+             b_arm5=-2.0 * MU, b_arm6=-4.2 * MU, b_arm7=0.0 * MU, b_ring=0.1 * MU,
+             h_disk=0.40 * KPC, w_disk=0.27 * KPC, Bn=1.4 * MU, Bs=-1.1 * MU,
+             rn=9.22 * KPC, rs=16.7 * KPC, wh=0.20 * KPC, z0=5.3 * KPC, B0_X=4.6 * MU,
+             Xtheta=49.0 * CGS.rad, rpc_X=4.8 * KPC, r0_X=2.9 * KPC)
+    v.update(kw)
+    return np.array([v[k] for k in JF12_PARAMS])
+
+
+def test_helix_known_answers(orc):
+    p = [1.0 * MU, 2.0 * MU, 0.5 * MU, 3 * KPC, 5 * KPC]
+    r = 4 * KPC
+    # phi' = atan2(y, x) + pi/2: +x axis -> (0, by, bz); +y -> (-bx, 0, bz);
+    # -x -> (0, -by, bz)
+    for pos, expect in (((r, 0, 0.3 * KPC), (0, 2 * MU, 0.5 * MU)),
+                        ((0, r, 0), (-1 * MU, 0, 0.5 * MU)),
+                        ((-r, 0, -KPC), (0, -2 * MU, 0.5 * MU))):
+        b = field(orc, capi.HAMX_BREG_HELIX, p, pos)
+        np.testing.assert_allclose(b, expect, atol=1e-21)
+    # strictly inside r_min < r < r_max
+    for rr in (3 * KPC, 5 * KPC, 2 * KPC, 6 * KPC):
+        assert np.all(field(orc, capi.HAMX_BREG_HELIX, p, (rr, 0, 0)) == 0)
+
+
+def _arm_point(rng, p):
+    """A point inside a spiral region (away from its boundaries, the ring
+    boundary and the X-field's rc_X surface) with the whole 1e-3 kpc
+    stencil in the same region."""
+    rc = np.array([5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5])
+    k = -1 / math.tan(math.pi / 180 * (90 - 11.5))
+    while True:
+        i = rng.integers(0, 7)
+        r_negx = rng.uniform(rc[i] + 0.2, rc[i + 1] - 0.2)
+        r = rng.uniform(5.6, 19.0)
+        phi = math.pi + math.log(r_negx / r) / k
+        if not -math.pi < phi <= math.pi:
+            continue
+        z = rng.choice([-1, 1]) * rng.uniform(0.3, 3.0)
+        rc_x = p[18] / KPC + abs(z) / math.tan(p[17])
+        if abs(r - rc_x) < 0.2 or math.sqrt(r * r + z * z) < 1.5:
+            continue
+        return np.array([r * math.cos(phi), r * math.sin(phi), z]) * KPC
+
+
+def test_jf12_divergence_free_inside_regions(orc):
+    """Disk (B_r r constant along a spiral region, boundaries parallel to the
+    field), toroidal halo and X-field are each divergence-free; a wrong sign
+    or term in any of them shows as div B of order |B| / length."""
+    p = jf12_params()
+    rng = np.random.default_rng(5)
+    h = 1e-3 * KPC
+    for _ in range(60):
+        x = _arm_point(rng, p)
+        div, scale = 0.0, 0.0
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            bp = field(orc, capi.HAMX_BREG_JF12, p, x + e)
+            bm = field(orc, capi.HAMX_BREG_JF12, p, x - e)
+            div += (bp[a] - bm[a]) / (2 * h)
+            scale += np.abs(bp - bm).max() / (2 * h)
+        b = np.linalg.norm(field(orc, capi.HAMX_BREG_JF12, p, x))
+        # |div| * kpc / |B| is O(1) for a broken component, <1e-5 here
+        assert abs(div) * KPC < 1e-5 * max(b, 1e-9 * MU), (x / KPC, div, scale, b)
+
+
+def test_jf12_disk_zero_net_radial_flux(orc):
+    """The 8th arm's strength closes the disk's radial flux through any
+    circle (breg_jf12.cc:94-99)."""
+    p = jf12_params(Bn=0.0, Bs=0.0, B0_X=0.0)
+    n = 40000
+    phi = (np.arange(n) + 0.5) * 2 * math.pi / n
+    for r_kpc in (6.0, 9.0, 12.0, 17.0):
+        r = r_kpc * KPC
+        br = np.empty(n)
+        for j, ph in enumerate(phi):
+            b = field(orc, capi.HAMX_BREG_JF12, p, (r * math.cos(ph), r * math.sin(ph), 0.1 * KPC))
+            br[j] = b[0] * math.cos(ph) + b[1] * math.sin(ph)
+        assert abs(br.sum()) < 2e-3 * np.abs(br).sum(), r_kpc
+
+
+def test_jf12_closed_forms(orc):
+    base = jf12_params()
+    # disk alone at z = h_disk: the logistic profile is 1/2 there, |B| =
+    # |b_arm| (5 kpc / r) / 2 in a spiral region, b_ring (5 kpc / r) / 2 in the ring
+    p = jf12_params(Bn=0.0, Bs=0.0, B0_X=0.0)
+    rng = np.random.default_rng(2)
+    arm_b = set(abs(v) for v in p[:7])
+    for _ in range(20):
+        x = _arm_point(rng, p)
+        x[2] = p[8]
+        r = math.hypot(x[0], x[1])
+        b = np.linalg.norm(field(orc, capi.HAMX_BREG_JF12, p, x))
+        mags = [v * (5 * KPC / r) / 2 for v in arm_b] + [abs(sum(
+            -f * v for f, v in zip((0.130, 0.165, 0.094, 0.122, 0.13, 0.118, 0.084), p[:7]))
+            / 0.156) * (5 * KPC / r) / 2]
+        assert min(abs(b - m) for m in mags) < 1e-12 * MU, (x / KPC, b)
+        if b < 1e-3 * MU:
+            continue  # a zero-strength arm
+        # the field follows the 11.5 degree spiral
+        er = x[:2] / r
+        bb = field(orc, capi.HAMX_BREG_JF12, p, x)
+        ephi = np.array([-er[1], er[0]])
+        ang = math.atan2(bb[:2] @ er, bb[:2] @ ephi)
+        assert min(abs(ang - math.radians(11.5)), abs(abs(ang) - math.pi + math.radians(11.5))) < 1e-9
+    b = field(orc, capi.HAMX_BREG_JF12, p, (4 * KPC, 0, p[8]))
+    np.testing.assert_allclose(np.linalg.norm(b), p[7] * (5 / 4) / 2, rtol=1e-12)
+    # halo alone at its transition radius: half the amplitude, exp(-|z|/z0),
+    # times the logistic profile; north uses (Bn, rn), south (Bs, rs)
+    ph = jf12_params(**{f"b_arm{i}": 0.0 for i in range(1, 8)}, b_ring=0.0, B0_X=0.0)
+    for z, amp, rh in ((1.3 * KPC, ph[10], ph[12]), (-0.9 * KPC, ph[11], ph[13])):
+        lz = 1 / (1 + math.exp(-2 / ph[9] * (abs(z) - ph[8])))
+        b = field(orc, capi.HAMX_BREG_JF12, ph, (0, rh, z))  # phi = 90 deg
+        np.testing.assert_allclose(-b[0], amp / 2 * math.exp(-abs(z) / ph[15]) * lz, rtol=1e-12)
+        assert abs(b[1]) < 1e-12 * abs(amp) and abs(b[2]) < 1e-30
+    # X-field alone: vertical in the mid-plane inside rpc_X with B0_X e^{-r/r0_X};
+    # outside rc_X the elevation angle is Xtheta, pointing outwards above the plane
+    # and inwards below
+    px = jf12_params(**{f"b_arm{i}": 0.0 for i in range(1, 8)}, b_ring=0.0, Bn=0.0, Bs=0.0)
+    b = field(orc, capi.HAMX_BREG_JF12, px, (3 * KPC, 0, 0))
+    np.testing.assert_allclose(b[2], px[16] * math.exp(-3 / 2.9), rtol=1e-12)
+    assert abs(b[0]) < 1e-15 * px[16]
+    for z in (0.5 * KPC, -0.7 * KPC):
+        b = field(orc, capi.HAMX_BREG_JF12, px, (14 * KPC, 0, z))
+        assert b[2] > 0
+        np.testing.assert_allclose(math.atan2(b[2], math.copysign(1, z) * b[0]), px[17], rtol=1e-12)
+    # outside the model's volume every component is 1e-12 G
+    for pos in ((21 * KPC, 0, 0), (0.5 * KPC, 0, 0.2 * KPC)):
+        assert np.all(field(orc, capi.HAMX_BREG_JF12, base, pos) == 1e-12)
+
+
+def test_param_maps_helix_and_jf12():
+    """The Param mirror reads the reference's XML blocks into the
+    HAMX_BREG_* layouts (param.cc:258-322 units)."""
+    from hammurabix_amd import configs
+    text = configs.plain_forward()
+    helix = ('<helix><bx value="1.0"/><by value="2.0"/><bz value="0.5"/>'
+             '<r_min value="3.0"/><r_max value="5.0"/></helix>')
+    t = text.replace('type="lsa"', 'type="helix"').replace('<lsa>', helix + '<lsa>')
+    d = Param(text=t).los_desc()
+    assert d.breg_model == capi.HAMX_BREG_HELIX
+    np.testing.assert_allclose(list(d.breg[:5]), [1e-6, 2e-6, 0.5e-6, 3 * KPC, 5 * KPC])
+    vals = dict(b_arm1=0.1, b_arm2=3.0, b_arm3=-0.9, b_arm4=-0.8, b_arm5=-2.0, b_arm6=-4.2,
+                b_arm7=0.0, b_ring=0.1, h_disk=0.4, w_disk=0.27, Bn=1.4, Bs=-1.1, rn=9.22,
+                rs=16.7, wh=0.2, z0=5.3, B0_X=4.6, Xtheta=49.0, rpc_X=4.8, r0_X=2.9)
+    block = "<jf12>" + "".join(f'<{k} value="{v}"/>' for k, v in vals.items()) + "</jf12>"
+    t = text.replace('type="lsa"', 'type="jf12"').replace('<lsa>', block + '<lsa>')
+    d = Param(text=t).los_desc()
+    assert d.breg_model == capi.HAMX_BREG_JF12
+    np.testing.assert_allclose(list(d.breg[:20]), jf12_params(), rtol=1e-15)
