@@ -25,8 +25,8 @@ HAMX_KEEP_TAPE = 8
 HAMX_USE_TAPE = 16
 
 (HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID,
- HAMX_BREG_HELIX, HAMX_BREG_JF12) = range(7)
-HAMX_BREG_NPARAM = 24
+ HAMX_BREG_HELIX, HAMX_BREG_JF12, HAMX_BREG_JAFFE) = range(8)
+HAMX_BREG_NPARAM = 40
 HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID = range(3)
 HAMX_CRE_NONE, HAMX_CRE_ANA, HAMX_CRE_UNIF = range(3)
 HAMX_SCHED_AUTO, HAMX_SCHED_THREAD, HAMX_SCHED_WAVE = range(3)

@@ -147,7 +147,7 @@ __device__ inline void lerp_cell_packed(const double4 *__restrict__ f,
 // ---------------------------------------------------------------- models --
 
 // Regular magnetic field models (breg_lsa.cc:9-26, breg_cart.cc, breg_unif.cc,
-// breg_helix.cc:13-30, breg_jf12.cc:8-189).
+// breg_helix.cc:13-30, breg_jf12.cc:8-189, breg_jaffe.cc:11-243).
 __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
   Vec3 b{0., 0., 0.};
   if (model == HAMX_BREG_LSA) {
@@ -172,12 +172,14 @@ __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
     b.x = p[0] * cos(p[2]);
     b.y = p[0] * sin(p[2]);
     b.z = p[1];
-  } else if (model == HAMX_BREG_HELIX || model == HAMX_BREG_JF12) {
+  } else if (model == HAMX_BREG_HELIX || model == HAMX_BREG_JF12 || model == HAMX_BREG_JAFFE) {
     double v[3];
     if (model == HAMX_BREG_HELIX)
       breg_helix_eval(p, pos.x, pos.y, pos.z, v);
-    else
+    else if (model == HAMX_BREG_JF12)
       breg_jf12_eval(p, pos.x, pos.y, pos.z, v);
+    else
+      breg_jaffe_eval(p, pos.x, pos.y, pos.z, v);
     b = Vec3{v[0], v[1], v[2]};
   }
   return b;

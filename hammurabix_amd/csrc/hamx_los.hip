@@ -468,7 +468,7 @@ int validate_los(hamx_ctx *ctx, const hamx_los_desc *d) {
                    d->nside_sync);
   if (d->do_sync && !(d->sync_freq > 0.))
     return set_err(ctx, HAMX_ERR_ARG, "sync frequency must be positive");
-  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JF12)
+  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JAFFE)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
   if (d->tereg_model < HAMX_TEREG_NONE || d->tereg_model > HAMX_TEREG_GRID)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported tereg model %d", d->tereg_model);

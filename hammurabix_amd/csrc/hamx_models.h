@@ -112,4 +112,117 @@ HAMX_HD inline void breg_jf12_eval(const double *p, double x, double y, double z
   b[2] = b_z;
 }
 
+// Breg_jaffe::write_field (breg_jaffe.cc:11-243, scalings bfield.h:202-224):
+// orientation (spiral arms at pitch arm_pitch, with a molecular ring or an
+// elliptical bar inside), radial and vertical scalings, and per-arm (or
+// ring / bar) compression with Gaussian cross-sections.  As in the reference,
+// a single compression factor (the ring / bar region, or a one-arm model)
+// takes the ring / bar amplitude.
+HAMX_HD inline void breg_jaffe_eval(const double *p, double x, double y, double z,
+                                    double b[3]) {
+  const bool quadruple = p[0] != 0., bss = p[1] != 0., ring = p[9] != 0., bar = p[10] != 0.;
+  const double r2 = x * x + y * y;
+  const double r = sqrt(x * x + y * y);
+  const double cos_p = cos(p[28]), sin_p = sin(p[28]);
+  const double r_lim = p[12], bar_lim = p[14] + 0.5 * p[30];
+  // orientation
+  double o[3] = {0., 0., 0.};
+  if (!(r < 0.5 * cgs::kpc)) {
+    const double qs = (z > p[3]) ? (1 - 2 * static_cast<double>(quadruple)) : 1.;
+    const double s_bss = 1 - 2 * static_cast<double>(bss);
+    if (ring) {
+      if (r > r_lim) {
+        o[0] = (cos_p * (y / r) - sin_p * (x / r)) * qs;
+        o[1] = (-cos_p * (x / r) - sin_p * (y / r)) * qs;
+      } else {
+        o[0] = s_bss * y / r;
+        o[1] = (2 * static_cast<double>(bss) - 1) * x / r;
+      }
+    } else if (bar) {
+      const double cf = cos(p[16]), sf = sin(p[16]);
+      const double xb = cf * x - sf * y, yb = sf * x + cf * y;
+      if (r > bar_lim) {
+        o[0] = (cos_p * (y / r) - sin_p * (x / r)) * qs;
+        o[1] = (-cos_p * (x / r) - sin_p * (y / r)) * qs;
+      } else if (yb != 0) {
+        const double nx = copysign(1., yb);
+        const double ny = -copysign(1., yb) * (xb / yb) * p[15] * p[15] / (p[14] * p[14]);
+        o[0] = (cf * nx + sf * ny) * s_bss;
+        o[1] = (-sf * nx + cf * ny) * s_bss;
+        const double l2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+        if (l2 != 0) {
+          const double inv = 1. / sqrt(l2);
+          o[0] = o[0] * inv;
+          o[1] = o[1] * inv;
+          o[2] = o[2] * inv;
+        }
+      } else {
+        o[0] = (2 * static_cast<double>(bss) - 1) * copysign(1., xb) * sf;
+        o[1] = (2 * static_cast<double>(bss) - 1) * copysign(1., xb) * cf;
+      }
+    }
+  }
+  // radial and vertical scalings
+  const double s1 = 1. - exp(-r2 / (p[6] * p[6]));
+  const double s2 = exp(-r2 / (p[7] * p[7]));
+  const double s3 = exp(-r2 * r2 / (p[8] * p[8] * p[8] * p[8]));
+  const double rs = s1 * (s2 + s3);
+  const double ch_d = cosh(z / p[3]), ch_h = cosh(z / p[5]), ch_a = cosh(z / p[19]);
+  const double ds = 1. / (ch_d * ch_d), hs = 1. / (ch_h * ch_h), zs = 1. / (ch_a * ch_a);
+  const double amp = p[2] * ds + p[4] * hs;
+  for (int c = 0; c < 3; ++c)
+    b[c] = o[c] * rs * amp;
+  // distances to the arms (or the ring / bar), then their compression
+  double dist[4];
+  int n = 0;
+  const int arm_num = static_cast<int>(p[17]);
+  if (ring || bar) {
+    const bool inner = ring ? (r < r_lim) : (r < bar_lim);
+    if (inner) {
+      if (ring) {
+        dist[0] = fabs(p[12] - r);
+      } else {
+        const double cf = cos(p[16]), sf = sin(p[16]);
+        const double cos_t = cf * x / r - sf * y / r;
+        const double sin_t = cf * y / r + sf * x / r;
+        dist[0] = fabs(p[14] * p[15] /
+                           sqrt(p[14] * p[14] * sin_t * sin_t + p[15] * p[15] * cos_t * cos_t) -
+                       r);
+      }
+      n = 1;
+    } else {
+      const double beta_inv = -sin_p / cos_p;
+      double theta = atan2(y, x);
+      if (theta < 0)
+        theta += 2 * cgs::pi;
+      for (int i = 0; i < arm_num && i < 4; ++i) {
+        const double d_ang = p[20 + i] - theta;
+        const double d0 = fabs(p[18] * exp(d_ang * beta_inv) - r);
+        const double dp = fabs(p[18] * exp((d_ang + 2 * cgs::pi) * beta_inv) - r);
+        const double dm = fabs(p[18] * exp((d_ang - 2 * cgs::pi) * beta_inv) - r);
+        const double m = d0 < dp ? d0 : dp;
+        dist[n++] = (m < dm ? m : dm) * cos_p;
+      }
+    }
+  }
+  if (n == 0)
+    return;
+  const double rc = sqrt(x * x + y * y) / p[31];
+  const double c0 = 1. / p[29] - 1.;
+  const double d0_inv = (rs * zs) / p[30];
+  const double factor = c0 * rs * zs;
+  const double inner_b = ring ? p[11] : (bar ? p[13] : 0.);
+  double cdrop = 1.;
+  if (rc > 1)
+    cdrop = pow(rc, -p[32]);
+  for (int i = 0; i < n; ++i) {
+    const double a = dist[i];
+    const double f = rc > 1 ? factor * cdrop * exp(-a * a * cdrop * cdrop * d0_inv * d0_inv)
+                            : factor * exp(-a * a * d0_inv * d0_inv);
+    const double amp_i = n == 1 ? inner_b : p[24 + i];
+    for (int c = 0; c < 3; ++c)
+      b[c] += o[c] * f * amp_i;
+  }
+}
+
 } // namespace hamx

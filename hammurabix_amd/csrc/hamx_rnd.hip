@@ -482,7 +482,7 @@ extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
   if (!d || !bx || !by || !bz)
     return set_err(ctx, HAMX_ERR_ARG, "hamx_brnd_es: null descriptor or output");
   HAMX_TRY(check_gen_box(ctx, d->box));
-  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JF12)
+  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JAFFE)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
   if (!(d->rho > 0.))
     return set_err(ctx, HAMX_ERR_ARG, "anisotropy ratio rho must be positive");

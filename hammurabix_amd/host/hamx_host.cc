@@ -371,6 +371,18 @@ Vec3 Breg_jf12::write_field(const Vec3 &pos, const Param *par) const {
   hamx::breg_jf12_eval(p, pos[0], pos[1], pos[2], b);
   return Vec3(b[0], b[1], b[2]);
 }
+// breg_jaffe.cc:11-243
+int Breg_jaffe::device_model(const Param *par, double *p) const {
+  for (int i = 0; i < 33; ++i)
+    p[i] = par->breg_jaffe[i];
+  return HAMX_BREG_JAFFE;
+}
+Vec3 Breg_jaffe::write_field(const Vec3 &pos, const Param *par) const {
+  double p[HAMX_BREG_NPARAM] = {}, b[3];
+  device_model(par, p);
+  hamx::breg_jaffe_eval(p, pos[0], pos[1], pos[2], b);
+  return Vec3(b[0], b[1], b[2]);
+}
 
 double TEreg::write_field(const Vec3 &, const Param *) const { return 0.; }
 int TEreg::device_model(const Param *, double *) const { return -1; }
@@ -722,6 +734,8 @@ void Pipeline::assemble_breg() {
     breg = std::make_unique<Breg_helix>();
   } else if (par->breg_type == "jf12") {
     breg = std::make_unique<Breg_jf12>();
+  } else if (par->breg_type == "jaffe") {
+    breg = std::make_unique<Breg_jaffe>();
   } else {
     throw std::runtime_error("unsupported breg model");
   }

@@ -72,6 +72,8 @@ public:
   // in the HAMX_BREG_JF12 parameter order (b_arm1..7, b_ring, h_disk, w_disk,
   // Bn, Bs, rn, rs, wh, z0, B0_X, Xtheta, rpc_X, r0_X)
   double breg_jf12[20] = {};
+  // in the HAMX_BREG_JAFFE parameter order (include/hamx.h)
+  double breg_jaffe[33] = {};
   unsigned long long brnd_seed = 0;
   struct { double rms = 0, k0 = 0, k1 = 0, a0 = 0, a1 = 0, rho = 0, r0 = 0, z0 = 0; } brnd_es;
   std::string tereg_type, ternd_type, ternd_method;
@@ -202,6 +204,11 @@ public:
   int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
 };
 class Breg_jf12 : public Breg {
+public:
+  Vec3 write_field(const Vec3 &pos, const Param *par) const override;
+  int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;
+};
+class Breg_jaffe : public Breg {
 public:
   Vec3 write_field(const Vec3 &pos, const Param *par) const override;
   int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const override;

@@ -183,7 +183,45 @@ Param::Param(const std::string &xml_file) {
       breg_jf12[17] = fval(s, "Xtheta") * units::rad;
       breg_jf12[18] = fval(s, "rpc_X") * units::kpc;
       breg_jf12[19] = fval(s, "r0_X") * units::kpc;
-    } else if (breg_type != "jaffe") {
+    } else if (breg_type == "jaffe") { // param.cc:173-257
+      const Element *s = trace(mf, {"regular", "jaffe"});
+      auto cue = [&](const char *k) { return trace(s, {k})->attr_bool("cue") ? 1. : 0.; };
+      double *q = breg_jaffe;
+      q[0] = cue("quadruple");
+      q[1] = cue("bss");
+      q[2] = fval(s, "disk_amp") * units::muGauss;
+      q[3] = fval(s, "disk_z0") * units::kpc;
+      q[4] = fval(s, "halo_amp") * units::muGauss;
+      q[5] = fval(s, "halo_z0") * units::kpc;
+      q[6] = fval(s, "r_inner") * units::kpc;
+      q[7] = fval(s, "r_scale") * units::kpc;
+      q[8] = fval(s, "r_peak") * units::kpc;
+      q[9] = cue("ring");
+      q[10] = cue("bar");
+      q[11] = fval(s, "ring_amp") * units::muGauss;
+      q[12] = fval(s, "ring_r") * units::kpc;
+      q[13] = fval(s, "bar_amp") * units::muGauss;
+      q[14] = fval(s, "bar_a") * units::kpc;
+      q[15] = fval(s, "bar_b") * units::kpc;
+      q[16] = fval(s, "bar_phi0") * units::rad;
+      const unsigned long long arms = uval(s, "arm_num");
+      if (arms > 4)
+        throw std::runtime_error("jaffe: at most 4 spiral arms (arm_phi1..4)");
+      q[17] = static_cast<double>(arms);
+      q[18] = fval(s, "arm_r0") * units::kpc;
+      q[19] = fval(s, "arm_z0") * units::kpc;
+      const char *phi[] = {"arm_phi1", "arm_phi2", "arm_phi3", "arm_phi4"};
+      const char *amp[] = {"arm_amp1", "arm_amp2", "arm_amp3", "arm_amp4"};
+      for (int i = 0; i < 4; ++i) {
+        q[20 + i] = fval(s, phi[i]) * units::rad;
+        q[24 + i] = fval(s, amp[i]) * units::muGauss;
+      }
+      q[28] = fval(s, "arm_pitch") * units::rad;
+      q[29] = fval(s, "comp_c");
+      q[30] = fval(s, "comp_d") * units::kpc;
+      q[31] = fval(s, "comp_r") * units::kpc;
+      q[32] = fval(s, "comp_p");
+    } else {
       throw std::runtime_error("unsupported breg model");
     }
   }

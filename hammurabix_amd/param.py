@@ -157,6 +157,14 @@ JF12_PARAMS = ("b_arm1", "b_arm2", "b_arm3", "b_arm4", "b_arm5", "b_arm6", "b_ar
                "rpc_X", "r0_X")
 
 
+# Parameter order of HAMX_BREG_JAFFE (include/hamx.h)
+JAFFE_PARAMS = ("quadruple", "bss", "disk_amp", "disk_z0", "halo_amp", "halo_z0", "r_inner",
+                "r_scale", "r_peak", "ring", "bar", "ring_amp", "ring_r", "bar_amp", "bar_a",
+                "bar_b", "bar_phi0", "arm_num", "arm_r0", "arm_z0", "arm_phi1", "arm_phi2",
+                "arm_phi3", "arm_phi4", "arm_amp1", "arm_amp2", "arm_amp3", "arm_amp4",
+                "arm_pitch", "comp_c", "comp_d", "comp_r", "comp_p")
+
+
 class Param:
     """Parsed parameter set (param.cc:12-28)."""
 
@@ -310,8 +318,27 @@ class Param:
                 for k in ("Bn", "Bs", "B0_X"):
                     self.breg_jf12[k] = s.f("value", k) * CGS.muGauss
                 self.breg_jf12["Xtheta"] = s.f("value", "Xtheta") * CGS.rad
-            elif self.breg_type == "jaffe":
-                pass  # parsed by the reference; no device evaluator yet
+            elif self.breg_type == "jaffe":  # param.cc:173-257
+                s = _trace(root, reg + ["jaffe"])
+                q = {"quadruple": float(s.b("cue", "quadruple")),
+                     "bss": float(s.b("cue", "bss")),
+                     "ring": float(s.b("cue", "ring")), "bar": float(s.b("cue", "bar"))}
+                for k in ("disk_amp", "halo_amp", "ring_amp", "bar_amp"):
+                    q[k] = s.f("value", k) * CGS.muGauss
+                for k in ("disk_z0", "halo_z0", "r_inner", "r_scale", "r_peak", "ring_r",
+                          "bar_a", "bar_b", "arm_r0", "arm_z0", "comp_d", "comp_r"):
+                    q[k] = s.f("value", k) * CGS.kpc
+                q["bar_phi0"] = s.f("value", "bar_phi0") * CGS.rad
+                q["arm_num"] = float(s.u("value", "arm_num"))
+                if q["arm_num"] > 4:
+                    raise RuntimeError("jaffe: at most 4 spiral arms (arm_phi1..4)")
+                for i in range(1, 5):
+                    q[f"arm_phi{i}"] = s.f("value", f"arm_phi{i}") * CGS.rad
+                    q[f"arm_amp{i}"] = s.f("value", f"arm_amp{i}") * CGS.muGauss
+                q["arm_pitch"] = s.f("value", "arm_pitch") * CGS.rad
+                q["comp_c"] = s.f("value", "comp_c")
+                q["comp_p"] = s.f("value", "comp_p")
+                self.breg_jaffe = q
             else:
                 raise RuntimeError("unsupported breg model")
         if g.read_permission or g.write_permission:
@@ -469,6 +496,9 @@ class Param:
         elif self.breg_type == "jf12":
             d.breg_model = capi.HAMX_BREG_JF12
             d.breg[0:20] = [self.breg_jf12[k] for k in JF12_PARAMS]
+        elif self.breg_type == "jaffe":
+            d.breg_model = capi.HAMX_BREG_JAFFE
+            d.breg[0:33] = [self.breg_jaffe[k] for k in JAFFE_PARAMS]
         else:
             raise RuntimeError(f"breg model '{self.breg_type}' has no device evaluator")
         # regular TE

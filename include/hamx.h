@@ -60,7 +60,15 @@ extern "C" {
                              [9] w_disk [cm], [10] Bn, [11] Bs [G], [12] rn, [13] rs,
                              [14] wh, [15] z0 [cm], [16] B0_X [G], [17] Xtheta [rad],
                              [18] rpc_X, [19] r0_X [cm] (breg_jf12.cc:8-189) */
-#define HAMX_BREG_NPARAM 24 /* parameters per regular-field model */
+#define HAMX_BREG_JAFFE 7 /* "jaffe" (breg_jaffe.cc:11-243): breg[0] quadruple, [1] bss,
+                             [2] disk_amp [G], [3] disk_z0 [cm], [4] halo_amp [G],
+                             [5] halo_z0, [6] r_inner, [7] r_scale, [8] r_peak [cm],
+                             [9] ring, [10] bar (0 / 1), [11] ring_amp [G], [12] ring_r [cm],
+                             [13] bar_amp [G], [14] bar_a, [15] bar_b [cm], [16] bar_phi0
+                             [rad], [17] arm_num (<= 4), [18] arm_r0, [19] arm_z0 [cm],
+                             [20..23] arm_phi0 [rad], [24..27] arm_amp [G], [28] arm_pitch
+                             [rad], [29] comp_c, [30] comp_d, [31] comp_r [cm], [32] comp_p */
+#define HAMX_BREG_NPARAM 40 /* parameters per regular-field model */
 /* regular thermal electrons, Param::tereg_type (param.cc:462-667) */
 #define HAMX_TEREG_NONE 0 /* zero density */
 #define HAMX_TEREG_UNIF 1 /* "unif": tereg[0..1] = n0 [cm^-3], r0 [cm] */

@@ -9,6 +9,7 @@
 // status of each part (what pins it) is listed in oracle/README.md.
 #include "hamx_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -476,8 +477,145 @@ V3 breg_jf12(const double *p, const V3 &pos) {
             cyl[0] * std::sin(phi) + cyl[1] * std::cos(phi), cyl[2]);
 }
 
+// Breg_jaffe (breg_jaffe.cc:11-243, bfield.h:202-224), restated part by part:
+// orientation, radial scaling, height scalings, distance to each arm (or the
+// ring / bar), compression, and the sum of write_field.
+struct Jaffe {
+  const double *p;
+  bool quadruple() const { return p[0] != 0.; }
+  bool bss() const { return p[1] != 0.; }
+  bool ring() const { return p[9] != 0.; }
+  bool bar() const { return p[10] != 0.; }
+  V3 orientation(const V3 &pos) const {
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    const double r_lim = p[12], bar_lim = p[14] + 0.5 * p[30];
+    const double cos_p = std::cos(p[28]), sin_p = std::sin(p[28]);
+    V3 t = v3(0., 0., 0.);
+    double quad = 1;
+    if (r < 0.5 * kKpc)
+      return t;
+    if (pos[2] > p[3])
+      quad = (1 - 2 * static_cast<double>(quadruple()));
+    if (ring()) {
+      if (r > r_lim) {
+        t[0] = (cos_p * (pos[1] / r) - sin_p * (pos[0] / r)) * quad;
+        t[1] = (-cos_p * (pos[0] / r) - sin_p * (pos[1] / r)) * quad;
+      } else {
+        t[0] = (1 - 2 * static_cast<double>(bss())) * pos[1] / r;
+        t[1] = (2 * static_cast<double>(bss()) - 1) * pos[0] / r;
+      }
+    } else if (bar()) {
+      const double cos_phi = std::cos(p[16]), sin_phi = std::sin(p[16]);
+      const double x = cos_phi * pos[0] - sin_phi * pos[1];
+      const double y = sin_phi * pos[0] + cos_phi * pos[1];
+      if (r > bar_lim) {
+        t[0] = (cos_p * (pos[1] / r) - sin_p * (pos[0] / r)) * quad;
+        t[1] = (-cos_p * (pos[0] / r) - sin_p * (pos[1] / r)) * quad;
+      } else if (y != 0) {
+        const double new_x = std::copysign(1, y);
+        const double new_y = -std::copysign(1, y) * (x / y) * p[15] * p[15] / (p[14] * p[14]);
+        t[0] = (cos_phi * new_x + sin_phi * new_y) * (1 - 2 * static_cast<double>(bss()));
+        t[1] = (-sin_phi * new_x + cos_phi * new_y) * (1 - 2 * static_cast<double>(bss()));
+        const double l2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        if (l2 != 0) {
+          const double den = 1. / std::sqrt(l2);
+          for (int i = 0; i < 3; ++i)
+            t[i] = t[i] * den;
+        }
+      } else {
+        t[0] = (2 * static_cast<double>(bss()) - 1) * std::copysign(1, x) * sin_phi;
+        t[1] = (2 * static_cast<double>(bss()) - 1) * std::copysign(1, x) * cos_phi;
+      }
+    }
+    return t;
+  }
+  double radial_scaling(const V3 &pos) const {
+    const double r2 = pos[0] * pos[0] + pos[1] * pos[1];
+    const double s1 = 1. - std::exp(-r2 / (p[6] * p[6]));
+    const double s2 = std::exp(-r2 / (p[7] * p[7]));
+    const double s3 = std::exp(-r2 * r2 / (p[8] * p[8] * p[8] * p[8]));
+    return s1 * (s2 + s3);
+  }
+  double sech2(double z, double z0) const {
+    return 1. / (std::cosh(z / z0) * std::cosh(z / z0));
+  }
+  std::vector<double> dist2arm(const V3 &pos) const {
+    std::vector<double> d;
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+    const double r_lim = p[12], bar_lim = p[14] + 0.5 * p[30];
+    const double cos_p = std::cos(p[28]), sin_p = std::sin(p[28]);
+    const double beta_inv = -sin_p / cos_p;
+    double theta = std::atan2(pos[1], pos[0]);
+    if (theta < 0)
+      theta += 2 * M_PI;
+    auto arms = [&]() {
+      for (int i = 0; i < static_cast<int>(p[17]) && i < 4; ++i) {
+        const double d_ang = p[20 + i] - theta;
+        const double d_rad = std::fabs(p[18] * std::exp(d_ang * beta_inv) - r);
+        const double d_rad_p = std::fabs(p[18] * std::exp((d_ang + 2 * M_PI) * beta_inv) - r);
+        const double d_rad_m = std::fabs(p[18] * std::exp((d_ang - 2 * M_PI) * beta_inv) - r);
+        d.push_back(std::min(std::min(d_rad, d_rad_p), d_rad_m) * cos_p);
+      }
+    };
+    if (ring()) {
+      if (r < r_lim)
+        d.push_back(std::fabs(p[12] - r));
+      else
+        arms();
+    } else if (bar()) {
+      const double cos_tmp = std::cos(p[16]) * pos[0] / r - std::sin(p[16]) * pos[1] / r;
+      const double sin_tmp = std::cos(p[16]) * pos[1] / r + std::sin(p[16]) * pos[0] / r;
+      if (r < bar_lim)
+        d.push_back(std::fabs(p[14] * p[15] /
+                                  std::sqrt(p[14] * p[14] * sin_tmp * sin_tmp +
+                                            p[15] * p[15] * cos_tmp * cos_tmp) -
+                              r));
+      else
+        arms();
+    }
+    return d;
+  }
+  std::vector<double> arm_compress(const V3 &pos) const {
+    const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]) / p[31];
+    const double c0 = 1. / p[29] - 1.;
+    std::vector<double> a0 = dist2arm(pos);
+    const double r_scaling = radial_scaling(pos);
+    const double z_scaling = sech2(pos[2], p[19]);
+    const double d0_inv = (r_scaling * z_scaling) / p[30];
+    const double factor = c0 * r_scaling * z_scaling;
+    if (r > 1) {
+      const double cdrop = std::pow(r, -p[32]);
+      for (auto &a : a0)
+        a = factor * cdrop * std::exp(-a * a * cdrop * cdrop * d0_inv * d0_inv);
+    } else {
+      for (auto &a : a0)
+        a = factor * std::exp(-a * a * d0_inv * d0_inv);
+    }
+    return a0;
+  }
+  V3 field(const V3 &pos) const {
+    const double inner_b = ring() ? p[11] : (bar() ? p[13] : 0.);
+    const V3 bhat = orientation(pos);
+    const double rs = radial_scaling(pos);
+    const double h = p[2] * sech2(pos[2], p[3]) + p[4] * sech2(pos[2], p[5]);
+    V3 bt;
+    for (int i = 0; i < 3; ++i)
+      bt[i] = bhat[i] * rs * h;
+    const std::vector<double> arm = arm_compress(pos);
+    if (arm.size() == 1) {
+      for (int i = 0; i < 3; ++i)
+        bt[i] += bhat[i] * arm[0] * inner_b;
+    } else {
+      for (size_t j = 0; j < arm.size(); ++j)
+        for (int i = 0; i < 3; ++i)
+          bt[i] += bhat[i] * arm[j] * p[24 + j];
+    }
+    return bt;
+  }
+};
+
 // Breg_lsa::write_field (breg_lsa.cc:9-26), Breg_cart, Breg_unif,
-// Breg_helix (breg_helix.cc:13-30), Breg_jf12
+// Breg_helix (breg_helix.cc:13-30), Breg_jf12, Breg_jaffe
 V3 breg_model(int model, const double *p, const V3 &pos) {
   if (model == HAMX_BREG_LSA) {
     const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
@@ -506,6 +644,8 @@ V3 breg_model(int model, const double *p, const V3 &pos) {
   }
   if (model == HAMX_BREG_JF12)
     return breg_jf12(p, pos);
+  if (model == HAMX_BREG_JAFFE)
+    return Jaffe{p}.field(pos);
   return v3(0., 0., 0.);
 }
 

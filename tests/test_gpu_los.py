@@ -131,21 +131,27 @@ def test_cart_unif_models_with_cre_unif(ctx, model):
     assert_parity(gpu_maps(ctx, d), write_grid(d), label=f"model{model}")
 
 
-@pytest.mark.parametrize("model", [capi.HAMX_BREG_HELIX, capi.HAMX_BREG_JF12])
-def test_helix_and_jf12_models(ctx, model):
-    """The helix and JF12 regular fields on the device against the oracle's
-    restatements (breg_helix.cc:13-30, breg_jf12.cc:8-189), with random
-    grids added, at the template's parameters."""
-    from test_oracle_models import jf12_params
-    print(f"case: breg model {model}", flush=True)
+@pytest.mark.parametrize("model,variant", [(capi.HAMX_BREG_HELIX, ""), (capi.HAMX_BREG_JF12, ""),
+                                           (capi.HAMX_BREG_JAFFE, "bar"),
+                                           (capi.HAMX_BREG_JAFFE, "ring")])
+def test_helix_jf12_jaffe_models(ctx, model, variant):
+    """The helix, JF12 and Jaffe regular fields on the device against the
+    oracle's restatements (breg_helix.cc:13-30, breg_jf12.cc:8-189,
+    breg_jaffe.cc:11-243), with random grids added, at the template's
+    parameters (Jaffe also with the ring, quadruple and bss)."""
+    from test_oracle_models import jaffe_params, jf12_params
+    print(f"case: breg model {model} {variant}", flush=True)
     par = Param(text=configs.random_forward(nside_sim=4, nside_out=8, oc_r_res=0.05,
                                             grid=(16, 16, 8)))
     d = par.los_desc()
     d.breg_model = model
     if model == capi.HAMX_BREG_HELIX:
         d.breg[0:5] = [1e-6, 2e-6, 0.5e-6, 3 * CGS.kpc, 12 * CGS.kpc]
-    else:
+    elif model == capi.HAMX_BREG_JF12:
         d.breg[0:20] = list(jf12_params())
+    else:
+        over = dict(ring=1.0, bar=0.0, quadruple=1.0, bss=1.0) if variant == "ring" else {}
+        d.breg[0:33] = list(jaffe_params(**over))
     b, te, box = random_grids((16, 16, 8), seed=12)
     kw = dict(brnd=b, brnd_box=box, ternd=te, ternd_box=box)
     assert_parity(gpu_maps(ctx, d, **kw), write_grid(d, **kw), label=f"model{model}")

@@ -102,8 +102,8 @@ def test_cli_and_pipeline_with_mask(tmp_path, ctx):
 
 
 @pytest.mark.gpu
-def test_cli_jf12_and_helix_match_python_pipeline_and_oracle(tmp_path, ctx):
-    """The C++ host reads the reference's <jf12> / <helix> blocks and runs them
+def test_cli_jf12_helix_jaffe_match_python_pipeline_and_oracle(tmp_path, ctx):
+    """The C++ host reads the reference's <jf12> / <helix> / <jaffe> blocks and runs them
     through the device evaluators: its maps equal the Python pipeline's and
     the oracle's (no random fields)."""
     need_cli()
@@ -115,7 +115,18 @@ def test_cli_jf12_and_helix_match_python_pipeline_and_oracle(tmp_path, ctx):
             '<Xtheta value="49"/><rpc_X value="4.8"/><r0_X value="2.9"/></jf12>')
     helix = ('<helix><bx value="1.0"/><by value="2.0"/><bz value="0.5"/>'
              '<r_min value="3.0"/><r_max value="12.0"/></helix>')
-    for name, block in (("jf12", jf12), ("helix", helix)):
+    jaffe = ('<jaffe><quadruple cue="0"/><bss cue="0"/><disk_amp value="0.167"/>'
+             '<disk_z0 value="0.1"/><halo_amp value="1.38"/><halo_z0 value="3.0"/>'
+             '<r_inner value="0.5"/><r_scale value="20"/><r_peak value="0"/><ring cue="0"/>'
+             '<bar cue="1"/><ring_amp value="0.023"/><ring_r value="5.0"/>'
+             '<bar_amp value="0.023"/><bar_a value="5.0"/><bar_b value="3.0"/>'
+             '<bar_phi0 value="45.0"/><arm_num value="4"/><arm_r0 value="7.1"/>'
+             '<arm_z0 value="0.1"/><arm_phi1 value="70"/><arm_phi2 value="160"/>'
+             '<arm_phi3 value="250"/><arm_phi4 value="340"/><arm_amp1 value="2"/>'
+             '<arm_amp2 value="0.133"/><arm_amp3 value="-3.78"/><arm_amp4 value="0.32"/>'
+             '<arm_pitch value="11.5"/><comp_c value="0.5"/><comp_d value="0.3"/>'
+             '<comp_r value="12"/><comp_p value="3"/></jaffe>')
+    for name, block in (("jf12", jf12), ("helix", helix), ("jaffe", jaffe)):
         out = str(tmp_path / name) + "_"
         text = configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05, out=out)
         text = text.replace('type="lsa"', f'type="{name}"').replace('<lsa>', block + '<lsa>')

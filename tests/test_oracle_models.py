@@ -180,3 +180,144 @@ def test_param_maps_helix_and_jf12():
     d = Param(text=t).los_desc()
     assert d.breg_model == capi.HAMX_BREG_JF12
     np.testing.assert_allclose(list(d.breg[:20]), jf12_params(), rtol=1e-15)
+    jv = dict(disk_amp=0.167, disk_z0=0.1, halo_amp=1.38, halo_z0=3.0, r_inner=0.5, r_scale=20,
+              r_peak=0, ring_amp=0.023, ring_r=5.0, bar_amp=0.023, bar_a=5.0, bar_b=3.0,
+              bar_phi0=45.0, arm_num=4, arm_r0=7.1, arm_z0=0.1, arm_phi1=70, arm_phi2=160,
+              arm_phi3=250, arm_phi4=340, arm_amp1=2, arm_amp2=0.133, arm_amp3=-3.78,
+              arm_amp4=0.32, arm_pitch=11.5, comp_c=0.5, comp_d=0.3, comp_r=12, comp_p=3)
+    block = ('<jaffe><quadruple cue="0"/><bss cue="0"/><ring cue="0"/><bar cue="1"/>' +
+             "".join(f'<{k} value="{v}"/>' for k, v in jv.items()) + "</jaffe>")
+    t = text.replace('type="lsa"', 'type="jaffe"').replace('<lsa>', block + '<lsa>')
+    d = Param(text=t).los_desc()
+    assert d.breg_model == capi.HAMX_BREG_JAFFE
+    np.testing.assert_allclose(list(d.breg[:33]), jaffe_params(), rtol=1e-15)
+
+
+def jaffe_params(**kw):
+    """The template's Jaffe values (templates/params_template.xml jaffe
+    block: bar on, ring off) in HAMX_BREG_JAFFE order and CGS units."""
+    from hammurabix_amd.param import JAFFE_PARAMS
+    v = dict(quadruple=0.0, bss=0.0, disk_amp=0.167 * MU, disk_z0=0.1 * KPC, halo_amp=1.38 * MU,
+             halo_z0=3.0 * KPC, r_inner=0.5 * KPC, r_scale=20 * KPC, r_peak=0.0, ring=0.0,
+             bar=1.0, ring_amp=0.023 * MU, ring_r=5.0 * KPC, bar_amp=0.023 * MU, bar_a=5.0 * KPC,
+             bar_b=3.0 * KPC, bar_phi0=45.0 * CGS.rad, arm_num=4.0, arm_r0=7.1 * KPC,
+             arm_z0=0.1 * KPC, arm_phi1=70 * CGS.rad, arm_phi2=160 * CGS.rad,
+             arm_phi3=250 * CGS.rad, arm_phi4=340 * CGS.rad, arm_amp1=2 * MU,
+             arm_amp2=0.133 * MU, arm_amp3=-3.78 * MU, arm_amp4=0.32 * MU,
+             arm_pitch=11.5 * CGS.rad, comp_c=0.5, comp_d=0.3 * KPC, comp_r=12 * KPC, comp_p=3.0)
+    v.update(kw)
+    return np.array([v[k] for k in JAFFE_PARAMS])
+
+
+def _rs(p, r):
+    """radial_scaling (breg_jaffe.cc:104-117) for r_peak = 0 (s3 = 0)."""
+    return (1 - math.exp(-(r / p[6]) ** 2)) * math.exp(-(r / p[7]) ** 2)
+
+
+def _sech2(z, z0):
+    return 1 / math.cosh(z / z0) ** 2
+
+
+def test_jaffe_orientations(orc):
+    """Spiral field -sin(p) r^ - cos(p) phi^ (flipped above disk_z0 with
+    quadruple), ring field -(1 - 2 bss) phi^, bar field tangent to the bar's
+    ellipses, nothing inside 0.5 kpc."""
+    pitch = 11.5 * CGS.rad
+    nocomp = dict(halo_amp=0.0, bar_amp=0.0, ring_amp=0.0, arm_amp1=0.0, arm_amp2=0.0,
+                  arm_amp3=0.0, arm_amp4=0.0)
+    rng = np.random.default_rng(4)
+    for ring, quad in ((0.0, 0.0), (1.0, 1.0)):
+        p = jaffe_params(ring=ring, bar=1.0 - ring, quadruple=quad, **nocomp)
+        for _ in range(20):
+            r = rng.uniform(9.0, 15.0) * KPC
+            ph = rng.uniform(-math.pi, math.pi)
+            z = rng.uniform(-0.5, 0.5) * KPC
+            x = np.array([r * math.cos(ph), r * math.sin(ph), z])
+            b = field(orc, capi.HAMX_BREG_JAFFE, p, x)
+            er = np.array([math.cos(ph), math.sin(ph), 0])
+            ephi = np.array([-math.sin(ph), math.cos(ph), 0])
+            sign = -1.0 if (quad and z > p[3]) else 1.0
+            mag = _rs(p, r) * p[2] * _sech2(z, p[3])
+            np.testing.assert_allclose(b @ er, -sign * math.sin(pitch) * mag, rtol=1e-12, atol=1e-30)
+            np.testing.assert_allclose(b @ ephi, -sign * math.cos(pitch) * mag, rtol=1e-12)
+            assert b[2] == 0
+    for bss in (0.0, 1.0):
+        p = jaffe_params(ring=1.0, bar=0.0, bss=bss, **nocomp)
+        x = np.array([3.0 * KPC * math.cos(0.7), 3.0 * KPC * math.sin(0.7), 0.02 * KPC])
+        b = field(orc, capi.HAMX_BREG_JAFFE, p, x)
+        ephi = np.array([-math.sin(0.7), math.cos(0.7), 0])
+        np.testing.assert_allclose(b @ ephi / np.linalg.norm(b), -(1 - 2 * bss), rtol=1e-12)
+    p = jaffe_params(**nocomp)
+    phi0, a, bb = p[16], p[14], p[15]
+    for _ in range(20):
+        r = rng.uniform(0.8, 4.5) * KPC
+        ph = rng.uniform(-math.pi, math.pi)
+        x = np.array([r * math.cos(ph), r * math.sin(ph), 0.03 * KPC])
+        b = field(orc, capi.HAMX_BREG_JAFFE, p, x)
+        xr = math.cos(phi0) * x[0] - math.sin(phi0) * x[1]
+        yr = math.sin(phi0) * x[0] + math.cos(phi0) * x[1]
+        grad = np.array([2 * xr / a ** 2 * math.cos(phi0) + 2 * yr / bb ** 2 * math.sin(phi0),
+                         -2 * xr / a ** 2 * math.sin(phi0) + 2 * yr / bb ** 2 * math.cos(phi0), 0])
+        assert abs(b @ grad) < 1e-12 * np.linalg.norm(b) * np.linalg.norm(grad)
+        np.testing.assert_allclose(np.linalg.norm(b), _rs(p, r) * p[2] * _sech2(x[2], p[3]),
+                                   rtol=1e-12)
+    assert np.all(field(orc, capi.HAMX_BREG_JAFFE, jaffe_params(), (0.3 * KPC, 0.2 * KPC, 0)) == 0)
+
+
+def test_jaffe_arm_ridges_and_compression(orc):
+    """On an arm's ridge r = arm_r0 exp((phi0_i - theta) beta_inv) (beta_inv
+    = -tan(pitch), any of the three windings) the compression factor is
+    (1/comp_c - 1) rs sech^2(z/arm_z0) (times (r/comp_r)^-comp_p beyond
+    comp_r), falling off as exp(-(dr cos(pitch) d0_inv)^2) across it; the
+    bar's edge gets the bar amplitude; a one-arm model uses the inner
+    amplitude in the spiral region (breg_jaffe.cc:26-35)."""
+    pitch = 11.5 * CGS.rad
+    base = dict(disk_amp=0.0, halo_amp=0.0, comp_d=0.02 * KPC)
+    binv = -math.tan(pitch)
+    z = 0.04 * KPC
+    # (arm, theta, winding, overrides): direct ridges, the -2 pi winding (beyond
+    # comp_r) and the +2 pi winding (it needs an arm angle below 64 degrees)
+    cases = [(0, math.radians(50), 0, {}), (1, math.radians(110), 0, {}),
+             (3, math.radians(230), -1, {}), (0, 6.0, 1, dict(arm_phi1=30 * CGS.rad)),
+             (3, math.radians(300), 0, {})]
+    for i, theta, wind, over in cases:
+        p = jaffe_params(**base, **over)
+        if True:
+            r = p[18] * math.exp((p[20 + i] - theta + 2 * math.pi * wind) * binv)
+            assert p[14] + 0.5 * p[30] + 0.5 * KPC < r < 19 * KPC, (i, r / KPC)
+            rs = _rs(p, r)
+            zs = _sech2(z, p[19])
+            fac = (1 / p[29] - 1) * rs * zs
+            cd = (r / p[31]) ** -p[32] if r > p[31] else 1.0
+            x = np.array([r * math.cos(theta), r * math.sin(theta), z])
+            b = np.linalg.norm(field(orc, capi.HAMX_BREG_JAFFE, p, x))
+            np.testing.assert_allclose(b, abs(fac * cd * p[24 + i]), rtol=1e-9)
+            dr = 0.005 * KPC
+            x2 = x * (r + dr) / r
+            x2[2] = z
+            d0_inv = rs * zs / p[30]
+            rs2 = _rs(p, r + dr)
+            d0_inv2 = rs2 * zs / p[30]
+            cd2 = ((r + dr) / p[31]) ** -p[32] if r + dr > p[31] else 1.0
+            fac2 = (1 / p[29] - 1) * rs2 * zs
+            expect = abs(fac2 * cd2 * p[24 + i]) * math.exp(-(dr * math.cos(pitch) * cd2 * d0_inv2) ** 2)
+            np.testing.assert_allclose(np.linalg.norm(field(orc, capi.HAMX_BREG_JAFFE, p, x2)),
+                                       expect, rtol=1e-6)
+            assert d0_inv > 0
+    # the bar's edge: distance 0, amplitude bar_amp
+    p = jaffe_params(**base)
+    phi = 0.4
+    ang = phi + p[16]
+    r = p[14] * p[15] / math.sqrt(p[14] ** 2 * math.sin(ang) ** 2 + p[15] ** 2 * math.cos(ang) ** 2)
+    x = np.array([r * math.cos(phi), r * math.sin(phi), z])
+    np.testing.assert_allclose(np.linalg.norm(field(orc, capi.HAMX_BREG_JAFFE, p, x)),
+                               abs((1 / p[29] - 1) * _rs(p, r) * _sech2(z, p[19]) * p[13]),
+                               rtol=1e-9)
+    # one arm: the spiral region takes the ring amplitude
+    p = jaffe_params(ring=1.0, bar=0.0, arm_num=1.0, **base)
+    theta = math.radians(40)
+    r = p[18] * math.exp((p[20] - theta) * binv)
+    x = np.array([r * math.cos(theta), r * math.sin(theta), z])
+    np.testing.assert_allclose(np.linalg.norm(field(orc, capi.HAMX_BREG_JAFFE, p, x)),
+                               abs((1 / p[29] - 1) * _rs(p, r) * _sech2(z, p[19]) * p[11]),
+                               rtol=1e-9)
