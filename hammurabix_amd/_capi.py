@@ -81,7 +81,7 @@ class HamxEsDesc(C.Structure):
                 ("breg_model", C.c_int32), ("reserved", C.c_int32),
                 ("breg", C.c_double * HAMX_BREG_NPARAM),
                 ("breg_x", C.c_void_p), ("breg_y", C.c_void_p), ("breg_z", C.c_void_p),
-                ("breg_box", HamxBox)]
+                ("breg_box", HamxBox), ("jf12", C.c_double * 13)]
 
 
 class HamxDftDesc(C.Structure):
@@ -106,6 +106,8 @@ _PROTOS = {
     "hamx_los_samples": (C.c_int, [C.POINTER(HamxLosDesc), C.POINTER(C.c_uint64)]),
     "hamx_brnd_es": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "hamx_brnd_jf12": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hamx_ternd_dft": (C.c_int, [C.c_void_p, C.POINTER(HamxDftDesc), C.c_uint64,
                                  C.c_void_p, C.c_uint32]),
     "hamx_fft3d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64,

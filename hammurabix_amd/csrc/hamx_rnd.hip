@@ -204,6 +204,9 @@ struct EsArgs {
   Vec3 obs;
   double rms, k0, k1, a0, a1, rho, r0, z0;
   int breg_model;
+  int jf12; // Brnd_jf12: its energy profile and literal constants
+  double jp[13];
+  double third, twothirds, sqrt3; // 1/3, 2/3, sqrt(3) as the generator's source writes them
   double breg[HAMX_BREG_NPARAM];
   const double *bgx, *bgy, *bgz;
   GridGeo g_breg;
@@ -219,6 +222,38 @@ __device__ inline double es_spectrum(const EsArgs &a, double k) {
   const double P = band1 * pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.) +
                    band2 / pow(k / a.k0, a.a1) + band3 / pow(k / a.k0, a.a0);
   return P * p0 * unit;
+}
+
+// Brnd_jf12::spatial_profile (brnd_jf12.cc:54-114): the squared disk (JF12
+// spiral regions as in the regular field, b0_int inside 5 kpc, Gaussian in
+// z) and halo amplitudes, in muG^2; 0 outside r < 20 kpc, |pos| > 1 kpc.
+__device__ inline double jf12_profile(const double *p, const Vec3 &pos) {
+  const double kpc = cgs::kpc;
+  const double r = sqrt(pos.x * pos.x + pos.y * pos.y);
+  const double rho = sqrt(pos.x * pos.x + pos.y * pos.y + pos.z * pos.z);
+  const double z = pos.z;
+  const double phi = atan2(pos.y, pos.x);
+  if (r > 20. * kpc || rho < 1. * kpc)
+    return 0.0;
+  const double rc[8] = {5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5};
+  const double inc = 11.5;
+  double disk = 0.0 * cgs::mugauss;
+  if (r < 5. * kpc) {
+    disk = p[8];
+  } else {
+    const double k = -1 / tan(cgs::pi / 180. * (90 - inc));
+    double r_negx = r * exp(k * (phi - cgs::pi));
+    if (r_negx > rc[7] * kpc)
+      r_negx = r * exp(k * (phi + cgs::pi));
+    if (r_negx > rc[7] * kpc)
+      r_negx = r * exp(k * (phi + 3 * cgs::pi));
+    for (int i = 7; i >= 0; --i)
+      if (r_negx < rc[i] * kpc)
+        disk = p[i] * (5. * kpc) / r;
+  }
+  disk = disk * exp(-0.5 * z * z / (p[9] * p[9]));
+  const double halo = p[10] * exp(-fabs(r / p[11])) * exp(-fabs(z / p[12]));
+  return (disk * disk + halo * halo) / (cgs::mugauss * cgs::mugauss);
 }
 
 // STEP I (brnd_es.cc:92-140)
@@ -239,7 +274,7 @@ __global__ __launch_bounds__(256) void k_es_draw(double2 *c0, double2 *c1,
     const double ky = wavenum(j, a.b.ny, a.b.ly);
     const double kz = wavenum(l, a.b.nz, a.b.lz);
     const double ks = sqrt(kx * kx + ky * ky + kz * kz);
-    const double sigma = sqrt(cgs::onethird * es_spectrum(a, ks) * a.b.dk3);
+    const double sigma = sqrt(a.third * es_spectrum(a, ks) * a.b.dk3);
     double g0, g1, g2, g3;
     gauss_pair(seed, 0, 2 * idx, g0, g1);
     gauss_pair(seed, 0, 2 * idx + 1, g2, g3);
@@ -254,7 +289,7 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
   const double b_var_invsq = 1. / sqrt(3. * var[0]);
   const double rho = a.rho;
   const double rho2 = rho * rho;
-  const double rhonorm = 1. / sqrt(cgs::onethird * rho2 + cgs::twothirds / rho2);
+  const double rhonorm = 1. / sqrt(a.third * rho2 + a.twothirds / rho2);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < a.b.n; idx += stride) {
@@ -262,7 +297,8 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
     const uint64_t j = (idx / a.b.nz) % a.b.ny;
     const uint64_t l = idx % a.b.nz;
     const Vec3 pos = vertex_pos(a.b, i, j, l);
-    const double ratio = sqrt(profile(a.obs, a.r0, a.z0, pos)) * a.rms * b_var_invsq;
+    const double prof = a.jf12 ? jf12_profile(a.jp, pos) : profile(a.obs, a.r0, a.z0, pos);
+    const double ratio = sqrt(prof) * a.rms * b_var_invsq;
     const double2 v0 = c0[idx], v1 = c1[idx];
     Vec3 b{v0.x * ratio, v0.y * ratio, v1.y * ratio};
     // anisotropy direction: versor of the regular field (brnd_es.cc:19-22)
@@ -332,9 +368,8 @@ __global__ __launch_bounds__(256) void k_es_gs(const double2 *c0, const double2 
     if (!(k2 == 0 || b2 == 0)) {
       const double inv = 1. / k2;
       const double kb = dot0(k, b);
-      fr = Vec3{cgs::sqrthree * (b.x - k.x * kb * inv),
-                cgs::sqrthree * (b.y - k.y * kb * inv),
-                cgs::sqrthree * (b.z - k.z * kb * inv)};
+      fr = Vec3{a.sqrt3 * (b.x - k.x * kb * inv), a.sqrt3 * (b.y - k.y * kb * inv),
+                a.sqrt3 * (b.z - k.z * kb * inv)};
     }
     bx[idx] = cgs::sqrtwo * fr.x;
     by[idx] = cgs::sqrtwo * fr.y;
@@ -475,12 +510,14 @@ extern "C" int hamx_gauss_pairs(hamx_ctx *ctx, uint64_t seed, uint32_t stream,
   return HAMX_OK;
 }
 
-extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
-                            double *bx, double *by, double *bz, uint32_t flags) {
+namespace hamx {
+int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *bx,
+                  double *by, double *bz, uint32_t flags, bool jf12) {
   if (!ctx)
     return HAMX_ERR_ARG;
   if (!d || !bx || !by || !bz)
-    return set_err(ctx, HAMX_ERR_ARG, "hamx_brnd_es: null descriptor or output");
+    return set_err(ctx, HAMX_ERR_ARG, "%s: null descriptor or output",
+                   jf12 ? "hamx_brnd_jf12" : "hamx_brnd_es");
   HAMX_TRY(check_gen_box(ctx, d->box));
   if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JAFFE)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
@@ -503,6 +540,12 @@ extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
   a.z0 = d->z0;
   a.breg_model = d->breg_model;
   std::memcpy(a.breg, d->breg, sizeof(a.breg));
+  a.jf12 = jf12 ? 1 : 0;
+  std::memcpy(a.jp, d->jf12, sizeof(a.jp));
+  // the generators' own literals (brnd_es.cc:132,190,71; brnd_jf12.cc:214,240,119)
+  a.third = jf12 ? 0.33333333 : cgs::onethird;
+  a.twothirds = jf12 ? 0.66666667 : cgs::twothirds;
+  a.sqrt3 = jf12 ? 1.73205081 : cgs::sqrthree;
   if (d->breg_model == HAMX_BREG_GRID) {
     if (!(d->breg_x && d->breg_y && d->breg_z))
       return set_err(ctx, HAMX_ERR_ARG, "breg grid model without breg grid arrays");
@@ -563,6 +606,17 @@ extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
   }
   HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HAMX_OK;
+}
+} // namespace hamx
+
+extern "C" int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                            double *bx, double *by, double *bz, uint32_t flags) {
+  return brnd_generate(ctx, d, seed, bx, by, bz, flags, false);
+}
+
+extern "C" int hamx_brnd_jf12(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                              double *bx, double *by, double *bz, uint32_t flags) {
+  return brnd_generate(ctx, d, seed, bx, by, bz, flags, true);
 }
 
 extern "C" int hamx_ternd_dft(hamx_ctx *ctx, const hamx_dft_desc *d, uint64_t seed,

@@ -479,11 +479,22 @@ void Brnd_es::write_grid(const Param *par, const Breg *breg, const Grid_breg *gb
     d.breg_z = gbreg->bz.get();
     d.breg_box = to_box(par->grid_breg);
   }
+  for (int i = 0; i < 13; ++i)
+    d.jf12[i] = par->brnd_jf12[i];
   hamx_ctx *ctx = context();
-  check(hamx_brnd_es(ctx, &d, random_seed(par->brnd_seed), grid->bx.get(), grid->by.get(),
-                     grid->bz.get(), 0),
-        ctx);
+  check(generate(ctx, &d, random_seed(par->brnd_seed), grid), ctx);
   grid->invalidate_device();
+}
+
+int Brnd_es::generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                      Grid_brnd *grid) const {
+  return hamx_brnd_es(ctx, d, seed, grid->bx.get(), grid->by.get(), grid->bz.get(), 0);
+}
+
+// Brnd_jf12::write_grid (brnd_jf12.cc:127-385) on the device
+int Brnd_jf12::generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                        Grid_brnd *grid) const {
+  return hamx_brnd_jf12(ctx, d, seed, grid->bx.get(), grid->by.get(), grid->bz.get(), 0);
 }
 
 // TErnd_dft::write_grid (ternd_dft.cc:35-128); seeded by brnd_seed as in
@@ -771,6 +782,8 @@ void Pipeline::assemble_brnd() {
   } else if (par->grid_brnd.build_permission) {
     if (par->brnd_type == "global" && par->brnd_method == "es")
       brnd = std::make_unique<Brnd_es>();
+    else if (par->brnd_type == "global" && par->brnd_method == "jf12")
+      brnd = std::make_unique<Brnd_jf12>();
     else
       throw std::runtime_error("unsupported brnd model");
     brnd->write_grid(par.get(), breg.get(), grid_breg.get(), grid_brnd.get());

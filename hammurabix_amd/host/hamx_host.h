@@ -76,6 +76,9 @@ public:
   double breg_jaffe[33] = {};
   unsigned long long brnd_seed = 0;
   struct { double rms = 0, k0 = 0, k1 = 0, a0 = 0, a1 = 0, rho = 0, r0 = 0, z0 = 0; } brnd_es;
+  // Brnd_jf12: spectrum in brnd_es (r0, z0 unused); energy profile b0_1..b0_8,
+  // b0_int, z0_spiral, b0_halo, r0_halo, z0_halo (param.cc:375-409)
+  double brnd_jf12[13] = {};
   std::string tereg_type, ternd_type, ternd_method;
   struct { double n0 = 0, r0 = 0; } tereg_unif;
   unsigned long long ternd_seed = 0;
@@ -223,6 +226,15 @@ class Brnd_es : public Brnd {
 public:
   void write_grid(const Param *par, const Breg *breg, const Grid_breg *gbreg,
                   Grid_brnd *grid) const override;
+
+protected:
+  virtual int generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, Grid_brnd *grid) const;
+};
+// Brnd_jf12 (brnd_jf12.cc): Brnd_es with the JF12 energy profile
+class Brnd_jf12 : public Brnd_es {
+protected:
+  int generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+               Grid_brnd *grid) const override;
 };
 
 class TEreg {

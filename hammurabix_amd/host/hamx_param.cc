@@ -246,7 +246,21 @@ Param::Param(const std::string &xml_file) {
         brnd_es.rho = fval(s, "rho");
         brnd_es.r0 = fval(s, "r0") * units::kpc;
         brnd_es.z0 = fval(s, "z0") * units::kpc;
-      } else if (brnd_method != "jf12" && brnd_method != "jaffe") {
+      } else if (brnd_method == "jf12") {
+        const Element *s = trace(mf, {"random", "global", "jf12"});
+        brnd_es.rms = fval(s, "rms") * units::muGauss;
+        brnd_es.k0 = fval(s, "k0");
+        brnd_es.a0 = fval(s, "a0");
+        brnd_es.k1 = fval(s, "k1");
+        brnd_es.a1 = fval(s, "a1");
+        brnd_es.rho = fval(s, "rho");
+        const char *keys[13] = {"b0_1", "b0_2", "b0_3", "b0_4", "b0_5", "b0_6", "b0_7",
+                                "b0_8", "b0_int", "z0_spiral", "b0_halo", "r0_halo", "z0_halo"};
+        for (int i = 0; i < 13; ++i) {
+          const bool length = i == 9 || i == 11 || i == 12;
+          brnd_jf12[i] = fval(s, keys[i]) * (length ? units::kpc : units::muGauss);
+        }
+      } else if (brnd_method != "jaffe") {
         throw std::runtime_error("unsupported brnd model");
       }
     } else if (brnd_type == "local") {

@@ -165,6 +165,13 @@ JAFFE_PARAMS = ("quadruple", "bss", "disk_amp", "disk_z0", "halo_amp", "halo_z0"
                 "arm_pitch", "comp_c", "comp_d", "comp_r", "comp_p")
 
 
+# Energy profile of Brnd_jf12 in hamx_es_desc.jf12 order, with the unit each XML
+# value is given in (param.cc:375-409)
+BRND_JF12_PARAMS = tuple((f"b0_{i}", CGS.muGauss) for i in range(1, 9)) + (
+    ("b0_int", CGS.muGauss), ("z0_spiral", CGS.kpc), ("b0_halo", CGS.muGauss),
+    ("r0_halo", CGS.kpc), ("z0_halo", CGS.kpc))
+
+
 class Param:
     """Parsed parameter set (param.cc:12-28)."""
 
@@ -364,8 +371,15 @@ class Param:
                                         rho=s.f("value", "rho"),
                                         r0=s.f("value", "r0") * CGS.kpc,
                                         z0=s.f("value", "z0") * CGS.kpc)
-                elif self.brnd_method in ("jf12", "jaffe"):
-                    pass
+                elif self.brnd_method == "jf12":
+                    s = _trace(root, ["magneticfield", "random", "global", "jf12"])
+                    self.brnd_es = dict(rms=s.f("value", "rms") * CGS.muGauss,
+                                        k0=s.f("value", "k0"), a0=s.f("value", "a0"),
+                                        k1=s.f("value", "k1"), a1=s.f("value", "a1"),
+                                        rho=s.f("value", "rho"), r0=0.0, z0=0.0)
+                    self.brnd_jf12 = [s.f("value", k) * u for k, u in BRND_JF12_PARAMS]
+                elif self.brnd_method == "jaffe":
+                    _trace(root, ["magneticfield", "random", "global", "jaffe"])
                 else:
                     raise RuntimeError("unsupported brnd model")
             elif self.brnd_type == "local":
@@ -547,6 +561,8 @@ class Param:
             d.breg[0:4] = [q["b0"], q["psi0"], q["psi1"], q["chi0"]]
         if d.breg_model == capi.HAMX_BREG_GRID:
             d.breg_box = self.grid_breg.box()
+        if self.brnd_method == "jf12":
+            d.jf12[:] = self.brnd_jf12
         return d
 
     def dft_desc(self) -> capi.HamxDftDesc:

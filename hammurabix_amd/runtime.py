@@ -217,14 +217,24 @@ class Context:
     # -------------------------------------------------------- random fields
     def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
                 breg_grid: Optional[tuple[Any, Any, Any]] = None) -> None:
+        """Brnd_es::write_grid (brnd_es.cc:76-308) into bx, by, bz [G]."""
+        self._brnd(self.lib.hamx_brnd_es, desc, seed, bx, by, bz, breg_grid)
+
+    def brnd_jf12(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
+                  breg_grid: Optional[tuple[Any, Any, Any]] = None) -> None:
+        """Brnd_jf12::write_grid (brnd_jf12.cc:127-385): the ES steps with the JF12
+        energy profile taken from desc.jf12."""
+        self._brnd(self.lib.hamx_brnd_jf12, desc, seed, bx, by, bz, breg_grid)
+
+    def _brnd(self, fn, desc, seed, bx, by, bz, breg_grid) -> None:
         flags = capi.HAMX_OUT_DEVICE if _residency([bx, by, bz], "output") else 0
         if breg_grid is not None:
             desc.breg_x, desc.breg_y, desc.breg_z = (_ptr(a)[0] for a in breg_grid)
             if _residency(list(breg_grid), "input"):
                 flags |= capi.HAMX_IN_DEVICE
         _order_after_torch([bx, by, bz] + (list(breg_grid) if breg_grid is not None else []))
-        self._check(self.lib.hamx_brnd_es(self.handle, C.byref(desc), seed,
-                                          _ptr(bx)[0], _ptr(by)[0], _ptr(bz)[0], flags))
+        self._check(fn(self.handle, C.byref(desc), seed,
+                       _ptr(bx)[0], _ptr(by)[0], _ptr(bz)[0], flags))
 
     def ternd_dft(self, desc: capi.HamxDftDesc, seed: int, te: Any) -> None:
         flags = capi.HAMX_OUT_DEVICE if _residency([te], "output") else 0

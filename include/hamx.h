@@ -160,6 +160,9 @@ typedef struct hamx_es_desc {
   double breg[HAMX_BREG_NPARAM];
   const double *breg_x, *breg_y, *breg_z;
   hamx_box breg_box;
+  /* Brnd_jf12 energy profile (hamx_brnd_jf12 only; brnd_jf12.cc:54-114):
+   * b0_1..b0_8, b0_int [G], z0_spiral [cm], b0_halo [G], r0_halo, z0_halo [cm] */
+  double jf12[13];
 } hamx_es_desc;
 
 /* TErnd_dft generator (ternd_dft.cc:35-128) on box `box`. */
@@ -329,6 +332,12 @@ void hamx_batch_destroy(hamx_batch *b);
  * Outputs bx, by, bz: nx*ny*nz doubles each. */
 int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
                  double *bx, double *by, double *bz, uint32_t flags);
+
+/* Brnd_jf12::write_grid (bfield.h:329-330, brnd_jf12.cc:127-385): the same steps as Brnd_es with
+ * the JF12 energy profile (d->jf12; d->r0 / z0 unused) and the reference's
+ * literal constants of that file (0.33333333, 0.66666667, 1.73205081). */
+int hamx_brnd_jf12(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
+                   double *bx, double *by, double *bz, uint32_t flags);
 
 /* TErnd_dft::write_grid (tefield.h:159-160, ternd_dft.cc:35-128).
  * Output te: nx*ny*nz doubles. */

@@ -75,6 +75,9 @@ void orc_fft3d(double *data, uint64_t nx, uint64_t ny, uint64_t nz, int sign);
 double orc_variance(const double *arr, uint64_t n);
 int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
                 double *bz);
+int orc_brnd_jf12(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
+                  double *bz);
+double orc_jf12_profile(const double *q, const double pos[3]);
 int orc_ternd_dft(const hamx_dft_desc *d, uint64_t seed, double *te);
 double orc_es_spectrum(const hamx_es_desc *d, double k);
 double orc_dft_spectrum(const hamx_dft_desc *d, double k);

@@ -20,10 +20,10 @@ namespace {
 
 const double kPi = 3.141592653589793238462643383279502884197;
 const double kTwoPi = 6.283185307179586476925286766559005768394;
-const double kOneThird = 0.333333333333333333333333333333333333333;
-const double kTwoThirds = 0.666666666666666666666666666666666666667;
+const double kOneThirdES = 0.333333333333333333333333333333333333333;
+const double kTwoThirdsES = 0.666666666666666666666666666666666666667;
 const double kSqrt2 = 1.4142135623730950488016887242096980785697;
-const double kSqrt3 = 1.7320508075688772935274463415058723669428;
+const double kSqrt3ES = 1.7320508075688772935274463415058723669428;
 const double kKpc = 3.0856775806e+21;
 
 inline void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo) {
@@ -249,8 +249,47 @@ double orc_dft_spectrum(const hamx_dft_desc *d, double k) {
 }
 
 // Brnd_es::write_grid (brnd_es.cc:76-308)
-int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
-                double *bz) {
+// Brnd_jf12::spatial_profile (brnd_jf12.cc:54-114)
+static double jf12_profile(const double *q, const double pos[3]) {
+  const double b_arms[8] = {q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7]};
+  const double b0_int = q[8], z0_spiral = q[9], b0_halo = q[10], r0_halo = q[11],
+               z0_halo = q[12];
+  const double Rmax = 20. * kKpc, rho_GC = 1. * kKpc;
+  const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+  const double rho = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+  const double z = pos[2];
+  const double phi = std::atan2(pos[1], pos[0]);
+  const double rc_B[8] = {5.1, 6.3, 7.1, 8.3, 9.8, 11.4, 12.7, 15.5};
+  const double inc = 11.5;
+  const double muG = 1e-6;
+  double scaling_disk = 0.0 * muG;
+  double scaling_halo = 0.0 * muG;
+  if (r > Rmax || rho < rho_GC)
+    return 0.0;
+  if (r < 5. * kKpc) {
+    scaling_disk = b0_int;
+  } else {
+    double r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi - M_PI));
+    if (r_negx > rc_B[7] * kKpc)
+      r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi + M_PI));
+    if (r_negx > rc_B[7] * kKpc)
+      r_negx = r * std::exp(-1 / std::tan(M_PI / 180. * (90 - inc)) * (phi + 3 * M_PI));
+    for (int i = 7; i >= 0; i--)
+      if (r_negx < rc_B[i] * kKpc)
+        scaling_disk = b_arms[i] * (5. * kKpc) / r;
+  }
+  scaling_disk = scaling_disk * std::exp(-0.5 * z * z / (z0_spiral * z0_spiral));
+  scaling_halo = b0_halo * std::exp(-std::fabs(r / r0_halo)) * std::exp(-std::fabs(z / z0_halo));
+  return (scaling_disk * scaling_disk + scaling_halo * scaling_halo) / (muG * muG);
+}
+
+// Brnd_es::write_grid (brnd_es.cc:76-308); with jf12, Brnd_jf12::write_grid
+// (brnd_jf12.cc:127-385): its energy profile and its literal constants.
+static int brnd_generate(const hamx_es_desc *d, uint64_t seed, double *bx, double *by, double *bz,
+                  bool jf12) {
+  const double kOneThird = jf12 ? 0.33333333 : kOneThirdES;
+  const double kTwoThirds = jf12 ? 0.66666667 : kTwoThirdsES;
+  const double kSqrt3 = jf12 ? 1.73205081 : kSqrt3ES;
   const uint64_t nx = d->box.nx, ny = d->box.ny, nz = d->box.nz;
   const uint64_t N = nx * ny * nz;
   std::vector<double> c0(2 * N, 0.), c1(2 * N, 0.);
@@ -292,8 +331,9 @@ int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
       pos[1] = j * ly / (ny - 1) + d->box.y_min;
       for (uint64_t l = 0; l < nz; ++l) {
         pos[2] = l * lz / (nz - 1) + d->box.z_min;
-        const double ratio =
-            std::sqrt(profile(d->observer, d->r0, d->z0, pos)) * d->rms * b_var_invsq;
+        const double prof =
+            jf12 ? jf12_profile(d->jf12, pos) : profile(d->observer, d->r0, d->z0, pos);
+        const double ratio = std::sqrt(prof) * d->rms * b_var_invsq;
         const uint64_t idx = i * ny * nz + j * nz + l;
         V3 b_re = {{c0[2 * idx] * ratio, c0[2 * idx + 1] * ratio,
                     c1[2 * idx + 1] * ratio}};
@@ -386,6 +426,16 @@ int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
   }
   return 0;
 }
+
+int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by, double *bz) {
+  return brnd_generate(d, seed, bx, by, bz, false);
+}
+
+int orc_brnd_jf12(const hamx_es_desc *d, uint64_t seed, double *bx, double *by, double *bz) {
+  return brnd_generate(d, seed, bx, by, bz, true);
+}
+
+double orc_jf12_profile(const double *q, const double pos[3]) { return jf12_profile(q, pos); }
 
 // TErnd_dft::write_grid (ternd_dft.cc:35-128)
 int orc_ternd_dft(const hamx_dft_desc *d, uint64_t seed, double *te) {

@@ -43,6 +43,8 @@ PROTOS = {
     "orc_fft3d": (None, [_vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]),
     "orc_variance": (C.c_double, [_vp, C.c_uint64]),
     "orc_brnd_es": (C.c_int, [C.POINTER(capi.HamxEsDesc), C.c_uint64, _vp, _vp, _vp]),
+    "orc_brnd_jf12": (C.c_int, [C.POINTER(capi.HamxEsDesc), C.c_uint64, _vp, _vp, _vp]),
+    "orc_jf12_profile": (C.c_double, [_vp, _vp]),
     "orc_ternd_dft": (C.c_int, [C.POINTER(capi.HamxDftDesc), C.c_uint64, _vp]),
     "orc_es_spectrum": (C.c_double, [C.POINTER(capi.HamxEsDesc), C.c_double]),
     "orc_dft_spectrum": (C.c_double, [C.POINTER(capi.HamxDftDesc), C.c_double]),

@@ -137,6 +137,33 @@ def test_es_matches_oracle(ctx, model, rho):
         assert err < 1e-11
 
 
+@pytest.mark.parametrize("rho", [1.0, 0.5])
+def test_jf12_matches_oracle(ctx, rho):
+    """Brnd_jf12 (template energy profile) on a box reaching past r = 20 kpc
+    and through the 1 kpc hole, element-wise against the oracle."""
+    orc = load_oracle()
+    shape = (32, 32, 16)
+    d = es_desc(shape, capi.HAMX_BREG_LSA, rho=rho, k0=0.5)
+    d.box = capi.HamxBox(*shape, -24 * CGS.kpc, 24 * CGS.kpc, -24 * CGS.kpc, 24 * CGS.kpc,
+                         -3 * CGS.kpc, 3 * CGS.kpc)
+    mu = CGS.muGauss
+    d.jf12[:] = [10.81 * mu, 6.96 * mu, 9.59 * mu, 6.96 * mu, 1.96 * mu, 16.34 * mu,
+                 37.29 * mu, 10.35 * mu, 7.63 * mu, 0.61 * CGS.kpc, 4.68 * mu,
+                 10.97 * CGS.kpc, 2.84 * CGS.kpc]
+    n = int(np.prod(shape))
+    g = [np.zeros(n) for _ in range(3)]
+    r = [np.zeros(n) for _ in range(3)]
+    ctx.brnd_jf12(d, 21, *g)
+    assert orc.orc_brnd_jf12(C.byref(d), 21, *(ptr(a) for a in r)) == 0
+    e = [np.zeros(n) for _ in range(3)]
+    ctx.brnd_es(d, 21, *e)
+    for a, b, c in zip(g, r, e):
+        err = rel_l2(a, b)
+        print("brnd_jf12 rel L2", err)
+        assert err < 1e-11
+        assert rel_l2(c, b) > 1e-3
+
+
 def test_es_with_regular_field_grid(ctx):
     """Anisotropy direction from an imported breg grid (Breg::read_grid)."""
     orc = load_oracle()

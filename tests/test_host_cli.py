@@ -143,3 +143,45 @@ def test_cli_jf12_helix_jaffe_match_python_pipeline_and_oracle(tmp_path, ctx):
             got = np.fromfile(f"{out}sync_23.0_{c}.bin", dtype="<f8")
             assert np.array_equal(got, py[("sync", "23.0", "8", c)]), (name, c)
             assert rel_l2(got, ref[k]) < 1e-10, (name, c)
+
+
+@pytest.mark.gpu
+def test_cli_random_jf12_matches_python_pipeline_and_oracle(tmp_path, ctx):
+    """<random><global type="jf12">: the C++ host and the Python pipeline read
+    the reference's block (param.cc:375-409), write the same Brnd_jf12 grid,
+    and that grid is the oracle's."""
+    import ctypes as C
+    from oracle_lib import load_oracle, ptr
+    need_cli()
+    out = str(tmp_path) + "/"
+    text = configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1, grid=(32, 32, 16),
+                                  te_k0=0.3, freqs=[23.0], out=out)
+    es = text[text.index('<global type="es">'):text.index("</es>") + 5]
+    jf = ('<global type="jf12"><jf12><rms value="1.0"/><k0 value="0.5"/><k1 value="0.1"/>'
+          '<a0 value="1.7"/><a1 value="0.0"/><rho value="0.7"/>'
+          '<b0_1 value="10.81"/><b0_2 value="6.96"/><b0_3 value="9.59"/><b0_4 value="6.96"/>'
+          '<b0_5 value="1.96"/><b0_6 value="16.34"/><b0_7 value="37.29"/><b0_8 value="10.35"/>'
+          '<b0_int value="7.63"/><z0_spiral value="0.61"/><b0_halo value="4.68"/>'
+          '<r0_halo value="10.97"/><z0_halo value="2.84"/></jf12>')
+    text = text.replace(es, jf)
+    bfile = str(tmp_path / "brnd.bin")
+    text = text.replace('<brnd read="0" write="0" filename="brnd.bin"/>',
+                        f'<brnd read="0" write="1" filename="{bfile}"/>')
+    x = tmp_path / "p.xml"
+    x.write_text(text)
+    r = run(str(x))
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(bfile, dtype="<f8").reshape(-1, 3)
+    pipe = Pipeline(text=text, ctx=ctx, export=False)
+    py = pipe.run()
+    for i in range(3):
+        assert np.array_equal(raw[:, i], pipe.brnd[i].cpu().numpy()), i
+    q = np.fromfile(str(tmp_path / "sync_23.0_Q.bin"), dtype="<f8")
+    assert np.array_equal(q, py[("sync", "23.0", "16", "Q")])
+    par = Param(text=text)
+    n = par.grid_brnd.full_size
+    ref = [np.zeros(n) for _ in range(3)]
+    assert load_oracle().orc_brnd_jf12(C.byref(par.es_desc()), par.brnd_seed,
+                                       *(ptr(a) for a in ref)) == 0
+    for i in range(3):
+        assert rel_l2(raw[:, i], ref[i]) < 1e-11, i
