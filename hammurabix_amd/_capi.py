@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-HAMX_ABI_VERSION = 1
+HAMX_ABI_VERSION = 2
 HAMX_MAX_SHELL = 16
 
 HAMX_OK = 0
@@ -28,7 +28,7 @@ HAMX_USE_TAPE = 16
  HAMX_BREG_HELIX, HAMX_BREG_JF12, HAMX_BREG_JAFFE) = range(8)
 HAMX_BREG_NPARAM = 40
 HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID = range(3)
-HAMX_CRE_NONE, HAMX_CRE_ANA, HAMX_CRE_UNIF = range(3)
+HAMX_CRE_NONE, HAMX_CRE_ANA, HAMX_CRE_UNIF, HAMX_CRE_GRID = range(4)
 HAMX_SCHED_AUTO, HAMX_SCHED_THREAD, HAMX_SCHED_WAVE = range(3)
 
 _dp = C.POINTER(C.c_double)
@@ -48,7 +48,10 @@ class HamxFields(C.Structure):
                 ("breg_x", C.c_void_p), ("breg_y", C.c_void_p), ("breg_z", C.c_void_p),
                 ("breg_box", HamxBox),
                 ("tereg", C.c_void_p), ("tereg_box", HamxBox),
-                ("mask", C.c_void_p), ("nside_mask", C.c_uint32), ("reserved", C.c_uint32)]
+                ("mask", C.c_void_p), ("nside_mask", C.c_uint32), ("reserved", C.c_uint32),
+                ("cre", C.c_void_p), ("cre_box", HamxBox),
+                ("cre_E_min", C.c_double), ("cre_E_fact", C.c_double),
+                ("cre_nE", C.c_uint32), ("reserved2", C.c_uint32)]
 
 
 class HamxLosDesc(C.Structure):
@@ -106,6 +109,11 @@ _PROTOS = {
     "hamx_los_samples": (C.c_int, [C.POINTER(HamxLosDesc), C.POINTER(C.c_uint64)]),
     "hamx_brnd_es": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "hamx_cre_write_grid": (C.c_int, [C.c_void_p, C.POINTER(HamxLosDesc), C.POINTER(HamxBox),
+                                      C.c_double, C.c_double, C.c_uint32, C.c_void_p,
+                                      C.c_uint32]),
+    "hamx_sync_kernels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.c_void_p, C.c_uint32]),
     "hamx_brnd_jf12": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hamx_ternd_dft": (C.c_int, [C.c_void_p, C.POINTER(HamxDftDesc), C.c_uint64,

@@ -34,7 +34,7 @@ _TEMPLATE = """<?xml version="1.0"?>
     <brnd read="{brnd_read}" write="0" filename="{brnd_file}"/>
     <tereg read="0" write="0" filename="tereg.bin"/>
     <ternd read="{ternd_read}" write="0" filename="{ternd_file}"/>
-    <cre read="0" write="0" filename="cre.bin"/>
+    <cre read="{cre_read}" write="{cre_write}" filename="{cre_file}"/>
   </fieldio>
   <grid>
     <observer>
@@ -54,6 +54,13 @@ _TEMPLATE = """<?xml version="1.0"?>
       <y_min value="-20.0"/> <y_max value="20.0"/>
       <z_min value="-4.0"/> <z_max value="4.0"/>
     </box_ternd>
+    <box_cre>
+      <nx value="{cre_nx}"/> <ny value="{cre_ny}"/> <nz value="{cre_nz}"/> <nE value="{cre_nE}"/>
+      <x_min value="-20.0"/> <x_max value="20.0"/>
+      <y_min value="-20.0"/> <y_max value="20.0"/>
+      <z_min value="-4.0"/> <z_max value="4.0"/>
+      <E_min value="{cre_E_min}"/> <E_max value="{cre_E_max}"/>
+    </box_cre>
     <shell>
       <layer type="auto">
         <auto>
@@ -145,7 +152,13 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         seed: int = 1, b0: float = 6.0, psi0: float = 27.0, te_k0: float = 2.0,
         do_dm: bool = True,
         do_fd: bool = True, brnd_file: str = "", ternd_file: str = "",
-        out: str = "", mask_file: str = "", nside_mask: int = 32) -> str:
+        out: str = "", mask_file: str = "", nside_mask: int = 32,
+        cre_file: str = "", cre_read: bool = False, cre_write: bool = False,
+        cre_grid: tuple[int, int, int, int] = (40, 40, 8, 32),
+        cre_E: tuple[float, float] = (0.1, 1000.0)) -> str:
+    """The parameter file; cre_read / cre_write with cre_file switch on the
+    CRE grid I/O on box_cre (cre_grid = nx, ny, nz, nE; cre_E = E_min, E_max
+    in GeV)."""
     freqs = [23.0] if freqs is None else freqs
     return _TEMPLATE.format(
         do_dm=int(do_dm), do_fd=int(do_fd), nside_out=nside_out,
@@ -155,7 +168,10 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         nx=grid[0], ny=grid[1], nz=grid[2], shell_num=shell_num, nside_sim=nside_sim,
         oc_r_max=oc_r_max, oc_r_res=oc_r_res, b0=b0, psi0=psi0,
         brnd_on=int(random), ternd_on=int(random), seed=seed, te_k0=te_k0,
-        mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask)
+        mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask,
+        cre_read=int(cre_read), cre_write=int(cre_write), cre_file=cre_file or "cre.bin",
+        cre_nx=cre_grid[0], cre_ny=cre_grid[1], cre_nz=cre_grid[2], cre_nE=cre_grid[3],
+        cre_E_min=cre_E[0], cre_E_max=cre_E[1])
 
 
 def plain_forward(**kw) -> str:

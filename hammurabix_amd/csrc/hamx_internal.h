@@ -25,7 +25,7 @@ struct hamx_ctx {
   std::vector<hamx_kernel_time> times;
   // Grow-only device workspace, one buffer per slot.  Slots are owned by the
   // entry point that uses them; a call reuses the previous call's memory.
-  static constexpr int kSlots = 48;
+  static constexpr int kSlots = 56;
   void *ws[kSlots] = {};
   size_t ws_bytes[kSlots] = {};
   // What the tape in workspace slots kSlotAdjCkpt / kSlotTapeFd was recorded
@@ -33,6 +33,9 @@ struct hamx_ctx {
   bool tape_valid = false;
   hamx_los_desc tape_desc{};
   hamx_fields tape_fields{};
+  // Host copy of small per-call tables uploaded asynchronously; it lives
+  // until the next call so the copy never reads freed memory.
+  std::vector<double> host_tab;
   // Pinned host staging buffer for host-resident outputs.
   void *pinned = nullptr;
   size_t pinned_bytes = 0;
@@ -88,6 +91,10 @@ enum Slot : int {
   kSlotTapeFd,    // every ray's FD over each shell (forward tape)
   kSlotMaskPivot, // the caller's mask map (host input staged)
   kSlotMaskShell, // the mask duplicated to the current shell's nside
+  kSlotCre,       // the caller's CRE flux grid (host input staged)
+  kSlotCreTab,    // energy tables of the CRE grid emissivities
+  kSlotSyncIO,    // hamx_sync_kernels staging (x, F, G)
+  kSlotCreOut,    // hamx_cre_write_grid output staging
   kSlotCount
 };
 static_assert(kSlotCount <= hamx_ctx::kSlots, "workspace slots");

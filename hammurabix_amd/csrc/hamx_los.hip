@@ -39,6 +39,7 @@
 #include "hamx_healpix.h"
 #include "hamx_internal.h"
 #include "hamx_march.h"
+#include "hamx_sync.h"
 
 namespace hamx {
 
@@ -89,7 +90,72 @@ __device__ inline double cre_index(const Phys &ph, const Vec3 &pos) {
   return -ph.cre[0] + ph.cre[1] * r + ph.cre[2] * z;
 }
 
-template <bool PACKED>
+// CREfield::read_grid_num (cre.cc:144-199) of energy bin e in a cell found by
+// grid_cell on the CRE box: the same z, y, x interpolation order over the
+// [x][y][z][E] layout (z neighbours nE apart).
+__device__ inline double lerp_cell_e(const double *f, const Cell &c, uint64_t nE,
+                                     uint64_t sy, uint64_t sx, uint64_t e) {
+  const uint64_t i00 = c.idx * nE + e, i01 = i00 + sy, i10 = i00 + sx, i11 = i00 + sx + sy;
+  const double i1 = f[i00] * (1. - c.zd) + f[i00 + nE] * c.zd;
+  const double i2 = f[i01] * (1 - c.zd) + f[i01 + nE] * c.zd;
+  const double j1 = f[i10] * (1 - c.zd) + f[i10 + nE] * c.zd;
+  const double j2 = f[i11] * (1 - c.zd) + f[i11 + nE] * c.zd;
+  const double w1 = i1 * (1 - c.yd) + i2 * c.yd;
+  const double w2 = j1 * (1 - c.yd) + j2 * c.yd;
+  return w1 * (1 - c.xd) + w2 * c.xd;
+}
+
+// sync_emissivity_t and _p with the CRE grid (integrator.cc:286-325 and
+// 355-394) for one sample: the midpoint rule over the energy bins of
+// F(x) resp. G(x) times the bin-mean flux / beta, bins whose mean x exceeds
+// 100 skipped, then the fore factor.  Each bin's gather is shared by J_tot
+// and J_pol and reused by the next bin.  Outside the grid every gathered
+// flux is 0 and so are both sums.
+__device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, const Vec3 &pos,
+                                           double b_per, double &jt, double &jp) {
+  jt = 0.;
+  jp = 0.;
+  Cell c;
+  if (!grid_cell(gr.g_cre, pos, c))
+    return;
+  const double ab = fabs(b_per);
+  const double den = ph.three_qe * ab;
+  const double xft = ph.xft_num / den;
+  const double xfp = ph.xfp_num / den;
+  const uint64_t nE = gr.cre_nE;
+  const uint64_t sy = gr.g_cre.nz * nE, sx = gr.g_cre.ny * sy;
+  const double *ke2 = gr.cre_tab, *beta = ke2 + nE, *dE = beta + nE;
+  double xt0 = xft / ke2[0], xp0 = xfp / ke2[0];
+  double g0 = 0.;
+  bool have0 = false;
+  double Jt = 0., Jp = 0.;
+  for (uint64_t i = 0; i + 1 < nE; ++i) {
+    const double xt1 = xft / ke2[i + 1], xp1 = xfp / ke2[i + 1];
+    const double xvt = 0.5 * (xt1 + xt0), xvp = 0.5 * (xp1 + xp0);
+    const bool on_t = !(xvt > 100), on_p = !(xvp > 100);
+    if (on_t || on_p) {
+      if (!have0)
+        g0 = lerp_cell_e(gr.cre, c, nE, sy, sx, i) / beta[i];
+      const double g1 = lerp_cell_e(gr.cre, c, nE, sy, sx, i + 1) / beta[i + 1];
+      const double flux = 0.5 * (g1 + g0);
+      if (on_t)
+        Jt += sync_kernel<false>(xvt) * flux * dE[i];
+      if (on_p)
+        Jp += sync_kernel<true>(xvp) * flux * dE[i];
+      g0 = g1;
+      have0 = true;
+    } else {
+      have0 = false;
+    }
+    xt0 = xt1;
+    xp0 = xp1;
+  }
+  const double fore = ph.fore_num * ab / ph.fore_den;
+  jt = Jt * fore;
+  jp = Jp * fore;
+}
+
+template <bool PACKED, bool CREG>
 __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
                                    const ShellArgs &sa, const RayGeo &r,
                                    uint64_t k, SampleOut &s) {
@@ -150,6 +216,14 @@ __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
     const Vec3 cr{B.y * r.l.z - B.z * r.l.y, B.z * r.l.x - B.x * r.l.z,
                   B.x * r.l.y - B.y * r.l.x};
     const double b_per = length0(cr);
+    if (CREG) {
+      double jt, jp;
+      cre_grid_emissivity(ph, gr, pos, b_per, jt, jp);
+      s.jt = jt * sa.delta * ph.i2bt;
+      s.jp = jp * sa.delta * ph.i2bt;
+      s.ipa = atan2(-dot0(r.eth, B), -dot0(r.eph, B));
+      return true;
+    }
     // analytic emissivities (integrator.cc:333-346, 402-415)
     double index, g1, g2, g3;
     if (ph.idx_const) {
@@ -208,7 +282,7 @@ __device__ inline bool masked_out(const Phys &ph, const ShellArgs &sa, uint64_t 
   return true;
 }
 
-template <bool PACKED>
+template <bool PACKED, bool CREG>
 __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
                                                   ShellArgs sa) {
   const int lane = threadIdx.x & 63;
@@ -226,7 +300,7 @@ __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
     SampleOut s;
     bool valid = false;
     if (k < r.k_hi)
-      valid = eval_sample<PACKED>(ph, gr, sa, r, k, s);
+      valid = eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s);
     const double fdk = valid ? s.fdc : 0.;
     if (ph.do_sync) {
       const double incl = wave_incl_scan(fdk, lane);
@@ -329,7 +403,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
-template <bool PACKED>
+template <bool PACKED, bool CREG>
 __global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
                                                     ShellArgs sa) {
   const uint64_t ipix = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -342,7 +416,7 @@ __global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
   double dm = 0., fd = 0., I = 0., Q = 0., U = 0.;
   for (uint64_t k = r.k_lo; k < r.k_hi; ++k) {
     SampleOut s;
-    if (!eval_sample<PACKED>(ph, gr, sa, r, k, s))
+    if (!eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s))
       continue;
     if (ph.do_dm)
       dm += s.te * sa.delta;
@@ -472,7 +546,7 @@ int validate_los(hamx_ctx *ctx, const hamx_los_desc *d) {
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
   if (d->tereg_model < HAMX_TEREG_NONE || d->tereg_model > HAMX_TEREG_GRID)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported tereg model %d", d->tereg_model);
-  if (d->cre_model < HAMX_CRE_NONE || d->cre_model > HAMX_CRE_UNIF)
+  if (d->cre_model < HAMX_CRE_NONE || d->cre_model > HAMX_CRE_GRID)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported cre model %d", d->cre_model);
   if (d->do_sync && d->cre_model == HAMX_CRE_NONE)
     return set_err(ctx, HAMX_ERR_ARG, "synchrotron output requires a CRE model");
@@ -508,6 +582,10 @@ Phys make_phys(const hamx_los_desc *d) {
     ph.lambda_sq = (cgs::c_light / f) * (cgs::c_light / f);
     ph.i2bt = cgs::c_light * cgs::c_light / (2. * cgs::kB * f * f);
     ph.A_num = 2. * cgs::pi * f * cgs::mec;
+    ph.xft_num = 4. * cgs::pi * f * cgs::mec * cgs::mec2 * cgs::mec2;
+    ph.xfp_num = 2. * cgs::mec * cgs::mec2 * cgs::mec2 * 2. * cgs::pi * f;
+    ph.fore_num = 1.73205081 * cgs::qe * cgs::qe * cgs::qe;
+    ph.fore_den = cgs::mec2 * cgs::c_light * cgs::GeV * cgs::m * cgs::m * cgs::sec;
     const double p = cgs::h_planck * f / (cgs::kB * 2.725);
     ph.tc_em1 = std::exp(p) - 1.;
     ph.tc_den = p * p * std::exp(p);
@@ -634,10 +712,43 @@ int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
     gr.tereg = static_cast<const double *>(p);
     gr.g_tereg = grid_geo(f.tereg_box);
   }
+  if (d->cre_model == HAMX_CRE_GRID) {
+    if (!f.cre)
+      return set_err(ctx, HAMX_ERR_ARG, "cre grid model without cre grid array");
+    HAMX_TRY(check_box(ctx, f.cre_box, "cre"));
+    if (f.cre_nE < 2)
+      return set_err(ctx, HAMX_ERR_ARG, "cre grid needs at least 2 energies");
+    if (!(f.cre_E_min > cgs::mec2) || !(f.cre_E_fact > 0.))
+      return set_err(ctx, HAMX_ERR_ARG,
+                     "cre energy axis must start above mec2 and increase");
+    const uint64_t nE = f.cre_nE;
+    HAMX_TRY(stage_in(ctx, kSlotCre, f.cre, box_size(f.cre_box) * nE * sizeof(double), in_dev,
+                      &p));
+    gr.cre = static_cast<const double *>(p);
+    gr.g_cre = grid_geo(f.cre_box);
+    gr.cre_nE = nE;
+    // E_i, x denominators and beta as integrator.cc:300-304 forms them
+    std::vector<double> &tab = ctx->host_tab;
+    tab.assign(3 * nE, 0.);
+    std::vector<double> KE(nE);
+    for (uint64_t i = 0; i != nE; ++i) {
+      KE[i] = f.cre_E_min * std::exp(i * f.cre_E_fact);
+      tab[i] = KE[i] * KE[i];
+      tab[nE + i] = std::sqrt(1 - cgs::mec2 / KE[i]);
+    }
+    for (uint64_t i = 0; i + 1 < nE; ++i)
+      tab[2 * nE + i] = std::fabs(KE[i + 1] - KE[i]);
+    void *tp;
+    HAMX_TRY(ws_get(ctx, kSlotCreTab, tab.size() * sizeof(double), &tp));
+    HAMX_HIP(ctx, hipMemcpyAsync(tp, tab.data(), tab.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    gr.cre_tab = static_cast<const double *>(tp);
+  }
   // Pack the random B components and TE density into one record per vertex
   // when they share a box: 64 B contiguous per z-pair of corners.
   packed = false;
-  if (has_brnd && (!f.ternd || same_box(f.brnd_box, f.ternd_box))) {
+  if (d->cre_model != HAMX_CRE_GRID && has_brnd &&
+      (!f.ternd || same_box(f.brnd_box, f.ternd_box))) {
     const uint64_t n = box_size(f.brnd_box);
     void *pk;
     HAMX_TRY(ws_get(ctx, kSlotPacked, n * sizeof(double4), &pk));
@@ -773,6 +884,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
                     (d->cre_model == HAMX_CRE_ANA || d->cre_model == HAMX_CRE_UNIF) &&
                     (packed || !grids_any);
   const bool pw1 = fast && 0.5 * (ph.idx + 1) == -1.0;
+  const bool creg = d->cre_model == HAMX_CRE_GRID && d->do_sync;
   for (uint32_t s = 0; s < d->n_shell; ++s) {
     ShellArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -819,19 +931,25 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
                            ctx->stream, ph, fc, gr, sa);
     } else if (sched == HAMX_SCHED_WAVE) {
       const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
-      if (packed)
-        hipLaunchKernelGGL(k_los_wave<true>, dim3(blocks), dim3(256), 0, ctx->stream,
-                           ph, gr, sa);
-      else
-        hipLaunchKernelGGL(k_los_wave<false>, dim3(blocks), dim3(256), 0, ctx->stream,
-                           ph, gr, sa);
-    } else {
-      const unsigned blocks = static_cast<unsigned>((npix + 255) / 256);
-      if (packed)
-        hipLaunchKernelGGL(k_los_thread<true>, dim3(blocks), dim3(256), 0,
+      if (creg)
+        hipLaunchKernelGGL((k_los_wave<false, true>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+      else if (packed)
+        hipLaunchKernelGGL((k_los_wave<true, false>), dim3(blocks), dim3(256), 0,
                            ctx->stream, ph, gr, sa);
       else
-        hipLaunchKernelGGL(k_los_thread<false>, dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL((k_los_wave<false, false>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+    } else {
+      const unsigned blocks = static_cast<unsigned>((npix + 255) / 256);
+      if (creg)
+        hipLaunchKernelGGL((k_los_thread<false, true>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+      else if (packed)
+        hipLaunchKernelGGL((k_los_thread<true, false>), dim3(blocks), dim3(256), 0,
+                           ctx->stream, ph, gr, sa);
+      else
+        hipLaunchKernelGGL((k_los_thread<false, false>), dim3(blocks), dim3(256), 0,
                            ctx->stream, ph, gr, sa);
     }
     HAMX_HIP(ctx, hipGetLastError());
@@ -913,4 +1031,150 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
   if (ctx && (flags & HAMX_KEEP_TAPE))
     return los_forward_taped(ctx, d, fields, out, flags);
   return los_forward_impl(ctx, d, fields, out, flags, nullptr, nullptr);
+}
+
+// ------------------------------------------------------ CRE grid helpers --
+namespace hamx {
+namespace {
+
+struct CreWriteArgs {
+  hamx_box box;
+  double lx, ly, lz;
+  const double *E; // E_m [erg], host-formed
+  uint64_t nE, n;
+  double norm;     // j0 * gamma0^(-flux_idx(observer))
+  double den0;     // 1 - 1/gamma0
+};
+
+// CREfield::write_grid (cre.cc:204-225) with CRE_ana / CRE_unif::write_field
+// (cre_ana.cc:52-63, cre_unif.cc:48-59): one thread per output value, E
+// fastest, so the stores are contiguous.
+__global__ __launch_bounds__(256) void k_cre_write(Phys ph, CreWriteArgs a, double *out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < a.n;
+       t += stride) {
+    const uint64_t m = t % a.nE, v = t / a.nE;
+    const uint64_t k = v % a.box.nz, j = (v / a.box.nz) % a.box.ny,
+                   i = v / (a.box.nz * a.box.ny);
+    const Vec3 pos{a.lx * static_cast<double>(i) / static_cast<double>(a.box.nx - 1) + a.box.x_min,
+                   a.ly * static_cast<double>(j) / static_cast<double>(a.box.ny - 1) + a.box.y_min,
+                   a.lz * static_cast<double>(k) / static_cast<double>(a.box.nz - 1) + a.box.z_min};
+    const double gamma = a.E[m] / cgs::mec2;
+    const double beta_ratio = sqrt((1. - 1. / gamma) / a.den0);
+    const double idx = ph.cre_model == HAMX_CRE_ANA ? cre_index(ph, pos) : -(ph.cre[0]);
+    out[t] = a.norm * beta_ratio * pow(gamma, idx) * cre_profile(ph, pos);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sync_fg(const double *x, uint64_t n, double *F,
+                                                 double *G) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += stride) {
+    F[t] = sync_kernel<false>(x[t]);
+    G[t] = sync_kernel<true>(x[t]);
+  }
+}
+
+unsigned grid_blocks(uint64_t n) {
+  uint64_t b = (n + 255) / 256;
+  if (b > 65536)
+    b = 65536;
+  return static_cast<unsigned>(b < 1 ? 1 : b);
+}
+
+} // namespace
+} // namespace hamx
+
+extern "C" int hamx_cre_write_grid(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_box *box,
+                                   double E_min, double E_fact, uint32_t nE, double *out,
+                                   uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !box || !out)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_cre_write_grid: null argument");
+  if (d->cre_model != HAMX_CRE_ANA && d->cre_model != HAMX_CRE_UNIF)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                   "hamx_cre_write_grid: cre model %d is not analytic or uniform", d->cre_model);
+  HAMX_TRY(check_box(ctx, *box, "cre"));
+  if (nE < 1 || !(E_min > 0.))
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_cre_write_grid: empty energy axis");
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  hamx_los_desc dd = *d;
+  dd.do_sync = 0;
+  const Phys ph = make_phys(&dd);
+  CreWriteArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.box = *box;
+  a.lx = box->x_max - box->x_min;
+  a.ly = box->y_max - box->y_min;
+  a.lz = box->z_max - box->z_min;
+  a.nE = nE;
+  a.n = box_size(*box) * nE;
+  const bool ana = d->cre_model == HAMX_CRE_ANA;
+  const double E0 = ana ? d->cre[5] : d->cre[2];
+  const double j0 = ana ? d->cre[6] : d->cre[3];
+  const double gamma0 = E0 / cgs::mec2;
+  double idx_obs = -(d->cre[0]);
+  if (ana) {
+    const double r_obs =
+        std::sqrt(d->observer[0] * d->observer[0] + d->observer[1] * d->observer[1]) / cgs::kpc;
+    const double z_obs = std::fabs(d->observer[2]) / cgs::kpc;
+    idx_obs = -d->cre[0] + d->cre[1] * r_obs + d->cre[2] * z_obs;
+  }
+  a.norm = j0 * std::pow(gamma0, -idx_obs);
+  a.den0 = 1. - 1. / gamma0;
+  std::vector<double> &tab = ctx->host_tab;
+  tab.resize(nE);
+  for (uint32_t m = 0; m != nE; ++m)
+    tab[m] = E_min * std::exp(m * E_fact);
+  void *w;
+  HAMX_TRY(ws_get(ctx, kSlotCreTab, nE * sizeof(double), &w));
+  HAMX_HIP(ctx, hipMemcpyAsync(w, tab.data(), nE * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+  a.E = static_cast<const double *>(w);
+  double *o = out;
+  if (!out_dev) {
+    HAMX_TRY(ws_get(ctx, kSlotCreOut, a.n * sizeof(double), &w));
+    o = static_cast<double *>(w);
+  }
+  hipLaunchKernelGGL(k_cre_write, dim3(grid_blocks(a.n)), dim3(256), 0, ctx->stream, ph, a, o);
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev)
+    HAMX_HIP(ctx, hipMemcpyAsync(out, o, a.n * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+
+extern "C" int hamx_sync_kernels(hamx_ctx *ctx, const double *x, uint64_t n, double *F,
+                                 double *G, uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!x || !F || !G)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_sync_kernels: null argument");
+  if (n == 0)
+    return HAMX_OK;
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  void *w;
+  HAMX_TRY(ws_get(ctx, kSlotSyncIO, 3 * n * sizeof(double), &w));
+  double *buf = static_cast<double *>(w);
+  const double *xd = x;
+  if (!in_dev) {
+    HAMX_HIP(ctx, hipMemcpyAsync(buf, x, n * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    xd = buf;
+  }
+  double *Fd = out_dev ? F : buf + n, *Gd = out_dev ? G : buf + 2 * n;
+  hipLaunchKernelGGL(k_sync_fg, dim3(grid_blocks(n)), dim3(256), 0, ctx->stream, xd, n, Fd, Gd);
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev) {
+    HAMX_HIP(ctx, hipMemcpyAsync(F, Fd, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HAMX_HIP(ctx, hipMemcpyAsync(G, Gd, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
 }

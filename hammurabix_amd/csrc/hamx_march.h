@@ -39,6 +39,10 @@ struct Phys {
   int idx_const;        // spectral index constant along rays
   double idx, g_t1, g_t2, g_p1; // index and its Gamma factors when constant
   double tc_em1, tc_den; // temp_convert: (e^p - 1) and p^2 e^p, integrator.cc:275-279
+  // CRE grid emissivities (integrator.cc:296-298,325-327,365-367,379-381):
+  // the x_fact numerators of J_tot and J_pol in their own operation orders,
+  // and the fore factor's 1.73205081 qe^3 and denominator.
+  double xft_num, xfp_num, fore_num, fore_den;
 };
 
 struct Grids {
@@ -52,6 +56,12 @@ struct Grids {
   GridGeo g_breg;
   const double *tereg;
   GridGeo g_tereg;
+  // CRE_num grid [x][y][z][E] and its energy tables: E_i^2, beta_i, and
+  // |E_{i+1} - E_i| (cre_nE, cre_nE, cre_nE - 1 doubles from cre_tab)
+  const double *cre;
+  GridGeo g_cre;
+  uint64_t cre_nE;
+  const double *cre_tab;
 };
 
 struct ShellArgs {

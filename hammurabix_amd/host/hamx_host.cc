@@ -413,6 +413,42 @@ int TEreg_unif::device_model(const Param *par, double *p) const {
 }
 
 int CREfield::device_model(const Param *, double *) const { return -1; }
+int CRE_num::device_model(const Param *, double *) const { return HAMX_CRE_GRID; }
+
+void CREfield::write_grid(const Param *par, Grid_cre *grid) const {
+  hamx_los_desc d;
+  std::memset(&d, 0, sizeof(d));
+  for (int i = 0; i < 3; ++i)
+    d.observer[i] = par->observer[i];
+  d.cre_model = device_model(par, d.cre);
+  if (d.cre_model != HAMX_CRE_ANA && d.cre_model != HAMX_CRE_UNIF)
+    throw std::runtime_error("hamx: CRE grid output needs the analytic or uniform model");
+  const hamx_box b = to_box(par->grid_cre);
+  hamx_ctx *ctx = context();
+  check(hamx_cre_write_grid(ctx, &d, &b, par->grid_cre.E_min, par->grid_cre.E_fact,
+                            static_cast<uint32_t>(par->grid_cre.nE), grid->cre_flux.get(), 0),
+        ctx);
+  grid->dev_valid = false;
+}
+
+Grid_cre::Grid_cre(const Param *par) {
+  if (par->grid_cre.read_permission || par->grid_cre.write_permission)
+    build_grid(par);
+}
+Grid_cre::~Grid_cre() {
+  if (dev_flux)
+    hamx_dev_free(context(), dev_flux);
+}
+void Grid_cre::build_grid(const Param *par) {
+  cre_flux = std::make_unique<double[]>(par->grid_cre.cre_size);
+}
+void Grid_cre::export_grid(const Param *par) {
+  write_doubles(par->grid_cre.filename, {cre_flux.get()}, par->grid_cre.cre_size);
+}
+void Grid_cre::import_grid(const Param *par) {
+  read_doubles(par->grid_cre.filename, {cre_flux.get()}, par->grid_cre.cre_size);
+  dev_valid = false;
+}
 int CRE_ana::device_model(const Param *par, double *p) const {
   p[0] = par->cre_ana.alpha;
   p[1] = par->cre_ana.beta;
@@ -520,7 +556,7 @@ void TErnd_dft::write_grid(const Param *par, const TEreg *, const Grid_tereg *,
 void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
                             const TErnd *, const CREfield *cre, const Grid_breg *gbreg,
                             const Grid_brnd *gbrnd, const Grid_tereg *gtereg,
-                            const Grid_ternd *gternd, const Grid_cre *, Grid_obs *gobs,
+                            const Grid_ternd *gternd, const Grid_cre *gcre, Grid_obs *gobs,
                             const Param *par) const {
   const auto &g = par->grid_obs;
   if (g.total_shell < 1 || g.total_shell > HAMX_MAX_SHELL)
@@ -603,9 +639,9 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
     f.tereg = gtereg->te.get();
     f.tereg_box = to_box(par->grid_tereg);
   }
-  if (par->grid_cre.read_permission)
-    throw std::runtime_error("hamx: CRE grid input has no device path yet");
   d.cre_model = cre ? cre->device_model(par, d.cre) : HAMX_CRE_NONE;
+  if (par->grid_cre.read_permission)
+    d.cre_model = HAMX_CRE_GRID; // integrator.cc:287,356
   if (d.cre_model < 0)
     throw std::runtime_error("hamx: CRE model has no device evaluator");
   d.schedule = HAMX_SCHED_AUTO;
@@ -652,6 +688,25 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
     f.ternd = gternd->dev_te;
     f.ternd_box = to_box(par->grid_ternd);
   }
+  if (d.cre_model == HAMX_CRE_GRID) {
+    if (!gcre || !gcre->cre_flux)
+      throw std::runtime_error("hamx: CRE grid requested but Grid_cre holds no flux");
+    const size_t bytes = par->grid_cre.cre_size * sizeof(double);
+    if (!gcre->dev_valid) {
+      if (!gcre->dev_flux) {
+        void *p;
+        check(hamx_dev_alloc(ctx, bytes, &p), ctx);
+        gcre->dev_flux = static_cast<double *>(p);
+      }
+      check(hamx_memcpy_h2d(ctx, gcre->dev_flux, gcre->cre_flux.get(), bytes), ctx);
+      gcre->dev_valid = true;
+    }
+    f.cre = gcre->dev_flux;
+    f.cre_box = to_box(par->grid_cre);
+    f.cre_E_min = par->grid_cre.E_min;
+    f.cre_E_fact = par->grid_cre.E_fact;
+    f.cre_nE = static_cast<uint32_t>(par->grid_cre.nE);
+  }
   // every input of one call shares a residency: regular-field grids stay
   // on the host, so the random grids then go by their host copies
   if (any_host_regular) {
@@ -662,7 +717,9 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
     }
     if (f.ternd)
       f.ternd = gternd->te.get();
-  } else if (f.brnd_x || f.ternd) {
+    if (f.cre)
+      f.cre = gcre->cre_flux.get();
+  } else if (f.brnd_x || f.ternd || f.cre) {
     flags |= HAMX_IN_DEVICE;
   }
   // the mask follows the residency of the other inputs
@@ -796,15 +853,22 @@ void Pipeline::assemble_brnd() {
 
 // pipeline.cc:145-163
 void Pipeline::assemble_cre() {
-  if (par->grid_cre.read_permission)
-    throw std::runtime_error("hamx: CRE grid input has no device path yet");
-  if (par->grid_cre.build_permission) {
+  if (par->grid_cre.read_permission) {
+    grid_cre->import_grid(par.get());
+    cre = std::make_unique<CRE_num>();
+  } else if (par->grid_cre.build_permission) {
     if (par->cre_type == "analytic")
       cre = std::make_unique<CRE_ana>();
     else if (par->cre_type == "unif")
       cre = std::make_unique<CRE_unif>();
     else
       throw std::runtime_error("unsupported cre model");
+  }
+  if (par->grid_cre.write_permission) {
+    if (!cre)
+      throw std::runtime_error("hamx: CRE grid output needs a CRE model");
+    cre->write_grid(par.get(), grid_cre.get());
+    grid_cre->export_grid(par.get());
   }
 }
 

@@ -48,6 +48,9 @@ public:
     bool build_permission = false, read_permission = false, write_permission = false;
     double x_max = 0, x_min = 0, y_max = 0, y_min = 0, z_max = 0, z_min = 0;
     unsigned long long nx = 0, ny = 0, nz = 0, full_size = 0;
+    // energy axis of the CRE grid (param.cc:760-776)
+    double E_min = 0, E_max = 0, E_fact = 0;
+    unsigned long long nE = 0, cre_size = 0;
   } grid_breg, grid_brnd, grid_tereg, grid_ternd, grid_cre;
   struct param_obs_grid {
     bool write_permission = false;
@@ -147,10 +150,19 @@ public:
   void invalidate_device() const { dev_valid = false; }
 };
 
+// Grid_cre (grid_cre.cc): the CRE flux [x][y][z][E], kept on the device
+// once uploaded.
 class Grid_cre : public Grid {
 public:
   Grid_cre() = default;
-  explicit Grid_cre(const Param *) {}
+  explicit Grid_cre(const Param *par);
+  ~Grid_cre() override;
+  void build_grid(const Param *) override;
+  void export_grid(const Param *) override;
+  void import_grid(const Param *) override;
+  std::unique_ptr<double[]> cre_flux;
+  mutable double *dev_flux = nullptr;
+  mutable bool dev_valid = false;
 };
 
 // A RING-ordered HEALPix map: nside and npix doubles.
@@ -267,6 +279,13 @@ public:
   virtual ~CREfield() = default;
   // device evaluator: HAMX_CRE_* and its parameters, or -1 (none)
   virtual int device_model(const Param *par, double p[8]) const;
+  // CREfield::write_grid (cre.cc:204-225) on the device
+  virtual void write_grid(const Param *par, Grid_cre *grid) const;
+};
+// CRE_num (crefield.h:132): the imported grid
+class CRE_num : public CREfield {
+public:
+  int device_model(const Param *par, double p[8]) const override;
 };
 class CRE_ana : public CREfield {
 public:

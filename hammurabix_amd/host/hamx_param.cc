@@ -342,6 +342,15 @@ Param::Param(const std::string &xml_file) {
       throw std::runtime_error("unsupported cre model");
     }
   }
+  if (grid_cre.read_permission || grid_cre.write_permission) {
+    const Element *b = trace(root, {"grid", "box_cre"});
+    grid_cre.E_min = units::GeV * fval(b, "E_min");
+    grid_cre.E_max = units::GeV * fval(b, "E_max");
+    grid_cre.nE = uval(b, "nE");
+    grid_cre.E_fact = std::log(grid_cre.E_max / grid_cre.E_min) / (grid_cre.nE - 1);
+    read_box(root, "box_cre", grid_cre);
+    grid_cre.cre_size = grid_cre.nE * grid_cre.full_size;
+  }
 }
 
 } // namespace hamx_host

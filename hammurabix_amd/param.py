@@ -98,6 +98,11 @@ class GridParam:
     y_max: float = 0.0
     z_min: float = 0.0
     z_max: float = 0.0
+    # energy axis of the CRE grid (param.cc:760-776)
+    E_min: float = 0.0
+    E_max: float = 0.0
+    E_fact: float = 0.0
+    nE: int = 0
 
     @property
     def full_size(self) -> int:
@@ -462,6 +467,34 @@ class Param:
                                      j0=s.f("value", "j0"))
             else:
                 raise RuntimeError("unsupported cre model")
+        if g.read_permission or g.write_permission:
+            b = _trace(root, ["grid", "box_cre"])
+            g.E_min = CGS.GeV * b.f("value", "E_min")
+            g.E_max = CGS.GeV * b.f("value", "E_max")
+            g.nE = b.u("value", "nE")
+            g.E_fact = math.log(g.E_max / g.E_min) / (g.nE - 1)
+            _box(root, "box_cre", g)
+
+    def cre_desc(self) -> capi.HamxLosDesc:
+        """The observer and the analytic / uniform CRE model of the parameter
+        set, as hamx_cre_write_grid reads them (CREfield::write_grid)."""
+        d = capi.HamxLosDesc()
+        d.observer[:] = self.observer
+        self._cre_model(d)
+        return d
+
+    def _cre_model(self, d: capi.HamxLosDesc) -> None:
+        if not self.grid_cre.build_permission:
+            d.cre_model = capi.HAMX_CRE_NONE
+        elif self.cre_type == "analytic":
+            d.cre_model = capi.HAMX_CRE_ANA
+            p = self.cre_ana
+            d.cre[0:7] = [p["alpha"], p["beta"], p["theta"], p["r0"], p["z0"], p["E0"],
+                          p["j0"]]
+        elif self.cre_type == "unif":
+            d.cre_model = capi.HAMX_CRE_UNIF
+            p = self.cre_unif
+            d.cre[0:4] = [p["alpha"], p["r0"], p["E0"], p["j0"]]
 
     # ----------------------------------------------------------- descriptors
     def los_desc(self, schedule: int = capi.HAMX_SCHED_AUTO) -> capi.HamxLosDesc:
@@ -525,20 +558,11 @@ class Param:
             d.tereg[0:2] = [self.tereg_unif["n0"], self.tereg_unif["r0"]]
         else:
             raise RuntimeError(f"tereg model '{self.tereg_type}' has no device evaluator")
-        # CRE
+        # CRE: an imported grid takes precedence (integrator.cc:287,356)
         if self.grid_cre.read_permission:
-            raise RuntimeError("CRE grid input has no device path yet")
-        if not self.grid_cre.build_permission:
-            d.cre_model = capi.HAMX_CRE_NONE
-        elif self.cre_type == "analytic":
-            d.cre_model = capi.HAMX_CRE_ANA
-            p = self.cre_ana
-            d.cre[0:7] = [p["alpha"], p["beta"], p["theta"], p["r0"], p["z0"], p["E0"],
-                          p["j0"]]
-        elif self.cre_type == "unif":
-            d.cre_model = capi.HAMX_CRE_UNIF
-            p = self.cre_unif
-            d.cre[0:4] = [p["alpha"], p["r0"], p["E0"], p["j0"]]
+            d.cre_model = capi.HAMX_CRE_GRID
+        else:
+            self._cre_model(d)
         d.schedule = schedule
         return d
 

@@ -132,8 +132,20 @@ class Pipeline:
             export_map(g.filename, inter.ravel())
 
     def assemble_cre(self) -> None:
-        if self.par.grid_cre.read_permission:
-            raise RuntimeError("CRE grid input has no device path yet")
+        """pipeline.cc:145-162: an imported CRE flux grid (CRE_num), or the
+        analytic / uniform model, whose grid is written when asked
+        (CREfield::write_grid, cre.cc:204-225)."""
+        g = self.par.grid_cre
+        self.cre_grid = None
+        if g.read_permission:
+            self.cre_grid = self._torch.from_numpy(
+                import_grid(g.filename, g.full_size * g.nE, 1)).to(self._dev)
+        elif g.write_permission:
+            if not g.build_permission:
+                raise RuntimeError("CRE grid output needs a CRE model")
+            out = self._zeros(g.full_size * g.nE)
+            self.ctx.cre_write_grid(self.par.cre_desc(), g.box(), g.E_min, g.E_fact, g.nE, out)
+            export_map(g.filename, out.cpu().numpy())
 
     def _fields(self) -> dict:
         kw = {}
@@ -147,6 +159,9 @@ class Pipeline:
             kw.update(tereg=self.tereg_grid, tereg_box=self.par.grid_tereg.box())
         if getattr(self, "mask", None) is not None:
             kw.update(mask=self.mask, nside_mask=self.par.grid_obs.nside_mask)
+        if getattr(self, "cre_grid", None) is not None:
+            g = self.par.grid_cre
+            kw.update(cre=self.cre_grid, cre_box=g.box(), cre_E=(g.E_min, g.E_fact, g.nE))
         return kw
 
     def assemble_obs(self) -> None:

@@ -125,13 +125,22 @@ class Context:
                     breg: Optional[tuple[Any, Any, Any]] = None,
                     breg_box: Optional[capi.HamxBox] = None,
                     tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None,
-                    keep_tape: bool = False, mask: Any = None, nside_mask: int = 0) -> None:
+                    keep_tape: bool = False, mask: Any = None, nside_mask: int = 0,
+                    cre: Any = None, cre_box: Optional[capi.HamxBox] = None,
+                    cre_E: Optional[tuple[float, float, int]] = None) -> None:
         """One Integrator::write_grid pass into `maps` (keys dm, fd, is, qs, us).
         keep_tape leaves the pass's tape in the context for los_adjoint; mask
         (a map at nside_mask) restricts the integration to pixels where its
-        duplicate at each shell's nside is 1.0."""
+        duplicate at each shell's nside is 1.0; cre is the CRE flux grid of
+        HAMX_CRE_GRID on cre_box with the energy axis cre_E = (E_min, E_fact,
+        nE), stored [x][y][z][E]."""
         f = capi.HamxFields()
         inputs = []
+        if cre is not None:
+            f.cre = _ptr(cre)[0]
+            f.cre_box = cre_box
+            f.cre_E_min, f.cre_E_fact, f.cre_nE = cre_E
+            inputs.append(cre)
         if mask is not None:
             f.mask = _ptr(mask)[0]
             f.nside_mask = nside_mask
@@ -235,6 +244,25 @@ class Context:
         _order_after_torch([bx, by, bz] + (list(breg_grid) if breg_grid is not None else []))
         self._check(fn(self.handle, C.byref(desc), seed,
                        _ptr(bx)[0], _ptr(by)[0], _ptr(bz)[0], flags))
+
+    def cre_write_grid(self, desc: capi.HamxLosDesc, box: capi.HamxBox, E_min: float,
+                       E_fact: float, nE: int, out: Any) -> None:
+        """CREfield::write_grid (cre.cc:204-225) of the analytic / uniform model
+        in `desc` into `out` ([x][y][z][E], box_size(box) * nE values)."""
+        flags = capi.HAMX_OUT_DEVICE if _residency([out], "output") else 0
+        _order_after_torch([out])
+        self._check(self.lib.hamx_cre_write_grid(self.handle, C.byref(desc), C.byref(box),
+                                                 E_min, E_fact, nE, _ptr(out)[0], flags))
+
+    def sync_kernels(self, x: Any, F: Any, G: Any) -> None:
+        """The device synchrotron kernels F(x) and G(x) for 0 < x <= 100."""
+        n = len(x)
+        flags = capi.HAMX_IN_DEVICE if _residency([x], "input") else 0
+        if _residency([F, G], "output"):
+            flags |= capi.HAMX_OUT_DEVICE
+        _order_after_torch([x, F, G])
+        self._check(self.lib.hamx_sync_kernels(self.handle, _ptr(x)[0], n, _ptr(F)[0],
+                                               _ptr(G)[0], flags))
 
     def ternd_dft(self, desc: capi.HamxDftDesc, seed: int, te: Any) -> None:
         flags = capi.HAMX_OUT_DEVICE if _residency([te], "output") else 0

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define HAMX_ABI_VERSION 1
+#define HAMX_ABI_VERSION 2
 #define HAMX_MAX_SHELL 16
 
 /* ---- error codes ------------------------------------------------------ */
@@ -77,6 +77,9 @@ extern "C" {
 #define HAMX_CRE_NONE 0 /* no CRE model: sync output is rejected */
 #define HAMX_CRE_ANA 1  /* "analytic": cre[0..6] = alpha, beta, theta, r0 [cm], z0 [cm], E0 [erg], j0 */
 #define HAMX_CRE_UNIF 2 /* "unif":     cre[0..3] = alpha, r0 [cm], E0 [erg], j0 */
+#define HAMX_CRE_GRID 3 /* <fieldio><cre read="1">: CRE_num over hamx_fields.cre (cre.cc:144-199);
+                           the synchrotron emissivities integrate the flux over the energy axis
+                           with the kernels F and G (integrator.cc:281-332,350-401) */
 
 /* ---- LOS march schedules ---------------------------------------------- */
 #define HAMX_SCHED_AUTO 0   /* wavefront per ray */
@@ -111,6 +114,15 @@ typedef struct hamx_fields {
   const double *mask;
   uint32_t nside_mask;
   uint32_t reserved;
+  /* Grid_cre::cre_flux (grid.h:167) for HAMX_CRE_GRID: differential flux in
+   * [GeV m^2 s sr]^-1 on the vertex box cre_box and the energy axis
+   * E_i = cre_E_min * exp(i * cre_E_fact) [erg], i < cre_nE, stored
+   * [x][y][z][E] with E fastest (toolkit::index4d as cre.cc:152-199 uses it). */
+  const double *cre;
+  hamx_box cre_box;
+  double cre_E_min, cre_E_fact;
+  uint32_t cre_nE;
+  uint32_t reserved2;
 } hamx_fields;
 
 /* One Integrator::write_grid pass (integrator.cc:23-134) for one <sync>
@@ -332,6 +344,22 @@ void hamx_batch_destroy(hamx_batch *b);
  * Outputs bx, by, bz: nx*ny*nz doubles each. */
 int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
                  double *bx, double *by, double *bz, uint32_t flags);
+
+/* CREfield::write_grid (crefield.h:53, cre.cc:204-225) for the analytic and
+ * uniform models: out[((i*ny + j)*nz + k)*nE + m] = write_field(pos_ijk, E_m)
+ * (cre_ana.cc:52-63, cre_unif.cc:48-59) on the vertex box `box` with
+ * E_m = E_min * exp(m * E_fact).  The model and observer come from `d`
+ * (cre_model HAMX_CRE_ANA or HAMX_CRE_UNIF).  flags: HAMX_OUT_DEVICE. */
+int hamx_cre_write_grid(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_box *box,
+                        double E_min, double E_fact, uint32_t nE, double *out,
+                        uint32_t flags);
+
+/* The synchrotron kernels of the grid emissivities, as the march evaluates
+ * them: F[i] = x int_x^inf K_5/3(t) dt and G[i] = x K_2/3(x) at x[i] in
+ * (0, 100] (replacing gsl_sf_synchrotron_1 / _2, integrator.cc:321,390).
+ * flags: HAMX_IN_DEVICE | HAMX_OUT_DEVICE as for the other calls. */
+int hamx_sync_kernels(hamx_ctx *ctx, const double *x, uint64_t n, double *F, double *G,
+                      uint32_t flags);
 
 /* Brnd_jf12::write_grid (bfield.h:329-330, brnd_jf12.cc:127-385): the same steps as Brnd_es with
  * the JF12 energy profile (d->jf12; d->r0 / z0 unused) and the reference's

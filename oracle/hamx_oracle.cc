@@ -706,6 +706,103 @@ double sync_p(const Cre &cre, double freq, const V3 &pos, double bper) {
                  std::tgamma(-0.25 * index - 1. / 12.));
 }
 
+// ------------------------------------------------ CRE grid (CRE_num) --
+// The synchrotron kernels the reference takes from GSL
+// (gsl_sf_synchrotron_1 / _2, integrator.cc:321,390), restated from their
+// definitions F(x) = x int_x^inf K_5/3(t) dt and G(x) = x K_2/3(x) through
+// K_nu(t) = int_0^inf exp(-t cosh s) cosh(nu s) ds, so that
+//   F(x) = x int_0^inf cosh(5s/3) / cosh(s) exp(-x cosh s) ds,
+//   G(x) = x int_0^inf cosh(2s/3) exp(-x cosh s) ds,
+// each summed by the trapezoid rule.  The integrands are analytic in a strip
+// about the real axis, so the rule converges exponentially; the step shrinks
+// as 1/sqrt(1+x) with the width of the peak at s = 0, and the sum stops past
+// the peak once a term is below 1e-18 of it.
+double sync_kernel(double x, double nu, bool over_cosh) {
+  const double h = 0.05 / std::sqrt(1. + x);
+  double sum = 0.5 * std::exp(-x);
+  for (int k = 1; k < 4000000; ++k) {
+    const double s = k * h;
+    const double c = std::cosh(s);
+    double w = std::cosh(nu * s) * std::exp(-x * c);
+    if (over_cosh)
+      w /= c;
+    sum += w;
+    if (x * c > 1. && w < 1e-18 * sum)
+      break;
+  }
+  return x * h * sum;
+}
+double sync_F(double x) { return sync_kernel(x, 5. / 3., true); }
+double sync_G(double x) { return sync_kernel(x, 2. / 3., false); }
+
+struct CreGrid {
+  const double *f;
+  hamx_box box;
+  double E_min, E_fact;
+  uint64_t nE;
+  // CREfield::read_grid_num (cre.cc:144-199): the trilinear gather of energy
+  // bin e, with toolkit::index4d (toolkit.h:38-44) called as
+  // index4d(nx, ny, nz, nE, x, y, z, e) = ((x * ny + y) * nz + z) * nE + e
+  // (the first extent is unused).
+  double read_num(const V3 &pos, uint64_t e) const {
+    uint64_t idx;
+    double xd, yd, zd;
+    if (!cell(box, pos, idx, xd, yd, zd))
+      return 0.;
+    const uint64_t xl = idx / (box.ny * box.nz), yl = (idx / box.nz) % box.ny, zl = idx % box.nz;
+    auto at = [&](uint64_t x, uint64_t y, uint64_t z) {
+      return f[x * box.ny * box.nz * nE + y * box.nz * nE + z * nE + e];
+    };
+    const double i1 = at(xl, yl, zl) * (1. - zd) + at(xl, yl, zl + 1) * zd;
+    const double i2 = at(xl, yl + 1, zl) * (1 - zd) + at(xl, yl + 1, zl + 1) * zd;
+    const double j1 = at(xl + 1, yl, zl) * (1 - zd) + at(xl + 1, yl, zl + 1) * zd;
+    const double j2 = at(xl + 1, yl + 1, zl) * (1 - zd) + at(xl + 1, yl + 1, zl + 1) * zd;
+    const double w1 = i1 * (1 - yd) + i2 * yd;
+    const double w2 = j1 * (1 - yd) + j2 * yd;
+    return w1 * (1 - xd) + w2 * xd;
+  }
+};
+
+// sync_emissivity_t / _p grid branch (integrator.cc:286-325 and 355-394):
+// the midpoint rule over the energy bins, skipping bins whose mean x
+// exceeds 100, with the flux over beta = sqrt(1 - mec2/E) averaged at the bin
+// ends; `pol` selects G and the J_pol form of x_fact.
+double sync_grid(const CreGrid &g, double freq, const V3 &pos, double bper, bool pol) {
+  const uint64_t nE = g.nE;
+  std::vector<double> KE(nE), x(nE), beta(nE);
+  const double x_fact =
+      pol ? (2. * kMec * kMec2 * kMec2 * 2. * kPi * freq) / (3. * kQe * std::fabs(bper))
+          : 4. * kPi * freq * kMec * kMec2 * kMec2 / (3. * kQe * std::fabs(bper));
+  for (uint64_t i = 0; i != nE; ++i) {
+    KE[i] = g.E_min * std::exp(i * g.E_fact);
+    x[i] = x_fact / (KE[i] * KE[i]);
+    beta[i] = std::sqrt(1 - kMec2 / KE[i]);
+  }
+  double J = 0;
+  for (uint64_t i = 0; i + 1 < nE; ++i) {
+    const double xv = 0.5 * (x[i + 1] + x[i]);
+    if (xv > 100)
+      continue;
+    const double dE = std::fabs(KE[i + 1] - KE[i]);
+    const double flux = 0.5 * (g.read_num(pos, i + 1) / beta[i + 1] + g.read_num(pos, i) / beta[i]);
+    J += (pol ? sync_G(xv) : sync_F(xv)) * flux * dE;
+  }
+  const double fore_factor = 1.73205081 * kQe * kQe * kQe * std::fabs(bper) /
+                             (kMec2 * kC * kGeV * kM * kM * kSec);
+  return J * fore_factor;
+}
+
+// CRE_ana / CRE_unif::write_field (cre_ana.cc:52-63, cre_unif.cc:48-59)
+double cre_write_field(const Cre &cre, const V3 &pos, double En) {
+  const double E0 = cre.model == HAMX_CRE_UNIF ? cre.p[2] : cre.p[5];
+  const double j0 = cre.model == HAMX_CRE_UNIF ? cre.p[3] : cre.p[6];
+  const double gamma = En / kMec2;
+  const double gamma0 = E0 / kMec2;
+  const double beta_ratio = std::sqrt((1. - 1. / gamma) / (1. - 1. / gamma0));
+  const double norm = j0 * std::pow(gamma0, -cre.flux_idx(cre.obs));
+  return norm * beta_ratio * std::pow(gamma, cre.flux_idx(pos)) * cre.spatial_profile(pos);
+}
+
 // ----------------------------------------------------------- integrator --
 V3 los_versor(double the, double phi) {
   return v3(std::cos(phi) * std::sin(the), std::sin(phi) * std::sin(the),
@@ -762,6 +859,9 @@ void radial(const hamx_los_desc *d, const hamx_fields *f, const Shell &sh,
   cre.model = d->cre_model;
   cre.p = d->cre;
   cre.obs = obs;
+  CreGrid cg{};
+  if (d->cre_model == HAMX_CRE_GRID)
+    cg = CreGrid{f->cre, f->cre_box, f->cre_E_min, f->cre_E_fact, f->cre_nE};
   const bool has_brnd = f && f->brnd_x && f->brnd_y && f->brnd_z;
   for (uint64_t k = 0; k < sh.step; ++k) {
     const V3 oc = mul(l, sh.dist[k]);
@@ -792,8 +892,13 @@ void radial(const hamx_los_desc *d, const hamx_fields *f, const Shell &sh,
     if (do_fd_any)
       fd += te * b_par * fd_fore * sh.delta;
     if (d->do_sync) {
-      const double jt = sync_t(cre, d->sync_freq, pos, b_per) * sh.delta * i2bt;
-      const double jp = sync_p(cre, d->sync_freq, pos, b_per) * sh.delta * i2bt;
+      const bool grid = d->cre_model == HAMX_CRE_GRID;
+      const double jt = (grid ? sync_grid(cg, d->sync_freq, pos, b_per, false)
+                              : sync_t(cre, d->sync_freq, pos, b_per)) *
+                        sh.delta * i2bt;
+      const double jp = (grid ? sync_grid(cg, d->sync_freq, pos, b_per, true)
+                              : sync_p(cre, d->sync_freq, pos, b_per)) *
+                        sh.delta * i2bt;
       is += jt;
       const double qui = (inner + fd) * lambda_square + sync_ipa(B, the, phi);
       qs += std::cos(2. * qui) * jp;
@@ -903,6 +1008,54 @@ void orc_sync_emissivity(const hamx_los_desc *d, const double pos[3],
   *j_pol = sync_p(cre, d->sync_freq, p, b_per);
 }
 
+void orc_sync_kernels(const double *x, uint64_t n, double *F, double *G) {
+  for (uint64_t i = 0; i < n; ++i) {
+    F[i] = sync_F(x[i]);
+    G[i] = sync_G(x[i]);
+  }
+}
+
+void orc_sync_emissivity_grid(const hamx_los_desc *d, const hamx_fields *f,
+                              const double pos[3], double b_per, double *j_tot,
+                              double *j_pol) {
+  const CreGrid g{f->cre, f->cre_box, f->cre_E_min, f->cre_E_fact, f->cre_nE};
+  const V3 p = v3(pos[0], pos[1], pos[2]);
+  *j_tot = sync_grid(g, d->sync_freq, p, b_per, false);
+  *j_pol = sync_grid(g, d->sync_freq, p, b_per, true);
+}
+
+double orc_cre_read_grid_num(const hamx_fields *f, const double pos[3], uint64_t e) {
+  const CreGrid g{f->cre, f->cre_box, f->cre_E_min, f->cre_E_fact, f->cre_nE};
+  return g.read_num(v3(pos[0], pos[1], pos[2]), e);
+}
+
+// CREfield::write_grid (cre.cc:204-225)
+int orc_cre_write_grid(const hamx_los_desc *d, const hamx_box *box, double E_min,
+                       double E_fact, uint32_t nE, double *out) {
+  if (d->cre_model != HAMX_CRE_ANA && d->cre_model != HAMX_CRE_UNIF)
+    return 1;
+  Cre cre;
+  cre.model = d->cre_model;
+  cre.p = d->cre;
+  cre.obs = v3(d->observer[0], d->observer[1], d->observer[2]);
+  const double lx = box->x_max - box->x_min;
+  const double ly = box->y_max - box->y_min;
+  const double lz = box->z_max - box->z_min;
+  for (uint64_t i = 0; i != box->nx; ++i) {
+    for (uint64_t j = 0; j != box->ny; ++j) {
+      for (uint64_t k = 0; k != box->nz; ++k) {
+        const V3 pos = v3(lx * i / (box->nx - 1) + box->x_min, ly * j / (box->ny - 1) + box->y_min,
+                          lz * k / (box->nz - 1) + box->z_min);
+        for (uint64_t m = 0; m != nE; ++m) {
+          const double E = E_min * std::exp(m * E_fact);
+          out[((i * box->ny + j) * box->nz + k) * nE + m] = cre_write_field(cre, pos, E);
+        }
+      }
+    }
+  }
+  return 0;
+}
+
 void orc_radial_integration(const hamx_los_desc *d, const hamx_fields *f,
                             uint32_t shell, double theta, double phi,
                             double fd_inner, double out[5]) {
@@ -953,6 +1106,8 @@ int orc_write_grid(const hamx_los_desc *d, const hamx_fields *f, hamx_maps *out,
     if (!pow2(d->nside_shell[s]))
       return HAMX_ERR_NSIDE;
   if (d->do_sync && d->cre_model == HAMX_CRE_NONE)
+    return HAMX_ERR_ARG;
+  if (d->cre_model == HAMX_CRE_GRID && (!f || !f->cre || f->cre_nE < 2))
     return HAMX_ERR_ARG;
 #ifdef _OPENMP
   if (nthreads > 0)

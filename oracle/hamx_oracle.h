@@ -73,6 +73,13 @@ void orc_gauss_pairs(uint64_t seed, uint32_t stream, uint64_t first_pair,
                      uint64_t count, double *out);
 void orc_fft3d(double *data, uint64_t nx, uint64_t ny, uint64_t nz, int sign);
 double orc_variance(const double *arr, uint64_t n);
+void orc_sync_kernels(const double *x, uint64_t n, double *F, double *G);
+void orc_sync_emissivity_grid(const hamx_los_desc *d, const hamx_fields *f,
+                              const double pos[3], double b_per, double *j_tot,
+                              double *j_pol);
+double orc_cre_read_grid_num(const hamx_fields *f, const double pos[3], uint64_t e);
+int orc_cre_write_grid(const hamx_los_desc *d, const hamx_box *box, double E_min,
+                       double E_fact, uint32_t nE, double *out);
 int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
                 double *bz);
 int orc_brnd_jf12(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,

@@ -43,6 +43,12 @@ PROTOS = {
     "orc_fft3d": (None, [_vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]),
     "orc_variance": (C.c_double, [_vp, C.c_uint64]),
     "orc_brnd_es": (C.c_int, [C.POINTER(capi.HamxEsDesc), C.c_uint64, _vp, _vp, _vp]),
+    "orc_sync_kernels": (None, [_vp, C.c_uint64, _vp, _vp]),
+    "orc_sync_emissivity_grid": (None, [C.POINTER(capi.HamxLosDesc), C.POINTER(capi.HamxFields),
+                                        _vp, C.c_double, _dp, _dp]),
+    "orc_cre_read_grid_num": (C.c_double, [C.POINTER(capi.HamxFields), _vp, C.c_uint64]),
+    "orc_cre_write_grid": (C.c_int, [C.POINTER(capi.HamxLosDesc), C.POINTER(capi.HamxBox),
+                                     C.c_double, C.c_double, C.c_uint32, _vp]),
     "orc_brnd_jf12": (C.c_int, [C.POINTER(capi.HamxEsDesc), C.c_uint64, _vp, _vp, _vp]),
     "orc_jf12_profile": (C.c_double, [_vp, _vp]),
     "orc_ternd_dft": (C.c_int, [C.POINTER(capi.HamxDftDesc), C.c_uint64, _vp]),
@@ -80,8 +86,12 @@ def ptr(a):
 
 def fields_struct(brnd=None, brnd_box=None, ternd=None, ternd_box=None,
                   breg=None, breg_box=None, tereg=None, tereg_box=None,
-                  mask=None, nside_mask=0):
+                  mask=None, nside_mask=0, cre=None, cre_box=None, cre_E=None):
     f = capi.HamxFields()
+    if cre is not None:
+        f.cre = ptr(cre)
+        f.cre_box = cre_box
+        f.cre_E_min, f.cre_E_fact, f.cre_nE = cre_E
     if mask is not None:
         f.mask = ptr(mask)
         f.nside_mask = nside_mask

@@ -68,13 +68,13 @@ def test_profile_cutoffs_inner_disk_and_halo(orc):
     r = math.hypot(x, y)
     disk = 7.63 * math.exp(-0.5 * (z / 0.61) ** 2)
     halo = 4.68 * math.exp(-r / 10.97) * math.exp(-z / 2.84)
-    assert prof(orc, q, [x, y, z]) == pytest.approx(disk ** 2 + halo ** 2, rel=1e-13)
+    assert prof(orc, q, [x, y, z]) == pytest.approx(disk ** 2 + halo ** 2, rel=1e-13, abs=0)
     # halo alone, both signs of z
     qh = template(**{f"b0_{i}": 0.0 for i in range(1, 9)}, b0_int=0.0)
     for pos in ([3.0, 4.0, -1.5], [-9.0, 6.0, 2.5], [0.2, 0.1, 1.2]):
         r = math.hypot(pos[0], pos[1])
         want = (4.68 * math.exp(-r / 10.97) * math.exp(-abs(pos[2]) / 2.84)) ** 2
-        assert prof(orc, qh, pos) == pytest.approx(want, rel=1e-13)
+        assert prof(orc, qh, pos) == pytest.approx(want, rel=1e-13, abs=0)
 
 
 ARMS = (10.81, 6.96, 9.59, 6.96, 1.96, 16.34, 37.29, 10.35)
@@ -89,11 +89,11 @@ def _disk_only():
 def test_profile_regions_on_the_negative_x_axis(orc, rx, want):
     assert region(rx) == want
     got = prof(orc, _disk_only(), [-rx, 0.0, 0.0])
-    assert got == pytest.approx((ARMS[want] * 5.0 / rx) ** 2, rel=1e-13)
+    assert got == pytest.approx((ARMS[want] * 5.0 / rx) ** 2, rel=1e-13, abs=0)
     z = 0.35
     got = prof(orc, _disk_only(), [-rx, 0.0, z])
     assert got == pytest.approx((ARMS[want] * 5.0 / rx * math.exp(-0.5 * (z / 0.61) ** 2)) ** 2,
-                                rel=1e-13)
+                                rel=1e-13, abs=0)
 
 
 @pytest.mark.parametrize("rx,dphi", [(12.0, math.pi / 2), (12.0, math.pi), (15.0, 1.5 * math.pi),
@@ -106,7 +106,7 @@ def test_profile_constant_region_along_the_spiral(orc, rx, dphi):
     r = rx * math.exp(-TANP * dphi)
     assert r > 5.0
     got = prof(orc, _disk_only(), [r * math.cos(phi), r * math.sin(phi), 0.0])
-    assert got == pytest.approx((ARMS[region(rx)] * 5.0 / r) ** 2, rel=1e-10)
+    assert got == pytest.approx((ARMS[region(rx)] * 5.0 / r) ** 2, rel=1e-10, abs=0)
 
 
 def test_profile_wraps_to_inner_crossings(orc):
@@ -118,12 +118,12 @@ def test_profile_wraps_to_inner_crossings(orc):
     r_in = 10.0 * math.exp(-TANP * math.pi / 2)
     assert 10.0 * math.exp(TANP * 1.5 * math.pi) > 15.5
     assert prof(orc, q, [0.0, -10.0, 0.0]) == pytest.approx(
-        (ARMS[region(r_in)] * 5.0 / 10.0) ** 2, rel=1e-13)
+        (ARMS[region(r_in)] * 5.0 / 10.0) ** 2, rel=1e-13, abs=0)
     assert region(r_in) == 3
     # (-16, 0): on the axis but beyond 15.5 kpc; one turn inward
     r_in = 16.0 * math.exp(-TANP * 2 * math.pi)
     assert prof(orc, q, [-16.0, 0.0, 0.0]) == pytest.approx(
-        (ARMS[region(r_in)] * 5.0 / 16.0) ** 2, rel=1e-13)
+        (ARMS[region(r_in)] * 5.0 / 16.0) ** 2, rel=1e-13, abs=0)
     assert region(r_in) == 0
     # just below the negative x-axis (phi ~ -pi) at r = 17: the crossings at
     # phi = pi and -pi are both beyond 15.5 kpc, the one a turn further in is not
@@ -132,7 +132,7 @@ def test_profile_wraps_to_inner_crossings(orc):
     r_in = r * math.exp(-TANP * (phi + 3 * math.pi))
     assert r * math.exp(-TANP * (phi + math.pi)) > 15.5
     got = prof(orc, q, [r * math.cos(phi), r * math.sin(phi), 0.0])
-    assert got == pytest.approx((ARMS[region(r_in)] * 5.0 / r) ** 2, rel=1e-12)
+    assert got == pytest.approx((ARMS[region(r_in)] * 5.0 / r) ** 2, rel=1e-12, abs=0)
 
 
 def _es_desc(shape, rho):
@@ -171,7 +171,7 @@ def test_generator_is_es_with_its_constants(orc, rho):
         assert rms > 0 and abs(b.mean()) > 1e-5 * rms
         assert rel_l2(a - a.mean(), fluct * factor) < 1e-13
         # the mean differs from the fluctuation's factor by 1.4e-9
-        assert a.mean() == pytest.approx(b.mean() * f_mean, rel=1e-10)
+        assert a.mean() == pytest.approx(b.mean() * f_mean, rel=1e-10, abs=0)
         assert rel_l2(a, 2 * b) > 0.5 * abs(factor / 2 - 1)
 
 
@@ -186,5 +186,5 @@ def test_param_reads_the_jf12_random_block():
     p = Param(text=text.replace(block, jf))
     assert p.brnd_method == "jf12"
     d = p.es_desc()
-    assert d.rms == pytest.approx(1.5 * MU) and d.k0 == 10.0 and d.rho == 0.8
+    assert d.rms == pytest.approx(1.5 * MU, rel=1e-15, abs=0) and d.k0 == 10.0 and d.rho == 0.8
     np.testing.assert_allclose(np.array(d.jf12), template(), rtol=1e-15)

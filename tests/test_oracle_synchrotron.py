@@ -1,0 +1,272 @@
+"""The CRE grid path of the oracle (CRE_num, integrator.cc:281-401,
+cre.cc:144-225) pinned without GSL and without the reference build.
+
+* The synchrotron kernels F(x) = x int_x^inf K_5/3 and G(x) = x K_2/3(x)
+  that replace gsl_sf_synchrotron_1 / _2: against scipy.special.kv (G
+  directly, F through scipy.integrate.quad of K_5/3), the closed-form
+  moments int_0^inf x^mu F dx = 2^(mu+1)/(mu+2) Gamma(mu/2 + 7/3)
+  Gamma(mu/2 + 2/3) and int_0^inf x^mu G dx = 2^mu Gamma(mu/2 + 4/3)
+  Gamma(mu/2 + 2/3), and both asymptotes.
+* The device tables (hamx_synchrotron.h): regenerated from their script
+  they are unchanged, and their Clenshaw evaluation, restated here, matches
+  the oracle's kernels to 2e-13.
+* CREfield::read_grid_num: exact on fields linear in position, per energy.
+* CRE_ana / CRE_unif::write_field on the grid: j0 at E0 at the observer, the
+  power law in gamma with the sqrt(1 - 1/gamma) ratio, zero outside the
+  uniform sphere.
+* The grid emissivities converge to the analytic ones: a CRE_unif flux
+  written on a fine energy grid and integrated with F and G gives the
+  Gamma-function emissivities of the analytic branch (integrator.cc:333-346,
+  402-415) to the midpoint rule's O(1/nE^2), within 1e-3 at nE = 400 and
+  1.5e-4 at nE = 1600.  This ties the grid path's x_fact, fore factor,
+  flux / beta and units to the analytic path, which has its own pins."""
+import ctypes as C
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+from hammurabix_amd import _capi as capi
+from hammurabix_amd.units import CGS
+from oracle_lib import fields_struct, ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "hammurabix_amd", "csrc")
+
+
+def kernels(orc, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    F, G = np.zeros_like(x), np.zeros_like(x)
+    orc.orc_sync_kernels(ptr(x), len(x), ptr(F), ptr(G))
+    return F, G
+
+
+def F_scipy(x):
+    kv = lambda t: special.kv(5 / 3, t)  # noqa: E731
+    if x < 1:
+        a, _ = integrate.quad(lambda s: kv(math.exp(s)) * math.exp(s), math.log(x), 0.,
+                              epsabs=0, epsrel=2e-14, limit=500)
+        b, _ = integrate.quad(kv, 1., np.inf, epsabs=0, epsrel=2e-14, limit=500)
+        return x * (a + b)
+    v, _ = integrate.quad(kv, x, np.inf, epsabs=0, epsrel=2e-14, limit=500)
+    return x * v
+
+
+def test_kernels_against_scipy(orc):
+    xs = np.geomspace(1e-12, 100., 41)
+    F, G = kernels(orc, xs)
+    for x, f, g in zip(xs, F, G):
+        assert g == pytest.approx(x * special.kv(2 / 3, x), rel=1e-13, abs=0), x
+        assert f == pytest.approx(F_scipy(x), rel=1e-12, abs=0), x
+
+
+@pytest.mark.parametrize("mu", [0.0, 1.0, 2.5])
+def test_kernel_moments(orc, mu):
+    def moment(which):
+        def f(s):  # x = e^s
+            x = math.exp(s)
+            F, G = kernels(orc, [x])
+            return x ** (mu + 1) * (F[0] if which == "F" else G[0])
+        v, _ = integrate.quad(f, math.log(1e-14), math.log(120.), epsabs=0, epsrel=1e-11,
+                              limit=400)
+        return v
+    want_f = 2 ** (mu + 1) / (mu + 2) * math.gamma(mu / 2 + 7 / 3) * math.gamma(mu / 2 + 2 / 3)
+    want_g = 2 ** mu * math.gamma(mu / 2 + 4 / 3) * math.gamma(mu / 2 + 2 / 3)
+    assert moment("F") == pytest.approx(want_f, rel=1e-9, abs=0)
+    assert moment("G") == pytest.approx(want_g, rel=1e-9, abs=0)
+
+
+def test_kernel_asymptotes(orc):
+    c = 2 ** (2 / 3) * math.gamma(2 / 3)
+    F, G = kernels(orc, [1e-24, 1e-18])
+    for x, f, g in zip((1e-24, 1e-18), F, G):
+        assert f == pytest.approx(c * x ** (1 / 3), rel=max(3 * x ** (2 / 3), 1e-14), abs=0)
+        assert g == pytest.approx(c / 2 * x ** (1 / 3), rel=max(3 * x ** (2 / 3), 1e-14), abs=0)
+    F, G = kernels(orc, [100.])
+    lead = math.sqrt(math.pi * 100. / 2) * math.exp(-100.)
+    assert F[0] == pytest.approx(lead * (1 + 55 / 7200), rel=2e-4, abs=0)
+    assert G[0] == pytest.approx(lead * (1 + 7 / 7200), rel=2e-4, abs=0)
+
+
+def test_device_tables_are_current(tmp_path):
+    out = tmp_path / "t.h"
+    subprocess.run([sys.executable, os.path.join(CSRC, "gen_hamx_synchrotron.py"), str(out)],
+                   check=True, capture_output=True, timeout=300)
+    with open(os.path.join(CSRC, "hamx_synchrotron.h")) as f:
+        assert out.read_text() == f.read()
+
+
+def _parse_tables():
+    """kF / kG and the constants from the committed header."""
+    text = open(os.path.join(CSRC, "hamx_synchrotron.h")).read()
+    consts = {}
+    for line in text.splitlines():
+        if line.startswith("constexpr"):
+            name, val = line.split("=")[0].split()[-1], line.split("=")[1].split(";")[0]
+            consts[name] = float(val)
+    tabs = {}
+    for name in ("kF", "kG"):
+        body = text.split(f"{name}[kSeg][kCoef] = {{")[1].split("};")[0]
+        rows = [r.strip().strip(",").strip("{}") for r in body.strip().splitlines()]
+        tabs[name] = np.array([[float(v) for v in r.split(",")] for r in rows])
+    return consts, tabs
+
+
+def _eval_table(consts, tab, lnc, x):
+    """The device evaluator of hamx_sync.h restated: segment, Clenshaw, exp."""
+    u = math.log(x)
+    dtop = consts["kUTop"] - u
+    fine = consts["kFineN"] * consts["kFineW"]
+    if dtop >= fine + consts["kCoarseN"] * consts["kCoarseW"]:
+        return math.exp(lnc + u / 3.)
+    if dtop < fine:
+        s = min(int(dtop / consts["kFineW"]) if dtop > 0 else 0, int(consts["kFineN"]) - 1)
+        w, deg = consts["kFineW"], int(consts["kFineDeg"])
+        lo = consts["kUTop"] - (s + 1) * w
+    else:
+        sc = min(int((dtop - fine) / consts["kCoarseW"]), int(consts["kCoarseN"]) - 1)
+        s = int(consts["kFineN"]) + sc
+        w, deg = consts["kCoarseW"], int(consts["kCoarseDeg"])
+        lo = consts["kUTop"] - fine - (sc + 1) * w
+    t = 2. * (u - lo) / w - 1.
+    b1 = b2 = 0.
+    for k in range(deg, 0, -1):
+        b1, b2 = tab[s][k] + 2. * t * b1 - b2, b1
+    return math.exp(tab[s][0] + t * b1 - b2 - x)
+
+
+def test_table_evaluation_matches_oracle(orc):
+    consts, tabs = _parse_tables()
+    xs = np.concatenate([np.geomspace(1e-25, 100., 400), [99.999, 100.0, 1.0, math.e]])
+    F, G = kernels(orc, xs)
+    for x, f, g in zip(xs, F, G):
+        tf = _eval_table(consts, tabs["kF"], consts["kLnF0"], x)
+        tg = _eval_table(consts, tabs["kG"], consts["kLnG0"], x)
+        assert tf == pytest.approx(f, rel=2e-13, abs=0), x
+        assert tg == pytest.approx(g, rel=2e-13, abs=0), x
+
+
+KPC = CGS.kpc
+OBS = [-8.3 * KPC, 0.0, 0.006 * KPC]
+BOX = capi.HamxBox(5, 4, 3, -10 * KPC, -6 * KPC, -2 * KPC, 2 * KPC, -1 * KPC, 1 * KPC)
+
+
+def test_read_grid_num_exact_on_linear_fields(orc):
+    nE = 3
+    n = 5 * 4 * 3
+    xs = np.linspace(-10, -6, 5)[:, None, None, None]
+    ys = np.linspace(-2, 2, 4)[None, :, None, None]
+    zs = np.linspace(-1, 1, 3)[None, None, :, None]
+    e = np.arange(nE)[None, None, None, :]
+    flux = np.ascontiguousarray((1.0 + 0.3 * xs - 0.7 * ys + 2.1 * zs) * (e + 1.5)).ravel()
+    assert flux.size == n * nE
+    f = fields_struct(cre=flux, cre_box=BOX, cre_E=(CGS.GeV, 0.5, nE))
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        p = np.array([rng.uniform(-9.9, -6.1), rng.uniform(-1.9, 1.9), rng.uniform(-0.9, 0.9)])
+        for k in range(nE):
+            got = orc.orc_cre_read_grid_num(C.byref(f), ptr(p * KPC), k)
+            want = (1.0 + 0.3 * p[0] - 0.7 * p[1] + 2.1 * p[2]) * (k + 1.5)
+            assert got == pytest.approx(want, rel=1e-12, abs=1e-12)
+    # on a face and outside: zero (cre.cc:151-166)
+    for p in ([-10., 0., 0.], [-5., 0., 0.], [-8., 2., 0.]):
+        assert orc.orc_cre_read_grid_num(C.byref(f), ptr(np.array(p) * KPC), 1) == 0.0
+
+
+def _desc(model):
+    d = capi.HamxLosDesc()
+    d.observer[:] = OBS
+    d.cre_model = model
+    if model == capi.HAMX_CRE_UNIF:
+        d.cre[0:4] = [3.0, 5 * KPC, 10 * CGS.GeV, 0.0217]
+    else:
+        d.cre[0:7] = [3.0, 0.1, 0.05, 5 * KPC, 1 * KPC, 10 * CGS.GeV, 0.0217]
+    d.sync_freq = 23e9
+    return d
+
+
+def _write(orc, d, box, E_min, E_fact, nE):
+    out = np.zeros(box.nx * box.ny * box.nz * nE)
+    assert orc.orc_cre_write_grid(C.byref(d), C.byref(box), E_min, E_fact, nE, ptr(out)) == 0
+    return out.reshape(box.nx, box.ny, box.nz, nE)
+
+
+def test_write_grid_known_answers(orc):
+    mec2 = 0.51099907e-3 * CGS.GeV
+    E0 = 10 * CGS.GeV
+    # the observer is a vertex of this box; the energy axis starts at E0
+    box = capi.HamxBox(3, 3, 3, -9.3 * KPC, -7.3 * KPC, -1 * KPC, 1 * KPC,
+                       -0.994 * KPC, 1.006 * KPC)
+    for model in (capi.HAMX_CRE_UNIF, capi.HAMX_CRE_ANA):
+        d = _desc(model)
+        g = _write(orc, d, box, E0, 0.7, 4)
+        obs_val = g[1, 1, 1]
+        prof = 1.0 if model == capi.HAMX_CRE_UNIF else 1 / math.cosh(0.006 / 1.0) ** 2
+        assert obs_val[0] == pytest.approx(0.0217 * prof, rel=1e-13, abs=0)
+        g0 = E0 / mec2
+        # the spectral index at the observer: -alpha (+ beta r + theta |z| in kpc)
+        idx = -3.0 if model == capi.HAMX_CRE_UNIF else -3.0 + 0.1 * 8.3 + 0.05 * 0.006
+        for m in range(1, 4):
+            gm = E0 * math.exp(m * 0.7) / mec2
+            want = 0.0217 * prof * (gm / g0) ** idx * math.sqrt((1 - 1 / gm) / (1 - 1 / g0))
+            assert obs_val[m] == pytest.approx(want, rel=1e-12, abs=0)
+    # CRE_unif vanishes beyond r0 = 5 kpc from the observer
+    far = capi.HamxBox(2, 2, 2, 0.0, 1 * KPC, 0.0, 1 * KPC, 0.0, 1 * KPC)
+    assert np.all(_write(orc, _desc(capi.HAMX_CRE_UNIF), far, E0, 0.5, 3) == 0.0)
+
+
+@pytest.mark.parametrize("nE,tol", [(400, 1.5e-3), (1600, 1.5e-4)])
+def test_grid_emissivity_converges_to_analytic(orc, nE, tol):
+    d = _desc(capi.HAMX_CRE_UNIF)
+    box = capi.HamxBox(4, 4, 4, -10 * KPC, -6 * KPC, -2 * KPC, 2 * KPC, -2 * KPC, 2 * KPC)
+    E_min, E_max = 0.01 * CGS.GeV, 1e4 * CGS.GeV
+    E_fact = math.log(E_max / E_min) / (nE - 1)
+    flux = np.ascontiguousarray(_write(orc, d, box, E_min, E_fact, nE).ravel())
+    f = fields_struct(cre=flux, cre_box=box, cre_E=(E_min, E_fact, nE))
+    pos = np.array([-8.0, 0.3, 0.1]) * KPC
+    jt, jp, at, ap = (C.c_double() for _ in range(4))
+    for b in (1e-6, 5e-6):
+        orc.orc_sync_emissivity_grid(C.byref(d), C.byref(f), ptr(pos), b, C.byref(jt),
+                                     C.byref(jp))
+        orc.orc_sync_emissivity(C.byref(d), ptr(pos), b, C.byref(at), C.byref(ap))
+        assert jt.value / at.value == pytest.approx(1.0, abs=tol)
+        assert jp.value / ap.value == pytest.approx(1.0, abs=tol)
+
+
+def test_grid_emissivity_one_bin_known_answer(orc):
+    """Two energies, a constant flux: J = fore F(xv) (c/beta_1 + c/beta_0)/2 dE
+    with the reference's beta = sqrt(1 - mec2/E) (integrator.cc:304, not the
+    physical sqrt(1 - 1/gamma^2)): at E = 4/3 and 2 mec2 it is 1/2 and
+    1/sqrt 2.  F comes from scipy; the bin is integrated while its mean x is
+    at most 100 and skipped beyond (integrator.cc:312-315)."""
+    mec2 = 0.51099907e-3 * CGS.GeV
+    mec = mec2 / 2.99792458e10
+    qe = 4.80320425e-10
+    E0, E1 = 4 / 3 * mec2, 2 * mec2
+    d = _desc(capi.HAMX_CRE_UNIF)
+    box = capi.HamxBox(2, 2, 2, -9 * KPC, -7 * KPC, -1 * KPC, 1 * KPC, -1 * KPC, 1 * KPC)
+    c = 0.37
+    flux = np.full(8 * 2, c)
+    f = fields_struct(cre=flux, cre_box=box, cre_E=(E0, math.log(E1 / E0), 2))
+    pos = np.array([-8.0, 0.2, 0.1]) * KPC
+    nu = d.sync_freq
+    fore_b = math.sqrt(3) * qe ** 3 / (mec2 * 2.99792458e10 * CGS.GeV * 1e4)
+    for xv_target, on in ((50.0, True), (100.5, False)):
+        # B such that the bin's mean x is xv_target
+        mean_inv_e2 = 0.5 * (1 / E0 ** 2 + 1 / E1 ** 2)
+        b = 4 * math.pi * nu * mec * mec2 ** 2 * mean_inv_e2 / (3 * qe * xv_target)
+        jt, jp = C.c_double(), C.c_double()
+        orc.orc_sync_emissivity_grid(C.byref(d), C.byref(f), ptr(pos), b, C.byref(jt),
+                                     C.byref(jp))
+        if not on:
+            assert jt.value == 0.0 and jp.value == 0.0
+            continue
+        flux_mean = 0.5 * (c / math.sqrt(0.5) + c / 0.5)
+        want_t = fore_b * b * F_scipy(xv_target) * flux_mean * (E1 - E0)
+        want_p = fore_b * b * xv_target * special.kv(2 / 3, xv_target) * flux_mean * (E1 - E0)
+        assert jt.value == pytest.approx(want_t, rel=1e-8, abs=0)
+        assert jp.value == pytest.approx(want_p, rel=1e-8, abs=0)
