@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", choices=("random_forward", "batch", "large"),
+    ap.add_argument("--workload", choices=("random_forward", "batch", "large", "cre_grid"),
                     default="random_forward",
                     help="random_forward: the plain FP64 path on one realisation (default); "
                          "batch: 64 realisations x parameter sets x 1-4 frequencies in one "
@@ -380,6 +380,115 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+CRE_GRID = dict(shape=(80, 80, 80), nE=80, E_GeV=(0.1, 1000.0))
+
+
+def run_cre_grid(args):
+    """Workload "cre_grid": the random_forward pass with the CRE taken from an
+    imported flux grid (CRE_num, integrator.cc:281-401) instead of the analytic
+    model: the analytic model's flux written on the device on an 80x80x80 box
+    over [-20,20]x[-20,20]x[-4,4] kpc with 80 energies from 0.1 to 1000 GeV
+    (the template's box_cre sizes).  Every sample integrates F and G over the
+    79 energy bins, so the march is FP64 arithmetic bound, not HBM bound."""
+    import math
+
+    import numpy as np
+    import torch
+    from hammurabix_amd import Context, Param, _capi as capi, configs, los_samples
+    from hammurabix_amd.units import CGS
+
+    par = Param(text=configs.random_forward())
+    desc = par.los_desc(schedule=capi.HAMX_SCHED_AUTO)
+    cre_desc = par.cre_desc()
+    desc.cre_model = capi.HAMX_CRE_GRID
+    S = los_samples(desc)
+    nE = CRE_GRID["nE"]
+    E_min, E_max = (e * CGS.GeV for e in CRE_GRID["E_GeV"])
+    E_fact = math.log(E_max / E_min) / (nE - 1)
+    cbox = capi.HamxBox(*CRE_GRID["shape"], -20 * CGS.kpc, 20 * CGS.kpc, -20 * CGS.kpc,
+                        20 * CGS.kpc, -4 * CGS.kpc, 4 * CGS.kpc)
+    dev = torch.device("cuda:0")
+    g = par.grid_brnd
+    n = g.full_size
+    with Context(0) as ctx:
+        bx, by, bz, te = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
+        ctx.brnd_es(par.es_desc(), 1, bx, by, bz)
+        ctx.ternd_dft(par.dft_desc(), 1, te)
+        cre = torch.empty(cbox.nx * cbox.ny * cbox.nz * nE, dtype=torch.float64, device=dev)
+        ctx.cre_write_grid(cre_desc, cbox, E_min, E_fact, nE, cre)
+        npix = 12 * 256 * 256
+        maps = {k: torch.empty(npix, dtype=torch.float64, device=dev)
+                for k in ("dm", "fd", "is", "qs", "us")}
+        kw = dict(brnd=(bx, by, bz), brnd_box=g.box(), ternd=te,
+                  ternd_box=par.grid_ternd.box(), cre=cre, cre_box=cbox,
+                  cre_E=(E_min, E_fact, nE))
+        for _ in range(args.warmup):
+            ctx.los_forward(desc, maps, **kw)
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.los_forward(desc, maps, **kw)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        times = ctx.kernel_times()
+        ctx.profile(False)
+        march = [t for t in times if t["kernel"].startswith("los_")]
+        march_ms = sum(t["ms"] for t in march)
+        finite = all(bool(torch.isfinite(m).all()) for m in maps.values())
+        host = None
+        if not args.no_cpu:
+            host = tuple(np.ascontiguousarray(a.cpu().numpy()) for a in (bx, by, bz, te, cre))
+    out = {
+        "metric": "LOS samples/s with an imported CRE grid (CRE_num) at nside 256",
+        "value": S * args.steps / elapsed,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "cre_grid: random_forward (LSA, unif TE, ES random B and DFT random "
+                        "TE on 256x256x64, shells nside 64/128/256, sync 23 GHz) with the CRE "
+                        "from an 80x80x80 grid of 80 energies (0.1-1000 GeV)",
+            "samples_per_step": S,
+            "energy_bins_per_sample": nE - 1,
+        },
+        "roofline": None,
+        "detail": {
+            "march_kernels": sorted({t["kernel"] for t in march}),
+            "march_ms_per_step": march_ms / args.steps,
+            "energy_bin_evaluations_per_s": S * (nE - 1) * args.steps / (march_ms * 1e-3),
+            "maps_finite": finite,
+            "bound": "FP64 VALU: two logarithms, two exponentials and two Chebyshev sums "
+                     "per energy bin and sample",
+        },
+    }
+    if host is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import write_grid
+        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        p2 = Param(text=configs.random_forward(nside_sim=2, nside_out=8))
+        d2 = p2.los_desc()
+        d2.cre_model = capi.HAMX_CRE_GRID
+        S2 = los_samples(d2)
+        t0 = time.perf_counter()
+        write_grid(d2, nthreads=threads, brnd=host[:3], brnd_box=g.box(), ternd=host[3],
+                   ternd_box=par.grid_ternd.box(), cre=host[4], cre_box=cbox,
+                   cre_E=(E_min, E_fact, nE))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(
+            value=S2 / dt, unit="samples/s", cores=threads, kind="port",
+            sample=f"the same pass with shells at nside 2/4/8 (1/1024 of the pixels, "
+                   f"{S2} samples), same grids; {dt:.2f} s")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -391,6 +500,9 @@ def main():
         return
     if args.workload in ("batch", "large"):
         run_batch(args)
+        return
+    if args.workload == "cre_grid":
+        run_cre_grid(args)
         return
     import numpy as np
     import torch

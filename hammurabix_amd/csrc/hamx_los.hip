@@ -111,8 +111,14 @@ __device__ inline double lerp_cell_e(const double *f, const Cell &c, uint64_t nE
 // 100 skipped, then the fore factor.  Each bin's gather is shared by J_tot
 // and J_pol and reused by the next bin.  Outside the grid every gathered
 // flux is 0 and so are both sums.
+// The kernels of one bin come from one evaluation: u = ln(x_fact) +
+// ln((1/E_{i+1}^2 + 1/E_i^2) / 2) (the second term tabulated per bin) in
+// place of ln(xv), and G at the J_pol mean xvp, which may differ from the
+// J_tot mean xvt in the last bits (the two x_fact products round
+// differently), as G(xvt) (1 + xvt - xvp), exact to first order.
 __device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, const Vec3 &pos,
-                                           double b_per, double &jt, double &jp) {
+                                           double b_per, const double *stab, double &jt,
+                                           double &jp) {
   jt = 0.;
   jp = 0.;
   Cell c;
@@ -124,7 +130,8 @@ __device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, cons
   const double xfp = ph.xfp_num / den;
   const uint64_t nE = gr.cre_nE;
   const uint64_t sy = gr.g_cre.nz * nE, sx = gr.g_cre.ny * sy;
-  const double *ke2 = gr.cre_tab, *beta = ke2 + nE, *dE = beta + nE;
+  const double *ke2 = gr.cre_tab, *beta = ke2 + nE, *dE = beta + nE, *lnc = dE + nE;
+  const double lxt = log(xft);
   double xt0 = xft / ke2[0], xp0 = xfp / ke2[0];
   double g0 = 0.;
   bool have0 = false;
@@ -138,10 +145,12 @@ __device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, cons
         g0 = lerp_cell_e(gr.cre, c, nE, sy, sx, i) / beta[i];
       const double g1 = lerp_cell_e(gr.cre, c, nE, sy, sx, i + 1) / beta[i + 1];
       const double flux = 0.5 * (g1 + g0);
+      double F, G;
+      sync_fg(stab, lxt + lnc[i], xvt, F, G);
       if (on_t)
-        Jt += sync_kernel<false>(xvt) * flux * dE[i];
+        Jt += F * flux * dE[i];
       if (on_p)
-        Jp += sync_kernel<true>(xvp) * flux * dE[i];
+        Jp += G * (1. + (xvt - xvp)) * flux * dE[i];
       g0 = g1;
       have0 = true;
     } else {
@@ -158,7 +167,7 @@ __device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, cons
 template <bool PACKED, bool CREG>
 __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
                                    const ShellArgs &sa, const RayGeo &r,
-                                   uint64_t k, SampleOut &s) {
+                                   uint64_t k, SampleOut &s, const double *stab) {
   const double dist = sa.d_start + (static_cast<double>(k) + 0.5) * sa.delta;
   const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y,
                  r.l.z * dist + ph.obs.z};
@@ -218,7 +227,7 @@ __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
     const double b_per = length0(cr);
     if (CREG) {
       double jt, jp;
-      cre_grid_emissivity(ph, gr, pos, b_per, jt, jp);
+      cre_grid_emissivity(ph, gr, pos, b_per, stab, jt, jp);
       s.jt = jt * sa.delta * ph.i2bt;
       s.jp = jp * sa.delta * ph.i2bt;
       s.ipa = atan2(-dot0(r.eth, B), -dot0(r.eph, B));
@@ -282,9 +291,20 @@ __device__ inline bool masked_out(const Phys &ph, const ShellArgs &sa, uint64_t 
   return true;
 }
 
+// Left to itself the register allocator gives these generic marches 256
+// VGPRs plus AGPRs (one wave per SIMD).  Two waves per SIMD (a few dozen
+// spilled registers) measured 1.7x faster for a JF12 field on the
+// random_forward workload; with the CRE grid's energy loop three waves
+// measured 2.1x faster than one (2.3x slower for JF12, hence per case).
 template <bool PACKED, bool CREG>
-__global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
-                                                  ShellArgs sa) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CREG ? 3 : 2))) void
+k_los_wave(Phys ph, Grids gr, ShellArgs sa) {
+  // the synchrotron kernel tables of the CRE grid emissivities, in LDS
+  __shared__ double stab[CREG ? kSyncTab : 1];
+  if (CREG) {
+    load_sync_tab(stab);
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   const uint64_t ipix =
       static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -300,7 +320,7 @@ __global__ __launch_bounds__(256) void k_los_wave(Phys ph, Grids gr,
     SampleOut s;
     bool valid = false;
     if (k < r.k_hi)
-      valid = eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s);
+      valid = eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s, stab);
     const double fdk = valid ? s.fdc : 0.;
     if (ph.do_sync) {
       const double incl = wave_incl_scan(fdk, lane);
@@ -404,8 +424,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 template <bool PACKED, bool CREG>
-__global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
-                                                    ShellArgs sa) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CREG ? 3 : 2))) void
+k_los_thread(Phys ph, Grids gr, ShellArgs sa) {
+  __shared__ double stab[CREG ? kSyncTab : 1];
+  if (CREG) {
+    load_sync_tab(stab);
+    __syncthreads();
+  }
   const uint64_t ipix = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (ipix >= sa.hs.npix)
     return;
@@ -416,7 +441,7 @@ __global__ __launch_bounds__(256) void k_los_thread(Phys ph, Grids gr,
   double dm = 0., fd = 0., I = 0., Q = 0., U = 0.;
   for (uint64_t k = r.k_lo; k < r.k_hi; ++k) {
     SampleOut s;
-    if (!eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s))
+    if (!eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s, stab))
       continue;
     if (ph.do_dm)
       dm += s.te * sa.delta;
@@ -729,15 +754,17 @@ int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
     gr.cre_nE = nE;
     // E_i, x denominators and beta as integrator.cc:300-304 forms them
     std::vector<double> &tab = ctx->host_tab;
-    tab.assign(3 * nE, 0.);
+    tab.assign(4 * nE, 0.);
     std::vector<double> KE(nE);
     for (uint64_t i = 0; i != nE; ++i) {
       KE[i] = f.cre_E_min * std::exp(i * f.cre_E_fact);
       tab[i] = KE[i] * KE[i];
       tab[nE + i] = std::sqrt(1 - cgs::mec2 / KE[i]);
     }
-    for (uint64_t i = 0; i + 1 < nE; ++i)
+    for (uint64_t i = 0; i + 1 < nE; ++i) {
       tab[2 * nE + i] = std::fabs(KE[i + 1] - KE[i]);
+      tab[3 * nE + i] = std::log(0.5 * (1. / tab[i + 1] + 1. / tab[i]));
+    }
     void *tp;
     HAMX_TRY(ws_get(ctx, kSlotCreTab, tab.size() * sizeof(double), &tp));
     HAMX_HIP(ctx, hipMemcpyAsync(tp, tab.data(), tab.size() * sizeof(double),
@@ -1068,11 +1095,13 @@ __global__ __launch_bounds__(256) void k_cre_write(Phys ph, CreWriteArgs a, doub
 
 __global__ __launch_bounds__(256) void k_sync_fg(const double *x, uint64_t n, double *F,
                                                  double *G) {
+  __shared__ double tab[kSyncTab];
+  load_sync_tab(tab);
+  __syncthreads();
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
        t += stride) {
-    F[t] = sync_kernel<false>(x[t]);
-    G[t] = sync_kernel<true>(x[t]);
+    sync_fg(tab, log(x[t]), x[t], F[t], G[t]);
   }
 }
 

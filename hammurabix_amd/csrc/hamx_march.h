@@ -56,8 +56,9 @@ struct Grids {
   GridGeo g_breg;
   const double *tereg;
   GridGeo g_tereg;
-  // CRE_num grid [x][y][z][E] and its energy tables: E_i^2, beta_i, and
-  // |E_{i+1} - E_i| (cre_nE, cre_nE, cre_nE - 1 doubles from cre_tab)
+  // CRE_num grid [x][y][z][E] and its energy tables: E_i^2, beta_i,
+  // |E_{i+1} - E_i| and ln((1/E_{i+1}^2 + 1/E_i^2) / 2), cre_nE doubles
+  // each from cre_tab
   const double *cre;
   GridGeo g_cre;
   uint64_t cre_nE;

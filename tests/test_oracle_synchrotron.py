@@ -7,9 +7,10 @@ cre.cc:144-225) pinned without GSL and without the reference build.
   moments int_0^inf x^mu F dx = 2^(mu+1)/(mu+2) Gamma(mu/2 + 7/3)
   Gamma(mu/2 + 2/3) and int_0^inf x^mu G dx = 2^mu Gamma(mu/2 + 4/3)
   Gamma(mu/2 + 2/3), and both asymptotes.
-* The device tables (hamx_synchrotron.h): regenerated from their script
-  they are unchanged, and their Clenshaw evaluation, restated here, matches
-  the oracle's kernels to 2e-13.
+* The device tables (hamx_synchrotron.h, P = ln F + x and R = G / F in
+  u = ln x): regenerated from their script they are unchanged, and their
+  Clenshaw evaluation, restated here, matches the oracle's kernels to
+  2e-13.
 * CREfield::read_grid_num: exact on fields linear in position, per energy.
 * CRE_ana / CRE_unif::write_field on the grid: j0 at E0 at the observer, the
   power law in gamma with the sqrt(1 - 1/gamma) ratio, zero outside the
@@ -109,20 +110,22 @@ def _parse_tables():
             name, val = line.split("=")[0].split()[-1], line.split("=")[1].split(";")[0]
             consts[name] = float(val)
     tabs = {}
-    for name in ("kF", "kG"):
+    for name in ("kF", "kR"):
         body = text.split(f"{name}[kSeg][kCoef] = {{")[1].split("};")[0]
         rows = [r.strip().strip(",").strip("{}") for r in body.strip().splitlines()]
         tabs[name] = np.array([[float(v) for v in r.split(",")] for r in rows])
     return consts, tabs
 
 
-def _eval_table(consts, tab, lnc, x):
-    """The device evaluator of hamx_sync.h restated: segment, Clenshaw, exp."""
+def _eval_table(consts, tabs, x):
+    """The device evaluator of hamx_sync.h restated: segment, Clenshaw,
+    F = exp(P - x), G = F R."""
     u = math.log(x)
     dtop = consts["kUTop"] - u
     fine = consts["kFineN"] * consts["kFineW"]
     if dtop >= fine + consts["kCoarseN"] * consts["kCoarseW"]:
-        return math.exp(lnc + u / 3.)
+        f = math.exp(consts["kLnF0"] + u / 3.)
+        return f, consts["kR0"] * f
     if dtop < fine:
         s = min(int(dtop / consts["kFineW"]) if dtop > 0 else 0, int(consts["kFineN"]) - 1)
         w, deg = consts["kFineW"], int(consts["kFineDeg"])
@@ -133,10 +136,14 @@ def _eval_table(consts, tab, lnc, x):
         w, deg = consts["kCoarseW"], int(consts["kCoarseDeg"])
         lo = consts["kUTop"] - fine - (sc + 1) * w
     t = 2. * (u - lo) / w - 1.
-    b1 = b2 = 0.
-    for k in range(deg, 0, -1):
-        b1, b2 = tab[s][k] + 2. * t * b1 - b2, b1
-    return math.exp(tab[s][0] + t * b1 - b2 - x)
+
+    def clenshaw(c):
+        b1 = b2 = 0.
+        for k in range(deg, 0, -1):
+            b1, b2 = c[k] + 2. * t * b1 - b2, b1
+        return c[0] + t * b1 - b2
+    f = math.exp(clenshaw(tabs["kF"][s]) - x)
+    return f, f * clenshaw(tabs["kR"][s])
 
 
 def test_table_evaluation_matches_oracle(orc):
@@ -144,8 +151,7 @@ def test_table_evaluation_matches_oracle(orc):
     xs = np.concatenate([np.geomspace(1e-25, 100., 400), [99.999, 100.0, 1.0, math.e]])
     F, G = kernels(orc, xs)
     for x, f, g in zip(xs, F, G):
-        tf = _eval_table(consts, tabs["kF"], consts["kLnF0"], x)
-        tg = _eval_table(consts, tabs["kG"], consts["kLnG0"], x)
+        tf, tg = _eval_table(consts, tabs, x)
         assert tf == pytest.approx(f, rel=2e-13, abs=0), x
         assert tg == pytest.approx(g, rel=2e-13, abs=0), x
 
