@@ -27,7 +27,8 @@ HAMX_USE_TAPE = 16
 (HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID,
  HAMX_BREG_HELIX, HAMX_BREG_JF12, HAMX_BREG_JAFFE) = range(8)
 HAMX_BREG_NPARAM = 40
-HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID = range(3)
+HAMX_TEREG_NONE, HAMX_TEREG_UNIF, HAMX_TEREG_GRID, HAMX_TEREG_YMW16 = range(4)
+HAMX_TEREG_NPARAM = 80
 HAMX_CRE_NONE, HAMX_CRE_ANA, HAMX_CRE_UNIF, HAMX_CRE_GRID = range(4)
 HAMX_SCHED_AUTO, HAMX_SCHED_THREAD, HAMX_SCHED_WAVE = range(3)
 
@@ -66,7 +67,7 @@ class HamxLosDesc(C.Structure):
                 ("nside_dm", C.c_uint32), ("nside_fd", C.c_uint32), ("nside_sync", C.c_uint32),
                 ("sync_freq", C.c_double),
                 ("breg_model", C.c_int32), ("breg", C.c_double * HAMX_BREG_NPARAM),
-                ("tereg_model", C.c_int32), ("tereg", C.c_double * 4),
+                ("tereg_model", C.c_int32), ("tereg", C.c_double * HAMX_TEREG_NPARAM),
                 ("cre_model", C.c_int32), ("cre", C.c_double * 8),
                 ("schedule", C.c_int32), ("reserved", C.c_int32)]
 

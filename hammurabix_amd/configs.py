@@ -107,7 +107,38 @@ _TEMPLATE = """<?xml version="1.0"?>
     </random>
   </magneticfield>
   <thermalelectron>
-    <regular cue="1" type="unif">
+    <regular cue="1" type="{tereg}">
+      <ymw16>
+        <warp> <r_warp value="8.4"/> <gamma_w value="0.14"/> <r0 value="8.3"/> </warp>
+        <thickdisk> <ad value="2500"/> <bd value="15000"/> <n1 value="0.01132"/> <h1 value="1673"/> </thickdisk>
+        <thindisk> <a2 value="1200"/> <b2 value="4000"/> <n2 value="0.404"/> <k2 value="1.54"/> </thindisk>
+        <spiralarm>
+          <b2s value="4000"/>
+          <ele_arm_0 value="0.135"/> <ele_arm_1 value="0.129"/> <ele_arm_2 value="0.103"/>
+          <ele_arm_3 value="0.116"/> <ele_arm_4 value="0.0057"/>
+          <wid_arm_0 value="300"/> <wid_arm_1 value="500"/> <wid_arm_2 value="300"/>
+          <wid_arm_3 value="500"/> <wid_arm_4 value="300"/>
+          <rref_arm_0 value="3.35"/> <rref_arm_1 value="3.707"/> <rref_arm_2 value="3.56"/>
+          <rref_arm_3 value="3.670"/> <rref_arm_4 value="8.21"/>
+          <phiref_arm_0 value="44.4"/> <phiref_arm_1 value="120.0"/> <phiref_arm_2 value="218.6"/>
+          <phiref_arm_3 value="330.3"/> <phiref_arm_4 value="55.1"/>
+          <pitch_arm_0 value="11.43"/> <pitch_arm_1 value="9.84"/> <pitch_arm_2 value="10.38"/>
+          <pitch_arm_3 value="10.54"/> <pitch_arm_4 value="2.77"/>
+          <aa value="11680"/> <ka value="5.015"/> <ncn value="2.4"/> <wcn value="8.2"/>
+          <thetacn value="109"/> <nsg value="0.626"/> <wsg value="20"/> <thetasg value="75.8"/>
+        </spiralarm>
+        <galcenter> <ngc value="6.2"/> <agc value="160"/> <hgc value="35"/> </galcenter>
+        <gumnebula> <ngn value="1.84"/> <wgn value="15.1"/> <agn value="125.8"/> <kgn value="1.4"/> </gumnebula>
+        <localbubble>
+          <j_lb value="0.480"/>
+          <nlb1 value="1.094"/> <detlb1 value="28.4"/> <wlb1 value="14.2"/> <hlb1 value="112.9"/>
+          <thetalb1 value="195.4"/>
+          <nlb2 value="2.33"/> <detlb2 value="14.7"/> <wlb2 value="15.6"/> <hlb2 value="43.6"/>
+          <thetalb2 value="278.2"/>
+        </localbubble>
+        <loopi> <nli value="1.907"/> <rli value="80.0"/> <wli value="15.0"/> <detthetali value="30.0"/>
+          <thetali value="40.0"/> </loopi>
+      </ymw16>
       <unif>
         <n0 value="0.01"/>
         <r0 value="3.0"/>
@@ -155,10 +186,11 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         out: str = "", mask_file: str = "", nside_mask: int = 32,
         cre_file: str = "", cre_read: bool = False, cre_write: bool = False,
         cre_grid: tuple[int, int, int, int] = (40, 40, 8, 32),
-        cre_E: tuple[float, float] = (0.1, 1000.0)) -> str:
+        cre_E: tuple[float, float] = (0.1, 1000.0), tereg: str = "unif") -> str:
     """The parameter file; cre_read / cre_write with cre_file switch on the
     CRE grid I/O on box_cre (cre_grid = nx, ny, nz, nE; cre_E = E_min, E_max
-    in GeV)."""
+    in GeV); tereg picks the regular thermal electron model ("unif" or
+    "ymw16", the template's YMW16 values)."""
     freqs = [23.0] if freqs is None else freqs
     return _TEMPLATE.format(
         do_dm=int(do_dm), do_fd=int(do_fd), nside_out=nside_out,
@@ -171,7 +203,7 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask,
         cre_read=int(cre_read), cre_write=int(cre_write), cre_file=cre_file or "cre.bin",
         cre_nx=cre_grid[0], cre_ny=cre_grid[1], cre_nz=cre_grid[2], cre_nE=cre_grid[3],
-        cre_E_min=cre_E[0], cre_E_max=cre_E[1])
+        cre_E_min=cre_E[0], cre_E_max=cre_E[1], tereg=tereg)
 
 
 def plain_forward(**kw) -> str:

@@ -291,7 +291,7 @@ bool same_desc(const hamx_los_desc &a, const hamx_los_desc &b) {
     eq = eq && a.breg[i] == b.breg[i];
   for (int i = 0; i < 8; ++i)
     eq = eq && a.cre[i] == b.cre[i];
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < HAMX_TEREG_NPARAM; ++i)
     eq = eq && a.tereg[i] == b.tereg[i];
   return eq;
 }

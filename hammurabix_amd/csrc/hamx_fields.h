@@ -147,7 +147,11 @@ __device__ inline void lerp_cell_packed(const double4 *__restrict__ f,
 // ---------------------------------------------------------------- models --
 
 // Regular magnetic field models (breg_lsa.cc:9-26, breg_cart.cc, breg_unif.cc,
-// breg_helix.cc:13-30, breg_jf12.cc:8-189, breg_jaffe.cc:11-243).
+// breg_helix.cc:13-30, breg_jf12.cc:8-189, breg_jaffe.cc:11-243).  With
+// HEAVY false the JF12 and Jaffe evaluators are compiled out (they return a
+// zero field), which keeps their registers out of kernels that never see
+// them.
+template <bool HEAVY = true>
 __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
   Vec3 b{0., 0., 0.};
   if (model == HAMX_BREG_LSA) {
@@ -176,10 +180,12 @@ __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
     double v[3];
     if (model == HAMX_BREG_HELIX)
       breg_helix_eval(p, pos.x, pos.y, pos.z, v);
-    else if (model == HAMX_BREG_JF12)
+    else if (HEAVY && model == HAMX_BREG_JF12)
       breg_jf12_eval(p, pos.x, pos.y, pos.z, v);
-    else
+    else if (HEAVY)
       breg_jaffe_eval(p, pos.x, pos.y, pos.z, v);
+    else
+      v[0] = v[1] = v[2] = 0.;
     b = Vec3{v[0], v[1], v[2]};
   }
   return b;

@@ -164,7 +164,7 @@ __device__ inline void cre_grid_emissivity(const Phys &ph, const Grids &gr, cons
   jp = Jp * fore;
 }
 
-template <bool PACKED, bool CREG>
+template <bool PACKED, bool CREG, bool HEAVY>
 __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
                                    const ShellArgs &sa, const RayGeo &r,
                                    uint64_t k, SampleOut &s, const double *stab) {
@@ -186,11 +186,13 @@ __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
   if (ph.breg_model == HAMX_BREG_GRID)
     B = gather3(gr.breg_x, gr.breg_y, gr.breg_z, gr.g_breg, pos);
   else
-    B = breg_eval(ph.breg_model, ph.breg, pos);
+    B = breg_eval<HEAVY>(ph.breg_model, ph.breg, pos);
   // regular + random thermal electrons (integrator.cc:197-201)
   double te = 0.;
   if (ph.tereg_model == HAMX_TEREG_UNIF)
     te = tereg_unif(ph.tereg, pos, ph.obs);
+  else if (HEAVY && ph.tereg_model == HAMX_TEREG_YMW16)
+    te = tereg_ymw16_eval(ph.tereg, pos.x, pos.y, pos.z);
   else if (ph.tereg_model == HAMX_TEREG_GRID)
     te = gather1(gr.tereg, gr.g_tereg, pos);
   if (PACKED) {
@@ -296,7 +298,7 @@ __device__ inline bool masked_out(const Phys &ph, const ShellArgs &sa, uint64_t 
 // spilled registers) measured 1.7x faster for a JF12 field on the
 // random_forward workload; with the CRE grid's energy loop three waves
 // measured 2.1x faster than one (2.3x slower for JF12, hence per case).
-template <bool PACKED, bool CREG>
+template <bool PACKED, bool CREG, bool HEAVY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CREG ? 3 : 2))) void
 k_los_wave(Phys ph, Grids gr, ShellArgs sa) {
   // the synchrotron kernel tables of the CRE grid emissivities, in LDS
@@ -320,7 +322,7 @@ k_los_wave(Phys ph, Grids gr, ShellArgs sa) {
     SampleOut s;
     bool valid = false;
     if (k < r.k_hi)
-      valid = eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s, stab);
+      valid = eval_sample<PACKED, CREG, HEAVY>(ph, gr, sa, r, k, s, stab);
     const double fdk = valid ? s.fdc : 0.;
     if (ph.do_sync) {
       const double incl = wave_incl_scan(fdk, lane);
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
-template <bool PACKED, bool CREG>
+template <bool PACKED, bool CREG, bool HEAVY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CREG ? 3 : 2))) void
 k_los_thread(Phys ph, Grids gr, ShellArgs sa) {
   __shared__ double stab[CREG ? kSyncTab : 1];
@@ -441,7 +443,7 @@ k_los_thread(Phys ph, Grids gr, ShellArgs sa) {
   double dm = 0., fd = 0., I = 0., Q = 0., U = 0.;
   for (uint64_t k = r.k_lo; k < r.k_hi; ++k) {
     SampleOut s;
-    if (!eval_sample<PACKED, CREG>(ph, gr, sa, r, k, s, stab))
+    if (!eval_sample<PACKED, CREG, HEAVY>(ph, gr, sa, r, k, s, stab))
       continue;
     if (ph.do_dm)
       dm += s.te * sa.delta;
@@ -569,7 +571,7 @@ int validate_los(hamx_ctx *ctx, const hamx_los_desc *d) {
     return set_err(ctx, HAMX_ERR_ARG, "sync frequency must be positive");
   if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JAFFE)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
-  if (d->tereg_model < HAMX_TEREG_NONE || d->tereg_model > HAMX_TEREG_GRID)
+  if (d->tereg_model < HAMX_TEREG_NONE || d->tereg_model > HAMX_TEREG_YMW16)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported tereg model %d", d->tereg_model);
   if (d->cre_model < HAMX_CRE_NONE || d->cre_model > HAMX_CRE_GRID)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported cre model %d", d->cre_model);
@@ -912,6 +914,9 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
                     (packed || !grids_any);
   const bool pw1 = fast && 0.5 * (ph.idx + 1) == -1.0;
   const bool creg = d->cre_model == HAMX_CRE_GRID && d->do_sync;
+  // models with large evaluators get kernels of their own (register use)
+  const bool heavy = d->breg_model == HAMX_BREG_JF12 || d->breg_model == HAMX_BREG_JAFFE ||
+                     d->tereg_model == HAMX_TEREG_YMW16;
   for (uint32_t s = 0; s < d->n_shell; ++s) {
     ShellArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -958,26 +963,57 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
                            ctx->stream, ph, fc, gr, sa);
     } else if (sched == HAMX_SCHED_WAVE) {
       const unsigned blocks = static_cast<unsigned>((npix + 3) / 4);
+#define HAMX_LAUNCH_WAVE(P, C, H)                                                          \
+  hipLaunchKernelGGL((k_los_wave<P, C, H>), dim3(blocks), dim3(256), 0, ctx->stream, ph, gr, sa)
       if (creg)
-        hipLaunchKernelGGL((k_los_wave<false, true>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_WAVE(false, true, true);
+        else
+          HAMX_LAUNCH_WAVE(false, true, false);
+      }
       else if (packed)
-        hipLaunchKernelGGL((k_los_wave<true, false>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_WAVE(true, false, true);
+        else
+          HAMX_LAUNCH_WAVE(true, false, false);
+      }
       else
-        hipLaunchKernelGGL((k_los_wave<false, false>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_WAVE(false, false, true);
+        else
+          HAMX_LAUNCH_WAVE(false, false, false);
+      }
+#undef HAMX_LAUNCH_WAVE
     } else {
       const unsigned blocks = static_cast<unsigned>((npix + 255) / 256);
+#define HAMX_LAUNCH_THREAD(P, C, H)                                                        \
+  hipLaunchKernelGGL((k_los_thread<P, C, H>), dim3(blocks), dim3(256), 0, ctx->stream, ph, gr, \
+                     sa)
       if (creg)
-        hipLaunchKernelGGL((k_los_thread<false, true>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_THREAD(false, true, true);
+        else
+          HAMX_LAUNCH_THREAD(false, true, false);
+      }
       else if (packed)
-        hipLaunchKernelGGL((k_los_thread<true, false>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_THREAD(true, false, true);
+        else
+          HAMX_LAUNCH_THREAD(true, false, false);
+      }
       else
-        hipLaunchKernelGGL((k_los_thread<false, false>), dim3(blocks), dim3(256), 0,
-                           ctx->stream, ph, gr, sa);
+        {
+        if (heavy)
+          HAMX_LAUNCH_THREAD(false, false, true);
+        else
+          HAMX_LAUNCH_THREAD(false, false, false);
+      }
+#undef HAMX_LAUNCH_THREAD
     }
     HAMX_HIP(ctx, hipGetLastError());
     prof_end(ctx, pm,

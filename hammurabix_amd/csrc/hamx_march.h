@@ -25,7 +25,7 @@ struct Phys {
   double gc_r_min, gc_r_max, gc_z_min, gc_z_max;
   int do_dm, do_fd, do_sync;
   int breg_model, tereg_model, cre_model;
-  double breg[HAMX_BREG_NPARAM], tereg[4], cre[8];
+  double breg[HAMX_BREG_NPARAM], tereg[HAMX_TEREG_NPARAM], cre[8];
   double fd_fore;       // -(qe^3) / (2 pi mec2^2), integrator.cc:160-161
   double lambda_sq;     // (c/nu)^2, integrator.cc:164-165
   double i2bt;          // c^2 / (2 kB nu^2), integrator.cc:168-170

@@ -225,4 +225,209 @@ HAMX_HD inline void breg_jaffe_eval(const double *p, double x, double y, double 
   }
 }
 
+// ------------------------------------------------------------------ YMW16 --
+// TEreg_ymw16::write_field (tereg_ymw16.cc:10-333): the YMW16 thermal
+// electron density in its own frame (x, y, z) = (y, -x, z) of ours, with the
+// Galactic warp applied to z beyond r_warp and nothing beyond 25 kpc.
+// Components: thick disk, thin disk, spiral arms (the reference's arm loop
+// stops before the Local arm, index 4), Galactic centre, Gum Nebula, Local
+// Bubble and Loop I, combined by the reference's weighting rules.  Parameter
+// order (HAMX_TEREG_YMW16 in hamx.h):
+//   0 r_warp, 1 r0, 2 gamma_w | 3 ad, 4 bd, 5 n1, 6 h1 | 7 a2, 8 b2, 9 n2, 10 k2 |
+//   11 b2s, 12-16 narm, 17-21 warm, 22-26 rmin, 27-31 phimin [rad],
+//   32-36 tan(pitch), 37-41 cos(pitch), 42 aa, 43 ka, 44 ncn, 45 thetacn [deg],
+//   46 wcn [deg], 47 nsg, 48 thetasg [deg], 49 wsg [deg] |
+//   50 ngc, 51 agc, 52 hgc | 53 ngn, 54 wgn, 55 agn, 56 kgn |
+//   57 j_lb, 58 nlb1, 59 thetalb1 [deg], 60 detlb1 [deg], 61 wlb1, 62 hlb1,
+//   63 nlb2, 64 thetalb2 [deg], 65 detlb2 [deg], 66 wlb2, 67 hlb2 |
+//   68 nli, 69 rli, 70 wli, 71 detthetali [deg], 72 thetali [deg];
+// lengths in cm, densities in cm^-3.
+namespace ymw16 {
+
+HAMX_HD inline double sech2(double v) {
+  const double s = 1. / cosh(v);
+  return s * s;
+}
+
+// thick disk (tereg_ymw16.cc:82-94)
+HAMX_HD inline double thick(const double *p, double z, double r) {
+  if (z > 10. * p[6])
+    return 0.;
+  const double gd = r > p[4] ? sech2((r - p[4]) / p[3]) : 1.;
+  return p[5] * gd * sech2(z / p[6]);
+}
+
+// thin disk (tereg_ymw16.cc:97-115)
+HAMX_HD inline double thin(const double *p, double z, double r) {
+  const double pc = cgs::pc;
+  const double h0 = p[10] * (32 * pc + 1.6e-3 * r + (4.e-7 / pc) * r * r);
+  if (z > 10. * h0)
+    return 0.;
+  const double gd = r > p[4] ? sech2((r - p[4]) / p[3]) : 1.;
+  return p[9] * gd * sech2((r - p[8]) / p[7]) * sech2(z / h0);
+}
+
+// spiral arms (tereg_ymw16.cc:118-210); theta * rad compared with degrees is
+// the reference's expression, kept as written
+HAMX_HD inline double spiral(const double *p, double x, double y, double z, double r) {
+  const double pc = cgs::pc, rad = cgs::rad, twopi = 2. * cgs::pi;
+  double scaling = 1.;
+  if (r > p[4]) {
+    if ((r - p[4]) > 10. * p[3])
+      return 0.;
+    scaling = sech2((r - p[4]) / p[3]);
+  }
+  const double h0 = p[43] * (32 * pc + 1.6e-3 * r + (4.e-7 / pc) * (r * r));
+  if (z > 10. * h0)
+    return 0.;
+  scaling *= sech2(z / h0);
+  if ((r - p[11]) > 10. * p[42])
+    return 0.;
+  scaling *= sech2((r - p[11]) / p[42]);
+  double theta = atan2(y, x);
+  if (theta < 0)
+    theta += twopi;
+  double ne = 0.;
+  for (int i = 0; i < 4; ++i) {
+    double dphi = theta - p[27 + i];
+    if (dphi < 0)
+      dphi += twopi;
+    const double d = fabs(p[22 + i] * exp(dphi * p[32 + i]) - r);
+    const double dw = fabs(p[22 + i] * exp((dphi + twopi) * p[32 + i]) - r);
+    const double smin = (d < dw ? d : dw) * p[37 + i];
+    if (smin > 10. * p[17 + i])
+      continue;
+    const double arm = sech2(smin / p[17 + i]);
+    if (i != 2) {
+      ne += p[12 + i] * scaling * arm;
+      continue;
+    }
+    const double th = theta * rad;
+    const double sg = (th - p[48]) / p[49];
+    const double g_sg = 1. - p[47] * exp(-(sg * sg));
+    double g_cn;
+    if (r > 6 * cgs::kpc && th > p[45]) {
+      g_cn = 1. + p[44];
+    } else {
+      const double cn = (th - p[45]) / p[46];
+      g_cn = 1. + p[44] * exp(-(cn * cn));
+    }
+    ne += p[12 + i] * scaling * (g_sg * g_cn * arm);
+  }
+  return ne;
+}
+
+// Galactic centre (tereg_ymw16.cc:213-229)
+HAMX_HD inline double galcen(const double *p, double x, double y, double z) {
+  const double xg = 50. * cgs::pc, zg = -7. * cgs::pc;
+  const double R2 = (x - xg) * (x - xg) + (y - 0.) * (y - 0.);
+  if (R2 > 10. * p[51] * p[51])
+    return 0.;
+  const double Ar = exp(-R2 / (p[51] * p[51]));
+  if (fabs(z - zg) > 10. * p[52])
+    return 0.;
+  return p[50] * Ar * sech2((z - zg) / p[52]);
+}
+
+// Gum Nebula (tereg_ymw16.cc:232-270): an ellipsoidal shell about
+// (l, b, d) = (264 deg, -4 deg, 450 pc) from the Sun
+HAMX_HD inline double gum(const double *p, double x, double y, double z) {
+  if (y < 0 || x > 0)
+    return 0.;
+  const double lc = 264. * cgs::rad, bc = -4. * cgs::rad, dc = 450. * cgs::pc;
+  const double xc = dc * cos(bc) * sin(lc);
+  const double yc = p[1] - dc * cos(bc) * cos(lc);
+  const double zc = dc * sin(bc);
+  const double theta = atan2(fabs(z - zc), sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc)));
+  const double tt = tan(theta);
+  const double a = p[55], k = p[56];
+  const double zp = (a * k) / sqrt(1. + k * k / (tt * tt));
+  const double xyp = zp / tt;
+  const double xy_dist = sqrt(a * a - xyp * xyp) * static_cast<double>(a > xyp);
+  const double alpha = atan2(k * xyp, xy_dist) + theta;
+  const double R2 = (x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc);
+  const double r2 = zp * zp + xyp * xyp;
+  const double sa = sin(alpha);
+  const double D2 = (R2 + r2 - 2. * sqrt(R2 * r2)) * sa * sa;
+  if (D2 > 10. * p[54] * p[54])
+    return 0.;
+  return p[53] * exp(-D2 / (p[54] * p[54]));
+}
+
+// the distance from the Local Bubble's axis (tereg_ymw16.cc:46-48,280-282)
+HAMX_HD inline double lb_radius(double x, double y, double z) {
+  const double u = (y - 8.34 * cgs::kpc) * 0.94 - 0.34 * z;
+  return sqrt(u * u + x * x);
+}
+
+// Local Bubble (tereg_ymw16.cc:273-307), l in degrees
+HAMX_HD inline double localbubble(const double *p, double x, double y, double z, double l,
+                                  double Rlb) {
+  if (y < 0)
+    return 0.;
+  const double rLB = lb_radius(x, y, z);
+  double ne = 0.;
+  for (int c = 0; c < 2; ++c) {
+    const double *q = p + 58 + 5 * c; // n, theta, det, w, h
+    const double a = fabs(l + 360. - q[1]), b = fabs(q[1] - l);
+    const double dl = a < b ? a : b;
+    if (dl < 10. * q[2] || (rLB - Rlb) < 10. * q[3] || z < 10. * q[4])
+      ne += q[0] * sech2(dl / q[2]) * sech2((rLB - Rlb) / q[3]) * sech2(z / q[4]);
+  }
+  return ne;
+}
+
+// Loop I (tereg_ymw16.cc:310-333)
+HAMX_HD inline double loop1(const double *p, double x, double y, double z) {
+  if (y < 0)
+    return 0.;
+  const double th_li = p[72] * cgs::rad;
+  const double xc = -10.156 * cgs::pc, yc = 8106.207 * cgs::pc, zc = 10.467 * cgs::pc;
+  const double rLI = sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc));
+  const double theta = acos(((x - xc) * cos(th_li) + (z - zc) * sin(th_li)) / rLI) / cgs::rad;
+  if (theta > 10. * p[71] || (rLI - p[69]) > 10. * p[70])
+    return 0.;
+  const double dr = (rLI - p[69]) / p[70], dt = theta / p[71];
+  return p[68] * exp(-(dr * dr)) * exp(-(dt * dt));
+}
+
+} // namespace ymw16
+
+HAMX_HD inline double tereg_ymw16_eval(const double *p, double px, double py, double pz) {
+  double x = py, y = -px, z = pz;
+  const double r = sqrt(x * x + y * y);
+  if (r >= p[0]) {
+    const double tw = atan2(y, x);
+    z -= p[2] * (r - p[0]) * cos(tw);
+  }
+  if (sqrt(x * x + y * y + z * z) > 25 * cgs::kpc)
+    return 0.;
+  const double l = atan2(x, p[1] - y) / cgs::rad;
+  const double n_thick = ymw16::thick(p, z, r);
+  const double n_thin = ymw16::thin(p, z, r);
+  const double n_arm = ymw16::spiral(p, x, y, z, r);
+  const double n_gc = ymw16::galcen(p, x, y, z);
+  const double n_gum = ymw16::gum(p, x, y, z);
+  const double Rlb = 110. * cgs::pc;
+  const double n_lb = ymw16::localbubble(p, x, y, z, l, Rlb);
+  const double n_li = ymw16::loop1(p, x, y, z);
+  const double n_disk = n_thin > n_arm ? n_thin : n_arm;
+  double n0 = n_thick + n_disk;
+  double w_lb = 0., w_gum = 0., w_loop = 0.;
+  if (ymw16::lb_radius(x, y, z) < Rlb) {
+    n0 = p[57] * n_thick + n_disk;
+    if (n_lb > n0)
+      w_lb = 1;
+  } else if (n_lb > n0 && n_lb > n_gum) {
+    w_lb = 1;
+  }
+  if (n_li > n0)
+    w_loop = 1;
+  if (n_gum > n0)
+    w_gum = 1;
+  return (1 - w_lb) * ((1 - w_gum) * ((1 - w_loop) * (n0 + n_gc) + w_loop * n_li) +
+                       w_gum * n_gum) +
+         w_lb * n_lb;
+}
+
 } // namespace hamx

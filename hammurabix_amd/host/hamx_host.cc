@@ -411,6 +411,14 @@ int TEreg_unif::device_model(const Param *par, double *p) const {
   p[1] = par->tereg_unif.r0;
   return HAMX_TEREG_UNIF;
 }
+double TEreg_ymw16::write_field(const Vec3 &pos, const Param *par) const {
+  return hamx::tereg_ymw16_eval(par->tereg_ymw16, pos[0], pos[1], pos[2]);
+}
+int TEreg_ymw16::device_model(const Param *par, double *p) const {
+  for (int i = 0; i < 73; ++i)
+    p[i] = par->tereg_ymw16[i];
+  return HAMX_TEREG_YMW16;
+}
 
 int CREfield::device_model(const Param *, double *) const { return -1; }
 int CRE_num::device_model(const Param *, double *) const { return HAMX_CRE_GRID; }
@@ -775,6 +783,8 @@ void Pipeline::assemble_tereg() {
     tereg = std::make_unique<TEreg>();
   } else if (par->tereg_type == "unif") {
     tereg = std::make_unique<TEreg_unif>();
+  } else if (par->tereg_type == "ymw16") {
+    tereg = std::make_unique<TEreg_ymw16>();
   } else {
     throw std::runtime_error("unsupported tereg model");
   }

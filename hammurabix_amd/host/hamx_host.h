@@ -84,6 +84,8 @@ public:
   double brnd_jf12[13] = {};
   std::string tereg_type, ternd_type, ternd_method;
   struct { double n0 = 0, r0 = 0; } tereg_unif;
+  // YMW16 in HAMX_TEREG_YMW16 order and CGS units (param.cc:480-641)
+  double tereg_ymw16[73] = {};
   unsigned long long ternd_seed = 0;
   struct { double rms = 0, k0 = 0, a0 = 0, r0 = 0, z0 = 0; } ternd_dft;
   std::string cre_type;
@@ -255,12 +257,19 @@ public:
   virtual double write_field(const Vec3 &pos, const Param *par) const;
   double read_field(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const;
   void write_grid(const Param *par, Grid_tereg *grid) const;
-  virtual int device_model(const Param *par, double p[4]) const;
+  // device evaluator: HAMX_TEREG_* and its HAMX_TEREG_NPARAM parameters, or -1
+  virtual int device_model(const Param *par, double *p) const;
 };
 class TEreg_unif : public TEreg {
 public:
   double write_field(const Vec3 &pos, const Param *par) const override;
-  int device_model(const Param *par, double p[4]) const override;
+  int device_model(const Param *par, double *p) const override;
+};
+// TEreg_ymw16 (tereg_ymw16.cc): the shared host/device evaluator
+class TEreg_ymw16 : public TEreg {
+public:
+  double write_field(const Vec3 &pos, const Param *par) const override;
+  int device_model(const Param *par, double *p) const override;
 };
 
 class TErnd {

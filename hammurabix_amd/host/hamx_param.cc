@@ -2,6 +2,7 @@
 // hamx::xml).  Unsupported type strings throw the reference's messages.
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 #include "hamx_host.h"
 #include "hamx_xml.h"
@@ -14,6 +15,7 @@ namespace units {
 const double pi = 3.141592653589793238462643383279502884197;
 const double rad = pi / 180.;
 const double kpc = 3.0856775806e+21;
+const double pc = 3.0856775806e+18;
 const double eV = 1.60217733e-12;
 const double GeV = 1.e9 * eV;
 const double GHz = 1.e9;
@@ -284,7 +286,65 @@ Param::Param(const std::string &xml_file) {
       const Element *s = trace(te, {"regular", "unif"});
       tereg_unif.n0 = fval(s, "n0");
       tereg_unif.r0 = fval(s, "r0") * units::kpc;
-    } else if (tereg_type != "ymw16") {
+    } else if (tereg_type == "ymw16") {
+      auto blk = [&](const char *name) { return trace(te, {"regular", "ymw16", name}); };
+      const Element *w = blk("warp"), *tk = blk("thickdisk"), *tn = blk("thindisk");
+      const Element *sa = blk("spiralarm"), *gc = blk("galcenter"), *gn = blk("gumnebula");
+      const Element *lb = blk("localbubble"), *li = blk("loopi");
+      const double pc = units::pc, kpc = units::kpc, rad = units::rad;
+      double *p = tereg_ymw16;
+      int n = 0;
+      p[n++] = fval(w, "r_warp") * kpc;
+      p[n++] = fval(w, "r0") * kpc;
+      p[n++] = fval(w, "gamma_w");
+      for (const char *k : {"ad", "bd"})
+        p[n++] = fval(tk, k) * pc;
+      p[n++] = fval(tk, "n1");
+      p[n++] = fval(tk, "h1") * pc;
+      for (const char *k : {"a2", "b2"})
+        p[n++] = fval(tn, k) * pc;
+      p[n++] = fval(tn, "n2");
+      p[n++] = fval(tn, "k2");
+      p[n++] = fval(sa, "b2s") * pc;
+      auto arm = [&](const char *stem, double unit) {
+        for (int i = 0; i < 5; ++i)
+          p[n++] = fval(sa, (std::string(stem) + std::to_string(i)).c_str()) * unit;
+      };
+      arm("ele_arm_", 1.);
+      arm("wid_arm_", pc);
+      arm("rref_arm_", kpc);
+      arm("phiref_arm_", rad);
+      for (int i = 0; i < 5; ++i)
+        p[n++] = std::tan(fval(sa, ("pitch_arm_" + std::to_string(i)).c_str()) * rad);
+      for (int i = 0; i < 5; ++i)
+        p[n++] = std::cos(fval(sa, ("pitch_arm_" + std::to_string(i)).c_str()) * rad);
+      p[n++] = fval(sa, "aa") * pc;
+      for (const char *k : {"ka", "ncn", "thetacn", "wcn", "nsg", "thetasg", "wsg"})
+        p[n++] = fval(sa, k);
+      p[n++] = fval(gc, "ngc");
+      p[n++] = fval(gc, "agc") * pc;
+      p[n++] = fval(gc, "hgc") * pc;
+      p[n++] = fval(gn, "ngn");
+      p[n++] = fval(gn, "wgn") * pc;
+      p[n++] = fval(gn, "agn") * pc;
+      p[n++] = fval(gn, "kgn");
+      p[n++] = fval(lb, "j_lb");
+      for (const char *c : {"1", "2"}) {
+        const std::string s(c);
+        p[n++] = fval(lb, ("nlb" + s).c_str());
+        p[n++] = fval(lb, ("thetalb" + s).c_str());
+        p[n++] = fval(lb, ("detlb" + s).c_str());
+        p[n++] = fval(lb, ("wlb" + s).c_str()) * pc;
+        p[n++] = fval(lb, ("hlb" + s).c_str()) * pc;
+      }
+      p[n++] = fval(li, "nli");
+      p[n++] = fval(li, "rli") * pc;
+      p[n++] = fval(li, "wli") * pc;
+      p[n++] = fval(li, "detthetali");
+      p[n++] = fval(li, "thetali");
+      if (n != 73)
+        throw std::logic_error("ymw16 parameter count");
+    } else {
       throw std::runtime_error("unsupported tereg model");
     }
   }

@@ -177,6 +177,47 @@ BRND_JF12_PARAMS = tuple((f"b0_{i}", CGS.muGauss) for i in range(1, 9)) + (
     ("r0_halo", CGS.kpc), ("z0_halo", CGS.kpc))
 
 
+def ymw16_params(root: ET.Element) -> list[float]:
+    """The <ymw16> block in HAMX_TEREG_YMW16 order and CGS units
+    (param.cc:480-641): lengths in pc or kpc to cm, angles of the arms to
+    radians and their pitch to its tangent and cosine, the other angles kept
+    in degrees."""
+    def blk(name):
+        return _trace(root, ["thermalelectron", "regular", "ymw16", name])
+    w, tk, tn = blk("warp"), blk("thickdisk"), blk("thindisk")
+    sa, gc, gn = blk("spiralarm"), blk("galcenter"), blk("gumnebula")
+    lb, li = blk("localbubble"), blk("loopi")
+    pc, kpc, rad = CGS.pc, CGS.kpc, CGS.rad
+    arms = range(5)
+    pitch = [sa.f("value", f"pitch_arm_{i}") * rad for i in arms]
+    p = [w.f("value", "r_warp") * kpc, w.f("value", "r0") * kpc, w.f("value", "gamma_w"),
+         tk.f("value", "ad") * pc, tk.f("value", "bd") * pc, tk.f("value", "n1"),
+         tk.f("value", "h1") * pc,
+         tn.f("value", "a2") * pc, tn.f("value", "b2") * pc, tn.f("value", "n2"),
+         tn.f("value", "k2"),
+         sa.f("value", "b2s") * pc]
+    p += [sa.f("value", f"ele_arm_{i}") for i in arms]
+    p += [sa.f("value", f"wid_arm_{i}") * pc for i in arms]
+    p += [sa.f("value", f"rref_arm_{i}") * kpc for i in arms]
+    p += [sa.f("value", f"phiref_arm_{i}") * rad for i in arms]
+    p += [math.tan(a) for a in pitch] + [math.cos(a) for a in pitch]
+    p += [sa.f("value", "aa") * pc, sa.f("value", "ka"), sa.f("value", "ncn"),
+          sa.f("value", "thetacn"), sa.f("value", "wcn"), sa.f("value", "nsg"),
+          sa.f("value", "thetasg"), sa.f("value", "wsg"),
+          gc.f("value", "ngc"), gc.f("value", "agc") * pc, gc.f("value", "hgc") * pc,
+          gn.f("value", "ngn"), gn.f("value", "wgn") * pc, gn.f("value", "agn") * pc,
+          gn.f("value", "kgn"),
+          lb.f("value", "j_lb"),
+          lb.f("value", "nlb1"), lb.f("value", "thetalb1"), lb.f("value", "detlb1"),
+          lb.f("value", "wlb1") * pc, lb.f("value", "hlb1") * pc,
+          lb.f("value", "nlb2"), lb.f("value", "thetalb2"), lb.f("value", "detlb2"),
+          lb.f("value", "wlb2") * pc, lb.f("value", "hlb2") * pc,
+          li.f("value", "nli"), li.f("value", "rli") * pc, li.f("value", "wli") * pc,
+          li.f("value", "detthetali"), li.f("value", "thetali")]
+    assert len(p) == 73
+    return p
+
+
 class Param:
     """Parsed parameter set (param.cc:12-28)."""
 
@@ -410,7 +451,7 @@ class Param:
                 self.tereg_unif = dict(n0=s.f("value", "n0"),
                                        r0=s.f("value", "r0") * CGS.kpc)
             elif self.tereg_type == "ymw16":
-                pass
+                self.tereg_ymw16 = ymw16_params(root)
             else:
                 raise RuntimeError("unsupported tereg model")
         if g.read_permission or g.write_permission:
@@ -556,6 +597,9 @@ class Param:
         elif self.tereg_type == "unif":
             d.tereg_model = capi.HAMX_TEREG_UNIF
             d.tereg[0:2] = [self.tereg_unif["n0"], self.tereg_unif["r0"]]
+        elif self.tereg_type == "ymw16":
+            d.tereg_model = capi.HAMX_TEREG_YMW16
+            d.tereg[0:73] = self.tereg_ymw16
         else:
             raise RuntimeError(f"tereg model '{self.tereg_type}' has no device evaluator")
         # CRE: an imported grid takes precedence (integrator.cc:287,356)

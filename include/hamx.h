@@ -73,6 +73,17 @@ extern "C" {
 #define HAMX_TEREG_NONE 0 /* zero density */
 #define HAMX_TEREG_UNIF 1 /* "unif": tereg[0..1] = n0 [cm^-3], r0 [cm] */
 #define HAMX_TEREG_GRID 2 /* <fieldio><tereg read="1">: gather of fields.tereg */
+#define HAMX_TEREG_YMW16 3 /* "ymw16" (tereg_ymw16.cc:10-333, param.cc:480-641): tereg[0..72] =
+     [0] r_warp [cm], [1] r0 [cm], [2] gamma_w;  thick disk [3] ad, [4] bd [cm], [5] n1 [cm^-3], [6] h1 [cm];
+     thin disk [7] a2, [8] b2 [cm], [9] n2, [10] k2;  spiral arms [11] b2s [cm], [12..16] ele_arm_0..4,
+     [17..21] wid_arm_0..4 [cm], [22..26] rref_arm_0..4 [cm], [27..31] phiref_arm_0..4 [rad],
+     [32..36] tan(pitch_arm_0..4), [37..41] cos(pitch_arm_0..4), [42] aa [cm], [43] ka, [44] ncn,
+     [45] thetacn [deg], [46] wcn [deg], [47] nsg, [48] thetasg [deg], [49] wsg [deg];
+     Galactic centre [50] ngc, [51] agc, [52] hgc [cm];  Gum [53] ngn, [54] wgn, [55] agn [cm], [56] kgn;
+     Local Bubble [57] j_lb, [58] nlb1, [59] thetalb1 [deg], [60] detlb1 [deg], [61] wlb1, [62] hlb1 [cm],
+     [63] nlb2, [64] thetalb2, [65] detlb2 [deg], [66] wlb2, [67] hlb2 [cm];
+     Loop I [68] nli, [69] rli, [70] wli [cm], [71] detthetali, [72] thetali [deg] */
+#define HAMX_TEREG_NPARAM 80
 /* cosmic-ray electrons, Param::cre_type (param.cc:731-778) */
 #define HAMX_CRE_NONE 0 /* no CRE model: sync output is rejected */
 #define HAMX_CRE_ANA 1  /* "analytic": cre[0..6] = alpha, beta, theta, r0 [cm], z0 [cm], E0 [erg], j0 */
@@ -140,7 +151,7 @@ typedef struct hamx_los_desc {
   int32_t breg_model;
   double breg[HAMX_BREG_NPARAM];
   int32_t tereg_model;
-  double tereg[4];
+  double tereg[HAMX_TEREG_NPARAM];
   int32_t cre_model;
   double cre[8];
   int32_t schedule; /* HAMX_SCHED_* */

@@ -32,6 +32,8 @@ const double kTwoThirds = 0.666666666666666666666666666666666666667;
 const double kSqrt2 = 1.4142135623730950488016887242096980785697;
 const double kSqrt3 = 1.7320508075688772935274463415058723669428;
 const double kKpc = 3.0856775806e+21;
+const double kPc = 3.0856775806e+18;
+const double kRad = kPi / 180.;
 const double kM = 1.e2;
 const double kSec = 1.;
 const double kEv = 1.60217733e-12;
@@ -706,6 +708,185 @@ double sync_p(const Cre &cre, double freq, const V3 &pos, double bper) {
                  std::tgamma(-0.25 * index - 1. / 12.));
 }
 
+// TEreg_ymw16 (tereg_ymw16.cc:10-333), restated component by component with
+// the parameters named as Param::tereg_ymw16 names them (param.cc:480-641).
+struct Ymw16 {
+  const double *p;
+  double r_warp() const { return p[0]; }
+  double r0() const { return p[1]; }
+  static double sech_sq(double v) { return std::pow(1. / std::cosh(v), 2); }
+  // thick disk
+  double thick(double zz, double rr) const {
+    const double ad = p[3], bd = p[4], n1 = p[5], h1 = p[6];
+    if (zz > 10. * h1)
+      return 0.;
+    double gd = 1.;
+    if (rr > bd)
+      gd = sech_sq((rr - bd) / ad);
+    return n1 * gd * sech_sq(zz / h1);
+  }
+  // thin disk
+  double thin(double zz, double rr) const {
+    const double ad = p[3], bd = p[4], a2 = p[7], b2 = p[8], n2 = p[9], k2 = p[10];
+    const double h0 = k2 * (32 * kPc + 1.6e-3 * rr + (4.e-7 / kPc) * rr * rr);
+    if (zz > 10. * h0)
+      return 0.;
+    double gd = 1.;
+    if (rr > bd)
+      gd = sech_sq((rr - bd) / ad);
+    return n2 * gd * sech_sq((rr - b2) / a2) * sech_sq(zz / h0);
+  }
+  // spiral arms; the loop runs over arms 0..3 only, as the reference's does
+  double spiral(double xx, double yy, double zz, double rr) const {
+    const double ad = p[3], bd = p[4], b2s = p[11], aa = p[42], ka = p[43];
+    const double *narm = p + 12, *warm = p + 17, *rmin = p + 22, *phimin = p + 27;
+    const double *tpitch = p + 32, *cpitch = p + 37;
+    const double ncn = p[44], thetacn = p[45], wcn = p[46], nsg = p[47], thetasg = p[48],
+                 wsg = p[49];
+    double scaling = 1.;
+    if (rr > bd) {
+      if ((rr - bd) > 10. * ad)
+        return 0.;
+      scaling = sech_sq((rr - bd) / ad);
+    }
+    const double h0 = ka * (32 * kPc + 1.6e-3 * rr + (4.e-7 / kPc) * std::pow(rr, 2));
+    if (zz > 10. * h0)
+      return 0.;
+    scaling *= sech_sq(zz / h0);
+    if ((rr - b2s) > 10. * aa)
+      return 0.;
+    scaling *= sech_sq((rr - b2s) / aa);
+    double theta = std::atan2(yy, xx);
+    if (theta < 0)
+      theta += 2 * kPi;
+    double ne3s = 0.;
+    for (int i = 0; i < 4; ++i) {
+      double d_phi = theta - phimin[i];
+      if (d_phi < 0)
+        d_phi += 2. * kPi;
+      const double d = std::fabs(rmin[i] * std::exp(d_phi * tpitch[i]) - rr);
+      const double d_p = std::fabs(rmin[i] * std::exp((d_phi + 2. * kPi) * tpitch[i]) - rr);
+      const double smin = std::min(d, d_p) * cpitch[i];
+      if (smin > 10. * warm[i])
+        continue;
+      if (i != 2) {
+        ne3s += narm[i] * scaling * sech_sq(smin / warm[i]);
+      } else if (rr > 6 * kKpc && theta * kRad > thetacn) {
+        const double ga = (1. - nsg * std::exp(-std::pow((theta * kRad - thetasg) / wsg, 2))) *
+                          (1. + ncn) * sech_sq(smin / warm[i]);
+        ne3s += narm[i] * scaling * ga;
+      } else {
+        const double ga = (1. - nsg * std::exp(-std::pow((theta * kRad - thetasg) / wsg, 2))) *
+                          (1. + ncn * std::exp(-std::pow((theta * kRad - thetacn) / wcn, 2))) *
+                          sech_sq(smin / warm[i]);
+        ne3s += narm[i] * scaling * ga;
+      }
+    }
+    return ne3s;
+  }
+  double galcen(double xx, double yy, double zz) const {
+    const double ngc = p[50], agc = p[51], hgc = p[52];
+    const double Xgc = 50. * kPc, Ygc = 0., Zgc = -7. * kPc;
+    const double R2gc = (xx - Xgc) * (xx - Xgc) + (yy - Ygc) * (yy - Ygc);
+    if (R2gc > 10. * agc * agc)
+      return 0.;
+    const double Ar = std::exp(-R2gc / (agc * agc));
+    if (std::fabs(zz - Zgc) > 10. * hgc)
+      return 0.;
+    return ngc * Ar * sech_sq((zz - Zgc) / hgc);
+  }
+  double gum(double xx, double yy, double zz) const {
+    const double ngn = p[53], wgn = p[54], agn = p[55], kgn = p[56];
+    if (yy < 0 or xx > 0)
+      return 0.;
+    const double lc = 264. * kRad, bc = -4. * kRad, dc = 450. * kPc;
+    const double xc = dc * std::cos(bc) * std::sin(lc);
+    const double yc = r0() - dc * std::cos(bc) * std::cos(lc);
+    const double zc = dc * std::sin(bc);
+    const double theta = std::atan2(std::fabs(zz - zc),
+                                    std::sqrt((xx - xc) * (xx - xc) + (yy - yc) * (yy - yc)));
+    const double tantheta = std::tan(theta);
+    const double zp = (agn * kgn) / std::sqrt(1. + kgn * kgn / (tantheta * tantheta));
+    const double xyp = zp / tantheta;
+    const double xy_dist = std::sqrt(agn * agn - xyp * xyp) * double(agn > xyp);
+    const double alpha = std::atan2(kgn * xyp, xy_dist) + theta;
+    const double R2 = (xx - xc) * (xx - xc) + (yy - yc) * (yy - yc) + (zz - zc) * (zz - zc);
+    const double r2 = zp * zp + xyp * xyp;
+    const double D2min = (R2 + r2 - 2. * std::sqrt(R2 * r2)) * std::sin(alpha) * std::sin(alpha);
+    if (D2min > 10. * wgn * wgn)
+      return 0.;
+    return ngn * std::exp(-D2min / (wgn * wgn));
+  }
+  static double rlb(double xx, double yy, double zz) {
+    return std::sqrt(std::pow(((yy - 8.34 * kKpc) * 0.94 - 0.34 * zz), 2) + std::pow(xx, 2));
+  }
+  double localbubble(double xx, double yy, double zz, double ll, double Rlb) const {
+    if (yy < 0)
+      return 0.;
+    double nel = 0.;
+    const double rLB = rlb(xx, yy, zz);
+    const double nlb1 = p[58], thetalb1 = p[59], detlb1 = p[60], wlb1 = p[61], hlb1 = p[62];
+    const double nlb2 = p[63], thetalb2 = p[64], detlb2 = p[65], wlb2 = p[66], hlb2 = p[67];
+    const double dl1 = std::min(std::fabs(ll + 360. - thetalb1), std::fabs(thetalb1 - ll));
+    if (dl1 < 10. * detlb1 or (rLB - Rlb) < 10. * wlb1 or zz < 10. * hlb1)
+      nel += nlb1 * sech_sq(dl1 / detlb1) * sech_sq((rLB - Rlb) / wlb1) * sech_sq(zz / hlb1);
+    const double dl2 = std::min(std::fabs(ll + 360 - thetalb2), std::fabs(thetalb2 - ll));
+    if (dl2 < 10. * detlb2 or (rLB - Rlb) < 10. * wlb2 or zz < 10. * hlb2)
+      nel += nlb2 * sech_sq(dl2 / detlb2) * sech_sq((rLB - Rlb) / wlb2) * sech_sq(zz / hlb2);
+    return nel;
+  }
+  double nps(double xx, double yy, double zz) const {
+    const double nli = p[68], rli = p[69], wli = p[70], detthetali = p[71], thetali = p[72];
+    if (yy < 0)
+      return 0.;
+    const double theta_LI = thetali * kRad;
+    const double x_c = -10.156 * kPc, y_c = 8106.207 * kPc, z_c = 10.467 * kPc;
+    const double rLI = std::sqrt((xx - x_c) * (xx - x_c) + (yy - y_c) * (yy - y_c) +
+                                 (zz - z_c) * (zz - z_c));
+    const double theta =
+        std::acos(((xx - x_c) * std::cos(theta_LI) + (zz - z_c) * std::sin(theta_LI)) / rLI) /
+        kRad;
+    if (theta > 10. * detthetali or (rLI - rli) > 10. * wli)
+      return 0.;
+    return nli * std::exp(-std::pow((rLI - rli) / wli, 2)) *
+           std::exp(-std::pow(theta / detthetali, 2));
+  }
+  double field(const V3 &pos) const {
+    V3 g = v3(pos[1], -pos[0], pos[2]);
+    const double r_cyl = std::sqrt(g[0] * g[0] + g[1] * g[1]);
+    if (r_cyl >= r_warp())
+      g.e[2] -= p[2] * (r_cyl - r_warp()) * std::cos(std::atan2(g[1], g[0]));
+    if (length(g) > 25 * kKpc)
+      return 0.;
+    double comp[8] = {0.};
+    double w_lb = 0., w_gum = 0., w_loop = 0.;
+    const double ec_l = std::atan2(g[0], r0() - g[1]) / kRad;
+    comp[1] = thick(g[2], r_cyl);
+    comp[2] = thin(g[2], r_cyl);
+    comp[3] = spiral(g[0], g[1], g[2], r_cyl);
+    comp[4] = galcen(g[0], g[1], g[2]);
+    comp[5] = gum(g[0], g[1], g[2]);
+    const double Rlb = 110. * kPc;
+    comp[6] = localbubble(g[0], g[1], g[2], ec_l, Rlb);
+    comp[7] = nps(g[0], g[1], g[2]);
+    comp[0] = comp[1] + std::max(comp[2], comp[3]);
+    if (rlb(g[0], g[1], g[2]) < Rlb) {
+      comp[0] = p[57] * comp[1] + std::max(comp[2], comp[3]);
+      if (comp[6] > comp[0])
+        w_lb = 1;
+    } else if (comp[6] > comp[0] and comp[6] > comp[5]) {
+      w_lb = 1;
+    }
+    if (comp[7] > comp[0])
+      w_loop = 1;
+    if (comp[5] > comp[0])
+      w_gum = 1;
+    return (1 - w_lb) * ((1 - w_gum) * ((1 - w_loop) * (comp[0] + comp[4]) + w_loop * comp[7]) +
+                         w_gum * comp[5]) +
+           w_lb * comp[6];
+  }
+};
+
 // ------------------------------------------------ CRE grid (CRE_num) --
 // The synchrotron kernels the reference takes from GSL
 // (gsl_sf_synchrotron_1 / _2, integrator.cc:321,390), restated from their
@@ -884,6 +1065,8 @@ void radial(const hamx_los_desc *d, const hamx_fields *f, const Shell &sh,
       te = (length(sub(pos, obs)) > d->tereg[1]) ? 0. : d->tereg[0];
     else if (d->tereg_model == HAMX_TEREG_GRID)
       te = read_grid1(f->tereg, f->tereg_box, pos);
+    else if (d->tereg_model == HAMX_TEREG_YMW16)
+      te = Ymw16{d->tereg}.field(pos);
     if (f && f->ternd)
       te += read_grid1(f->ternd, f->ternd_box, pos);
     te *= double(te > 0.);
@@ -1022,6 +1205,10 @@ void orc_sync_emissivity_grid(const hamx_los_desc *d, const hamx_fields *f,
   const V3 p = v3(pos[0], pos[1], pos[2]);
   *j_tot = sync_grid(g, d->sync_freq, p, b_per, false);
   *j_pol = sync_grid(g, d->sync_freq, p, b_per, true);
+}
+
+double orc_tereg_ymw16(const double *p, const double pos[3]) {
+  return Ymw16{p}.field(v3(pos[0], pos[1], pos[2]));
 }
 
 double orc_cre_read_grid_num(const hamx_fields *f, const double pos[3], uint64_t e) {

@@ -77,6 +77,7 @@ void orc_sync_kernels(const double *x, uint64_t n, double *F, double *G);
 void orc_sync_emissivity_grid(const hamx_los_desc *d, const hamx_fields *f,
                               const double pos[3], double b_per, double *j_tot,
                               double *j_pol);
+double orc_tereg_ymw16(const double *p, const double pos[3]);
 double orc_cre_read_grid_num(const hamx_fields *f, const double pos[3], uint64_t e);
 int orc_cre_write_grid(const hamx_los_desc *d, const hamx_box *box, double E_min,
                        double E_fact, uint32_t nE, double *out);

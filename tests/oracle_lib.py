@@ -46,6 +46,7 @@ PROTOS = {
     "orc_sync_kernels": (None, [_vp, C.c_uint64, _vp, _vp]),
     "orc_sync_emissivity_grid": (None, [C.POINTER(capi.HamxLosDesc), C.POINTER(capi.HamxFields),
                                         _vp, C.c_double, _dp, _dp]),
+    "orc_tereg_ymw16": (C.c_double, [_vp, _vp]),
     "orc_cre_read_grid_num": (C.c_double, [C.POINTER(capi.HamxFields), _vp, C.c_uint64]),
     "orc_cre_write_grid": (C.c_int, [C.POINTER(capi.HamxLosDesc), C.POINTER(capi.HamxBox),
                                      C.c_double, C.c_double, C.c_uint32, _vp]),

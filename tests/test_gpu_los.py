@@ -157,6 +157,28 @@ def test_helix_jf12_jaffe_models(ctx, model, variant):
     assert_parity(gpu_maps(ctx, d, **kw), write_grid(d, **kw), label=f"model{model}")
 
 
+@pytest.mark.parametrize("sched", [capi.HAMX_SCHED_WAVE, capi.HAMX_SCHED_THREAD])
+@pytest.mark.parametrize("breg", ["lsa", "jf12"])
+def test_ymw16_thermal_electrons(ctx, sched, breg):
+    """YMW16 (tereg_ymw16.cc:10-333) at the template's parameters on the
+    device against the oracle's restatement, with random grids added; DM
+    probes the density directly, FD and the Faraday-rotated Q/U through B."""
+    from test_oracle_models import jf12_params
+    print(f"case: ymw16 schedule {sched} breg {breg}", flush=True)
+    par = Param(text=configs.random_forward(nside_sim=4, nside_out=8, oc_r_res=0.05,
+                                            grid=(16, 16, 8), tereg="ymw16"))
+    d = par.los_desc(schedule=sched)
+    assert d.tereg_model == capi.HAMX_TEREG_YMW16
+    if breg == "jf12":
+        d.breg_model = capi.HAMX_BREG_JF12
+        d.breg[0:20] = list(jf12_params())
+    b, te, box = random_grids((16, 16, 8), seed=13, te_scale=0.0003)
+    kw = dict(brnd=b, brnd_box=box, ternd=te, ternd_box=box)
+    ref = write_grid(d, **kw)
+    assert np.linalg.norm(ref["dm"]) > 0
+    assert_parity(gpu_maps(ctx, d, **kw), ref, label=f"ymw16/{sched}/{breg}")
+
+
 def test_empty_shell_and_no_samples(ctx):
     """A shell thinner than one step contributes nothing; so does a sky with
     every sample outside the galactocentric limits."""

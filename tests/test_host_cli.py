@@ -146,6 +146,28 @@ def test_cli_jf12_helix_jaffe_match_python_pipeline_and_oracle(tmp_path, ctx):
 
 
 @pytest.mark.gpu
+def test_cli_ymw16_matches_python_pipeline_and_oracle(tmp_path, ctx):
+    """<thermalelectron><regular type="ymw16">: the C++ host and the Python
+    pipeline read the reference's block (param.cc:480-641) and give the
+    oracle's maps."""
+    need_cli()
+    out = str(tmp_path) + "/"
+    text = configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05, out=out,
+                                 tereg="ymw16")
+    x = tmp_path / "y.xml"
+    x.write_text(text)
+    r = run(str(x))
+    assert r.returncode == 0, r.stderr
+    py = Pipeline(text=text, ctx=ctx, export=False).run()
+    ref = write_grid(Param(text=text).los_desc())
+    dm = np.fromfile(out + "dm.bin", dtype="<f8")
+    assert np.array_equal(dm, py[("dm", "nan", "8", "nan")])
+    assert rel_l2(dm, ref["dm"] * CGS.ccm / CGS.pc) < 1e-10
+    q = np.fromfile(out + "sync_23.0_Q.bin", dtype="<f8")
+    assert rel_l2(q, ref["qs"]) < 1e-10
+
+
+@pytest.mark.gpu
 def test_cli_random_jf12_matches_python_pipeline_and_oracle(tmp_path, ctx):
     """<random><global type="jf12">: the C++ host and the Python pipeline read
     the reference's block (param.cc:375-409), write the same Brnd_jf12 grid,
