@@ -88,6 +88,18 @@ class HamxEsDesc(C.Structure):
                 ("breg_box", HamxBox), ("jf12", C.c_double * 13)]
 
 
+class HamxMhdDesc(C.Structure):
+    _fields_ = [("box", HamxBox), ("observer", C.c_double * 3),
+                ("pa0", C.c_double), ("pf0", C.c_double), ("ps0", C.c_double),
+                ("k0", C.c_double), ("k1", C.c_double),
+                ("aa0", C.c_double), ("af0", C.c_double), ("as0", C.c_double),
+                ("a1", C.c_double), ("beta", C.c_double), ("ma", C.c_double),
+                ("breg_model", C.c_int32), ("reserved", C.c_int32),
+                ("breg", C.c_double * HAMX_BREG_NPARAM),
+                ("breg_x", C.c_void_p), ("breg_y", C.c_void_p), ("breg_z", C.c_void_p),
+                ("breg_box", HamxBox)]
+
+
 class HamxDftDesc(C.Structure):
     _fields_ = [("box", HamxBox), ("observer", C.c_double * 3),
                 ("rms", C.c_double), ("k0", C.c_double), ("a0", C.c_double),
@@ -115,6 +127,8 @@ _PROTOS = {
                                       C.c_uint32]),
     "hamx_sync_kernels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                     C.c_void_p, C.c_uint32]),
+    "hamx_brnd_mhd": (C.c_int, [C.c_void_p, C.POINTER(HamxMhdDesc), C.c_uint64,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hamx_brnd_jf12": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "hamx_ternd_dft": (C.c_int, [C.c_void_p, C.POINTER(HamxDftDesc), C.c_uint64,

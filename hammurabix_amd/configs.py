@@ -91,7 +91,14 @@ _TEMPLATE = """<?xml version="1.0"?>
         <chi0 value="25.0"/>
       </lsa>
     </regular>
-    <random cue="{brnd_on}" type="global" seed="{seed}">
+    <random cue="{brnd_on}" type="{brnd_type}" seed="{seed}">
+      <local type="mhd">
+        <mhd>
+          <pa0 value="1"/> <pf0 value="1"/> <ps0 value="1"/>
+          <k0 value="0.3"/> <aa0 value="1.7"/> <af0 value="1.5"/> <as0 value="1.7"/>
+          <k1 value="0.1"/> <a1 value="0.0"/> <beta value="0.1"/> <ma value="0.5"/>
+        </mhd>
+      </local>
       <global type="es">
         <es>
           <rms value="2.0"/>
@@ -186,11 +193,13 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         out: str = "", mask_file: str = "", nside_mask: int = 32,
         cre_file: str = "", cre_read: bool = False, cre_write: bool = False,
         cre_grid: tuple[int, int, int, int] = (40, 40, 8, 32),
-        cre_E: tuple[float, float] = (0.1, 1000.0), tereg: str = "unif") -> str:
+        cre_E: tuple[float, float] = (0.1, 1000.0), tereg: str = "unif",
+        brnd_type: str = "global") -> str:
     """The parameter file; cre_read / cre_write with cre_file switch on the
     CRE grid I/O on box_cre (cre_grid = nx, ny, nz, nE; cre_E = E_min, E_max
     in GeV); tereg picks the regular thermal electron model ("unif" or
-    "ymw16", the template's YMW16 values)."""
+    "ymw16", the template's YMW16 values); brnd_type "local" switches the
+    random B to the MHD generator (k0 0.3 /kpc, the template's other values)."""
     freqs = [23.0] if freqs is None else freqs
     return _TEMPLATE.format(
         do_dm=int(do_dm), do_fd=int(do_fd), nside_out=nside_out,
@@ -203,7 +212,7 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask,
         cre_read=int(cre_read), cre_write=int(cre_write), cre_file=cre_file or "cre.bin",
         cre_nx=cre_grid[0], cre_ny=cre_grid[1], cre_nz=cre_grid[2], cre_nE=cre_grid[3],
-        cre_E_min=cre_E[0], cre_E_max=cre_E[1], tereg=tereg)
+        cre_E_min=cre_E[0], cre_E_max=cre_E[1], tereg=tereg, brnd_type=brnd_type)
 
 
 def plain_forward(**kw) -> str:

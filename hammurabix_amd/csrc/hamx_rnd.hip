@@ -63,6 +63,14 @@ __device__ inline void gauss_pair(uint64_t seed, uint32_t stream, uint64_t m,
   g1 = r * s;
 }
 
+// A 53-bit uniform in [0, 1) from the same counter-based stream.
+__device__ inline double philox_uniform(uint64_t seed, uint32_t stream, uint64_t m) {
+  uint32_t c[4] = {static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), stream, 0u};
+  philox4x32_10(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+  const uint64_t a = (static_cast<uint64_t>(c[1]) << 32) | c[0];
+  return static_cast<double>(a >> 11) * 0x1.0p-53;
+}
+
 __global__ void k_gauss_pairs(double *out, uint64_t seed, uint32_t stream,
                               uint64_t first, uint64_t count) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -392,10 +400,11 @@ __global__ __launch_bounds__(256) void k_es_reassemble(double2 *c0, double2 *c1,
 }
 
 // Hermitian extraction / N (brnd_es.cc:274-307)
+// Hermitian extraction of the real field (brnd_es.cc:274-307 with
+// scale = 1/N; brnd_mhd.cc:139-167 with scale = 1).
 __global__ __launch_bounds__(256) void k_es_final(const double2 *c0, const double2 *c1,
                                                   double *bx, double *by, double *bz,
-                                                  BoxArgs b) {
-  const double inv_n = 1. / static_cast<double>(b.n);
+                                                  BoxArgs b, double inv_n) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < b.n; idx += stride) {
@@ -597,7 +606,244 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
   HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
   HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
   hipLaunchKernelGGL(k_es_final, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox,
-                     oy, oz, a.b);
+                     oy, oz, a.b, 1. / static_cast<double>(n));
+  HAMX_HIP(ctx, hipGetLastError());
+  if (!out_dev) {
+    HAMX_HIP(ctx, hipMemcpyAsync(bx, ox, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HAMX_HIP(ctx, hipMemcpyAsync(by, oy, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HAMX_HIP(ctx, hipMemcpyAsync(bz, oz, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return HAMX_OK;
+}
+// -------------------------------------------------------------- Brnd_mhd --
+// Brnd_mhd::write_grid (brnd_mhd.cc:16-167): Alfven, fast and slow modes
+// about the regular field at the observer, drawn in k-space and brought back
+// by one backward transform per complex array; no energy-profile rescale.
+struct MhdArgs {
+  BoxArgs b;
+  double pa0, pf0, ps0, k0, k1, aa0, af0, as0, a1, beta, ma;
+};
+
+// Brnd_mhd::spectrum_a / _f / _s (brnd_mhd.cc:211-262): one form, the mode
+// choosing its index a0 and norm p0.
+__device__ inline double mhd_spectrum(const MhdArgs &a, double k, double a0, double p0) {
+  const double unit = 0.25 / (cgs::pi * k * k);
+  const double band1 = static_cast<double>(k < a.k1);
+  const double band2 = static_cast<double>(k > a.k1) * static_cast<double>(k < a.k0);
+  const double band3 = static_cast<double>(k > a.k0);
+  const double P = band1 * pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.0) +
+                   band2 / pow(k / a.k0, a.a1) + band3 / pow(k / a.k0, a0);
+  return P * p0 * unit;
+}
+
+// dynamo, h_s, h_f, F_a = F_s (bfield.h:388-412, brnd_mhd.cc:181-209)
+__device__ inline double mhd_dynamo(double beta, double c) {
+  return 1 + 0.25 * beta * beta + beta * (1. - 2. * c * c);
+}
+__device__ inline double mhd_h_s(double beta, double c) {
+  const double sq = sqrt(mhd_dynamo(beta, c));
+  const double dpp = 1 + sq + 0.5 * beta;
+  const double dmm = 1 - sq - 0.5 * beta;
+  const double dmp = 1 - sq + 0.5 * beta;
+  const double den = 1 / (dmp * (c * c + dpp * dpp * (1 - c * c) / (dmm * dmm)));
+  return isfinite(den) ? 2. * c * c * den * static_cast<double>(den > 0) : 0.;
+}
+__device__ inline double mhd_h_f(double beta, double c) {
+  if (c == 0)
+    return 0;
+  const double sq = sqrt(mhd_dynamo(beta, c));
+  const double dpp = 1 + sq + 0.5 * beta;
+  const double dpm = 1 + sq - 0.5 * beta;
+  const double dmp = 1 - sq + 0.5 * beta;
+  const double den = 1 / (dpp * (c * c + dmp * dmp * (1 - c * c) / (dpm * dpm)));
+  return isfinite(den) ? 2. * c * c * den : 0.;
+}
+__device__ inline double mhd_aniso(double ma, double c) {
+  return exp(-1. * pow(ma, -1.33333333) * fabs(c) / pow(1 - c * c, 0.33333333));
+}
+
+// Hamvec::versor and crossprod (hamvec.h:176-205)
+__device__ inline Vec3 versor3(const Vec3 &v) {
+  double l2 = 0.;
+  l2 += v.x * v.x;
+  l2 += v.y * v.y;
+  l2 += v.z * v.z;
+  if (l2 == 0)
+    return Vec3{0., 0., 0.};
+  const double den = 1. / sqrt(l2);
+  return Vec3{v.x * den, v.y * den, v.z * den};
+}
+__device__ inline Vec3 cross3(const Vec3 &a, const Vec3 &b) {
+  return Vec3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Brnd_mhd::e_plus / e_minus (brnd_mhd.cc:169-179): normalised unless
+// their squared length is below 1e-12
+__device__ inline Vec3 unit_or_small(const Vec3 &t) {
+  double l2 = 0.;
+  l2 += t.x * t.x;
+  l2 += t.y * t.y;
+  l2 += t.z * t.z;
+  if (l2 < 1e-12)
+    return t;
+  const double l = sqrt(l2);
+  return Vec3{t.x / l, t.y / l, t.z / l};
+}
+
+// The regular field at the observer (breg->read_field(observer)).
+__global__ void k_mhd_bobs(EsArgs a, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0)
+    return;
+  Vec3 b{0., 0., 0.};
+  if (a.breg_model == HAMX_BREG_GRID) {
+    Cell c;
+    if (grid_cell(a.g_breg, a.obs, c)) {
+      const uint64_t sy = a.g_breg.nz, sx = a.g_breg.ny * a.g_breg.nz;
+      b = Vec3{lerp_cell(a.bgx, c, sy, sx), lerp_cell(a.bgy, c, sy, sx),
+               lerp_cell(a.bgz, c, sy, sx)};
+    }
+  } else {
+    b = breg_eval(a.breg_model, a.breg, a.obs);
+  }
+  out[0] = b.x;
+  out[1] = b.y;
+  out[2] = b.z;
+}
+
+// Philox streams of the MHD draws: Gaussians (2 pairs per mode) and the
+// fast/slow power share of the modes along the field.
+constexpr uint32_t kMhdGaussStream = 3, kMhdShareStream = 4;
+
+__global__ __launch_bounds__(256) void k_mhd_draw(double2 *c0, double2 *c1, MhdArgs a,
+                                                  uint64_t seed, const double *bobs) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const Vec3 B{bobs[0], bobs[1], bobs[2]};
+  const Vec3 bh = versor3(B);
+  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < a.b.n; idx += stride) {
+    if (idx == 0) {
+      c0[0] = double2{0., 0.};
+      c1[0] = double2{0., 0.};
+      continue;
+    }
+    const uint64_t i = idx / (a.b.ny * a.b.nz);
+    const uint64_t j = (idx / a.b.nz) % a.b.ny;
+    const uint64_t l = idx % a.b.nz;
+    const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
+                 wavenum(l, a.b.nz, a.b.lz)};
+    const double ks = length0(k);
+    const Vec3 kh = versor3(k);
+    const Vec3 kxb = cross3(kh, bh);
+    Vec3 ep = unit_or_small(kxb);
+    Vec3 em = unit_or_small(cross3(kxb, kh));
+    const double dk3 = a.b.dk3;
+    double g0, g1, g2, g3;
+    gauss_pair(seed, kMhdGaussStream, 2 * idx, g0, g1);
+    Vec3 bk;
+    if (dot0(ep, ep) > 1e-6) {
+      const double ang = dot0(bh, kh);
+      const double Pa = mhd_spectrum(a, ks, a.aa0, a.pa0) * mhd_aniso(a.ma, ang) * dk3;
+      const double Pf = mhd_spectrum(a, ks, a.af0, a.pf0) * mhd_h_f(a.beta, ang) * dk3;
+      const double Ps = mhd_spectrum(a, ks, a.as0, a.ps0) * mhd_aniso(a.ma, ang) *
+                        mhd_h_s(a.beta, ang) * dk3;
+      gauss_pair(seed, kMhdGaussStream, 2 * idx + 1, g2, g3);
+      const double Ap = g0 * sqrt(2. * Pa);
+      const double Am = g1 * sqrt(2. * Pf) + g2 * sqrt(2. * Ps);
+      bk = Vec3{ep.x * Ap + em.x * Am, ep.y * Ap + em.y * Am, ep.z * Ap + em.z * Am};
+    } else {
+      const double Pf = mhd_spectrum(a, ks, a.af0, a.pf0) * mhd_h_f(a.beta, 1) * dk3;
+      if (i == 0 && j == 0) {
+        ep.x = k.x;
+        em.y = k.y;
+      } else {
+        const double kp = sqrt(k.x * k.x + k.y * k.y);
+        ep = Vec3{k.x * k.z / (ks * kp), k.y * k.z / (ks * kp), -kp / ks};
+        em = Vec3{-k.y / kp, k.x / kp, 0.};
+      }
+      const double Af = g0 * sqrt(2. * Pf);
+      const double share = philox_uniform(seed, kMhdShareStream, idx);
+      bk = Vec3{ep.x * Af * share + em.x * Af * (1. - share),
+                ep.y * Af * share + em.y * Af * (1. - share),
+                ep.z * Af * share + em.z * Af * (1. - share)};
+    }
+    c0[idx] = double2{bk.x, bk.y};
+    c1[idx] = double2{bk.y, bk.z};
+  }
+}
+
+int brnd_mhd(hamx_ctx *ctx, const hamx_mhd_desc *d, uint64_t seed, double *bx, double *by,
+             double *bz, uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !bx || !by || !bz)
+    return set_err(ctx, HAMX_ERR_ARG, "hamx_brnd_mhd: null descriptor or output");
+  HAMX_TRY(check_gen_box(ctx, d->box));
+  if (d->breg_model < HAMX_BREG_NONE || d->breg_model > HAMX_BREG_JAFFE)
+    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "unsupported breg model %d", d->breg_model);
+  HAMX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool in_dev = (flags & HAMX_IN_DEVICE) != 0;
+  const bool out_dev = (flags & HAMX_OUT_DEVICE) != 0;
+  EsArgs e;
+  std::memset(&e, 0, sizeof(e));
+  e.obs = Vec3{d->observer[0], d->observer[1], d->observer[2]};
+  e.breg_model = d->breg_model;
+  std::memcpy(e.breg, d->breg, sizeof(e.breg));
+  if (d->breg_model == HAMX_BREG_GRID) {
+    if (!(d->breg_x && d->breg_y && d->breg_z))
+      return set_err(ctx, HAMX_ERR_ARG, "breg grid model without breg grid arrays");
+    if (d->breg_box.nx < 2 || d->breg_box.ny < 2 || d->breg_box.nz < 2)
+      return set_err(ctx, HAMX_ERR_ARG, "breg grid needs at least 2 vertices per axis");
+    const size_t nb = d->breg_box.nx * d->breg_box.ny * d->breg_box.nz * sizeof(double);
+    const void *p;
+    HAMX_TRY(stage_in(ctx, kSlotBregX, d->breg_x, nb, in_dev, &p));
+    e.bgx = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregY, d->breg_y, nb, in_dev, &p));
+    e.bgy = static_cast<const double *>(p);
+    HAMX_TRY(stage_in(ctx, kSlotBregZ, d->breg_z, nb, in_dev, &p));
+    e.bgz = static_cast<const double *>(p);
+    e.g_breg = grid_geo(d->breg_box);
+  }
+  MhdArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.b = box_args(d->box);
+  a.pa0 = d->pa0;
+  a.pf0 = d->pf0;
+  a.ps0 = d->ps0;
+  a.k0 = d->k0;
+  a.k1 = d->k1;
+  a.aa0 = d->aa0;
+  a.af0 = d->af0;
+  a.as0 = d->as0;
+  a.a1 = d->a1;
+  a.beta = d->beta;
+  a.ma = d->ma;
+  const uint64_t n = a.b.n;
+  void *w;
+  HAMX_TRY(ws_get(ctx, kSlotRndC0, n * sizeof(double2), &w));
+  double2 *c0 = static_cast<double2 *>(w);
+  HAMX_TRY(ws_get(ctx, kSlotRndC1, n * sizeof(double2), &w));
+  double2 *c1 = static_cast<double2 *>(w);
+  HAMX_TRY(ws_get(ctx, kSlotReduce, (16 + kRedBlocks) * sizeof(double), &w));
+  double *bobs = static_cast<double *>(w);
+  double *ox = bx, *oy = by, *oz = bz;
+  if (!out_dev) {
+    HAMX_TRY(ws_get(ctx, kSlotRndOut0, n * sizeof(double), &w));
+    ox = static_cast<double *>(w);
+    HAMX_TRY(ws_get(ctx, kSlotRndOut1, n * sizeof(double), &w));
+    oy = static_cast<double *>(w);
+    HAMX_TRY(ws_get(ctx, kSlotRndOut2, n * sizeof(double), &w));
+    oz = static_cast<double *>(w);
+  }
+  hipLaunchKernelGGL(k_mhd_bobs, dim3(1), dim3(64), 0, ctx->stream, e, bobs);
+  HAMX_HIP(ctx, hipGetLastError());
+  const unsigned blocks = ew_blocks(n);
+  hipLaunchKernelGGL(k_mhd_draw, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a, seed,
+                     static_cast<const double *>(bobs));
+  HAMX_HIP(ctx, hipGetLastError());
+  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  hipLaunchKernelGGL(k_es_final, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox, oy, oz,
+                     a.b, 1.);
   HAMX_HIP(ctx, hipGetLastError());
   if (!out_dev) {
     HAMX_HIP(ctx, hipMemcpyAsync(bx, ox, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -660,4 +906,9 @@ extern "C" int hamx_ternd_dft(hamx_ctx *ctx, const hamx_dft_desc *d, uint64_t se
     HAMX_HIP(ctx, hipMemcpyAsync(te, ot, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return HAMX_OK;
+}
+
+extern "C" int hamx_brnd_mhd(hamx_ctx *ctx, const hamx_mhd_desc *d, uint64_t seed,
+                             double *bx, double *by, double *bz, uint32_t flags) {
+  return brnd_mhd(ctx, d, seed, bx, by, bz, flags);
 }

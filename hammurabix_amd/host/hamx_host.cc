@@ -482,24 +482,12 @@ void TErnd::write_grid(const Param *, const TEreg *, const Grid_tereg *, Grid_te
   throw std::runtime_error("wrong inheritance");
 }
 
-// Brnd_es::write_grid (brnd_es.cc:76-308) on the device
-void Brnd_es::write_grid(const Param *par, const Breg *breg, const Grid_breg *gbreg,
-                         Grid_brnd *grid) const {
-  hamx_es_desc d;
-  std::memset(&d, 0, sizeof(d));
-  d.box = to_box(par->grid_brnd);
-  for (int i = 0; i < 3; ++i)
-    d.observer[i] = par->observer[i];
-  d.rms = par->brnd_es.rms;
-  d.k0 = par->brnd_es.k0;
-  d.k1 = par->brnd_es.k1;
-  d.a0 = par->brnd_es.a0;
-  d.a1 = par->brnd_es.a1;
-  d.rho = par->brnd_es.rho;
-  d.r0 = par->brnd_es.r0;
-  d.z0 = par->brnd_es.z0;
-  // anisotropy direction: the regular field (brnd_es.cc:19-22)
-  std::unique_ptr<Grid_breg> baked;
+// The regular field a random generator reads (brnd_es.cc:19-22,
+// brnd_mhd.cc:33): an imported grid, a device model, or a user model baked
+// onto box_breg (kept alive in `baked`).
+template <typename Desc>
+void generator_breg(const Param *par, const Breg *breg, const Grid_breg *gbreg, Desc &d,
+                    std::unique_ptr<Grid_breg> &baked) {
   if (par->grid_breg.read_permission && gbreg) {
     d.breg_model = HAMX_BREG_GRID;
   } else if (par->grid_breg.build_permission && breg) {
@@ -523,6 +511,55 @@ void Brnd_es::write_grid(const Param *par, const Breg *breg, const Grid_breg *gb
     d.breg_z = gbreg->bz.get();
     d.breg_box = to_box(par->grid_breg);
   }
+}
+
+// Brnd_mhd::write_grid (brnd_mhd.cc:16-167) on the device
+void Brnd_mhd::write_grid(const Param *par, const Breg *breg, const Grid_breg *gbreg,
+                          Grid_brnd *grid) const {
+  hamx_mhd_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.box = to_box(par->grid_brnd);
+  for (int i = 0; i < 3; ++i)
+    d.observer[i] = par->observer[i];
+  d.pa0 = par->brnd_mhd.pa0;
+  d.pf0 = par->brnd_mhd.pf0;
+  d.ps0 = par->brnd_mhd.ps0;
+  d.k0 = par->brnd_mhd.k0;
+  d.k1 = par->brnd_mhd.k1;
+  d.aa0 = par->brnd_mhd.aa0;
+  d.af0 = par->brnd_mhd.af0;
+  d.as0 = par->brnd_mhd.as0;
+  d.a1 = par->brnd_mhd.a1;
+  d.beta = par->brnd_mhd.beta;
+  d.ma = par->brnd_mhd.ma;
+  std::unique_ptr<Grid_breg> baked;
+  generator_breg(par, breg, gbreg, d, baked);
+  hamx_ctx *ctx = context();
+  check(hamx_brnd_mhd(ctx, &d, random_seed(par->brnd_seed), grid->bx.get(), grid->by.get(),
+                      grid->bz.get(), 0),
+        ctx);
+  grid->invalidate_device();
+}
+
+// Brnd_es::write_grid (brnd_es.cc:76-308) on the device
+void Brnd_es::write_grid(const Param *par, const Breg *breg, const Grid_breg *gbreg,
+                         Grid_brnd *grid) const {
+  hamx_es_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.box = to_box(par->grid_brnd);
+  for (int i = 0; i < 3; ++i)
+    d.observer[i] = par->observer[i];
+  d.rms = par->brnd_es.rms;
+  d.k0 = par->brnd_es.k0;
+  d.k1 = par->brnd_es.k1;
+  d.a0 = par->brnd_es.a0;
+  d.a1 = par->brnd_es.a1;
+  d.rho = par->brnd_es.rho;
+  d.r0 = par->brnd_es.r0;
+  d.z0 = par->brnd_es.z0;
+  // anisotropy direction: the regular field (brnd_es.cc:19-22)
+  std::unique_ptr<Grid_breg> baked;
+  generator_breg(par, breg, gbreg, d, baked);
   for (int i = 0; i < 13; ++i)
     d.jf12[i] = par->brnd_jf12[i];
   hamx_ctx *ctx = context();
@@ -851,6 +888,8 @@ void Pipeline::assemble_brnd() {
       brnd = std::make_unique<Brnd_es>();
     else if (par->brnd_type == "global" && par->brnd_method == "jf12")
       brnd = std::make_unique<Brnd_jf12>();
+    else if (par->brnd_type == "local" && par->brnd_method == "mhd")
+      brnd = std::make_unique<Brnd_mhd>();
     else
       throw std::runtime_error("unsupported brnd model");
     brnd->write_grid(par.get(), breg.get(), grid_breg.get(), grid_brnd.get());

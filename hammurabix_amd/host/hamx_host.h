@@ -82,6 +82,11 @@ public:
   // Brnd_jf12: spectrum in brnd_es (r0, z0 unused); energy profile b0_1..b0_8,
   // b0_int, z0_spiral, b0_halo, r0_halo, z0_halo (param.cc:375-409)
   double brnd_jf12[13] = {};
+  // Brnd_mhd (param.cc:424-440); power norms in G^2
+  struct {
+    double pa0 = 0, pf0 = 0, ps0 = 0, k0 = 0, k1 = 0, aa0 = 0, af0 = 0, as0 = 0, a1 = 0,
+           beta = 0, ma = 0;
+  } brnd_mhd;
   std::string tereg_type, ternd_type, ternd_method;
   struct { double n0 = 0, r0 = 0; } tereg_unif;
   // YMW16 in HAMX_TEREG_YMW16 order and CGS units (param.cc:480-641)
@@ -243,6 +248,12 @@ public:
 
 protected:
   virtual int generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, Grid_brnd *grid) const;
+};
+// Brnd_mhd (brnd_mhd.cc): Alfven, fast and slow modes
+class Brnd_mhd : public Brnd {
+public:
+  void write_grid(const Param *par, const Breg *breg, const Grid_breg *gbreg,
+                  Grid_brnd *grid) const override;
 };
 // Brnd_jf12 (brnd_jf12.cc): Brnd_es with the JF12 energy profile
 class Brnd_jf12 : public Brnd_es {

@@ -269,6 +269,19 @@ Param::Param(const std::string &xml_file) {
       brnd_method = trace(mf, {"random", "local"})->attr_string("type");
       if (brnd_method != "mhd")
         throw std::runtime_error("unsupported brnd model");
+      const Element *s = trace(mf, {"random", "local", "mhd"});
+      const double mu2 = units::muGauss * units::muGauss;
+      brnd_mhd.pa0 = fval(s, "pa0") * mu2;
+      brnd_mhd.pf0 = fval(s, "pf0") * mu2;
+      brnd_mhd.ps0 = fval(s, "ps0") * mu2;
+      brnd_mhd.k0 = fval(s, "k0");
+      brnd_mhd.aa0 = fval(s, "aa0");
+      brnd_mhd.af0 = fval(s, "af0");
+      brnd_mhd.as0 = fval(s, "as0");
+      brnd_mhd.k1 = fval(s, "k1");
+      brnd_mhd.a1 = fval(s, "a1");
+      brnd_mhd.beta = fval(s, "beta");
+      brnd_mhd.ma = fval(s, "ma");
     } else {
       throw std::runtime_error("unsupported brnd type");
     }

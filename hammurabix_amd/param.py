@@ -433,6 +433,15 @@ class Param:
                 self.brnd_method = lc.s("type")
                 if self.brnd_method != "mhd":
                     raise RuntimeError("unsupported brnd model")
+                s = _trace(root, ["magneticfield", "random", "local", "mhd"])
+                mu2 = CGS.muGauss * CGS.muGauss
+                self.brnd_mhd = dict(pa0=s.f("value", "pa0") * mu2,
+                                     pf0=s.f("value", "pf0") * mu2,
+                                     ps0=s.f("value", "ps0") * mu2,
+                                     k0=s.f("value", "k0"), aa0=s.f("value", "aa0"),
+                                     af0=s.f("value", "af0"), as0=s.f("value", "as0"),
+                                     k1=s.f("value", "k1"), a1=s.f("value", "a1"),
+                                     beta=s.f("value", "beta"), ma=s.f("value", "ma"))
             else:
                 raise RuntimeError("unsupported brnd type")
         if g.read_permission or g.write_permission or g.build_permission:
@@ -619,6 +628,14 @@ class Param:
         d.rms, d.k0, d.k1 = p["rms"], p["k0"], p["k1"]
         d.a0, d.a1, d.rho = p["a0"], p["a1"], p["rho"]
         d.r0, d.z0 = p["r0"], p["z0"]
+        self._generator_breg(d)
+        if self.brnd_method == "jf12":
+            d.jf12[:] = self.brnd_jf12
+        return d
+
+    def _generator_breg(self, d) -> None:
+        """The regular field a random generator reads (anisotropy direction,
+        the field at the observer): the LOS descriptor's model."""
         ld = self.los_desc() if self.grid_obs.write_permission else None
         if ld is not None:
             d.breg_model = ld.breg_model
@@ -629,8 +646,15 @@ class Param:
             d.breg[0:4] = [q["b0"], q["psi0"], q["psi1"], q["chi0"]]
         if d.breg_model == capi.HAMX_BREG_GRID:
             d.breg_box = self.grid_breg.box()
-        if self.brnd_method == "jf12":
-            d.jf12[:] = self.brnd_jf12
+
+    def mhd_desc(self) -> capi.HamxMhdDesc:
+        """Descriptor of Brnd_mhd::write_grid on box_brnd (brnd_mhd.cc:16-167)."""
+        d = capi.HamxMhdDesc()
+        d.box = self.grid_brnd.box()
+        d.observer[:] = self.observer
+        for k, v in self.brnd_mhd.items():
+            setattr(d, k, v)
+        self._generator_breg(d)
         return d
 
     def dft_desc(self) -> capi.HamxDftDesc:

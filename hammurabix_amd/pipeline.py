@@ -119,14 +119,18 @@ class Pipeline:
             self.brnd = tuple(self._torch.from_numpy(np.ascontiguousarray(raw[:, i]))
                               .to(self._dev) for i in range(3))
         elif g.build_permission:
-            if self.par.brnd_type != "global" or self.par.brnd_method not in ("es", "jf12"):
+            kind = (self.par.brnd_type, self.par.brnd_method)
+            if kind not in (("global", "es"), ("global", "jf12"), ("local", "mhd")):
                 raise RuntimeError(f"random B '{self.par.brnd_type}/{self.par.brnd_method}'"
                                    " has no device generator")
             self.brnd = tuple(self._zeros(g.full_size) for _ in range(3))
-            desc = self.par.es_desc()
             breg = getattr(self, "breg_grid", None)
-            gen = self.ctx.brnd_jf12 if self.par.brnd_method == "jf12" else self.ctx.brnd_es
-            gen(desc, self.par.brnd_seed, *self.brnd, breg_grid=breg)
+            if kind[1] == "mhd":
+                self.ctx.brnd_mhd(self.par.mhd_desc(), self.par.brnd_seed, *self.brnd,
+                                  breg_grid=breg)
+            else:
+                gen = self.ctx.brnd_jf12 if kind[1] == "jf12" else self.ctx.brnd_es
+                gen(self.par.es_desc(), self.par.brnd_seed, *self.brnd, breg_grid=breg)
         if g.write_permission and self.brnd is not None:
             inter = np.stack([a.cpu().numpy() for a in self.brnd], axis=1)
             export_map(g.filename, inter.ravel())

@@ -235,6 +235,12 @@ class Context:
         energy profile taken from desc.jf12."""
         self._brnd(self.lib.hamx_brnd_jf12, desc, seed, bx, by, bz, breg_grid)
 
+    def brnd_mhd(self, desc: capi.HamxMhdDesc, seed: int, bx: Any, by: Any, bz: Any,
+                 breg_grid: Optional[tuple[Any, Any, Any]] = None) -> None:
+        """Brnd_mhd::write_grid (brnd_mhd.cc:16-167): Alfven, fast and slow
+        modes about the regular field at the observer."""
+        self._brnd(self.lib.hamx_brnd_mhd, desc, seed, bx, by, bz, breg_grid)
+
     def _brnd(self, fn, desc, seed, bx, by, bz, breg_grid) -> None:
         flags = capi.HAMX_OUT_DEVICE if _residency([bx, by, bz], "output") else 0
         if breg_grid is not None:

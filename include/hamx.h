@@ -188,6 +188,23 @@ typedef struct hamx_es_desc {
   double jf12[13];
 } hamx_es_desc;
 
+/* Brnd_mhd generator (brnd_mhd.cc:16-262) on box `box`: Alfven, fast and
+ * slow modes about the regular field at the observer (breg_model / breg, or
+ * the grid in breg_* for HAMX_BREG_GRID), Param::brnd_mhd (param.cc:424-440). */
+typedef struct hamx_mhd_desc {
+  hamx_box box;
+  double observer[3];
+  double pa0, pf0, ps0;     /* mode power norms, G^2 */
+  double k0, k1;            /* 1/kpc */
+  double aa0, af0, as0, a1; /* spectral indices */
+  double beta, ma;          /* plasma beta, Alfven Mach number */
+  int32_t breg_model;
+  int32_t reserved;
+  double breg[HAMX_BREG_NPARAM];
+  const double *breg_x, *breg_y, *breg_z;
+  hamx_box breg_box;
+} hamx_mhd_desc;
+
 /* TErnd_dft generator (ternd_dft.cc:35-128) on box `box`. */
 typedef struct hamx_dft_desc {
   hamx_box box;
@@ -355,6 +372,13 @@ void hamx_batch_destroy(hamx_batch *b);
  * Outputs bx, by, bz: nx*ny*nz doubles each. */
 int hamx_brnd_es(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed,
                  double *bx, double *by, double *bz, uint32_t flags);
+
+/* Brnd_mhd::write_grid (bfield.h:380-381, brnd_mhd.cc:16-167): k-space
+ * modes with one backward transform per complex array and the real parts
+ * extracted; Gaussian and uniform variates from the same counter-based
+ * stream as hamx_brnd_es. */
+int hamx_brnd_mhd(hamx_ctx *ctx, const hamx_mhd_desc *d, uint64_t seed,
+                  double *bx, double *by, double *bz, uint32_t flags);
 
 /* CREfield::write_grid (crefield.h:53, cre.cc:204-225) for the analytic and
  * uniform models: out[((i*ny + j)*nz + k)*nE + m] = write_field(pos_ijk, E_m)

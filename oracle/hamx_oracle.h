@@ -86,6 +86,10 @@ int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
 int orc_brnd_jf12(const hamx_es_desc *d, uint64_t seed, double *bx, double *by,
                   double *bz);
 double orc_jf12_profile(const double *q, const double pos[3]);
+int orc_brnd_mhd(const hamx_mhd_desc *d, uint64_t seed, double *bx, double *by,
+                 double *bz);
+void orc_mhd_terms(double beta, double ma, double cosa, double out[3]);
+double orc_mhd_spectrum(const hamx_mhd_desc *d, double k, double a0, double p0);
 int orc_ternd_dft(const hamx_dft_desc *d, uint64_t seed, double *te);
 double orc_es_spectrum(const hamx_es_desc *d, double k);
 double orc_dft_spectrum(const hamx_dft_desc *d, double k);

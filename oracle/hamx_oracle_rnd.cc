@@ -222,6 +222,87 @@ V3 breg_at(const hamx_es_desc *d, const double pos[3]) {
   return v;
 }
 
+// ---------------------------------------------------------------- MHD --
+// A 53-bit uniform in [0, 1) from the counter-based stream.
+double uniform53(uint64_t seed, uint32_t stream, uint64_t m) {
+  const uint32_t ctr[4] = {static_cast<uint32_t>(m), static_cast<uint32_t>(m >> 32), stream, 0u};
+  const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+  uint32_t x[4];
+  philox(ctr, key, x);
+  const uint64_t a = (static_cast<uint64_t>(x[1]) << 32) | x[0];
+  return static_cast<double>(a >> 11) * 0x1.0p-53;
+}
+
+// Brnd_mhd (brnd_mhd.cc:16-262, bfield.h:370-443), with the parameters of
+// Param::brnd_mhd.
+struct Mhd {
+  const hamx_mhd_desc *d;
+  double spectrum(double k, double a0, double p0) const {
+    const double unit = 0.25 / (kPi * k * k);
+    const double band1 = double(k < d->k1);
+    const double band2 = double(k > d->k1) * double(k < d->k0);
+    const double band3 = double(k > d->k0);
+    const double P = band1 * std::pow(d->k0 / d->k1, d->a1) * std::pow(k / d->k1, 6.0) +
+                     band2 / std::pow(k / d->k0, d->a1) + band3 / std::pow(k / d->k0, a0);
+    return P * p0 * unit;
+  }
+  static double dynamo(double beta, double cosa) {
+    return 1 + 0.25 * beta * beta + beta * (1. - 2. * cosa * cosa);
+  }
+  static double h_s(double beta, double cosa) {
+    const double sqrtD = std::sqrt(dynamo(beta, cosa));
+    const double dpp = 1 + sqrtD + 0.5 * beta;
+    const double dmm = 1 - sqrtD - 0.5 * beta;
+    const double dmp = 1 - sqrtD + 0.5 * beta;
+    const double demon = 1 / (dmp * (cosa * cosa + dpp * dpp * (1 - cosa * cosa) / (dmm * dmm)));
+    if (std::isfinite(demon))
+      return 2. * cosa * cosa * demon * (demon > 0);
+    return 0;
+  }
+  static double h_f(double beta, double cosa) {
+    if (cosa == 0)
+      return 0;
+    const double sqrtD = std::sqrt(dynamo(beta, cosa));
+    const double dpp = 1 + sqrtD + 0.5 * beta;
+    const double dpm = 1 + sqrtD - 0.5 * beta;
+    const double dmp = 1 - sqrtD + 0.5 * beta;
+    const double demon = 1 / (dpp * (cosa * cosa + dmp * dmp * (1 - cosa * cosa) / (dpm * dpm)));
+    if (std::isfinite(demon))
+      return 2. * cosa * cosa * demon;
+    return 0;
+  }
+  static double F(double ma, double cosa) {
+    return std::exp(-1. * std::pow(ma, -1.33333333) * std::fabs(cosa) /
+                    std::pow(1 - cosa * cosa, 0.33333333));
+  }
+};
+
+V3 versor(const V3 &v) {
+  V3 t = {{0., 0., 0.}};
+  const double l2 = lensq(v);
+  if (l2 != 0) {
+    const double den = 1. / std::sqrt(l2);
+    for (int c = 0; c < 3; ++c)
+      t.e[c] = v.e[c] * den;
+  }
+  return t;
+}
+V3 crossv(const V3 &a, const V3 &b) {
+  V3 t;
+  for (int i = 0; i < 3; ++i)
+    t.e[i] = a.e[(i + 1) % 3] * b.e[(i + 2) % 3] - a.e[(i + 2) % 3] * b.e[(i + 1) % 3];
+  return t;
+}
+V3 normalised_unless_small(const V3 &t) {
+  if (lensq(t) < 1e-12)
+    return t;
+  const double l = std::sqrt(lensq(t));
+  V3 r;
+  for (int c = 0; c < 3; ++c)
+    r.e[c] = t.e[c] / l;
+  return r;
+}
+
 } // namespace
 
 extern "C" {
@@ -425,6 +506,105 @@ static int brnd_generate(const hamx_es_desc *d, uint64_t seed, double *bx, doubl
     }
   }
   return 0;
+}
+
+int orc_brnd_mhd(const hamx_mhd_desc *d, uint64_t seed, double *bx, double *by, double *bz) {
+  const uint64_t nx = d->box.nx, ny = d->box.ny, nz = d->box.nz;
+  const uint64_t N = nx * ny * nz;
+  std::vector<double> c0(2 * N, 0.), c1(2 * N, 0.);
+  const double lx = d->box.x_max - d->box.x_min;
+  const double ly = d->box.y_max - d->box.y_min;
+  const double lz = d->box.z_max - d->box.z_min;
+  hamx_es_desc e;
+  std::memset(&e, 0, sizeof(e));
+  e.breg_model = d->breg_model;
+  std::memcpy(e.breg, d->breg, sizeof(e.breg));
+  e.breg_x = d->breg_x;
+  e.breg_y = d->breg_y;
+  e.breg_z = d->breg_z;
+  e.breg_box = d->breg_box;
+  const V3 B = breg_at(&e, d->observer);
+  const V3 bv = versor(B);
+  const Mhd m{d};
+  const double dk3 = kKpc * kKpc * kKpc / (lx * ly * lz);
+#pragma omp parallel for
+  for (int64_t i = 0; i < (int64_t)nx; ++i) {
+    for (uint64_t j = 0; j < ny; ++j) {
+      for (uint64_t l = 0; l < nz; ++l) {
+        const uint64_t idx = i * ny * nz + j * nz + l;
+        if (idx == 0)
+          continue;
+        const V3 k = {{wavenum(i, nx, lx), wavenum(j, ny, ly), wavenum(l, nz, lz)}};
+        const double ks = std::sqrt(lensq(k));
+        const V3 kv = versor(k);
+        V3 ep = normalised_unless_small(crossv(kv, bv));
+        V3 em = normalised_unless_small(crossv(crossv(kv, bv), kv));
+        double g0, g1, g2, g3;
+        gauss_pair(seed, 3, 2 * idx, g0, g1);
+        V3 b;
+        if (lensq(ep) > 1e-6) {
+          const double ang = dotv(bv, kv);
+          const double Pa = m.spectrum(ks, d->aa0, d->pa0) * Mhd::F(d->ma, ang) * dk3;
+          const double Pf = m.spectrum(ks, d->af0, d->pf0) * Mhd::h_f(d->beta, ang) * dk3;
+          const double Ps = m.spectrum(ks, d->as0, d->ps0) * Mhd::F(d->ma, ang) *
+                            Mhd::h_s(d->beta, ang) * dk3;
+          gauss_pair(seed, 3, 2 * idx + 1, g2, g3);
+          const double Ap = g0 * std::sqrt(2. * Pa);
+          const double Am = g1 * std::sqrt(2. * Pf) + g2 * std::sqrt(2. * Ps);
+          for (int c = 0; c < 3; ++c)
+            b.e[c] = ep.e[c] * Ap + em.e[c] * Am;
+        } else {
+          const double Pf = m.spectrum(ks, d->af0, d->pf0) * Mhd::h_f(d->beta, 1) * dk3;
+          if (i == 0 and j == 0) {
+            ep.e[0] = k.e[0];
+            em.e[1] = k.e[1];
+          } else {
+            const double kp = std::sqrt(k.e[0] * k.e[0] + k.e[1] * k.e[1]);
+            ep = V3{{k.e[0] * k.e[2] / (ks * kp), k.e[1] * k.e[2] / (ks * kp), -kp / ks}};
+            em = V3{{-k.e[1] / kp, k.e[0] / kp, 0.}};
+          }
+          const double Af = g0 * std::sqrt(2. * Pf);
+          const double share = uniform53(seed, 4, idx);
+          for (int c = 0; c < 3; ++c)
+            b.e[c] = ep.e[c] * Af * share + em.e[c] * Af * (1. - share);
+        }
+        c0[2 * idx] = b.e[0];
+        c0[2 * idx + 1] = b.e[1];
+        c1[2 * idx] = b.e[1];
+        c1[2 * idx + 1] = b.e[2];
+      }
+    }
+  }
+  fft3d(c0.data(), nx, ny, nz, +1);
+  fft3d(c1.data(), nx, ny, nz, +1);
+  // real parts, symmetrised (brnd_mhd.cc:139-167)
+#pragma omp parallel for
+  for (int64_t i = 0; i < (int64_t)nx; ++i) {
+    const uint64_t is = i == 0 ? 0 : nx - i;
+    for (uint64_t j = 0; j < ny; ++j) {
+      const uint64_t js = j == 0 ? 0 : ny - j;
+      for (uint64_t l = 0; l < nz; ++l) {
+        const uint64_t ls = l == 0 ? 0 : nz - l;
+        const uint64_t idx = i * ny * nz + j * nz + l;
+        const uint64_t sym = is * ny * nz + js * nz + ls;
+        bx[idx] = 0.5 * (c0[2 * idx] + c0[2 * sym]);
+        by[idx] = 0.5 * (c1[2 * idx] + c1[2 * sym]);
+        bz[idx] = 0.5 * (c1[2 * idx + 1] + c1[2 * sym + 1]);
+      }
+    }
+  }
+  return 0;
+}
+
+// h_f, h_s and F_a (= F_s) of Brnd_mhd
+void orc_mhd_terms(double beta, double ma, double cosa, double out[3]) {
+  out[0] = Mhd::h_f(beta, cosa);
+  out[1] = Mhd::h_s(beta, cosa);
+  out[2] = Mhd::F(ma, cosa);
+}
+
+double orc_mhd_spectrum(const hamx_mhd_desc *d, double k, double a0, double p0) {
+  return Mhd{d}.spectrum(k, a0, p0);
 }
 
 int orc_brnd_es(const hamx_es_desc *d, uint64_t seed, double *bx, double *by, double *bz) {

@@ -52,6 +52,10 @@ PROTOS = {
                                      C.c_double, C.c_double, C.c_uint32, _vp]),
     "orc_brnd_jf12": (C.c_int, [C.POINTER(capi.HamxEsDesc), C.c_uint64, _vp, _vp, _vp]),
     "orc_jf12_profile": (C.c_double, [_vp, _vp]),
+    "orc_brnd_mhd": (C.c_int, [C.POINTER(capi.HamxMhdDesc), C.c_uint64, _vp, _vp, _vp]),
+    "orc_mhd_terms": (None, [C.c_double, C.c_double, C.c_double, _vp]),
+    "orc_mhd_spectrum": (C.c_double, [C.POINTER(capi.HamxMhdDesc), C.c_double, C.c_double,
+                                      C.c_double]),
     "orc_ternd_dft": (C.c_int, [C.POINTER(capi.HamxDftDesc), C.c_uint64, _vp]),
     "orc_es_spectrum": (C.c_double, [C.POINTER(capi.HamxEsDesc), C.c_double]),
     "orc_dft_spectrum": (C.c_double, [C.POINTER(capi.HamxDftDesc), C.c_double]),

@@ -164,6 +164,53 @@ def test_jf12_matches_oracle(ctx, rho):
         assert rel_l2(c, b) > 1e-3
 
 
+def mhd_desc(shape, model, b=(1e-6, 0.7e-6, 0.4e-6)):
+    d = capi.HamxMhdDesc()
+    d.box = capi.HamxBox(*shape, *BOX)
+    d.observer[:] = OBS
+    mu2 = CGS.muGauss ** 2
+    d.pa0, d.pf0, d.ps0 = 1.0 * mu2, 0.8 * mu2, 1.2 * mu2
+    d.k0, d.k1, d.a1, d.aa0, d.af0, d.as0 = 0.3, 0.1, 0.0, 1.7, 1.5, 1.7
+    d.beta, d.ma = 0.1, 0.5
+    d.breg_model = model
+    if model == capi.HAMX_BREG_CART:
+        d.breg[0:3] = list(b)
+    elif model == capi.HAMX_BREG_LSA:
+        d.breg[0:4] = [6e-6, 27 * CGS.rad, 0.9 * CGS.rad, 25 * CGS.rad]
+    return d
+
+
+@pytest.mark.parametrize("case", ["cart", "cart_z", "lsa", "none", "grid"])
+def test_mhd_matches_oracle(ctx, case):
+    """Brnd_mhd on the device against the oracle with the same Philox
+    streams: a tilted uniform field, one along z (the modes along the field
+    take the degenerate branch, kx = ky = 0 its special case), LSA, no
+    regular field (every mode degenerate) and an imported regular grid."""
+    orc = load_oracle()
+    print("case: mhd", case, flush=True)
+    shape = (32, 32, 16)
+    model = {"cart": capi.HAMX_BREG_CART, "cart_z": capi.HAMX_BREG_CART,
+             "lsa": capi.HAMX_BREG_LSA, "none": capi.HAMX_BREG_NONE,
+             "grid": capi.HAMX_BREG_GRID}[case]
+    d = mhd_desc(shape, model, b=(0.0, 0.0, 2e-6) if case == "cart_z" else (1e-6, 0.7e-6, 0.4e-6))
+    grid = None
+    if case == "grid":
+        rng = np.random.default_rng(5)
+        grid = tuple(np.ascontiguousarray(rng.normal(size=10 * 10 * 6) * 1e-6) for _ in range(3))
+        d.breg_box = capi.HamxBox(10, 10, 6, *BOX)
+        d.breg_x, d.breg_y, d.breg_z = (ptr(a) for a in grid)
+    n = int(np.prod(shape))
+    g = [np.zeros(n) for _ in range(3)]
+    r = [np.zeros(n) for _ in range(3)]
+    assert orc.orc_brnd_mhd(C.byref(d), 9, *(ptr(a) for a in r)) == 0
+    ctx.brnd_mhd(d, 9, *g, breg_grid=grid)
+    for a, b in zip(g, r):
+        assert np.linalg.norm(b) > 0
+        err = rel_l2(a, b)
+        print("brnd_mhd rel L2", err)
+        assert err < 1e-11
+
+
 def test_es_with_regular_field_grid(ctx):
     """Anisotropy direction from an imported breg grid (Breg::read_grid)."""
     orc = load_oracle()

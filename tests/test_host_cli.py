@@ -168,6 +168,41 @@ def test_cli_ymw16_matches_python_pipeline_and_oracle(tmp_path, ctx):
 
 
 @pytest.mark.gpu
+def test_cli_random_mhd_matches_python_pipeline_and_oracle(tmp_path, ctx):
+    """<random type="local"><mhd>: the C++ host and the Python pipeline read
+    the reference's block (param.cc:419-440), write the same Brnd_mhd grid,
+    and that grid is the oracle's."""
+    import ctypes as C
+    from oracle_lib import load_oracle, ptr
+    need_cli()
+    out = str(tmp_path) + "/"
+    text = configs.random_forward(nside_sim=4, nside_out=16, oc_r_res=0.1, grid=(32, 32, 16),
+                                  te_k0=0.3, freqs=[23.0], out=out, brnd_type="local")
+    bfile = str(tmp_path / "brnd.bin")
+    text = text.replace('<brnd read="0" write="0" filename="brnd.bin"/>',
+                        f'<brnd read="0" write="1" filename="{bfile}"/>')
+    x = tmp_path / "p.xml"
+    x.write_text(text)
+    r = run(str(x))
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(bfile, dtype="<f8").reshape(-1, 3)
+    pipe = Pipeline(text=text, ctx=ctx, export=False)
+    py = pipe.run()
+    for i in range(3):
+        assert np.array_equal(raw[:, i], pipe.brnd[i].cpu().numpy()), i
+    q = np.fromfile(str(tmp_path / "sync_23.0_Q.bin"), dtype="<f8")
+    assert np.array_equal(q, py[("sync", "23.0", "16", "Q")])
+    par = Param(text=text)
+    n = par.grid_brnd.full_size
+    ref = [np.zeros(n) for _ in range(3)]
+    assert load_oracle().orc_brnd_mhd(C.byref(par.mhd_desc()), par.brnd_seed,
+                                      *(ptr(a) for a in ref)) == 0
+    for i in range(3):
+        assert np.linalg.norm(ref[i]) > 0
+        assert rel_l2(raw[:, i], ref[i]) < 1e-11, i
+
+
+@pytest.mark.gpu
 def test_cli_random_jf12_matches_python_pipeline_and_oracle(tmp_path, ctx):
     """<random><global type="jf12">: the C++ host and the Python pipeline read
     the reference's block (param.cc:375-409), write the same Brnd_jf12 grid,
