@@ -83,7 +83,19 @@ def traffic_from_profiles():
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(par_text_fn, grids_host, threads):
+    # threads pinned close to each other, as the reference's release runs
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from hammurabix_amd import Param, los_samples
     from oracle_lib import write_grid
@@ -92,14 +104,18 @@ def cpu_baseline(par_text_fn, grids_host, threads):
     bx, by, bz, te = grids_host
     box = par.grid_brnd.box()
     S = los_samples(d)
-    t0 = time.perf_counter()
-    write_grid(d, nthreads=threads, brnd=(bx, by, bz), brnd_box=box, ternd=te,
-               ternd_box=par.grid_ternd.box())
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        write_grid(d, nthreads=threads, brnd=(bx, by, bz), brnd_box=box, ternd=te,
+                   ternd_box=par.grid_ternd.box())
+        runs.append(time.perf_counter() - t0)
+    dt = sorted(runs)[1]
     return dict(value=S / dt, unit="samples/s", cores=threads, kind="port",
                 sample=f"random_forward with shells at nside 16/32/64 (1/16 of the "
-                       f"pixels, {S} samples), same grids and radial sampling; "
-                       f"{dt:.2f} s")
+                       f"pixels, {S} samples), same grids and radial sampling; median of "
+                       f"3 runs {dt:.2f} s; {os.cpu_count()} host CPUs visible ({cpu_model()}), "
+                       f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}")
 
 
 def plain_passes(d0, freqs):
@@ -523,6 +539,12 @@ def main():
         ctx.brnd_es(par.es_desc(), 1, bx, by, bz)
         ctx.ternd_dft(par.dft_desc(), 1, te)
         gen_ms = (time.perf_counter() - t0) * 1e3
+        # the ES generator alone, warm: algorithmic 848 B per vertex (draw,
+        # 3 x 2 FFTs, variance, rescale, Gram-Schmidt, reassembly, extraction)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ctx.brnd_es(par.es_desc(), 1, bx, by, bz)
+        es_ms = (time.perf_counter() - t0) * 1e3 / 3
         npix = 12 * 256 * 256
         maps = {k: torch.empty(npix, dtype=torch.float64, device=dev)
                 for k in ("dm", "fd", "is", "qs", "us")}
@@ -611,6 +633,8 @@ def main():
             "march_ms_per_step": march_ms / args.steps,
             "accumulate_ms_per_step": acc_ms / args.steps,
             "random_field_generation_ms": gen_ms,
+            "es_generator_ms": es_ms,
+            "es_generator_algorithmic_GBps": 848 * n / (es_ms * 1e-3) / 1e9,
             "maps_finite": finite,
             "map_gradient_evals_per_s": 1.0 / mg_s,
             "map_gradient_ms": mg_s * 1e3,
