@@ -180,6 +180,11 @@ def likelihood_seeds(ctx, b, pars, specs, dev):
             pick(sp["adj"], k, f).copy_((o - pick(sp["maps"], k, f)) / (sigma * sigma))
 
 
+def progress(msg):
+    """One line per finished phase on stderr, so long runs show they are alive."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_batch(args):
     """The batch workloads.
 
@@ -241,9 +246,11 @@ def run_batch(args):
             specs.append(spec)
         torch.cuda.synchronize()
         gen_s = time.perf_counter() - t0
+        progress(f"{n_inst} realisations generated in {gen_s:.1f} s")
         t0 = time.perf_counter()
         b = ctx.batch(specs, precision=prec, tune=True, adjoint=with_adj)
         plan_s = time.perf_counter() - t0
+        progress(f"batch planned in {plan_s:.1f} s")
         if with_adj:
             likelihood_seeds(ctx, b, pars, specs, dev)
         info = b.info()
@@ -255,6 +262,7 @@ def run_batch(args):
             b.run()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
+        progress(f"forward replays timed: {elapsed * 1e3 / args.steps:.1f} ms each")
         mg = {}
         if with_adj:
             # map+gradient evaluations: forward replay + reverse replay
@@ -267,6 +275,7 @@ def run_batch(args):
                 b.run_adjoint()
             torch.cuda.synchronize()
             mg_s = (time.perf_counter() - t0) / args.steps
+            progress(f"forward + reverse replays timed: {mg_s * 1e3:.1f} ms each")
             mg = dict(batch_map_gradient_ms=mg_s * 1e3,
                       batch_map_gradient_evals_per_s=n_inst / mg_s)
         # the plain path, pass by pass, and the maps it gives
@@ -280,7 +289,8 @@ def run_batch(args):
         map_err = {}  # worst rel. L2 per map kind and frequency
         worst_dln = 0.0
         dp_err = dict.fromkeys(capi.PARAM_NAMES, 0.0)
-        for sp in specs[:n_check]:
+        for ci, sp in enumerate(specs[:n_check]):
+            progress(f"plain-path comparison, instance {ci + 1} of {n_check}")
             freqs = sp["freqs"]
             kw = dict(brnd=sp["brnd"], brnd_box=sp["box"], ternd=sp["ternd"],
                       ternd_box=sp["box"])
