@@ -776,18 +776,114 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Per-wavefront LDS merge of the lanes' gradient flushes.  The 64 rays of a
+// NESTED 8x8 patch leave the same grid vertices behind a few steps apart, so
+// a flush goes first to a direct-mapped cache of vertices (tag = vertex
+// index, 4 FP32 sums) and reaches the grid only when its slot is taken by
+// another vertex or at the end of the work item.  One wavefront owns its
+// cache, so LDS program order is the only ordering needed.
+constexpr int kVtxSlotsLog2 = 10;
+constexpr int kVtxSlots = 1 << kVtxSlotsLog2;
+constexpr uint32_t kVtxEmpty = 0xffffffffu;
+
+struct VtxCache {
+  uint32_t *tag; // kVtxSlots tags (kVtxEmpty: free); nullptr: no cache
+  float4 *val;   // kVtxSlots sums {x, y, z, te}
+};
+
+constexpr int vtx_cache_bytes() { return kVtxSlots * (4 + 16); }
+
+__device__ inline VtxCache vtx_cache_at(void *base) {
+  VtxCache c;
+  c.val = static_cast<float4 *>(base);
+  c.tag = reinterpret_cast<uint32_t *>(c.val + kVtxSlots);
+  return c;
+}
+
+__device__ inline void vtx_cache_clear(const VtxCache &c, int lane) {
+  for (int i = lane; i < kVtxSlots; i += 64) {
+    c.tag[i] = kVtxEmpty;
+    c.val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  wave_lds_sync();
+}
+
+__device__ inline void grid_add4(float *__restrict__ g, uint64_t v, float4 a) {
+  float *p = g + 4 * v;
+  grid_atomic_add(p + 0, a.x);
+  grid_atomic_add(p + 1, a.y);
+  grid_atomic_add(p + 2, a.z);
+  grid_atomic_add(p + 3, a.w);
+}
+
+__device__ inline void lds_add4(float4 *p, float4 a) {
+  float *q = reinterpret_cast<float *>(p);
+  unsafeAtomicAdd(q + 0, a.x);
+  unsafeAtomicAdd(q + 1, a.y);
+  unsafeAtomicAdd(q + 2, a.z);
+  unsafeAtomicAdd(q + 3, a.w);
+}
+
+// Adds `a` to vertex v (< kVtxEmpty) for every active lane.  Lanes that read
+// the same slot read the same tag, so among the lanes that miss exactly one
+// compare-and-swap succeeds: that lane evicts the slot's old sums to the
+// grid and the slot then belongs to its vertex.  Missing lanes of that vertex
+// add into the slot, the others go to the grid directly.
+__device__ inline void vtx_put(const VtxCache &c, float *__restrict__ g, uint64_t v, float4 a) {
+  if (c.tag == nullptr) {
+    grid_add4(g, v, a);
+    return;
+  }
+  const uint32_t key = static_cast<uint32_t>(v);
+  const uint32_t sl = (key * 0x9e3779b1u) >> (32 - kVtxSlotsLog2);
+  const uint32_t t = c.tag[sl];
+  const bool hit = t == key;
+  if (hit)
+    lds_add4(c.val + sl, a);
+  wave_lds_sync();
+  if (!hit) {
+    const bool own = atomicCAS(c.tag + sl, t, key) == t;
+    float4 old = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (own) {
+      old = c.val[sl];
+      c.val[sl] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    wave_lds_sync();
+    if (own && t != kVtxEmpty)
+      grid_add4(g, t, old);
+    if (c.tag[sl] == key)
+      lds_add4(c.val + sl, a);
+    else
+      grid_add4(g, v, a);
+  }
+  wave_lds_sync();
+}
+
+// Every held vertex -> the grid; the cache is left empty.
+__device__ inline void vtx_cache_flush(const VtxCache &c, float *__restrict__ g, int lane) {
+  wave_lds_sync();
+  for (int i = lane; i < kVtxSlots; i += 64) {
+    const uint32_t t = c.tag[i];
+    if (t != kVtxEmpty) {
+      grid_add4(g, t, c.val[i]);
+      c.tag[i] = kVtxEmpty;
+      c.val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  wave_lds_sync();
+}
+
 // acc (8 corners x {x, y, z, te}) -> the packed gradient grid at cell
-// `cell`, then acc = 0.
+// `cell` (through the vertex cache), then acc = 0.
 __device__ inline void flush_cell_f32(float (&acc)[32], uint64_t cell, float *__restrict__ g,
-                                      uint64_t sx, uint64_t sy) {
+                                      uint64_t sx, uint64_t sy, const VtxCache &vc) {
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    float *p = g + 4 * (cell + (c >> 2) * sx + ((c >> 1) & 1) * sy + (c & 1));
+    const uint64_t v = cell + (c >> 2) * sx + ((c >> 1) & 1) * sy + (c & 1);
+    vtx_put(vc, g, v, make_float4(acc[4 * c], acc[4 * c + 1], acc[4 * c + 2], acc[4 * c + 3]));
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      unsafeAtomicAdd(p + m, acc[4 * c + m]);
+    for (int m = 0; m < 4; ++m)
       acc[4 * c + m] = 0.f;
-    }
   }
 }
 
@@ -797,7 +893,8 @@ __device__ inline void flush_cell_f32(float (&acc)[32], uint64_t cell, float *__
 // move to their place in it, and the new cell's far corners start at 0.
 template <int BIT>
 __device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
-                                      float *__restrict__ g, uint64_t sx, uint64_t sy) {
+                                      float *__restrict__ g, uint64_t sx, uint64_t sy,
+                                      const VtxCache &vc) {
 #pragma unroll
   for (int lo = 0; lo < 8; ++lo) {
     if (lo & BIT)
@@ -805,21 +902,21 @@ __device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
     const int hi = lo | BIT;
     const uint64_t vlo = cell + (lo >> 2) * sx + ((lo >> 1) & 1) * sy + (lo & 1);
     const uint64_t vhi = cell + (hi >> 2) * sx + ((hi >> 1) & 1) * sy + (hi & 1);
-    float *p = g + 4 * (up ? vlo : vhi);
+    float left[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const float left = up ? acc[4 * lo + m] : acc[4 * hi + m];
+      left[m] = up ? acc[4 * lo + m] : acc[4 * hi + m];
       const float kept = up ? acc[4 * hi + m] : acc[4 * lo + m];
-      unsafeAtomicAdd(p + m, left);
       acc[4 * lo + m] = up ? kept : 0.f;
       acc[4 * hi + m] = up ? 0.f : kept;
     }
+    vtx_put(vc, g, up ? vlo : vhi, make_float4(left[0], left[1], left[2], left[3]));
   }
 }
 
 template <bool GRIDS, bool PW1>
 __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
-                              const RayGeo &r) {
+                              const RayGeo &r, VtxCache vc) {
   const bool has_cache = in.do_fd && s > 0;
   const BShell &sh = in.sh[s];
   const BAdj &A = in.adj;
@@ -853,6 +950,9 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
   }
   const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
   float *ggrid = static_cast<float *>(A.ggrid);
+  // vertex indices are the cache's 32-bit tags
+  if ((in.geo.fx + 1.) * static_cast<double>(sx) >= static_cast<double>(kVtxEmpty))
+    vc.tag = nullptr;
   double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0., q0 = 0.;
   double after = 0., dacc = 0.; // FD and R contributions of the samples beyond
   float acc[32];
@@ -935,17 +1035,17 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
             // between two valid cells, so is valid)
             uint64_t c = cur;
             if (dx) {
-              shift_cell_f32<4>(acc, dx > 0, c, ggrid, sx, sy);
+              shift_cell_f32<4>(acc, dx > 0, c, ggrid, sx, sy, vc);
               c = dx > 0 ? c + sx : c - sx;
             }
             if (dy) {
-              shift_cell_f32<2>(acc, dy > 0, c, ggrid, sx, sy);
+              shift_cell_f32<2>(acc, dy > 0, c, ggrid, sx, sy, vc);
               c = dy > 0 ? c + sy : c - sy;
             }
             if (dz)
-              shift_cell_f32<1>(acc, dz > 0, c, ggrid, sx, sy);
+              shift_cell_f32<1>(acc, dz > 0, c, ggrid, sx, sy, vc);
           } else {
-            flush_cell_f32(acc, cur, ggrid, sx, sy);
+            flush_cell_f32(acc, cur, ggrid, sx, sy, vc);
           }
         }
         cur = o.idx;
@@ -967,7 +1067,9 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
     }
   }
   if (GRIDS && ggrid && cur != ~0ull)
-    flush_cell_f32(acc, cur, ggrid, sx, sy);
+    flush_cell_f32(acc, cur, ggrid, sx, sy, vc);
+  if (GRIDS && ggrid && vc.tag)
+    vtx_cache_flush(vc, ggrid, static_cast<int>(threadIdx.x & 63));
   if (live) {
     double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
     rp[0] = p0;
@@ -1009,7 +1111,7 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
     RayGeo r{};
     if (live)
       ray_setup_call(in.ph, shell_args_rev(in, s), ipix, r);
-    adj_lanes_f32<GRIDS, PW1>(in, s, ipix, live, r);
+    adj_lanes_f32<GRIDS, PW1>(in, s, ipix, live, r, vtx_cache_at(slab));
     return;
   }
   const uint64_t p1 = (p0 + a.rays < npix) ? p0 + a.rays : npix;
@@ -1063,6 +1165,11 @@ __attribute__((amdgpu_waves_per_eu(PREC == HAMX_PREC_F32 && !REV ? 3 : 1))) void
   if constexpr (REV && GRIDS && PREC == HAMX_PREC_F64) {
     __shared__ __align__(16) unsigned char caches[4][cell_cache_bytes<double>()];
     slab = caches[wave];
+  }
+  if constexpr (REV && GRIDS && PREC == HAMX_PREC_F32) {
+    __shared__ __align__(16) unsigned char vcaches[4][vtx_cache_bytes()];
+    slab = vcaches[wave];
+    vtx_cache_clear(vtx_cache_at(slab), lane);
   }
   if (!a.persistent) {
     uint64_t b = blockIdx.x;

@@ -200,6 +200,18 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Atomic add to a gradient grid in device memory, issued as a global (not
+// flat) atomic: a flat instruction also counts against the LDS wait counter,
+// so each LDS round trip after it would wait for the atomic's round trip to
+// memory as well.
+template <typename T>
+__device__ inline void grid_atomic_add(T *p, T v) {
+  auto *gp = (__attribute__((address_space(1))) T *)p;
+  [[clang::atomic(ignore_denormal_mode)]] {
+    __hip_atomic_fetch_add(gp, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Rays per wavefront in the specialised march (kFastRays) and its adjoint
 // (kAdjRays): their set-up runs one ray per lane into LDS, then they are
 // marched one after another.  kAdjCellCache picks the adjoint's grid
@@ -267,10 +279,10 @@ __device__ inline void scatter_runs(T *__restrict__ g, T *slab, bool has, uint64
         for (int c = 0; c < 2; ++c) {
           const T *q = slot + 4 * (4 * a + 2 * b + c);
           T *p = g + 4 * (idx + a * sx + b * sy + c);
-          unsafeAtomicAdd(p + 0, q[0]);
-          unsafeAtomicAdd(p + 1, q[1]);
-          unsafeAtomicAdd(p + 2, q[2]);
-          unsafeAtomicAdd(p + 3, q[3]);
+          grid_atomic_add(p + 0, q[0]);
+          grid_atomic_add(p + 1, q[1]);
+          grid_atomic_add(p + 2, q[2]);
+          grid_atomic_add(p + 3, q[3]);
         }
   }
   wave_lds_sync();
@@ -321,7 +333,7 @@ __device__ inline void cell_flush_slot(const CellCache<T> &c, int s, unsigned lo
   if (lane < 32) {
     const int corner = lane >> 2, comp = lane & 3;
     const uint64_t v = cell + (corner >> 2) * sx + ((corner >> 1) & 1) * sy + (corner & 1);
-    unsafeAtomicAdd(g + 4 * v + comp, c.val[32 * s + lane]);
+    grid_atomic_add(g + 4 * v + comp, c.val[32 * s + lane]);
   }
 }
 
@@ -403,10 +415,10 @@ __device__ inline void scatter_cached(const CellCache<T> &c, T *__restrict__ g, 
             atomicAdd(q + 3, w * g3);
           } else {
             T *p = g + 4 * (idx + a * sx + b * sy + cz);
-            unsafeAtomicAdd(p + 0, w * g0);
-            unsafeAtomicAdd(p + 1, w * g1);
-            unsafeAtomicAdd(p + 2, w * g2);
-            unsafeAtomicAdd(p + 3, w * g3);
+            grid_atomic_add(p + 0, w * g0);
+            grid_atomic_add(p + 1, w * g1);
+            grid_atomic_add(p + 2, w * g2);
+            grid_atomic_add(p + 3, w * g3);
           }
         }
   }
