@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     if (k < r.k_hi)
       valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
-    const double chunk = __shfl(incl, 63, 64);
+    const double chunk = wave_last(incl);
     const double fd_here = (fd_total - after - chunk) + incl;
     after += chunk;
     double c2q = 1., s2q = 0., dq = 0.;

@@ -177,7 +177,7 @@ __device__ void march_ray_f64(const BInst &in, int s, uint64_t ipix, int lane,
       valid = eval_fast<GRIDS, PW1>(in.ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
     const double fd_here = carry + incl;
-    carry += __shfl(incl, 63, 64);
+    carry += wave_last(incl);
     if (valid) {
       dm += o.te;
       I += o.jt;
@@ -235,6 +235,8 @@ struct S32 {
 
 __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, uint64_t sy,
                                      float zd, float yd, float4 &w) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
   const float4 a0 = f[i0], a1 = f[i0 + 1];
   const float4 b0 = f[i0 + sy], b1 = f[i0 + sy + 1];
   const float z1 = 1.f - zd, y1 = 1.f - yd;
@@ -249,6 +251,8 @@ __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, 
 // produces), then the Taylor series of sin and cos on [-pi/4, pi/4] to the
 // terms r^9 and r^10 (truncation below 2e-9 there).
 __device__ inline void sincos_f32(float x, float &sn, float &cs) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
   const double xd = static_cast<double>(x);
   const double k = rint(xd * 0.63661977236758134308); // 2/pi
   const float r = static_cast<float>(xd - k * 1.57079632679489661923);
@@ -294,6 +298,8 @@ template <bool GRIDS, bool PW1>
 __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo &r,
                                 const float3 &lf, const float3 &ethf, const float3 &ephf,
                                 uint64_t k, S32 &o) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
   const Phys &ph = in.ph;
   const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
   const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
@@ -316,14 +322,14 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   if (o.lsa_in) {
     const float xk = static_cast<float>(pos.x * inv_kpc);
     const float yk = static_cast<float>(pos.y * inv_kpc);
-    const float inv_r = 1.f / rk;
+    const float inv_r = __builtin_amdgcn_rcpf(rk);
     o.cos_p = xk * inv_r;
     o.sin_p = yk * inv_r;
     // accurate (not the hardware-approximate) functions: the field's
     // systematic error feeds the FD, and the Q/U phase at low frequency
     o.lnr = logf(rk * 0.125f);
     const float e2 = expf(-2.f * fabsf(zk));
-    o.tnh = copysignf((1.f - e2) / (1.f + e2), zk);
+    o.tnh = copysignf((1.f - e2) * __builtin_amdgcn_rcpf(1.f + e2), zk);
     const float psi = c.psi0 + c.psi1 * o.lnr;
     const float chi = c.chi0 * o.tnh;
     sincos_f32(psi, o.sps, o.cps);
@@ -386,9 +392,10 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
     prof = (dx * dx + dy * dy + dz * dz > in.cre_r02) ? 0.f : 1.f;
   } else {
-    const float e = __expf(zk * c.inv_z0k);
-    const float ch = 0.5f * (e + 1.f / e);
-    prof = __expf((c.R0k - rk) * c.inv_r0k) / (ch * ch);
+    // sech^2(z/z0) = 4 e^2 / (1 + e^2)^2 with e = exp(-|z|/z0) <= 1
+    const float e = __expf(-fabsf(zk) * c.inv_z0k);
+    const float e2 = e * e, d = 1.f + e2;
+    prof = __expf((c.R0k - rk) * c.inv_r0k) * (4.f * e2) * __builtin_amdgcn_rcpf(d * d);
   }
   if (PW1)
     o.w = prof * o.bper2;
@@ -398,7 +405,7 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   o.y = -(ethf.x * bx + ethf.y * by + ethf.z * bz);
   o.n2 = o.x * o.x + o.y * o.y;
   if (o.n2 > 0.f) {
-    const float inv = 1.f / o.n2;
+    const float inv = __builtin_amdgcn_rcpf(o.n2); // 1 ulp: scales Q/U only
     o.c2i = (o.x * o.x - o.y * o.y) * inv;
     o.s2i = 2.f * o.x * o.y * inv;
   } else {
@@ -411,6 +418,8 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
 template <bool GRIDS, bool PW1>
 __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane,
                               const RayGeo &r) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
   const BShell &sh = in.sh[s];
   const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
                                 static_cast<float>(r.l.z));
@@ -430,7 +439,7 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane,
     const double fk = valid ? o.fdk * sh.fdu : 0.;
     const double incl = wave_incl_scan(fk, lane);
     const double fd_here = carry + incl;
-    carry += __shfl(incl, 63, 64);
+    carry += wave_last(incl);
     if (valid) {
       dm += o.te;
       I += o.w;
@@ -533,7 +542,7 @@ __device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, voi
     if (k < r.k_hi)
       valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk * sh.fdu : 0., lane);
-    const double chunk = __shfl(incl, 63, 64);
+    const double chunk = wave_last(incl);
     const double fd_here = (fd_total - after - chunk) + incl;
     after += chunk;
     double D = 0.;
@@ -663,7 +672,7 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, voi
     if (k < r.k_hi)
       valid = eval_fast<GRIDS, PW1>(in.ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
-    const double chunk = __shfl(incl, 63, 64);
+    const double chunk = wave_last(incl);
     const double fd_here = (fd_total - after - chunk) + incl;
     after += chunk;
     double D = 0., dJp = 0., alpha_p = 0., dipa = 0.;
@@ -917,6 +926,8 @@ __device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
 template <bool GRIDS, bool PW1>
 __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
                               const RayGeo &r, VtxCache vc) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
   const bool has_cache = in.do_fd && s > 0;
   const BShell &sh = in.sh[s];
   const BAdj &A = in.adj;

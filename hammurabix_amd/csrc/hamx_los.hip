@@ -327,7 +327,7 @@ k_los_wave(Phys ph, Grids gr, ShellArgs sa) {
     if (ph.do_sync) {
       const double incl = wave_incl_scan(fdk, lane);
       const double fd_here = fd_carry + incl;
-      fd_carry += __shfl(incl, 63, 64);
+      fd_carry += wave_last(incl);
       if (valid) {
         const double qui = (r.fd_inner + fd_here) * ph.lambda_sq + s.ipa;
         double s2, c2;
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       const double fdk = valid ? o.fdk : 0.;
       const double incl = wave_incl_scan(fdk, lane);
       const double fd_here = fd_carry + incl;
-      fd_carry += __shfl(incl, 63, 64);
+      fd_carry += wave_last(incl);
       if (valid) {
         double sph, cph;
         sincos(two_l2 * (r.fd_inner + fd_here), &sph, &cph);

@@ -170,13 +170,38 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-__device__ inline double wave_incl_scan(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_up(v, o, 64);
-    if (lane >= o)
-      v += t;
-  }
+// v moved across lanes by the DPP control CTRL (both 32-bit halves), 0 in
+// the lanes whose row ROW_MASK leaves out or whose source lies outside the
+// row.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROW_MASK, 0xf, true);
+  const int hi =
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// Lane 63's value in every lane (scalar reads; the wavefront is converged).
+__device__ inline double wave_last(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// Inclusive prefix sum over the 64 lanes: shifts by 1, 2, 4 and 8 within
+// each 16-lane row, then lane 15 of rows 0 and 2 into rows 1 and 3, then
+// lane 31 into rows 2 and 3 (DPP moves, no LDS traffic).
+__device__ inline double wave_incl_scan(double v, int) {
+  v += dpp_f64<0x111, 0xf>(v); // row_shr:1
+  v += dpp_f64<0x112, 0xf>(v); // row_shr:2
+  v += dpp_f64<0x114, 0xf>(v); // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v); // row_shr:8
+  v += dpp_f64<0x142, 0xa>(v); // row_bcast:15
+  v += dpp_f64<0x143, 0xc>(v); // row_bcast:31
   return v;
 }
 
