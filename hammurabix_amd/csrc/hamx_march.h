@@ -183,14 +183,16 @@ __device__ inline double dpp_f64(double v) {
                               static_cast<long long>(static_cast<unsigned>(lo)));
 }
 
-// Lane 63's value in every lane (scalar reads; the wavefront is converged).
-__device__ inline double wave_last(double v) {
+// Lane L's value in every lane (scalar reads; the wavefront is converged).
+__device__ inline double wave_lane(double v, int L) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), 63);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), L);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), L);
   return __longlong_as_double((static_cast<long long>(hi) << 32) |
                               static_cast<long long>(static_cast<unsigned>(lo)));
 }
+
+__device__ inline double wave_last(double v) { return wave_lane(v, 63); }
 
 // Inclusive prefix sum over the 64 lanes: shifts by 1, 2, 4 and 8 within
 // each 16-lane row, then lane 15 of rows 0 and 2 into rows 1 and 3, then
@@ -206,14 +208,18 @@ __device__ inline double wave_incl_scan(double v, int) {
 }
 
 // Sum over lanes >= lane (a reverse inclusive scan).
+// Shifts by 1, 2, 4 and 8 within each 16-lane row (DPP), then each row adds
+// the totals of the rows above it, read from lanes 16, 32 and 48.
 __device__ inline double wave_suffix_scan(double v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_down(v, o, 64);
-    if (lane + o < 64)
-      v += t;
-  }
-  return v;
+  v += dpp_f64<0x101, 0xf>(v); // row_shl:1
+  v += dpp_f64<0x102, 0xf>(v); // row_shl:2
+  v += dpp_f64<0x104, 0xf>(v); // row_shl:4
+  v += dpp_f64<0x108, 0xf>(v); // row_shl:8
+  const double t3 = wave_lane(v, 48);
+  const double t23 = wave_lane(v, 32) + t3;
+  const double t123 = wave_lane(v, 16) + t23;
+  const int row = lane >> 4;
+  return v + (row == 0 ? t123 : row == 1 ? t23 : row == 2 ? t3 : 0.);
 }
 
 // Orders the LDS accesses of one wavefront around this point (the lanes of
