@@ -294,20 +294,48 @@ __device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c,
   return true;
 }
 
+// The sample's FP64 position and the decisions taken on it (limit tests,
+// the LSA window's squared radius, the TE and CRE spheres), evaluated as
+// written: this function sits outside the FP32 code's contraction pragma,
+// so no product here fuses into an FMA and the decisions round as the FP64
+// plain path's do.
+struct Geo64 {
+  Vec3 pos;
+  double rr2;        // squared cylindrical radius
+  bool te_in, cre_in; // inside the uniform TE / CRE spheres
+};
+
+__device__ inline bool geo_f64(const BInst &in, const BShell &sh, const RayGeo &r, uint64_t k,
+                               Geo64 &g) {
+  const Phys &ph = in.ph;
+  const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
+  g.pos = Vec3{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
+  const Vec3 &pos = g.pos;
+  const double len2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
+  if (len2 < in.r_min2 || len2 > in.r_max2 || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+    return false;
+  g.rr2 = pos.x * pos.x + pos.y * pos.y;
+  const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  g.te_in = !(d2 > in.te_r02);
+  g.cre_in = !(d2 > in.cre_r02);
+  return true;
+}
+
 template <bool GRIDS, bool PW1>
 __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo &r,
                                 const float3 &lf, const float3 &ethf, const float3 &ephf,
                                 uint64_t k, S32 &o) {
-  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer
+  // instructions); geo_f64's body keeps the FP64 decisions unfused
   #pragma clang fp contract(fast)
-  const Phys &ph = in.ph;
-  const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
-  const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
-  const double len2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
-  if (len2 < in.r_min2 || len2 > in.r_max2 || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+  Geo64 geo;
+  if (!geo_f64(in, sh, r, k, geo))
     return false;
+  const Phys &ph = in.ph;
+  const Vec3 &pos = geo.pos;
   const double inv_kpc = 1.0 / cgs::kpc;
-  const double rr2 = pos.x * pos.x + pos.y * pos.y;
+  const double rr2 = geo.rr2;
   const float rk = sqrtf(static_cast<float>(rr2 * (inv_kpc * inv_kpc)));
   const float zk = static_cast<float>(pos.z * inv_kpc);
   const F32Const &c = in.f;
@@ -342,12 +370,9 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     bpar = c.b0d * (static_cast<double>(ux) * r.l.x + static_cast<double>(uy) * r.l.y +
                     static_cast<double>(o.sch) * r.l.z);
   }
-  {
-    const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
-    o.te_in = !(dx * dx + dy * dy + dz * dz > in.te_r02);
-    if (o.te_in)
-      te_d = c.n0d;
-  }
+  o.te_in = geo.te_in;
+  if (o.te_in)
+    te_d = c.n0d;
   o.cell_ok = false;
   if (GRIDS) {
     Cell cell;
@@ -389,8 +414,7 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
   o.bper2 = cx * cx + cy * cy + cz * cz;
   float prof;
   if (ph.cre_model == HAMX_CRE_UNIF) {
-    const double dx = pos.x - ph.obs.x, dy = pos.y - ph.obs.y, dz = pos.z - ph.obs.z;
-    prof = (dx * dx + dy * dy + dz * dz > in.cre_r02) ? 0.f : 1.f;
+    prof = geo.cre_in ? 1.f : 0.f;
   } else {
     // sech^2(z/z0) = 4 e^2 / (1 + e^2)^2 with e = exp(-|z|/z0) <= 1
     const float e = __expf(-fabsf(zk) * c.inv_z0k);

@@ -33,8 +33,9 @@ def _ptr(a: Any) -> tuple[Optional[int], Optional[bool]]:
     if _is_torch(a):
         if not a.is_contiguous():
             raise ValueError("tensor must be contiguous")
-        if a.dtype.itemsize != 8:
-            raise ValueError("tensor must hold float64 values")
+        import torch
+        if a.dtype != torch.float64:  # int64 / complex64 would be read as doubles
+            raise ValueError(f"tensor must hold float64 values, not {a.dtype}")
         return a.data_ptr(), bool(a.is_cuda)
     if isinstance(a, np.ndarray):
         if not a.flags.c_contiguous or a.dtype != np.float64:

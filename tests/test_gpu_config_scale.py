@@ -1,0 +1,320 @@
+"""Parity with the CPU oracle at the workloads' own sizes.
+
+* random_forward at full size -- the timed workload: ES random B and DFT
+  random TE generated on the GPU at 256x256x64 (seed 1), shells at nside
+  64/128/256 with 750/750/1500 steps of 0.01 kpc, sync at 23 GHz.  The
+  generated grids equal the oracle's element-wise (same Philox stream), and
+  every map of the HIP pass equals the oracle's write_grid on the same grids
+  within 1e-10 relative L2.
+* the adjoint workload: shells at nside 32/64/128 over the same 256x256x64
+  grids, a Gaussian log-likelihood of sync Q/U at 23 GHz and FD against
+  maps of a held-out realisation (seed 1001) with 10% Gaussian noise.  The
+  HIP adjoint's gradient in the six parameters equals central differences
+  of the oracle's likelihood with h = 1e-6 relative; directional derivatives
+  over the random B and TE grids likewise.
+* the 64-instance batch's instances (uneven top nside, 1-4 frequencies,
+  GPU-generated 256x256x64 ES/DFT with their own seeds, frozen b0 / psi0
+  draws) with the sky shrunk 8x per axis so the oracle covers them: the
+  batched maps (FP64 and FP32 samples) against the oracle's passes in the
+  reference pipeline's order (pipeline.cc:165-190: the last-listed
+  frequency first with DM and FD, the others sync only with no FD carry).
+* the large batched adjoint's instances (512x512x128 grids, 4 shells,
+  0.005 kpc steps, 4 frequencies) at a small sky against the oracle, and at
+  full size: finite maps and gradients, and DM, FD and the last-listed
+  frequency's Q/U identical to a batch instance that lists that frequency
+  alone.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from hammurabix_amd import Param, _capi as capi, configs
+from oracle_lib import load_oracle, npix, ptr, rel_l2, write_grid
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)  # the oracle's OpenMP threads
+MAP_TOL = 1e-10                          # FP64 plain path vs oracle
+F32_TOL = 1e-5                           # FP32 batch samples vs FP64
+
+
+def torch_dev():
+    import torch
+    return torch, torch.device("cuda:0")
+
+
+def device_grids(ctx, par, seed):
+    """ES random B and DFT random TE of `seed` on the device."""
+    torch, dev = torch_dev()
+    n = par.grid_brnd.full_size
+    bx, by, bz, te = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4))
+    ctx.brnd_es(par.es_desc(), seed, bx, by, bz)
+    ctx.ternd_dft(par.dft_desc(), seed, te)
+    torch.cuda.synchronize()
+    return (bx, by, bz), te
+
+
+def host(a):
+    return np.ascontiguousarray(a.cpu().numpy())
+
+
+def oracle_kw(par, b, te):
+    return dict(brnd=tuple(host(a) for a in b), brnd_box=par.grid_brnd.box(),
+                ternd=host(te), ternd_box=par.grid_ternd.box())
+
+
+def pipeline_passes(d0, freqs):
+    """Pipeline::assemble_obs's passes: (pass, frequency index, descriptor)."""
+    for p, fi in enumerate(reversed(range(len(freqs)))):
+        d = capi.HamxLosDesc()
+        C.memmove(C.byref(d), C.byref(d0), C.sizeof(d))
+        d.sync_freq, d.do_sync = freqs[fi], 1
+        if p > 0:
+            d.do_dm = d.do_fd = 0
+        yield p, fi, d
+
+
+def oracle_pipeline(d0, freqs, kw):
+    exp = {k: [None] * len(freqs) for k in ("is", "qs", "us")}
+    for p, fi, d in pipeline_passes(d0, freqs):
+        m = write_grid(d, nthreads=NTHREADS, **kw)
+        for k in ("is", "qs", "us"):
+            exp[k][fi] = m[k]
+        if p == 0:
+            exp["dm"], exp["fd"] = m["dm"], m["fd"]
+    return exp
+
+
+def compare_batch(spec, exp, tol, label):
+    m = spec["maps"]
+    pairs = [("dm", m["dm"], exp["dm"]), ("fd", m["fd"], exp["fd"])]
+    for k in ("is", "qs", "us"):
+        pairs += [(f"{k}@{f / 1e9:g}GHz", a, e)
+                  for f, a, e in zip(spec["freqs"], m[k], exp[k])]
+    worst = 0.0
+    for name, got, ref in pairs:
+        g = host(got)
+        assert np.all(np.isfinite(g)), f"{label} {name}"
+        err = rel_l2(g, ref)
+        print(f"{label} {name}: rel L2 vs oracle {err:.3e}", flush=True)
+        worst = max(worst, err)
+        assert err <= tol, f"{label} {name}: rel L2 {err:.3e} > {tol}"
+    return worst
+
+
+def batch_spec(par, freqs_hz, b, te, with_adj=False):
+    torch, dev = torch_dev()
+    d = par.los_desc()
+    z = lambda ns: torch.zeros(npix(ns), dtype=torch.float64, device=dev)  # noqa: E731
+    maps = dict(dm=z(d.nside_dm), fd=z(d.nside_fd),
+                **{k: [z(d.nside_sync) for _ in freqs_hz] for k in ("is", "qs", "us")})
+    spec = dict(desc=d, freqs=list(freqs_hz), maps=maps, brnd=b, ternd=te,
+                box=par.grid_brnd.box())
+    if with_adj:
+        n = par.grid_brnd.full_size
+        gen = torch.Generator(device=dev).manual_seed(31)
+        spec["adj"] = {"dm": torch.randn(npix(d.nside_dm), dtype=torch.float64, device=dev,
+                                         generator=gen),
+                       "fd": torch.randn(npix(d.nside_fd), dtype=torch.float64, device=dev,
+                                         generator=gen),
+                       **{k: [torch.randn(npix(d.nside_sync), dtype=torch.float64, device=dev,
+                                          generator=gen) for _ in freqs_hz]
+                          for k in ("is", "qs", "us")}}
+        spec["grads"] = {
+            "dparams": torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev),
+            "dbrnd": tuple(torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)),
+            "dternd": torch.zeros(n, dtype=torch.float64, device=dev)}
+    return spec
+
+
+# ----------------------------------------------------------- random_forward
+
+def test_random_forward_full_size_matches_oracle(ctx):
+    torch, dev = torch_dev()
+    print("case: random_forward full size (256x256x64 grids, nside 64/128/256)", flush=True)
+    par = Param(text=configs.random_forward())
+    d = par.los_desc()
+    assert [d.nside_shell[i] for i in range(d.n_shell)] == [64, 128, 256]
+    b, te = device_grids(ctx, par, 1)
+    orc = load_oracle()
+    n = par.grid_brnd.full_size
+    rb = [np.zeros(n) for _ in range(3)]
+    assert orc.orc_brnd_es(C.byref(par.es_desc()), 1, *(ptr(a) for a in rb)) == 0
+    for a, r in zip(b, rb):
+        err = rel_l2(host(a), r)
+        print(f"brnd_es 256x256x64 rel L2 vs oracle {err:.3e}", flush=True)
+        assert err < 1e-11
+    rt = np.zeros(n)
+    assert orc.orc_ternd_dft(C.byref(par.dft_desc()), 1, ptr(rt)) == 0
+    err = rel_l2(host(te), rt)
+    print(f"ternd_dft 256x256x64 rel L2 vs oracle {err:.3e}", flush=True)
+    assert err < 1e-12
+    maps = {k: torch.zeros(npix(256), dtype=torch.float64, device=dev)
+            for k in ("dm", "fd", "is", "qs", "us")}
+    ctx.los_forward(d, maps, brnd=b, brnd_box=par.grid_brnd.box(), ternd=te,
+                    ternd_box=par.grid_ternd.box())
+    ref = write_grid(d, nthreads=NTHREADS, **oracle_kw(par, b, te))
+    for k in ref:
+        g = host(maps[k])
+        assert np.all(np.isfinite(g)), k
+        err = rel_l2(g, ref[k])
+        print(f"random_forward full size {k}: rel L2 vs oracle {err:.3e}", flush=True)
+        assert err <= MAP_TOL, k
+
+
+# ---------------------------------------------------------------- adjoint
+
+def test_adjoint_workload_vs_oracle_central_differences(ctx):
+    torch, dev = torch_dev()
+    print("case: adjoint workload (nside 32/64/128 on 256x256x64) vs oracle FD", flush=True)
+    par = Param(text=configs.random_forward(nside_sim=32, nside_out=128))
+    d = par.los_desc()
+    assert [d.nside_shell[i] for i in range(d.n_shell)] == [32, 64, 128]
+    b, te = device_grids(ctx, par, 1)
+    kw_dev = dict(brnd=b, brnd_box=par.grid_brnd.box(), ternd=te,
+                  ternd_box=par.grid_ternd.box())
+    kw = oracle_kw(par, b, te)
+    # observed maps: the held-out realisation plus 10% Gaussian noise
+    bo, teo = device_grids(ctx, par, 1001)
+    obs_m = {k: np.zeros(npix(128)) for k in ("dm", "fd", "is", "qs", "us")}
+    ctx.los_forward(d, obs_m, brnd=tuple(host(a) for a in bo), brnd_box=par.grid_brnd.box(),
+                    ternd=host(teo), ternd_box=par.grid_ternd.box())
+    rng = np.random.default_rng(1001)
+    obs, sig = {}, {}
+    for k in ("qs", "us", "fd"):
+        sig[k] = 0.1 * float(np.sqrt(np.mean(obs_m[k] ** 2)))
+        obs[k] = obs_m[k] + sig[k] * rng.normal(size=obs_m[k].shape)
+
+    def loglike(m):
+        return -0.5 * sum(float(np.sum((m[k] - obs[k]) ** 2)) / sig[k] ** 2 for k in obs)
+
+    m = {k: np.zeros(npix(128)) for k in ("dm", "fd", "is", "qs", "us")}
+    ctx.los_forward(d, m, **kw)
+    adj = {k: np.zeros(npix(128)) for k in ("dm", "is")}
+    for k in obs:
+        adj[k] = np.ascontiguousarray(-(m[k] - obs[k]) / sig[k] ** 2)
+    n = par.grid_brnd.full_size
+    dp = np.zeros(capi.HAMX_NPARAM)
+    gb = tuple(np.zeros(n) for _ in range(3))
+    gt = np.zeros(n)
+    ctx.los_adjoint(d, adj, dp, dbrnd=gb, dternd=gt, **kw)
+    assert np.all(np.isfinite(dp)) and all(np.all(np.isfinite(g)) for g in gb)
+    params = [("breg", 0), ("breg", 1), ("breg", 2), ("breg", 3), ("cre", 0), ("tereg", 0)]
+    for i, ((arr, j), name) in enumerate(zip(params, capi.PARAM_NAMES)):
+        p0 = getattr(d, arr)[j]
+        h = 1e-6 * abs(p0)
+        getattr(d, arr)[j] = p0 + h
+        lp = loglike(write_grid(d, nthreads=NTHREADS, **kw))
+        getattr(d, arr)[j] = p0 - h
+        lm = loglike(write_grid(d, nthreads=NTHREADS, **kw))
+        getattr(d, arr)[j] = p0
+        fd = (lp - lm) / (2 * h)
+        rel = abs(dp[i] - fd) / abs(fd)
+        print(f"d lnL/d{name}: adjoint {dp[i]:.10e} oracle FD {fd:.10e} rel {rel:.2e}",
+              flush=True)
+        assert rel <= 1e-5, name
+    # directional derivatives over the grids (the te > 0 clamp makes L only
+    # piecewise smooth in the TE grid, hence the looser TE bound)
+    vrng = np.random.default_rng(7)
+    v = [vrng.normal(size=n) for _ in range(3)]
+    h = 1e-6 * 2e-6
+    lp = loglike(write_grid(d, nthreads=NTHREADS, **dict(kw, brnd=tuple(
+        np.ascontiguousarray(a + h * va) for a, va in zip(kw["brnd"], v)))))
+    lm = loglike(write_grid(d, nthreads=NTHREADS, **dict(kw, brnd=tuple(
+        np.ascontiguousarray(a - h * va) for a, va in zip(kw["brnd"], v)))))
+    fd = (lp - lm) / (2 * h)
+    got = sum(float(np.dot(g, va)) for g, va in zip(gb, v))
+    print(f"<dlnL/dB, v>: adjoint {got:.10e} oracle FD {fd:.10e}", flush=True)
+    assert abs(got - fd) <= 1e-5 * abs(fd)
+    vt = vrng.normal(size=n)
+    h = 1e-6 * 1e-3
+    lp = loglike(write_grid(d, nthreads=NTHREADS,
+                            **dict(kw, ternd=np.ascontiguousarray(kw["ternd"] + h * vt))))
+    lm = loglike(write_grid(d, nthreads=NTHREADS,
+                            **dict(kw, ternd=np.ascontiguousarray(kw["ternd"] - h * vt))))
+    fd = (lp - lm) / (2 * h)
+    got = float(np.dot(gt, vt))
+    print(f"<dlnL/dte, v>: adjoint {got:.10e} oracle FD {fd:.10e}", flush=True)
+    assert abs(got - fd) <= 1e-3 * abs(fd)
+    del kw_dev
+
+
+# ------------------------------------------------------------------ batches
+
+@pytest.mark.parametrize("prec,tol", [(capi.HAMX_PREC_F64, MAP_TOL),
+                                      (capi.HAMX_PREC_F32, F32_TOL)])
+def test_batch_instances_vs_oracle(ctx, prec, tol):
+    """Instances 0-5 of the 64-instance batch (top nside 64/128/256 and 1-4
+    frequencies each), shells at 1/8 of their nside."""
+    print(f"case: batch instances 0-5 vs oracle, precision {prec}", flush=True)
+    specs, refs = [], []
+    for i in range(6):
+        par = Param(text=configs.batch_instance(i))
+        top = configs.BATCH_TOP_NSIDE[i % 3]
+        small = Param(text=configs.batch_instance(i, nside_sim=top // 32, nside_out=top // 8))
+        b, te = device_grids(ctx, par, par.brnd_seed)
+        freqs = [f * 1e9 for f in configs.BATCH_FREQS_GHZ[:1 + i % 4]]
+        spec = batch_spec(small, freqs, b, te)
+        specs.append(spec)
+        refs.append(oracle_pipeline(spec["desc"], freqs, oracle_kw(par, b, te)))
+    with ctx.batch(specs, precision=prec, tune=False) as bt:
+        bt.run()
+        worst = max(compare_batch(s, r, tol, f"prec{prec} inst{i}")
+                    for i, (s, r) in enumerate(zip(specs, refs)))
+    print(f"batch vs oracle worst rel L2 {worst:.3e}", flush=True)
+
+
+def test_large_instances_small_sky_vs_oracle(ctx):
+    """Two instances of the large batch (512x512x128 GPU-generated grids, 4
+    shells at 0.005 kpc, 4 frequencies) on shells at nside 4/8/16/32."""
+    print("case: large instances at a small sky vs oracle", flush=True)
+    specs, refs = [], []
+    for i in range(2):
+        par = Param(text=configs.large_instance(i, nside_sim=4, nside_out=32))
+        d = par.los_desc()
+        assert [d.nside_shell[s] for s in range(d.n_shell)] == [4, 8, 16, 32]
+        b, te = device_grids(ctx, par, par.brnd_seed)
+        freqs = [f * 1e9 for f in configs.BATCH_FREQS_GHZ]
+        spec = batch_spec(par, freqs, b, te)
+        specs.append(spec)
+        refs.append(oracle_pipeline(spec["desc"], freqs, oracle_kw(par, b, te)))
+    with ctx.batch(specs, precision=capi.HAMX_PREC_F32, tune=False) as bt:
+        bt.run()
+        for i, (s, r) in enumerate(zip(specs, refs)):
+            compare_batch(s, r, F32_TOL, f"large inst{i}")
+
+
+def test_large_full_size_properties(ctx):
+    """One large realisation at full size (shells at nside 128-1024, 4
+    frequencies) next to the same realisation listing only 30 GHz: DM, FD
+    and the 30 GHz maps are the same (the last-listed frequency's pass
+    carries DM and FD), every map and gradient is finite."""
+    torch, dev = torch_dev()
+    print("case: large instance full size, properties", flush=True)
+    par = Param(text=configs.large_instance(0))
+    b, te = device_grids(ctx, par, par.brnd_seed)
+    freqs = [f * 1e9 for f in configs.BATCH_FREQS_GHZ]
+    s4 = batch_spec(par, freqs, b, te, with_adj=True)
+    s1 = batch_spec(par, freqs[-1:], b, te)
+    with ctx.batch([s4, s1], precision=capi.HAMX_PREC_F32, tune=False, adjoint=True) as bt:
+        bt.run()
+        bt.run_adjoint()
+        torch.cuda.synchronize()
+    m4, m1 = s4["maps"], s1["maps"]
+    for k in ("dm", "fd"):
+        assert bool(torch.isfinite(m4[k]).all())
+        err = rel_l2(host(m4[k]), host(m1[k]))
+        print(f"{k}: 4 frequencies vs 30 GHz alone rel L2 {err:.3e}", flush=True)
+        assert err <= 1e-12
+    for k in ("is", "qs", "us"):
+        assert all(bool(torch.isfinite(a).all()) for a in m4[k])
+        err = rel_l2(host(m4[k][-1]), host(m1[k][0]))
+        print(f"{k}@30GHz: 4 frequencies vs alone rel L2 {err:.3e}", flush=True)
+        assert err <= 1e-12
+    g = s4["grads"]
+    assert bool(torch.isfinite(g["dparams"]).all())
+    assert all(bool(torch.isfinite(a).all()) for a in g["dbrnd"])
+    assert bool(torch.isfinite(g["dternd"]).all())
+    assert float(g["dbrnd"][0].abs().max()) > 0
