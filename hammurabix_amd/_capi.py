@@ -24,6 +24,9 @@ HAMX_OUT_DEVICE = 2
 HAMX_KEEP_TAPE = 8
 HAMX_USE_TAPE = 16
 
+HAMX_STATIC_MAX_NSIDE = 256
+HAMX_STATIC_SCRATCH_BYTES = 40 * 1024 * 1024
+
 (HAMX_BREG_NONE, HAMX_BREG_LSA, HAMX_BREG_CART, HAMX_BREG_UNIF, HAMX_BREG_GRID,
  HAMX_BREG_HELIX, HAMX_BREG_JF12, HAMX_BREG_JAFFE) = range(8)
 HAMX_BREG_NPARAM = 40
@@ -119,6 +122,9 @@ _PROTOS = {
     "hamx_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "hamx_los_forward": (C.c_int, [C.c_void_p, C.POINTER(HamxLosDesc),
                                    C.POINTER(HamxFields), C.POINTER(HamxMaps), C.c_uint32]),
+    "hamx_los_forward_static": (C.c_int, [C.c_void_p, C.POINTER(HamxLosDesc),
+                                          C.POINTER(HamxFields), C.POINTER(HamxMaps),
+                                          C.c_uint32]),
     "hamx_los_samples": (C.c_int, [C.POINTER(HamxLosDesc), C.POINTER(C.c_uint64)]),
     "hamx_brnd_es": (C.c_int, [C.c_void_p, C.POINTER(HamxEsDesc), C.c_uint64,
                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),

@@ -177,6 +177,7 @@ int hamx_open(int device, hamx_ctx **out) {
     delete ctx;
     return HAMX_ERR_DEVICE;
   }
+  static_scratch_init(ctx);
   *out = ctx;
   return HAMX_OK;
 }

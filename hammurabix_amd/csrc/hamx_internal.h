@@ -146,6 +146,9 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
               uint64_t nz, int sign);
 // Free the per-context twiddle tables (called by hamx_close).
 void release_twiddles(hamx_ctx *ctx);
+// Look up, once per device, the device address of hamx_los_forward_static's
+// scratch array (called by hamx_open with the context's device current).
+int static_scratch_init(hamx_ctx *ctx);
 
 // Kernel timing: prof_begin records a start event before a launch when
 // profiling is on; prof_end records the stop event and queues the record,

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "hamx.h"
 #include "hamx_cgs.h"
@@ -686,8 +687,22 @@ FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool p
   return fc;
 }
 
+int scratch_get(hamx_ctx *ctx, Arena *ar, int slot, size_t bytes, void **out) {
+  if (!ar)
+    return ws_get(ctx, slot, bytes, out);
+  const size_t off = (ar->used + 255) & ~static_cast<size_t>(255);
+  if (off > ar->cap || bytes > ar->cap - off)
+    return set_err(ctx, HAMX_ERR_ARG,
+                   "the pass needs more than the %zu bytes of static scratch "
+                   "(%zu in use, %zu more requested): shells above nside %d need "
+                   "hamx_los_forward", ar->cap, off, bytes, HAMX_STATIC_MAX_NSIDE);
+  *out = ar->base + off;
+  ar->used = off + bytes;
+  return HAMX_OK;
+}
+
 int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
-                 bool in_dev, Grids &gr, bool &packed, bool &grids_any) {
+                 bool in_dev, Grids &gr, bool &packed, bool &grids_any, Arena *ar) {
   std::memset(&gr, 0, sizeof(gr));
   const bool has_brnd = f.brnd_x || f.brnd_y || f.brnd_z;
   if (has_brnd && !(f.brnd_x && f.brnd_y && f.brnd_z))
@@ -767,6 +782,9 @@ int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
       tab[2 * nE + i] = std::fabs(KE[i + 1] - KE[i]);
       tab[3 * nE + i] = std::log(0.5 * (1. / tab[i + 1] + 1. / tab[i]));
     }
+    if (ar)
+      return set_err(ctx, HAMX_ERR_UNSUPPORTED,
+                     "a CRE grid needs host-built energy tables: use hamx_los_forward");
     void *tp;
     HAMX_TRY(ws_get(ctx, kSlotCreTab, tab.size() * sizeof(double), &tp));
     HAMX_HIP(ctx, hipMemcpyAsync(tp, tab.data(), tab.size() * sizeof(double),
@@ -776,7 +794,7 @@ int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
   // Pack the random B components and TE density into one record per vertex
   // when they share a box: 64 B contiguous per z-pair of corners.
   packed = false;
-  if (d->cre_model != HAMX_CRE_GRID && has_brnd &&
+  if (!ar && d->cre_model != HAMX_CRE_GRID && has_brnd &&
       (!f.ternd || same_box(f.brnd_box, f.ternd_box))) {
     const uint64_t n = box_size(f.brnd_box);
     void *pk;
@@ -819,7 +837,8 @@ extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
 
 namespace hamx {
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
-                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd) {
+                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd,
+                     Arena *ar) {
   if (!ctx)
     return HAMX_ERR_ARG;
   if (!d || !out)
@@ -835,7 +854,7 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
   // ---- field grids
   Grids gr;
   bool packed = false, grids_any = false;
-  HAMX_TRY(stage_fields(ctx, d, f, in_dev, gr, packed, grids_any));
+  HAMX_TRY(stage_fields(ctx, d, f, in_dev, gr, packed, grids_any, ar));
 
   // ---- mask, duplicated to every shell's nside (integrator.cc:36-39)
   const double *mask_piv = nullptr;
@@ -863,10 +882,10 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     if (out_dev && user) {
       *dst = user;
     } else {
-      HAMX_TRY(ws_get(ctx, slot, h.npix * sizeof(double), &q));
+      HAMX_TRY(scratch_get(ctx, ar, slot, h.npix * sizeof(double), &q));
       *dst = static_cast<double *>(q);
     }
-    return fill_f64(ctx, *dst, h.npix, 0.);
+    return HAMX_OK;
   };
   if (d->do_dm)
     HAMX_TRY(out_map(kSlotOutDm, out->dm, h_dm, &o_dm));
@@ -889,21 +908,37 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
          *t_us = nullptr;
   const size_t tb = max_npix * sizeof(double);
   if (d->do_dm) {
-    HAMX_TRY(ws_get(ctx, kSlotTmpDm, tb, &q));
+    HAMX_TRY(scratch_get(ctx, ar, kSlotTmpDm, tb, &q));
     t_dm = static_cast<double *>(q);
   }
-  if (d->do_fd || d->do_sync) {
-    HAMX_TRY(ws_get(ctx, kSlotTmpFd, tb, &q));
+  if ((d->do_fd || d->do_sync) && !shell_fd) {
+    HAMX_TRY(scratch_get(ctx, ar, kSlotTmpFd, tb, &q));
     t_fd = static_cast<double *>(q);
   }
   if (d->do_sync) {
-    HAMX_TRY(ws_get(ctx, kSlotTmpIs, tb, &q));
+    HAMX_TRY(scratch_get(ctx, ar, kSlotTmpIs, tb, &q));
     t_is = static_cast<double *>(q);
-    HAMX_TRY(ws_get(ctx, kSlotTmpQs, tb, &q));
+    HAMX_TRY(scratch_get(ctx, ar, kSlotTmpQs, tb, &q));
     t_qs = static_cast<double *>(q);
-    HAMX_TRY(ws_get(ctx, kSlotTmpUs, tb, &q));
+    HAMX_TRY(scratch_get(ctx, ar, kSlotTmpUs, tb, &q));
     t_us = static_cast<double *>(q);
   }
+  // the mask at each shell's nside (one buffer, sized for the largest shell)
+  double *mask_shell = nullptr;
+  if (mask_piv) {
+    HAMX_TRY(scratch_get(ctx, ar, kSlotMaskShell, tb, &q));
+    mask_shell = static_cast<double *>(q);
+  }
+  // every buffer is in place: zero the output maps
+  if (o_dm)
+    HAMX_TRY(fill_f64(ctx, o_dm, h_dm.npix, 0.));
+  if (o_is) {
+    HAMX_TRY(fill_f64(ctx, o_is, h_sync.npix, 0.));
+    HAMX_TRY(fill_f64(ctx, o_qs, h_sync.npix, 0.));
+    HAMX_TRY(fill_f64(ctx, o_us, h_sync.npix, 0.));
+  }
+  if (o_fd)
+    HAMX_TRY(fill_f64(ctx, o_fd, h_fd.npix, 0.));
 
   const int sched = d->schedule == HAMX_SCHED_THREAD ? HAMX_SCHED_THREAD : HAMX_SCHED_WAVE;
   // the specialised kernel covers LSA + uniform TE + constant-index CRE with
@@ -934,12 +969,10 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     sa.t_us = t_us;
     const uint64_t npix = sa.hs.npix;
     if (mask_piv) {
-      void *mq;
-      HAMX_TRY(ws_get(ctx, kSlotMaskShell, npix * sizeof(double), &mq));
       hipLaunchKernelGGL(k_mask_dup, dim3(static_cast<unsigned>((npix + 255) / 256)), dim3(256),
-                         0, ctx->stream, static_cast<double *>(mq), sa.hs, mask_piv, h_mask);
+                         0, ctx->stream, mask_shell, sa.hs, mask_piv, h_mask);
       HAMX_HIP(ctx, hipGetLastError());
-      sa.mask = static_cast<const double *>(mq);
+      sa.mask = mask_shell;
     }
     if (ckpt && d->do_fd) // FD map before shell s, for the adjoint
       HAMX_HIP(ctx, hipMemcpyAsync(ckpt + s * h_fd.npix, o_fd, h_fd.npix * sizeof(double),
@@ -1094,6 +1127,86 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
   if (ctx && (flags & HAMX_KEEP_TAPE))
     return los_forward_taped(ctx, d, fields, out, flags);
   return los_forward_impl(ctx, d, fields, out, flags, nullptr, nullptr);
+}
+
+// ------------------------------------------- the allocation-free forward --
+namespace hamx {
+namespace {
+
+// The scratch of hamx_los_forward_static: the five shell maps and the mask at
+// the shell's nside, for shells up to nside HAMX_STATIC_MAX_NSIDE.  Every GPU
+// the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: kernels and
+// copies take the device address looked up by static_scratch_init.
+static_assert(HAMX_STATIC_SCRATCH_BYTES >=
+                  6 * (12ull * HAMX_STATIC_MAX_NSIDE * HAMX_STATIC_MAX_NSIDE * sizeof(double) +
+                       256),
+              "static scratch below six maps at HAMX_STATIC_MAX_NSIDE");
+static __device__ __align__(256) double scratch[HAMX_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` per device
+int g_scratch_err[kMaxDevices] = {};    // hipError_t of a failed lookup
+
+} // namespace
+
+int static_scratch_init(hamx_ctx *ctx) {
+  // with ctx->device current (hamx_open)
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return HAMX_OK; // hamx_los_forward_static reports it
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[ctx->device])
+    return HAMX_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    g_scratch_err[ctx->device] = e != hipSuccess ? static_cast<int>(e) : -1;
+    return HAMX_OK; // the other entry points do not need it
+  }
+  g_scratch_dev[ctx->device] = p;
+  return HAMX_OK;
+}
+
+} // namespace hamx
+
+extern "C" int hamx_los_forward_static(hamx_ctx *ctx, const hamx_los_desc *d,
+                                       const hamx_fields *fields, hamx_maps *out,
+                                       uint32_t flags) {
+  if (!ctx)
+    return HAMX_ERR_ARG;
+  if (!d || !out)
+    return set_err(ctx, HAMX_ERR_ARG, "null descriptor or output maps");
+  const uint32_t dev_io = HAMX_IN_DEVICE | HAMX_OUT_DEVICE;
+  if ((flags & dev_io) != dev_io || (flags & ~dev_io))
+    return set_err(ctx, HAMX_ERR_ARG,
+                   "hamx_los_forward_static takes device inputs and outputs "
+                   "(flags HAMX_IN_DEVICE | HAMX_OUT_DEVICE) and no other flag");
+  if ((d->do_dm && !out->dm) || (d->do_sync && !(out->is && out->qs && out->us)) ||
+      ((d->do_fd || d->do_sync) && !out->fd))
+    return set_err(ctx, HAMX_ERR_ARG,
+                   "hamx_los_forward_static needs every output map of the pass "
+                   "(dm with do_dm, is/qs/us with do_sync, fd with do_fd or do_sync)");
+  for (uint32_t s = 0; s < d->n_shell && s < HAMX_MAX_SHELL; ++s)
+    if (d->nside_shell[s] > HAMX_STATIC_MAX_NSIDE)
+      return set_err(ctx, HAMX_ERR_ARG,
+                     "shell %u at nside %u: hamx_los_forward_static holds shells up to "
+                     "nside %d in its %d bytes of scratch (use hamx_los_forward)",
+                     s, d->nside_shell[s], HAMX_STATIC_MAX_NSIDE, HAMX_STATIC_SCRATCH_BYTES);
+  void *base = nullptr;
+  int err = 0;
+  if (ctx->device >= 0 && ctx->device < kMaxDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_dev[ctx->device];
+    err = g_scratch_err[ctx->device];
+  }
+  if (!base)
+    return set_err(ctx, HAMX_ERR_DEVICE,
+                   "no static scratch on device %d (address lookup returned %d)",
+                   ctx->device, err);
+  Arena ar{static_cast<char *>(base), HAMX_STATIC_SCRATCH_BYTES, 0};
+  return los_forward_impl(ctx, d, fields, out, flags, nullptr, nullptr, &ar);
 }
 
 // ------------------------------------------------------ CRE grid helpers --

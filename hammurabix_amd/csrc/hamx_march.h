@@ -611,11 +611,23 @@ void shell_ref(const hamx_los_desc *d, uint32_t s, uint64_t &step, double &d_sta
 int validate_los(hamx_ctx *ctx, const hamx_los_desc *d);
 Phys make_phys(const hamx_los_desc *d);
 FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool pw1);
+// A fixed device buffer handed out in call order at 256-byte alignment
+// (the scratch of hamx_los_forward_static).  Nothing is allocated: a request
+// beyond `cap` fails.
+struct Arena {
+  char *base;
+  size_t cap, used;
+};
+// Device buffer of `bytes`: carved from `ar` when it is non-null, else the
+// context's grow-only workspace slot `slot` (which may allocate).
+int scratch_get(hamx_ctx *ctx, Arena *ar, int slot, size_t bytes, void **out);
 // Stage the field grids of a call on the device (uploading host inputs into
 // the workspace) and pack brnd + ternd into one double4 record per vertex
-// when they share a box.
+// when they share a box.  With an arena (device inputs only) the grids are
+// gathered where they lie and nothing is packed.
 int stage_fields(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields &f,
-                 bool in_dev, Grids &gr, bool &packed, bool &grids_any);
+                 bool in_dev, Grids &gr, bool &packed, bool &grids_any,
+                 Arena *ar = nullptr);
 // First pixel of shell s in a buffer holding one value per ray of every
 // shell: sum over t < s of 12 nside_t^2.
 inline uint64_t shell_offset(const hamx_los_desc *d, uint32_t s) {
@@ -628,8 +640,12 @@ inline uint64_t shell_offset(const hamx_los_desc *d, uint32_t s) {
 // carried, the accumulated FD map before shell s is copied to
 // ckpt + s * npix(fd map); when shell_fd is non-null each ray's FD over
 // shell s is written to shell_fd + shell_offset(d, s) (device memory).
+// With an arena every scratch buffer (shell maps, the mask at the shell's
+// nside, output maps the caller left NULL) is carved from it before the
+// first launch, so an arena too small fails before any work is queued.
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
-                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd);
+                     hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd,
+                     Arena *ar = nullptr);
 // hamx_los_forward recording the adjoint's tape in the context workspace.
 int los_forward_taped(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
                       hamx_maps *out, uint32_t flags);

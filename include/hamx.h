@@ -258,6 +258,22 @@ int hamx_memcpy_d2h(hamx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
                      const hamx_fields *fields, hamx_maps *out, uint32_t flags);
 
+/* hamx_los_forward without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Inputs and
+ * outputs are device pointers (flags must be exactly HAMX_IN_DEVICE |
+ * HAMX_OUT_DEVICE) and every output map of the pass must be given.  The shell
+ * maps and the mask at the shell's nside live in one fixed device array of
+ * HAMX_STATIC_SCRATCH_BYTES per GPU (held until the process ends), so shells
+ * are limited to nside HAMX_STATIC_MAX_NSIDE; a CRE grid and the tape are not
+ * supported, and random B and TE grids are gathered where they lie (not
+ * packed).  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (from several contexts or threads) race.  Synchronous. */
+#define HAMX_STATIC_MAX_NSIDE 256
+#define HAMX_STATIC_SCRATCH_BYTES (40 * 1024 * 1024)
+int hamx_los_forward_static(hamx_ctx *ctx, const hamx_los_desc *d,
+                            const hamx_fields *fields, hamx_maps *out, uint32_t flags);
+
 /* Reverse mode of hamx_los_forward.  Given adj = dL/d(maps) for a scalar L
  * of the pass's output maps (same sizes as the forward maps; NULL maps are
  * zero), returns
