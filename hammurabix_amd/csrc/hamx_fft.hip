@@ -1,30 +1,56 @@
 // In-place 3D complex128 FFT for gfx950: one kernel launch per axis, each
-// workgroup transforming a tile of C lines of length n held in LDS with the
-// Stockham autosort formulation (radix-4 passes, one radix-2 pass when
-// log2 n is odd).  Unnormalised, exponent sign * 2 pi i jk / n: sign = +1 is
-// the reference's FFTW_BACKWARD plan, -1 its FFTW_FORWARD plan
-// (grid_brnd.cc:42-54, grid_ternd.cc:35-37).
+// workgroup transforming a tile of C lines held in LDS.  Unnormalised,
+// exponent sign * 2 pi i jk / n: sign = +1 is the reference's FFTW_BACKWARD
+// plan, -1 its FFTW_FORWARD plan (grid_brnd.cc:42-54, grid_ternd.cc:35-37).
+// FFTW plans any length; so does this file:
+//
+// * n = 2^a 3^b 5^c 7^d <= 8192: mixed-radix Stockham autosort passes in
+//   LDS (radix 4 while 4 divides what is left, then 2, 3, 5, 7).  A pass of
+//   radix R stages its inputs in registers, synchronises, then writes its
+//   outputs, so one LDS buffer of C * n values suffices.
+// * any other n <= 4096: Bluestein's algorithm.  With c_j = exp(sign i pi j^2 / n),
+//   X_k = c_k sum_j (x_j c_j) conj(c_{k-j}): the line times the chirp, zero
+//   padded to a power of two M >= 2n - 1, is convolved with conj(c) by an
+//   M-point forward transform, a product with the transform of conj(c)
+//   (tabulated once per length and sign), and an M-point inverse transform,
+//   all in LDS; the outputs are multiplied by the chirp and 1/M.
 //
 // Global access: along the contiguous z axis each line is a contiguous run;
 // along y and x the tile's C lines are C adjacent columns, so every tile row
-// is C*16 contiguous bytes (C = 16 -> 256 B rows).  Twiddles come from a
-// per-length table exp(2 pi i m / n) computed on the host in long double.
+// is C*16 contiguous bytes (C = 16 -> 256 B rows).  Twiddles and chirps come
+// from per-length tables computed on the host in long double.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
+#include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "hamx_internal.h"
 
 namespace hamx {
 
+constexpr uint32_t kMaxPass = 16;
+constexpr uint32_t kTileSmall = 4096; // complex128 per workgroup: 64 KB of LDS
+constexpr uint32_t kTileLarge = 8192; // 128 KB: lines of 4097 to 8192 points
+
+struct FftPlan {
+  uint32_t n;     // transform length
+  uint32_t npass; // Stockham passes
+  uint8_t radix[kMaxPass];
+};
+
 struct FftAxis {
   uint64_t s;      // element stride of the transformed axis
   uint64_t inner;  // lines with consecutive base addresses
   uint64_t outer;  // base-address step between groups of `inner` lines
-  uint32_t n;      // line length (power of 2)
+  uint64_t lines;  // lines along this axis
+  uint32_t n;      // line length
+  uint32_t M;      // 0: direct; else the Bluestein convolution length
   uint32_t C;      // lines per workgroup
+  FftPlan plan;    // passes of n (direct) or of M (Bluestein)
 };
 
 __device__ inline double2 cmul(double2 a, double2 b) {
@@ -33,6 +59,7 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
+// exp(SIGN 2 pi i j / n) from the table of exp(+2 pi i m / n)
 template <int SIGN>
 __device__ inline double2 twiddle(const double2 *tw, uint32_t j) {
   double2 w = tw[j];
@@ -41,159 +68,339 @@ __device__ inline double2 twiddle(const double2 *tw, uint32_t j) {
   return w;
 }
 
-template <int SIGN>
-__global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data,
-                                                  FftAxis ax,
+// y_t = sum_m x_m W^(t m), W = exp(SIGN 2 pi i / R); w[t] = W^t.
+template <int R, int SIGN>
+__device__ inline void small_dft(const double2 (&x)[R], const double2 (&w)[R], double2 (&y)[R]) {
+  if constexpr (R == 2) {
+    y[0] = cadd(x[0], x[1]);
+    y[1] = csub(x[0], x[1]);
+  } else if constexpr (R == 4) {
+    const double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]), a2 = cadd(x[1], x[3]);
+    const double2 d13 = csub(x[1], x[3]);
+    // (x1 - x3) * SIGN * i
+    const double2 a3 = SIGN > 0 ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+    y[0] = cadd(a0, a2);
+    y[1] = cadd(a1, a3);
+    y[2] = csub(a0, a2);
+    y[3] = csub(a1, a3);
+  } else {
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      double2 acc = x[0];
+#pragma unroll
+      for (int m = 1; m < R; ++m)
+        acc = cadd(acc, cmul(x[m], w[(t * m) % R]));
+      y[t] = acc;
+    }
+  }
+}
+
+// One radix-R Stockham pass over C lines of length n in LDS: p is the
+// product of the radices before it, tw the table of exp(2 pi i m / n).
+template <int R, int SIGN, uint32_t TILE>
+__device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint32_t p,
+                                     const double2 *__restrict__ tw) {
+  constexpr uint32_t NB = (TILE / R + 255) / 256; // butterflies per thread
+  const uint32_t q = n / R, nbf = C * q, tstep = n / (R * p);
+  double2 x[NB][R];
+#pragma unroll
+  for (uint32_t u = 0; u < NB; ++u) {
+    const uint32_t b = threadIdx.x + u * 256;
+    if (b < nbf) {
+      const uint32_t c = b / q, i = b - c * q;
+      const double2 *line = lds + c * n;
+#pragma unroll
+      for (int m = 0; m < R; ++m)
+        x[u][m] = line[i + m * q];
+    }
+  }
+  __syncthreads();
+  double2 w[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t)
+    w[t] = twiddle<SIGN>(tw, t * q); // exp(SIGN 2 pi i t / R)
+#pragma unroll
+  for (uint32_t u = 0; u < NB; ++u) {
+    const uint32_t b = threadIdx.x + u * 256;
+    if (b < nbf) {
+      const uint32_t c = b / q, i = b - c * q;
+      const uint32_t k = i % p;
+      double2 xs[R], y[R];
+      xs[0] = x[u][0];
+#pragma unroll
+      for (int m = 1; m < R; ++m)
+        xs[m] = cmul(x[u][m], twiddle<SIGN>(tw, m * k * tstep));
+      small_dft<R, SIGN>(xs, w, y);
+      double2 *line = lds + c * n;
+      const uint32_t j = (i - k) * R + k;
+#pragma unroll
+      for (int m = 0; m < R; ++m)
+        line[j + m * p] = y[m];
+    }
+  }
+  __syncthreads();
+}
+
+// Every pass of a plan, in order.  Entered and left with LDS synchronised.
+// POW2 kernels (plans of radix 4 and 2 only) leave the odd radices out, so
+// their register staging stays at the radix-4 / radix-2 size.
+template <int SIGN, uint32_t TILE, bool POW2>
+__device__ inline void run_plan(double2 *lds, const FftPlan &pl, uint32_t C,
+                                const double2 *__restrict__ tw) {
+  uint32_t p = 1;
+  for (uint32_t s = 0; s < pl.npass; ++s) {
+    const uint32_t r = pl.radix[s];
+    if (r == 4) {
+      stockham_pass<4, SIGN, TILE>(lds, pl.n, C, p, tw);
+    } else if (r == 2) {
+      stockham_pass<2, SIGN, TILE>(lds, pl.n, C, p, tw);
+    } else if constexpr (!POW2) {
+      if (r == 3)
+        stockham_pass<3, SIGN, TILE>(lds, pl.n, C, p, tw);
+      else if (r == 5)
+        stockham_pass<5, SIGN, TILE>(lds, pl.n, C, p, tw);
+      else
+        stockham_pass<7, SIGN, TILE>(lds, pl.n, C, p, tw);
+    }
+    p *= r;
+  }
+}
+
+// Tile element e -> (line c of the tile, position t on the line): along the
+// contiguous axis a line is a run; along strided axes the C lines of a tile
+// are adjacent columns, so consecutive threads take consecutive columns.
+__device__ inline void tile_coords(const FftAxis &ax, uint32_t len, uint32_t e, uint32_t &c,
+                                   uint32_t &t) {
+  if (ax.s == 1) {
+    c = e / len;
+    t = e - c * len;
+  } else {
+    c = e % ax.C;
+    t = e / ax.C;
+  }
+}
+
+__device__ inline uint64_t line_base(const FftAxis &ax, uint64_t L) {
+  return (L / ax.inner) * ax.outer + (L % ax.inner);
+}
+
+template <int SIGN, uint32_t TILE, bool POW2>
+__global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data, FftAxis ax,
                                                   const double2 *__restrict__ tw) {
-  extern __shared__ double2 lds[];
+  __shared__ double2 lds[TILE];
   const uint32_t n = ax.n, C = ax.C, total = C * n;
   const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * C;
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     uint32_t c, t;
-    if (ax.s == 1) {
-      c = e / n;
-      t = e % n;
-    } else {
-      c = e % C;
-      t = e / C;
-    }
+    tile_coords(ax, n, e, c, t);
     const uint64_t L = L0 + c;
-    const uint64_t base = (L / ax.inner) * ax.outer + (L % ax.inner);
-    lds[c * n + t] = data[base + static_cast<uint64_t>(t) * ax.s];
+    lds[c * n + t] = L < ax.lines ? data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s]
+                                  : make_double2(0., 0.);
   }
   __syncthreads();
-  // Every pass stages its inputs in registers (at most 16 complex values
-  // per thread: C*n <= 4096 and 256 threads), then writes the outputs.
-  uint32_t p = 1;
-  // radix-4 Stockham passes
-  while (p * 4 <= n) {
-    const uint32_t q4 = n >> 2;
-    const uint32_t nbf = C * q4;
-    double2 x[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t b = threadIdx.x + q * 256;
-      if (b < nbf) {
-        const uint32_t c = b / q4, i = b % q4;
-        const double2 *line = lds + c * n;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          x[q][m] = line[i + m * q4];
-      }
-    }
-    __syncthreads();
-    const uint32_t tstep = n / (4 * p);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t b = threadIdx.x + q * 256;
-      if (b < nbf) {
-        const uint32_t c = b / q4, i = b % q4;
-        const uint32_t k = i & (p - 1);
-        const double2 x0 = x[q][0];
-        const double2 x1 = cmul(x[q][1], twiddle<SIGN>(tw, k * tstep));
-        const double2 x2 = cmul(x[q][2], twiddle<SIGN>(tw, 2 * k * tstep));
-        const double2 x3 = cmul(x[q][3], twiddle<SIGN>(tw, 3 * k * tstep));
-        const double2 a0 = cadd(x0, x2), a1 = csub(x0, x2), a2 = cadd(x1, x3);
-        const double2 d13 = csub(x1, x3);
-        // (x1 - x3) * SIGN * i
-        const double2 a3 =
-            SIGN > 0 ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
-        double2 *line = lds + c * n;
-        const uint32_t j = ((i - k) << 2) + k;
-        line[j] = cadd(a0, a2);
-        line[j + p] = cadd(a1, a3);
-        line[j + 2 * p] = csub(a0, a2);
-        line[j + 3 * p] = csub(a1, a3);
-      }
-    }
-    __syncthreads();
-    p <<= 2;
-  }
-  // final radix-2 pass (p = n/2)
-  if (p * 2 == n) {
-    const uint32_t q2 = n >> 1;
-    const uint32_t nbf = C * q2;
-    double2 x[8][2];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t b = threadIdx.x + q * 256;
-      if (b < nbf) {
-        const uint32_t c = b / q2, i = b % q2;
-        x[q][0] = lds[c * n + i];
-        x[q][1] = lds[c * n + i + q2];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t b = threadIdx.x + q * 256;
-      if (b < nbf) {
-        const uint32_t c = b / q2, i = b % q2;
-        const double2 x1 = cmul(x[q][1], twiddle<SIGN>(tw, i));
-        lds[c * n + i] = cadd(x[q][0], x1);
-        lds[c * n + i + p] = csub(x[q][0], x1);
-      }
-    }
-    __syncthreads();
-  }
+  run_plan<SIGN, TILE, POW2>(lds, ax.plan, C, tw);
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     uint32_t c, t;
-    if (ax.s == 1) {
-      c = e / n;
-      t = e % n;
-    } else {
-      c = e % C;
-      t = e / C;
-    }
+    tile_coords(ax, n, e, c, t);
     const uint64_t L = L0 + c;
-    const uint64_t base = (L / ax.inner) * ax.outer + (L % ax.inner);
-    data[base + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
+    if (L < ax.lines)
+      data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
   }
 }
 
-// exp(+2 pi i m / n), m in [0, n), cached per context and length.
-static std::map<std::pair<hamx_ctx *, uint64_t>, double2 *> g_twiddles;
+// Bluestein: chirp[j] = exp(SIGN i pi (j^2 mod 2n) / n) for j < n; bhat the
+// M-point forward transform of conj(chirp) laid out circularly; twm the table
+// of exp(2 pi i m / M).
+template <int SIGN, uint32_t TILE>
+__global__ __launch_bounds__(256) void k_fft_axis_bluestein(double2 *__restrict__ data,
+                                                            FftAxis ax,
+                                                            const double2 *__restrict__ twm,
+                                                            const double2 *__restrict__ chirp,
+                                                            const double2 *__restrict__ bhat) {
+  __shared__ double2 lds[TILE];
+  const uint32_t n = ax.n, M = ax.M, C = ax.C, total = C * M;
+  const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * C;
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    uint32_t c, t;
+    tile_coords(ax, M, e, c, t);
+    const uint64_t L = L0 + c;
+    double2 v = make_double2(0., 0.);
+    if (t < n && L < ax.lines)
+      v = cmul(data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s], chirp[t]);
+    lds[c * M + t] = v;
+  }
+  __syncthreads();
+  run_plan<-1, TILE, true>(lds, ax.plan, C, twm);
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x)
+    lds[e] = cmul(lds[e], bhat[e % M]);
+  __syncthreads();
+  run_plan<1, TILE, true>(lds, ax.plan, C, twm);
+  const double inv_m = 1.0 / static_cast<double>(M);
+  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    uint32_t c, t;
+    tile_coords(ax, M, e, c, t);
+    const uint64_t L = L0 + c;
+    if (t < n && L < ax.lines) {
+      const double2 v = cmul(lds[c * M + t], chirp[t]);
+      data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s] =
+          make_double2(v.x * inv_m, v.y * inv_m);
+    }
+  }
+}
 
-static int twiddle_table(hamx_ctx *ctx, uint64_t n, const double2 **out) {
-  auto key = std::make_pair(ctx, n);
-  auto it = g_twiddles.find(key);
-  if (it != g_twiddles.end()) {
-    *out = it->second;
-    return HAMX_OK;
-  }
-  std::vector<double2> h(n);
-  const long double twopi = 6.283185307179586476925286766559005768394L;
-  for (uint64_t m = 0; m < n; ++m) {
-    const long double a = twopi * static_cast<long double>(m) / static_cast<long double>(n);
-    h[m] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
-  }
+// ------------------------------------------------------------------ host --
+namespace {
+
+using cld = std::complex<long double>;
+const long double kTwoPiL = 6.283185307179586476925286766559005768394L;
+
+// Radices of a Stockham plan for n: 4 while 4 divides, then 2, 3, 5, 7.
+bool make_plan(uint32_t n, FftPlan &pl) {
+  pl.n = n;
+  pl.npass = 0;
+  uint32_t m = n;
+  auto push = [&](uint32_t r) {
+    if (pl.npass >= kMaxPass)
+      return false;
+    pl.radix[pl.npass++] = static_cast<uint8_t>(r);
+    m /= r;
+    return true;
+  };
+  while (m % 4 == 0)
+    if (!push(4))
+      return false;
+  if (m % 2 == 0 && !push(2))
+    return false;
+  for (uint32_t r : {3u, 5u, 7u})
+    while (m % r == 0)
+      if (!push(r))
+        return false;
+  return m == 1;
+}
+
+// Device tables cached per context: kind 0 = twiddles exp(2 pi i m / n),
+// kind 1 / 2 = Bluestein chirp / kernel transform of sign (kind - 1.5 > 0).
+std::map<std::tuple<hamx_ctx *, uint64_t, int>, double2 *> g_tables;
+
+int upload(hamx_ctx *ctx, const std::vector<double2> &h, std::tuple<hamx_ctx *, uint64_t, int> key,
+           const double2 **out) {
   double2 *d = nullptr;
-  HAMX_HIP(ctx, hipMalloc(&d, n * sizeof(double2)));
-  HAMX_HIP(ctx, hipMemcpy(d, h.data(), n * sizeof(double2), hipMemcpyHostToDevice));
-  g_twiddles[key] = d;
+  HAMX_HIP(ctx, hipMalloc(&d, h.size() * sizeof(double2)));
+  HAMX_HIP(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  g_tables[key] = d;
   *out = d;
   return HAMX_OK;
 }
 
+int twiddle_table(hamx_ctx *ctx, uint64_t n, const double2 **out) {
+  auto key = std::make_tuple(ctx, n, 0);
+  auto it = g_tables.find(key);
+  if (it != g_tables.end()) {
+    *out = it->second;
+    return HAMX_OK;
+  }
+  std::vector<double2> h(n);
+  for (uint64_t m = 0; m < n; ++m) {
+    const long double a = kTwoPiL * static_cast<long double>(m) / static_cast<long double>(n);
+    h[m] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
+  }
+  return upload(ctx, h, key, out);
+}
+
+// In-place radix-2 transform of a power-of-two length, exponent sign (host,
+// long double; for the Bluestein kernel table only).
+void host_fft_pow2(std::vector<cld> &a, int sign) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1)
+      j ^= bit;
+    j ^= bit;
+    if (i < j)
+      std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t k = 0; k < len / 2; ++k) {
+      const long double ang = sign * kTwoPiL * static_cast<long double>(k) /
+                              static_cast<long double>(len);
+      const cld w(cosl(ang), sinl(ang));
+      for (size_t s = 0; s < n; s += len) {
+        const cld u = a[s + k], v = a[s + k + len / 2] * w;
+        a[s + k] = u + v;
+        a[s + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+int bluestein_tables(hamx_ctx *ctx, uint64_t n, uint64_t M, int sign, const double2 **chirp,
+                     const double2 **bhat) {
+  const int kc = sign > 0 ? 2 : 1;
+  auto kchirp = std::make_tuple(ctx, n, 2 * kc), kbhat = std::make_tuple(ctx, n, 2 * kc + 1);
+  auto ic = g_tables.find(kchirp), ib = g_tables.find(kbhat);
+  if (ic != g_tables.end() && ib != g_tables.end()) {
+    *chirp = ic->second;
+    *bhat = ib->second;
+    return HAMX_OK;
+  }
+  // c_j = exp(sign i pi (j^2 mod 2n) / n): the reduction keeps the angle exact
+  std::vector<cld> c(n);
+  for (uint64_t j = 0; j < n; ++j) {
+    const uint64_t r = (j * j) % (2 * n);
+    const long double a = sign * (kTwoPiL / 2) * static_cast<long double>(r) /
+                          static_cast<long double>(n);
+    c[j] = cld(cosl(a), sinl(a));
+  }
+  std::vector<cld> b(M, cld(0, 0));
+  b[0] = std::conj(c[0]);
+  for (uint64_t j = 1; j < n; ++j)
+    b[j] = b[M - j] = std::conj(c[j]);
+  host_fft_pow2(b, -1);
+  std::vector<double2> hc(n), hb(M);
+  for (uint64_t j = 0; j < n; ++j)
+    hc[j] = make_double2(static_cast<double>(c[j].real()), static_cast<double>(c[j].imag()));
+  for (uint64_t j = 0; j < M; ++j)
+    hb[j] = make_double2(static_cast<double>(b[j].real()), static_cast<double>(b[j].imag()));
+  HAMX_TRY(upload(ctx, hc, kchirp, chirp));
+  return upload(ctx, hb, kbhat, bhat);
+}
+
+uint64_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v)
+    p <<= 1;
+  return p;
+}
+
+} // namespace
+
 void release_twiddles(hamx_ctx *ctx) {
-  for (auto it = g_twiddles.begin(); it != g_twiddles.end();) {
-    if (it->first.first == ctx) {
+  for (auto it = g_tables.begin(); it != g_tables.end();) {
+    if (std::get<0>(it->first) == ctx) {
       (void)hipFree(it->second);
-      it = g_twiddles.erase(it);
+      it = g_tables.erase(it);
     } else {
       ++it;
     }
   }
 }
 
+bool fft_length_ok(uint64_t n) {
+  FftPlan pl;
+  if (n >= 1 && n <= kTileLarge && make_plan(static_cast<uint32_t>(n), pl))
+    return true;
+  return n >= 1 && n <= kTileSmall;
+}
+
 int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
               uint64_t nz, int sign) {
   const uint64_t dims[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a)
-    if (!is_pow2(dims[a]) || dims[a] > 4096)
+    if (!fft_length_ok(dims[a]))
       return set_err(ctx, HAMX_ERR_UNSUPPORTED,
-                     "FFT axis length %llu is not a power of two in [1, 4096]",
-                     static_cast<unsigned long long>(dims[a]));
+                     "FFT axis length %llu: lengths of the form 2^a 3^b 5^c 7^d up to %u "
+                     "and any other length up to %u are supported",
+                     static_cast<unsigned long long>(dims[a]), kTileLarge, kTileSmall);
   const uint64_t total = nx * ny * nz;
   // axis order z, y, x (any order gives the same transform)
   for (int a = 2; a >= 0; --a) {
@@ -201,6 +408,7 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
     if (n < 2)
       continue;
     FftAxis ax;
+    std::memset(&ax, 0, sizeof(ax));
     ax.n = static_cast<uint32_t>(n);
     if (a == 2) {
       ax.s = 1;
@@ -215,27 +423,73 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
       ax.inner = ny * nz;
       ax.outer = total;
     }
-    // lines per workgroup: fill 4096 complex of LDS (64 KB); for strided
-    // axes the lines must be adjacent columns, so C divides `inner`.
-    uint64_t C = 4096 / n;
-    if (C > 16 && ax.s != 1)
+    ax.lines = total / n;
+    const bool direct = make_plan(ax.n, ax.plan);
+    uint64_t len = n;
+    if (!direct) {
+      ax.M = static_cast<uint32_t>(next_pow2(2 * n - 1));
+      len = ax.M;
+      make_plan(ax.M, ax.plan);
+    }
+    const uint32_t tile = len <= kTileSmall ? kTileSmall : kTileLarge;
+    // lines per workgroup: fill the tile; for strided axes the lines are
+    // adjacent columns (at most 16, 256 B rows)
+    uint64_t C = tile / len;
+    if (ax.s != 1 && C > 16)
       C = 16;
     if (ax.s != 1 && C > ax.inner)
       C = ax.inner;
-    const uint64_t lines = total / n;
-    if (C > lines)
-      C = lines;
+    if (C > ax.lines)
+      C = ax.lines;
     ax.C = static_cast<uint32_t>(C);
-    const double2 *tw;
-    HAMX_TRY(twiddle_table(ctx, n, &tw));
-    const size_t lds = static_cast<size_t>(C * n) * sizeof(double2);
-    const unsigned blocks = static_cast<unsigned>(lines / C);
-    if (sign > 0)
-      hipLaunchKernelGGL(k_fft_axis<1>, dim3(blocks), dim3(256), lds, ctx->stream,
-                         data, ax, tw);
-    else
-      hipLaunchKernelGGL(k_fft_axis<-1>, dim3(blocks), dim3(256), lds, ctx->stream,
-                         data, ax, tw);
+    const unsigned blocks = static_cast<unsigned>((ax.lines + C - 1) / C);
+    if (direct) {
+      const double2 *tw;
+      HAMX_TRY(twiddle_table(ctx, n, &tw));
+      const bool pow2 = is_pow2(n);
+#define HAMX_FFT_LAUNCH(S, T, P)                                                           \
+  hipLaunchKernelGGL((k_fft_axis<S, T, P>), dim3(blocks), dim3(256), 0, ctx->stream, data, ax, \
+                     tw)
+#define HAMX_FFT_LAUNCH_T(S, T)                                                            \
+  do {                                                                                     \
+    if (pow2)                                                                              \
+      HAMX_FFT_LAUNCH(S, T, true);                                                         \
+    else                                                                                   \
+      HAMX_FFT_LAUNCH(S, T, false);                                                        \
+  } while (0)
+      if (sign > 0) {
+        if (tile == kTileSmall)
+          HAMX_FFT_LAUNCH_T(1, kTileSmall);
+        else
+          HAMX_FFT_LAUNCH_T(1, kTileLarge);
+      } else {
+        if (tile == kTileSmall)
+          HAMX_FFT_LAUNCH_T(-1, kTileSmall);
+        else
+          HAMX_FFT_LAUNCH_T(-1, kTileLarge);
+      }
+#undef HAMX_FFT_LAUNCH_T
+#undef HAMX_FFT_LAUNCH
+    } else {
+      const double2 *twm, *chirp, *bhat;
+      HAMX_TRY(twiddle_table(ctx, ax.M, &twm));
+      HAMX_TRY(bluestein_tables(ctx, n, ax.M, sign, &chirp, &bhat));
+#define HAMX_FFT_LAUNCH(S, T)                                                              \
+  hipLaunchKernelGGL((k_fft_axis_bluestein<S, T>), dim3(blocks), dim3(256), 0, ctx->stream, \
+                     data, ax, twm, chirp, bhat)
+      if (sign > 0) {
+        if (tile == kTileSmall)
+          HAMX_FFT_LAUNCH(1, kTileSmall);
+        else
+          HAMX_FFT_LAUNCH(1, kTileLarge);
+      } else {
+        if (tile == kTileSmall)
+          HAMX_FFT_LAUNCH(-1, kTileSmall);
+        else
+          HAMX_FFT_LAUNCH(-1, kTileLarge);
+      }
+#undef HAMX_FFT_LAUNCH
+    }
     HAMX_HIP(ctx, hipGetLastError());
   }
   return HAMX_OK;

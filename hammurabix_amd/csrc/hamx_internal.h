@@ -141,6 +141,9 @@ int fill_f64(hamx_ctx *ctx, double *p, size_t n, double v);
 
 inline bool is_pow2(uint64_t n) { return n != 0 && (n & (n - 1)) == 0; }
 
+// True when fft3d_dev transforms lines of length n: 2^a 3^b 5^c 7^d up to
+// 8192, any other length up to 4096 (hamx_fft.hip).
+bool fft_length_ok(uint64_t n);
 // In-place unnormalised 3D c2c FFT of device data (hamx_fft.hip).
 int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
               uint64_t nz, int sign);

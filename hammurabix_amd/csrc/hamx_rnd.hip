@@ -479,10 +479,12 @@ static unsigned ew_blocks(uint64_t n) {
 static int check_gen_box(hamx_ctx *ctx, const hamx_box &b) {
   if (b.nx < 2 || b.ny < 2 || b.nz < 2)
     return set_err(ctx, HAMX_ERR_ARG, "random-field grid needs at least 2 vertices per axis");
-  if (!is_pow2(b.nx) || !is_pow2(b.ny) || !is_pow2(b.nz) || b.nx > 4096 ||
-      b.ny > 4096 || b.nz > 4096)
+  if (!fft_length_ok(b.nx) || !fft_length_ok(b.ny) || !fft_length_ok(b.nz))
     return set_err(ctx, HAMX_ERR_UNSUPPORTED,
-                   "random-field grid axes must be powers of two <= 4096");
+                   "random-field grid %llux%llux%llu: FFT axes must be 2^a 3^b 5^c 7^d "
+                   "up to 8192 or any length up to 4096",
+                   static_cast<unsigned long long>(b.nx), static_cast<unsigned long long>(b.ny),
+                   static_cast<unsigned long long>(b.nz));
   if (!(b.x_max > b.x_min) || !(b.y_max > b.y_min) || !(b.z_max > b.z_min))
     return set_err(ctx, HAMX_ERR_ARG, "random-field box limits are not increasing");
   return HAMX_OK;

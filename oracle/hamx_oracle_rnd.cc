@@ -13,6 +13,7 @@
 #include "hamx_oracle.h"
 
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <vector>
 
@@ -75,9 +76,9 @@ void gauss_pair(uint64_t seed, uint32_t stream, uint64_t m, double &g0,
   g1 = r * std::sin(t);
 }
 
-// Iterative radix-2 complex FFT of one strided line, exponent sign*2*pi*i*jk/n
+// Iterative radix-2 complex FFT of one line, exponent sign*2*pi*i*jk/n
 // (FFTW convention: sign = +1 is FFTW_BACKWARD), unnormalised.
-void fft_line(std::vector<double> &buf, uint64_t n, int sign) {
+void fft_line_pow2(std::vector<double> &buf, uint64_t n, int sign) {
   // bit reversal
   for (uint64_t i = 1, j = 0; i < n; ++i) {
     uint64_t bit = n >> 1;
@@ -104,6 +105,61 @@ void fft_line(std::vector<double> &buf, uint64_t n, int sign) {
         buf[2 * a + 1] += bi;
       }
     }
+  }
+}
+
+// Any length: decimation in time by the smallest prime factor p of n,
+// X_k = sum_r exp(sign 2 pi i r k / n) FFT_{n/p}(x_{p m + r})_{k mod n/p},
+// and the plain DFT for a prime length.  Angles are reduced modulo n in
+// integers before the division, so they are exact up to one rounding.
+void dft_any(const std::complex<double> *x, uint64_t stride, uint64_t n, int sign,
+             std::complex<double> *out) {
+  auto root = [&](uint64_t m) {
+    const double a = sign * kTwoPi * static_cast<double>(m % n) / static_cast<double>(n);
+    return std::complex<double>(std::cos(a), std::sin(a));
+  };
+  if (n == 1) {
+    out[0] = x[0];
+    return;
+  }
+  uint64_t p = 2;
+  while (p * p <= n && n % p != 0)
+    ++p;
+  if (n % p != 0)
+    p = n; // prime
+  if (p == n) {
+    for (uint64_t k = 0; k < n; ++k) {
+      std::complex<double> acc(0., 0.);
+      for (uint64_t j = 0; j < n; ++j)
+        acc += x[j * stride] * root(j * k);
+      out[k] = acc;
+    }
+    return;
+  }
+  const uint64_t m = n / p;
+  std::vector<std::complex<double>> sub(n);
+  for (uint64_t r = 0; r < p; ++r)
+    dft_any(x + r * stride, stride * p, m, sign, sub.data() + r * m);
+  for (uint64_t k = 0; k < n; ++k) {
+    std::complex<double> acc = sub[k % m];
+    for (uint64_t r = 1; r < p; ++r)
+      acc += sub[r * m + k % m] * root(r * k);
+    out[k] = acc;
+  }
+}
+
+void fft_line(std::vector<double> &buf, uint64_t n, int sign) {
+  if ((n & (n - 1)) == 0) {
+    fft_line_pow2(buf, n, sign);
+    return;
+  }
+  std::vector<std::complex<double>> in(n), out(n);
+  for (uint64_t t = 0; t < n; ++t)
+    in[t] = std::complex<double>(buf[2 * t], buf[2 * t + 1]);
+  dft_any(in.data(), 1, n, sign, out.data());
+  for (uint64_t t = 0; t < n; ++t) {
+    buf[2 * t] = out[t].real();
+    buf[2 * t + 1] = out[t].imag();
   }
 }
 

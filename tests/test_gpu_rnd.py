@@ -54,7 +54,12 @@ def test_gauss_stream_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("shape", [(8, 4, 2), (16, 32, 8), (4, 4, 64), (64, 32, 1),
-                                   (2, 1024, 2), (4096, 2, 2)])
+                                   (2, 1024, 2), (4096, 2, 2),
+                                   # mixed radix 2/3/5/7 (the template's 800 and 160)
+                                   (12, 10, 6), (800, 3, 2), (3, 160, 5), (7, 9, 5),
+                                   (6561, 1, 2), (5000, 2, 1), (8192, 1, 2),
+                                   # Bluestein (prime factors above 7)
+                                   (13, 11, 22), (97, 2, 3), (4, 4093, 1), (2, 3, 1000 + 9)])
 @pytest.mark.parametrize("sign", [1, -1])
 def test_fft_matches_numpy(ctx, shape, sign):
     rng = np.random.default_rng(1)
@@ -320,8 +325,9 @@ def test_generators_on_the_workload_config(ctx):
 
 
 def test_generator_argument_errors(ctx):
-    d = es_desc((30, 32, 16))
-    b = [np.zeros(30 * 32 * 16) for _ in range(3)]
+    # 4099 is prime and above the Bluestein limit of 4096 points
+    d = es_desc((4099, 2, 2))
+    b = [np.zeros(4099 * 2 * 2) for _ in range(3)]
     with pytest.raises(HamxError) as e:
         ctx.brnd_es(d, 1, *b)
     assert e.value.code == capi.HAMX_ERR_UNSUPPORTED
@@ -329,3 +335,61 @@ def test_generator_argument_errors(ctx):
     b = [np.zeros(32 * 32 * 16) for _ in range(3)]
     with pytest.raises(HamxError):
         ctx.brnd_es(d, 1, *b)
+
+
+@pytest.mark.parametrize("shape", [(40, 30, 20), (28, 22, 10)])
+def test_generators_match_oracle_on_mixed_radix_grids(ctx, shape):
+    """ES (LSA, rho 1 and 0.5) and DFT on grids of mixed-radix (2, 3, 5) and
+    Bluestein (7 x 4, 2 x 11) axes: element-wise equal to the oracle driven by
+    the same Philox stream."""
+    print("case: generators on", shape, flush=True)
+    orc = load_oracle()
+    n = int(np.prod(shape))
+    for rho in (1.0, 0.5):
+        d = es_desc(shape, rho=rho, k0=0.3)
+        g = [np.zeros(n) for _ in range(3)]
+        r = [np.zeros(n) for _ in range(3)]
+        ctx.brnd_es(d, 3, *g)
+        assert orc.orc_brnd_es(C.byref(d), 3, *(ptr(a) for a in r)) == 0
+        for a, b in zip(g, r):
+            err = rel_l2(a, b)
+            print(f"brnd_es {shape} rho {rho} rel L2 {err:.3e}", flush=True)
+            assert err < 1e-11
+    d = dft_desc(shape, k0=0.3)
+    g, r = np.zeros(n), np.zeros(n)
+    ctx.ternd_dft(d, 4, g)
+    assert orc.orc_ternd_dft(C.byref(d), 4, ptr(r)) == 0
+    err = rel_l2(g, r)
+    print(f"ternd_dft {shape} rel L2 {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_template_boxes_generate_on_the_gpu(ctx):
+    """The reference template's own 800x800x160 box_brnd / box_ternd
+    (params_template.xml:46-86; 800 = 2^5 5^2, 160 = 2^5 5) generate on the
+    GPU: every value is finite, the DFT field over sqrt(profile) has the
+    requested rms within 1%, and an isotropic ES field with a flat profile
+    has the total rms sqrt(2) rms of brnd_es.cc:153-307 within 1%."""
+    import torch
+    shape = (800, 800, 160)
+    n = int(np.prod(shape))
+    print("case: template boxes 800x800x160", flush=True)
+    te = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ctx.ternd_dft(dft_desc(shape), 1, te)
+    assert bool(torch.isfinite(te).all())
+    x = torch.linspace(-20, 20, shape[0], dtype=torch.float64, device="cuda")
+    y = torch.linspace(-20, 20, shape[1], dtype=torch.float64, device="cuda")
+    z = torch.linspace(-4, 4, shape[2], dtype=torch.float64, device="cuda")
+    X, Y, Z = torch.meshgrid(x, y, z, indexing="ij")
+    prof = torch.exp(-(torch.sqrt(X ** 2 + Y ** 2) - 8.3) / 8) * torch.exp(-(Z.abs() - 0.006))
+    rms = float(torch.sqrt(((te.view(shape) / prof.sqrt()) ** 2).mean()))
+    print("dft 800x800x160 rms ratio", rms / 0.001, flush=True)
+    assert abs(rms / 0.001 - 1) < 0.01
+    del te, X, Y, Z, prof
+    b = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    ctx.brnd_es(es_desc(shape, capi.HAMX_BREG_CART, r0=1e30, z0=1e30, k0=0.5), 1, *b)
+    comp = [float(torch.sqrt((a ** 2).mean())) / 2e-6 for a in b]
+    total = math.sqrt(sum(c * c for c in comp))
+    print("es 800x800x160 component rms", comp, "total", total, flush=True)
+    assert all(bool(torch.isfinite(a).all()) for a in b)
+    assert abs(total / math.sqrt(2) - 1) < 0.01

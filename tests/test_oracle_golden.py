@@ -392,7 +392,9 @@ def test_gauss_pairs_are_standard_normal(orc):
     assert abs((g ** 4).mean() - 3) < 0.05
 
 
-@pytest.mark.parametrize("shape", [(8, 4, 2), (16, 32, 8), (4, 4, 64)])
+@pytest.mark.parametrize("shape", [(8, 4, 2), (16, 32, 8), (4, 4, 64),
+                                   (12, 10, 6), (7, 9, 5), (13, 11, 22), (97, 2, 3),
+                                   (50, 40, 20)])
 def test_fft_matches_numpy(orc, shape):
     rng = np.random.default_rng(5)
     a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
