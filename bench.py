@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_SAMPLE = 256
+# VALU issue peak of one MI355X: 256 CUs x 4 SIMDs, each issuing one wave64
+# vector instruction per 4 cycles (16 lanes per cycle) at 2.4 GHz
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+ROOFLINE_FILE = os.path.join(ROOT, "profiles", "los_fast_random_forward_roofline.txt")
 
 
 def parse():
@@ -54,6 +58,9 @@ def parse():
                     help="batch sample arithmetic (batch workload)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="batch workload: forward only (no map+gradient evaluations)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="random_forward: skip the 64-instance batched map+gradient "
+                         "measurement that the default run reports in its detail")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device three times (a plain copy "
                          "kernel of known size, to calibrate FETCH_SIZE / WRITE_SIZE)")
@@ -71,16 +78,19 @@ def copy_calibrate(mib):
     print(json.dumps({"copy_calibrate_bytes": n * 8}), flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per march launch from the committed rocprofv3 PMC summary
-    (profiles/los_fast_random_forward_pmc.txt), or None."""
-    path = os.path.join(ROOT, "profiles", "los_fast_random_forward_pmc.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+def roofline_inputs():
+    """Per-pass counter figures of the march from the committed rocprofv3
+    summary (key: value lines): VALU wave instructions per geometric sample,
+    HBM bytes per write_grid pass (FETCH_SIZE + WRITE_SIZE, calibrated) and
+    the L2 hit rate.  Missing keys are None."""
+    keys = ("valu_wave_insts_per_sample", "hbm_bytes_per_pass", "l2_hit_rate")
+    out = dict.fromkeys(keys)
+    if os.path.exists(ROOFLINE_FILE):
+        for line in open(ROOFLINE_FILE):
+            k, _, v = line.partition(":")
+            if k.strip() in out:
+                out[k.strip()] = float(v)
+    return out
 
 
 def cpu_model():
@@ -186,7 +196,16 @@ def progress(msg):
 
 
 def run_batch(args):
-    """The batch workloads.
+    """Print the bench line of the batch workloads (see measure_batch)."""
+    large = args.workload == "large"
+    n_inst = args.instances if args.instances else (16 if large else 64)
+    prec = "f32" if args.precision == "f32" else "f64"
+    print(json.dumps(measure_batch(large, n_inst, args.check, prec, not args.no_adjoint,
+                                   args.steps, args.warmup)), flush=True)
+
+
+def measure_batch(large, n_inst, check, precision, with_adj, steps, warmup):
+    """The batch workloads; returns the bench line as a dict.
 
     batch: instance i has random fields of seed i + 1 on the 256x256x64 box,
     top nside (64, 128, 256)[i % 3] over 3 shells, the first 1 + i % 4 of
@@ -210,12 +229,9 @@ def run_batch(args):
     from hammurabix_amd import Context, Param, _capi as capi, configs, los_samples
     from hammurabix_amd.autograd import desc_params
     dev = torch.device("cuda:0")
-    prec = capi.HAMX_PREC_F32 if args.precision == "f32" else capi.HAMX_PREC_F64
-    large = args.workload == "large"
-    n_inst = args.instances if args.instances else (16 if large else 64)
-    n_check = min(n_inst, args.check if args.check else (1 if large else n_inst))
+    prec = capi.HAMX_PREC_F32 if precision == "f32" else capi.HAMX_PREC_F64
+    n_check = min(n_inst, check if check else (1 if large else n_inst))
     make = configs.large_instance if large else configs.batch_instance
-    with_adj = not args.no_adjoint
     specs, pars = [], []
     with Context(0) as ctx:
         t0 = time.perf_counter()
@@ -254,15 +270,15 @@ def run_batch(args):
         if with_adj:
             likelihood_seeds(ctx, b, pars, specs, dev)
         info = b.info()
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             b.run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             b.run()
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - t0
-        progress(f"forward replays timed: {elapsed * 1e3 / args.steps:.1f} ms each")
+        progress(f"forward replays timed: {elapsed * 1e3 / steps:.1f} ms each")
         mg = {}
         if with_adj:
             # map+gradient evaluations: forward replay + reverse replay
@@ -270,11 +286,11 @@ def run_batch(args):
             b.run_adjoint()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for _ in range(args.steps):
+            for _ in range(steps):
                 b.run()
                 b.run_adjoint()
             torch.cuda.synchronize()
-            mg_s = (time.perf_counter() - t0) / args.steps
+            mg_s = (time.perf_counter() - t0) / steps
             progress(f"forward + reverse replays timed: {mg_s * 1e3:.1f} ms each")
             mg = dict(batch_map_gradient_ms=mg_s * 1e3,
                       batch_map_gradient_evals_per_s=n_inst / mg_s)
@@ -343,8 +359,8 @@ def run_batch(args):
                     err = float(torch.linalg.vector_norm(got - ref)) / nr if nr else 0.0
                     worst_grid = max(worst_grid, err)
         b.close()
-    ms = elapsed * 1e3 / args.steps
-    value = info["samples_ref"] * args.steps / elapsed
+    ms = elapsed * 1e3 / steps
+    value = info["samples_ref"] * steps / elapsed
     plain_rate = plain_samples / plain_s
     grid_bytes = 128 if prec == capi.HAMX_PREC_F32 else 256
     achieved = info["samples_geo"] * grid_bytes / (ms * 1e-3) / 1e9
@@ -354,8 +370,8 @@ def run_batch(args):
         "value": value,
         "unit": "samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
@@ -403,7 +419,7 @@ def run_batch(args):
             "realisation_generation_s": gen_s,
         },
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 CRE_GRID = dict(shape=(80, 80, 80), nE=80, E_GeV=(0.1, 1000.0))
@@ -606,8 +622,33 @@ def main():
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = S_ref * args.steps / elapsed
-    achieved = march_samples * BYTES_PER_SAMPLE / (march_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    # the roofline, per write_grid pass (3 k_los_fast launches, one per shell)
+    march_s_pass = march_ms * 1e-3 / args.steps
+    samples_pass = march_samples / args.steps
+    rf = roofline_inputs()
+    roofline = None
+    if rf["valu_wave_insts_per_sample"] is not None:
+        achieved = rf["valu_wave_insts_per_sample"] * samples_pass / march_s_pass / 1e9
+        hbm = rf["hbm_bytes_per_pass"]
+        roofline = {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": VALU_PEAK_GINST,
+            "unit": "Ginst/s",
+            "frac": achieved / VALU_PEAK_GINST,
+            "traffic": hbm,
+            "basis": "one write_grid pass = 3 k_los_fast launches; wave64 VALU instructions "
+                     "per pass (SQ_INSTS_VALU per geometric sample x samples) / live march "
+                     "time; traffic = FETCH_SIZE + WRITE_SIZE bytes per pass",
+            "hbm_GBps": hbm / march_s_pass / 1e9 if hbm else None,
+            "hbm_frac": hbm / march_s_pass / 1e9 / HBM_PEAK_GBS if hbm else None,
+            "l2_hit_rate": rf["l2_hit_rate"],
+            "algorithmic_GBps": samples_pass * BYTES_PER_SAMPLE / march_s_pass / 1e9,
+        }
+    batch = None
+    if not args.no_batch:
+        progress("batch map+gradient (64 instances)")
+        batch = measure_batch(False, 64, 0, "f32", True, max(1, min(args.steps, 5)), 1)
     out = {
         "metric": "LOS samples/s and map+gradient evals/s at nside 256, vs plain GPU at "
                   "≤1e-5 rel",
@@ -630,14 +671,7 @@ def main():
             "shells": [int(desc.nside_shell[i]) for i in range(desc.n_shell)],
             "grid": [g.nx, g.ny, g.nz],
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
+        "roofline": roofline,
         "detail": {
             "march_kernels": march_kernels,
             "march_ms_per_step": march_ms / args.steps,
@@ -652,6 +686,22 @@ def main():
             "gradients_finite": grad_finite,
         },
     }
+    if batch is not None:
+        bd = batch["detail"]
+        out["detail"]["batch"] = {
+            "workload": batch["config"]["workload"],
+            "precision": batch["dtype"],
+            "forward_ms": batch["ms_per_step"],
+            "forward_samples_per_s": batch["value"],
+            **{k: bd[k] for k in (
+                "batch_map_gradient_ms", "batch_map_gradient_evals_per_s",
+                "plain_map_gradient_evals_per_s", "map_gradient_speedup_vs_plain",
+                "speedup_vs_plain_pass_by_pass", "max_rel_l2_vs_plain_fp64",
+                "max_rel_err_by_param", "max_rel_err_dparams_vs_plain_fp64",
+                "max_rel_l2_dlnparams_vs_plain_fp64",
+                "max_rel_l2_grid_gradient_vs_plain_fp64") if k in bd},
+            "launch_shape": batch["config"]["launch_shape"],
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(configs.random_forward, grids_host,
                                            min(args.cpu_threads, os.cpu_count() or 1))
