@@ -30,9 +30,9 @@ _TEMPLATE = """<?xml version="1.0"?>
   </observable>
   <mask cue="{mask_on}" filename="{mask_file}" nside="{nside_mask}"/>
   <fieldio>
-    <breg read="0" write="0" filename="breg.bin"/>
+    <breg read="0" write="{breg_write}" filename="{breg_file}"/>
     <brnd read="{brnd_read}" write="0" filename="{brnd_file}"/>
-    <tereg read="0" write="0" filename="tereg.bin"/>
+    <tereg read="0" write="{tereg_write}" filename="{tereg_file}"/>
     <ternd read="{ternd_read}" write="0" filename="{ternd_file}"/>
     <cre read="{cre_read}" write="{cre_write}" filename="{cre_file}"/>
   </fieldio>
@@ -42,6 +42,18 @@ _TEMPLATE = """<?xml version="1.0"?>
       <y value="0"/>
       <z value="0.006"/>
     </observer>
+    <box_breg>
+      <nx value="{rg_nx}"/> <ny value="{rg_ny}"/> <nz value="{rg_nz}"/>
+      <x_min value="-20.0"/> <x_max value="20.0"/>
+      <y_min value="-20.0"/> <y_max value="20.0"/>
+      <z_min value="-4.0"/> <z_max value="4.0"/>
+    </box_breg>
+    <box_tereg>
+      <nx value="{rg_nx}"/> <ny value="{rg_ny}"/> <nz value="{rg_nz}"/>
+      <x_min value="-20.0"/> <x_max value="20.0"/>
+      <y_min value="-20.0"/> <y_max value="20.0"/>
+      <z_min value="-4.0"/> <z_max value="4.0"/>
+    </box_tereg>
     <box_brnd>
       <nx value="{nx}"/> <ny value="{ny}"/> <nz value="{nz}"/>
       <x_min value="-20.0"/> <x_max value="20.0"/>
@@ -163,7 +175,7 @@ _TEMPLATE = """<?xml version="1.0"?>
       </global>
     </random>
   </thermalelectron>
-  <cre cue="1" type="analytic">
+  <cre cue="1" type="{cre_type}">
     <analytic>
       <alpha value="3.0"/>
       <beta value="0.0"/>
@@ -173,6 +185,12 @@ _TEMPLATE = """<?xml version="1.0"?>
       <E0 value="20.6"/>
       <j0 value="0.0217"/>
     </analytic>
+    <unif>
+      <alpha value="3.0"/>
+      <r0 value="5.0"/>
+      <E0 value="20.6"/>
+      <j0 value="0.0217"/>
+    </unif>
   </cre>
 </root>
 """
@@ -194,12 +212,17 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         cre_file: str = "", cre_read: bool = False, cre_write: bool = False,
         cre_grid: tuple[int, int, int, int] = (40, 40, 8, 32),
         cre_E: tuple[float, float] = (0.1, 1000.0), tereg: str = "unif",
-        brnd_type: str = "global") -> str:
+        brnd_type: str = "global", breg_file: str = "", tereg_file: str = "",
+        regular_grid: tuple[int, int, int] = (80, 80, 16), cre_type: str = "analytic") -> str:
     """The parameter file; cre_read / cre_write with cre_file switch on the
     CRE grid I/O on box_cre (cre_grid = nx, ny, nz, nE; cre_E = E_min, E_max
     in GeV); tereg picks the regular thermal electron model ("unif" or
     "ymw16", the template's YMW16 values); brnd_type "local" switches the
-    random B to the MHD generator (k0 0.3 /kpc, the template's other values)."""
+    random B to the MHD generator (k0 0.3 /kpc, the template's other
+    values); breg_file / tereg_file switch on the export of the regular B /
+    TE grid on box_breg / box_tereg (regular_grid = nx, ny, nz of both
+    boxes); cre_type "unif" switches the CRE to the uniform sphere (alpha 3,
+    r0 5 kpc)."""
     freqs = [23.0] if freqs is None else freqs
     return _TEMPLATE.format(
         do_dm=int(do_dm), do_fd=int(do_fd), nside_out=nside_out,
@@ -212,7 +235,11 @@ def xml(*, shell_num: int, nside_sim: int, nside_out: int, oc_r_max: float = 30.
         mask_on=int(bool(mask_file)), mask_file=mask_file or "mask.bin", nside_mask=nside_mask,
         cre_read=int(cre_read), cre_write=int(cre_write), cre_file=cre_file or "cre.bin",
         cre_nx=cre_grid[0], cre_ny=cre_grid[1], cre_nz=cre_grid[2], cre_nE=cre_grid[3],
-        cre_E_min=cre_E[0], cre_E_max=cre_E[1], tereg=tereg, brnd_type=brnd_type)
+        cre_E_min=cre_E[0], cre_E_max=cre_E[1], tereg=tereg, brnd_type=brnd_type,
+        breg_write=int(bool(breg_file)), breg_file=breg_file or "breg.bin",
+        tereg_write=int(bool(tereg_file)), tereg_file=tereg_file or "tereg.bin",
+        rg_nx=regular_grid[0], rg_ny=regular_grid[1], rg_nz=regular_grid[2],
+        cre_type=cre_type)
 
 
 def plain_forward(**kw) -> str:

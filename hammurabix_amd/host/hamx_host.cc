@@ -1,5 +1,6 @@
 // Field plugins, grids, integrator and pipeline over the C-ABI.
 #include "hamx_host.h"
+#include "hamx_cgs.h"
 #include "hamx_models.h"
 
 #include <chrono>
@@ -8,6 +9,7 @@
 #include <functional>
 #include <stdexcept>
 #include <thread>
+#include <typeinfo>
 
 namespace hamx_host {
 
@@ -16,6 +18,8 @@ namespace {
 const double kKpc = 3.0856775806e+21;
 const double kPc = 3.0856775806e+18;
 const double kM = 1.e2;
+// Is `obj` exactly of the library class T (not a user class derived from it)?
+template <class T, class U> bool exactly(const U &obj) { return typeid(obj) == typeid(T); }
 
 // Throw the library's message as the reference's std::runtime_error.
 void check(int rc, hamx_ctx *ctx) {
@@ -281,7 +285,7 @@ int Breg::device_model(const Param *, double *) const { return -1; }
 Vec3 Breg::read_grid(const Vec3 &pos, const Param *par, const Grid_breg *grid) const {
   unsigned long long idx;
   double xd, yd, zd;
-  if (!grid || !host_cell(par->grid_breg, pos, idx, xd, yd, zd))
+  if (!grid || !grid->bx || !host_cell(par->grid_breg, pos, idx, xd, yd, zd))
     return Vec3();
   return Vec3(host_lerp(grid->bx.get(), par->grid_breg, idx, xd, yd, zd),
               host_lerp(grid->by.get(), par->grid_breg, idx, xd, yd, zd),
@@ -319,6 +323,8 @@ Vec3 Breg_lsa::write_field(const Vec3 &pos, const Param *par) const {
               b0 * std::sin(chi));
 }
 int Breg_lsa::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_lsa>(*this))
+    return -1;
   p[0] = par->breg_lsa.b0;
   p[1] = par->breg_lsa.psi0;
   p[2] = par->breg_lsa.psi1;
@@ -329,6 +335,8 @@ Vec3 Breg_cart::write_field(const Vec3 &, const Param *par) const {
   return Vec3(par->breg_cart.bx, par->breg_cart.by, par->breg_cart.bz);
 }
 int Breg_cart::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_cart>(*this))
+    return -1;
   p[0] = par->breg_cart.bx;
   p[1] = par->breg_cart.by;
   p[2] = par->breg_cart.bz;
@@ -339,6 +347,8 @@ Vec3 Breg_unif::write_field(const Vec3 &, const Param *par) const {
               par->breg_unif.bp * std::sin(par->breg_unif.l0), par->breg_unif.bv);
 }
 int Breg_unif::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_unif>(*this))
+    return -1;
   p[0] = par->breg_unif.bp;
   p[1] = par->breg_unif.bv;
   p[2] = par->breg_unif.l0;
@@ -346,54 +356,83 @@ int Breg_unif::device_model(const Param *par, double *p) const {
 }
 // breg_helix.cc:13-30 and breg_jf12.cc:8-189, through the evaluators the
 // device march uses (hamx_models.h)
-int Breg_helix::device_model(const Param *par, double *p) const {
+namespace {
+void helix_params(const Param *par, double *p) {
   p[0] = par->breg_helix.bx;
   p[1] = par->breg_helix.by;
   p[2] = par->breg_helix.bz;
   p[3] = par->breg_helix.r_min;
   p[4] = par->breg_helix.r_max;
+}
+} // namespace
+int Breg_helix::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_helix>(*this))
+    return -1;
+  helix_params(par, p);
   return HAMX_BREG_HELIX;
 }
 Vec3 Breg_helix::write_field(const Vec3 &pos, const Param *par) const {
   double p[HAMX_BREG_NPARAM] = {}, b[3];
-  device_model(par, p);
+  helix_params(par, p);
   hamx::breg_helix_eval(p, pos[0], pos[1], pos[2], b);
   return Vec3(b[0], b[1], b[2]);
 }
 int Breg_jf12::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_jf12>(*this))
+    return -1;
   for (int i = 0; i < 20; ++i)
     p[i] = par->breg_jf12[i];
   return HAMX_BREG_JF12;
 }
 Vec3 Breg_jf12::write_field(const Vec3 &pos, const Param *par) const {
   double p[HAMX_BREG_NPARAM] = {}, b[3];
-  device_model(par, p);
+  for (int i = 0; i < 20; ++i)
+    p[i] = par->breg_jf12[i];
   hamx::breg_jf12_eval(p, pos[0], pos[1], pos[2], b);
   return Vec3(b[0], b[1], b[2]);
 }
 // breg_jaffe.cc:11-243
 int Breg_jaffe::device_model(const Param *par, double *p) const {
+  if (!exactly<Breg_jaffe>(*this))
+    return -1;
   for (int i = 0; i < 33; ++i)
     p[i] = par->breg_jaffe[i];
   return HAMX_BREG_JAFFE;
 }
 Vec3 Breg_jaffe::write_field(const Vec3 &pos, const Param *par) const {
   double p[HAMX_BREG_NPARAM] = {}, b[3];
-  device_model(par, p);
+  for (int i = 0; i < 33; ++i)
+    p[i] = par->breg_jaffe[i];
   hamx::breg_jaffe_eval(p, pos[0], pos[1], pos[2], b);
   return Vec3(b[0], b[1], b[2]);
 }
 
+// brnd.cc:12-88: the random field is the trilinear gather of its grid
+Vec3 Brnd::read_field(const Vec3 &pos, const Param *par, const Grid_brnd *grid) const {
+  return read_grid(pos, par, grid);
+}
+Vec3 Brnd::read_grid(const Vec3 &pos, const Param *par, const Grid_brnd *grid) const {
+  unsigned long long idx;
+  double xd, yd, zd;
+  if (!grid || !grid->bx || !host_cell(par->grid_brnd, pos, idx, xd, yd, zd))
+    return Vec3();
+  return Vec3(host_lerp(grid->bx.get(), par->grid_brnd, idx, xd, yd, zd),
+              host_lerp(grid->by.get(), par->grid_brnd, idx, xd, yd, zd),
+              host_lerp(grid->bz.get(), par->grid_brnd, idx, xd, yd, zd));
+}
+
 double TEreg::write_field(const Vec3 &, const Param *) const { return 0.; }
 int TEreg::device_model(const Param *, double *) const { return -1; }
+double TEreg::read_grid(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const {
+  unsigned long long idx;
+  double xd, yd, zd;
+  if (!grid || !grid->te || !host_cell(par->grid_tereg, pos, idx, xd, yd, zd))
+    return 0.;
+  return host_lerp(grid->te.get(), par->grid_tereg, idx, xd, yd, zd);
+}
 double TEreg::read_field(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const {
-  if (par->grid_tereg.read_permission) {
-    unsigned long long idx;
-    double xd, yd, zd;
-    if (!grid || !host_cell(par->grid_tereg, pos, idx, xd, yd, zd))
-      return 0.;
-    return host_lerp(grid->te.get(), par->grid_tereg, idx, xd, yd, zd);
-  }
+  if (par->grid_tereg.read_permission)
+    return read_grid(pos, par, grid);
   if (par->grid_tereg.build_permission)
     return write_field(pos, par);
   return 0.;
@@ -407,6 +446,8 @@ double TEreg_unif::write_field(const Vec3 &pos, const Param *par) const {
   return length(d) > par->tereg_unif.r0 ? 0. : par->tereg_unif.n0;
 }
 int TEreg_unif::device_model(const Param *par, double *p) const {
+  if (!exactly<TEreg_unif>(*this))
+    return -1;
   p[0] = par->tereg_unif.n0;
   p[1] = par->tereg_unif.r0;
   return HAMX_TEREG_UNIF;
@@ -415,28 +456,226 @@ double TEreg_ymw16::write_field(const Vec3 &pos, const Param *par) const {
   return hamx::tereg_ymw16_eval(par->tereg_ymw16, pos[0], pos[1], pos[2]);
 }
 int TEreg_ymw16::device_model(const Param *par, double *p) const {
+  if (!exactly<TEreg_ymw16>(*this))
+    return -1;
   for (int i = 0; i < 73; ++i)
     p[i] = par->tereg_ymw16[i];
   return HAMX_TEREG_YMW16;
 }
 
+// ternd.cc:12-71
+double TErnd::read_field(const Vec3 &pos, const Param *par, const Grid_ternd *grid) const {
+  return read_grid(pos, par, grid);
+}
+double TErnd::read_grid(const Vec3 &pos, const Param *par, const Grid_ternd *grid) const {
+  unsigned long long idx;
+  double xd, yd, zd;
+  if (!grid || !grid->te || !host_cell(par->grid_ternd, pos, idx, xd, yd, zd))
+    return 0.;
+  return host_lerp(grid->te.get(), par->grid_ternd, idx, xd, yd, zd);
+}
+
+// ------------------------------------------------------------------ CRE --
 int CREfield::device_model(const Param *, double *) const { return -1; }
 int CRE_num::device_model(const Param *, double *) const { return HAMX_CRE_GRID; }
 
+double CREfield::write_field(const Vec3 &, const double &, const Param *) const { return 0.; }
+double CREfield::flux_norm(const Vec3 &, const Param *) const {
+  throw std::runtime_error("wrong inheritance");
+}
+double CREfield::flux_idx(const Vec3 &, const Param *) const {
+  throw std::runtime_error("wrong inheritance");
+}
+double CREfield::spatial_profile(const Vec3 &, const Param *) const {
+  throw std::runtime_error("wrong inheritance");
+}
+
+double CREfield::read_field(const Vec3 &pos, const double &E, const Param *par,
+                            const Grid_cre *grid) const {
+  if (par->grid_cre.read_permission)
+    return read_grid(pos, E, par, grid);
+  if (par->grid_cre.build_permission)
+    return write_field(pos, E, par);
+  return 0.;
+}
+
+namespace {
+// One grid axis of box_cre: the lower vertex and the fraction, or false on
+// and outside the faces (cre.cc:58-83).
+bool cre_axis(double pos, double lo, double hi, unsigned long long n, unsigned long long &l,
+              double &d) {
+  const double t = (n - 1) * (pos - lo) / (hi - lo);
+  if (t <= 0 || t >= n - 1)
+    return false;
+  l = static_cast<unsigned long long>(std::floor(t));
+  d = t - l;
+  return true;
+}
+// toolkit::index4d(n1, n2, n3, n4, e, i, j, l) = e n2 n3 n4 + i n3 n4 + j n4 + l
+unsigned long long index4d(unsigned long long n2, unsigned long long n3, unsigned long long n4,
+                           unsigned long long e, unsigned long long i, unsigned long long j,
+                           unsigned long long l) {
+  return e * n2 * n3 * n4 + i * n3 * n4 + j * n4 + l;
+}
+} // namespace
+
+double CREfield::read_grid(const Vec3 &pos, const double &E, const Param *par,
+                           const Grid_cre *grid) const {
+  const auto &g = par->grid_cre;
+  double t = std::log(E / g.E_min) / std::log(g.E_max / g.E_min);
+  if (t <= 0 || t >= g.nE - 1)
+    return 0.;
+  const auto El = static_cast<unsigned long long>(std::floor(t));
+  const double Ed = t - El;
+  unsigned long long xl, yl, zl;
+  double xd, yd, zd;
+  if (!cre_axis(pos[0], g.x_min, g.x_max, g.nx, xl, xd) ||
+      !cre_axis(pos[1], g.y_min, g.y_max, g.ny, yl, yd) ||
+      !cre_axis(pos[2], g.z_min, g.z_max, g.nz, zl, zd))
+    return 0.;
+  const double *f = grid->cre_flux.get();
+  // [x][y][z][E] flat index, with the argument order of the reference's
+  // first pair of reads kept as written there (cre.cc:84-89): nE leads the
+  // extents, so those two reads land elsewhere in the array
+  auto at = [&](unsigned long long x, unsigned long long y, unsigned long long z,
+                unsigned long long e) { return f[index4d(g.ny, g.nz, g.nE, x, y, z, e)]; };
+  auto at_first = [&](unsigned long long x, unsigned long long y, unsigned long long z,
+                      unsigned long long e) {
+    const unsigned long long i = index4d(g.nx, g.ny, g.nz, x, y, z, e);
+    return i < g.cre_size ? f[i] : 0.; // beyond the array when nx > nE
+  };
+  auto plane = [&](unsigned long long e, bool first) {
+    const double i1 = first ? at_first(xl, yl, zl, e) * (1. - zd) + at_first(xl, yl, zl + 1, e) * zd
+                            : at(xl, yl, zl, e) * (1. - zd) + at(xl, yl, zl + 1, e) * zd;
+    const double i2 = at(xl, yl + 1, zl, e) * (1 - zd) + at(xl, yl + 1, zl + 1, e) * zd;
+    const double j1 = at(xl + 1, yl, zl, e) * (1 - zd) + at(xl + 1, yl, zl + 1, e) * zd;
+    const double j2 = at(xl + 1, yl + 1, zl, e) * (1 - zd) + at(xl + 1, yl + 1, zl + 1, e) * zd;
+    const double w1 = i1 * (1 - yd) + i2 * yd;
+    const double w2 = j1 * (1 - yd) + j2 * yd;
+    return w1 * (1 - xd) + w2 * xd;
+  };
+  const double q1 = plane(El, true), q2 = plane(El + 1, false);
+  return q1 * (1 - Ed) + q2 * Ed;
+}
+
+double CREfield::read_grid_num(const Vec3 &pos, const ham_uint &Eidx, const Param *par,
+                               const Grid_cre *grid) const {
+  const auto &g = par->grid_cre;
+  unsigned long long xl, yl, zl;
+  double xd, yd, zd;
+  if (!cre_axis(pos[0], g.x_min, g.x_max, g.nx, xl, xd) ||
+      !cre_axis(pos[1], g.y_min, g.y_max, g.ny, yl, yd) ||
+      !cre_axis(pos[2], g.z_min, g.z_max, g.nz, zl, zd))
+    return 0.;
+  const double *f = grid->cre_flux.get();
+  auto at = [&](unsigned long long x, unsigned long long y, unsigned long long z) {
+    return f[index4d(g.ny, g.nz, g.nE, x, y, z, Eidx)];
+  };
+  const double i1 = at(xl, yl, zl) * (1. - zd) + at(xl, yl, zl + 1) * zd;
+  const double i2 = at(xl, yl + 1, zl) * (1 - zd) + at(xl, yl + 1, zl + 1) * zd;
+  const double j1 = at(xl + 1, yl, zl) * (1 - zd) + at(xl + 1, yl, zl + 1) * zd;
+  const double j2 = at(xl + 1, yl + 1, zl) * (1 - zd) + at(xl + 1, yl + 1, zl + 1) * zd;
+  const double w1 = i1 * (1 - yd) + i2 * yd;
+  const double w2 = j1 * (1 - yd) + j2 * yd;
+  return w1 * (1 - xd) + w2 * xd;
+}
+
 void CREfield::write_grid(const Param *par, Grid_cre *grid) const {
+  const auto &g = par->grid_cre;
+  if (!grid->cre_flux || g.cre_size == 0)
+    throw std::runtime_error("hamx: CRE grid output needs box_cre");
   hamx_los_desc d;
   std::memset(&d, 0, sizeof(d));
   for (int i = 0; i < 3; ++i)
     d.observer[i] = par->observer[i];
   d.cre_model = device_model(par, d.cre);
-  if (d.cre_model != HAMX_CRE_ANA && d.cre_model != HAMX_CRE_UNIF)
-    throw std::runtime_error("hamx: CRE grid output needs the analytic or uniform model");
-  const hamx_box b = to_box(par->grid_cre);
-  hamx_ctx *ctx = context();
-  check(hamx_cre_write_grid(ctx, &d, &b, par->grid_cre.E_min, par->grid_cre.E_fact,
-                            static_cast<uint32_t>(par->grid_cre.nE), grid->cre_flux.get(), 0),
-        ctx);
+  if (d.cre_model == HAMX_CRE_ANA || d.cre_model == HAMX_CRE_UNIF) {
+    // the library's models: every (vertex, energy) on the device
+    const hamx_box b = to_box(g);
+    hamx_ctx *ctx = context();
+    check(hamx_cre_write_grid(ctx, &d, &b, g.E_min, g.E_fact, static_cast<uint32_t>(g.nE),
+                              grid->cre_flux.get(), 0),
+          ctx);
+  } else {
+    // any other model: write_field at every vertex and energy (cre.cc:202-227)
+    double *out = grid->cre_flux.get();
+    sample_vertices(g, [&](const Vec3 &p, unsigned long long v) {
+      for (unsigned long long m = 0; m != g.nE; ++m)
+        out[v * g.nE + m] = write_field(p, g.E_min * std::exp(m * g.E_fact), par);
+    });
+  }
   grid->dev_valid = false;
+}
+
+// CRE_ana (cre_ana.cc:11-63)
+double CRE_ana::spatial_profile(const Vec3 &pos, const Param *par) const {
+  const double R0 = std::sqrt(par->observer[0] * par->observer[0] +
+                              par->observer[1] * par->observer[1]);
+  const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]);
+  const double ch = std::cosh(pos[2] / par->cre_ana.z0);
+  return std::exp((R0 - r) / par->cre_ana.r0) * (1. / (ch * ch));
+}
+double CRE_ana::flux_idx(const Vec3 &pos, const Param *par) const {
+  const double r = std::sqrt(pos[0] * pos[0] + pos[1] * pos[1]) / hamx::cgs::kpc;
+  const double z = std::fabs(pos[2]) / hamx::cgs::kpc;
+  return -par->cre_ana.alpha + par->cre_ana.beta * r + par->cre_ana.theta * z;
+}
+double CRE_ana::flux_norm(const Vec3 &pos, const Param *par) const {
+  namespace c = hamx::cgs;
+  const double gamma0 = par->cre_ana.E0 / c::mec2;
+  const double beta0 = std::sqrt(1. - 1. / gamma0);
+  const double unit = 4. * c::pi * c::mec / (c::GeV * c::m * c::m * c::sec * beta0);
+  const double norm = par->cre_ana.j0 * std::pow(gamma0, -flux_idx(par->observer, par));
+  return norm * unit * spatial_profile(pos, par);
+}
+double CRE_ana::write_field(const Vec3 &pos, const double &E, const Param *par) const {
+  namespace c = hamx::cgs;
+  const double gamma = E / c::mec2, gamma0 = par->cre_ana.E0 / c::mec2;
+  const double beta_ratio = std::sqrt((1. - 1. / gamma) / (1. - 1. / gamma0));
+  const double norm = par->cre_ana.j0 * std::pow(gamma0, -flux_idx(par->observer, par));
+  return norm * beta_ratio * std::pow(gamma, flux_idx(pos, par)) * spatial_profile(pos, par);
+}
+int CRE_ana::device_model(const Param *par, double *p) const {
+  if (!exactly<CRE_ana>(*this))
+    return -1;
+  p[0] = par->cre_ana.alpha;
+  p[1] = par->cre_ana.beta;
+  p[2] = par->cre_ana.theta;
+  p[3] = par->cre_ana.r0;
+  p[4] = par->cre_ana.z0;
+  p[5] = par->cre_ana.E0;
+  p[6] = par->cre_ana.j0;
+  return HAMX_CRE_ANA;
+}
+
+// CRE_unif (cre_unif.cc:10-59)
+double CRE_unif::spatial_profile(const Vec3 &pos, const Param *par) const {
+  return (pos - par->observer).length() > par->cre_unif.r0 ? 0. : 1.;
+}
+double CRE_unif::flux_idx(const Vec3 &, const Param *par) const { return -(par->cre_unif.alpha); }
+double CRE_unif::flux_norm(const Vec3 &pos, const Param *par) const {
+  namespace c = hamx::cgs;
+  const double gamma0 = par->cre_unif.E0 / c::mec2;
+  const double beta0 = std::sqrt(1. - 1. / gamma0);
+  const double unit = 4. * c::pi * c::mec / (c::GeV * c::m * c::m * c::sec * beta0);
+  const double norm = par->cre_unif.j0 * std::pow(gamma0, -flux_idx(par->observer, par));
+  return norm * unit * spatial_profile(pos, par);
+}
+double CRE_unif::write_field(const Vec3 &pos, const double &E, const Param *par) const {
+  namespace c = hamx::cgs;
+  const double gamma = E / c::mec2, gamma0 = par->cre_unif.E0 / c::mec2;
+  const double beta_ratio = std::sqrt((1. - 1. / gamma) / (1. - 1. / gamma0));
+  const double norm = par->cre_unif.j0 * std::pow(gamma0, -flux_idx(par->observer, par));
+  return norm * beta_ratio * std::pow(gamma, flux_idx(pos, par)) * spatial_profile(pos, par);
+}
+int CRE_unif::device_model(const Param *par, double *p) const {
+  if (!exactly<CRE_unif>(*this))
+    return -1;
+  p[0] = par->cre_unif.alpha;
+  p[1] = par->cre_unif.r0;
+  p[2] = par->cre_unif.E0;
+  p[3] = par->cre_unif.j0;
+  return HAMX_CRE_UNIF;
 }
 
 Grid_cre::Grid_cre(const Param *par) {
@@ -456,23 +695,6 @@ void Grid_cre::export_grid(const Param *par) {
 void Grid_cre::import_grid(const Param *par) {
   read_doubles(par->grid_cre.filename, {cre_flux.get()}, par->grid_cre.cre_size);
   dev_valid = false;
-}
-int CRE_ana::device_model(const Param *par, double *p) const {
-  p[0] = par->cre_ana.alpha;
-  p[1] = par->cre_ana.beta;
-  p[2] = par->cre_ana.theta;
-  p[3] = par->cre_ana.r0;
-  p[4] = par->cre_ana.z0;
-  p[5] = par->cre_ana.E0;
-  p[6] = par->cre_ana.j0;
-  return HAMX_CRE_ANA;
-}
-int CRE_unif::device_model(const Param *par, double *p) const {
-  p[0] = par->cre_unif.alpha;
-  p[1] = par->cre_unif.r0;
-  p[2] = par->cre_unif.E0;
-  p[3] = par->cre_unif.j0;
-  return HAMX_CRE_UNIF;
 }
 
 void Brnd::write_grid(const Param *, const Breg *, const Grid_breg *, Grid_brnd *) const {
@@ -650,10 +872,17 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
       if (par->grid_breg.full_size == 0)
         throw std::runtime_error("regular B model has no device evaluator and no box_breg "
                                  "grid to bake it on");
+      // what the model's read_field gives at the box_breg vertices
       baked_b = std::make_unique<Grid_breg>();
       baked_b->build_grid(par);
-      breg->write_grid(par, baked_b.get());
-      gbreg = baked_b.get();
+      Grid_breg *bb = baked_b.get();
+      sample_vertices(par->grid_breg, [&](const Vec3 &p, unsigned long long i) {
+        const Vec3 b = breg->read_field(p, par, gbreg);
+        bb->bx[i] = b[0];
+        bb->by[i] = b[1];
+        bb->bz[i] = b[2];
+      });
+      gbreg = bb;
       d.breg_model = HAMX_BREG_GRID;
     }
   }
@@ -675,8 +904,11 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
                                  "grid to bake it on");
       baked_t = std::make_unique<Grid_tereg>();
       baked_t->build_grid(par);
-      tereg->write_grid(par, baked_t.get());
-      gtereg = baked_t.get();
+      Grid_tereg *tb = baked_t.get();
+      sample_vertices(par->grid_tereg, [&](const Vec3 &p, unsigned long long i) {
+        tb->te[i] = tereg->read_field(p, par, gtereg);
+      });
+      gtereg = tb;
       d.tereg_model = HAMX_TEREG_GRID;
     }
   }
@@ -685,10 +917,21 @@ void Integrator::write_grid(const Breg *breg, const Brnd *, const TEreg *tereg,
     f.tereg_box = to_box(par->grid_tereg);
   }
   d.cre_model = cre ? cre->device_model(par, d.cre) : HAMX_CRE_NONE;
-  if (par->grid_cre.read_permission)
+  std::unique_ptr<Grid_cre> baked_c;
+  if (par->grid_cre.read_permission) {
     d.cre_model = HAMX_CRE_GRID; // integrator.cc:287,356
-  if (d.cre_model < 0)
-    throw std::runtime_error("hamx: CRE model has no device evaluator");
+  } else if (d.cre_model < 0) {
+    // a model without a device evaluator: its flux on box_cre
+    // (CREfield::write_grid, cre.cc:202-227), integrated as CRE_num
+    if (par->grid_cre.cre_size == 0 || par->grid_cre.nE < 2)
+      throw std::runtime_error("CRE model has no device evaluator and no box_cre grid to "
+                               "write it on");
+    baked_c = std::make_unique<Grid_cre>();
+    baked_c->build_grid(par);
+    cre->write_grid(par, baked_c.get());
+    gcre = baked_c.get();
+    d.cre_model = HAMX_CRE_GRID;
+  }
   d.schedule = HAMX_SCHED_AUTO;
 
   // random grids: uploaded once, kept on the device across passes

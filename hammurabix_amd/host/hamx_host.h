@@ -6,17 +6,27 @@
 // the std::runtime_error behaviour follow the reference, so a program
 // written against the reference's classes compiles against these.
 //
-// Field plugins: a model with a device evaluator (LSA, cartesian and
-// uniform regular B; uniform TE; analytic and uniform CRE) runs on the GPU.
-// A user-derived model overriding write_field() has none; the integrator
-// then bakes it onto its vertex grid (box_breg / box_tereg) with the
-// reference's own write_grid sampling and gathers it trilinearly on the
-// device -- an approximation of the analytic model at grid resolution.
+// Field plugins: the reference's virtual interface (bfield.h:44-97,
+// tefield.h:40-81, crefield.h:24-84) with its signatures over
+// Hamvec<3, ham_float>.  A library model with a device evaluator (LSA,
+// cartesian, uniform, helix, JF12 and Jaffe regular B; uniform and YMW16 TE;
+// analytic and uniform CRE) runs on the GPU.  Any other model -- a user class
+// derived from Breg / TEreg / CREfield, or from one of the library's models
+// -- has none: the integrator bakes what its read_field() returns at the
+// vertices of box_breg / box_tereg (write_field() by default, the reference's
+// write_grid sampling, breg.cc:99-121, tereg.cc:76-95), and a user CRE model
+// is written onto box_cre by CREfield::write_grid (cre.cc:202-227) and
+// integrated as an imported CRE grid (CRE_num, integrator.cc:281-401).  The
+// device then gathers those grids trilinearly: an approximation of the
+// user's model at grid resolution (and, for a CRE model, the energy
+// integral over the box_cre energies in place of the analytic power law).
 #pragma once
 
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <memory>
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -24,14 +34,121 @@
 
 namespace hamx_host {
 
+// ---------------------------------------------------------------- types --
+// the reference's scalar types (hamtype.h)
+typedef std::uint_fast32_t ham_uint;
+typedef std::int_fast32_t ham_int;
+typedef std::int_fast8_t ham_sint;
+typedef double ham_float;
+
 // -------------------------------------------------------------- vectors --
-struct Vec3 {
-  std::array<double, 3> e{{0., 0., 0.}};
-  Vec3() = default;
-  Vec3(double x, double y, double z) : e{{x, y, z}} {}
-  double operator[](int i) const { return e[i]; }
-  double &operator[](int i) { return e[i]; }
+// A small fixed-size vector with the member functions of the reference's
+// Hamvec (hamvec.h): element access, +, -, scalar * and /, comparisons,
+// length, lengthsq, flip, versor, dotprod, crossprod and stream output.
+template <ham_sint dim, typename T> class Hamvec {
+public:
+  Hamvec() { e.fill(T(0)); }
+  Hamvec(const T &x) {
+    e.fill(T(0));
+    e[0] = x;
+  }
+  Hamvec(const T &x, const T &y) {
+    e.fill(T(0));
+    e[0] = x;
+    if (dim > 1)
+      e[1] = y;
+  }
+  Hamvec(const T &x, const T &y, const T &z) {
+    e.fill(T(0));
+    e[0] = x;
+    if (dim > 1)
+      e[1] = y;
+    if (dim > 2)
+      e[2] = z;
+  }
+  T operator[](const ham_sint &i) const { return e[i]; }
+  T &operator[](const ham_sint &i) { return e[i]; }
+  Hamvec operator+(const Hamvec &v) const {
+    Hamvec r(*this);
+    return r += v;
+  }
+  Hamvec &operator+=(const Hamvec &v) {
+    for (int i = 0; i < dim; ++i)
+      e[i] += v.e[i];
+    return *this;
+  }
+  Hamvec operator-(const Hamvec &v) const {
+    Hamvec r(*this);
+    return r -= v;
+  }
+  Hamvec &operator-=(const Hamvec &v) {
+    for (int i = 0; i < dim; ++i)
+      e[i] -= v.e[i];
+    return *this;
+  }
+  Hamvec operator*(const T &s) const {
+    Hamvec r(*this);
+    return r *= s;
+  }
+  Hamvec &operator*=(const T &s) {
+    for (int i = 0; i < dim; ++i)
+      e[i] *= s;
+    return *this;
+  }
+  Hamvec operator/(const T &s) const {
+    Hamvec r(*this);
+    return r /= s;
+  }
+  Hamvec &operator/=(const T &s) {
+    for (int i = 0; i < dim; ++i)
+      e[i] /= s;
+    return *this;
+  }
+  bool operator==(const Hamvec &v) const { return e == v.e; }
+  bool operator!=(const Hamvec &v) const { return e != v.e; }
+  ham_float lengthsq() const {
+    ham_float s = 0;
+    for (int i = 0; i < dim; ++i)
+      s += static_cast<ham_float>(e[i]) * static_cast<ham_float>(e[i]);
+    return s;
+  }
+  ham_float length() const { return std::sqrt(lengthsq()); }
+  void flip() {
+    for (int i = 0; i < dim; ++i)
+      e[i] = -e[i];
+  }
+  Hamvec<dim, ham_float> versor() const {
+    Hamvec<dim, ham_float> r;
+    const ham_float l = length();
+    if (l == 0)
+      return r;
+    for (int i = 0; i < dim; ++i)
+      r[i] = static_cast<ham_float>(e[i]) / l;
+    return r;
+  }
+  ham_float dotprod(const Hamvec &v) const {
+    ham_float s = 0;
+    for (int i = 0; i < dim; ++i)
+      s += static_cast<ham_float>(e[i]) * static_cast<ham_float>(v.e[i]);
+    return s;
+  }
+  Hamvec crossprod(const Hamvec &v) const {
+    static_assert(dim == 3, "the cross product is defined in three dimensions");
+    return Hamvec(e[1] * v.e[2] - e[2] * v.e[1], e[2] * v.e[0] - e[0] * v.e[2],
+                  e[0] * v.e[1] - e[1] * v.e[0]);
+  }
+  friend std::ostream &operator<<(std::ostream &os, const Hamvec &v) {
+    os << "(";
+    for (int i = 0; i < dim; ++i)
+      os << (i ? "," : "") << v.e[i];
+    return os << ")";
+  }
+
+private:
+  std::array<T, static_cast<size_t>(dim)> e;
 };
+
+using Vec3 = Hamvec<3, ham_float>;
 
 // ---------------------------------------------------------------- Param --
 class Param {
@@ -192,17 +309,25 @@ public:
 };
 
 // --------------------------------------------------------- field plugins --
+// Every class below keeps the reference's virtual members.  device_model()
+// is this build's addition: the HAMX_* model of the library's own classes
+// (and its parameters), -1 for anything else -- a user class, including one
+// derived from a library model, so that what the user class computes is
+// what gets integrated.
 class Breg {
 public:
+  Breg() = default;
+  Breg(const Breg &) = delete;
+  Breg &operator=(const Breg &) = delete;
   virtual ~Breg() = default;
+  // breg.cc:11-21: the grid when imported, else write_field when built
+  virtual Vec3 read_field(const Vec3 &pos, const Param *par, const Grid_breg *grid) const;
   // field at a galactocentric position [cm] in Gauss; base class: zero
   virtual Vec3 write_field(const Vec3 &pos, const Param *par) const;
-  // dispatch of breg.cc:11-21 (host evaluation)
-  Vec3 read_field(const Vec3 &pos, const Param *par, const Grid_breg *grid) const;
-  Vec3 read_grid(const Vec3 &pos, const Param *par, const Grid_breg *grid) const;
+  // trilinear gather of the box_breg grid (breg.cc:23-97)
+  virtual Vec3 read_grid(const Vec3 &pos, const Param *par, const Grid_breg *grid) const;
   // sample write_field on the box_breg vertices (breg.cc:99-121)
-  void write_grid(const Param *par, Grid_breg *grid) const;
-  // device evaluator: HAMX_BREG_* and its parameters, or -1 (none)
+  virtual void write_grid(const Param *par, Grid_breg *grid) const;
   virtual int device_model(const Param *par, double p[HAMX_BREG_NPARAM]) const;
 };
 class Breg_lsa : public Breg {
@@ -238,7 +363,15 @@ public:
 
 class Brnd {
 public:
+  Brnd() = default;
+  Brnd(const Brnd &) = delete;
+  Brnd &operator=(const Brnd &) = delete;
   virtual ~Brnd() = default;
+  // brnd.cc:12-19: the random field is read from its grid
+  virtual Vec3 read_field(const Vec3 &pos, const Param *par, const Grid_brnd *grid) const;
+  // trilinear gather of the box_brnd grid (brnd.cc:21-88): zero on and
+  // outside the faces, z then y then x
+  virtual Vec3 read_grid(const Vec3 &pos, const Param *par, const Grid_brnd *grid) const;
   virtual void write_grid(const Param *, const Breg *, const Grid_breg *, Grid_brnd *) const;
 };
 class Brnd_es : public Brnd {
@@ -264,11 +397,18 @@ protected:
 
 class TEreg {
 public:
+  TEreg() = default;
+  TEreg(const TEreg &) = delete;
+  TEreg &operator=(const TEreg &) = delete;
   virtual ~TEreg() = default;
+  // tereg.cc:11-20: the grid when imported, else write_field when built
+  virtual double read_field(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const;
+  // density at a galactocentric position [cm] in cm^-3; base class: zero
   virtual double write_field(const Vec3 &pos, const Param *par) const;
-  double read_field(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const;
-  void write_grid(const Param *par, Grid_tereg *grid) const;
-  // device evaluator: HAMX_TEREG_* and its HAMX_TEREG_NPARAM parameters, or -1
+  // trilinear gather of the box_tereg grid (tereg.cc:22-74)
+  virtual double read_grid(const Vec3 &pos, const Param *par, const Grid_tereg *grid) const;
+  // sample write_field on the box_tereg vertices (tereg.cc:76-95)
+  virtual void write_grid(const Param *par, Grid_tereg *grid) const;
   virtual int device_model(const Param *par, double *p) const;
 };
 class TEreg_unif : public TEreg {
@@ -285,7 +425,14 @@ public:
 
 class TErnd {
 public:
+  TErnd() = default;
+  TErnd(const TErnd &) = delete;
+  TErnd &operator=(const TErnd &) = delete;
   virtual ~TErnd() = default;
+  // ternd.cc:12-19: read from the grid
+  virtual double read_field(const Vec3 &pos, const Param *par, const Grid_ternd *grid) const;
+  // trilinear gather of the box_ternd grid (ternd.cc:21-71)
+  virtual double read_grid(const Vec3 &pos, const Param *par, const Grid_ternd *grid) const;
   virtual void write_grid(const Param *, const TEreg *, const Grid_tereg *, Grid_ternd *) const;
 };
 class TErnd_dft : public TErnd {
@@ -296,23 +443,54 @@ public:
 
 class CREfield {
 public:
+  CREfield() = default;
+  CREfield(const CREfield &) = delete;
+  CREfield &operator=(const CREfield &) = delete;
   virtual ~CREfield() = default;
-  // device evaluator: HAMX_CRE_* and its parameters, or -1 (none)
-  virtual int device_model(const Param *par, double p[8]) const;
-  // CREfield::write_grid (cre.cc:204-225) on the device
+  // flux at position and energy E [erg] from the box_cre grid, linear in
+  // log E and trilinear in space (cre.cc:49-137) [GeV m^2 s sr]^-1
+  virtual double read_grid(const Vec3 &pos, const double &E, const Param *par,
+                           const Grid_cre *grid) const;
+  // flux at energy index Eidx of the grid, trilinear in space (cre.cc:139-199)
+  double read_grid_num(const Vec3 &pos, const ham_uint &Eidx, const Param *par,
+                       const Grid_cre *grid) const;
+  // CREfield::write_grid (cre.cc:202-227): write_field on the box_cre
+  // vertices and energies (on the device for the library's analytic and
+  // uniform models)
   virtual void write_grid(const Param *par, Grid_cre *grid) const;
+  // cre.cc:12-22: the grid when imported, else write_field when built
+  virtual double read_field(const Vec3 &pos, const double &E, const Param *par,
+                            const Grid_cre *grid) const;
+  // the flux PHI(E) [GeV m^2 s sr]^-1 at a position; base class: zero
+  virtual double write_field(const Vec3 &pos, const double &E, const Param *par) const;
+  // N(gamma) = flux_norm * gamma^flux_idx; the base class throws
+  // (cre.cc:28-44), as the reference's does
+  virtual double flux_norm(const Vec3 &pos, const Param *par) const;
+  virtual double flux_idx(const Vec3 &pos, const Param *par) const;
+  virtual double spatial_profile(const Vec3 &pos, const Param *par) const;
+  virtual int device_model(const Param *par, double p[8]) const;
 };
 // CRE_num (crefield.h:132): the imported grid
 class CRE_num : public CREfield {
 public:
   int device_model(const Param *par, double p[8]) const override;
 };
+// CRE_ana (cre_ana.cc:11-63)
 class CRE_ana : public CREfield {
 public:
+  double write_field(const Vec3 &pos, const double &E, const Param *par) const override;
+  double flux_norm(const Vec3 &pos, const Param *par) const override;
+  double flux_idx(const Vec3 &pos, const Param *par) const override;
+  double spatial_profile(const Vec3 &pos, const Param *par) const override;
   int device_model(const Param *par, double p[8]) const override;
 };
+// CRE_unif (cre_unif.cc:10-59)
 class CRE_unif : public CREfield {
 public:
+  double write_field(const Vec3 &pos, const double &E, const Param *par) const override;
+  double flux_norm(const Vec3 &pos, const Param *par) const override;
+  double flux_idx(const Vec3 &pos, const Param *par) const override;
+  double spatial_profile(const Vec3 &pos, const Param *par) const override;
   int device_model(const Param *par, double p[8]) const override;
 };
 
@@ -331,16 +509,23 @@ public:
 };
 
 // ------------------------------------------------------------- pipeline --
+// pipeline.h:14-45: a user model is added the reference's way, by a
+// Pipeline subclass whose assemble_* installs it (the members are public
+// here, protected there, so a derived class reaches them either way).
 class Pipeline {
 public:
+  Pipeline() = default;
   explicit Pipeline(const std::string &xml_file);
-  void assemble_grid();
-  void assemble_tereg();
-  void assemble_breg();
-  void assemble_ternd();
-  void assemble_brnd();
-  void assemble_cre();
-  void assemble_obs();
+  Pipeline(const Pipeline &) = delete;
+  Pipeline &operator=(const Pipeline &) = delete;
+  virtual ~Pipeline() = default;
+  virtual void assemble_grid();
+  virtual void assemble_tereg();
+  virtual void assemble_breg();
+  virtual void assemble_ternd();
+  virtual void assemble_brnd();
+  virtual void assemble_cre();
+  virtual void assemble_obs();
   std::unique_ptr<Param> par;
   std::unique_ptr<Grid_breg> grid_breg;
   std::unique_ptr<Grid_brnd> grid_brnd;

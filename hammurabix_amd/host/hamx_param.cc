@@ -44,6 +44,11 @@ void read_box(const Element *root, const char *name, Param::param_grid &g) {
   g.z_min = units::kpc * fval(b, "z_min");
 }
 
+bool has_box(const Element *root, const char *name) {
+  const Element *g = root->child("grid");
+  return g && g->child(name);
+}
+
 void read_fieldio(const Element *root, const char *name, Param::param_grid &g) {
   const Element *fio = trace(root, {"fieldio"});
   if (const Element *e = fio->child(name)) {
@@ -227,7 +232,9 @@ Param::Param(const std::string &xml_file) {
       throw std::runtime_error("unsupported breg model");
     }
   }
-  if (grid_breg.read_permission || grid_breg.write_permission)
+  // also without field I/O when the file has the box: a model without a
+  // device evaluator is baked on it
+  if (grid_breg.read_permission || grid_breg.write_permission || has_box(root, "box_breg"))
     read_box(root, "box_breg", grid_breg);
 
   // random magnetic field (param.cc:345-460)
@@ -361,7 +368,7 @@ Param::Param(const std::string &xml_file) {
       throw std::runtime_error("unsupported tereg model");
     }
   }
-  if (grid_tereg.read_permission || grid_tereg.write_permission)
+  if (grid_tereg.read_permission || grid_tereg.write_permission || has_box(root, "box_tereg"))
     read_box(root, "box_tereg", grid_tereg);
 
   // random thermal electrons (param.cc:669-716); "dgm" names the DFT method
@@ -415,7 +422,7 @@ Param::Param(const std::string &xml_file) {
       throw std::runtime_error("unsupported cre model");
     }
   }
-  if (grid_cre.read_permission || grid_cre.write_permission) {
+  if (grid_cre.read_permission || grid_cre.write_permission || has_box(root, "box_cre")) {
     const Element *b = trace(root, {"grid", "box_cre"});
     grid_cre.E_min = units::GeV * fval(b, "E_min");
     grid_cre.E_max = units::GeV * fval(b, "E_max");
