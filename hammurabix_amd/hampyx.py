@@ -15,6 +15,12 @@ The reference launches the ``hamx`` executable as a subprocess and reads the
 maps back from temporary files; here the parameter tree is handed to the
 device pipeline in memory and the maps come back as numpy arrays.
 ``exe_path`` is accepted for call compatibility and not used.
+
+``differentiable()`` (no counterpart in the reference) runs the same passes
+through the adjoint-backed ``torch.autograd.Function`` of ``autograd.py``
+and returns the ``sim_map`` entries as float64 device tensors: a backward
+pass through any scalar of them reaches the field parameters and the random
+grids across every ``<sync>`` pass.
 """
 from __future__ import annotations
 
@@ -24,6 +30,7 @@ from typing import Optional
 
 from .pipeline import Pipeline
 from .runtime import Context
+from .units import CGS
 
 
 class Hampyx:
@@ -62,6 +69,68 @@ class Hampyx:
             for k, v in maps.items():
                 print(k, v.shape)
         self.sim_map.update(maps)
+
+    def differentiable(self, params=None, grids_grad: bool = False):
+        """The pipeline's passes (pipeline.cc:165-190: the last-listed <sync>
+        entry first with DM and FD, the others sync only) as differentiable
+        maps.  Returns (maps, params, grids): maps has the sim_map keys and
+        units (DM in pc cm^-3, FD in rad m^-2) as float64 device tensors;
+        params is the float64 tensor of (b0, psi0, psi1, chi0, alpha, n0) in
+        the descriptor's CGS / radian units that every pass reads (created with
+        requires_grad from the parameter file when not given); grids is
+        (bx, by, bz, te) of the random fields on the device, requiring grad
+        when grids_grad.  Models: LSA regular B, uniform TE, analytic or
+        uniform CRE with a constant index, random grids on one box."""
+        from .autograd import desc_params, los_maps
+        text = et.tostring(self.tree.getroot(), encoding="unicode")
+        if self._ctx is None:
+            self._ctx = Context(0)  # kept open: the backward pass runs on it
+        ctx = self._ctx
+        pipe = Pipeline(text=text, ctx=ctx, export=False)
+        pipe.assemble_grid()
+        pipe.assemble_tereg()
+        pipe.assemble_breg()
+        pipe.assemble_ternd()
+        pipe.assemble_brnd()
+        pipe.assemble_cre()
+        pipe.load_mask()
+        if getattr(pipe, "mask", None) is not None or getattr(pipe, "cre_grid", None) is not None:
+            raise ValueError("differentiable(): masks and CRE grids have no adjoint")
+        g = pipe.par.grid_obs
+        if not g.do_sync:
+            raise ValueError("differentiable(): the adjoint needs a <sync> output")
+        d0 = pipe.par.los_desc()
+        if params is None:
+            params = desc_params(d0).requires_grad_(True)
+        brnd = pipe.brnd
+        te = pipe.ternd
+        if grids_grad:
+            brnd = tuple(a.detach().clone().requires_grad_(True) for a in brnd) if brnd else None
+            te = te.detach().clone().requires_grad_(True) if te is not None else None
+        kw = dict(brnd=brnd, brnd_box=pipe.par.grid_brnd.box() if brnd else None,
+                  ternd=te, ternd_box=pipe.par.grid_ternd.box() if te is not None else None)
+        out = {}
+        n = len(g.do_sync)
+        for p in range(n):
+            fi = n - 1 - p  # the pass's sync entry
+            d = pipe.par.los_desc()
+            d.sync_freq = g.sim_sync_freq[fi]
+            d.nside_sync = g.nside_sync[fi]
+            d.do_sync = 1 if g.do_sync[fi] else 0
+            if p > 0:
+                d.do_dm = d.do_fd = 0
+            maps = los_maps(ctx, d, params, **kw)
+            if d.do_sync:
+                key = ("sync", g.sim_sync_freq_str[fi], str(d.nside_sync))
+                for comp, k in zip("IQU", ("is", "qs", "us")):
+                    out[key + (comp,)] = maps[k]
+            if d.do_dm:
+                out[("dm", "nan", str(d.nside_dm), "nan")] = maps["dm"] * (CGS.ccm / CGS.pc)
+            if d.do_fd:
+                out[("fd", "nan", str(d.nside_fd), "nan")] = maps["fd"] * (CGS.m * CGS.m)
+        self.sim_map.update({k: v.detach().cpu().numpy() for k, v in out.items()})
+        grids = (brnd[0], brnd[1], brnd[2], te) if brnd else (None, None, None, te)
+        return out, params, grids
 
     # ------------------------------------------------------ XML editing
     @staticmethod
