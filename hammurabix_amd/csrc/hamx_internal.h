@@ -95,6 +95,7 @@ enum Slot : int {
   kSlotCreTab,    // energy tables of the CRE grid emissivities
   kSlotSyncIO,    // hamx_sync_kernels staging (x, F, G)
   kSlotCreOut,    // hamx_cre_write_grid output staging
+  kSlotFlag,      // hamx_los_forward's non-finite output flag
   kSlotCount
 };
 static_assert(kSlotCount <= hamx_ctx::kSlots, "workspace slots");

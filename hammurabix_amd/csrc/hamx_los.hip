@@ -29,6 +29,7 @@
 // restatement to rounding.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -836,6 +837,29 @@ extern "C" int hamx_los_samples(const hamx_los_desc *d, uint64_t *samples) {
 }
 
 namespace hamx {
+namespace {
+// A NaN or an infinity in an input grid or parameter reaches the shell maps
+// (the reference's debug build asserts isfinite on B_par, B_per, te and the
+// phase, integrator.cc:192,195,202,223; its release build carries the NaN):
+// flag[0] becomes 1 when any of the given values is not finite.
+struct FiniteArgs {
+  const double *m[5];
+  uint64_t n[5];
+  int k;
+};
+
+__global__ __launch_bounds__(256) void k_nonfinite(FiniteArgs a, double *flag) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  bool bad = false;
+  for (int j = 0; j < a.k; ++j)
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+         i < a.n[j]; i += stride)
+      bad |= !isfinite(a.m[j][i]);
+  if (bad)
+    flag[0] = 1.;
+}
+} // namespace
+
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
                      hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd,
                      Arena *ar) {
@@ -929,6 +953,9 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     HAMX_TRY(scratch_get(ctx, ar, kSlotMaskShell, tb, &q));
     mask_shell = static_cast<double *>(q);
   }
+  HAMX_TRY(scratch_get(ctx, ar, kSlotFlag, sizeof(double), &q));
+  double *nonfinite = static_cast<double *>(q);
+  HAMX_TRY(fill_f64(ctx, nonfinite, 1, 0.));
   // every buffer is in place: zero the output maps
   if (o_dm)
     HAMX_TRY(fill_f64(ctx, o_dm, h_dm.npix, 0.));
@@ -1052,6 +1079,24 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     prof_end(ctx, pm,
              fast ? "los_fast" : (sched == HAMX_SCHED_WAVE ? "los_wave" : "los_thread"), s,
              npix * sa.step);
+    {
+      // every ray's observables finite (checked here: the HEALPix
+      // interpolation of the accumulation skips a NaN as an undefined pixel)
+      FiniteArgs fa;
+      std::memset(&fa, 0, sizeof(fa));
+      for (const double *m : {static_cast<const double *>(sa.t_dm),
+                              static_cast<const double *>(sa.t_fd),
+                              static_cast<const double *>(sa.t_is),
+                              static_cast<const double *>(sa.t_qs),
+                              static_cast<const double *>(sa.t_us)})
+        if (m) {
+          fa.m[fa.k] = m;
+          fa.n[fa.k++] = npix;
+        }
+      const unsigned fb = static_cast<unsigned>(std::min<uint64_t>((npix + 255) / 256, 1024));
+      hipLaunchKernelGGL(k_nonfinite, dim3(fb), dim3(256), 0, ctx->stream, fa, nonfinite);
+      HAMX_HIP(ctx, hipGetLastError());
+    }
     // accumulate shell maps into the outputs (integrator.cc:122-132)
     auto acc = [&](AccPairs &pairs, const Healpix &hd) -> int {
       if (pairs.n == 0)
@@ -1099,6 +1144,11 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
     }
   }
 
+  // ---- the shells' non-finite flag
+  double host_flag = 0.;
+  HAMX_HIP(ctx, hipMemcpyAsync(&host_flag, nonfinite, sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+
   // ---- results to the caller
   auto copy_out = [&](double *user, const double *dev, const Healpix &h) -> int {
     if (!user || out_dev)
@@ -1117,6 +1167,10 @@ int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *f
   if (d->do_fd || d->do_sync)
     HAMX_TRY(copy_out(out->fd, o_fd, h_fd));
   HAMX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (host_flag != 0.)
+    return set_err(ctx, HAMX_ERR_ARG,
+                   "non-finite values on the lines of sight: an input grid or parameter "
+                   "holds a NaN or an infinity");
   return HAMX_OK;
 }
 } // namespace hamx
@@ -1133,15 +1187,16 @@ extern "C" int hamx_los_forward(hamx_ctx *ctx, const hamx_los_desc *d,
 namespace hamx {
 namespace {
 
-// The scratch of hamx_los_forward_static: the five shell maps and the mask at
-// the shell's nside, for shells up to nside HAMX_STATIC_MAX_NSIDE.  Every GPU
-// the library runs on holds its own copy until the process ends.  In host
-// code `scratch` names a host-side stand-in, not device memory: kernels and
-// copies take the device address looked up by static_scratch_init.
+// The scratch of hamx_los_forward_static: the five shell maps, the mask at
+// the shell's nside (for shells up to nside HAMX_STATIC_MAX_NSIDE) and the
+// non-finite output flag.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not device memory: kernels and copies take the device address looked up by
+// static_scratch_init.
 static_assert(HAMX_STATIC_SCRATCH_BYTES >=
                   6 * (12ull * HAMX_STATIC_MAX_NSIDE * HAMX_STATIC_MAX_NSIDE * sizeof(double) +
-                       256),
-              "static scratch below six maps at HAMX_STATIC_MAX_NSIDE");
+                       256) + 512,
+              "static scratch below six maps at HAMX_STATIC_MAX_NSIDE and the flag");
 static __device__ __align__(256) double scratch[HAMX_STATIC_SCRATCH_BYTES / sizeof(double)];
 
 constexpr int kMaxDevices = 64;

@@ -273,3 +273,27 @@ def test_masked_pass_matches_oracle(ctx):
     assert_parity(m, ref, label="masked")
     full = write_grid(d, **kw)
     assert any(rel_l2(ref[k], full[k]) > 1e-3 for k in ref)  # the mask matters
+
+
+@pytest.mark.parametrize("where", ["brnd", "ternd"])
+def test_nan_in_a_grid_is_an_error(ctx, where):
+    """A NaN in an input grid (the reference's debug build asserts isfinite on
+    B_par, B_per, te and the phase, integrator.cc:192,195,202,223) comes back
+    as an error naming the non-finite values; the context stays usable."""
+    par = Param(text=configs.random_forward(nside_sim=4, nside_out=8, oc_r_res=0.1,
+                                            grid=(16, 16, 8)))
+    d = par.los_desc()
+    b, te, box = random_grids((16, 16, 8), seed=21)
+    bad_b = tuple(a.copy() for a in b)
+    bad_te = te.copy()
+    # a vertex near the observer, inside the box, that the rays sample
+    v = (5 * 16 + 8) * 8 + 4
+    if where == "brnd":
+        bad_b[1][v] = np.nan
+    else:
+        bad_te[v] = np.nan
+    with pytest.raises(HamxError) as e:
+        gpu_maps(ctx, d, brnd=bad_b, brnd_box=box, ternd=bad_te, ternd_box=box)
+    assert e.value.code == capi.HAMX_ERR_ARG and "non-finite" in str(e.value)
+    assert_parity(gpu_maps(ctx, d, brnd=b, brnd_box=box, ternd=te, ternd_box=box),
+                  write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=box), label="after-nan")
