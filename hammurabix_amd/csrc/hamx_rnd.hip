@@ -224,11 +224,15 @@ struct EsArgs {
 __device__ inline double es_spectrum(const EsArgs &a, double k) {
   const double p0 = a.rms * a.rms;
   const double unit = 1. / (4. * cgs::pi * k * k);
-  const double band1 = static_cast<double>(k < a.k1);
-  const double band2 = static_cast<double>(k > a.k1) * static_cast<double>(k < a.k0);
-  const double band3 = static_cast<double>(k > a.k0);
-  const double P = band1 * pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.) +
-                   band2 / pow(k / a.k0, a.a1) + band3 / pow(k / a.k0, a.a0);
+  // the reference's sum of band-masked terms, with each power evaluated only
+  // where its band is on (a masked-off term adds an exact zero)
+  double P = 0.;
+  if (k < a.k1)
+    P += pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.);
+  if (k > a.k1 && k < a.k0)
+    P += 1. / pow(k / a.k0, a.a1);
+  if (k > a.k0)
+    P += 1. / pow(k / a.k0, a.a0);
   return P * p0 * unit;
 }
 
@@ -291,7 +295,11 @@ __global__ __launch_bounds__(256) void k_es_draw(double2 *c0, double2 *c1,
   }
 }
 
-// STEP II (brnd_es.cc:157-205)
+// STEP II (brnd_es.cc:157-205).  HEAVY compiles in the JF12 energy profile,
+// the JF12 / Jaffe regular fields and the imported regular-field grid; the
+// light kernel (LSA, cartesian, uniform, helix fields and the ES profile)
+// then keeps few enough registers for several waves per SIMD.
+template <bool HEAVY>
 __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
                                                     EsArgs a, const double *var) {
   const double b_var_invsq = 1. / sqrt(3. * var[0]);
@@ -305,13 +313,14 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
     const uint64_t j = (idx / a.b.nz) % a.b.ny;
     const uint64_t l = idx % a.b.nz;
     const Vec3 pos = vertex_pos(a.b, i, j, l);
-    const double prof = a.jf12 ? jf12_profile(a.jp, pos) : profile(a.obs, a.r0, a.z0, pos);
+    const double prof =
+        (HEAVY && a.jf12) ? jf12_profile(a.jp, pos) : profile(a.obs, a.r0, a.z0, pos);
     const double ratio = sqrt(prof) * a.rms * b_var_invsq;
     const double2 v0 = c0[idx], v1 = c1[idx];
     Vec3 b{v0.x * ratio, v0.y * ratio, v1.y * ratio};
     // anisotropy direction: versor of the regular field (brnd_es.cc:19-22)
     Vec3 braw;
-    if (a.breg_model == HAMX_BREG_GRID) {
+    if (HEAVY && a.breg_model == HAMX_BREG_GRID) {
       Cell c;
       if (grid_cell(a.g_breg, pos, c)) {
         const uint64_t sy = a.g_breg.nz, sx = a.g_breg.ny * a.g_breg.nz;
@@ -321,7 +330,7 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
         braw = Vec3{0., 0., 0.};
       }
     } else {
-      braw = breg_eval(a.breg_model, a.breg, pos);
+      braw = breg_eval<HEAVY>(a.breg_model, a.breg, pos);
     }
     Vec3 H{0., 0., 0.};
     double l2 = 0.;
@@ -595,8 +604,14 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
   HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
   HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
   HAMX_TRY(variance_dev(ctx, reinterpret_cast<const double *>(c0), n, red));
-  hipLaunchKernelGGL(k_es_rescale, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a,
-                     static_cast<const double *>(red + 1));
+  const bool heavy = a.jf12 || a.breg_model == HAMX_BREG_GRID ||
+                     a.breg_model == HAMX_BREG_JF12 || a.breg_model == HAMX_BREG_JAFFE;
+  if (heavy)
+    hipLaunchKernelGGL(k_es_rescale<true>, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a,
+                       static_cast<const double *>(red + 1));
+  else
+    hipLaunchKernelGGL(k_es_rescale<false>, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1,
+                       a, static_cast<const double *>(red + 1));
   HAMX_HIP(ctx, hipGetLastError());
   HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, -1));
   HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, -1));
