@@ -815,21 +815,32 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 // index, 4 FP32 sums) and reaches the grid only when its slot is taken by
 // another vertex or at the end of the work item.  One wavefront owns its
 // cache, so LDS program order is the only ordering needed.
-constexpr int kVtxSlotsLog2 = 10;
+//
+// A lane leaving a cell hands its vertex sums to the cache through the
+// wavefront's flush list: in a sample step only a few lanes leave their
+// cell, so the lanes that do append their entries to one dense list, and the
+// whole wavefront merges the list into the cache 64 entries at a time once
+// it is full (or at the end of the work item).
+constexpr int kVtxSlotsLog2 = 9;
 constexpr int kVtxSlots = 1 << kVtxSlotsLog2;
+constexpr uint32_t kFlushCap = 512; // entries of the flush list
 constexpr uint32_t kVtxEmpty = 0xffffffffu;
 
 struct VtxCache {
-  uint32_t *tag; // kVtxSlots tags (kVtxEmpty: free); nullptr: no cache
-  float4 *val;   // kVtxSlots sums {x, y, z, te}
+  uint32_t *tag;  // kVtxSlots tags (kVtxEmpty: free); nullptr: no cache
+  float4 *val;    // kVtxSlots sums {x, y, z, te}
+  uint32_t *ltag; // kFlushCap flush-list vertices
+  float4 *lval;   // kFlushCap flush-list sums
 };
 
-constexpr int vtx_cache_bytes() { return kVtxSlots * (4 + 16); }
+constexpr int vtx_cache_bytes() { return (kVtxSlots + kFlushCap) * (4 + 16); }
 
 __device__ inline VtxCache vtx_cache_at(void *base) {
   VtxCache c;
   c.val = static_cast<float4 *>(base);
   c.tag = reinterpret_cast<uint32_t *>(c.val + kVtxSlots);
+  c.lval = reinterpret_cast<float4 *>(c.tag + kVtxSlots);
+  c.ltag = reinterpret_cast<uint32_t *>(c.lval + kFlushCap);
   return c;
 }
 
@@ -862,12 +873,7 @@ __device__ inline void lds_add4(float4 *p, float4 a) {
 // compare-and-swap succeeds: that lane evicts the slot's old sums to the
 // grid and the slot then belongs to its vertex.  Missing lanes of that vertex
 // add into the slot, the others go to the grid directly.
-__device__ inline void vtx_put(const VtxCache &c, float *__restrict__ g, uint64_t v, float4 a) {
-  if (c.tag == nullptr) {
-    grid_add4(g, v, a);
-    return;
-  }
-  const uint32_t key = static_cast<uint32_t>(v);
+__device__ inline void vtx_put(const VtxCache &c, float *__restrict__ g, uint32_t key, float4 a) {
   const uint32_t sl = (key * 0x9e3779b1u) >> (32 - kVtxSlotsLog2);
   const uint32_t t = c.tag[sl];
   const bool hit = t == key;
@@ -887,7 +893,7 @@ __device__ inline void vtx_put(const VtxCache &c, float *__restrict__ g, uint64_
     if (c.tag[sl] == key)
       lds_add4(c.val + sl, a);
     else
-      grid_add4(g, v, a);
+      grid_add4(g, key, a);
   }
   wave_lds_sync();
 }
@@ -906,28 +912,86 @@ __device__ inline void vtx_cache_flush(const VtxCache &c, float *__restrict__ g,
   wave_lds_sync();
 }
 
-// acc (8 corners x {x, y, z, te}) -> the packed gradient grid at cell
-// `cell` (through the vertex cache), then acc = 0.
-__device__ inline void flush_cell_f32(float (&acc)[32], uint64_t cell, float *__restrict__ g,
-                                      uint64_t sx, uint64_t sy, const VtxCache &vc) {
+// The first n (wave-uniform) flush-list entries -> the vertex cache, one
+// entry per lane; called by the whole wavefront.
+__device__ inline void flush_list_drain(const VtxCache &c, float *__restrict__ g, uint32_t n,
+                                        int lane) {
+  wave_lds_sync(); // the entries written by other lanes are visible
+  for (uint32_t b = 0; b < n; b += 64) {
+    const uint32_t e = b + static_cast<uint32_t>(lane);
+    if (e < n)
+      vtx_put(c, g, c.ltag[e], c.lval[e]);
+  }
+  wave_lds_sync(); // read before the list is written again
+}
+
+// One vertex sum of a lane that takes part in a list step: to the list at
+// `pos`, or straight to the grid when there is no cache (vertex indices
+// beyond 32 bits).
+__device__ inline void list_put(const VtxCache &c, float *__restrict__ g, uint32_t pos,
+                                uint64_t v, float4 a) {
+  if (c.tag) {
+    c.ltag[pos] = static_cast<uint32_t>(v);
+    c.lval[pos] = a;
+  } else {
+    grid_add4(g, v, a);
+  }
+}
+
+// Called by the whole wavefront: the lanes with `part` each get `per` flush
+// list entries; returns the lane's first entry (meaningful where part).
+// The list is drained first when the entries would not fit.
+__device__ inline uint32_t list_reserve(const VtxCache &c, float *__restrict__ g, bool part,
+                                        uint32_t per, uint32_t &nlist, int lane,
+                                        uint64_t &mask) {
+  mask = __ballot(part);
+  if (mask == 0 || c.tag == nullptr)
+    return 0;
+  const uint32_t total = per * static_cast<uint32_t>(__popcll(mask));
+  if (nlist + total > kFlushCap) {
+    flush_list_drain(c, g, nlist, lane);
+    nlist = 0;
+  }
+  const uint32_t pos =
+      nlist + per * static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+  nlist += total;
+  return pos;
+}
+
+// acc (8 corners x {x, y, z, te}) of the lanes with `part` -> the flush list
+// as the 8 vertices of cell `cell`, then acc = 0 there.  Whole wavefront.
+__device__ inline void list_cell_f32(float (&acc)[32], bool part, uint64_t cell,
+                                     float *__restrict__ g, uint64_t sx, uint64_t sy,
+                                     const VtxCache &vc, uint32_t &nlist, int lane) {
+  uint64_t mask;
+  const uint32_t pos = list_reserve(vc, g, part, 8, nlist, lane, mask);
+  if (mask == 0 || !part)
+    return;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const uint64_t v = cell + (c >> 2) * sx + ((c >> 1) & 1) * sy + (c & 1);
-    vtx_put(vc, g, v, make_float4(acc[4 * c], acc[4 * c + 1], acc[4 * c + 2], acc[4 * c + 3]));
+    list_put(vc, g, pos + c,
+             v, make_float4(acc[4 * c], acc[4 * c + 1], acc[4 * c + 2], acc[4 * c + 3]));
 #pragma unroll
     for (int m = 0; m < 4; ++m)
       acc[4 * c + m] = 0.f;
   }
 }
 
-// The lane's ray moves from `cell` to its neighbour along one axis (BIT: 4
-// x, 2 y, 1 z in the corner numbering 4a + 2b + c), upwards when `up`: the
-// 4 corners it leaves behind go to the grid, the 4 shared with the new cell
-// move to their place in it, and the new cell's far corners start at 0.
+// The rays of the lanes with `part` move from `cell` to its neighbour along
+// one axis (BIT: 4 x, 2 y, 1 z in the corner numbering 4a + 2b + c),
+// upwards when `up`: the 4 corners they leave behind go to the flush list,
+// the 4 shared with the new cell move to their place in it, the new cell's
+// far corners start at 0, and `cell` becomes the new cell.  Whole wavefront.
 template <int BIT>
-__device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
+__device__ inline void list_shift_f32(float (&acc)[32], bool part, bool up, uint64_t &cell,
                                       float *__restrict__ g, uint64_t sx, uint64_t sy,
-                                      const VtxCache &vc) {
+                                      const VtxCache &vc, uint32_t &nlist, int lane) {
+  uint64_t mask;
+  const uint32_t pos = list_reserve(vc, g, part, 4, nlist, lane, mask);
+  if (mask == 0 || !part)
+    return;
+  int j = 0;
 #pragma unroll
   for (int lo = 0; lo < 8; ++lo) {
     if (lo & BIT)
@@ -943,8 +1007,11 @@ __device__ inline void shift_cell_f32(float (&acc)[32], bool up, uint64_t cell,
       acc[4 * lo + m] = up ? kept : 0.f;
       acc[4 * hi + m] = up ? 0.f : kept;
     }
-    vtx_put(vc, g, up ? vlo : vhi, make_float4(left[0], left[1], left[2], left[3]));
+    list_put(vc, g, pos + j, up ? vlo : vhi, make_float4(left[0], left[1], left[2], left[3]));
+    ++j;
   }
+  const uint64_t st = BIT == 4 ? sx : (BIT == 2 ? sy : 1);
+  cell = up ? cell + st : cell - st;
 }
 
 template <bool GRIDS, bool PW1>
@@ -999,112 +1066,124 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
   // the union of the lanes' step ranges, far end first
   const uint64_t k_end = wave_max_u64(live ? r.k_hi : 0ull);
   const uint64_t k_beg = wave_min_u64(live ? r.k_lo : ~0ull);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  uint32_t nlist = 0; // entries in the wavefront's flush list
   for (uint64_t k = k_end; k-- > k_beg;) {
-    if (!live || k < r.k_lo || k >= r.k_hi)
-      continue;
     S32 o;
-    if (!eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o))
-      continue;
-    const double fd_here = fd_total - after; // FD through this sample
-    after += o.fdk * sh.fdu;
-    double D = 0.;
-    float dLdw = static_cast<float>(aI), alpha_p = 0.f, dipa = 0.f;
+    bool valid = false;
+    if (live && k >= r.k_lo && k < r.k_hi)
+      valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
+    float dbx = 0.f, dby = 0.f, dbz = 0.f;
+    double dte = 0.;
+    if (valid) {
+      const double fd_here = fd_total - after; // FD through this sample
+      after += o.fdk * sh.fdu;
+      double D = 0.;
+      float dLdw = static_cast<float>(aI), alpha_p = 0.f, dipa = 0.f;
 #pragma unroll
-    for (int f = 0; f < kMaxF; ++f) {
-      if (f < nf) {
-        const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
-        double rv = X * in.rev[f];
-        rv -= rint(rv);
-        const float a = static_cast<float>(rv);
-        const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
-        const float c2 = cs * o.c2i - sn * o.s2i, s2 = sn * o.c2i + cs * o.s2i;
-        const float P = aQ[f] * c2 + aU[f] * s2;
-        dLdw += P;
-        alpha_p += P * lpf[f];
-        const float dq = 2.f * o.w * (aU[f] * c2 - aQ[f] * s2);
-        dipa += dq;
-        D += A.l2[f] * static_cast<double>(dq);
-        if (f == 0)
-          q0 += dq;
-      }
-    }
-    dacc += D;
-    const double R = dacc + gfd; // dL/d(FD contribution of this sample)
-    const double dte = o.te_raw > 0.f ? R * o.b_pard * sh.fdu + gdm * delta : 0.;
-    const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
-    float dbx = ft * lf.x, dby = ft * lf.y, dbz = ft * lf.z;
-    if (o.bper2 > 0.f) {
-      const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
-      dbx += coef * (o.bx - o.b_par * lf.x);
-      dby += coef * (o.by - o.b_par * lf.y);
-      dbz += coef * (o.bz - o.b_par * lf.z);
-      // ln |B_perp| in Gauss: the B grid and b0 are in muG here
-      const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
-      p4 += static_cast<double>(o.w) *
-            (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
-    }
-    if (o.n2 > 0.f) {
-      const float cc = dipa / o.n2;
-      dbx += cc * (o.y * ephf.x - o.x * ethf.x);
-      dby += cc * (o.y * ephf.y - o.x * ethf.y);
-      dbz += cc * (o.y * ephf.z - o.x * ethf.z);
-    }
-    if (o.te_in)
-      p5 += dte;
-    if (o.lsa_in) {
-      const float u = o.sps * o.cos_p - o.cps * o.sin_p;
-      const float v = o.sps * o.sin_p + o.cps * o.cos_p;
-      p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
-      const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
-      p1 += gpsi;
-      p2 += gpsi * o.lnr;
-      const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
-      p3 += gchi * o.tnh;
-    }
-    if (GRIDS && ggrid && o.cell_ok) {
-      if (o.idx != cur) {
-        if (cur != ~0ull) {
-          const int dx = o.cx - ccx, dy = o.cy - ccy, dz = o.cz - ccz;
-          if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-            // face steps, one axis at a time (every intermediate cell lies
-            // between two valid cells, so is valid)
-            uint64_t c = cur;
-            if (dx) {
-              shift_cell_f32<4>(acc, dx > 0, c, ggrid, sx, sy, vc);
-              c = dx > 0 ? c + sx : c - sx;
-            }
-            if (dy) {
-              shift_cell_f32<2>(acc, dy > 0, c, ggrid, sx, sy, vc);
-              c = dy > 0 ? c + sy : c - sy;
-            }
-            if (dz)
-              shift_cell_f32<1>(acc, dz > 0, c, ggrid, sx, sy, vc);
-          } else {
-            flush_cell_f32(acc, cur, ggrid, sx, sy, vc);
-          }
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
+          double rv = X * in.rev[f];
+          rv -= rint(rv);
+          const float a = static_cast<float>(rv);
+          const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
+          const float c2 = cs * o.c2i - sn * o.s2i, s2 = sn * o.c2i + cs * o.s2i;
+          const float P = aQ[f] * c2 + aU[f] * s2;
+          dLdw += P;
+          alpha_p += P * lpf[f];
+          const float dq = 2.f * o.w * (aU[f] * c2 - aQ[f] * s2);
+          dipa += dq;
+          D += A.l2[f] * static_cast<double>(dq);
+          if (f == 0)
+            q0 += dq;
         }
-        cur = o.idx;
-        ccx = o.cx;
-        ccy = o.cy;
-        ccz = o.cz;
       }
-      const float gte = static_cast<float>(dte);
-      const float wx[2] = {1.f - o.xd, o.xd}, wy[2] = {1.f - o.yd, o.yd},
-                  wz[2] = {1.f - o.zd, o.zd};
+      dacc += D;
+      const double R = dacc + gfd; // dL/d(FD contribution of this sample)
+      dte = o.te_raw > 0.f ? R * o.b_pard * sh.fdu + gdm * delta : 0.;
+      const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
+      dbx = ft * lf.x;
+      dby = ft * lf.y;
+      dbz = ft * lf.z;
+      if (o.bper2 > 0.f) {
+        const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
+        dbx += coef * (o.bx - o.b_par * lf.x);
+        dby += coef * (o.by - o.b_par * lf.y);
+        dbz += coef * (o.bz - o.b_par * lf.z);
+        // ln |B_perp| in Gauss: the B grid and b0 are in muG here
+        const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
+        p4 += static_cast<double>(o.w) *
+              (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
+      }
+      if (o.n2 > 0.f) {
+        const float cc = dipa / o.n2;
+        dbx += cc * (o.y * ephf.x - o.x * ethf.x);
+        dby += cc * (o.y * ephf.y - o.x * ethf.y);
+        dbz += cc * (o.y * ephf.z - o.x * ethf.z);
+      }
+      if (o.te_in)
+        p5 += dte;
+      if (o.lsa_in) {
+        const float u = o.sps * o.cos_p - o.cps * o.sin_p;
+        const float v = o.sps * o.sin_p + o.cps * o.cos_p;
+        p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
+        const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
+        p1 += gpsi;
+        p2 += gpsi * o.lnr;
+        const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
+        p3 += gchi * o.tnh;
+      }
+    }
+    if (GRIDS && ggrid) {
+      // the whole wavefront: lanes whose ray leaves its cell hand on the
+      // corner sums of the cell (face steps one axis at a time: every
+      // intermediate cell lies between two valid cells, so is valid)
+      const bool in_cell = valid && o.cell_ok;
+      const bool moves = in_cell && cur != ~0ull && o.idx != cur;
+      int dx = 0, dy = 0, dz = 0;
+      if (moves) {
+        dx = o.cx - ccx;
+        dy = o.cy - ccy;
+        dz = o.cz - ccz;
+      }
+      const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+      uint64_t c = cur;
+      list_shift_f32<4>(acc, moves && near && dx != 0, dx > 0, c, ggrid, sx, sy, vc, nlist,
+                        lane);
+      list_shift_f32<2>(acc, moves && near && dy != 0, dy > 0, c, ggrid, sx, sy, vc, nlist,
+                        lane);
+      list_shift_f32<1>(acc, moves && near && dz != 0, dz > 0, c, ggrid, sx, sy, vc, nlist,
+                        lane);
+      list_cell_f32(acc, moves && !near, cur, ggrid, sx, sy, vc, nlist, lane);
+      if (in_cell) {
+        if (o.idx != cur) {
+          cur = o.idx;
+          ccx = o.cx;
+          ccy = o.cy;
+          ccz = o.cz;
+        }
+        const float gte = static_cast<float>(dte);
+        const float wx[2] = {1.f - o.xd, o.xd}, wy[2] = {1.f - o.yd, o.yd},
+                    wz[2] = {1.f - o.zd, o.zd};
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float w = wx[c >> 2] * wy[(c >> 1) & 1] * wz[c & 1];
-        acc[4 * c + 0] += w * dbx;
-        acc[4 * c + 1] += w * dby;
-        acc[4 * c + 2] += w * dbz;
-        acc[4 * c + 3] += w * gte;
+        for (int cn = 0; cn < 8; ++cn) {
+          const float w = wx[cn >> 2] * wy[(cn >> 1) & 1] * wz[cn & 1];
+          acc[4 * cn + 0] += w * dbx;
+          acc[4 * cn + 1] += w * dby;
+          acc[4 * cn + 2] += w * dbz;
+          acc[4 * cn + 3] += w * gte;
+        }
       }
     }
   }
-  if (GRIDS && ggrid && cur != ~0ull)
-    flush_cell_f32(acc, cur, ggrid, sx, sy, vc);
-  if (GRIDS && ggrid && vc.tag)
-    vtx_cache_flush(vc, ggrid, static_cast<int>(threadIdx.x & 63));
+  if (GRIDS && ggrid) {
+    list_cell_f32(acc, cur != ~0ull, cur, ggrid, sx, sy, vc, nlist, lane);
+    if (vc.tag) {
+      flush_list_drain(vc, ggrid, nlist, lane);
+      vtx_cache_flush(vc, ggrid, lane);
+    }
+  }
   if (live) {
     double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
     rp[0] = p0;
