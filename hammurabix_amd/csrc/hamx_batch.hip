@@ -1210,7 +1210,13 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
       hi = mid;
   }
   const uint32_t id = __builtin_amdgcn_readfirstlane(a.ids[lo]);
-  const BInst &in = a.inst[id];
+  // The instance records are written by the host before the launch and only
+  // read here: reading them through the constant address space lets the
+  // compiler keep their (wave-uniform) fields in scalar loads and registers
+  // instead of reloading them with vector loads in every sample step, where
+  // each reload would also wait for the step's gradient atomics.
+  using ConstInst = const __attribute__((address_space(4))) BInst;
+  const BInst &in = *(const BInst *)((ConstInst *)a.inst + id);
   const int s = static_cast<int>(a.level);
   const uint64_t npix = in.sh[s].hs.npix;
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
@@ -1222,9 +1228,10 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
     const uint64_t pl = p0 + static_cast<uint64_t>(lane);
     const bool live = pl < npix;
     const uint64_t ipix = live ? hp_nest2ring(hs, pl) : 0;
-    RayGeo r{};
+    RayGeo rs{};
     if (live)
-      ray_setup_call(in.ph, shell_args_rev(in, s), ipix, r);
+      ray_setup_call(in.ph, shell_args_rev(in, s), ipix, rs);
+    const RayGeo r = rs; // a copy that stays in registers (rs went to the call's stack)
     adj_lanes_f32<GRIDS, PW1>(in, s, ipix, live, r, vtx_cache_at(slab));
     return;
   }

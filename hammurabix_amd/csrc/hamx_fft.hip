@@ -33,7 +33,8 @@
 namespace hamx {
 
 constexpr uint32_t kMaxPass = 16;
-constexpr uint32_t kTileSmall = 4096; // complex128 per workgroup: 64 KB of LDS
+constexpr uint32_t kTileMini = 2048;  // complex128 per workgroup: 32 KB of LDS
+constexpr uint32_t kTileSmall = 4096; // 64 KB: lines of 2049 to 4096 points
 constexpr uint32_t kTileLarge = 8192; // 128 KB: lines of 4097 to 8192 points
 
 struct FftPlan {
@@ -431,7 +432,7 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
       len = ax.M;
       make_plan(ax.M, ax.plan);
     }
-    const uint32_t tile = len <= kTileSmall ? kTileSmall : kTileLarge;
+    const uint32_t tile = len <= kTileMini ? kTileMini : (len <= kTileSmall ? kTileSmall : kTileLarge);
     // lines per workgroup: fill the tile; for strided axes the lines are
     // adjacent columns (at most 16, 256 B rows)
     uint64_t C = tile / len;
@@ -457,17 +458,20 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
     else                                                                                   \
       HAMX_FFT_LAUNCH(S, T, false);                                                        \
   } while (0)
-      if (sign > 0) {
-        if (tile == kTileSmall)
-          HAMX_FFT_LAUNCH_T(1, kTileSmall);
-        else
-          HAMX_FFT_LAUNCH_T(1, kTileLarge);
-      } else {
-        if (tile == kTileSmall)
-          HAMX_FFT_LAUNCH_T(-1, kTileSmall);
-        else
-          HAMX_FFT_LAUNCH_T(-1, kTileLarge);
-      }
+#define HAMX_FFT_LAUNCH_S(S)                                                               \
+  do {                                                                                     \
+    if (tile == kTileMini)                                                                 \
+      HAMX_FFT_LAUNCH_T(S, kTileMini);                                                     \
+    else if (tile == kTileSmall)                                                           \
+      HAMX_FFT_LAUNCH_T(S, kTileSmall);                                                    \
+    else                                                                                   \
+      HAMX_FFT_LAUNCH_T(S, kTileLarge);                                                    \
+  } while (0)
+      if (sign > 0)
+        HAMX_FFT_LAUNCH_S(1);
+      else
+        HAMX_FFT_LAUNCH_S(-1);
+#undef HAMX_FFT_LAUNCH_S
 #undef HAMX_FFT_LAUNCH_T
 #undef HAMX_FFT_LAUNCH
     } else {
@@ -477,17 +481,20 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
 #define HAMX_FFT_LAUNCH(S, T)                                                              \
   hipLaunchKernelGGL((k_fft_axis_bluestein<S, T>), dim3(blocks), dim3(256), 0, ctx->stream, \
                      data, ax, twm, chirp, bhat)
-      if (sign > 0) {
-        if (tile == kTileSmall)
-          HAMX_FFT_LAUNCH(1, kTileSmall);
-        else
-          HAMX_FFT_LAUNCH(1, kTileLarge);
-      } else {
-        if (tile == kTileSmall)
-          HAMX_FFT_LAUNCH(-1, kTileSmall);
-        else
-          HAMX_FFT_LAUNCH(-1, kTileLarge);
-      }
+#define HAMX_FFT_LAUNCH_S(S)                                                               \
+  do {                                                                                     \
+    if (tile == kTileMini)                                                                 \
+      HAMX_FFT_LAUNCH(S, kTileMini);                                                       \
+    else if (tile == kTileSmall)                                                           \
+      HAMX_FFT_LAUNCH(S, kTileSmall);                                                      \
+    else                                                                                   \
+      HAMX_FFT_LAUNCH(S, kTileLarge);                                                      \
+  } while (0)
+      if (sign > 0)
+        HAMX_FFT_LAUNCH_S(1);
+      else
+        HAMX_FFT_LAUNCH_S(-1);
+#undef HAMX_FFT_LAUNCH_S
 #undef HAMX_FFT_LAUNCH
     }
     HAMX_HIP(ctx, hipGetLastError());
