@@ -188,7 +188,14 @@ __device__ inline uint64_t line_base(const FftAxis &ax, uint64_t L) {
 template <int SIGN, uint32_t TILE, bool POW2>
 __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data, FftAxis ax,
                                                   const double2 *__restrict__ tw) {
-  __shared__ double2 lds[TILE];
+  // power-of-two lines in the mini tile: the twiddle table is copied into
+  // LDS after the tile (dynamic LDS of C n + n values), so the passes read
+  // it with LDS latency (the mixed-radix kernels keep it in memory: their
+  // extra registers would cost a wave per SIMD)
+  constexpr bool kLdsTw = POW2 && TILE <= kTileMini;
+  extern __shared__ double2 dyn_lds[];
+  __shared__ double2 static_lds[kLdsTw ? 1 : TILE];
+  double2 *lds = kLdsTw ? dyn_lds : static_lds;
   const uint32_t n = ax.n, C = ax.C, total = C * n;
   const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * C;
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
@@ -198,8 +205,15 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data, Ff
     lds[c * n + t] = L < ax.lines ? data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s]
                                   : make_double2(0., 0.);
   }
+  const double2 *twl = tw;
+  if constexpr (kLdsTw) {
+    double2 *t = lds + total;
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x)
+      t[e] = tw[e];
+    twl = t;
+  }
   __syncthreads();
-  run_plan<SIGN, TILE, POW2>(lds, ax.plan, C, tw);
+  run_plan<SIGN, TILE, POW2>(lds, ax.plan, C, twl);
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     uint32_t c, t;
     tile_coords(ax, n, e, c, t);
@@ -449,8 +463,10 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
       HAMX_TRY(twiddle_table(ctx, n, &tw));
       const bool pow2 = is_pow2(n);
 #define HAMX_FFT_LAUNCH(S, T, P)                                                           \
-  hipLaunchKernelGGL((k_fft_axis<S, T, P>), dim3(blocks), dim3(256), 0, ctx->stream, data, ax, \
-                     tw)
+  hipLaunchKernelGGL((k_fft_axis<S, T, P>), dim3(blocks), dim3(256),                       \
+                     (P) && (T) <= kTileMini ? (ax.C + 1) * ax.n * sizeof(double2) : 0,        \
+                     ctx->stream,                                                          \
+                     data, ax, tw)
 #define HAMX_FFT_LAUNCH_T(S, T)                                                            \
   do {                                                                                     \
     if (pow2)                                                                              \

@@ -358,53 +358,52 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
   }
 }
 
-// STEP III Gram-Schmidt (brnd_es.cc:211-259), results in bx, by, bz.
-__global__ __launch_bounds__(256) void k_es_gs(const double2 *c0, const double2 *c1,
-                                               double *bx, double *by, double *bz,
-                                               EsArgs a) {
+// STEP III Gram-Schmidt (brnd_es.cc:211-259) of the symmetrised vector b at
+// index idx: the part of b normal to the wave vector, times sqrt(3).
+__device__ inline Vec3 es_gs_at(const EsArgs &a, uint64_t idx, const Vec3 &b) {
+  const uint64_t i = idx / (a.b.ny * a.b.nz);
+  const uint64_t j = (idx / a.b.nz) % a.b.ny;
+  const uint64_t l = idx % a.b.nz;
+  const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
+               wavenum(l, a.b.nz, a.b.lz)};
+  double k2 = 0.;
+  k2 += k.x * k.x;
+  k2 += k.y * k.y;
+  k2 += k.z * k.z;
+  double b2 = 0.;
+  b2 += b.x * b.x;
+  b2 += b.y * b.y;
+  b2 += b.z * b.z;
+  if (k2 == 0 || b2 == 0)
+    return b;
+  const double inv = 1. / k2;
+  const double kb = dot0(k, b);
+  return Vec3{a.sqrt3 * (b.x - k.x * kb * inv), a.sqrt3 * (b.y - k.y * kb * inv),
+              a.sqrt3 * (b.z - k.z * kb * inv)};
+}
+
+// STEP III and the reassembly c0 = bx + i by, c1 = by + i bz with
+// b = sqrt(2) * the Gram-Schmidt result (brnd_es.cc:211-270), in place.  The
+// symmetrised b of index idx reads idx and its mirror sym; the mirror's b is
+// the same sum, so the thread of the lower index of each pair reads the two
+// elements once and writes both results, each with its own wave vector.
+__global__ __launch_bounds__(256) void k_es_gs_pairs(double2 *c0, double2 *c1, EsArgs a) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < a.b.n; idx += stride) {
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
     const uint64_t sym = sym_index(a.b, idx);
-    const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
-                 wavenum(l, a.b.nz, a.b.lz)};
+    if (sym < idx)
+      continue; // the pair belongs to the thread of sym
     const double2 p0 = c0[idx], q0 = c0[sym], p1 = c1[idx], q1 = c1[sym];
     const Vec3 b{0.5 * (p0.x + q0.x), 0.5 * (p1.x + q1.x), 0.5 * (p1.y + q1.y)};
-    double k2 = 0.;
-    k2 += k.x * k.x;
-    k2 += k.y * k.y;
-    k2 += k.z * k.z;
-    double b2 = 0.;
-    b2 += b.x * b.x;
-    b2 += b.y * b.y;
-    b2 += b.z * b.z;
-    Vec3 fr = b;
-    if (!(k2 == 0 || b2 == 0)) {
-      const double inv = 1. / k2;
-      const double kb = dot0(k, b);
-      fr = Vec3{a.sqrt3 * (b.x - k.x * kb * inv), a.sqrt3 * (b.y - k.y * kb * inv),
-                a.sqrt3 * (b.z - k.z * kb * inv)};
+    const Vec3 fi = es_gs_at(a, idx, b);
+    c0[idx] = make_double2(cgs::sqrtwo * fi.x, cgs::sqrtwo * fi.y);
+    c1[idx] = make_double2(cgs::sqrtwo * fi.y, cgs::sqrtwo * fi.z);
+    if (sym != idx) {
+      const Vec3 fs = es_gs_at(a, sym, b);
+      c0[sym] = make_double2(cgs::sqrtwo * fs.x, cgs::sqrtwo * fs.y);
+      c1[sym] = make_double2(cgs::sqrtwo * fs.y, cgs::sqrtwo * fs.z);
     }
-    bx[idx] = cgs::sqrtwo * fr.x;
-    by[idx] = cgs::sqrtwo * fr.y;
-    bz[idx] = cgs::sqrtwo * fr.z;
-  }
-}
-
-// reassemble c0 = bx + i by, c1 = by + i bz (brnd_es.cc:261-270)
-__global__ __launch_bounds__(256) void k_es_reassemble(double2 *c0, double2 *c1,
-                                                       const double *bx,
-                                                       const double *by,
-                                                       const double *bz, uint64_t n) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-       idx < n; idx += stride) {
-    const double y = by[idx];
-    c0[idx] = make_double2(bx[idx], y);
-    c1[idx] = make_double2(y, bz[idx]);
   }
 }
 
@@ -615,10 +614,7 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
   HAMX_HIP(ctx, hipGetLastError());
   HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, -1));
   HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, -1));
-  hipLaunchKernelGGL(k_es_gs, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox, oy,
-                     oz, a);
-  hipLaunchKernelGGL(k_es_reassemble, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1,
-                     ox, oy, oz, n);
+  hipLaunchKernelGGL(k_es_gs_pairs, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a);
   HAMX_HIP(ctx, hipGetLastError());
   HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
   HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
