@@ -7,6 +7,8 @@ device tensors; calling ``backward`` on any scalar of them runs the adjoint
 and fills ``params.grad`` (d/d of b0, psi0, psi1, chi0 of the LSA field,
 alpha of the CRE model, n0 of the uniform thermal electrons, in the CGS /
 radian units of the descriptor) and the random-grid tensors' ``.grad``.
+Any regular B and TE model of the forward and a mask may be given; the CRE
+is the analytic or uniform model with a constant index.
 
 The forward and the adjoint run on the library's stream and are synchronous
 with respect to the caller.  The forward keeps its tape (FD checkpoints) in
@@ -57,15 +59,15 @@ def map_sizes(desc: capi.HamxLosDesc) -> dict[str, int]:
 
 class _LosFunction(torch.autograd.Function):
     @staticmethod
-    def forward(fctx, params, bx, by, bz, te, ctx, desc, brnd_box, ternd_box, keys):
+    def forward(fctx, params, bx, by, bz, te, ctx, desc, brnd_box, ternd_box, keys, fixed):
         d = _with_params(desc, params)
         dev = torch.device("cuda", torch.cuda.current_device())
         sizes = map_sizes(d)
         maps = {k: torch.empty(sizes[k], dtype=torch.float64, device=dev) for k in keys}
         brnd = (bx, by, bz) if bx is not None else None
         ctx.los_forward(d, maps, brnd=brnd, brnd_box=brnd_box, ternd=te,
-                        ternd_box=ternd_box, keep_tape=True)
-        fctx.ctx, fctx.desc, fctx.keys = ctx, d, keys
+                        ternd_box=ternd_box, keep_tape=True, **fixed)
+        fctx.ctx, fctx.desc, fctx.keys, fctx.fixed = ctx, d, keys, fixed
         fctx.brnd_box, fctx.ternd_box = brnd_box, ternd_box
         fctx.save_for_backward(params, bx, by, bz, te)
         return tuple(maps[k] for k in keys)
@@ -90,7 +92,7 @@ class _LosFunction(torch.autograd.Function):
         dternd = torch.zeros(n, dtype=torch.float64, device=dev) if want_t else None
         brnd = (bx, by, bz) if bx is not None else None
         kw = dict(brnd=brnd, brnd_box=fctx.brnd_box, ternd=te, ternd_box=fctx.ternd_box,
-                  dbrnd=dbrnd, dternd=dternd)
+                  dbrnd=dbrnd, dternd=dternd, **fctx.fixed)
         try:  # the tape of this forward, unless another pass has replaced it
             fctx.ctx.los_adjoint(d, adj, dparams, use_tape=True, **kw)
         except HamxError as e:
@@ -99,23 +101,31 @@ class _LosFunction(torch.autograd.Function):
             fctx.ctx.los_adjoint(d, adj, dparams, **kw)
         gp = dparams.to(params.device, params.dtype) if fctx.needs_input_grad[0] else None
         gb = dbrnd if dbrnd is not None else (None, None, None)
-        return (gp, gb[0], gb[1], gb[2], dternd, None, None, None, None, None)
+        return (gp, gb[0], gb[1], gb[2], dternd, None, None, None, None, None, None)
 
 
 def los_maps(ctx: Context, desc: capi.HamxLosDesc, params: torch.Tensor,
              brnd: Optional[tuple[torch.Tensor, torch.Tensor, torch.Tensor]] = None,
              brnd_box: Optional[capi.HamxBox] = None, ternd: Optional[torch.Tensor] = None,
-             ternd_box: Optional[capi.HamxBox] = None) -> dict[str, torch.Tensor]:
+             ternd_box: Optional[capi.HamxBox] = None,
+             **fixed: Any) -> dict[str, torch.Tensor]:
     """Differentiable write_grid pass.  `params` (6 float64 values, any device)
     replaces the descriptor's b0, psi0, psi1, chi0, alpha and n0; grids are
     float64 device tensors.  With a TE grid the B grid must be given on the
-    same box (the adjoint keeps one record per vertex)."""
+    same box (the adjoint keeps one record per vertex).  `fixed` holds the
+    los_forward field arguments that are not differentiated: mask /
+    nside_mask, the regular-field grids breg / breg_box and tereg /
+    tereg_box.  The gradients of b0, psi0, psi1 and chi0 are zero unless the
+    regular B model is LSA, n0's unless the TE model is the uniform sphere."""
     if ternd is not None and brnd is None:
         raise ValueError("a differentiable TE grid needs the random B grid on the same box")
+    allowed = {"mask", "nside_mask", "breg", "breg_box", "tereg", "tereg_box"}
+    if set(fixed) - allowed:
+        raise ValueError(f"los_maps: unknown field arguments {sorted(set(fixed) - allowed)}")
     keys = tuple(k for k in MAP_KEYS if k in map_sizes(desc))
     bx, by, bz = brnd if brnd is not None else (None, None, None)
     outs = _LosFunction.apply(params, bx, by, bz, ternd, ctx, desc, brnd_box, ternd_box,
-                              keys)
+                              keys, fixed)
     return dict(zip(keys, outs))
 
 

@@ -51,7 +51,7 @@ struct AdjShell {
   uint64_t sy, sx;                        // corner strides of the packed grid
 };
 
-template <bool GRIDS, bool PW1>
+template <bool GRIDS, bool PW1, bool MODELS>
 __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst ac,
                                                  Grids gr, ShellArgs sa, AdjShell as) {
   // kAdjRays consecutive rays per wavefront (set up one per lane), and a
@@ -77,6 +77,18 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
   for (uint64_t jr = 0; jr < nr; ++jr) {
   const uint64_t ipix = p0 + jr;
   const RayGeo r = slots[wave][jr];
+  if (sa.mask && sa.mask[ipix] != 1.0) {
+    // a masked-out ray is zero whatever the fields (integrator.cc:63-70):
+    // no partials, no grid gradient, no dependence on the inner FD
+    if (lane == 0) {
+      double *rp = as.ray_par + 8 * ipix;
+      for (int m = 0; m < 6; ++m)
+        rp[m] = 0.;
+      if (as.gfdin)
+        as.gfdin[ipix] = 0.;
+    }
+    continue; // uniform: one ray per wavefront at a time
+  }
   const double gq = as.gq ? as.gq[ipix] * ac.tc : 0.;
   const double gu = as.gu ? as.gu[ipix] * ac.tc : 0.;
   const double gi = as.gi ? as.gi[ipix] * ac.tc : 0.;
@@ -96,7 +108,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     FastSample o;
     bool valid = false;
     if (k < r.k_hi)
-      valid = eval_fast<GRIDS, PW1>(ph, fc, gr, sa, r, k, o);
+      valid = eval_fast<GRIDS, PW1, MODELS>(ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
     const double chunk = wave_last(incl);
     const double fd_here = (fd_total - after - chunk) + incl;
@@ -357,14 +369,14 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
   std::memset(&none, 0, sizeof(none));
   const hamx_fields &f = fields ? *fields : none;
   const Phys ph = make_phys(d);
-  if (f.mask)
-    return set_err(ctx, HAMX_ERR_UNSUPPORTED, "hamx_los_adjoint: masked passes are not supported");
-  if (!(d->breg_model == HAMX_BREG_LSA && d->tereg_model == HAMX_TEREG_UNIF &&
-        ph.idx_const && d->do_sync &&
+  if (!(ph.idx_const && d->do_sync &&
         (d->cre_model == HAMX_CRE_ANA || d->cre_model == HAMX_CRE_UNIF)))
     return set_err(ctx, HAMX_ERR_UNSUPPORTED,
-                   "hamx_los_adjoint supports LSA B, uniform TE and a constant-index CRE "
-                   "with synchrotron output");
+                   "hamx_los_adjoint supports the analytic or uniform CRE with a constant "
+                   "index, with synchrotron output");
+  // any regular B and TE model: the LSA parameters' partials are zero for
+  // the other B models, n0's for TE models other than the uniform sphere
+  const bool models = !(d->breg_model == HAMX_BREG_LSA && d->tereg_model == HAMX_TEREG_UNIF);
 
   // ---- the forward's tape
   const uint64_t nside_fd_map = d->do_fd ? d->nside_fd : d->nside_sync;
@@ -393,6 +405,18 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
   if (grids_any && !packed)
     return set_err(ctx, HAMX_ERR_UNSUPPORTED,
                    "hamx_los_adjoint needs the random B and TE grids on one box");
+  // the mask, duplicated to each shell's nside as the forward does
+  const double *mask_piv = nullptr;
+  Healpix h_mask;
+  std::memset(&h_mask, 0, sizeof(h_mask));
+  if (f.mask) {
+    if (!is_pow2(f.nside_mask))
+      return set_err(ctx, HAMX_ERR_NSIDE, "nside_mask %u is not a power of 2", f.nside_mask);
+    h_mask = healpix_make(f.nside_mask);
+    const void *mp;
+    HAMX_TRY(stage_in(ctx, kSlotMaskPivot, f.mask, h_mask.npix * sizeof(double), in_dev, &mp));
+    mask_piv = static_cast<const double *>(mp);
+  }
   const void *cp;
   const double *a_is = nullptr, *a_qs = nullptr, *a_us = nullptr, *a_dm = nullptr,
                *a_fd = nullptr;
@@ -422,6 +446,11 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
   for (int m = 0; m < 5; ++m) {
     HAMX_TRY(ws_get(ctx, kSlotAdjShell0 + m, max_npix * sizeof(double), &q));
     sg[m] = static_cast<double *>(q);
+  }
+  double *mask_shell = nullptr;
+  if (mask_piv) {
+    HAMX_TRY(ws_get(ctx, kSlotMaskShell, max_npix * sizeof(double), &q));
+    mask_shell = static_cast<double *>(q);
   }
   HAMX_TRY(ws_get(ctx, kSlotAdjG, h_fd.npix * sizeof(double), &q));
   double *G = static_cast<double *>(q);
@@ -479,6 +508,10 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
     sa.fd_map = has_fdin ? ckpt + static_cast<uint64_t>(s) * h_fd.npix : nullptr;
     sa.hfd = h_fd;
     const uint64_t npix = sa.hs.npix;
+    if (mask_piv) {
+      HAMX_TRY(mask_at_nside(ctx, mask_shell, sa.hs, mask_piv, h_mask));
+      sa.mask = mask_shell;
+    }
     for (int m = 0; m < 5; ++m)
       HAMX_TRY(fill_f64(ctx, sg[m], npix, 0.));
     // shell-map adjoints: A_s^T of the output adjoints
@@ -530,18 +563,26 @@ extern "C" int hamx_los_adjoint(hamx_ctx *ctx, const hamx_los_desc *d,
     const unsigned blocks =
         static_cast<unsigned>((npix + 4 * kAdjRays - 1) / (4 * kAdjRays));
     ProfMark pm = prof_begin(ctx);
+#define HAMX_ADJ_LAUNCH(G, P, M)                                                           \
+  hipLaunchKernelGGL((k_los_adj<G, P, M>), dim3(blocks), dim3(256), 0, ctx->stream, ph, fc, ac, \
+                     gr, sa, as)
+#define HAMX_ADJ_LAUNCH_M(G, P)                                                            \
+  do {                                                                                     \
+    if (models)                                                                            \
+      HAMX_ADJ_LAUNCH(G, P, true);                                                         \
+    else                                                                                   \
+      HAMX_ADJ_LAUNCH(G, P, false);                                                        \
+  } while (0)
     if (packed && pw1)
-      hipLaunchKernelGGL((k_los_adj<true, true>), dim3(blocks), dim3(256), 0, ctx->stream,
-                         ph, fc, ac, gr, sa, as);
+      HAMX_ADJ_LAUNCH_M(true, true);
     else if (packed)
-      hipLaunchKernelGGL((k_los_adj<true, false>), dim3(blocks), dim3(256), 0, ctx->stream,
-                         ph, fc, ac, gr, sa, as);
+      HAMX_ADJ_LAUNCH_M(true, false);
     else if (pw1)
-      hipLaunchKernelGGL((k_los_adj<false, true>), dim3(blocks), dim3(256), 0, ctx->stream,
-                         ph, fc, ac, gr, sa, as);
+      HAMX_ADJ_LAUNCH_M(false, true);
     else
-      hipLaunchKernelGGL((k_los_adj<false, false>), dim3(blocks), dim3(256), 0, ctx->stream,
-                         ph, fc, ac, gr, sa, as);
+      HAMX_ADJ_LAUNCH_M(false, false);
+#undef HAMX_ADJ_LAUNCH_M
+#undef HAMX_ADJ_LAUNCH
     HAMX_HIP(ctx, hipGetLastError());
     prof_end(ctx, pm, "los_adjoint", static_cast<uint32_t>(s), npix * sa.step);
     unsigned rb = blocks_for(npix);

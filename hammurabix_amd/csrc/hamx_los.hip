@@ -55,25 +55,6 @@ struct SampleOut {
   double ipa;  // intrinsic polarisation angle
 };
 
-__device__ inline Vec3 gather3(const double *fx, const double *fy,
-                               const double *fz, const GridGeo &g,
-                               const Vec3 &pos) {
-  Cell c;
-  if (!grid_cell(g, pos, c))
-    return Vec3{0., 0., 0.};
-  const uint64_t sy = g.nz, sx = g.ny * g.nz;
-  return Vec3{lerp_cell(fx, c, sy, sx), lerp_cell(fy, c, sy, sx),
-              lerp_cell(fz, c, sy, sx)};
-}
-
-__device__ inline double gather1(const double *f, const GridGeo &g,
-                                 const Vec3 &pos) {
-  Cell c;
-  if (!grid_cell(g, pos, c))
-    return 0.;
-  return lerp_cell(f, c, g.nz, g.ny * g.nz);
-}
-
 // CRE_ana / CRE_unif spatial profile (cre_ana.cc:11-19, cre_unif.cc)
 __device__ inline double cre_profile(const Phys &ph, const Vec3 &pos) {
   if (ph.cre_model == HAMX_CRE_ANA) {
@@ -507,6 +488,14 @@ __global__ __launch_bounds__(256) void k_mask_dup(double *out, Healpix ht, const
   const uint64_t m = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (m < ht.npix)
     out[m] = hp_mask_value(ht, m, piv, hp);
+}
+
+int mask_at_nside(hamx_ctx *ctx, double *out, const Healpix &ht, const double *piv,
+                  const Healpix &hp) {
+  hipLaunchKernelGGL(k_mask_dup, dim3(static_cast<unsigned>((ht.npix + 255) / 256)), dim3(256),
+                     0, ctx->stream, out, ht, piv, hp);
+  HAMX_HIP(ctx, hipGetLastError());
+  return HAMX_OK;
 }
 
 // Pack SoA brnd (x,y,z) and ternd into one double4 record per vertex.

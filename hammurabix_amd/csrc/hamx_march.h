@@ -83,6 +83,26 @@ struct RayGeo {
   uint64_t k_lo, k_hi;
 };
 
+// Trilinear gathers of separate component grids (0 outside the box).
+__device__ inline Vec3 gather3(const double *fx, const double *fy,
+                               const double *fz, const GridGeo &g,
+                               const Vec3 &pos) {
+  Cell c;
+  if (!grid_cell(g, pos, c))
+    return Vec3{0., 0., 0.};
+  const uint64_t sy = g.nz, sx = g.ny * g.nz;
+  return Vec3{lerp_cell(fx, c, sy, sx), lerp_cell(fy, c, sy, sx),
+              lerp_cell(fz, c, sy, sx)};
+}
+
+__device__ inline double gather1(const double *f, const GridGeo &g,
+                                 const Vec3 &pos) {
+  Cell c;
+  if (!grid_cell(g, pos, c))
+    return 0.;
+  return lerp_cell(f, c, g.nz, g.ny * g.nz);
+}
+
 // ---------------------------------------------------------------- per ray --
 
 __device__ inline void ray_interval(const Phys &ph, const ShellArgs &sa,
@@ -493,7 +513,10 @@ struct FastSample {
   double fdk, jt, jp, c2i, s2i;
 };
 
-template <bool GRIDS, bool PW1>
+// MODELS: the regular B and TE come from the descriptor's models (any of
+// them, an LSA field with its parameter partials; a uniform TE sphere with
+// n0's) instead of being LSA and the uniform sphere.
+template <bool GRIDS, bool PW1, bool MODELS = false>
 __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grids &gr,
                                  const ShellArgs &sa, const RayGeo &r, uint64_t k,
                                  FastSample &o) {
@@ -506,7 +529,14 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   // WMAP-LSA (breg_lsa.cc:9-26)
   Vec3 B{0., 0., 0.};
   o.rr = sqrt(pos.x * pos.x + pos.y * pos.y);
-  o.lsa_in = !(o.rr > (20. * cgs::kpc) || o.rr < (3. * cgs::kpc));
+  const bool lsa = !MODELS || ph.breg_model == HAMX_BREG_LSA;
+  o.lsa_in = lsa && !(o.rr > (20. * cgs::kpc) || o.rr < (3. * cgs::kpc));
+  if (MODELS && !lsa) {
+    if (ph.breg_model == HAMX_BREG_GRID)
+      B = gather3(gr.breg_x, gr.breg_y, gr.breg_z, gr.g_breg, pos);
+    else
+      B = breg_eval<true>(ph.breg_model, ph.breg, pos);
+  }
   if (o.lsa_in) {
     const double inv_r = 1.0 / o.rr;
     o.cos_p = pos.x * inv_r;
@@ -524,11 +554,17 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   }
   // uniform TE sphere (tereg_unif.cc:6-12)
   double te = 0.;
-  {
+  if (!MODELS || ph.tereg_model == HAMX_TEREG_UNIF) {
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
     o.te_in = !(length0(d) > fc.te_r0);
     if (o.te_in)
       te = fc.n0;
+  } else {
+    o.te_in = false; // no n0 partial
+    if (ph.tereg_model == HAMX_TEREG_YMW16)
+      te = tereg_ymw16_eval(ph.tereg, pos.x, pos.y, pos.z);
+    else if (ph.tereg_model == HAMX_TEREG_GRID)
+      te = gather1(gr.tereg, gr.g_tereg, pos);
   }
   o.cell_ok = false;
   if (GRIDS) {
@@ -646,6 +682,10 @@ inline uint64_t shell_offset(const hamx_los_desc *d, uint32_t s) {
 int los_forward_impl(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
                      hamx_maps *out, uint32_t flags, double *ckpt, double *shell_fd,
                      Arena *ar = nullptr);
+// The caller's mask map (nside of hp) at the nside of ht, into out
+// (Hampisk::duplicate), queued on the context stream.
+int mask_at_nside(hamx_ctx *ctx, double *out, const Healpix &ht, const double *piv,
+                  const Healpix &hp);
 // hamx_los_forward recording the adjoint's tape in the context workspace.
 int los_forward_taped(hamx_ctx *ctx, const hamx_los_desc *d, const hamx_fields *fields,
                       hamx_maps *out, uint32_t flags);

@@ -79,8 +79,10 @@ class Hampyx:
         the descriptor's CGS / radian units that every pass reads (created with
         requires_grad from the parameter file when not given); grids is
         (bx, by, bz, te) of the random fields on the device, requiring grad
-        when grids_grad.  Models: LSA regular B, uniform TE, analytic or
-        uniform CRE with a constant index, random grids on one box."""
+        when grids_grad.  Models: any regular B and TE model (the gradients
+        of b0, psi0, psi1, chi0 are zero unless B is LSA, n0's unless TE is
+        the uniform sphere), a mask, the analytic or uniform CRE with a
+        constant index, random grids on one box."""
         from .autograd import desc_params, los_maps
         text = et.tostring(self.tree.getroot(), encoding="unicode")
         if self._ctx is None:
@@ -94,8 +96,8 @@ class Hampyx:
         pipe.assemble_brnd()
         pipe.assemble_cre()
         pipe.load_mask()
-        if getattr(pipe, "mask", None) is not None or getattr(pipe, "cre_grid", None) is not None:
-            raise ValueError("differentiable(): masks and CRE grids have no adjoint")
+        if getattr(pipe, "cre_grid", None) is not None:
+            raise ValueError("differentiable(): the CRE grid has no adjoint")
         g = pipe.par.grid_obs
         if not g.do_sync:
             raise ValueError("differentiable(): the adjoint needs a <sync> output")
@@ -109,6 +111,9 @@ class Hampyx:
             te = te.detach().clone().requires_grad_(True) if te is not None else None
         kw = dict(brnd=brnd, brnd_box=pipe.par.grid_brnd.box() if brnd else None,
                   ternd=te, ternd_box=pipe.par.grid_ternd.box() if te is not None else None)
+        # the mask and the regular-field grids go in as they are
+        kw.update({k: v for k, v in pipe._fields().items()
+                   if k in ("mask", "nside_mask", "breg", "breg_box", "tereg", "tereg_box")})
         out = {}
         n = len(g.do_sync)
         for p in range(n):

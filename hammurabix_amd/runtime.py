@@ -69,6 +69,39 @@ def box(nx: int, ny: int, nz: int, x_min: float, x_max: float, y_min: float,
     return capi.HamxBox(nx, ny, nz, x_min, x_max, y_min, y_max, z_min, z_max)
 
 
+def _fields(brnd, brnd_box, ternd, ternd_box, breg, breg_box, tereg, tereg_box, mask,
+            nside_mask, cre, cre_box, cre_E):
+    """hamx_fields of a LOS call and the arrays it points at."""
+    f = capi.HamxFields()
+    inputs = []
+    if cre is not None:
+        f.cre = _ptr(cre)[0]
+        f.cre_box = cre_box
+        f.cre_E_min, f.cre_E_fact, f.cre_nE = cre_E
+        inputs.append(cre)
+    if mask is not None:
+        f.mask = _ptr(mask)[0]
+        f.nside_mask = nside_mask
+        inputs.append(mask)
+    if brnd is not None:
+        f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
+        f.brnd_box = brnd_box
+        inputs += list(brnd)
+    if ternd is not None:
+        f.ternd = _ptr(ternd)[0]
+        f.ternd_box = ternd_box
+        inputs.append(ternd)
+    if breg is not None:
+        f.breg_x, f.breg_y, f.breg_z = (_ptr(a)[0] for a in breg)
+        f.breg_box = breg_box
+        inputs += list(breg)
+    if tereg is not None:
+        f.tereg = _ptr(tereg)[0]
+        f.tereg_box = tereg_box
+        inputs.append(tereg)
+    return f, inputs
+
+
 class Context:
     """One HIP stream plus the library's device workspace on `device`."""
 
@@ -135,33 +168,8 @@ class Context:
         duplicate at each shell's nside is 1.0; cre is the CRE flux grid of
         HAMX_CRE_GRID on cre_box with the energy axis cre_E = (E_min, E_fact,
         nE), stored [x][y][z][E]."""
-        f = capi.HamxFields()
-        inputs = []
-        if cre is not None:
-            f.cre = _ptr(cre)[0]
-            f.cre_box = cre_box
-            f.cre_E_min, f.cre_E_fact, f.cre_nE = cre_E
-            inputs.append(cre)
-        if mask is not None:
-            f.mask = _ptr(mask)[0]
-            f.nside_mask = nside_mask
-            inputs.append(mask)
-        if brnd is not None:
-            f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
-            f.brnd_box = brnd_box
-            inputs += list(brnd)
-        if ternd is not None:
-            f.ternd = _ptr(ternd)[0]
-            f.ternd_box = ternd_box
-            inputs.append(ternd)
-        if breg is not None:
-            f.breg_x, f.breg_y, f.breg_z = (_ptr(a)[0] for a in breg)
-            f.breg_box = breg_box
-            inputs += list(breg)
-        if tereg is not None:
-            f.tereg = _ptr(tereg)[0]
-            f.tereg_box = tereg_box
-            inputs.append(tereg)
+        f, inputs = _fields(brnd, brnd_box, ternd, ternd_box, breg, breg_box, tereg, tereg_box,
+                            mask, nside_mask, cre, cre_box, cre_E)
         outs = [maps.get(k) for k in ("dm", "fd", "is", "qs", "us")]
         m = capi.HamxMaps(*[_ptr(a)[0] for a in outs])
         flags = 0
@@ -180,22 +188,20 @@ class Context:
                     brnd_box: Optional[capi.HamxBox] = None,
                     ternd: Any = None, ternd_box: Optional[capi.HamxBox] = None,
                     dbrnd: Optional[tuple[Any, Any, Any]] = None,
-                    dternd: Any = None, use_tape: bool = False) -> None:
+                    dternd: Any = None, use_tape: bool = False,
+                    breg: Optional[tuple[Any, Any, Any]] = None,
+                    breg_box: Optional[capi.HamxBox] = None,
+                    tereg: Any = None, tereg_box: Optional[capi.HamxBox] = None,
+                    mask: Any = None, nside_mask: int = 0) -> None:
         """Reverse mode of los_forward: `adj` holds dL/d(maps) (keys dm, fd, is,
         qs, us; missing maps are zero); writes dL/d(b0, psi0, psi1, chi0,
         alpha, n0) into `dparams` and dL/d(grid) into `dbrnd` / `dternd`.
-        use_tape takes the tape of the last los_forward(keep_tape=True) with the
-        same descriptor and arrays instead of re-running the forward."""
-        f = capi.HamxFields()
-        inputs = [adj.get(k) for k in ("dm", "fd", "is", "qs", "us")]
-        if brnd is not None:
-            f.brnd_x, f.brnd_y, f.brnd_z = (_ptr(a)[0] for a in brnd)
-            f.brnd_box = brnd_box
-            inputs += list(brnd)
-        if ternd is not None:
-            f.ternd = _ptr(ternd)[0]
-            f.ternd_box = ternd_box
-            inputs.append(ternd)
+        The field arguments are los_forward's.  use_tape takes the tape of the
+        last los_forward(keep_tape=True) with the same descriptor and arrays
+        instead of re-running the forward."""
+        f, fin = _fields(brnd, brnd_box, ternd, ternd_box, breg, breg_box, tereg, tereg_box,
+                         mask, nside_mask, None, None, None)
+        inputs = [adj.get(k) for k in ("dm", "fd", "is", "qs", "us")] + fin
         m = capi.HamxMaps(*[_ptr(a)[0] for a in inputs[:5]])
         outs = [dparams] + (list(dbrnd) if dbrnd is not None else []) + [dternd]
         flags = 0

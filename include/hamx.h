@@ -287,7 +287,10 @@ int hamx_los_forward_static(hamx_ctx *ctx, const hamx_los_desc *d,
  * shell): with HAMX_USE_TAPE it takes the one a hamx_los_forward call with
  * HAMX_KEEP_TAPE left in the context, otherwise it re-runs the forward to
  * record it.  Shells are then processed outermost first, each ray in one
- * sweep from its far end (running FD = ray total minus the samples beyond).  Supported models: LSA regular B, uniform TE,
+ * sweep from its far end (running FD = ray total minus the samples beyond).
+ * Supported: every regular B model and TE model of hamx_los_forward
+ * (dL/d(b0, psi0, psi1, chi0) is zero unless B is LSA, dL/d(n0) unless TE
+ * is the uniform sphere), a mask (masked-out rays contribute nothing), the
  * analytic or uniform CRE with a constant spectral index, synchrotron on,
  * random grids (if any) sharing one box.  No counterpart in the reference,
  * which has no gradients.  flags as for hamx_los_forward (inputs: fields

@@ -218,16 +218,6 @@ def test_linearity_in_the_adjoint_maps(ctx):
         np.testing.assert_allclose(c, a + 2 * b_, rtol=1e-9, atol=1e-12 * np.abs(c).max())
 
 
-def test_adjoint_rejects_unsupported_models(ctx):
-    d = Param(text=configs.plain_forward(nside_sim=4, nside_out=4)).los_desc()
-    d.breg_model = capi.HAMX_BREG_CART
-    d.breg[0:3] = [1e-6, 0, 0]
-    w = weights(d, 0)
-    with pytest.raises(HamxError) as e:
-        ctx.los_adjoint(d, w, np.zeros(capi.HAMX_NPARAM))
-    assert e.value.code == capi.HAMX_ERR_UNSUPPORTED
-
-
 def test_autograd_function_matches_c_abi_and_fd(ctx):
     """torch.autograd over the adjoint: the gradient of a Gaussian
     log-likelihood of Q/U and FD maps equals the C-ABI adjoint with
