@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <numeric>
 #include <string>
@@ -1326,6 +1327,12 @@ struct AccJob {
   int n, pad;
   double *dst[kAccMaps];
   const double *src[kAccMaps];
+  // transposed interpolation (k_batch_acc_t), CSR by pixel of the nside of
+  // hs: t_row[j]..t_row[j+1] index the pixels t_col of the nside of hd that
+  // interpolate from j, with coefficients t_coef = w / (sum of their 4 w);
+  // NULL: scatter with atomics
+  const uint32_t *t_row, *t_col;
+  const double *t_coef;
 };
 
 // dst += src (same nside) or dst += interpolate(src) at dst's pixel
@@ -1361,17 +1368,46 @@ __global__ __launch_bounds__(256) void k_batch_acc(const AccJob *jobs) {
 }
 
 // dst (shell adjoint, nside_s) += A^T(src) (output adjoint): the transpose
-// of k_batch_acc, with atomic scatter for the interpolating case.
+// of k_batch_acc.  Interpolating jobs with the transposed matrix gather per
+// shell pixel (each row of the transpose holds the output pixels that
+// interpolate from it, so no two threads write one value); without it they
+// scatter with atomics.  One job per blockIdx.y, whose records are read
+// through the constant address space (written before the launch).
 __global__ __launch_bounds__(256) void k_batch_acc_t(const AccJob *jobs) {
-  const AccJob &J = jobs[blockIdx.y];
+  using ConstJob = const __attribute__((address_space(4))) AccJob;
+  const AccJob &J = *(const AccJob *)((ConstJob *)jobs + blockIdx.y);
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= J.hd.npix)
-    return;
   if (J.hd.npix == J.hs.npix) {
+    if (i >= J.hd.npix)
+      return;
     for (int m = 0; m < J.n; ++m)
       J.dst[m][i] += J.src[m][i];
     return;
   }
+  if (J.t_row) {
+    if (i >= J.hs.npix)
+      return;
+    double acc[kAccMaps];
+#pragma unroll
+    for (int m = 0; m < kAccMaps; ++m)
+      acc[m] = 0.;
+    const uint32_t e1 = J.t_row[i + 1];
+    for (uint32_t e = J.t_row[i]; e < e1; ++e) {
+      const double c = J.t_coef[e];
+      const uint32_t q = J.t_col[e];
+#pragma unroll
+      for (int m = 0; m < kAccMaps; ++m)
+        if (m < J.n)
+          acc[m] += c * J.src[m][q];
+    }
+#pragma unroll
+    for (int m = 0; m < kAccMaps; ++m)
+      if (m < J.n)
+        J.dst[m][i] += acc[m];
+    return;
+  }
+  if (i >= J.hd.npix)
+    return;
   double theta, phi;
   hp_pix2ang(J.hd, i, theta, phi);
   uint64_t pix[4];
@@ -1383,6 +1419,26 @@ __global__ __launch_bounds__(256) void k_batch_acc_t(const AccJob *jobs) {
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       unsafeAtomicAdd(&J.dst[m][pix[q]], w[q] * g);
+  }
+}
+
+// The interpolation of k_batch_acc from nside(hs) to every pixel of
+// nside(hd): the 4 source pixels and w / (sum of the 4 w) per pixel.
+__global__ __launch_bounds__(256) void k_interp_coef(Healpix hd, Healpix hs, uint32_t *pix,
+                                                     double *coef) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= hd.npix)
+    return;
+  double theta, phi;
+  hp_pix2ang(hd, i, theta, phi);
+  uint64_t p[4];
+  double w[4];
+  hp_interpol_weights(hs, theta, phi, p, w);
+  const double wtot = ((w[0] + w[1]) + w[2]) + w[3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    pix[4 * i + q] = static_cast<uint32_t>(p[q]);
+    coef[4 * i + q] = w[q] / wtot;
   }
 }
 
@@ -1640,6 +1696,13 @@ struct hamx_batch {
   };
   Jobs a_init_fill, a_init_copy, a_red, a_unpack;
   std::vector<Jobs> a_zero, a_acct, a_fdt, f_ckpt; // per level
+  // transposed interpolations by (nside of the output, nside of the shell):
+  // CSR row pointers, columns and coefficients (see AccJob)
+  struct Transpose {
+    const uint32_t *row, *col;
+    const double *coef;
+  };
+  std::map<std::pair<uint64_t, uint64_t>, Transpose> transposes;
   hamx_batch_info info{};
 };
 
@@ -1665,6 +1728,75 @@ int upload(hamx_batch *b, const std::vector<T> &v, T **dptr) {
   if (!v.empty())
     HAMX_HIP(b->ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   *dptr = static_cast<T *>(p);
+  return HAMX_OK;
+}
+
+// The transpose of k_batch_acc's interpolation from nside(hs) to nside(hd),
+// as CSR by pixel of nside(hs) with the output pixels in increasing order
+// (so every replay sums in the same order), built once per nside pair of the
+// plan: the 4 pixels and coefficients per output pixel on the device, the
+// counting sort on the host.  Output maps beyond 2^26 pixels (nside > 2048)
+// keep the atomic scatter.
+int interp_transpose(hamx_batch *b, const Healpix &hd, const Healpix &hs, AccJob &j) {
+  j.t_row = j.t_col = nullptr;
+  j.t_coef = nullptr;
+  if (hd.npix == hs.npix || hd.npix > (1ull << 26))
+    return HAMX_OK;
+  const auto key = std::make_pair(hd.nside, hs.nside);
+  auto it = b->transposes.find(key);
+  if (it == b->transposes.end()) {
+    hamx_ctx *ctx = b->ctx;
+    const uint64_t nd = hd.npix, ns = hs.npix;
+    uint32_t *dpix = nullptr;
+    double *dcoef = nullptr;
+    if (hipMalloc(&dpix, 4 * nd * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&dcoef, 4 * nd * sizeof(double)) != hipSuccess) {
+      (void)hipFree(dpix);
+      return set_err(ctx, HAMX_ERR_NOMEM, "batch: transposed interpolation buffers");
+    }
+    hipLaunchKernelGGL(k_interp_coef, dim3(static_cast<unsigned>((nd + 255) / 256)), dim3(256),
+                       0, ctx->stream, hd, hs, dpix, dcoef);
+    std::vector<uint32_t> pix(4 * nd);
+    std::vector<double> coef(4 * nd);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy(pix.data(), dpix, pix.size() * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(coef.data(), dcoef, coef.size() * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dpix);
+    (void)hipFree(dcoef);
+    HAMX_HIP(ctx, e);
+    std::vector<uint32_t> row(ns + 1, 0), col(4 * nd);
+    std::vector<double> val(4 * nd);
+    for (uint64_t k = 0; k < 4 * nd; ++k) {
+      if (pix[k] >= ns)
+        return set_err(ctx, HAMX_ERR_DEVICE, "batch: interpolation pixel out of range");
+      ++row[pix[k] + 1];
+    }
+    for (uint64_t q = 0; q < ns; ++q)
+      row[q + 1] += row[q];
+    std::vector<uint32_t> fill(row.begin(), row.end() - 1);
+    for (uint64_t k = 0; k < 4 * nd; ++k) {
+      const uint32_t at = fill[pix[k]]++;
+      col[at] = static_cast<uint32_t>(k / 4);
+      val[at] = coef[k];
+    }
+    hamx_batch::Transpose t;
+    uint32_t *dr, *dc;
+    double *dv;
+    HAMX_TRY(upload(b, row, &dr));
+    HAMX_TRY(upload(b, col, &dc));
+    HAMX_TRY(upload(b, val, &dv));
+    t.row = dr;
+    t.col = dc;
+    t.coef = dv;
+    it = b->transposes.emplace(key, t).first;
+  }
+  j.t_row = it->second.row;
+  j.t_col = it->second.col;
+  j.t_coef = it->second.coef;
   return HAMX_OK;
 }
 
@@ -2316,8 +2448,10 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
           }
         }
         if (t.n) {
+          if ((rc = interp_transpose(b, t.hd, t.hs, t)) != HAMX_OK)
+            return fail(rc);
           ats.push_back(t);
-          at_max = std::max(at_max, t.hd.npix);
+          at_max = std::max({at_max, t.hd.npix, t.hs.npix});
         }
         if (in.do_dm) {
           zs.push_back(FillJob{A.g_dm, hs.npix});
@@ -2329,8 +2463,10 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
             j.n = 1;
             j.dst[0] = A.g_dm;
             j.src[0] = x.adj_dm;
+            if ((rc = interp_transpose(b, j.hd, j.hs, j)) != HAMX_OK)
+              return fail(rc);
             ats.push_back(j);
-            at_max = std::max(at_max, j.hd.npix);
+            at_max = std::max({at_max, j.hd.npix, j.hs.npix});
           }
         }
         if (in.do_fd) {
@@ -2342,8 +2478,10 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
           j.n = 1;
           j.dst[0] = A.g_fd;
           j.src[0] = A.G;
+          if ((rc = interp_transpose(b, j.hd, j.hs, j)) != HAMX_OK)
+            return fail(rc);
           ats.push_back(j);
-          at_max = std::max(at_max, j.hd.npix);
+          at_max = std::max({at_max, j.hd.npix, j.hs.npix});
           if (s > 0) {
             fts.push_back(FdtJob{A.G, A.gfdin, in.hfd, hs});
             ft_max = std::max(ft_max, hs.npix);
