@@ -407,9 +407,11 @@ __global__ __launch_bounds__(256) void k_es_gs_pairs(double2 *c0, double2 *c1, E
   }
 }
 
-// Hermitian extraction / N (brnd_es.cc:274-307)
 // Hermitian extraction of the real field (brnd_es.cc:274-307 with
-// scale = 1/N; brnd_mhd.cc:139-167 with scale = 1).
+// scale = 1/N; brnd_mhd.cc:139-167 with scale = 1): the real part of the
+// symmetrised pair, which is the same sum at index and mirror, so the
+// thread of the lower index of each mirror pair reads the two elements once
+// and writes both.
 __global__ __launch_bounds__(256) void k_es_final(const double2 *c0, const double2 *c1,
                                                   double *bx, double *by, double *bz,
                                                   BoxArgs b, double inv_n) {
@@ -417,10 +419,20 @@ __global__ __launch_bounds__(256) void k_es_final(const double2 *c0, const doubl
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < b.n; idx += stride) {
     const uint64_t sym = sym_index(b, idx);
+    if (sym < idx)
+      continue; // the pair belongs to the thread of sym
     const double2 p0 = c0[idx], q0 = c0[sym], p1 = c1[idx], q1 = c1[sym];
-    bx[idx] = 0.5 * (p0.x + q0.x) * inv_n;
-    by[idx] = 0.5 * (p1.x + q1.x) * inv_n;
-    bz[idx] = 0.5 * (p1.y + q1.y) * inv_n;
+    const double vx = 0.5 * (p0.x + q0.x) * inv_n;
+    const double vy = 0.5 * (p1.x + q1.x) * inv_n;
+    const double vz = 0.5 * (p1.y + q1.y) * inv_n;
+    bx[idx] = vx;
+    by[idx] = vy;
+    bz[idx] = vz;
+    if (sym != idx) {
+      bx[sym] = vx;
+      by[sym] = vy;
+      bz[sym] = vz;
+    }
   }
 }
 
