@@ -318,3 +318,82 @@ def test_large_full_size_properties(ctx):
     assert all(bool(torch.isfinite(a).all()) for a in g["dbrnd"])
     assert bool(torch.isfinite(g["dternd"]).all())
     assert float(g["dbrnd"][0].abs().max()) > 0
+
+
+# ------------------------------------------------- batch gradients, full sky
+
+# per-parameter bound of the FP32 batch on this likelihood (b0, psi0, psi1,
+# chi0, alpha, n0): the psi terms are small differences of large sums whose
+# per-sample FP32 rounding the low-frequency Faraday phases amplify; the
+# FP64 batch is the accurate mode
+F32_PARAM_TOL = (1e-3, 2e-2, 2e-1, 2e-3, 1e-5, 3e-3)
+
+
+def loglike_seeds(ctx, b, pars, specs, grids_of):
+    """dL/d(maps) of L = -1/2 sum (m - obs)^2 / sigma^2 over the Q/U maps of
+    every frequency and the FD map: obs is the batch's forward on a held-out
+    realisation (seed + 10000) with 10% Gaussian noise, m its forward on the
+    instance's own realisation."""
+    torch, dev = torch_dev()
+    keys = lambda sp: [(k, f) for k in ("qs", "us") for f in range(len(sp["freqs"]))] + [("fd", None)]  # noqa: E731
+    pick = lambda m, k, f: m[k][f] if f is not None else m[k]  # noqa: E731
+    for par, sp in zip(pars, specs):
+        grids_of(par, sp, par.brnd_seed + 10000)
+    b.run()
+    torch.cuda.synchronize()
+    obs = [{(k, f): pick(sp["maps"], k, f).clone() for k, f in keys(sp)} for sp in specs]
+    for par, sp in zip(pars, specs):
+        grids_of(par, sp, par.brnd_seed)
+    b.run()
+    torch.cuda.synchronize()
+    gen = torch.Generator(device=dev).manual_seed(20001)
+    for sp, ob in zip(specs, obs):
+        for t in sp["adj"]["is"]:
+            t.zero_()
+        sp["adj"]["dm"].zero_()
+        for (k, f), o in ob.items():
+            sigma = 0.1 * float(o.square().mean().sqrt())
+            o += sigma * torch.randn(o.shape[0], dtype=torch.float64, device=dev, generator=gen)
+            pick(sp["adj"], k, f).copy_((o - pick(sp["maps"], k, f)) / (sigma * sigma))
+
+
+@pytest.mark.parametrize("prec", [capi.HAMX_PREC_F64, capi.HAMX_PREC_F32])
+def test_batch_parameter_gradients_full_sky(ctx, prec):
+    """The 64-instance batch's first 6 instances (top nside 64/128/256, 1-4
+    frequencies) at full sky, with the bench's likelihood seeds: the batched
+    reverse mode against the plain FP64 adjoint summed over the pipeline's
+    passes, per parameter (FP64: 1e-9; FP32: the bounds above)."""
+    torch, dev = torch_dev()
+    print(f"case: batch parameter gradients, full sky, precision {prec}", flush=True)
+    pars = [Param(text=configs.batch_instance(i)) for i in range(6)]
+    specs = []
+    for par in pars:
+        b, te = device_grids(ctx, par, par.brnd_seed)
+        specs.append(batch_spec(par, par.grid_obs.sim_sync_freq, b, te, with_adj=True))
+
+    def grids_of(par, sp, seed):
+        ctx.brnd_es(par.es_desc(), seed, *sp["brnd"])
+        ctx.ternd_dft(par.dft_desc(), seed, sp["ternd"])
+
+    with ctx.batch(specs, precision=prec, tune=False, adjoint=True) as bt:
+        loglike_seeds(ctx, bt, pars, specs, grids_of)
+        bt.run()
+        bt.run_adjoint()
+        torch.cuda.synchronize()
+        for i, sp in enumerate(specs):
+            got = host(sp["grads"]["dparams"])
+            dp = np.zeros(capi.HAMX_NPARAM)
+            kw = dict(brnd=sp["brnd"], brnd_box=sp["box"], ternd=sp["ternd"],
+                      ternd_box=sp["box"])
+            for p, fi, d in pipeline_passes(sp["desc"], sp["freqs"]):
+                a = {k: sp["adj"][k][fi] for k in ("is", "qs", "us")}
+                if p == 0:
+                    a["dm"], a["fd"] = sp["adj"]["dm"], sp["adj"]["fd"]
+                g = torch.zeros(capi.HAMX_NPARAM, dtype=torch.float64, device=dev)
+                ctx.los_adjoint(d, a, g, **kw)
+                dp += host(g)
+            rel = np.abs(got - dp) / np.abs(dp)
+            print(f"prec{prec} inst{i} dparams rel {np.array2string(rel, precision=2)}",
+                  flush=True)
+            tol = np.full(6, 1e-9) if prec == capi.HAMX_PREC_F64 else np.array(F32_PARAM_TOL)
+            assert np.all(rel <= tol), (i, rel)
