@@ -15,10 +15,13 @@
 //   (tabulated once per length and sign), and an M-point inverse transform,
 //   all in LDS; the outputs are multiplied by the chirp and 1/M.
 //
-// Global access: along the contiguous z axis each line is a contiguous run;
-// along y and x the tile's C lines are C adjacent columns, so every tile row
-// is C*16 contiguous bytes (C = 16 -> 256 B rows).  Twiddles and chirps come
-// from per-length tables computed on the host in long double.
+// Tiles: 2048 values (32 KB of LDS, 5 waves per SIMD) for lines of up to
+// 2048 points, 4096 and 8192 for longer ones.  Global access: along the
+// contiguous z axis each line is a contiguous run; along y and x the tile's
+// C lines are C adjacent columns, so every tile row is C*16 contiguous bytes
+// (C = 8 -> 128 B rows for 256-point lines).  Twiddles and chirps come from
+// per-length tables computed on the host in long double; the power-of-two
+// kernels of the 2048 tile copy theirs into LDS behind the tile.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
