@@ -189,8 +189,11 @@ __device__ inline uint64_t line_base(const FftAxis &ax, uint64_t L) {
 }
 
 template <int SIGN, uint32_t TILE, bool POW2>
-__global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data, FftAxis ax,
+__global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
+                                                  double2 *__restrict__ data1, FftAxis ax,
                                                   const double2 *__restrict__ tw) {
+  // blockIdx.y picks one of two arrays transformed in the same launch
+  double2 *__restrict__ data = blockIdx.y ? data1 : data0;
   // power-of-two lines in the mini tile: the twiddle table is copied into
   // LDS after the tile (dynamic LDS of C n + n values), so the passes read
   // it with LDS latency (the mixed-radix kernels keep it in memory: their
@@ -230,11 +233,13 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data, Ff
 // M-point forward transform of conj(chirp) laid out circularly; twm the table
 // of exp(2 pi i m / M).
 template <int SIGN, uint32_t TILE>
-__global__ __launch_bounds__(256) void k_fft_axis_bluestein(double2 *__restrict__ data,
+__global__ __launch_bounds__(256) void k_fft_axis_bluestein(double2 *__restrict__ data0,
+                                                            double2 *__restrict__ data1,
                                                             FftAxis ax,
                                                             const double2 *__restrict__ twm,
                                                             const double2 *__restrict__ chirp,
                                                             const double2 *__restrict__ bhat) {
+  double2 *__restrict__ data = blockIdx.y ? data1 : data0;
   __shared__ double2 lds[TILE];
   const uint32_t n = ax.n, M = ax.M, C = ax.C, total = C * M;
   const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * C;
@@ -412,6 +417,12 @@ bool fft_length_ok(uint64_t n) {
 
 int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
               uint64_t nz, int sign) {
+  return fft3d_dev2(ctx, data, nullptr, nx, ny, nz, sign);
+}
+
+int fft3d_dev2(hamx_ctx *ctx, double2 *data, double2 *data2, uint64_t nx, uint64_t ny,
+               uint64_t nz, int sign) {
+  const unsigned arrays = data2 ? 2u : 1u;
   const uint64_t dims[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a)
     if (!fft_length_ok(dims[a]))
@@ -466,10 +477,10 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
       HAMX_TRY(twiddle_table(ctx, n, &tw));
       const bool pow2 = is_pow2(n);
 #define HAMX_FFT_LAUNCH(S, T, P)                                                           \
-  hipLaunchKernelGGL((k_fft_axis<S, T, P>), dim3(blocks), dim3(256),                       \
+  hipLaunchKernelGGL((k_fft_axis<S, T, P>), dim3(blocks, arrays), dim3(256),               \
                      (P) && (T) <= kTileMini ? (ax.C + 1) * ax.n * sizeof(double2) : 0,        \
                      ctx->stream,                                                          \
-                     data, ax, tw)
+                     data, data2, ax, tw)
 #define HAMX_FFT_LAUNCH_T(S, T)                                                            \
   do {                                                                                     \
     if (pow2)                                                                              \
@@ -498,8 +509,8 @@ int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
       HAMX_TRY(twiddle_table(ctx, ax.M, &twm));
       HAMX_TRY(bluestein_tables(ctx, n, ax.M, sign, &chirp, &bhat));
 #define HAMX_FFT_LAUNCH(S, T)                                                              \
-  hipLaunchKernelGGL((k_fft_axis_bluestein<S, T>), dim3(blocks), dim3(256), 0, ctx->stream, \
-                     data, ax, twm, chirp, bhat)
+  hipLaunchKernelGGL((k_fft_axis_bluestein<S, T>), dim3(blocks, arrays), dim3(256), 0,     \
+                     ctx->stream, data, data2, ax, twm, chirp, bhat)
 #define HAMX_FFT_LAUNCH_S(S)                                                               \
   do {                                                                                     \
     if (tile == kTileMini)                                                                 \

@@ -148,6 +148,10 @@ bool fft_length_ok(uint64_t n);
 // In-place unnormalised 3D c2c FFT of device data (hamx_fft.hip).
 int fft3d_dev(hamx_ctx *ctx, double2 *data, uint64_t nx, uint64_t ny,
               uint64_t nz, int sign);
+// The same transform of two arrays of one shape, both in each axis launch
+// (data2 may be NULL).
+int fft3d_dev2(hamx_ctx *ctx, double2 *data, double2 *data2, uint64_t nx, uint64_t ny,
+               uint64_t nz, int sign);
 // Free the per-context twiddle tables (called by hamx_close).
 void release_twiddles(hamx_ctx *ctx);
 // Look up, once per device, the device address of hamx_los_forward_static's

@@ -612,8 +612,7 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
   const unsigned blocks = ew_blocks(n);
   hipLaunchKernelGGL(k_es_draw, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a, seed);
   HAMX_HIP(ctx, hipGetLastError());
-  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
-  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev2(ctx, c0, c1, a.b.nx, a.b.ny, a.b.nz, +1));
   HAMX_TRY(variance_dev(ctx, reinterpret_cast<const double *>(c0), n, red));
   const bool heavy = a.jf12 || a.breg_model == HAMX_BREG_GRID ||
                      a.breg_model == HAMX_BREG_JF12 || a.breg_model == HAMX_BREG_JAFFE;
@@ -624,12 +623,10 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
     hipLaunchKernelGGL(k_es_rescale<false>, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1,
                        a, static_cast<const double *>(red + 1));
   HAMX_HIP(ctx, hipGetLastError());
-  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, -1));
-  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, -1));
+  HAMX_TRY(fft3d_dev2(ctx, c0, c1, a.b.nx, a.b.ny, a.b.nz, -1));
   hipLaunchKernelGGL(k_es_gs_pairs, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a);
   HAMX_HIP(ctx, hipGetLastError());
-  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
-  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev2(ctx, c0, c1, a.b.nx, a.b.ny, a.b.nz, +1));
   hipLaunchKernelGGL(k_es_final, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox,
                      oy, oz, a.b, 1. / static_cast<double>(n));
   HAMX_HIP(ctx, hipGetLastError());
@@ -865,8 +862,7 @@ int brnd_mhd(hamx_ctx *ctx, const hamx_mhd_desc *d, uint64_t seed, double *bx, d
   hipLaunchKernelGGL(k_mhd_draw, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, a, seed,
                      static_cast<const double *>(bobs));
   HAMX_HIP(ctx, hipGetLastError());
-  HAMX_TRY(fft3d_dev(ctx, c0, a.b.nx, a.b.ny, a.b.nz, +1));
-  HAMX_TRY(fft3d_dev(ctx, c1, a.b.nx, a.b.ny, a.b.nz, +1));
+  HAMX_TRY(fft3d_dev2(ctx, c0, c1, a.b.nx, a.b.ny, a.b.nz, +1));
   hipLaunchKernelGGL(k_es_final, dim3(blocks), dim3(256), 0, ctx->stream, c0, c1, ox, oy, oz,
                      a.b, 1.);
   HAMX_HIP(ctx, hipGetLastError());
