@@ -69,6 +69,10 @@ struct BShell {
 struct F32Const {
   double b0d, n0d; // b0 [muG] and n0 for the FD product, kept in FP64
   float psi0, psi1, chi0, b0u;
+  // the LSA angles psi(r) and chi(z) stay within [-pi/4, pi/4] over the
+  // field's window (checked on the host): their sines and cosines are then
+  // the series alone, with no quadrant reduction
+  int lsa_direct;
   float n0;
   float R0k, inv_r0k, inv_z0k;
   float hexp; // (1 - pw_e) / 2: the emissivity shape is prof * (B_perp^2)^hexp
@@ -247,22 +251,29 @@ __device__ inline void lerp_plane_f4(const float4 *__restrict__ f, uint64_t i0, 
   w.w = (a0.w * z1 + a1.w * zd) * y1 + (b0.w * z1 + b1.w * zd) * yd;
 }
 
-// sin and cos of an FP32 angle: quadrant reduction k = rint(2x/pi) with
-// the remainder x - k pi/2 taken in FP64 (exact for any angle a field model
-// produces), then the Taylor series of sin and cos on [-pi/4, pi/4] to the
-// terms r^9 and r^10 (truncation below 2e-9 there).
-__device__ inline void sincos_f32(float x, float &sn, float &cs) {
+// sin and cos of an FP32 angle |r| <= pi/4: the Taylor series to the terms
+// r^9 and r^10 (truncation below 2e-9 there; relative accuracy also for
+// small angles, where the hardware v_sin_f32's absolute error of ~1e-7 is a
+// large relative one, and smooth along a ray, so it would add up in the FD).
+__device__ inline void sincos_poly(float r, float &sn, float &cs) {
   // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
   #pragma clang fp contract(fast)
+  const float r2 = r * r;
+  sn = r * (1.f + r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f +
+                                                               r2 * (1.f / 362880.f)))));
+  cs = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f +
+                                                                       r2 * (-1.f / 3628800.f)))));
+}
+
+// sin and cos of any FP32 angle: quadrant reduction k = rint(2x/pi) with
+// the remainder x - k pi/2 taken in FP64 (exact for any angle a field model
+// produces), then sincos_poly on the remainder.
+__device__ inline void sincos_f32(float x, float &sn, float &cs) {
   const double xd = static_cast<double>(x);
   const double k = rint(xd * 0.63661977236758134308); // 2/pi
   const float r = static_cast<float>(xd - k * 1.57079632679489661923);
-  const float r2 = r * r;
-  const float s = r * (1.f + r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f +
-                                                                     r2 * (1.f / 362880.f)))));
-  const float c =
-      1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f +
-                                                                   r2 * (-1.f / 3628800.f)))));
+  float s, c;
+  sincos_poly(r, s, c);
   const int q = static_cast<int>(static_cast<int64_t>(k) & 3);
   sn = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
   cs = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
@@ -276,19 +287,19 @@ __device__ inline bool grid_cell_scaled(const BInst &in, const Vec3 &p, Cell &c,
   double t = (p.x - in.gx0) * in.gsx;
   if (!(t > 0 && t < g.fx))
     return false;
-  const uint64_t xl = static_cast<uint64_t>(floor(t));
+  const uint32_t xl = static_cast<uint32_t>(t); // t > 0: truncation is the floor
   c.xd = t - static_cast<double>(xl);
   t = (p.y - in.gy0) * in.gsy;
   if (!(t > 0 && t < g.fy))
     return false;
-  const uint64_t yl = static_cast<uint64_t>(floor(t));
+  const uint32_t yl = static_cast<uint32_t>(t);
   c.yd = t - static_cast<double>(yl);
   t = (p.z - in.gz0) * in.gsz;
   if (!(t > 0 && t < g.fz))
     return false;
-  const uint64_t zl = static_cast<uint64_t>(floor(t));
+  const uint32_t zl = static_cast<uint32_t>(t);
   c.zd = t - static_cast<double>(zl);
-  c.idx = (xl * g.ny + yl) * g.nz + zl;
+  c.idx = static_cast<uint64_t>(xl) * g.nyz + (yl * static_cast<uint32_t>(g.nz) + zl);
   cx = static_cast<int>(xl);
   cy = static_cast<int>(yl);
   cz = static_cast<int>(zl);
@@ -309,7 +320,8 @@ struct Geo64 {
 __device__ inline bool geo_f64(const BInst &in, const BShell &sh, const RayGeo &r, uint64_t k,
                                Geo64 &g) {
   const Phys &ph = in.ph;
-  const double dist = sh.d_start + (static_cast<double>(k) + 0.5) * sh.delta;
+  const double dist =
+      sh.d_start + (static_cast<double>(static_cast<uint32_t>(k)) + 0.5) * sh.delta;
   g.pos = Vec3{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
   const Vec3 &pos = g.pos;
   const double len2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
@@ -354,15 +366,22 @@ __device__ inline bool eval_f32(const BInst &in, const BShell &sh, const RayGeo 
     const float inv_r = __builtin_amdgcn_rcpf(rk);
     o.cos_p = xk * inv_r;
     o.sin_p = yk * inv_r;
-    // accurate (not the hardware-approximate) functions: the field's
-    // systematic error feeds the FD, and the Q/U phase at low frequency
-    o.lnr = logf(rk * 0.125f);
-    const float e2 = expf(-2.f * fabsf(zk));
+    // ln(r / 8 kpc) and exp(-2|z|) from the hardware v_log_f32 (log2) and
+    // v_exp_f32 (exp2), within 1 ulp: an absolute error of 1e-7 in ln r
+    // moves psi by psi1 * 1e-7, and the rounding of the exp2 argument moves
+    // tanh by less than 2e-8
+    o.lnr = __builtin_amdgcn_logf(rk * 0.125f) * 0.693147181f;
+    const float e2 = __builtin_amdgcn_exp2f(fabsf(zk) * -2.88539008f);
     o.tnh = copysignf((1.f - e2) * __builtin_amdgcn_rcpf(1.f + e2), zk);
     const float psi = c.psi0 + c.psi1 * o.lnr;
     const float chi = c.chi0 * o.tnh;
-    sincos_f32(psi, o.sps, o.cps);
-    sincos_f32(chi, o.sch, o.cch);
+    if (c.lsa_direct) {
+      sincos_poly(psi, o.sps, o.cps);
+      sincos_poly(chi, o.sch, o.cch);
+    } else {
+      sincos_f32(psi, o.sps, o.cps);
+      sincos_f32(chi, o.sch, o.cch);
+    }
     const float ux = o.cch * (o.sps * o.cos_p - o.cps * o.sin_p);
     const float uy = o.cch * (o.sps * o.sin_p + o.cps * o.cos_p);
     bx = c.b0u * ux;
@@ -522,142 +541,6 @@ __device__ inline ShellArgs shell_args_rev(const BInst &in, int s) {
   sa.fd_map = (in.do_fd && s > 0) ? in.adj.ckpt + static_cast<uint64_t>(s) * in.hfd.npix
                                   : nullptr;
   return sa;
-}
-
-template <bool GRIDS, bool PW1>
-__device__ void adj_ray_f32(const BInst &in, int s, uint64_t ipix, int lane, void *slab,
-                            uint32_t *chunk_id, const RayGeo &r) {
-  const bool has_cache = in.do_fd && s > 0;
-  const BShell &sh = in.sh[s];
-  const BAdj &A = in.adj;
-  const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
-                                static_cast<float>(r.l.z));
-  const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
-                                  static_cast<float>(r.eth.z));
-  const float3 ephf = make_float3(static_cast<float>(r.eph.x), static_cast<float>(r.eph.y),
-                                  static_cast<float>(r.eph.z));
-  const int nf = in.n_freq;
-  const double delta = sh.delta;
-  // per-ray adjoint constants: dL/dw from I, and the Q / U weights of each pass
-  double aI = 0., AIL = 0.;
-  float aQ[kMaxF] = {0.f, 0.f, 0.f, 0.f}, aU[kMaxF] = {0.f, 0.f, 0.f, 0.f};
-  float lpf[kMaxF] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int f = 0; f < kMaxF; ++f) {
-    if (f < nf) {
-      const double gi = A.g_i[f][ipix] * in.st[f] * delta;
-      aI += gi;
-      AIL += gi * A.lt[f];
-      aQ[f] = static_cast<float>(A.g_q[f][ipix] * in.sp[f] * delta);
-      aU[f] = static_cast<float>(A.g_u[f][ipix] * in.sp[f] * delta);
-      lpf[f] = static_cast<float>(A.lp[f]);
-    }
-  }
-  const double gdm = in.do_dm ? A.g_dm[ipix] : 0.;
-  const double gfd = in.do_fd ? A.g_fd[ipix] : 0.;
-  const double fd_total = in.t_fd[in.fd_off[s] + ipix];
-  const uint64_t sy = in.geo.nz, sx = in.geo.ny * in.geo.nz;
-  double p0 = 0., p1 = 0., p2 = 0., p3 = 0., p4 = 0., p5 = 0., q0 = 0.;
-  double after = 0., d_after = 0.;
-  const uint64_t n = r.k_hi - r.k_lo;
-  for (uint64_t c = (n + 63) / 64; c-- > 0;) {
-    const uint64_t k = r.k_lo + 64 * c + lane;
-    S32 o;
-    bool valid = false;
-    if (k < r.k_hi)
-      valid = eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o);
-    const double incl = wave_incl_scan(valid ? o.fdk * sh.fdu : 0., lane);
-    const double chunk = wave_last(incl);
-    const double fd_here = (fd_total - after - chunk) + incl;
-    after += chunk;
-    double D = 0.;
-    float dLdw = static_cast<float>(aI), alpha_p = 0.f, dipa = 0.f;
-    if (valid) {
-#pragma unroll
-      for (int f = 0; f < kMaxF; ++f) {
-        if (f < nf) {
-          const double X = in.fdin[f] ? r.fd_inner + fd_here : fd_here;
-          double rv = X * in.rev[f];
-          rv -= rint(rv);
-          const float a = static_cast<float>(rv);
-          const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
-          const float c2 = cs * o.c2i - sn * o.s2i, s2 = sn * o.c2i + cs * o.s2i;
-          const float P = aQ[f] * c2 + aU[f] * s2;
-          dLdw += P;
-          alpha_p += P * lpf[f];
-          const float dq = 2.f * o.w * (aU[f] * c2 - aQ[f] * s2);
-          dipa += dq;
-          D += A.l2[f] * static_cast<double>(dq);
-          if (f == 0)
-            q0 += dq;
-        }
-      }
-    }
-    const double dscan = wave_suffix_scan(D, lane);
-    const double R = dscan + d_after + gfd; // dL/d(FD contribution of this sample)
-    d_after += __shfl(dscan, 0, 64);
-    double dte = 0.;
-    float dbx = 0.f, dby = 0.f, dbz = 0.f;
-    if (valid) {
-      dte = o.te_raw > 0.f ? R * o.b_pard * sh.fdu + gdm * delta : 0.;
-      const float ft = static_cast<float>(R * static_cast<double>(o.te) * sh.fdu);
-      dbx = ft * lf.x;
-      dby = ft * lf.y;
-      dbz = ft * lf.z;
-      if (o.bper2 > 0.f) {
-        const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
-        dbx += coef * (o.bx - o.b_par * lf.x);
-        dby += coef * (o.by - o.b_par * lf.y);
-        dbz += coef * (o.bz - o.b_par * lf.z);
-        // ln |B_perp| in Gauss: the B grid and b0 are in muG here
-        const double lnb = 0.5 * static_cast<double>(__logf(o.bper2)) - 13.815510557964274;
-        p4 += static_cast<double>(o.w) *
-              (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
-      }
-      if (o.n2 > 0.f) {
-        const float cc = dipa / o.n2;
-        dbx += cc * (o.y * ephf.x - o.x * ethf.x);
-        dby += cc * (o.y * ephf.y - o.x * ethf.y);
-        dbz += cc * (o.y * ephf.z - o.x * ethf.z);
-      }
-      if (o.te_in)
-        p5 += dte;
-      if (o.lsa_in) {
-        const float u = o.sps * o.cos_p - o.cps * o.sin_p;
-        const float v = o.sps * o.sin_p + o.cps * o.cos_p;
-        p0 += 1e6 * static_cast<double>(dbx * (o.cch * u) + dby * (o.cch * v) + dbz * o.sch);
-        const float gpsi = in.f.b0u * o.cch * (dbx * v - dby * u);
-        p1 += gpsi;
-        p2 += gpsi * o.lnr;
-        const float gchi = in.f.b0u * (o.cch * dbz - o.sch * (dbx * u + dby * v));
-        p3 += gchi * o.tnh;
-      }
-    }
-    if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
-      const bool has = valid && o.cell_ok;
-      scatter_cached<float>(cell_cache_at<float>(slab), static_cast<float *>(A.ggrid), has,
-                            has ? o.idx : 0, o.xd, o.yd, o.zd, dbx, dby, dbz,
-                            static_cast<float>(dte), sx, sy, lane, ++*chunk_id);
-    }
-  }
-  p0 = wave_sum(p0);
-  p1 = wave_sum(p1);
-  p2 = wave_sum(p2);
-  p3 = wave_sum(p3);
-  p4 = wave_sum(p4);
-  p5 = wave_sum(p5);
-  q0 = wave_sum(q0);
-  if (lane == 0) {
-    double *rp = A.ray_par + 6 * (in.fd_off[s] + ipix);
-    rp[0] = p0;
-    rp[1] = p1;
-    rp[2] = p2;
-    rp[3] = p3;
-    rp[4] = p4;
-    rp[5] = p5;
-    if (has_cache && in.fdin[0])
-      A.gfdin[ipix] = A.l2[0] * q0;
-  }
 }
 
 template <bool GRIDS, bool PW1>
@@ -1108,7 +991,7 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
       dby = ft * lf.y;
       dbz = ft * lf.z;
       if (o.bper2 > 0.f) {
-        const float coef = dLdw * 2.f * in.f.hexp * o.w / o.bper2;
+        const float coef = dLdw * 2.f * in.f.hexp * o.w * __builtin_amdgcn_rcpf(o.bper2);
         dbx += coef * (o.bx - o.b_par * lf.x);
         dby += coef * (o.by - o.b_par * lf.y);
         dbz += coef * (o.bz - o.b_par * lf.z);
@@ -1118,7 +1001,7 @@ __device__ void adj_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
               (AIL + static_cast<double>(alpha_p) + 0.5 * lnb * static_cast<double>(dLdw));
       }
       if (o.n2 > 0.f) {
-        const float cc = dipa / o.n2;
+        const float cc = dipa * __builtin_amdgcn_rcpf(o.n2);
         dbx += cc * (o.y * ephf.x - o.x * ethf.x);
         dby += cc * (o.y * ephf.y - o.x * ethf.y);
         dbz += cc * (o.y * ephf.z - o.x * ethf.z);
@@ -1245,29 +1128,24 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
     const ShellArgs sa = REV ? shell_args_rev(in, s) : shell_args(in, s);
     ray_setup_call(in.ph, sa, p0 + lane, rays[lane]);
   }
-  using T = typename std::conditional<PREC == HAMX_PREC_F32, float, double>::type;
+  // (from here on REV implies FP64: the FP32 reverse returned above)
   const bool cached = REV && GRIDS && in.adj.ggrid;
   if (cached) // the item's rays share one cell cache, flushed at the end
-    cell_cache_init(cell_cache_at<T>(slab), lane);
+    cell_cache_init(cell_cache_at<double>(slab), lane);
   wave_lds_sync();
   uint32_t chunk = 0;
   for (uint64_t j = 0; j < nr; ++j) {
     const RayGeo r = rays[j];
     const uint64_t p = p0 + j;
-    if (REV) {
-      if (PREC == HAMX_PREC_F32)
-        adj_ray_f32<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
-      else
-        adj_ray_f64<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
-    } else {
-      if (PREC == HAMX_PREC_F32)
-        march_ray_f32<GRIDS, PW1>(in, s, p, lane, r);
-      else
-        march_ray_f64<GRIDS, PW1>(in, s, p, lane, r);
-    }
+    if constexpr (REV)
+      adj_ray_f64<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
+    else if constexpr (PREC == HAMX_PREC_F32)
+      march_ray_f32<GRIDS, PW1>(in, s, p, lane, r);
+    else
+      march_ray_f64<GRIDS, PW1>(in, s, p, lane, r);
   }
   if (cached)
-    cell_cache_flush_all(cell_cache_at<T>(slab), static_cast<T *>(in.adj.ggrid),
+    cell_cache_flush_all(cell_cache_at<double>(slab), static_cast<double *>(in.adj.ggrid),
                          in.geo.ny * in.geo.nz, in.geo.nz, lane);
   wave_lds_sync(); // the slots are rewritten by the next item
 }
@@ -2162,6 +2040,12 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
     in.f.psi0 = static_cast<float>(x.los.breg[1]);
     in.f.psi1 = static_cast<float>(x.los.breg[2]);
     in.f.chi0 = static_cast<float>(x.los.breg[3]);
+    // |ln(r / 8 kpc)| <= ln(8 / 3) over the LSA window 3 <= r <= 20 kpc
+    in.f.lsa_direct = (std::fabs(x.los.breg[1]) + std::fabs(x.los.breg[2]) * 0.99 <=
+                           0.25 * cgs::pi &&
+                       std::fabs(x.los.breg[3]) <= 0.25 * cgs::pi)
+                          ? 1
+                          : 0;
     in.f.b0u = static_cast<float>(x.los.breg[0] * 1e6);
     in.f.n0 = static_cast<float>(x.los.tereg[0]);
     in.f.n0d = x.los.tereg[0];

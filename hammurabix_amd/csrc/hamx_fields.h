@@ -42,7 +42,9 @@ __device__ inline double length0(const Vec3 &a) {
 struct GridGeo {
   double x_min, x_max, y_min, y_max, z_min, z_max;
   double fx, fy, fz; // (n-1) as double
+  double sx, sy, sz; // (n-1) / extent: vertex spacings per unit length
   uint64_t ny, nz;
+  uint64_t nyz; // ny * nz (check_box keeps it below 2^32)
 };
 
 __host__ inline GridGeo grid_geo(const hamx_box &b) {
@@ -56,8 +58,12 @@ __host__ inline GridGeo grid_geo(const hamx_box &b) {
   g.fx = static_cast<double>(b.nx - 1);
   g.fy = static_cast<double>(b.ny - 1);
   g.fz = static_cast<double>(b.nz - 1);
+  g.sx = g.fx / (b.x_max - b.x_min);
+  g.sy = g.fy / (b.y_max - b.y_min);
+  g.sz = g.fz / (b.z_max - b.z_min);
   g.ny = b.ny;
   g.nz = b.nz;
+  g.nyz = b.ny * b.nz;
   return g;
 }
 
@@ -72,23 +78,29 @@ struct Cell {
 // The interior tests are written !(0 < tmp < n-1): the same decision as the
 // reference's (tmp <= 0 || tmp >= n-1) for every finite tmp, and a NaN
 // position also falls outside instead of reaching an index.
+// tmp = (p - min) * ((n-1) / extent) is the reference's
+// (n-1) * (p - min) / extent within 2 ulp (a product instead of a division:
+// the two differ in a decision only for a position within 2 ulp of a grid
+// face); tmp lies in (0, n-1) with n - 1 < 2^32, so the 32-bit truncation is
+// the floor, and the cell index needs no 64-bit conversions.
 __device__ inline bool grid_cell(const GridGeo &g, const Vec3 &p, Cell &c) {
-  double tmp = g.fx * (p.x - g.x_min) / (g.x_max - g.x_min);
+  double tmp = (p.x - g.x_min) * g.sx;
   if (!(tmp > 0 && tmp < g.fx))
     return false;
-  const uint64_t xl = static_cast<uint64_t>(floor(tmp));
+  const uint32_t xl = static_cast<uint32_t>(tmp);
   c.xd = tmp - static_cast<double>(xl);
-  tmp = g.fy * (p.y - g.y_min) / (g.y_max - g.y_min);
+  tmp = (p.y - g.y_min) * g.sy;
   if (!(tmp > 0 && tmp < g.fy))
     return false;
-  const uint64_t yl = static_cast<uint64_t>(floor(tmp));
+  const uint32_t yl = static_cast<uint32_t>(tmp);
   c.yd = tmp - static_cast<double>(yl);
-  tmp = g.fz * (p.z - g.z_min) / (g.z_max - g.z_min);
+  tmp = (p.z - g.z_min) * g.sz;
   if (!(tmp > 0 && tmp < g.fz))
     return false;
-  const uint64_t zl = static_cast<uint64_t>(floor(tmp));
+  const uint32_t zl = static_cast<uint32_t>(tmp);
   c.zd = tmp - static_cast<double>(zl);
-  c.idx = xl * g.ny * g.nz + yl * g.nz + zl;
+  c.idx = static_cast<uint64_t>(xl) * g.nyz +
+          (yl * static_cast<uint32_t>(g.nz) + zl);
   return true;
 }
 

@@ -151,7 +151,8 @@ template <bool PACKED, bool CREG, bool HEAVY>
 __device__ inline bool eval_sample(const Phys &ph, const Grids &gr,
                                    const ShellArgs &sa, const RayGeo &r,
                                    uint64_t k, SampleOut &s, const double *stab) {
-  const double dist = sa.d_start + (static_cast<double>(k) + 0.5) * sa.delta;
+  const double dist =
+      sa.d_start + (static_cast<double>(static_cast<uint32_t>(k)) + 0.5) * sa.delta;
   const Vec3 pos{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y,
                  r.l.z * dist + ph.obs.z};
   // limit checks (integrator.cc:179-186)
@@ -527,6 +528,10 @@ int check_box(hamx_ctx *ctx, const hamx_box &b, const char *what) {
     return set_err(ctx, HAMX_ERR_ARG, "%s grid needs at least 2 vertices per axis", what);
   if (!(b.x_max > b.x_min) || !(b.y_max > b.y_min) || !(b.z_max > b.z_min))
     return set_err(ctx, HAMX_ERR_ARG, "%s grid box limits are not increasing", what);
+  // cell coordinates and the y-z plane index are 32-bit on the device
+  if (b.nx > 0xffffffffull || b.ny > 0xffffffffull || b.nz > 0xffffffffull ||
+      b.ny * b.nz > 0xffffffffull)
+    return set_err(ctx, HAMX_ERR_ARG, "%s grid has 2^32 or more vertices per y-z plane", what);
   return HAMX_OK;
 }
 
@@ -551,6 +556,14 @@ int validate_los(hamx_ctx *ctx, const hamx_los_desc *d) {
                      s, d->nside_shell[s]);
   if (!(d->oc_r_res > 0.))
     return set_err(ctx, HAMX_ERR_ARG, "oc_r_res must be positive");
+  for (uint32_t s = 0; s < d->n_shell; ++s) {
+    // the marches convert step indices with 32-bit conversions
+    uint64_t step;
+    double d_start, delta;
+    shell_ref(d, s, step, d_start, delta);
+    if (step >= (1ull << 31))
+      return set_err(ctx, HAMX_ERR_ARG, "shell %u has 2^31 or more radial steps", s);
+  }
   if (d->do_dm && !is_pow2(d->nside_dm))
     return set_err(ctx, HAMX_ERR_NSIDE, "nside_dm %u is not a power of 2", d->nside_dm);
   if (d->do_fd && !is_pow2(d->nside_fd))
@@ -579,6 +592,8 @@ Phys make_phys(const hamx_los_desc *d) {
   ph.obs = Vec3{d->observer[0], d->observer[1], d->observer[2]};
   ph.gc_r_min = d->gc_r_min;
   ph.gc_r_max = d->gc_r_max;
+  ph.gc_r_min2 = d->gc_r_min * d->gc_r_min;
+  ph.gc_r_max2 = d->gc_r_max * d->gc_r_max;
   ph.gc_z_min = d->gc_z_min;
   ph.gc_z_max = d->gc_z_max;
   ph.do_dm = d->do_dm != 0;
@@ -656,9 +671,11 @@ FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool p
   fc.inv_kpc = 1.0 / (1. * cgs::kpc);
   fc.n0 = d->tereg[0];
   fc.te_r0 = d->tereg[1];
+  fc.te_r02 = d->tereg[1] * d->tereg[1];
   fc.cre_unif = d->cre_model == HAMX_CRE_UNIF;
   if (fc.cre_unif) {
     fc.cre_r0 = d->cre[1];
+    fc.cre_r02 = d->cre[1] * d->cre[1];
   } else {
     fc.R0 = ph.R0;
     fc.inv_cre_r0 = 1.0 / d->cre[3];

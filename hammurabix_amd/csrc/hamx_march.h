@@ -23,6 +23,7 @@ namespace hamx {
 struct Phys {
   Vec3 obs;
   double gc_r_min, gc_r_max, gc_z_min, gc_z_max;
+  double gc_r_min2, gc_r_max2; // squared radial limits (eval_fast's decisions)
   int do_dm, do_fd, do_sync;
   int breg_model, tereg_model, cre_model;
   double breg[HAMX_BREG_NPARAM], tereg[HAMX_TEREG_NPARAM], cre[8];
@@ -480,9 +481,9 @@ __device__ inline void scatter_cached(const CellCache<T> &c, T *__restrict__ g, 
 struct FastConst {
   double psi0, psi1, chi0, b0;
   double inv_8kpc, inv_kpc;
-  double n0, te_r0;
+  double n0, te_r0, te_r02;
   int cre_unif; // 0: analytic profile, 1: uniform sphere
-  double R0, inv_cre_r0, inv_cre_z0, cre_r0;
+  double R0, inv_cre_r0, inv_cre_z0, cre_r0, cre_r02;
   double kt, kp;    // folded emissivity factors (include delta and i2bt)
   double pw_e;      // (index + 1) / 2
   double a_coef;    // A = a_coef / |B_perp|
@@ -498,7 +499,9 @@ struct FastConst {
 // emissivity constants, A^-1 as a product when (index+1)/2 = -1, products by
 // reciprocals for divisions by constants that feed no decision.  The
 // galactocentric limits, the LSA window, the TE sphere test, the te > 0
-// clamp and the grid-cell lookup decide exactly as in the reference.
+// clamp and the grid-cell lookup decide as in the reference, the radial
+// limits and the TE / CRE spheres on squared lengths (no square roots: the
+// decisions differ only for a sample within 1 ulp of a boundary).
 struct FastSample {
   Vec3 pos, B;
   double rr;                 // cylindrical radius
@@ -520,11 +523,14 @@ template <bool GRIDS, bool PW1, bool MODELS = false>
 __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grids &gr,
                                  const ShellArgs &sa, const RayGeo &r, uint64_t k,
                                  FastSample &o) {
-  const double dist = sa.d_start + (static_cast<double>(k) + 0.5) * sa.delta;
+  // k < 2^31 (validate_los): one 32-bit conversion
+  const double dist =
+      sa.d_start + (static_cast<double>(static_cast<uint32_t>(k)) + 0.5) * sa.delta;
   o.pos = Vec3{r.l.x * dist + ph.obs.x, r.l.y * dist + ph.obs.y, r.l.z * dist + ph.obs.z};
   const Vec3 &pos = o.pos;
-  const double len = length0(pos);
-  if (len < ph.gc_r_min || len > ph.gc_r_max || pos.z < ph.gc_z_min || pos.z > ph.gc_z_max)
+  const double len2 = pos.x * pos.x + pos.y * pos.y + pos.z * pos.z;
+  if (len2 < ph.gc_r_min2 || len2 > ph.gc_r_max2 || pos.z < ph.gc_z_min ||
+      pos.z > ph.gc_z_max)
     return false;
   // WMAP-LSA (breg_lsa.cc:9-26)
   Vec3 B{0., 0., 0.};
@@ -556,7 +562,7 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   double te = 0.;
   if (!MODELS || ph.tereg_model == HAMX_TEREG_UNIF) {
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
-    o.te_in = !(length0(d) > fc.te_r0);
+    o.te_in = !(d.x * d.x + d.y * d.y + d.z * d.z > fc.te_r02);
     if (o.te_in)
       te = fc.n0;
   } else {
@@ -593,7 +599,7 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   o.bper = sqrt(o.bper2);
   if (fc.cre_unif) {
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
-    o.prof = (length0(d) > fc.cre_r0) ? 0. : 1.;
+    o.prof = (d.x * d.x + d.y * d.y + d.z * d.z > fc.cre_r02) ? 0. : 1.;
   } else {
     const double ch = cosh(pos.z * fc.inv_cre_z0);
     o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) / (ch * ch);
