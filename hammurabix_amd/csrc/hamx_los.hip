@@ -667,6 +667,11 @@ FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool p
   fc.psi0 = d->breg[1];
   fc.psi1 = d->breg[2];
   fc.chi0 = d->breg[3];
+  // |ln(r / 8 kpc)| <= ln(8 / 3) over the LSA window 3 <= r <= 20 kpc
+  fc.lsa_direct = (std::fabs(fc.psi0) + std::fabs(fc.psi1) * 0.99 <= 0.25 * cgs::pi &&
+                   std::fabs(fc.chi0) <= 0.25 * cgs::pi)
+                      ? 1
+                      : 0;
   fc.inv_8kpc = 1.0 / (8. * cgs::kpc);
   fc.inv_kpc = 1.0 / (1. * cgs::kpc);
   fc.n0 = d->tereg[0];

@@ -478,8 +478,50 @@ __device__ inline void scatter_cached(const CellCache<T> &c, T *__restrict__ g, 
 }
 
 // Per-shell constants of the specialised march (see make_fast).
+// 1/x for normal x: the hardware v_rcp_f64 estimate refined by two Newton
+// steps (each squares the relative error), within 1 ulp of the quotient;
+// a division by a value that feeds no decision, without the IEEE division's
+// scaling and fix-up sequence.
+__device__ inline double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sin and cos of |x| <= pi/4 (no range reduction): the Taylor series to the
+// terms x^17 and x^18, whose remainders are below 1e-19 there; the small
+// part is added last.
+__device__ inline void sincos_series(double x, double &s, double &c) {
+  const double x2 = x * x;
+  const double ps =
+      -1.6666666666666666e-01 +
+      x2 * (8.3333333333333333e-03 +
+            x2 * (-1.9841269841269841e-04 +
+                  x2 * (2.7557319223985893e-06 +
+                        x2 * (-2.5052108385441720e-08 +
+                              x2 * (1.6059043836821613e-10 +
+                                    x2 * (-7.6471637318198165e-13 +
+                                          x2 * 2.8114572543455206e-15))))));
+  s = x + (x * x2) * ps;
+  const double pc =
+      4.1666666666666664e-02 +
+      x2 * (-1.3888888888888889e-03 +
+            x2 * (2.4801587301587302e-05 +
+                  x2 * (-2.7557319223985888e-07 +
+                        x2 * (2.0876756987868100e-09 +
+                              x2 * (-1.1470745597729725e-11 +
+                                    x2 * (4.7794773323873853e-14 +
+                                          x2 * -1.5619206968586225e-16))))));
+  c = 1.0 + x2 * (-0.5 + x2 * pc);
+}
+
 struct FastConst {
   double psi0, psi1, chi0, b0;
+  // psi(r) and chi(z) stay within [-pi/4, pi/4] over the LSA window
+  // (checked on the host): their sines and cosines are the series alone
+  int lsa_direct;
   double inv_8kpc, inv_kpc;
   double n0, te_r0, te_r02;
   int cre_unif; // 0: analytic profile, 1: uniform sphere
@@ -544,15 +586,24 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
       B = breg_eval<true>(ph.breg_model, ph.breg, pos);
   }
   if (o.lsa_in) {
-    const double inv_r = 1.0 / o.rr;
+    const double inv_r = rcp_f64(o.rr);
     o.cos_p = pos.x * inv_r;
     o.sin_p = pos.y * inv_r;
     o.lnr = log(o.rr * fc.inv_8kpc);
-    o.tnh = tanh(pos.z * fc.inv_kpc);
+    // tanh(u) = (1 - e) / (1 + e) with e = exp(-2|u|): the absolute error
+    // stays at the rounding of 1 - e (1e-16), which is what chi needs
+    const double u = pos.z * fc.inv_kpc;
+    const double e = exp(-2. * fabs(u));
+    o.tnh = copysign((1. - e) * rcp_f64(1. + e), u);
     const double psi = fc.psi0 + fc.psi1 * o.lnr;
     const double chi = fc.chi0 * o.tnh;
-    sincos(psi, &o.sps, &o.cps);
-    sincos(chi, &o.sch, &o.cch);
+    if (fc.lsa_direct) {
+      sincos_series(psi, o.sps, o.cps);
+      sincos_series(chi, o.sch, o.cch);
+    } else {
+      sincos(psi, &o.sps, &o.cps);
+      sincos(chi, &o.sch, &o.cch);
+    }
     const double bc = fc.b0 * o.cch;
     B.x = bc * (o.sps * o.cos_p - o.cps * o.sin_p);
     B.y = bc * (o.sps * o.sin_p + o.cps * o.cos_p);
@@ -601,8 +652,10 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     const Vec3 d{pos.x - ph.obs.x, pos.y - ph.obs.y, pos.z - ph.obs.z};
     o.prof = (d.x * d.x + d.y * d.y + d.z * d.z > fc.cre_r02) ? 0. : 1.;
   } else {
-    const double ch = cosh(pos.z * fc.inv_cre_z0);
-    o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) / (ch * ch);
+    // sech^2(z / z0) = 4 e / (1 + e)^2 with e = exp(-2 |z| / z0) <= 1
+    const double e = exp(-2. * fabs(pos.z * fc.inv_cre_z0));
+    const double d = 1. + e;
+    o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) * (4. * e) * rcp_f64(d * d);
   }
   double w;
   if (PW1)
@@ -616,7 +669,7 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   o.y = -dot0(r.eth, B);
   o.n2 = o.x * o.x + o.y * o.y;
   if (o.n2 > 0.) {
-    const double inv = 1.0 / o.n2;
+    const double inv = rcp_f64(o.n2);
     o.c2i = (o.x * o.x - o.y * o.y) * inv;
     o.s2i = 2. * o.x * o.y * inv;
   } else {
