@@ -685,6 +685,7 @@ FastConst make_fast(const hamx_los_desc *d, const Phys &ph, double delta, bool p
     fc.R0 = ph.R0;
     fc.inv_cre_r0 = 1.0 / d->cre[3];
     fc.inv_cre_z0 = 1.0 / d->cre[4];
+    fc.cre_z_kpc = fc.inv_cre_z0 == fc.inv_kpc ? 1 : 0;
   }
   fc.pw_e = 0.5 * (ph.idx + 1);
   fc.a_coef = ph.A_num / ph.three_qe;

@@ -526,6 +526,7 @@ struct FastConst {
   double n0, te_r0, te_r02;
   int cre_unif; // 0: analytic profile, 1: uniform sphere
   double R0, inv_cre_r0, inv_cre_z0, cre_r0, cre_r02;
+  int cre_z_kpc; // analytic CRE with z0 = 1 kpc: its sech^2 shares tanh's exponential
   double kt, kp;    // folded emissivity factors (include delta and i2bt)
   double pw_e;      // (index + 1) / 2
   double a_coef;    // A = a_coef / |B_perp|
@@ -585,6 +586,10 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     else
       B = breg_eval<true>(ph.breg_model, ph.breg, pos);
   }
+  // exp(-2|z| / kpc): the LSA's tanh(z / kpc) and, when the CRE scale
+  // height is 1 kpc, the CRE's sech^2(z / z0)
+  const double u = pos.z * fc.inv_kpc;
+  const double e_kpc = (o.lsa_in || fc.cre_z_kpc) ? exp(-2. * fabs(u)) : 0.;
   if (o.lsa_in) {
     const double inv_r = rcp_f64(o.rr);
     o.cos_p = pos.x * inv_r;
@@ -592,9 +597,7 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     o.lnr = log(o.rr * fc.inv_8kpc);
     // tanh(u) = (1 - e) / (1 + e) with e = exp(-2|u|): the absolute error
     // stays at the rounding of 1 - e (1e-16), which is what chi needs
-    const double u = pos.z * fc.inv_kpc;
-    const double e = exp(-2. * fabs(u));
-    o.tnh = copysign((1. - e) * rcp_f64(1. + e), u);
+    o.tnh = copysign((1. - e_kpc) * rcp_f64(1. + e_kpc), u);
     const double psi = fc.psi0 + fc.psi1 * o.lnr;
     const double chi = fc.chi0 * o.tnh;
     if (fc.lsa_direct) {
@@ -653,7 +656,7 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     o.prof = (d.x * d.x + d.y * d.y + d.z * d.z > fc.cre_r02) ? 0. : 1.;
   } else {
     // sech^2(z / z0) = 4 e / (1 + e)^2 with e = exp(-2 |z| / z0) <= 1
-    const double e = exp(-2. * fabs(pos.z * fc.inv_cre_z0));
+    const double e = fc.cre_z_kpc ? e_kpc : exp(-2. * fabs(pos.z * fc.inv_cre_z0));
     const double d = 1. + e;
     o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) * (4. * e) * rcp_f64(d * d);
   }
