@@ -57,8 +57,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
   // kAdjRays consecutive rays per wavefront (set up one per lane), and a
   // per-wavefront cell cache that merges their grid-gradient contributions
   __shared__ RayGeo slots[4][kAdjRays];
-  static_assert(kSlabValues * sizeof(double) <= cell_cache_bytes<double>(), "slab fits");
-  __shared__ __align__(16) unsigned char cache_mem[4][cell_cache_bytes<double>()];
+  __shared__ __align__(16) unsigned char cache_mem[4][staged_cache_bytes()];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 4 + wave) * kAdjRays;
   if (p0 >= sa.hs.npix)
@@ -68,12 +67,8 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
                           : static_cast<uint64_t>(kAdjRays);
   if (static_cast<uint64_t>(lane) < nr)
     ray_setup(ph, sa, p0 + lane, slots[wave][lane]);
-  const CellCache<double> cache = cell_cache_at<double>(cache_mem[wave]);
-  if (kAdjCellCache)
-    cell_cache_init(cache, lane);
-  else
-    wave_lds_sync();
-  uint32_t chunk_id = 0;
+  const StagedCache cache = staged_at(cache_mem[wave]);
+  staged_init(cache, lane);
   for (uint64_t jr = 0; jr < nr; ++jr) {
   const uint64_t ipix = p0 + jr;
   const RayGeo r = slots[wave][jr];
@@ -163,14 +158,8 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     }
     if (GRIDS && as.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      if (kAdjCellCache)
-        scatter_cached<double>(cache, as.ggrid, has, has ? o.cell.idx : 0, o.cell.xd,
-                               o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane,
-                               ++chunk_id);
-      else
-        scatter_runs<double>(as.ggrid, reinterpret_cast<double *>(cache_mem[wave]), has,
-                             has ? o.cell.idx : 0, o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y,
-                             dB.z, dte, as.sx, as.sy, lane);
+      scatter_staged(cache, as.ggrid, has, has ? o.cell.idx : 0, o.cell.xd, o.cell.yd,
+                     o.cell.zd, dB.x, dB.y, dB.z, dte, as.sx, as.sy, lane);
     }
   }
   p0 = wave_sum(p0);
@@ -191,8 +180,8 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
       as.gfdin[ipix] = ac.l2 * dq_after;
   }
   } // rays of the wavefront
-  if (kAdjCellCache && GRIDS && as.ggrid)
-    cell_cache_flush_all(cache, as.ggrid, as.sx, as.sy, lane);
+  if (GRIDS && as.ggrid)
+    staged_flush_all(cache, as.ggrid, as.sx, as.sy, lane);
 }
 
 // dst (shell maps) += A^T(src) for up to 5 map pairs (the transpose of

@@ -266,81 +266,16 @@ __device__ inline void grid_atomic_add(T *p, T v) {
 
 // Rays per wavefront in the specialised march (kFastRays) and its adjoint
 // (kAdjRays): their set-up runs one ray per lane into LDS, then they are
-// marched one after another.  kAdjCellCache picks the adjoint's grid
-// scatter: the per-wavefront cell cache (merges neighbouring rays) or
-// scatter_runs (merges the runs of one 64-sample chunk).  Measured on the
-// 256x256x64 three-shell workload (march / adjoint ms): 1 ray 14.0 / 44.8
-// (runs), 2 rays 13.4 / 42.1 (cache), 4 rays 14.5, 8 rays 21.8 / 66.1: more
-// rays per wavefront leave too few wavefronts to balance the uneven ray
-// lengths (rays leaving the z range early) across the SIMDs.
+// marched one after another.  Measured on the 256x256x64 three-shell
+// workload (march / adjoint ms): 1 ray 14.0 / 44.8, 2 rays 13.4 / 42.1,
+// 4 rays 14.5, 8 rays 21.8 / 66.1: more rays per wavefront leave too few
+// wavefronts to balance the uneven ray lengths (rays leaving the z range
+// early) across the SIMDs.
 constexpr int kFastRays = 2;
 constexpr int kAdjRays = 2;
-constexpr bool kAdjCellCache = true;
 
-// LDS slab of one wavefront for scatter_runs: one slot of 32 values (8
-// corners x {x, y, z, te}) per possible run.
-constexpr int kSlabValues = 64 * 32;
-
-// Trilinear scatter-add of (g0, g1, g2, g3) of every lane with `has` into
-// the packed {x, y, z, te} grid g at the 8 corners of cell idx (the
-// transpose of lerp_cell_packed).  All 64 lanes must call it.  The lanes of
-// a wavefront hold consecutive samples of one ray, which fall into few
-// cells, and a straight ray meets each cell in one contiguous run of
-// samples: the lanes of a run add their 32 corner values into one slot of
-// the wavefront's LDS slab, then the run's first lane adds the slot to the
-// grid, so a run costs 32 global atomics instead of 32 per sample.
-template <typename T>
-__device__ inline void scatter_runs(T *__restrict__ g, T *slab, bool has, uint64_t idx, T xd,
-                                    T yd, T zd, T g0, T g1, T g2, T g3, uint64_t sx,
-                                    uint64_t sy, int lane) {
-  const uint64_t key = has ? idx : ~0ull;
-  const uint64_t prev = __shfl_up(key, 1, 64);
-  const bool head = lane == 0 || key != prev;
-  const unsigned long long heads = __ballot(head);
-  const int run = __popcll(heads & ((2ull << lane) - 1ull)) - 1; // runs starting at <= lane
-  T *slot = slab + 32 * run;
-  if (head && has) {
-#pragma unroll
-    for (int i = 0; i < 32; ++i)
-      slot[i] = T(0);
-  }
-  wave_lds_sync();
-  if (has) {
-    const T wx[2] = {T(1) - xd, xd}, wy[2] = {T(1) - yd, yd}, wz[2] = {T(1) - zd, zd};
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const T w = wx[a] * wy[b] * wz[c];
-          T *q = slot + 4 * (4 * a + 2 * b + c);
-          atomicAdd(q + 0, w * g0);
-          atomicAdd(q + 1, w * g1);
-          atomicAdd(q + 2, w * g2);
-          atomicAdd(q + 3, w * g3);
-        }
-  }
-  wave_lds_sync();
-  if (head && has) {
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const T *q = slot + 4 * (4 * a + 2 * b + c);
-          T *p = g + 4 * (idx + a * sx + b * sy + c);
-          grid_atomic_add(p + 0, q[0]);
-          grid_atomic_add(p + 1, q[1]);
-          grid_atomic_add(p + 2, q[2]);
-          grid_atomic_add(p + 3, q[3]);
-        }
-  }
-  wave_lds_sync();
-}
-
-// Per-wavefront LDS cache of grid-gradient cells for the adjoint scatter.
+// Per-wavefront LDS cache of grid-gradient cells for the FP64 batch's
+// reverse sweep (LDS atomics; the plain adjoint uses scatter_staged below).
 // A straight ray meets each cell in one run of consecutive samples, and
 // the neighbouring rays a wavefront marches next (consecutive RING pixels)
 // cross mostly the same cells.  The cache holds kCacheSlots cells (32
@@ -473,6 +408,192 @@ __device__ inline void scatter_cached(const CellCache<T> &c, T *__restrict__ g, 
             grid_atomic_add(p + 3, w * g3);
           }
         }
+  }
+  wave_lds_sync();
+}
+
+// ------------------------------------------------ staged grid scatter --
+// The plain adjoint's grid-gradient scatter.  A contended FP64 LDS atomic
+// costs about 30 times a plain 64-lane LDS store or load (measured on
+// gfx950), so the 32 corner values of a chunk's samples are merged without
+// atomics: every lane stores its values into a per-wavefront staging area
+// (16 values at a time), and the lanes then sum each run of samples that
+// share a cell transposed, one lane per (value, run), 4 runs at a time.
+// Each run's sums go once into the run's slot of a per-wavefront cell
+// cache (plain stores: one wavefront owns its cache and, within a chunk,
+// each slot serves one run), or straight to the grid when an earlier run
+// of the chunk holds the slot.  A cell reaches the grid (32 global atomics)
+// when its slot is taken by another cell and at the end of the wavefront.
+constexpr int kStageSlots = 32;
+constexpr int kStageStride = 65; // doubles per staged row: 64 lanes + 1, so rows start in different banks
+
+struct StagedCache {
+  unsigned long long *key; // cell held by each slot (~0: empty)
+  double *val;             // kStageSlots x 32 values
+  double *stage;           // 16 rows x kStageStride
+};
+
+constexpr int staged_cache_bytes() {
+  return kStageSlots * (8 + 32 * 8) + 16 * kStageStride * 8;
+}
+
+__device__ inline StagedCache staged_at(void *base) {
+  StagedCache c;
+  c.key = static_cast<unsigned long long *>(base);
+  c.val = reinterpret_cast<double *>(c.key + kStageSlots);
+  c.stage = c.val + 32 * kStageSlots;
+  return c;
+}
+
+__device__ inline void staged_init(const StagedCache &c, int lane) {
+  if (lane < kStageSlots)
+    c.key[lane] = ~0ull;
+  wave_lds_sync();
+}
+
+__device__ inline int staged_hash(unsigned long long idx) {
+  return static_cast<int>((idx * 0x9E3779B97F4A7C15ull) >> 59); // 5 bits
+}
+
+__device__ inline unsigned long long lane_u64(unsigned long long v, int L) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), L);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), L);
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Value v (corner v >> 2, component v & 3) of cell `cell` into the grid.
+__device__ inline void staged_grid_add(double *__restrict__ g, unsigned long long cell, int v,
+                                       double x, uint64_t sx, uint64_t sy) {
+  const int corner = v >> 2;
+  const uint64_t vx = cell + (corner >> 2) * sx + ((corner >> 1) & 1) * sy + (corner & 1);
+  grid_atomic_add(g + 4 * vx + (v & 3), x);
+}
+
+// The slots in `slots` (bits), holding the cells the wavefront reads from
+// c.key, emptied into the grid: two slots per pass, lanes 0-31 and 32-63.
+__device__ inline void staged_flush(const StagedCache &c, double *__restrict__ g,
+                                    uint32_t slots, uint64_t sx, uint64_t sy, int lane) {
+  while (slots) { // uniform
+    const int s0 = __ffs(slots) - 1;
+    slots &= slots - 1;
+    const int s1 = slots ? __ffs(slots) - 1 : -1;
+    if (s1 >= 0)
+      slots &= slots - 1;
+    const int s = lane < 32 ? s0 : s1;
+    if (s >= 0) {
+      const unsigned long long cell = c.key[s];
+      staged_grid_add(g, cell, lane & 31, c.val[32 * s + (lane & 31)], sx, sy);
+    }
+  }
+}
+
+__device__ inline void staged_flush_all(const StagedCache &c, double *__restrict__ g,
+                                        uint64_t sx, uint64_t sy, int lane) {
+  const unsigned long long k = lane < kStageSlots ? c.key[lane] : ~0ull;
+  const uint32_t held = static_cast<uint32_t>(__ballot(k != ~0ull));
+  staged_flush(c, g, held, sx, sy, lane);
+  wave_lds_sync();
+  if (lane < kStageSlots)
+    c.key[lane] = ~0ull;
+  wave_lds_sync();
+}
+
+// Scatter-add of the trilinear corner contributions of every lane with
+// `has` (the transpose of lerp_cell_packed, lanes = consecutive samples of
+// one ray; a straight ray meets each cell in one run of them).  All 64
+// lanes must call it.
+__device__ inline void scatter_staged(const StagedCache &c, double *__restrict__ g, bool has,
+                                      uint64_t idx, double xd, double yd, double zd, double g0,
+                                      double g1, double g2, double g3, uint64_t sx, uint64_t sy,
+                                      int lane) {
+  const unsigned long long key = has ? idx : ~0ull;
+  const unsigned long long prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || key != prev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long runs = __ballot(head && has);
+  if (!runs)
+    return; // uniform
+  // slots: each run's head looks up its cell's slot; a slot an earlier
+  // run of this chunk took sends the later run to the grid
+  const int s_own = staged_hash(key);
+  const unsigned long long held = (head && has) ? c.key[s_own] : ~0ull;
+  unsigned long long direct = 0; // runs (by head lane) that go to the grid
+  uint32_t taken = 0;
+  for (unsigned long long t = runs; t; t &= t - 1) { // uniform, scalar
+    const int h = __ffsll(t) - 1;
+    const int sh = __builtin_amdgcn_readlane(s_own, h);
+    if ((taken >> sh) & 1u)
+      direct |= 1ull << h;
+    else
+      taken |= 1u << sh;
+  }
+  const bool mine = head && has && !((direct >> lane) & 1ull);
+  const bool fresh = mine && held != key;           // the run claims its slot
+  const bool evict = fresh && held != ~0ull;        // ... from another cell
+  const unsigned long long fresh_runs = __ballot(fresh);
+  uint32_t ev_slots = 0;
+  for (unsigned long long t = __ballot(evict); t; t &= t - 1)
+    ev_slots |= 1u << __builtin_amdgcn_readlane(s_own, __ffsll(t) - 1);
+  staged_flush(c, g, ev_slots, sx, sy, lane);
+  wave_lds_sync();
+  if (fresh)
+    c.key[s_own] = key;
+  // this lane's 32 corner values (0 without a sample)
+  const double wx[2] = {1. - xd, xd}, wy[2] = {1. - yd, yd}, wz[2] = {1. - zd, zd};
+  double v[32];
+#pragma unroll
+  for (int cn = 0; cn < 8; ++cn) {
+    const double w = has ? wx[cn >> 2] * wy[(cn >> 1) & 1] * wz[cn & 1] : 0.;
+    v[4 * cn + 0] = w * g0;
+    v[4 * cn + 1] = w * g1;
+    v[4 * cn + 2] = w * g2;
+    v[4 * cn + 3] = w * g3;
+  }
+  const int vi = lane & 15, grp = lane >> 4;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    wave_lds_sync(); // the previous half's loads are done before the rows are rewritten
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      c.stage[i * kStageStride + lane] = v[16 * half + i];
+    wave_lds_sync();
+    for (unsigned long long t = runs; t;) { // uniform: 4 runs per round
+      int a[4], b[4], len = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = b[j] = 0;
+        if (t) {
+          a[j] = __ffsll(t) - 1;
+          t &= t - 1;
+          const unsigned long long later = heads & ~((2ull << a[j]) - 1ull);
+          b[j] = later ? __ffsll(later) - 1 : 64;
+          len = max(len, b[j] - a[j]);
+        }
+      }
+      // the run of this lane's 16-lane group (run heads are uniform)
+      unsigned long long cl[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        cl[j] = lane_u64(key, a[j]);
+      const int ra = grp == 0 ? a[0] : grp == 1 ? a[1] : grp == 2 ? a[2] : a[3];
+      const int rb = grp == 0 ? b[0] : grp == 1 ? b[1] : grp == 2 ? b[2] : b[3];
+      const unsigned long long cell = grp == 0 ? cl[0] : grp == 1 ? cl[1] : grp == 2 ? cl[2] : cl[3];
+      double sum = 0.;
+      for (int k = 0; k < len; ++k) {
+        const int i = ra + k;
+        if (i < rb)
+          sum += c.stage[vi * kStageStride + i];
+      }
+      if (rb > ra) {
+        const int val = 16 * half + vi;
+        if ((direct >> ra) & 1ull) {
+          staged_grid_add(g, cell, val, sum, sx, sy);
+        } else {
+          double *q = c.val + 32 * staged_hash(cell) + val;
+          *q = ((fresh_runs >> ra) & 1ull) ? sum : *q + sum;
+        }
+      }
+    }
   }
   wave_lds_sync();
 }
