@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     double c2q = 1., s2q = 0., dq = 0.;
     if (valid) {
       double sph, cph;
-      sincos(ac.two_l2 * (r.fd_inner + fd_here), &sph, &cph);
+      sincos_fast(ac.two_l2 * (r.fd_inner + fd_here), sph, cph);
       c2q = cph * o.c2i - sph * o.s2i;
       s2q = sph * o.c2i + cph * o.s2i;
       dq = 2. * o.jp * (-gq * s2q + gu * c2q);

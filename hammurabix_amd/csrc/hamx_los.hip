@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       fd_carry += wave_last(incl);
       if (valid) {
         double sph, cph;
-        sincos(two_l2 * (r.fd_inner + fd_here), &sph, &cph);
+        sincos_fast(two_l2 * (r.fd_inner + fd_here), sph, cph);
         I += o.jt;
         Q += (cph * o.c2i - sph * o.s2i) * o.jp;
         U += (sph * o.c2i + cph * o.s2i) * o.jp;

@@ -638,6 +638,27 @@ __device__ inline void sincos_series(double x, double &s, double &c) {
   c = 1.0 + x2 * (-0.5 + x2 * pc);
 }
 
+// sin and cos of any x: for |x| < 2^19 pi a Cody-Waite reduction by pi/2,
+// whose three parts hold 33, 33 and 53 bits of pi/2 (k P1 and k P2 are
+// exact for |k| < 2^20, so r = x - k pi/2 is exact to the third part's
+// rounding), then the series on |r| <= pi/4 and the quadrant's signs;
+// beyond that the library sincos.
+__device__ inline void sincos_fast(double x, double &s, double &c) {
+  const double k = rint(x * 0.6366197723675814); // 2 / pi
+  if (fabs(k) < 1048576.0) {
+    double r = fma(-k, 1.5707963267341256, x);
+    r = fma(-k, 6.077100506303966e-11, r);
+    r = fma(-k, 2.0222662487959506e-21, r);
+    double sr, cr;
+    sincos_series(r, sr, cr);
+    const int q = static_cast<int>(k) & 3;
+    s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
+    c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
+  } else {
+    sincos(x, &s, &c);
+  }
+}
+
 struct FastConst {
   double psi0, psi1, chi0, b0;
   // psi(r) and chi(z) stay within [-pi/4, pi/4] over the LSA window
