@@ -54,6 +54,11 @@ struct FftAxis {
   uint32_t n;      // line length
   uint32_t M;      // 0: direct; else the Bluestein convolution length
   uint32_t C;      // lines per workgroup
+  // addressing of a tile's C lines: 1 = one contiguous run of C n elements
+  // (the contiguous axis); 2 = C adjacent columns (C a power of two, 2^cshift,
+  // dividing `inner`: a tile never straddles two groups of `inner` lines);
+  // 0 = per-element line bases
+  uint32_t fast, cshift;
   FftPlan plan;    // passes of n (direct) or of M (Bluestein)
 };
 
@@ -204,12 +209,28 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
   double2 *lds = kLdsTw ? dyn_lds : static_lds;
   const uint32_t n = ax.n, C = ax.C, total = C * n;
   const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * C;
-  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-    uint32_t c, t;
-    tile_coords(ax, n, e, c, t);
-    const uint64_t L = L0 + c;
-    lds[c * n + t] = L < ax.lines ? data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s]
-                                  : make_double2(0., 0.);
+  // the tile's first element: one line_base (and its integer divisions) per
+  // workgroup on the two fast paths instead of one per element
+  const uint64_t base0 = ax.fast ? line_base(ax, L0) : 0;
+  const uint64_t left = ax.lines - L0; // lines of this tile that exist (>= 1)
+  if (ax.fast == 1) {
+    const uint32_t valid = static_cast<uint32_t>(left < C ? left : C) * n;
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x)
+      lds[e] = e < valid ? data[base0 + e] : make_double2(0., 0.);
+  } else if (ax.fast == 2) {
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const uint32_t c = e & (C - 1), t = e >> ax.cshift;
+      lds[c * n + t] = c < left ? data[base0 + c + static_cast<uint64_t>(t) * ax.s]
+                                : make_double2(0., 0.);
+    }
+  } else {
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      uint32_t c, t;
+      tile_coords(ax, n, e, c, t);
+      const uint64_t L = L0 + c;
+      lds[c * n + t] = L < ax.lines ? data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s]
+                                    : make_double2(0., 0.);
+    }
   }
   const double2 *twl = tw;
   if constexpr (kLdsTw) {
@@ -220,12 +241,24 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
   }
   __syncthreads();
   run_plan<SIGN, TILE, POW2>(lds, ax.plan, C, twl);
-  for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-    uint32_t c, t;
-    tile_coords(ax, n, e, c, t);
-    const uint64_t L = L0 + c;
-    if (L < ax.lines)
-      data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
+  if (ax.fast == 1) {
+    const uint32_t valid = static_cast<uint32_t>(left < C ? left : C) * n;
+    for (uint32_t e = threadIdx.x; e < valid; e += blockDim.x)
+      data[base0 + e] = lds[e];
+  } else if (ax.fast == 2) {
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      const uint32_t c = e & (C - 1), t = e >> ax.cshift;
+      if (c < left)
+        data[base0 + c + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
+    }
+  } else {
+    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
+      uint32_t c, t;
+      tile_coords(ax, n, e, c, t);
+      const uint64_t L = L0 + c;
+      if (L < ax.lines)
+        data[line_base(ax, L) + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
+    }
   }
 }
 
@@ -471,6 +504,13 @@ int fft3d_dev2(hamx_ctx *ctx, double2 *data, double2 *data2, uint64_t nx, uint64
     if (C > ax.lines)
       C = ax.lines;
     ax.C = static_cast<uint32_t>(C);
+    if (ax.s == 1 && ax.inner == 1 && ax.outer == n) {
+      ax.fast = 1;
+    } else if (ax.s != 1 && is_pow2(C) && ax.inner % C == 0) {
+      ax.fast = 2;
+      while ((1ull << ax.cshift) < C)
+        ++ax.cshift;
+    }
     const unsigned blocks = static_cast<unsigned>((ax.lines + C - 1) / C);
     if (direct) {
       const double2 *tw;

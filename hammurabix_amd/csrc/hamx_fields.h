@@ -12,6 +12,7 @@
 
 #include "hamx.h"
 #include "hamx_cgs.h"
+#include "hamx_math.h"
 #include "hamx_models.h"
 
 namespace hamx {
@@ -173,10 +174,13 @@ __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
     const double cos_p = pos.x / r;
     const double sin_p = pos.y / r;
     const double psi = p[1] + p[2] * log(r / (8. * cgs::kpc));
-    const double chi = p[3] * tanh(pos.z / (1. * cgs::kpc));
+    // tanh(u) = (1 - e) / (1 + e), e = exp(-2|u|): absolute error ~1e-16
+    const double u = pos.z / (1. * cgs::kpc);
+    const double e = exp(-2. * fabs(u));
+    const double chi = p[3] * copysign((1. - e) * rcp_f64(1. + e), u);
     double sps, cps, sch, cch;
-    sincos(psi, &sps, &cps);
-    sincos(chi, &sch, &cch);
+    sincos_fast(psi, sps, cps);
+    sincos_fast(chi, sch, cch);
     b.x = p[0] * (sps * cch * cos_p - cps * cch * sin_p);
     b.y = p[0] * (sps * cch * sin_p + cps * cch * cos_p);
     b.z = p[0] * sch;

@@ -1,0 +1,79 @@
+// FP64 building blocks shared by the marches and the random-field kernels:
+// a Newton-refined reciprocal and sines / cosines by a Cody-Waite reduction
+// and Taylor series, each within a few ulp (the FP64 paths are held to 1e-10
+// and 1e-11 relative, not to the last bit of a library routine).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace hamx {
+
+// 1/x for normal x: the hardware v_rcp_f64 estimate refined by two Newton
+// steps (each squares the relative error), within 1 ulp of the quotient;
+// a division by a value that feeds no decision, without the IEEE division's
+// scaling and fix-up sequence.
+__device__ inline double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sin and cos of |x| <= pi/4 (no range reduction): the Taylor series to the
+// terms x^17 and x^18, whose remainders are below 1e-19 there; the small
+// part is added last.
+__device__ inline void sincos_series(double x, double &s, double &c) {
+  const double x2 = x * x;
+  const double ps =
+      -1.6666666666666666e-01 +
+      x2 * (8.3333333333333333e-03 +
+            x2 * (-1.9841269841269841e-04 +
+                  x2 * (2.7557319223985893e-06 +
+                        x2 * (-2.5052108385441720e-08 +
+                              x2 * (1.6059043836821613e-10 +
+                                    x2 * (-7.6471637318198165e-13 +
+                                          x2 * 2.8114572543455206e-15))))));
+  s = x + (x * x2) * ps;
+  const double pc =
+      4.1666666666666664e-02 +
+      x2 * (-1.3888888888888889e-03 +
+            x2 * (2.4801587301587302e-05 +
+                  x2 * (-2.7557319223985888e-07 +
+                        x2 * (2.0876756987868100e-09 +
+                              x2 * (-1.1470745597729725e-11 +
+                                    x2 * (4.7794773323873853e-14 +
+                                          x2 * -1.5619206968586225e-16))))));
+  c = 1.0 + x2 * (-0.5 + x2 * pc);
+}
+
+// The library sincos, out of line: the large-argument reduction it carries
+// then takes no registers in the callers, which reach it only beyond 2^19 pi.
+__device__ __noinline__ inline void sincos_lib(double x, double &s, double &c) {
+  sincos(x, &s, &c);
+}
+
+// sin and cos of any x: for |x| < 2^19 pi a Cody-Waite reduction by pi/2,
+// whose three parts hold 33, 33 and 53 bits of pi/2 (k P1 and k P2 are
+// exact for |k| < 2^20, so r = x - k pi/2 is exact to the third part's
+// rounding), then the series on |r| <= pi/4 and the quadrant's signs;
+// beyond that the library sincos.
+__device__ inline void sincos_fast(double x, double &s, double &c) {
+  const double k = rint(x * 0.6366197723675814); // 2 / pi
+  if (fabs(k) < 1048576.0) {
+    double r = fma(-k, 1.5707963267341256, x);
+    r = fma(-k, 6.077100506303966e-11, r);
+    r = fma(-k, 2.0222662487959506e-21, r);
+    double sr, cr;
+    sincos_series(r, sr, cr);
+    const int q = static_cast<int>(k) & 3;
+    s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
+    c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
+  } else {
+    sincos_lib(x, s, c);
+  }
+}
+
+} // namespace hamx

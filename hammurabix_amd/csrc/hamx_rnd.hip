@@ -24,6 +24,7 @@
 #include "hamx_cgs.h"
 #include "hamx_fields.h"
 #include "hamx_internal.h"
+#include "hamx_math.h"
 
 namespace hamx {
 
@@ -58,7 +59,7 @@ __device__ inline void gauss_pair(uint64_t seed, uint32_t stream, uint64_t m,
   const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
   const double r = sqrt(-2.0 * log(u1));
   double s, co;
-  sincos(cgs::twopi * u2, &s, &co);
+  sincos_fast(cgs::twopi * u2, s, co);
   g0 = r * co;
   g1 = r * s;
 }
@@ -107,6 +108,27 @@ static BoxArgs box_args(const hamx_box &b) {
   return a;
 }
 
+// (i, j, l) of the flat vertex index idx (x slowest, z fastest).  Integer
+// division is a software routine on the GPU, several times longer in 64 bits
+// than in 32: boxes below 2^32 vertices (all but the largest) split the index
+// with 32-bit divisions.
+__device__ inline void box_ijl(const BoxArgs &a, uint64_t idx, uint64_t &i, uint64_t &j,
+                               uint64_t &l) {
+  if (a.n <= 0xffffffffull) { // uniform
+    const uint32_t id = static_cast<uint32_t>(idx);
+    const uint32_t nz = static_cast<uint32_t>(a.nz), ny = static_cast<uint32_t>(a.ny);
+    const uint32_t il = id / nz;
+    const uint32_t i32 = il / ny;
+    l = id - il * nz;
+    j = il - i32 * ny;
+    i = i32;
+  } else {
+    i = idx / (a.ny * a.nz);
+    j = (idx / a.nz) % a.ny;
+    l = idx % a.nz;
+  }
+}
+
 // Wave number of index i on an axis of n points over length len
 // (brnd_es.cc:106-122: periodic spacing n/L).
 __device__ inline double wavenum(uint64_t i, uint64_t n, double len) {
@@ -129,13 +151,12 @@ __device__ inline double profile(const Vec3 &obs, double r0, double z0,
                                  const Vec3 &pos) {
   const double r_cyl = sqrt(pos.x * pos.x + pos.y * pos.y) - fabs(obs.x);
   const double z = fabs(pos.z) - fabs(obs.z);
-  return exp(-r_cyl / r0) * exp(-z / z0);
+  return exp(-r_cyl / r0 - z / z0); // one exponential: the product's within a few ulp
 }
 
 __device__ inline uint64_t sym_index(const BoxArgs &a, uint64_t idx) {
-  const uint64_t i = idx / (a.ny * a.nz);
-  const uint64_t j = (idx / a.nz) % a.ny;
-  const uint64_t l = idx % a.nz;
+  uint64_t i, j, l;
+  box_ijl(a, idx, i, j, l);
   const uint64_t is = i == 0 ? 0 : a.nx - i;
   const uint64_t js = j == 0 ? 0 : a.ny - j;
   const uint64_t ls = l == 0 ? 0 : a.nz - l;
@@ -211,6 +232,7 @@ struct EsArgs {
   BoxArgs b;
   Vec3 obs;
   double rms, k0, k1, a0, a1, rho, r0, z0;
+  double c01; // (k0 / k1)^a1, the low band's constant factor
   int breg_model;
   int jf12; // Brnd_jf12: its energy profile and literal constants
   double jp[13];
@@ -227,12 +249,15 @@ __device__ inline double es_spectrum(const EsArgs &a, double k) {
   // the reference's sum of band-masked terms, with each power evaluated only
   // where its band is on (a masked-off term adds an exact zero)
   double P = 0.;
-  if (k < a.k1)
-    P += pow(a.k0 / a.k1, a.a1) * pow(k / a.k1, 6.);
+  if (k < a.k1) {
+    const double x = k / a.k1, x3 = x * x * x;
+    P += a.c01 * (x3 * x3); // (k0 / k1)^a1 (k / k1)^6
+  }
+  // 1 / (k / k0)^a as exp(-a ln(k / k0)): within a few ulp of the power
   if (k > a.k1 && k < a.k0)
-    P += 1. / pow(k / a.k0, a.a1);
+    P += exp(-a.a1 * log(k / a.k0));
   if (k > a.k0)
-    P += 1. / pow(k / a.k0, a.a0);
+    P += exp(-a.a0 * log(k / a.k0));
   return P * p0 * unit;
 }
 
@@ -279,9 +304,8 @@ __global__ __launch_bounds__(256) void k_es_draw(double2 *c0, double2 *c1,
       c1[0] = make_double2(0., 0.);
       continue;
     }
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
+    uint64_t i, j, l;
+    box_ijl(a.b, idx, i, j, l);
     const double kx = wavenum(i, a.b.nx, a.b.lx);
     const double ky = wavenum(j, a.b.ny, a.b.ly);
     const double kz = wavenum(l, a.b.nz, a.b.lz);
@@ -306,12 +330,12 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
   const double rho = a.rho;
   const double rho2 = rho * rho;
   const double rhonorm = 1. / sqrt(a.third * rho2 + a.twothirds / rho2);
+  const double inv_rho = 1. / rho;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < a.b.n; idx += stride) {
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
+    uint64_t i, j, l;
+    box_ijl(a.b, idx, i, j, l);
     const Vec3 pos = vertex_pos(a.b, i, j, l);
     const double prof =
         (HEAVY && a.jf12) ? jf12_profile(a.jp, pos) : profile(a.obs, a.r0, a.z0, pos);
@@ -350,9 +374,9 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
     const double hb = dot0(H, b);
     const Vec3 par{H.x * hb, H.y * hb, H.z * hb};
     const Vec3 perp{b.x - par.x, b.y - par.y, b.z - par.z};
-    b = Vec3{(par.x * rho + perp.x / rho) * rhonorm,
-             (par.y * rho + perp.y / rho) * rhonorm,
-             (par.z * rho + perp.z / rho) * rhonorm};
+    b = Vec3{(par.x * rho + perp.x * inv_rho) * rhonorm,
+             (par.y * rho + perp.y * inv_rho) * rhonorm,
+             (par.z * rho + perp.z * inv_rho) * rhonorm};
     c0[idx] = make_double2(b.x, b.y);
     c1[idx] = make_double2(b.y, b.z);
   }
@@ -361,9 +385,8 @@ __global__ __launch_bounds__(256) void k_es_rescale(double2 *c0, double2 *c1,
 // STEP III Gram-Schmidt (brnd_es.cc:211-259) of the symmetrised vector b at
 // index idx: the part of b normal to the wave vector, times sqrt(3).
 __device__ inline Vec3 es_gs_at(const EsArgs &a, uint64_t idx, const Vec3 &b) {
-  const uint64_t i = idx / (a.b.ny * a.b.nz);
-  const uint64_t j = (idx / a.b.nz) % a.b.ny;
-  const uint64_t l = idx % a.b.nz;
+  uint64_t i, j, l;
+  box_ijl(a.b, idx, i, j, l);
   const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
                wavenum(l, a.b.nz, a.b.lz)};
   double k2 = 0.;
@@ -447,7 +470,7 @@ struct DftArgs {
 __device__ inline double dft_spectrum(const DftArgs &a, double k) {
   if (k < a.k0)
     return 0.;
-  return a.rms * pow(k / a.k0, a.a0) / (4 * cgs::pi * k * k);
+  return a.rms * exp(a.a0 * log(k / a.k0)) / (4 * cgs::pi * k * k); // (k / k0)^a0
 }
 
 __global__ __launch_bounds__(256) void k_dft_draw(double2 *tk, DftArgs a,
@@ -459,9 +482,8 @@ __global__ __launch_bounds__(256) void k_dft_draw(double2 *tk, DftArgs a,
       tk[0] = make_double2(0., 0.);
       continue;
     }
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
+    uint64_t i, j, l;
+    box_ijl(a.b, idx, i, j, l);
     const double kx = wavenum(i, a.b.nx, a.b.lx);
     const double ky = wavenum(j, a.b.ny, a.b.ly);
     const double kz = wavenum(l, a.b.nz, a.b.lz);
@@ -479,9 +501,8 @@ __global__ __launch_bounds__(256) void k_dft_rescale(const double2 *tk, double *
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t idx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
        idx < a.b.n; idx += stride) {
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
+    uint64_t i, j, l;
+    box_ijl(a.b, idx, i, j, l);
     const Vec3 pos = vertex_pos(a.b, i, j, l);
     const double ratio = sqrt(profile(a.obs, a.r0, a.z0, pos)) * a.rms * te_var_invsq;
     te[idx] = tk[idx].x * ratio;
@@ -566,6 +587,7 @@ int brnd_generate(hamx_ctx *ctx, const hamx_es_desc *d, uint64_t seed, double *b
   a.k1 = d->k1;
   a.a0 = d->a0;
   a.a1 = d->a1;
+  a.c01 = std::pow(d->k0 / d->k1, d->a1);
   a.rho = d->rho;
   a.r0 = d->r0;
   a.z0 = d->z0;
@@ -748,9 +770,8 @@ __global__ __launch_bounds__(256) void k_mhd_draw(double2 *c0, double2 *c1, MhdA
       c1[0] = double2{0., 0.};
       continue;
     }
-    const uint64_t i = idx / (a.b.ny * a.b.nz);
-    const uint64_t j = (idx / a.b.nz) % a.b.ny;
-    const uint64_t l = idx % a.b.nz;
+    uint64_t i, j, l;
+    box_ijl(a.b, idx, i, j, l);
     const Vec3 k{wavenum(i, a.b.nx, a.b.lx), wavenum(j, a.b.ny, a.b.ly),
                  wavenum(l, a.b.nz, a.b.lz)};
     const double ks = length0(k);
