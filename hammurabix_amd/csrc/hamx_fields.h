@@ -129,11 +129,14 @@ __device__ inline double4 lerp_plane_packed(const double4 *__restrict__ f, uint6
   const double4 a0 = f[i0], a1 = f[i0 + 1];
   const double4 b0 = f[i0 + sy], b1 = f[i0 + sy + 1];
   double4 w;
+  // a (1 - t) + b t with the second product fused: one rounding fewer (the
+  // value moves by at most an ulp) and one FP64 instruction fewer per lerp
+  const double z1 = 1. - zd, y1 = 1. - yd;
 #define HAMX_LERP2(C)                                                          \
   {                                                                            \
-    const double i1 = a0.C * (1. - zd) + a1.C * zd;                            \
-    const double i2 = b0.C * (1. - zd) + b1.C * zd;                            \
-    w.C = i1 * (1. - yd) + i2 * yd;                                            \
+    const double i1 = fma(a1.C, zd, a0.C * z1);                                \
+    const double i2 = fma(b1.C, zd, b0.C * z1);                                \
+    w.C = fma(i2, yd, i1 * y1);                                                \
   }
   HAMX_LERP2(x)
   HAMX_LERP2(y)
@@ -150,11 +153,11 @@ __device__ inline void lerp_cell_packed(const double4 *__restrict__ f,
                                         Vec3 &b, double &te) {
   const double4 w1 = lerp_plane_packed(f, c.idx, sy, c.zd, c.yd);
   const double4 w2 = lerp_plane_packed(f, c.idx + sx, sy, c.zd, c.yd);
-  const double xd = c.xd;
-  b.x = w1.x * (1. - xd) + w2.x * xd;
-  b.y = w1.y * (1. - xd) + w2.y * xd;
-  b.z = w1.z * (1. - xd) + w2.z * xd;
-  te = w1.w * (1. - xd) + w2.w * xd;
+  const double xd = c.xd, x1 = 1. - xd;
+  b.x = fma(w2.x, xd, w1.x * x1);
+  b.y = fma(w2.y, xd, w1.y * x1);
+  b.z = fma(w2.z, xd, w1.z * x1);
+  te = fma(w2.w, xd, w1.w * x1);
 }
 
 // ---------------------------------------------------------------- models --
