@@ -213,15 +213,44 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
   // workgroup on the two fast paths instead of one per element
   const uint64_t base0 = ax.fast ? line_base(ax, L0) : 0;
   const uint64_t left = ax.lines - L0; // lines of this tile that exist (>= 1)
+  // the fast paths issue all of a thread's loads before the first wait
+  // (a loop over the runtime tile size waited for each load before the next
+  // one, leaving one 16-byte load in flight per thread); 256 threads
+  // (in groups of 8, the 2048-point tile's whole share)
+  constexpr uint32_t kPer = TILE / 256, kGroup = kPer < 8 ? kPer : 8;
   if (ax.fast == 1) {
     const uint32_t valid = static_cast<uint32_t>(left < C ? left : C) * n;
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x)
-      lds[e] = e < valid ? data[base0 + e] : make_double2(0., 0.);
+    for (uint32_t g = 0; g < kPer; g += kGroup) {
+      double2 v[kGroup];
+#pragma unroll
+      for (uint32_t k = 0; k < kGroup; ++k) {
+        const uint32_t e = threadIdx.x + 256 * (g + k);
+        v[k] = e < valid ? data[base0 + e] : make_double2(0., 0.);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kGroup; ++k) {
+        const uint32_t e = threadIdx.x + 256 * (g + k);
+        if (e < total)
+          lds[e] = v[k];
+      }
+    }
   } else if (ax.fast == 2) {
-    for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
-      const uint32_t c = e & (C - 1), t = e >> ax.cshift;
-      lds[c * n + t] = c < left ? data[base0 + c + static_cast<uint64_t>(t) * ax.s]
-                                : make_double2(0., 0.);
+    for (uint32_t g = 0; g < kPer; g += kGroup) {
+      double2 v[kGroup];
+#pragma unroll
+      for (uint32_t k = 0; k < kGroup; ++k) {
+        const uint32_t e = threadIdx.x + 256 * (g + k);
+        const uint32_t c = e & (C - 1), t = e >> ax.cshift;
+        v[k] = (e < total && c < left) ? data[base0 + c + static_cast<uint64_t>(t) * ax.s]
+                                       : make_double2(0., 0.);
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kGroup; ++k) {
+        const uint32_t e = threadIdx.x + 256 * (g + k);
+        const uint32_t c = e & (C - 1), t = e >> ax.cshift;
+        if (e < total)
+          lds[c * n + t] = v[k];
+      }
     }
   } else {
     for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
