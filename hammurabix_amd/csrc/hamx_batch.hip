@@ -545,7 +545,7 @@ __device__ inline ShellArgs shell_args_rev(const BInst &in, int s) {
 
 template <bool GRIDS, bool PW1>
 __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, void *slab,
-                            uint32_t *chunk_id, const RayGeo &r) {
+                            const RayGeo &r) {
   const ShellArgs sa = shell_args_rev(in, s);
   const BAdj &A = in.adj;
   Grids gr;
@@ -641,9 +641,9 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, voi
     }
     if (GRIDS && A.ggrid) { // wave-cooperative: every lane calls it
       const bool has = valid && o.cell_ok;
-      scatter_cached<double>(cell_cache_at<double>(slab), static_cast<double *>(A.ggrid), has,
-                             has ? o.cell.idx : 0, o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y,
-                             dB.z, dte, sx, sy, lane, ++*chunk_id);
+      scatter_staged(staged_at(slab), static_cast<double *>(A.ggrid), has,
+                     has ? o.cell.idx : 0, o.cell.xd, o.cell.yd, o.cell.zd, dB.x, dB.y, dB.z, dte,
+                     sx, sy, lane);
     }
   }
   p0 = wave_sum(p0);
@@ -1131,22 +1131,21 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
   // (from here on REV implies FP64: the FP32 reverse returned above)
   const bool cached = REV && GRIDS && in.adj.ggrid;
   if (cached) // the item's rays share one cell cache, flushed at the end
-    cell_cache_init(cell_cache_at<double>(slab), lane);
+    staged_init(staged_at(slab), lane);
   wave_lds_sync();
-  uint32_t chunk = 0;
   for (uint64_t j = 0; j < nr; ++j) {
     const RayGeo r = rays[j];
     const uint64_t p = p0 + j;
     if constexpr (REV)
-      adj_ray_f64<GRIDS, PW1>(in, s, p, lane, slab, &chunk, r);
+      adj_ray_f64<GRIDS, PW1>(in, s, p, lane, slab, r);
     else if constexpr (PREC == HAMX_PREC_F32)
       march_ray_f32<GRIDS, PW1>(in, s, p, lane, r);
     else
       march_ray_f64<GRIDS, PW1>(in, s, p, lane, r);
   }
   if (cached)
-    cell_cache_flush_all(cell_cache_at<double>(slab), static_cast<double *>(in.adj.ggrid),
-                         in.geo.ny * in.geo.nz, in.geo.nz, lane);
+    staged_flush_all(staged_at(slab), static_cast<double *>(in.adj.ggrid),
+                     in.geo.ny * in.geo.nz, in.geo.nz, lane);
   wave_lds_sync(); // the slots are rewritten by the next item
 }
 
@@ -1163,7 +1162,7 @@ __attribute__((amdgpu_waves_per_eu(PREC == HAMX_PREC_F32 && !REV ? 3 : 1))) void
   RayGeo *rays = ray_slots[wave];
   void *slab = nullptr; // reverse mode: one LDS cell cache per wavefront
   if constexpr (REV && GRIDS && PREC == HAMX_PREC_F64) {
-    __shared__ __align__(16) unsigned char caches[4][cell_cache_bytes<double>()];
+    __shared__ __align__(16) unsigned char caches[4][staged_cache_bytes()];
     slab = caches[wave];
   }
   if constexpr (REV && GRIDS && PREC == HAMX_PREC_F32) {

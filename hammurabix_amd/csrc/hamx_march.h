@@ -275,144 +275,6 @@ __device__ inline void grid_atomic_add(T *p, T v) {
 constexpr int kFastRays = 2;
 constexpr int kAdjRays = 2;
 
-// Per-wavefront LDS cache of grid-gradient cells for the FP64 batch's
-// reverse sweep (LDS atomics; the plain adjoint uses scatter_staged below).
-// A straight ray meets each cell in one run of consecutive samples, and
-// the neighbouring rays a wavefront marches next (consecutive RING pixels)
-// cross mostly the same cells.  The cache holds kCacheSlots cells (32
-// values: 8 corners x {x, y, z, te}); a cell's values reach the grid with
-// 32 global atomics when it is evicted or at the end of the wavefront's
-// rays, instead of once per run.
-constexpr int kCacheSlots = 64;
-
-template <typename T>
-struct CellCache {
-  unsigned long long *key; // cell index held by each slot (~0: empty)
-  uint32_t *epoch;         // chunk that last used each slot
-  T *val;                  // kCacheSlots x 32 values
-};
-
-// Bytes of LDS one wavefront's cache takes.
-template <typename T>
-constexpr int cell_cache_bytes() {
-  return kCacheSlots * (8 + 4 + 32 * static_cast<int>(sizeof(T)));
-}
-
-template <typename T>
-__device__ inline CellCache<T> cell_cache_at(void *base) {
-  CellCache<T> c;
-  c.key = static_cast<unsigned long long *>(base);
-  c.epoch = reinterpret_cast<uint32_t *>(c.key + kCacheSlots);
-  c.val = reinterpret_cast<T *>(c.epoch + kCacheSlots);
-  return c;
-}
-
-template <typename T>
-__device__ inline void cell_cache_init(const CellCache<T> &c, int lane) {
-  c.key[lane] = ~0ull; // 64 lanes, 64 slots
-  c.epoch[lane] = 0u;
-  wave_lds_sync();
-}
-
-// Lanes 0..31 add slot s's 32 values to the grid at cell `cell`.
-template <typename T>
-__device__ inline void cell_flush_slot(const CellCache<T> &c, int s, unsigned long long cell,
-                                       T *__restrict__ g, uint64_t sx, uint64_t sy, int lane) {
-  if (lane < 32) {
-    const int corner = lane >> 2, comp = lane & 3;
-    const uint64_t v = cell + (corner >> 2) * sx + ((corner >> 1) & 1) * sy + (corner & 1);
-    grid_atomic_add(g + 4 * v + comp, c.val[32 * s + lane]);
-  }
-}
-
-template <typename T>
-__device__ inline void cell_cache_flush_all(const CellCache<T> &c, T *__restrict__ g,
-                                            uint64_t sx, uint64_t sy, int lane) {
-  for (int s = 0; s < kCacheSlots; ++s) {
-    const unsigned long long k = c.key[s]; // uniform
-    if (k != ~0ull)
-      cell_flush_slot(c, s, k, g, sx, sy, lane);
-  }
-  wave_lds_sync();
-  c.key[lane] = ~0ull;
-  wave_lds_sync();
-}
-
-__device__ inline int cell_hash(unsigned long long idx) {
-  return static_cast<int>((idx * 0x9E3779B97F4A7C15ull) >> 58); // 6 bits
-}
-
-// Scatter-add of the 8-corner trilinear contributions of every lane with
-// `has` (the transpose of lerp_cell_packed) through the wavefront's cell
-// cache.  All 64 lanes must call it; `chunk` is a wave-uniform counter that
-// changes from call to call.  A run whose cell's slot was already taken in
-// this call by another cell adds straight to the grid.
-template <typename T>
-__device__ inline void scatter_cached(const CellCache<T> &c, T *__restrict__ g, bool has,
-                                      uint64_t idx, T xd, T yd, T zd, T g0, T g1, T g2, T g3,
-                                      uint64_t sx, uint64_t sy, int lane, uint32_t chunk) {
-  const unsigned long long key = has ? idx : ~0ull;
-  const unsigned long long prev = __shfl_up(key, 1, 64);
-  const bool head = lane == 0 || key != prev;
-  const unsigned long long heads = __ballot(head);
-  // the head lane of this lane's run: the highest head at or below it
-  const int my_head = 63 - __clzll(heads & ((2ull << lane) - 1ull));
-  int my_slot = -1;
-  unsigned long long todo = __ballot(head && has);
-  while (todo) { // one run at a time (wave-uniform)
-    const int h = __ffsll(todo) - 1;
-    todo &= todo - 1;
-    const unsigned long long cell = __shfl(key, h, 64);
-    const int s = cell_hash(cell);
-    const unsigned long long held = c.key[s];
-    int slot = s;
-    if (held != cell) {
-      if (held != ~0ull && c.epoch[s] == chunk) {
-        slot = -1; // taken by another cell of this chunk: go straight to the grid
-      } else {
-        if (held != ~0ull)
-          cell_flush_slot(c, s, held, g, sx, sy, lane);
-        wave_lds_sync();
-        if (lane < 32)
-          c.val[32 * s + lane] = T(0);
-        if (lane == 0)
-          c.key[s] = cell;
-      }
-    }
-    if (slot >= 0 && lane == 0)
-      c.epoch[s] = chunk;
-    wave_lds_sync();
-    if (my_head == h)
-      my_slot = slot;
-  }
-  if (has) {
-    const T wx[2] = {T(1) - xd, xd}, wy[2] = {T(1) - yd, yd}, wz[2] = {T(1) - zd, zd};
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int cz = 0; cz < 2; ++cz) {
-          const T w = wx[a] * wy[b] * wz[cz];
-          const int corner = 4 * a + 2 * b + cz;
-          if (my_slot >= 0) {
-            T *q = c.val + 32 * my_slot + 4 * corner;
-            atomicAdd(q + 0, w * g0);
-            atomicAdd(q + 1, w * g1);
-            atomicAdd(q + 2, w * g2);
-            atomicAdd(q + 3, w * g3);
-          } else {
-            T *p = g + 4 * (idx + a * sx + b * sy + cz);
-            grid_atomic_add(p + 0, w * g0);
-            grid_atomic_add(p + 1, w * g1);
-            grid_atomic_add(p + 2, w * g2);
-            grid_atomic_add(p + 3, w * g3);
-          }
-        }
-  }
-  wave_lds_sync();
-}
-
 // ------------------------------------------------ staged grid scatter --
 // The plain adjoint's grid-gradient scatter.  A contended FP64 LDS atomic
 // costs about 30 times a plain 64-lane LDS store or load (measured on
