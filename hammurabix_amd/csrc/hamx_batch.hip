@@ -140,6 +140,24 @@ __device__ inline uint64_t uniform_u64(uint64_t v) {
   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+__device__ inline uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
 __device__ inline ShellArgs shell_args(const BInst &in, int s) {
   ShellArgs sa;
   sa.hs = in.sh[s].hs;
@@ -527,6 +545,63 @@ __device__ void march_ray_f32(const BInst &in, int s, uint64_t ipix, int lane,
   }
 }
 
+// The forward with one ray per lane (64-ray items, NESTED 8x8 patches as in
+// the reverse sweep): each lane walks its own ray from the observer outward,
+// its FD a lane-private running sum (no wave scan), its map sums in FP64.
+// The wavefront steps over the union of its lanes' step ranges.  The plan's
+// autotuner times this shape against the lanes-per-sample ones.
+template <bool GRIDS, bool PW1>
+__device__ void march_lanes_f32(const BInst &in, int s, uint64_t ipix, bool live,
+                                const RayGeo &r) {
+  // FP32 sample arithmetic: a * b + c may fuse (more accurate, fewer instructions)
+  #pragma clang fp contract(fast)
+  const BShell &sh = in.sh[s];
+  const float3 lf = make_float3(static_cast<float>(r.l.x), static_cast<float>(r.l.y),
+                                static_cast<float>(r.l.z));
+  const float3 ethf = make_float3(static_cast<float>(r.eth.x), static_cast<float>(r.eth.y),
+                                  static_cast<float>(r.eth.z));
+  const float3 ephf = make_float3(static_cast<float>(r.eph.x), static_cast<float>(r.eph.y),
+                                  static_cast<float>(r.eph.z));
+  const int nf = in.n_freq;
+  double dm = 0., I = 0., fd = 0., Q[kMaxF] = {0., 0., 0., 0.}, U[kMaxF] = {0., 0., 0., 0.};
+  const uint64_t k_end = wave_max_u64(live ? r.k_hi : 0ull);
+  const uint64_t k_beg = wave_min_u64(live ? r.k_lo : ~0ull);
+  for (uint64_t k = k_beg; k < k_end; ++k) {
+    S32 o;
+    if (live && k >= r.k_lo && k < r.k_hi && eval_f32<GRIDS, PW1>(in, sh, r, lf, ethf, ephf, k, o)) {
+      fd += o.fdk * sh.fdu; // the FD through this sample, itself included
+      dm += static_cast<double>(o.te);
+      I += static_cast<double>(o.w);
+#pragma unroll
+      for (int f = 0; f < kMaxF; ++f) {
+        if (f < nf) {
+          const double X = in.fdin[f] ? r.fd_inner + fd : fd;
+          double rv = X * in.rev[f];
+          rv -= rint(rv);
+          const float a = static_cast<float>(rv);
+          const float sn = __builtin_amdgcn_sinf(a), cs = __builtin_amdgcn_cosf(a);
+          Q[f] += static_cast<double>((cs * o.c2i - sn * o.s2i) * o.w);
+          U[f] += static_cast<double>((sn * o.c2i + cs * o.s2i) * o.w);
+        }
+      }
+    }
+  }
+  if (live) {
+    const double dl = sh.step ? sh.delta : 0.; // no samples: no finite delta
+    if (in.do_dm)
+      in.t_dm[ipix] = dm * dl;
+    in.t_fd[in.fd_off[s] + ipix] = fd;
+#pragma unroll
+    for (int f = 0; f < kMaxF; ++f) {
+      if (f < nf) {
+        in.t_i[f][ipix] = I * dl * in.st[f];
+        in.t_q[f][ipix] = Q[f] * dl * in.sp[f];
+        in.t_u[f][ipix] = U[f] * dl * in.sp[f];
+      }
+    }
+  }
+}
+
 // -------------------------------------------------------- reverse mode --
 //
 // One sweep per ray from its far end (as k_los_adj): the running FD is the
@@ -675,23 +750,6 @@ __device__ void adj_ray_f64(const BInst &in, int s, uint64_t ipix, int lane, voi
 // ray meets each cell in one run of samples).  No cross-lane reduction,
 // LDS atomic or wavefront barrier is left in the sample loop.
 
-__device__ inline uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint64_t w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-
-__device__ inline uint64_t wave_min_u64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint64_t w = __shfl_xor(v, o, 64);
-    v = w < v ? w : v;
-  }
-  return v;
-}
 
 // Per-wavefront LDS merge of the lanes' gradient flushes.  The 64 rays of a
 // NESTED 8x8 patch leave the same grid vertices behind a few steps apart, so
@@ -1104,6 +1162,20 @@ __device__ void do_item(const LevelArgs &a, uint64_t item, int lane, void *slab,
   const int s = static_cast<int>(a.level);
   const uint64_t npix = in.sh[s].hs.npix;
   const uint64_t p0 = (item - a.prefix[lo]) * a.rays;
+  if constexpr (!REV && PREC == HAMX_PREC_F32) {
+    if (a.rays == 64) { // one ray per lane, as the reverse sweep below
+      const Healpix &hs = in.sh[s].hs;
+      const uint64_t pl = p0 + static_cast<uint64_t>(lane);
+      const bool live = pl < npix;
+      const uint64_t ipix = live ? hp_nest2ring(hs, pl) : 0;
+      RayGeo rs{};
+      if (live)
+        ray_setup_call(in.ph, shell_args(in, s), ipix, rs);
+      const RayGeo r = rs;
+      march_lanes_f32<GRIDS, PW1>(in, s, ipix, live, r);
+      return;
+    }
+  }
   if constexpr (REV && PREC == HAMX_PREC_F32) {
     // one ray per lane (a.rays == 64): a NESTED 8x8 patch of the shell (the
     // lanes' samples at one step then lie within a few grid cells; shell
@@ -1512,6 +1584,9 @@ const Variant kVariants[] = {
     {1, 0, 16, 4, "persistent global queue, 16 rays/item, 4 waves/WG"},
     {1, 1, 4, 4, "persistent XCD queues, 4 rays/item, 4 waves/WG"},
     {1, 1, 16, 4, "persistent XCD queues, 16 rays/item, 4 waves/WG"},
+    // FP32 only: one ray per lane (64-ray items, NESTED 8x8 patches)
+    {1, 1, 64, 4, "persistent XCD queues, 64 rays/item (lane per ray)"},
+    {0, 1, 64, 4, "one-shot XCD-ordered, 64 rays/item (lane per ray)"},
 };
 constexpr uint32_t kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -2474,6 +2549,13 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
       return fail(set_err(ctx, HAMX_ERR_DEVICE, "hipEventCreate failed"));
     double best_ms = 1e300;
     for (uint32_t v = 0; v < kNumVariants && rc == HAMX_OK; ++v) {
+      // lane-per-ray items exist only in the FP32 march (the FP64 one sets
+      // up at most kItemRaysMax rays per item in LDS)
+      if (kVariants[v].rays > static_cast<uint32_t>(kItemRaysMax) &&
+          b->prec != HAMX_PREC_F32) {
+        b->info.variant_ms[v] = -1.;
+        continue;
+      }
       rc = launch_all(b, v);
       if (rc == HAMX_OK)
         (void)hipEventRecord(e0, ctx->stream);
