@@ -287,7 +287,7 @@ constexpr int kAdjRays = 2;
 // each slot serves one run), or straight to the grid when an earlier run
 // of the chunk holds the slot.  A cell reaches the grid (32 global atomics)
 // when its slot is taken by another cell and at the end of the wavefront.
-constexpr int kStageSlots = 32;
+constexpr int kStageSlots = 40; // the most that keeps two 4-wave workgroups per CU
 constexpr int kStageStride = 65; // doubles per staged row: 64 lanes + 1, so rows start in different banks
 
 struct StagedCache {
@@ -315,7 +315,8 @@ __device__ inline void staged_init(const StagedCache &c, int lane) {
 }
 
 __device__ inline int staged_hash(unsigned long long idx) {
-  return static_cast<int>((idx * 0x9E3779B97F4A7C15ull) >> 59); // 5 bits
+  // Fibonacci hash to 32 bits, then multiply-shift onto the slots
+  return static_cast<int>(((idx * 0x9E3779B97F4A7C15ull) >> 32) * kStageSlots >> 32);
 }
 
 __device__ inline unsigned long long lane_u64(unsigned long long v, int L) {
@@ -335,11 +336,11 @@ __device__ inline void staged_grid_add(double *__restrict__ g, unsigned long lon
 // The slots in `slots` (bits), holding the cells the wavefront reads from
 // c.key, emptied into the grid: two slots per pass, lanes 0-31 and 32-63.
 __device__ inline void staged_flush(const StagedCache &c, double *__restrict__ g,
-                                    uint32_t slots, uint64_t sx, uint64_t sy, int lane) {
+                                    unsigned long long slots, uint64_t sx, uint64_t sy, int lane) {
   while (slots) { // uniform
-    const int s0 = __ffs(slots) - 1;
+    const int s0 = __ffsll(slots) - 1;
     slots &= slots - 1;
-    const int s1 = slots ? __ffs(slots) - 1 : -1;
+    const int s1 = slots ? __ffsll(slots) - 1 : -1;
     if (s1 >= 0)
       slots &= slots - 1;
     const int s = lane < 32 ? s0 : s1;
@@ -353,7 +354,7 @@ __device__ inline void staged_flush(const StagedCache &c, double *__restrict__ g
 __device__ inline void staged_flush_all(const StagedCache &c, double *__restrict__ g,
                                         uint64_t sx, uint64_t sy, int lane) {
   const unsigned long long k = lane < kStageSlots ? c.key[lane] : ~0ull;
-  const uint32_t held = static_cast<uint32_t>(__ballot(k != ~0ull));
+  const unsigned long long held = __ballot(k != ~0ull);
   staged_flush(c, g, held, sx, sy, lane);
   wave_lds_sync();
   if (lane < kStageSlots)
@@ -381,22 +382,22 @@ __device__ inline void scatter_staged(const StagedCache &c, double *__restrict__
   const int s_own = staged_hash(key);
   const unsigned long long held = (head && has) ? c.key[s_own] : ~0ull;
   unsigned long long direct = 0; // runs (by head lane) that go to the grid
-  uint32_t taken = 0;
+  unsigned long long taken = 0;
   for (unsigned long long t = runs; t; t &= t - 1) { // uniform, scalar
     const int h = __ffsll(t) - 1;
     const int sh = __builtin_amdgcn_readlane(s_own, h);
-    if ((taken >> sh) & 1u)
+    if ((taken >> sh) & 1ull)
       direct |= 1ull << h;
     else
-      taken |= 1u << sh;
+      taken |= 1ull << sh;
   }
   const bool mine = head && has && !((direct >> lane) & 1ull);
   const bool fresh = mine && held != key;           // the run claims its slot
   const bool evict = fresh && held != ~0ull;        // ... from another cell
   const unsigned long long fresh_runs = __ballot(fresh);
-  uint32_t ev_slots = 0;
+  unsigned long long ev_slots = 0;
   for (unsigned long long t = __ballot(evict); t; t &= t - 1)
-    ev_slots |= 1u << __builtin_amdgcn_readlane(s_own, __ffsll(t) - 1);
+    ev_slots |= 1ull << __builtin_amdgcn_readlane(s_own, __ffsll(t) - 1);
   staged_flush(c, g, ev_slots, sx, sy, lane);
   wave_lds_sync();
   if (fresh)
