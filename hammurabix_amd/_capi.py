@@ -209,6 +209,7 @@ HAMX_PREC_F64 = 0
 HAMX_PREC_F32 = 1
 HAMX_BATCH_NO_TUNE = 4
 HAMX_BATCH_ADJOINT = 32
+HAMX_BATCH_LANES = 64
 
 
 class HamxBatchInst(C.Structure):

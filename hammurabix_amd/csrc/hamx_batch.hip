@@ -2541,8 +2541,16 @@ extern "C" int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uin
   b->info.samples_geo = samples_geo;
   b->info.samples_ref = samples_ref;
   b->info.levels = b->levels;
-  const bool tune = (flags & HAMX_BATCH_NO_TUNE) == 0;
+  const bool lanes = (flags & HAMX_BATCH_LANES) != 0;
+  const bool tune = (flags & HAMX_BATCH_NO_TUNE) == 0 && !lanes;
   uint32_t best = 0;
+  if (lanes) { // the first one-ray-per-lane shape
+    if (b->prec != HAMX_PREC_F32)
+      return fail(set_err(ctx, HAMX_ERR_ARG,
+                          "hamx_batch_create: HAMX_BATCH_LANES needs HAMX_PREC_F32"));
+    while (kVariants[best].rays <= static_cast<uint32_t>(kItemRaysMax))
+      ++best;
+  }
   if (tune) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)

@@ -219,7 +219,7 @@ class Context:
 
     # -------------------------------------------------------------- batch
     def batch(self, instances: list[dict], precision: int = capi.HAMX_PREC_F32,
-              tune: bool = True, adjoint: bool = False) -> "Batch":
+              tune: bool = True, adjoint: bool = False, lanes: bool = False) -> "Batch":
         """Plan a batched LOS run (hamx_batch_create).  Each instance is a dict:
         desc (HamxLosDesc), freqs (1-4 Hz values, <sync> list order), maps
         (dm / fd tensors or None, is / qs / us lists of tensors, one per
@@ -227,8 +227,10 @@ class Context:
         adjoint=True an instance may also hold adj (dL/d(maps) in the layout
         of maps) and grads (dparams: 6 values; dbrnd: 3 grid tensors;
         dternd), filled by Batch.run_adjoint().  Every array is a float64
-        CUDA tensor; addresses are fixed for the plan's lifetime."""
-        return Batch(self, instances, precision, tune, adjoint)
+        CUDA tensor; addresses are fixed for the plan's lifetime.  lanes=True
+        takes the FP32 forward with one ray per lane instead of timing the
+        launch shapes."""
+        return Batch(self, instances, precision, tune, adjoint, lanes)
 
     # -------------------------------------------------------- random fields
     def brnd_es(self, desc: capi.HamxEsDesc, seed: int, bx: Any, by: Any, bz: Any,
@@ -313,7 +315,7 @@ class Batch:
     """A planned batch: run() replays the captured graph."""
 
     def __init__(self, ctx: Context, instances: list[dict], precision: int, tune: bool,
-                 adjoint: bool = False):
+                 adjoint: bool = False, lanes: bool = False):
         self.ctx = ctx
         self.keep = []
         arr = (capi.HamxBatchInst * len(instances))()
@@ -375,6 +377,8 @@ class Batch:
             flags |= capi.HAMX_BATCH_NO_TUNE
         if adjoint:
             flags |= capi.HAMX_BATCH_ADJOINT
+        if lanes:
+            flags |= capi.HAMX_BATCH_LANES
         h = C.c_void_p()
         _order_after_torch(self.keep)
         ctx._check(ctx.lib.hamx_batch_create(ctx.handle, arr, len(instances), precision,

@@ -364,11 +364,13 @@ typedef struct hamx_batch_info {
  * captures the chosen one in a HIP graph.  Input grids are re-read and
  * output maps overwritten by every hamx_batch_run; their addresses are
  * fixed at creation.  flags: HAMX_IN_DEVICE | HAMX_OUT_DEVICE (required),
- * plus HAMX_BATCH_NO_TUNE to skip the timing and take the first shape, and
- * HAMX_BATCH_ADJOINT to record the forward's tape on every replay and
- * capture the reverse-mode graph as well. */
+ * plus HAMX_BATCH_NO_TUNE to skip the timing and take the first shape,
+ * HAMX_BATCH_LANES to skip it and take the forward with one ray per lane
+ * (HAMX_PREC_F32 only), and HAMX_BATCH_ADJOINT to record the forward's tape
+ * on every replay and capture the reverse-mode graph as well. */
 #define HAMX_BATCH_NO_TUNE 4u
 #define HAMX_BATCH_ADJOINT 32u
+#define HAMX_BATCH_LANES 64u
 int hamx_batch_create(hamx_ctx *ctx, const hamx_batch_inst *inst, uint32_t n_inst,
                       uint32_t precision, uint32_t flags, hamx_batch **out);
 /* Replay the captured graph once on the context's stream.  Synchronous. */

@@ -159,8 +159,11 @@ def test_graph_replays_and_follows_new_grids(ctx):
 
 
 def test_tuned_shapes_give_identical_maps(ctx):
-    """Every launch shape computes each ray the same way: the tuned plan's
-    maps equal the untuned plan's bit for bit."""
+    """The lanes-per-sample launch shapes compute each ray the same way, so
+    a tuned plan that picks one of them equals the untuned plan bit for bit;
+    the one-ray-per-lane forward sums its rays in another order (FP64 lane
+    sums instead of FP32 lane partials and a wave sum) and equals it to the
+    FP32 batch's rounding."""
     print("case: batch tuned vs untuned", flush=True)
     import torch
     specs_a = mixed_batch()
@@ -172,12 +175,72 @@ def test_tuned_shapes_give_identical_maps(ctx):
         info = b.info()
         print("tuned:", info["shape"], info["variant_ms"], flush=True)
         assert len(info["variant_ms"]) >= 4
+    lane_shape = "lane per ray" in info["shape"]
     for sa, sb in zip(specs_a, specs_b):
         for k in sa["maps"]:
             va, vb = sa["maps"][k], sb["maps"][k]
             for a, c in zip(va if isinstance(va, list) else [va],
                             vb if isinstance(vb, list) else [vb]):
-                assert torch.equal(a, c), k
+                if lane_shape:
+                    err = rel_l2(c.cpu().numpy(), a.cpu().numpy())
+                    assert err <= 1e-6, (k, err)
+                else:
+                    assert torch.equal(a, c), k
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_lane_per_ray_forward(ctx, adjoint):
+    """The FP32 forward with one ray per lane (HAMX_BATCH_LANES) against the
+    plain FP64 passes, at the FP32 batch's map tolerance; with the reverse
+    graph captured too, the same maps."""
+    print(f"case: batch lane per ray, adjoint {adjoint}", flush=True)
+    specs = mixed_batch()
+    if adjoint:
+        specs = with_adjoint(specs, seed=23)
+    with ctx.batch(specs, precision=capi.HAMX_PREC_F32, lanes=True, adjoint=adjoint) as b:
+        assert "lane per ray" in b.info()["shape"]
+        b.run()
+        for i, s in enumerate(specs):
+            compare(s, plain_passes(ctx, s), 1e-5, f"lanes inst{i}")
+        if not adjoint:
+            return
+        b.run_adjoint()
+    # the reverse graph reads the forward's tape: the same gradients as a
+    # plan with the lanes-per-sample forward, to the FP32 batch's rounding
+    ref = with_adjoint(mixed_batch(), seed=23)
+    with ctx.batch(ref, precision=capi.HAMX_PREC_F32, tune=False, adjoint=True) as b:
+        b.run()
+        b.run_adjoint()
+    for i, (s, r) in enumerate(zip(specs, ref)):
+        for name in ("dparams", "dbrnd", "dternd"):
+            if name not in r["grads"]:
+                continue
+            got, exp = s["grads"][name], r["grads"][name]
+            for g, e in zip(got if isinstance(got, tuple) else [got],
+                            exp if isinstance(exp, tuple) else [exp]):
+                err = rel_l2(g.cpu().numpy(), e.cpu().numpy())
+                print(f"lanes inst{i} {name}: rel L2 vs lanes-per-sample {err:.3e}", flush=True)
+                assert err <= 1e-5, (i, name, err)
+
+
+def test_lsa_angles_beyond_a_quarter_turn_in_batches(ctx):
+    """psi0 or chi0 beyond 45 degrees: the FP32 LSA's sines and cosines take
+    the quadrant-reducing path instead of the series, in both precisions."""
+    print("case: batch LSA angles beyond pi/4", flush=True)
+    g1 = grids((32, 32, 16), 1)
+    for prec, tol in ((capi.HAMX_PREC_F32, 1e-5), (capi.HAMX_PREC_F64, 1e-12)):
+        specs = [instance(4, 16, FREQS[2:3], psi0=1.2, g=g1),
+                 instance(4, 16, FREQS[1:3], g=g1)]
+        specs[1]["desc"].breg[3] = -1.3  # chi0
+        with ctx.batch(specs, precision=prec, tune=False) as b:
+            b.run()
+            for i, s in enumerate(specs):
+                compare(s, plain_passes(ctx, s), tol, f"quarter-turn prec{prec} inst{i}")
+
+
+def test_lane_per_ray_needs_f32(ctx):
+    with pytest.raises(RuntimeError, match="HAMX_BATCH_LANES"):
+        ctx.batch(mixed_batch(), precision=capi.HAMX_PREC_F64, lanes=True)
 
 
 def test_batch_rejects_bad_instances(ctx):

@@ -297,3 +297,39 @@ def test_nan_in_a_grid_is_an_error(ctx, where):
     assert e.value.code == capi.HAMX_ERR_ARG and "non-finite" in str(e.value)
     assert_parity(gpu_maps(ctx, d, brnd=b, brnd_box=box, ternd=te, ternd_box=box),
                   write_grid(d, brnd=b, brnd_box=box, ternd=te, ternd_box=box), label="after-nan")
+
+
+def test_phase_rotation_beyond_the_reduced_range(ctx):
+    """At 3 MHz (lambda^2 = 1e4 m^2) the Faraday phase 2 lambda^2 FD of the
+    outer samples exceeds 2^19 pi, where the march's Cody-Waite reduction
+    hands over to the library sine and cosine; the inner ones stay below it.
+    DM, FD and I at the plain-path bar; Q and U at 1e-6: an FD summed in
+    another order differs by ~1e-15 relative, which moves a phase of millions
+    of radians by ~1e-9 (a wrong sine or cosine would be off by O(1))."""
+    print("case: Faraday phase beyond 2^19 pi", flush=True)
+    par = Param(text=configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05))
+    d = par.los_desc()
+    d.sync_freq = 3e6
+    got, ref = gpu_maps(ctx, d), write_grid(d)
+    lam2 = (CGS.c_light / d.sync_freq) ** 2  # cm^2; the raw FD map is per cm^2
+    big = np.abs(2 * lam2 * ref["fd"]).max()
+    print(f"largest shell-end phase {big:.3e} rad (reduction limit {2**19 * math.pi:.3e})",
+          flush=True)
+    assert big > 2 ** 19 * math.pi
+    assert_parity({k: got[k] for k in ("dm", "fd", "is")}, {k: ref[k] for k in ("dm", "fd", "is")},
+                  label="3 MHz")
+    assert_parity({k: got[k] for k in ("qs", "us")}, {k: ref[k] for k in ("qs", "us")}, tol=1e-6,
+                  label="3 MHz")
+
+
+@pytest.mark.parametrize("angles", [(60.0, 25.0), (27.0, 70.0), (-100.0, -50.0)])
+def test_lsa_angles_beyond_a_quarter_turn(ctx, angles):
+    """psi0 or chi0 beyond 45 degrees: psi(r), chi(z) leave [-pi/4, pi/4], so
+    the LSA's sines and cosines take the library path instead of the series."""
+    psi0, chi0 = angles
+    print(f"case: LSA psi0 {psi0} chi0 {chi0} degrees", flush=True)
+    par = Param(text=configs.plain_forward(nside_sim=8, nside_out=8, oc_r_res=0.05))
+    d = par.los_desc()
+    d.breg[1] = math.radians(psi0)
+    d.breg[3] = math.radians(chi0)
+    assert_parity(gpu_maps(ctx, d), write_grid(d), label=f"lsa {psi0}/{chi0}")
