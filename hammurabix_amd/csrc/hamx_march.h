@@ -402,24 +402,23 @@ __device__ inline void scatter_staged(const StagedCache &c, double *__restrict__
   wave_lds_sync();
   if (fresh)
     c.key[s_own] = key;
-  // this lane's 32 corner values (0 without a sample)
-  const double wx[2] = {1. - xd, xd}, wy[2] = {1. - yd, yd}, wz[2] = {1. - zd, zd};
-  double v[32];
-#pragma unroll
-  for (int cn = 0; cn < 8; ++cn) {
-    const double w = has ? wx[cn >> 2] * wy[(cn >> 1) & 1] * wz[cn & 1] : 0.;
-    v[4 * cn + 0] = w * g0;
-    v[4 * cn + 1] = w * g1;
-    v[4 * cn + 2] = w * g2;
-    v[4 * cn + 3] = w * g3;
-  }
+  // this lane's corner values (0 without a sample), 16 per half: corners
+  // 4 half .. 4 half + 3 (the x = half plane), formed just before they are
+  // staged so that only half of them hold registers
+  const double wy[2] = {1. - yd, yd}, wz[2] = {1. - zd, zd};
   const int vi = lane & 15, grp = lane >> 4;
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
+    const double wxh = has ? (half ? xd : 1. - xd) : 0.;
     wave_lds_sync(); // the previous half's loads are done before the rows are rewritten
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      c.stage[i * kStageStride + lane] = v[16 * half + i];
+    for (int cq = 0; cq < 4; ++cq) {
+      const double w = wxh * wy[cq >> 1] * wz[cq & 1];
+      c.stage[(4 * cq + 0) * kStageStride + lane] = w * g0;
+      c.stage[(4 * cq + 1) * kStageStride + lane] = w * g1;
+      c.stage[(4 * cq + 2) * kStageStride + lane] = w * g2;
+      c.stage[(4 * cq + 3) * kStageStride + lane] = w * g3;
+    }
     wave_lds_sync();
     for (unsigned long long t = runs; t;) { // uniform: 4 runs per round
       int a[4], b[4], len = 0;
