@@ -276,7 +276,8 @@ constexpr int kFastRays = 2;
 constexpr int kAdjRays = 2;
 
 // ------------------------------------------------ staged grid scatter --
-// The plain adjoint's grid-gradient scatter.  A contended FP64 LDS atomic
+// The grid-gradient scatter of the FP64 reverse sweeps (the plain adjoint
+// and the FP64 batch), lanes = consecutive samples.  A contended FP64 LDS atomic
 // costs about 30 times a plain 64-lane LDS store or load (measured on
 // gfx950), so the 32 corner values of a chunk's samples are merged without
 // atomics: every lane stores its values into a per-wavefront staging area
