@@ -204,6 +204,20 @@ def run_batch(args):
                                    args.steps, args.warmup)), flush=True)
 
 
+def rms_scaled(pairs):
+    """max_i |batch_i - plain_i| / rms_i(plain_i) per parameter, over the
+    (batch, plain) gradient pairs of the checked instances."""
+    import torch
+    from hammurabix_amd import _capi as capi
+    if not pairs:
+        return {}
+    got = torch.stack([p[0] for p in pairs])
+    ref = torch.stack([p[1] for p in pairs])
+    rms = ref.square().mean(0).sqrt().clamp_min(1e-300)
+    worst = ((got - ref).abs().max(0).values / rms).tolist()
+    return dict(zip(capi.PARAM_NAMES, worst))
+
+
 def measure_batch(large, n_inst, check, precision, with_adj, steps, warmup):
     """The batch workloads; returns the bench line as a dict.
 
@@ -305,6 +319,7 @@ def measure_batch(large, n_inst, check, precision, with_adj, steps, warmup):
         map_err = {}  # worst rel. L2 per map kind and frequency
         worst_dln = 0.0
         dp_err = dict.fromkeys(capi.PARAM_NAMES, 0.0)
+        dp_pairs = []  # (batch, plain FP64) parameter gradients per checked instance
         for ci, sp in enumerate(specs[:n_check]):
             progress(f"plain-path comparison, instance {ci + 1} of {n_check}")
             freqs = sp["freqs"]
@@ -345,6 +360,7 @@ def measure_batch(large, n_inst, check, precision, with_adj, steps, warmup):
                     map_err[name] = max(map_err.get(name, 0.0), err)
             if with_adj:
                 g = sp["grads"]
+                dp_pairs.append((g["dparams"].clone(), acc_dp.clone()))
                 rel = (g["dparams"] - acc_dp).abs() / acc_dp.abs().clamp_min(1e-300)
                 worst_dp = max(worst_dp, float(rel.max()))
                 for name, e in zip(capi.PARAM_NAMES, rel.tolist()):
@@ -413,6 +429,11 @@ def measure_batch(large, n_inst, check, precision, with_adj, steps, warmup):
                     (mg["batch_map_gradient_evals_per_s"] * (plain_s + plain_mg_s) / n_check),
                 "max_rel_err_dparams_vs_plain_fp64": worst_dp,
                 "max_rel_err_by_param": dp_err,
+                # each parameter's worst absolute error over the RMS of its
+                # gradient across the checked instances: the error at the
+                # scale an optimiser or sampler sees, where the relative error
+                # of an instance whose gradient happens to be near zero is not
+                "max_err_by_param_over_rms_gradient": rms_scaled(dp_pairs),
                 "max_rel_l2_dlnparams_vs_plain_fp64": worst_dln,
                 "max_rel_l2_grid_gradient_vs_plain_fp64": worst_grid} if with_adj else {}),
             "plan_s": plan_s,
@@ -698,8 +719,8 @@ def main():
                 "batch_map_gradient_ms", "batch_map_gradient_evals_per_s",
                 "plain_map_gradient_evals_per_s", "map_gradient_speedup_vs_plain",
                 "speedup_vs_plain_pass_by_pass", "max_rel_l2_vs_plain_fp64",
-                "max_rel_err_by_param", "max_rel_err_dparams_vs_plain_fp64",
-                "max_rel_l2_dlnparams_vs_plain_fp64",
+                "max_rel_err_by_param", "max_err_by_param_over_rms_gradient",
+                "max_rel_err_dparams_vs_plain_fp64", "max_rel_l2_dlnparams_vs_plain_fp64",
                 "max_rel_l2_grid_gradient_vs_plain_fp64") if k in bd},
             "launch_shape": batch["config"]["launch_shape"],
         }
