@@ -176,10 +176,10 @@ __device__ inline Vec3 breg_eval(int model, const double *p, const Vec3 &pos) {
       return b;
     const double cos_p = pos.x / r;
     const double sin_p = pos.y / r;
-    const double psi = p[1] + p[2] * log(r / (8. * cgs::kpc));
+    const double psi = p[1] + p[2] * log_pos(r / (8. * cgs::kpc));
     // tanh(u) = (1 - e) / (1 + e), e = exp(-2|u|): absolute error ~1e-16
     const double u = pos.z / (1. * cgs::kpc);
-    const double e = exp(-2. * fabs(u));
+    const double e = exp_fast(-2. * fabs(u));
     const double chi = p[3] * copysign((1. - e) * rcp_f64(1. + e), u);
     double sps, cps, sch, cch;
     sincos_fast(psi, sps, cps);

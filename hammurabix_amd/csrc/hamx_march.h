@@ -535,12 +535,12 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   // exp(-2|z| / kpc): the LSA's tanh(z / kpc) and, when the CRE scale
   // height is 1 kpc, the CRE's sech^2(z / z0)
   const double u = pos.z * fc.inv_kpc;
-  const double e_kpc = (o.lsa_in || fc.cre_z_kpc) ? exp(-2. * fabs(u)) : 0.;
+  const double e_kpc = (o.lsa_in || fc.cre_z_kpc) ? exp_fast(-2. * fabs(u)) : 0.;
   if (o.lsa_in) {
     const double inv_r = rcp_f64(o.rr);
     o.cos_p = pos.x * inv_r;
     o.sin_p = pos.y * inv_r;
-    o.lnr = log(o.rr * fc.inv_8kpc);
+    o.lnr = log_pos(o.rr * fc.inv_8kpc);
     // tanh(u) = (1 - e) / (1 + e) with e = exp(-2|u|): the absolute error
     // stays at the rounding of 1 - e (1e-16), which is what chi needs
     o.tnh = copysign((1. - e_kpc) * rcp_f64(1. + e_kpc), u);
@@ -602,9 +602,9 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     o.prof = (d.x * d.x + d.y * d.y + d.z * d.z > fc.cre_r02) ? 0. : 1.;
   } else {
     // sech^2(z / z0) = 4 e / (1 + e)^2 with e = exp(-2 |z| / z0) <= 1
-    const double e = fc.cre_z_kpc ? e_kpc : exp(-2. * fabs(pos.z * fc.inv_cre_z0));
+    const double e = fc.cre_z_kpc ? e_kpc : exp_fast(-2. * fabs(pos.z * fc.inv_cre_z0));
     const double d = 1. + e;
-    o.prof = exp((fc.R0 - o.rr) * fc.inv_cre_r0) * (4. * e) * rcp_f64(d * d);
+    o.prof = exp_fast((fc.R0 - o.rr) * fc.inv_cre_r0) * (4. * e) * rcp_f64(d * d);
   }
   double w;
   if (PW1)

@@ -59,7 +59,7 @@ __device__ inline void gauss_pair(uint64_t seed, uint32_t stream, uint64_t m,
   const uint64_t b = (static_cast<uint64_t>(c[3]) << 32) | c[2];
   const double u1 = (static_cast<double>(a >> 11) + 1.0) * 0x1.0p-53;
   const double u2 = static_cast<double>(b >> 11) * 0x1.0p-53;
-  const double r = sqrt(-2.0 * log(u1));
+  const double r = sqrt(-2.0 * log_pos(u1));
   double s, co;
   sincos_fast(cgs::twopi * u2, s, co);
   g0 = r * co;
@@ -153,7 +153,7 @@ __device__ inline double profile(const Vec3 &obs, double r0, double z0,
                                  const Vec3 &pos) {
   const double r_cyl = sqrt(pos.x * pos.x + pos.y * pos.y) - fabs(obs.x);
   const double z = fabs(pos.z) - fabs(obs.z);
-  return exp(-r_cyl / r0 - z / z0); // one exponential: the product's within a few ulp
+  return exp_fast(-r_cyl / r0 - z / z0); // one exponential: the product's within a few ulp
 }
 
 __device__ inline uint64_t sym_index(const BoxArgs &a, uint64_t idx) {
@@ -257,9 +257,9 @@ __device__ inline double es_spectrum(const EsArgs &a, double k) {
   }
   // 1 / (k / k0)^a as exp(-a ln(k / k0)): within a few ulp of the power
   if (k > a.k1 && k < a.k0)
-    P += exp(-a.a1 * log(k / a.k0));
+    P += exp_fast(-a.a1 * log_pos(k / a.k0));
   if (k > a.k0)
-    P += exp(-a.a0 * log(k / a.k0));
+    P += exp_fast(-a.a0 * log_pos(k / a.k0));
   return P * p0 * unit;
 }
 
