@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
         dB.x += coef * (o.B.x - o.b_par * r.l.x);
         dB.y += coef * (o.B.y - o.b_par * r.l.y);
         dB.z += coef * (o.B.z - o.b_par * r.l.z);
-        const double lnb = 0.5 * log_pos(o.bper);
+        const double lnb = 0.25 * log_pos(o.bper2); // ln |B_perp|
         p4 += djt * o.jt * (ac.lt + lnb) + djp * o.jp * (ac.lp + lnb);
       }
       if (o.n2 > 0.) {

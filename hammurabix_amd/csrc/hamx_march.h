@@ -523,9 +523,12 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     return false;
   // WMAP-LSA (breg_lsa.cc:9-26)
   Vec3 B{0., 0., 0.};
-  o.rr = sqrt(pos.x * pos.x + pos.y * pos.y);
+  // r and 1/r from one reciprocal square root; the LSA window decides on r^2
+  const double rr2 = pos.x * pos.x + pos.y * pos.y;
+  const double inv_r = rr2 > 0. ? rsqrt_f64(rr2) : 0.;
+  o.rr = rr2 * inv_r;
   const bool lsa = !MODELS || ph.breg_model == HAMX_BREG_LSA;
-  o.lsa_in = lsa && !(o.rr > (20. * cgs::kpc) || o.rr < (3. * cgs::kpc));
+  o.lsa_in = lsa && !(rr2 > (400. * cgs::kpc * cgs::kpc) || rr2 < (9. * cgs::kpc * cgs::kpc));
   if (MODELS && !lsa) {
     if (ph.breg_model == HAMX_BREG_GRID)
       B = gather3(gr.breg_x, gr.breg_y, gr.breg_z, gr.g_breg, pos);
@@ -537,7 +540,6 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
   const double u = pos.z * fc.inv_kpc;
   const double e_kpc = (o.lsa_in || fc.cre_z_kpc) ? exp_fast(-2. * fabs(u)) : 0.;
   if (o.lsa_in) {
-    const double inv_r = rcp_f64(o.rr);
     o.cos_p = pos.x * inv_r;
     o.sin_p = pos.y * inv_r;
     o.lnr = log_pos(o.rr * fc.inv_8kpc);

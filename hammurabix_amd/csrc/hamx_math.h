@@ -22,6 +22,17 @@ __device__ inline double rcp_f64(double x) {
   return fma(r, e, r);
 }
 
+// 1/sqrt(x) for a positive normal x: the hardware v_rsq_f64 estimate
+// (about 26 correct bits) refined by two Newton steps y += y (1 - x y^2) / 2,
+// each of which doubles the correct bits.
+__device__ inline double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-(x * y), y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // sin and cos of |x| <= pi/4 (no range reduction): the Taylor series to the
 // terms x^17 and x^18, whose remainders are below 1e-19 there; the small
 // part is added last.
