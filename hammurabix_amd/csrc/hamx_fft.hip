@@ -106,21 +106,27 @@ __device__ inline void small_dft(const double2 (&x)[R], const double2 (&w)[R], d
 
 // One radix-R Stockham pass over C lines of length n in LDS: p is the
 // product of the radices before it, tw the table of exp(2 pi i m / n).
-template <int R, int SIGN, uint32_t TILE>
-__device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint32_t p,
-                                     const double2 *__restrict__ tw) {
+// Line c's point t sits at lds[c n + t] (blocked) or, when IL, at
+// lds[t C + c] (interleaved, C = 2^cshift): the strided axes load C adjacent
+// columns a row at a time, and with the interleaved layout those rows, and
+// every pass's butterfly operands (consecutive threads take consecutive
+// lines), are contiguous runs of LDS instead of addresses n apart in one bank.
+template <int R, int SIGN, uint32_t TILE, bool IL>
+__device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint32_t cshift,
+                                     uint32_t p, const double2 *__restrict__ tw) {
   constexpr uint32_t NB = (TILE / R + 255) / 256; // butterflies per thread
   const uint32_t q = n / R, nbf = C * q, tstep = n / (R * p);
+  const uint32_t ts = IL ? C : 1u; // distance between a line's points
   double2 x[NB][R];
 #pragma unroll
   for (uint32_t u = 0; u < NB; ++u) {
     const uint32_t b = threadIdx.x + u * 256;
     if (b < nbf) {
-      const uint32_t c = b / q, i = b - c * q;
-      const double2 *line = lds + c * n;
+      const uint32_t c = IL ? (b & (C - 1)) : b / q, i = IL ? (b >> cshift) : b - c * q;
+      const double2 *line = lds + (IL ? c : c * n);
 #pragma unroll
       for (int m = 0; m < R; ++m)
-        x[u][m] = line[i + m * q];
+        x[u][m] = line[(i + m * q) * ts];
     }
   }
   __syncthreads();
@@ -132,7 +138,7 @@ __device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint3
   for (uint32_t u = 0; u < NB; ++u) {
     const uint32_t b = threadIdx.x + u * 256;
     if (b < nbf) {
-      const uint32_t c = b / q, i = b - c * q;
+      const uint32_t c = IL ? (b & (C - 1)) : b / q, i = IL ? (b >> cshift) : b - c * q;
       const uint32_t k = i % p;
       double2 xs[R], y[R];
       xs[0] = x[u][0];
@@ -140,11 +146,11 @@ __device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint3
       for (int m = 1; m < R; ++m)
         xs[m] = cmul(x[u][m], twiddle<SIGN>(tw, m * k * tstep));
       small_dft<R, SIGN>(xs, w, y);
-      double2 *line = lds + c * n;
+      double2 *line = lds + (IL ? c : c * n);
       const uint32_t j = (i - k) * R + k;
 #pragma unroll
       for (int m = 0; m < R; ++m)
-        line[j + m * p] = y[m];
+        line[(j + m * p) * ts] = y[m];
     }
   }
   __syncthreads();
@@ -153,23 +159,23 @@ __device__ inline void stockham_pass(double2 *lds, uint32_t n, uint32_t C, uint3
 // Every pass of a plan, in order.  Entered and left with LDS synchronised.
 // POW2 kernels (plans of radix 4 and 2 only) leave the odd radices out, so
 // their register staging stays at the radix-4 / radix-2 size.
-template <int SIGN, uint32_t TILE, bool POW2>
-__device__ inline void run_plan(double2 *lds, const FftPlan &pl, uint32_t C,
+template <int SIGN, uint32_t TILE, bool POW2, bool IL>
+__device__ inline void run_plan(double2 *lds, const FftPlan &pl, uint32_t C, uint32_t cshift,
                                 const double2 *__restrict__ tw) {
   uint32_t p = 1;
   for (uint32_t s = 0; s < pl.npass; ++s) {
     const uint32_t r = pl.radix[s];
     if (r == 4) {
-      stockham_pass<4, SIGN, TILE>(lds, pl.n, C, p, tw);
+      stockham_pass<4, SIGN, TILE, IL>(lds, pl.n, C, cshift, p, tw);
     } else if (r == 2) {
-      stockham_pass<2, SIGN, TILE>(lds, pl.n, C, p, tw);
+      stockham_pass<2, SIGN, TILE, IL>(lds, pl.n, C, cshift, p, tw);
     } else if constexpr (!POW2) {
       if (r == 3)
-        stockham_pass<3, SIGN, TILE>(lds, pl.n, C, p, tw);
+        stockham_pass<3, SIGN, TILE, IL>(lds, pl.n, C, cshift, p, tw);
       else if (r == 5)
-        stockham_pass<5, SIGN, TILE>(lds, pl.n, C, p, tw);
+        stockham_pass<5, SIGN, TILE, IL>(lds, pl.n, C, cshift, p, tw);
       else
-        stockham_pass<7, SIGN, TILE>(lds, pl.n, C, p, tw);
+        stockham_pass<7, SIGN, TILE, IL>(lds, pl.n, C, cshift, p, tw);
     }
     p *= r;
   }
@@ -247,9 +253,8 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
 #pragma unroll
       for (uint32_t k = 0; k < kGroup; ++k) {
         const uint32_t e = threadIdx.x + 256 * (g + k);
-        const uint32_t c = e & (C - 1), t = e >> ax.cshift;
         if (e < total)
-          lds[c * n + t] = v[k];
+          lds[e] = v[k]; // interleaved: e = t C + c
       }
     }
   } else {
@@ -269,7 +274,10 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
     twl = t;
   }
   __syncthreads();
-  run_plan<SIGN, TILE, POW2>(lds, ax.plan, C, twl);
+  if (ax.fast == 2)
+    run_plan<SIGN, TILE, POW2, true>(lds, ax.plan, C, ax.cshift, twl);
+  else
+    run_plan<SIGN, TILE, POW2, false>(lds, ax.plan, C, 0u, twl);
   if (ax.fast == 1) {
     const uint32_t valid = static_cast<uint32_t>(left < C ? left : C) * n;
     for (uint32_t e = threadIdx.x; e < valid; e += blockDim.x)
@@ -278,7 +286,7 @@ __global__ __launch_bounds__(256) void k_fft_axis(double2 *__restrict__ data0,
     for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
       const uint32_t c = e & (C - 1), t = e >> ax.cshift;
       if (c < left)
-        data[base0 + c + static_cast<uint64_t>(t) * ax.s] = lds[c * n + t];
+        data[base0 + c + static_cast<uint64_t>(t) * ax.s] = lds[e];
     }
   } else {
     for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
@@ -315,11 +323,11 @@ __global__ __launch_bounds__(256) void k_fft_axis_bluestein(double2 *__restrict_
     lds[c * M + t] = v;
   }
   __syncthreads();
-  run_plan<-1, TILE, true>(lds, ax.plan, C, twm);
+  run_plan<-1, TILE, true, false>(lds, ax.plan, C, 0u, twm);
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x)
     lds[e] = cmul(lds[e], bhat[e % M]);
   __syncthreads();
-  run_plan<1, TILE, true>(lds, ax.plan, C, twm);
+  run_plan<1, TILE, true, false>(lds, ax.plan, C, 0u, twm);
   const double inv_m = 1.0 / static_cast<double>(M);
   for (uint32_t e = threadIdx.x; e < total; e += blockDim.x) {
     uint32_t c, t;
