@@ -32,6 +32,13 @@ HIP_FLAGS = [
     # FP64 parity with the CPU restatement: evaluate expressions as written
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep LLVM's machine loop-invariant code motion from hoisting every
+    # 64-bit constant out of the sample loops: in the FP64 marches the hoisted
+    # constants filled the scalar registers, which the allocator then spilled
+    # into vector-register lanes and read back with one VALU instruction each
+    # (measured: LOS march 6.37 -> 5.91 ms, reverse march 20.8 -> 19.9 ms,
+    # batched forward 83.5 -> 75.8 ms)
+    "-mllvm", "-disable-machine-licm",
     "-Wall",
 ]
 
