@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     FastSample o;
     bool valid = false;
     if (k < r.k_hi)
-      valid = eval_fast<GRIDS, PW1, MODELS, true>(ph, fc, gr, sa, r, k, o);
+      valid = eval_fast<GRIDS, PW1, MODELS>(ph, fc, gr, sa, r, k, o);
     const double incl = wave_incl_scan(valid ? o.fdk : 0., lane);
     const double chunk = wave_last(incl);
     const double fd_here = (fd_total - after - chunk) + incl;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void k_los_adj(Phys ph, FastConst fc, AdjConst
     double c2q = 1., s2q = 0., dq = 0.;
     if (valid) {
       double sph, cph;
-      sincos_fast<true>(ac.two_l2 * (r.fd_inner + fd_here), sph, cph);
+      sincos_fast(ac.two_l2 * (r.fd_inner + fd_here), sph, cph);
       c2q = cph * o.c2i - sph * o.s2i;
       s2q = sph * o.c2i + cph * o.s2i;
       dq = 2. * o.jp * (-gq * s2q + gu * c2q);

@@ -507,9 +507,8 @@ struct FastSample {
 
 // MODELS: the regular B and TE come from the descriptor's models (any of
 // them, an LSA field with its parameter partials; a uniform TE sphere with
-// n0's) instead of being LSA and the uniform sphere.  FUSED selects the
-// fused-multiply-add form of the LSA's sine and cosine series.
-template <bool GRIDS, bool PW1, bool MODELS = false, bool FUSED = false>
+// n0's) instead of being LSA and the uniform sphere.
+template <bool GRIDS, bool PW1, bool MODELS = false>
 __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grids &gr,
                                  const ShellArgs &sa, const RayGeo &r, uint64_t k,
                                  FastSample &o) {
@@ -550,8 +549,8 @@ __device__ inline bool eval_fast(const Phys &ph, const FastConst &fc, const Grid
     const double psi = fc.psi0 + fc.psi1 * o.lnr;
     const double chi = fc.chi0 * o.tnh;
     if (fc.lsa_direct) {
-      sincos_series<FUSED>(psi, o.sps, o.cps);
-      sincos_series<FUSED>(chi, o.sch, o.cch);
+      sincos_series(psi, o.sps, o.cps);
+      sincos_series(chi, o.sch, o.cch);
     } else {
       sincos(psi, &o.sps, &o.cps);
       sincos(chi, &o.sch, &o.cch);

@@ -36,44 +36,23 @@ __device__ inline double rsqrt_f64(double x) {
 // sin and cos of |x| <= pi/4 (no range reduction): sin x = x + x^3 ps(x^2),
 // cos x = 1 - x^2/2 + x^4 pc(x^2), with ps and pc degree-5 polynomials in
 // x^2 fitted (Chebyshev interpolation at 60 digits, coefficients rounded to
-// double) over |x| <= 0.8, a margin above pi/4.  On a 20000-point sweep of
-// [0, 0.8] against the exact values: within 0.71 ulp (sin) and 1.09 ulp
-// (cos) as separate products and sums, 0.7 and 0.85 ulp with FUSED
-// multiply-adds.  The two forms allocate registers differently in their
-// callers: the LOS march measured faster with the first, the reverse march
-// with the second.
-template <bool FUSED = false>
+// double) over |x| <= 0.8, a margin above pi/4.  Evaluated with fused
+// multiply-adds, within 0.7 ulp (sin) and 0.85 ulp (cos) of the exact values
+// on a 20000-point sweep of [0, 0.8].
 __device__ inline void sincos_series(double x, double &s, double &c) {
   const double x2 = x * x;
-  if constexpr (FUSED) {
-    double ps = fma(x2, 1.59128642249034849e-10, -2.50510573944921275e-08);
-    ps = fma(x2, ps, 2.75573157385868453e-06);
-    ps = fma(x2, ps, -1.98412698360435748e-04);
-    ps = fma(x2, ps, 8.33333333333046745e-03);
-    ps = fma(x2, ps, -1.66666666666666630e-01);
-    s = fma(x * x2, ps, x);
-    double pc = fma(x2, -1.13793385852981359e-11, 2.08760996923149261e-09);
-    pc = fma(x2, pc, -2.75573170440356690e-07);
-    pc = fma(x2, pc, 2.48015872983183947e-05);
-    pc = fma(x2, pc, -1.38888888888870961e-03);
-    pc = fma(x2, pc, 4.16666666666666644e-02);
-    c = fma(x2, fma(x2, pc, -0.5), 1.0);
-  } else {
-    const double ps =
-        -1.66666666666666630e-01 +
-        x2 * (8.33333333333046745e-03 +
-              x2 * (-1.98412698360435748e-04 +
-                    x2 * (2.75573157385868453e-06 +
-                          x2 * (-2.50510573944921275e-08 + x2 * 1.59128642249034849e-10))));
-    s = x + (x * x2) * ps;
-    const double pc =
-        4.16666666666666644e-02 +
-        x2 * (-1.38888888888870961e-03 +
-              x2 * (2.48015872983183947e-05 +
-                    x2 * (-2.75573170440356690e-07 +
-                          x2 * (2.08760996923149261e-09 + x2 * -1.13793385852981359e-11))));
-    c = 1.0 + x2 * (-0.5 + x2 * pc);
-  }
+  double ps = fma(x2, 1.59128642249034849e-10, -2.50510573944921275e-08);
+  ps = fma(x2, ps, 2.75573157385868453e-06);
+  ps = fma(x2, ps, -1.98412698360435748e-04);
+  ps = fma(x2, ps, 8.33333333333046745e-03);
+  ps = fma(x2, ps, -1.66666666666666630e-01);
+  s = fma(x * x2, ps, x);
+  double pc = fma(x2, -1.13793385852981359e-11, 2.08760996923149261e-09);
+  pc = fma(x2, pc, -2.75573170440356690e-07);
+  pc = fma(x2, pc, 2.48015872983183947e-05);
+  pc = fma(x2, pc, -1.38888888888870961e-03);
+  pc = fma(x2, pc, 4.16666666666666644e-02);
+  c = fma(x2, fma(x2, pc, -0.5), 1.0);
 }
 
 // ln x for a positive normal x (the callers' arguments: radii, |B_perp|,
@@ -142,7 +121,6 @@ __device__ __noinline__ inline void sincos_lib(double x, double &s, double &c) {
 // exact for |k| < 2^20, so r = x - k pi/2 is exact to the third part's
 // rounding), then the series on |r| <= pi/4 and the quadrant's signs;
 // beyond that the library sincos.
-template <bool FUSED = false>
 __device__ inline void sincos_fast(double x, double &s, double &c) {
   const double k = rint(x * 0.6366197723675814); // 2 / pi
   if (fabs(k) < 1048576.0) {
@@ -150,7 +128,7 @@ __device__ inline void sincos_fast(double x, double &s, double &c) {
     r = fma(-k, 6.077100506303966e-11, r);
     r = fma(-k, 2.0222662487959506e-21, r);
     double sr, cr;
-    sincos_series<FUSED>(r, sr, cr);
+    sincos_series(r, sr, cr);
     const int q = static_cast<int>(k) & 3;
     s = (q == 0) ? sr : (q == 1) ? cr : (q == 2) ? -sr : -cr;
     c = (q == 0) ? cr : (q == 1) ? -sr : (q == 2) ? -cr : sr;
