@@ -24,6 +24,11 @@
 // transposed trilinear gather (FP64 atomics into a packed {x,y,z,te} record
 // per vertex), parameter gradients from per-ray partial sums reduced by a
 // second kernel.
+//
+// Polynomial steps with the compiler's own fused multiply-add selection
+// (measured faster here than the scalar-operand form the marches use).
+#define HAMX_FMA_K_ASM 0
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
