@@ -55,6 +55,10 @@ HIP_FLAGS = [
 # from the oracle by 4e-2).
 CONTRACT_SRCS = {"hamx_los.hip", "hamx_adj.hip"}
 
+# The LOS march under LLVM's max-ILP machine scheduler (measured 5.31 -> 5.23 ms
+# per march; the reverse march measured 1.4% slower with it and keeps the default).
+ILP_SRCS = {"hamx_los.hip"}
+
 
 def _run(cmd: list[str]) -> None:
     print("+", " ".join(cmd), flush=True)
@@ -77,7 +81,10 @@ def build_hip() -> str:
     os.makedirs(objdir, exist_ok=True)
     for s in srcs:
         o = os.path.join(objdir, os.path.basename(s) + ".o")
-        extra = ["-ffp-contract=fast"] if os.path.basename(s) in CONTRACT_SRCS else []
+        base = os.path.basename(s)
+        extra = ["-ffp-contract=fast"] if base in CONTRACT_SRCS else []
+        if base in ILP_SRCS:
+            extra += ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
         _run([HIPCC, *HIP_FLAGS, *extra, "-I", INCLUDE, "-I", CSRC, "-c", s, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
